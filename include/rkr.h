@@ -1,0 +1,171 @@
+/*
+ * rkr.h -- C ABI of the MI355X reverse k-ranks library (librkr.so).
+ *
+ * Reverse k-ranks over users U (n_user x d), items P (n_item x d) and query
+ * items q: rank(u, q) = #{p in P : <u,p> > <u,q>} (strict count), and a query
+ * returns the k users with the smallest (rank, userID).  Scores follow the
+ * reference's InnerProduct (src/include/alg/SpaceInnerProduct.hpp:14-20):
+ * FP32 products and sums, each rounded, summed in ascending dimension order.
+ * Every rank this library returns is the exact strict count under that rule.
+ *
+ * The path is the reference's "US" (uniform sample) pipeline:
+ *   build: score GEMM -> per-user order statistics at sample ranks r_j
+ *          -> sampled score table T (n_user x n_sample), resident in HBM;
+ *   query: <u,q> for every user -> binary search of <u,q> in T[u] -> rank
+ *          bounds -> bound pruning against the k-th bound -> exact ranks of
+ *          the surviving users -> top-k by (rank, userID).
+ *
+ * Conventions: every entry point returns RKR_OK (0) or a negative status and
+ * never exits the process; rkr_last_error() describes the last failure on a
+ * context.  Host buffers are owned by the caller and read only during the
+ * call.  Device buffers are owned by the context and stay resident until
+ * rkr_destroy().  A context is driven by one host thread at a time.
+ */
+#ifndef RKR_H_
+#define RKR_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RKR_OK 0
+#define RKR_ERR_INVALID (-1)   /* bad argument (shape, k, n_sample, ...)      */
+#define RKR_ERR_STATE (-2)     /* call out of order (no data / no table)      */
+#define RKR_ERR_NOMEM (-3)     /* device or host allocation failed / budget   */
+#define RKR_ERR_HIP (-4)       /* HIP runtime error                           */
+#define RKR_ERR_IO (-5)        /* file missing, short or malformed            */
+#define RKR_ERR_NODEVICE (-6)  /* no usable GPU                               */
+
+typedef struct rkr_ctx rkr_ctx;
+
+/* Refinement forms (rkr_set_option RKR_OPT_FORM). */
+#define RKR_FORM_PLAIN 0     /* exact FP32 score for every (user, item)        */
+#define RKR_FORM_SCREENED 1  /* BF16 MFMA screen + exact recheck near <u,q>    */
+
+/* Output orders (rkr_set_option RKR_OPT_OUTPUT). */
+#define RKR_OUTPUT_EXACT 0      /* k smallest (exact rank, userID), ascending   */
+#define RKR_OUTPUT_REFERENCE 1  /* US.hpp:173-186 layout: users the bounds alone
+                                   settle first (userID order, rank = bound),
+                                   then refined users by (rank, userID)         */
+
+#define RKR_OPT_FORM 1
+#define RKR_OPT_OUTPUT 2
+#define RKR_OPT_QUERY_BATCH 3   /* max queries per device batch (0 = auto)      */
+#define RKR_OPT_BUILD_BATCH 4   /* max users per build batch (0 = auto)         */
+
+/* Per-query counters; mirrors SingleQueryPerformance
+ * (src/include/util/FileIO.hpp:150-173) plus the screening figures. */
+typedef struct rkr_query_stats {
+    int32_t query_id;
+    int32_t n_prune_user;    /* users the bounds exclude                        */
+    int32_t n_result_user;   /* users the bounds place in the answer            */
+    int32_t n_refine_user;   /* users whose rank needed counting                */
+    int32_t n_candidate;     /* users kept after bound pruning                  */
+    int32_t reserved;
+    int64_t ip_cost;         /* inner products evaluated (GEMV + refinement)    */
+    int64_t n_rechecked;     /* screened form: (user, item) scores recomputed   */
+    int64_t n_screened;      /* screened form: (user, item) scores screened     */
+} rkr_query_stats;
+
+/* Batch-level timings (device time from HIP events, milliseconds). */
+typedef struct rkr_batch_timing {
+    double total_ms;        /* host wall time of the rkr_query call            */
+    double ip_ms;           /* <u,q> GEMV/GEMM                                 */
+    double bound_ms;        /* table binary search                             */
+    double prune_ms;        /* k-th bound, candidate compaction                */
+    double refine_ms;       /* exact-rank refinement                           */
+    double topk_ms;         /* top-k selection                                 */
+    double h2d_ms, d2h_ms;  /* copies of queries and results                   */
+    int64_t n_batches;
+    int64_t n_refined;      /* (query, user) pairs counted                     */
+    int64_t n_rechecked;
+    int64_t n_screened;
+    double max_screen_err;  /* screened form: max |screen - FP64| observed     */
+} rkr_batch_timing;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Opens a context on HIP device `device`.  mem_budget_bytes caps the device
+ * memory the context may hold (0 = 90% of free memory).  Replaces the device
+ * set-up of GPUComputeSort (src/include/score_computation/GPUComputeSort.hpp:79-110). */
+int rkr_create(int device, size_t mem_budget_bytes, rkr_ctx **out);
+void rkr_destroy(rkr_ctx *ctx);
+const char *rkr_last_error(const rkr_ctx *ctx);
+int rkr_set_option(rkr_ctx *ctx, int option, int64_t value);
+int rkr_get_option(const rkr_ctx *ctx, int option, int64_t *value);
+/* Waits for all work of the context. */
+int rkr_sync(rkr_ctx *ctx);
+/* Library version string. */
+const char *rkr_version(void);
+
+/* ---- data ---------------------------------------------------------------- */
+
+/* L2-normalises users in place exactly as VectorMatrix::vectorNormalize
+ * (src/include/struct/VectorMatrix.hpp:51-66): FP32 sequential sum of
+ * squares, sqrt, rows with norm <= 0.001 left unchanged.  Host-side helper. */
+int rkr_normalize_users(float *users, int64_t n_user, int32_t dim);
+
+/* Uploads users and items (row-major FP32, caller-owned, copied).  When
+ * normalize_users != 0, users are normalised as rkr_normalize_users first
+ * (readData does this, src/include/util/VectorIO.hpp:57-61).  Replaces the
+ * H2D copies of GPUComputeSort (GPUComputeSort.hpp:90-97) and the data
+ * hand-over of US::BuildIndex (src/impl/US.hpp:246-261). */
+int rkr_set_data(rkr_ctx *ctx, const float *users, int64_t n_user, const float *items, int64_t n_item,
+                 int32_t dim, int normalize_users);
+
+/* ---- sampled score table --------------------------------------------------- */
+
+/* r_j = floor(j * (n_item-1) / n_sample), j < n_sample
+ * (FindUniformSample, src/include/alg/RankBoundRefinement/FindSampleRank/UniformSample.hpp:17-25).
+ * Fails when n_sample <= 0 or n_sample >= n_item (SampleSearch.hpp:35-38). */
+int rkr_uniform_sample_ranks(int64_t n_item, int32_t n_sample, int32_t *out_ranks);
+
+/* Builds T[u][j] = (r_j-th largest score of user u) on the device for every
+ * user: the fused product of ComputeScoreTableBatch::ComputeSortItemsBatch
+ * (src/include/score_computation/ComputeScoreTableBatch.hpp:79-92) and
+ * SampleSearch::LoopPreprocess (SampleSearch.hpp:80-85) as driven by
+ * BuildSampleIndexByComputation.cpp:111-144.  ranks must be strictly
+ * increasing in [0, n_item).  build_ms (optional) receives device time. */
+int rkr_build_table(rkr_ctx *ctx, const int32_t *ranks, int32_t n_sample, double *build_ms);
+
+/* Installs a table computed elsewhere (n_user x n_sample, row-major,
+ * rows non-increasing).  eps bounds |T - exact order statistic| under this
+ * library's score rule (0 for tables built by the rule; the query path
+ * widens its binary search by eps so answers stay exact). */
+int rkr_set_table(rkr_ctx *ctx, const int32_t *ranks, int32_t n_sample, const float *table, float eps);
+
+/* Copies table rows [user_begin, user_begin + n_rows) to out (n_rows x n_sample). */
+int rkr_get_table(rkr_ctx *ctx, int64_t user_begin, int64_t n_rows, float *out);
+int rkr_get_ranks(const rkr_ctx *ctx, int32_t *out_ranks, int32_t *out_n_sample);
+
+/* Table file in the layout of SampleSearch::SaveIndex / LoadIndex
+ * (SampleSearch.hpp:186-194, 229-241): size_t n_sample, size_t n_item,
+ * size_t n_user, int32 ranks[n_sample], float table[n_user][n_sample]. */
+int rkr_save_table(rkr_ctx *ctx, const char *path);
+int rkr_load_table(rkr_ctx *ctx, const char *path, float eps);
+
+/* ---- queries --------------------------------------------------------------- */
+
+/* Answers n_query reverse k-ranks queries (queries: n_query x dim, row-major
+ * FP32, caller-owned).  topk[i] is query i's k (1 <= k <= min(n_user, max_k)).
+ * Outputs are n_query x max_k, row-major; entries past topk[i] are -1.
+ * out_qip receives <u,q> of each returned user (may be NULL); stats
+ * (n_query entries) and timing may be NULL.  Replaces US::Index::Retrieval
+ * (src/impl/US.hpp:96-213), which BaseIndex::Retrieval declares
+ * (src/include/struct/MethodBase.hpp:14-30). */
+int rkr_query(rkr_ctx *ctx, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k,
+              int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
+              rkr_batch_timing *timing);
+
+/* Bytes of device memory the index holds (table + ranks), as
+ * SampleSearch::IndexSizeByte (SampleSearch.hpp:246-250). */
+int64_t rkr_index_size_bytes(const rkr_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RKR_H_ */

@@ -1,0 +1,240 @@
+"""ctypes view of the CPU oracle (liborc.so) and, when built, of the reference
+routines compiled into oracle/_ref/libref.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORC_PATH = os.path.join(HERE, "liborc.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref.so")
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+_i64 = C.c_int64
+_i32 = C.c_int32
+
+
+class QStat(C.Structure):
+    _fields_ = [("n_prune_user", C.c_int32), ("n_result_user", C.c_int32),
+                ("n_refine_user", C.c_int32), ("n_candidate", C.c_int32),
+                ("ip_cost", C.c_int64)]
+
+
+def build(force: bool = False) -> None:
+    """Compile liborc.so (and oracle/_ref when the reference is present)."""
+    if force or not os.path.exists(ORC_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_orc = None
+_ref = None
+
+
+def lib():
+    global _orc
+    if _orc is None:
+        build()
+        L = C.CDLL(ORC_PATH)
+        L.orc_normalize_users.argtypes = [_f32p, _i64, _i32]
+        L.orc_ip_f32.argtypes = [_f32p, _f32p, _i32]
+        L.orc_ip_f32.restype = C.c_float
+        L.orc_ip_f64.argtypes = [_f32p, _f32p, _i32]
+        L.orc_ip_f64.restype = C.c_double
+        L.orc_uniform_sample.argtypes = [_i64, _i32, _i32p]
+        L.orc_uniform_sample.restype = C.c_int
+        L.orc_score_rows.argtypes = [_f32p, _i64, _i64, _f32p, _i64, _i32, _f32p]
+        L.orc_build_table.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _i32p, _i32, _f32p]
+        L.orc_coarse_bsearch.argtypes = [_f32p, _i32, _i32p, _i64, C.c_float,
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.orc_safe_bounds.argtypes = [_f32p, _i32, _i32p, _i64, C.c_float, C.c_float,
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.orc_prune.argtypes = [_i32p, _i32p, _i64, _i32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                C.POINTER(C.c_int32), _i8p, _i8p]
+        L.orc_refine_rank.argtypes = [_f32p, _f32p, _i64, _i32, C.c_float]
+        L.orc_refine_rank.restype = C.c_int32
+        L.orc_us_retrieval.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _f32p, _i32p, _i32, _f32p, _i32, _i32,
+                                       _i32p, _i32p, _f32p, C.c_void_p]
+        L.orc_us_retrieval.restype = C.c_int
+        L.orc_exact_query.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _f32p, _i32p, _i32, C.c_float, _f32p,
+                                      _i32, _i32p, _i32p, _f32p, C.c_void_p]
+        L.orc_exact_query.restype = C.c_int
+        L.orc_bruteforce.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _f32p, _i32, _i32p, _i32p, _f32p]
+        L.orc_bruteforce.restype = C.c_int
+        L.orc_num_threads.restype = C.c_int
+        L.orc_set_num_threads.argtypes = [C.c_int]
+        _orc = L
+    return _orc
+
+
+def ref_lib():
+    """The reference's own routines (oracle/_ref), or None when not built."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_PATH):
+            return None
+        L = C.CDLL(REF_PATH)
+        L.ref_inner_product.argtypes = [_f32p, _f32p, C.c_int]
+        L.ref_inner_product.restype = C.c_float
+        L.ref_refine_rank.argtypes = [_f32p, _f32p, _i64, _i64, C.c_int, C.c_float, C.c_int]
+        L.ref_refine_rank.restype = C.c_int
+        L.ref_prune.argtypes = [_i32p, _i32p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.c_int), _i8p, _i8p]
+        L.ref_score_list.argtypes = [_f32p, _f32p, _i64, _i64, C.c_int, C.c_int, C.c_int, _f32p]
+        L.ref_normalize.argtypes = [_f32p, C.c_int, C.c_int]
+        _ref = L
+    return _ref
+
+
+def _c32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def normalize_users(users: np.ndarray) -> np.ndarray:
+    u = _c32(users).copy()
+    lib().orc_normalize_users(u, u.shape[0], u.shape[1])
+    return u
+
+
+def ip_f32(a, b) -> np.float32:
+    a, b = _c32(a), _c32(b)
+    return np.float32(lib().orc_ip_f32(a, b, a.shape[0]))
+
+
+def ip_f64(a, b) -> float:
+    a, b = _c32(a), _c32(b)
+    return lib().orc_ip_f64(a, b, a.shape[0])
+
+
+def uniform_sample(n_item: int, n_sample: int) -> np.ndarray:
+    r = np.zeros(max(n_sample, 1), np.int32)
+    if lib().orc_uniform_sample(n_item, n_sample, r) != 0:
+        raise ValueError("n_sample must satisfy 0 < n_sample < n_item")
+    return r[:n_sample]
+
+
+def score_rows(users, items, u0=0, nu=None) -> np.ndarray:
+    users, items = _c32(users), _c32(items)
+    nu = users.shape[0] - u0 if nu is None else nu
+    out = np.empty((nu, items.shape[0]), np.float32)
+    lib().orc_score_rows(users, u0, nu, items, items.shape[0], users.shape[1], out)
+    return out
+
+
+def build_table(users, items, ranks) -> np.ndarray:
+    users, items = _c32(users), _c32(items)
+    ranks = np.ascontiguousarray(ranks, np.int32)
+    T = np.empty((users.shape[0], ranks.shape[0]), np.float32)
+    lib().orc_build_table(users, users.shape[0], items, items.shape[0], users.shape[1], ranks,
+                          ranks.shape[0], T)
+    return T
+
+
+def coarse_bsearch(row, ranks, n_item, qip):
+    lb, ub = C.c_int32(), C.c_int32()
+    row = _c32(row)
+    lib().orc_coarse_bsearch(row, row.shape[0], np.ascontiguousarray(ranks, np.int32), n_item,
+                             float(np.float32(qip)), C.byref(lb), C.byref(ub))
+    return lb.value, ub.value
+
+
+def safe_bounds(row, ranks, n_item, qip, eps=0.0):
+    lb, ub = C.c_int32(), C.c_int32()
+    row = _c32(row)
+    lib().orc_safe_bounds(row, row.shape[0], np.ascontiguousarray(ranks, np.int32), n_item,
+                          float(np.float32(qip)), float(eps), C.byref(lb), C.byref(ub))
+    return lb.value, ub.value
+
+
+def prune(lb, ub, topk, prune_l=None, result_l=None, n_result=0, n_prune=0, refine=None):
+    lb = np.ascontiguousarray(lb, np.int32)
+    ub = np.ascontiguousarray(ub, np.int32)
+    n = lb.shape[0]
+    p = np.zeros(n, np.int8) if prune_l is None else np.ascontiguousarray(prune_l, np.int8).copy()
+    r = np.zeros(n, np.int8) if result_l is None else np.ascontiguousarray(result_l, np.int8).copy()
+    rs, nr, npr = C.c_int32(n if refine is None else refine), C.c_int32(n_result), C.c_int32(n_prune)
+    lib().orc_prune(lb, ub, n, topk, C.byref(rs), C.byref(nr), C.byref(npr), p, r)
+    return dict(refine=rs.value, n_result=nr.value, n_prune=npr.value, prune=p, result=r)
+
+
+def refine_rank(user, items, qip) -> int:
+    user, items = _c32(user), _c32(items)
+    return int(lib().orc_refine_rank(user, items, items.shape[0], items.shape[1], float(np.float32(qip))))
+
+
+def _query_out(nq, k):
+    return (np.empty((nq, k), np.int32), np.empty((nq, k), np.int32), np.empty((nq, k), np.float32))
+
+
+def us_retrieval(users, items, table, ranks, queries, topk, stats=False):
+    """The reference US query path (US.hpp:96-213) over every query."""
+    users, items, table, queries = _c32(users), _c32(items), _c32(table), _c32(queries)
+    ranks = np.ascontiguousarray(ranks, np.int32)
+    nq = queries.shape[0]
+    ou, orank, oq = _query_out(nq, topk)
+    st = (QStat * max(nq, 1))()
+    rc = lib().orc_us_retrieval(users, users.shape[0], items, items.shape[0], users.shape[1], table, ranks,
+                                ranks.shape[0], queries, nq, topk, ou, orank, oq, C.cast(st, C.c_void_p))
+    if rc != 0:
+        raise ValueError("invalid topk")
+    if stats:
+        return ou, orank, oq, [dict((f, getattr(s, f)) for f, _ in QStat._fields_) for s in st[:nq]]
+    return ou, orank, oq
+
+
+def exact_query(users, items, table, ranks, queries, topk, eps=0.0, stats=False):
+    """Exact reverse k-ranks through the sampled table, every query."""
+    users, items, table, queries = _c32(users), _c32(items), _c32(table), _c32(queries)
+    ranks = np.ascontiguousarray(ranks, np.int32)
+    nq = queries.shape[0]
+    ks = np.broadcast_to(np.asarray(topk, np.int32), (nq,))
+    kmax = int(ks.max()) if nq else 0
+    ou = np.full((nq, kmax), -1, np.int32)
+    orank = np.full((nq, kmax), -1, np.int32)
+    oq = np.full((nq, kmax), -1, np.float32)
+    sts = []
+    for i in range(nq):
+        k = int(ks[i])
+        u_, r_, q_ = (np.empty(k, np.int32), np.empty(k, np.int32), np.empty(k, np.float32))
+        st = QStat()
+        rc = lib().orc_exact_query(users, users.shape[0], items, items.shape[0], users.shape[1], table, ranks,
+                                   ranks.shape[0], float(eps), np.ascontiguousarray(queries[i]), k, u_, r_, q_,
+                                   C.cast(C.pointer(st), C.c_void_p))
+        if rc != 0:
+            raise ValueError("invalid topk")
+        ou[i, :k], orank[i, :k], oq[i, :k] = u_, r_, q_
+        sts.append(dict((f, getattr(st, f)) for f, _ in QStat._fields_))
+    if stats:
+        return ou, orank, oq, sts
+    return ou, orank, oq
+
+
+def bruteforce(users, items, queries, topk):
+    """Rank every user (strict count), order by (rank, userID), keep topk."""
+    users, items, queries = _c32(users), _c32(items), _c32(queries)
+    nq = queries.shape[0]
+    ou, orank, oq = _query_out(nq, topk)
+    for i in range(nq):
+        u_, r_, q_ = (np.empty(topk, np.int32), np.empty(topk, np.int32), np.empty(topk, np.float32))
+        rc = lib().orc_bruteforce(users, users.shape[0], items, items.shape[0], users.shape[1],
+                                  np.ascontiguousarray(queries[i]), topk, u_, r_, q_)
+        if rc != 0:
+            raise ValueError("invalid topk")
+        ou[i], orank[i], oq[i] = u_, r_, q_
+    return ou, orank, oq
+
+
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())

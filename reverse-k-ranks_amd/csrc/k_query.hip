@@ -1,0 +1,429 @@
+// k_query.hip -- query-side kernels between the <u,q> scores and the
+// exact-rank refinement: table binary search, per-query k-th bound,
+// candidate compaction and top-k selection.
+//
+// Bucket convention (SampleSearch::CoarseBinarySearch,
+// src/include/alg/RankBoundRefinement/SampleSearch.hpp:91-130): T[u] is
+// non-increasing and bucket(q) = #{j : T[u][j] >= q}.  Rank bounds are
+// functions of the bucket alone, so per-query k-th bounds come from a
+// (S+1)-bin histogram of buckets instead of the reference's two max-heaps
+// (PruneCandidateByBound.hpp:35-77).
+//
+// Exact output (RKR_OUTPUT_EXACT): with b_lo = #{T >= q - eps} and
+// b_hi = #{T > q + eps},
+//   rank <= lbs = (b_lo == S ? I : r[b_lo]),
+//   rank >= ubs = (b_hi == 0 ? 0 : r[b_hi - 1] + 1),
+// which hold under score ties too.  beta = bucket of the k-th smallest lbs;
+// a user can be in the answer only if ubs <= r[beta], i.e. b_hi <= beta.
+//
+// Reference output (RKR_OUTPUT_REFERENCE): lb/ub and the result / prune /
+// refine classification of PruneCandidateByBound.hpp:83-150, and the
+// US.hpp:173-186 answer layout.
+#include "rkr_internal.hpp"
+
+namespace rkr {
+namespace {
+
+constexpr int NT = 1024;
+constexpr int NW = NT / 64;
+constexpr int SMAXQ = kMaxSample + 1;
+
+enum { MODE_EXACT = 0, MODE_REF_EQUAL = 1, MODE_REF_SPLIT = 2, MODE_REF_ALL = 3 };
+
+__device__ __forceinline__ int32_t bucket_ge(const float *__restrict__ row, int32_t S, float q) {
+    int32_t lo = 0, hi = S;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (row[mid] >= q) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ int32_t bucket_gt(const float *__restrict__ row, int32_t S, float q) {
+    int32_t lo = 0, hi = S;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (row[mid] > q) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// grid: (user blocks, queries).  bk[b * U + u] = b_lo | b_hi << 16.
+__global__ void __launch_bounds__(256) k_bounds(const float *__restrict__ qip, const float *__restrict__ table,
+                                                int64_t n_user, int32_t S, float eps, uint32_t *__restrict__ bk) {
+    const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t b = blockIdx.y;
+    if (u >= n_user) return;
+    const float q = qip[b * n_user + u];
+    const float *row = table + u * (int64_t)S;
+    uint32_t lo, hi;
+    if (eps > 0.0f) {
+        const float qlo = __double2float_rd((double)q - (double)eps);
+        const float qhi = __double2float_ru((double)q + (double)eps);
+        lo = (uint32_t)bucket_ge(row, S, qlo);
+        hi = (uint32_t)bucket_gt(row, S, qhi);
+    } else {
+        lo = (uint32_t)bucket_ge(row, S, q);
+        // b_hi = b_lo minus the trailing run of entries equal to q
+        hi = lo;
+        while (hi > 0 && row[hi - 1] == q) --hi;
+    }
+    bk[b * n_user + u] = lo | (hi << 16);
+}
+
+// Reference bounds of a bucket (SampleSearch.hpp:107-125).
+__device__ __forceinline__ int32_t ref_lb(int32_t b, int32_t S, const int32_t *r, int64_t I) {
+    return b == S ? (int32_t)I : r[b];
+}
+__device__ __forceinline__ int32_t ref_ub(int32_t b, int32_t S, const int32_t *r, int64_t I) {
+    if (b == S) return r[b - 1];
+    if (b == 0) return 0;
+    if ((uint32_t)r[b] - (uint32_t)r[b - 1] <= 1u) return r[b];
+    return r[b - 1];
+}
+
+__device__ uint32_t wg_excl_scan(uint32_t v, uint32_t *red, uint32_t *total) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) red[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int i = 0; i < NW; ++i) { const uint32_t t = red[i]; red[i] = s; s += t; }
+        red[NW] = s;
+    }
+    __syncthreads();
+    const uint32_t r = red[w] + x - v;
+    *total = red[NW];
+    __syncthreads();
+    return r;
+}
+
+// One workgroup per query: bucket histograms, k-th bound, class counts.
+__global__ void __launch_bounds__(NT) k_hist_kth(const uint32_t *__restrict__ bk, int64_t n_user, int32_t S,
+                                                 const int32_t *__restrict__ ranks, int64_t n_item, int output_mode,
+                                                 QueryInfo *__restrict__ qi) {
+    __shared__ uint32_t h_lo[SMAXQ + 1];
+    __shared__ uint32_t h_hi[SMAXQ + 1];
+    const int64_t b = blockIdx.x;
+    for (int i = threadIdx.x; i <= S; i += NT) { h_lo[i] = 0; h_hi[i] = 0; }
+    __syncthreads();
+    const uint32_t *v = bk + b * n_user;
+    for (int64_t u = threadIdx.x; u < n_user; u += NT) {
+        const uint32_t x = v[u];
+        atomicAdd(&h_lo[x & 0xFFFFu], 1u);
+        atomicAdd(&h_hi[x >> 16], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    QueryInfo q = qi[b];
+    const int64_t k = q.topk;
+    int32_t beta = S;
+    int64_t cum = 0;
+    for (int32_t j = 0; j <= S; ++j) {
+        cum += h_lo[j];
+        if (cum >= k) { beta = j; break; }
+    }
+    q.beta = beta;
+    if (output_mode == RKR_OUTPUT_EXACT) {
+        int64_t nc = 0;
+        for (int32_t j = 0; j <= beta; ++j) nc += h_hi[j];
+        q.mode = MODE_EXACT;
+        q.gamma = beta;
+        q.n_cand = (int32_t)nc;
+        q.n_settled = 0;
+    } else {
+        const int32_t Lk = ref_lb(beta, S, ranks, n_item), Uk = ref_ub(beta, S, ranks, n_item);
+        int64_t n_res = 0, n_ref = 0;
+        for (int32_t j = 0; j <= S; ++j) {
+            const int64_t h = h_lo[j];
+            if (!h) continue;
+            const int32_t lb = ref_lb(j, S, ranks, n_item), ub = ref_ub(j, S, ranks, n_item);
+            if (Lk == Uk) {
+                if (ub <= Lk && lb <= Lk) n_res += h;
+            } else if (lb <= Uk) n_res += h;
+            else if (Lk <= ub) continue;
+            else n_ref += h;
+        }
+        if (Lk == Uk) {
+            q.mode = MODE_REF_EQUAL;  // PruneCandidateByBound.hpp:83-111
+            q.n_settled = (int32_t)(n_res < k ? n_res : k);
+            q.n_cand = 0;
+        } else if (n_res + n_ref == k) {
+            q.mode = MODE_REF_ALL;    // :139-150: every live user is a result
+            q.n_settled = (int32_t)k;
+            q.n_cand = 0;
+        } else {
+            q.mode = MODE_REF_SPLIT;  // :114-136
+            q.n_settled = (int32_t)(n_res < k ? n_res : k);
+            q.n_cand = (int32_t)n_ref;
+        }
+        q.gamma = Uk;
+        q.pad = Lk;
+    }
+    qi[b] = q;
+}
+
+// Exclusive offsets of each query's [settled | candidates] slice.
+__global__ void __launch_bounds__(NT) k_offsets(QueryInfo *__restrict__ qi, int32_t nq, int32_t *__restrict__ refine_total) {
+    __shared__ uint32_t red[NW + 1];
+    uint32_t base = 0;
+    for (int32_t c0 = 0; c0 < nq; c0 += NT) {
+        const int32_t i = c0 + threadIdx.x;
+        const uint32_t v = i < nq ? (uint32_t)(qi[i].n_settled + qi[i].n_cand) : 0u;
+        uint32_t tot;
+        const uint32_t e = wg_excl_scan(v, red, &tot);
+        if (i < nq) qi[i].cand_off = (int64_t)base + e;
+        base += tot;
+    }
+    if (threadIdx.x == 0) *refine_total = 0;
+}
+
+// One workgroup per query, users in ascending order: writes the settled users
+// (reference output, at most k, with rank = bound) and the candidates (with
+// their rank when the bounds fix it, else -1 plus a refine row).
+__global__ void __launch_bounds__(NT) k_compact(const uint32_t *__restrict__ bk, const float *__restrict__ qip,
+                                                int64_t n_user, int32_t S, const int32_t *__restrict__ ranks,
+                                                int64_t n_item, QueryInfo *__restrict__ qi,
+                                                int32_t *__restrict__ cand_user, int32_t *__restrict__ cand_rank,
+                                                int32_t *__restrict__ refine_total, int32_t *__restrict__ refine_cand,
+                                                int32_t *__restrict__ refine_user, float *__restrict__ refine_thr) {
+    __shared__ uint32_t red[NW + 1];
+    __shared__ int32_t n_refine;
+    const int64_t b = blockIdx.x;
+    const QueryInfo q = qi[b];
+    const uint32_t *v = bk + b * n_user;
+    const float *qb = qip + b * n_user;
+    uint32_t n_set = 0, n_can = 0;  // running counts
+    int32_t my_refine = 0;
+    if (threadIdx.x == 0) n_refine = 0;
+    const int64_t set_base = q.cand_off;
+    const int64_t can_base = q.cand_off + q.n_settled;
+    const int32_t Lk = q.pad, Uk = q.gamma;
+    for (int64_t u0 = 0; u0 < n_user; u0 += NT) {
+        const int64_t u = u0 + threadIdx.x;
+        bool settled = false, cand = false;
+        int32_t rank = -1;
+        if (u < n_user) {
+            const uint32_t x = v[u];
+            const int32_t blo = (int32_t)(x & 0xFFFFu), bhi = (int32_t)(x >> 16);
+            if (q.mode == MODE_EXACT) {
+                if (bhi <= q.beta) {
+                    cand = true;
+                    const int32_t lbs = blo == S ? (int32_t)n_item : ranks[blo];
+                    const int32_t ubs = bhi == 0 ? 0 : ranks[bhi - 1] + 1;
+                    if (lbs == ubs) rank = lbs;
+                }
+            } else {
+                const int32_t lb = ref_lb(blo, S, ranks, n_item), ub = ref_ub(blo, S, ranks, n_item);
+                if (q.mode == MODE_REF_EQUAL) {
+                    settled = (ub <= Lk && lb <= Lk);
+                } else if (q.mode == MODE_REF_ALL) {
+                    settled = !(lb > Uk && Lk <= ub);
+                } else {
+                    if (lb <= Uk) settled = true;
+                    else if (!(Lk <= ub)) { cand = true; if (lb == ub) rank = lb; }
+                }
+                if (settled) rank = lb;
+            }
+        }
+        uint32_t tot;
+        const uint32_t e = wg_excl_scan((settled ? 1u : 0u) | (cand ? 0x10000u : 0u), red, &tot);
+        const uint32_t my_set = n_set + (e & 0xFFFFu), my_can = n_can + (e >> 16);
+        if (settled && my_set < (uint32_t)q.n_settled) {
+            cand_user[set_base + my_set] = (int32_t)u;
+            cand_rank[set_base + my_set] = rank;
+        }
+        if (cand) {
+            const int64_t slot = can_base + my_can;
+            cand_user[slot] = (int32_t)u;
+            cand_rank[slot] = rank;
+            if (rank < 0) {
+                ++my_refine;
+                const int32_t r = atomicAdd(refine_total, 1);
+                refine_cand[r] = (int32_t)slot;
+                refine_user[r] = (int32_t)u;
+                refine_thr[r] = qb[u];
+            }
+        }
+        n_set += tot & 0xFFFFu;
+        n_can += tot >> 16;
+    }
+    if (my_refine) atomicAdd(&n_refine, my_refine);
+    __syncthreads();
+    if (threadIdx.x == 0) qi[b].n_refine = n_refine;
+}
+
+__global__ void k_scatter_counts(const int32_t *__restrict__ refine_total, const int32_t *__restrict__ refine_cand,
+                                 const int32_t *__restrict__ cnt, int64_t max_rows, int32_t *__restrict__ cand_rank) {
+    const int64_t n = min((int64_t)*refine_total, max_rows);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        cand_rank[refine_cand[i]] = cnt[i];
+}
+
+// ---- top-k -----------------------------------------------------------------
+constexpr int SORT_CAP = kMaxTopk;  // keys sorted in LDS
+
+__device__ void bitonic_sort_u64(uint64_t *keys, int n_pow2) {
+    for (int size = 2; size <= n_pow2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < n_pow2 / 2; i += NT) {
+                const int lo = 2 * i - (i & (stride - 1));
+                const int hi = lo + stride;
+                const bool up = ((lo & size) == 0);
+                const uint64_t a = keys[lo], c = keys[hi];
+                if ((a > c) == up) { keys[lo] = c; keys[hi] = a; }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t make_key(int32_t rank, int32_t user) {
+    return ((uint64_t)(uint32_t)rank << 32) | (uint32_t)user;
+}
+
+// One workgroup per query.  Output row: the settled users (reference
+// output) followed by the smallest (rank, userID) candidates.
+__global__ void __launch_bounds__(NT) k_topk(const QueryInfo *__restrict__ qi, const int32_t *__restrict__ cand_user,
+                                             const int32_t *__restrict__ cand_rank, const float *__restrict__ qip,
+                                             int64_t n_user, int32_t max_k, int32_t *__restrict__ out_user,
+                                             int32_t *__restrict__ out_rank, float *__restrict__ out_qip) {
+    __shared__ uint64_t keys[SORT_CAP];
+    __shared__ uint32_t hist[256];
+    __shared__ uint64_t sel_prefix;
+    __shared__ uint32_t sel_rem, n_sel;
+    const int64_t b = blockIdx.x;
+    const QueryInfo q = qi[b];
+    const int32_t k = q.topk;
+    int32_t *ou = out_user + b * (int64_t)max_k;
+    int32_t *orank = out_rank + b * (int64_t)max_k;
+    float *oq = out_qip + b * (int64_t)max_k;
+    const float *qb = qip + b * n_user;
+
+    for (int i = threadIdx.x; i < max_k; i += NT) { ou[i] = -1; orank[i] = -1; oq[i] = -1.0f; }
+    __syncthreads();
+    const int32_t ns = q.n_settled < k ? q.n_settled : k;
+    for (int i = threadIdx.x; i < ns; i += NT) {
+        const int32_t u = cand_user[q.cand_off + i];
+        ou[i] = u;
+        orank[i] = cand_rank[q.cand_off + i];
+        oq[i] = qb[u];
+    }
+    const int32_t need = k - ns;
+    const int64_t nc = q.n_cand;
+    if (need <= 0 || nc <= 0) return;
+    const int32_t *cu = cand_user + q.cand_off + q.n_settled;
+    const int32_t *cr = cand_rank + q.cand_off + q.n_settled;
+    const int32_t take = (int32_t)(nc < need ? nc : need);
+
+    int n_keys;
+    if (nc <= SORT_CAP) {
+        for (int64_t i = threadIdx.x; i < nc; i += NT) keys[i] = make_key(cr[i], cu[i]);
+        n_keys = (int)nc;
+    } else {
+        // MSD radix select of the take-th smallest key, 8 bits at a time;
+        // keys are distinct (userIDs are), so exactly `take` keys are <= it.
+        if (threadIdx.x == 0) { sel_prefix = 0; sel_rem = (uint32_t)take - 1; }
+        __syncthreads();
+        for (int shift = 56; shift >= 0; shift -= 8) {
+            for (int i = threadIdx.x; i < 256; i += NT) hist[i] = 0;
+            __syncthreads();
+            const uint64_t pre = sel_prefix;
+            const uint64_t mask_hi = (shift == 56) ? 0ull : (~0ull << (shift + 8));
+            for (int64_t i = threadIdx.x; i < nc; i += NT) {
+                const uint64_t key = make_key(cr[i], cu[i]);
+                if ((key & mask_hi) == pre) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t rem = sel_rem, d = 0;
+                for (; d < 255; ++d) {
+                    if (rem < hist[d]) break;
+                    rem -= hist[d];
+                }
+                sel_prefix = pre | ((uint64_t)d << shift);
+                sel_rem = rem;
+            }
+            __syncthreads();
+        }
+        const uint64_t kth = sel_prefix;
+        if (threadIdx.x == 0) n_sel = 0;
+        __syncthreads();
+        for (int64_t i = threadIdx.x; i < nc; i += NT) {
+            const uint64_t key = make_key(cr[i], cu[i]);
+            if (key <= kth) keys[atomicAdd(&n_sel, 1u)] = key;
+        }
+        __syncthreads();
+        n_keys = (int)n_sel;
+    }
+    int p2 = 1;
+    while (p2 < n_keys) p2 <<= 1;
+    for (int i = n_keys + threadIdx.x; i < p2; i += NT) keys[i] = ~0ull;
+    __syncthreads();
+    bitonic_sort_u64(keys, p2);
+    for (int i = threadIdx.x; i < take; i += NT) {
+        const uint64_t key = keys[i];
+        const int32_t u = (int32_t)(uint32_t)(key & 0xFFFFFFFFu);
+        ou[ns + i] = u;
+        orank[ns + i] = (int32_t)(key >> 32);
+        oq[ns + i] = qb[u];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_bounds(hipStream_t s, const float *qip, const float *table, int64_t n_user, int32_t S, int32_t nq,
+                         float eps, uint32_t *bk) {
+    if (n_user <= 0 || nq <= 0) return hipSuccess;
+    dim3 grid((unsigned)((n_user + 255) / 256), (unsigned)nq);
+    hipLaunchKernelGGL(k_bounds, grid, dim3(256), 0, s, qip, table, n_user, S, eps, bk);
+    return hipGetLastError();
+}
+
+hipError_t launch_hist_kth(hipStream_t s, const uint32_t *bk, int64_t n_user, int32_t S, const int32_t *ranks,
+                           int64_t n_item, int32_t nq, int output_mode, QueryInfo *qi) {
+    if (nq <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hist_kth, dim3((unsigned)nq), dim3(NT), 0, s, bk, n_user, S, ranks, n_item, output_mode, qi);
+    return hipGetLastError();
+}
+
+hipError_t launch_offsets(hipStream_t s, QueryInfo *qi, int32_t nq, int32_t *refine_total) {
+    hipLaunchKernelGGL(k_offsets, dim3(1), dim3(NT), 0, s, qi, nq, refine_total);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(hipStream_t s, const uint32_t *bk, const float *qip, int64_t n_user, int32_t S,
+                          const int32_t *ranks, int64_t n_item, int32_t nq, int output_mode, QueryInfo *qi,
+                          int32_t *cand_user, int32_t *cand_rank, int32_t *refine_total, int32_t *refine_cand,
+                          int32_t *refine_user, float *refine_thr) {
+    (void)output_mode;
+    if (nq <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)nq), dim3(NT), 0, s, bk, qip, n_user, S, ranks, n_item, qi, cand_user,
+                       cand_rank, refine_total, refine_cand, refine_user, refine_thr);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_counts(hipStream_t s, const int32_t *refine_total, const int32_t *refine_cand,
+                                 const int32_t *cnt, int64_t max_rows, int32_t *cand_rank) {
+    if (max_rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_counts, dim3(1024), dim3(256), 0, s, refine_total, refine_cand, cnt, max_rows,
+                       cand_rank);
+    return hipGetLastError();
+}
+
+hipError_t launch_topk(hipStream_t s, const QueryInfo *qi, int32_t nq, const int32_t *cand_user,
+                       const int32_t *cand_rank, const float *qip, int64_t n_user, int32_t max_k, int32_t *out_user,
+                       int32_t *out_rank, float *out_qip, int64_t *scratch_keys) {
+    (void)scratch_keys;
+    if (nq <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_topk, dim3((unsigned)nq), dim3(NT), 0, s, qi, cand_user, cand_rank, qip, n_user, max_k,
+                       out_user, out_rank, out_qip);
+    return hipGetLastError();
+}
+
+}  // namespace rkr

@@ -1,0 +1,325 @@
+// k_select.hip -- per-user order statistics at the sample ranks.
+//
+// For one row x[0..I) of exact scores and strictly increasing sample ranks
+// r_0 < ... < r_{S-1} < I, writes T[j] = the r_j-th largest value of x
+// (0-based, descending).  This is the value SampleSearch::LoopPreprocess
+// reads from a row sorted with std::greater
+// (src/include/alg/RankBoundRefinement/SampleSearch.hpp:80-85;
+//  src/include/score_computation/ComputeScoreTableBatch.hpp:83-89): the
+// r-th order statistic is unique as a value, so no full sort is needed.
+//
+// One 1024-thread workgroup per row, all state in LDS (up to 160 KB on
+// gfx950):
+//   1. min/max of the row;
+//   2. histogram over NB linear value bins; bin(x) is monotone in x, so a
+//      reverse prefix sum tells, for every r_j, its bin and its residual rank
+//      inside the bin;
+//   3. the bins holding targets are gathered into LDS (as many per pass as
+//      fit), and each bin is ranked by one wavefront with a counting select;
+//   4. bins too large for that are narrowed by repeated histograms over
+//      their value range (wg_select) -- this also makes ties and skewed
+//      rows exact.
+#include "rkr_internal.hpp"
+
+namespace rkr {
+namespace {
+
+constexpr int NT = 1024;          // threads per row
+constexpr int NW = NT / 64;       // wavefronts per row
+constexpr int NB = 4096;          // level-0 bins
+constexpr int NB2 = 1024;         // bins of the narrowing loop
+constexpr int SMAX = 4096;        // largest n_sample handled
+constexpr int GCAP = 22528;       // floats gathered per pass (88 KB)
+constexpr int WAVE_SEL_MAX = 2048;  // bins larger than this use wg_select
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint16_t DEEP = 0x8000;
+
+struct SelSmem {
+    uint32_t incl[NB + 1];        // incl[b] = #elements in bins >= b
+    uint32_t cur[NB];             // gather cursor of member bins, NONE otherwise
+    uint32_t hist2[NB2 + 1];
+    uint16_t tb[SMAX];            // bin of target j
+    uint16_t fb[SMAX];            // first target index of each distinct bin
+    uint16_t grp[SMAX + 1];       // group start (index into fb) | DEEP
+    uint16_t coop[SMAX];          // fb indices needing wg_select in this group
+    float red_f[2 * NW];
+    uint32_t red_u[NW + 1];
+    int32_t nfb, ngrp, ncoop, sel_b;
+    float gbuf[GCAP];
+};
+
+// Monotone in x for x in [lo, hi]; anything that is not >= 0 (NaN) maps to
+// bin 0 so an LDS index is always in range.
+__device__ __forceinline__ int bin_of(float x, float lo, double scale, int nb) {
+    const double t = ((double)x - (double)lo) * scale;
+    if (!(t >= 0.0)) return 0;
+    if (!(t < (double)nb)) return nb - 1;
+    return (int)t;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Workgroup min/max; every thread gets the result.
+__device__ void wg_minmax(SelSmem &sm, float &mn, float &mx) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    if (lane == 0) { sm.red_f[w] = mn; sm.red_f[NW + w] = mx; }
+    __syncthreads();
+    mn = sm.red_f[0];
+    mx = sm.red_f[NW];
+    for (int i = 1; i < NW; ++i) { mn = fminf(mn, sm.red_f[i]); mx = fmaxf(mx, sm.red_f[NW + i]); }
+    __syncthreads();
+}
+
+// Exclusive scan of one value per thread; returns the thread's prefix and
+// writes the total to *total.
+__device__ uint32_t wg_excl_scan(SelSmem &sm, uint32_t v, uint32_t *total) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sm.red_u[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int i = 0; i < NW; ++i) { const uint32_t t = sm.red_u[i]; sm.red_u[i] = s; s += t; }
+        sm.red_u[NW] = s;
+    }
+    __syncthreads();
+    const uint32_t r = sm.red_u[w] + x - v;
+    *total = sm.red_u[NW];
+    __syncthreads();
+    return r;
+}
+
+// Turns counts h[0..nb) into incl[b] = sum_{b' >= b} h[b'] in place
+// (incl[nb] = 0).  nb is a multiple of NT.
+__device__ void wg_reverse_incl(SelSmem &sm, uint32_t *h, int nb) {
+    const int per = nb / NT;
+    const int base = threadIdx.x * per;
+    uint32_t loc[4];
+    uint32_t s = 0;
+    for (int i = 0; i < per; ++i) { loc[i] = h[base + i]; s += loc[i]; }
+    uint32_t total;
+    uint32_t e = wg_excl_scan(sm, s, &total);
+    for (int i = 0; i < per; ++i) { h[base + i] = total - e; e += loc[i]; }
+    if (threadIdx.x == 0) h[nb] = 0;
+    __syncthreads();
+}
+
+// Largest b in [0, nb) with incl[b] > t (incl non-increasing, incl[0] > t).
+__device__ __forceinline__ int find_bin(const uint32_t *incl, int nb, uint32_t t) {
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (incl[mid] > t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// t-th largest (0-based) among the elements of src[0..n) that lie in the
+// value range [lo, hi], which must hold more than t of them.  All threads
+// call it and get the value.  Each round keeps the one bin of NB2 that
+// holds the answer, so the range shrinks until it is a single value.
+__device__ float wg_select(SelSmem &sm, const float *src, int64_t n, float lo, float hi, uint32_t t) {
+    // Every round drops at least one distinct value from the range, so the
+    // loop ends for finite inputs; the round cap only guards non-finite data
+    // (rejected by the host before any kernel runs).
+    for (int round = 0; round < 256 && lo < hi; ++round) {
+        const double scale = (double)NB2 / ((double)hi - (double)lo);
+        for (int i = threadIdx.x; i < NB2; i += NT) sm.hist2[i] = 0;
+        __syncthreads();
+        for (int64_t i = threadIdx.x; i < n; i += NT) {
+            const float x = src[i];
+            if (x >= lo && x <= hi) atomicAdd(&sm.hist2[bin_of(x, lo, scale, NB2)], 1u);
+        }
+        __syncthreads();
+        wg_reverse_incl(sm, sm.hist2, NB2);
+        if (threadIdx.x == 0) sm.sel_b = find_bin(sm.hist2, NB2, t);
+        __syncthreads();
+        const int b = sm.sel_b;
+        t -= sm.hist2[b + 1];
+        float mn = INFINITY, mx = -INFINITY;
+        for (int64_t i = threadIdx.x; i < n; i += NT) {
+            const float x = src[i];
+            if (x >= lo && x <= hi && bin_of(x, lo, scale, NB2) == b) { mn = fminf(mn, x); mx = fmaxf(mx, x); }
+        }
+        wg_minmax(sm, mn, mx);
+        lo = mn;
+        hi = mx;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ scores, int64_t n_item,
+                                                   const int32_t *__restrict__ ranks, int32_t S,
+                                                   float *__restrict__ table_out) {
+    __shared__ SelSmem sm;
+    const int64_t row = blockIdx.x;
+    const float *x = scores + row * n_item;
+    float *T = table_out + row * (int64_t)S;
+    const int tid = threadIdx.x;
+
+    // 1. range
+    float mn = INFINITY, mx = -INFINITY;
+    for (int64_t i = tid; i < n_item; i += NT) { const float v = x[i]; mn = fminf(mn, v); mx = fmaxf(mx, v); }
+    wg_minmax(sm, mn, mx);
+    const float lo0 = mn, hi0 = mx;
+    if (!(lo0 < hi0) || isinf(lo0) || isinf(hi0)) {
+        for (int j = tid; j < S; j += NT) T[j] = lo0;
+        return;
+    }
+    const double scale0 = (double)NB / ((double)hi0 - (double)lo0);
+
+    // 2. histogram + reverse prefix sums
+    for (int i = tid; i < NB; i += NT) { sm.incl[i] = 0; sm.cur[i] = NONE; }
+    __syncthreads();
+    for (int64_t i = tid; i < n_item; i += NT) atomicAdd(&sm.incl[bin_of(x[i], lo0, scale0, NB)], 1u);
+    __syncthreads();
+    wg_reverse_incl(sm, sm.incl, NB);
+
+    // bin of every target
+    for (int j = tid; j < S; j += NT) sm.tb[j] = (uint16_t)find_bin(sm.incl, NB, (uint32_t)ranks[j]);
+    __syncthreads();
+
+    // 3. distinct target bins, greedily packed into passes of <= GCAP floats
+    if (tid == 0) {
+        int nfb = 0, ngrp = 0;
+        uint32_t fill = 0;
+        for (int j = 0; j < S; ++j) {
+            if (j > 0 && sm.tb[j] == sm.tb[j - 1]) continue;
+            const int b = sm.tb[j];
+            const uint32_t h = sm.incl[b] - sm.incl[b + 1];
+            sm.fb[nfb] = (uint16_t)j;
+            if (h > GCAP) {
+                sm.grp[ngrp++] = (uint16_t)(nfb | DEEP);
+                fill = GCAP;  // next bin starts a new group
+            } else {
+                if (ngrp == 0 || fill + h > GCAP) { sm.grp[ngrp++] = (uint16_t)nfb; fill = 0; }
+                fill += h;
+            }
+            ++nfb;
+        }
+        sm.grp[ngrp] = (uint16_t)nfb;
+        sm.nfb = nfb;
+        sm.ngrp = ngrp;
+    }
+    __syncthreads();
+    const int nfb = sm.nfb, ngrp = sm.ngrp;
+
+    for (int g = 0; g < ngrp; ++g) {
+        const bool deep = (sm.grp[g] & DEEP) != 0;
+        const int f0 = sm.grp[g] & ~DEEP;
+        const int f1 = sm.grp[g + 1] & ~DEEP;
+        if (deep) {
+            // one bin larger than LDS: narrow it from the row in global memory
+            const int j0 = sm.fb[f0];
+            const int j1 = (f0 + 1 < nfb) ? sm.fb[f0 + 1] : S;
+            const int b = sm.tb[j0];
+            float bmn = INFINITY, bmx = -INFINITY;
+            for (int64_t i = tid; i < n_item; i += NT) {
+                const float v = x[i];
+                if (bin_of(v, lo0, scale0, NB) == b) { bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
+            }
+            wg_minmax(sm, bmn, bmx);
+            for (int j = j0; j < j1; ++j) {
+                const uint32_t t = (uint32_t)ranks[j] - sm.incl[b + 1];
+                const float v = wg_select(sm, x, n_item, bmn, bmx, t);
+                if (tid == 0) T[j] = v;
+            }
+            continue;
+        }
+        // segment offsets of the member bins
+        if (tid == 0) {
+            uint32_t off = 0;
+            for (int f = f0; f < f1; ++f) {
+                const int b = sm.tb[sm.fb[f]];
+                sm.cur[b] = off;
+                off += sm.incl[b] - sm.incl[b + 1];
+            }
+            sm.ncoop = 0;
+        }
+        __syncthreads();
+        for (int64_t i = tid; i < n_item; i += NT) {
+            const float v = x[i];
+            const int b = bin_of(v, lo0, scale0, NB);
+            if (sm.cur[b] != NONE) sm.gbuf[atomicAdd(&sm.cur[b], 1u)] = v;
+        }
+        __syncthreads();
+        // one wavefront per bin: position of each element in descending order
+        const int w = tid >> 6, lane = tid & 63;
+        for (int f = f0 + w; f < f1; f += NW) {
+            const int j0 = sm.fb[f];
+            const int j1 = (f + 1 < nfb) ? sm.fb[f + 1] : S;
+            const int b = sm.tb[j0];
+            const uint32_t h = sm.incl[b] - sm.incl[b + 1];
+            const uint32_t seg = sm.cur[b] - h;
+            if (h > WAVE_SEL_MAX) {
+                if (lane == 0) sm.coop[atomicAdd(&sm.ncoop, 1)] = (uint16_t)f;
+                continue;
+            }
+            const uint32_t above = sm.incl[b + 1];
+            for (uint32_t i = lane; i < h; i += 64) {
+                const float e = sm.gbuf[seg + i];
+                uint32_t pos = 0;
+                for (uint32_t k = 0; k < h; ++k) {
+                    const float o = sm.gbuf[seg + k];
+                    pos += (o > e || (o == e && k < i)) ? 1u : 0u;
+                }
+                const uint32_t r = above + pos;
+                for (int j = j0; j < j1; ++j)
+                    if ((uint32_t)ranks[j] == r) T[j] = e;
+            }
+        }
+        __syncthreads();
+        const int ncoop = sm.ncoop;
+        for (int c = 0; c < ncoop; ++c) {
+            const int f = sm.coop[c];
+            const int j0 = sm.fb[f];
+            const int j1 = (f + 1 < nfb) ? sm.fb[f + 1] : S;
+            const int b = sm.tb[j0];
+            const uint32_t h = sm.incl[b] - sm.incl[b + 1];
+            const uint32_t seg = sm.cur[b] - h;
+            float bmn = INFINITY, bmx = -INFINITY;
+            for (uint32_t i = tid; i < h; i += NT) { const float v = sm.gbuf[seg + i]; bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
+            wg_minmax(sm, bmn, bmx);
+            for (int j = j0; j < j1; ++j) {
+                const uint32_t t = (uint32_t)ranks[j] - sm.incl[b + 1];
+                const float v = wg_select(sm, sm.gbuf + seg, h, bmn, bmx, t);
+                if (tid == 0) T[j] = v;
+            }
+        }
+        // release the member bins
+        __syncthreads();
+        if (tid == 0)
+            for (int f = f0; f < f1; ++f) sm.cur[sm.tb[sm.fb[f]]] = NONE;
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+hipError_t launch_select_rows(hipStream_t s, const float *scores, int64_t nb, int64_t n_item, const int32_t *ranks,
+                              int32_t S, float *table_out) {
+    if (nb <= 0) return hipSuccess;
+    if (S > SMAX || S <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_select_rows, dim3((unsigned)nb), dim3(NT), 0, s, scores, n_item, ranks, S, table_out);
+    return hipGetLastError();
+}
+
+static_assert(sizeof(SelSmem) <= 163840, "selection state must fit the 160 KB LDS of a gfx950 CU");
+
+}  // namespace rkr

@@ -1,0 +1,592 @@
+// rkr_api.cpp -- the C ABI of librkr.so (include/rkr.h): context, resident
+// device data, table build and the batched query pipeline.  Host-side logic
+// only; every score, bound, count and selection runs in the HIP kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "rkr_internal.hpp"
+
+namespace rkr {
+
+int set_err(Ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+namespace {
+
+const char *kVersion = "rkr 0.1 (gfx950)";
+
+int dev_alloc(Ctx *c, void **p, size_t bytes, const char *what) {
+    *p = nullptr;
+    if (bytes == 0) return RKR_OK;
+    if (c->held + bytes > c->budget)
+        return set_err(c, RKR_ERR_NOMEM, std::string("device memory budget exceeded allocating ") + what + " (" +
+                                             std::to_string(bytes) + " bytes, budget " + std::to_string(c->budget) +
+                                             ", held " + std::to_string(c->held) + ")");
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_err(c, RKR_ERR_NOMEM, std::string("hipMalloc ") + what + ": " + hipGetErrorString(e));
+    }
+    c->held += bytes;
+    return RKR_OK;
+}
+
+void dev_free(Ctx *c, void *p, size_t bytes) {
+    if (!p) return;
+    hipFree(p);
+    c->held -= std::min(c->held, bytes);
+}
+
+template <class T>
+int dev_alloc_t(Ctx *c, T **p, size_t n, const char *what) {
+    return dev_alloc(c, reinterpret_cast<void **>(p), n * sizeof(T), what);
+}
+
+// Grow-only scratch buffer.
+int ensure(Ctx *c, DeviceBuf &b, size_t bytes, const char *what) {
+    if (b.bytes >= bytes) return RKR_OK;
+    dev_free(c, b.p, b.bytes);
+    b.p = nullptr;
+    b.bytes = 0;
+    size_t want = std::max(bytes, b.bytes * 3 / 2);
+    int rc = dev_alloc(c, &b.p, want, what);
+    if (rc != RKR_OK) {
+        rc = dev_alloc(c, &b.p, bytes, what);
+        if (rc != RKR_OK) return rc;
+        want = bytes;
+    }
+    b.bytes = want;
+    return RKR_OK;
+}
+
+void free_table(Ctx *c) {
+    dev_free(c, c->ranks, sizeof(int32_t) * (size_t)c->n_sample);
+    dev_free(c, c->table, sizeof(float) * (size_t)c->n_user * (size_t)c->n_sample);
+    c->ranks = nullptr;
+    c->table = nullptr;
+    c->n_sample = 0;
+    c->h_ranks.clear();
+    c->table_eps = 0.0f;
+}
+
+void free_data(Ctx *c) {
+    free_table(c);
+    const size_t ld = (size_t)c->ld;
+    dev_free(c, c->users, sizeof(float) * (size_t)c->n_user * ld);
+    dev_free(c, c->usersT, sizeof(float) * (size_t)((c->n_user + 63) / 64) * 64 * ld);
+    dev_free(c, c->items, sizeof(float) * (size_t)c->n_item * ld);
+    c->users = c->usersT = c->items = nullptr;
+    c->n_user = c->n_item = 0;
+    c->dim = c->ld = 0;
+    c->h_users.clear();
+}
+
+void free_scratch(Ctx *c) {
+    for (DeviceBuf *b : {&c->q_in, &c->qip, &c->bk, &c->qinfo, &c->cand, &c->refine_idx, &c->counts, &c->out,
+                         &c->misc}) {
+        dev_free(c, b->p, b->bytes);
+        b->p = nullptr;
+        b->bytes = 0;
+    }
+}
+
+// FP32 normalisation exactly as VectorMatrix::vectorNormalize
+// (src/include/struct/VectorMatrix.hpp:51-66).  This file is compiled with
+// -ffp-contract=off, so x*x is rounded before the add.
+void normalize_rows(float *u, int64_t n, int32_t d) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        float *row = u + i * d;
+        float l2 = 0.0f;
+        for (int32_t j = 0; j < d; ++j) {
+            const float sq = row[j] * row[j];
+            l2 = l2 + sq;
+        }
+        l2 = std::sqrt(l2);
+        if ((double)l2 <= 0.001) continue;
+        for (int32_t j = 0; j < d; ++j) row[j] = row[j] / l2;
+    }
+}
+
+// Largest |x| over a row-major matrix; -1 when a value is NaN or infinite.
+double max_abs(const float *x, size_t n) {
+    double m = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const float v = x[i];
+        if (!std::isfinite(v)) return -1.0;
+        const double a = std::fabs((double)v);
+        if (a > m) m = a;
+    }
+    return m;
+}
+
+// Every partial sum of a score is bounded by dim * max|a| * max|b|; keep it
+// well inside FP32 range so no score overflows.
+constexpr double kScoreLimit = 1e37;
+
+bool ranks_valid(const int32_t *r, int32_t S, int64_t n_item) {
+    if (S <= 0 || (int64_t)S >= n_item) return false;
+    for (int32_t j = 0; j < S; ++j) {
+        if (r[j] < 0 || (int64_t)r[j] >= n_item) return false;
+        if (j > 0 && r[j] <= r[j - 1]) return false;
+    }
+    return true;
+}
+
+double ev_ms(Ctx *c, int a, int b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0.0;
+    return ms;
+}
+
+}  // namespace
+}  // namespace rkr
+
+using namespace rkr;
+
+struct rkr_ctx : public rkr::Ctx {};
+
+extern "C" {
+
+const char *rkr_version(void) { return kVersion; }
+
+int rkr_create(int device, size_t mem_budget_bytes, rkr_ctx **out) {
+    if (!out) return RKR_ERR_INVALID;
+    *out = nullptr;
+    int n_dev = 0;
+    if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev <= 0) return RKR_ERR_NODEVICE;
+    if (device < 0 || device >= n_dev) return RKR_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return RKR_ERR_HIP;
+    rkr_ctx *c = new (std::nothrow) rkr_ctx();
+    if (!c) return RKR_ERR_NOMEM;
+    c->device = device;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { delete c; return RKR_ERR_HIP; }
+    c->budget = mem_budget_bytes ? std::min(mem_budget_bytes, free_b) : (size_t)(0.9 * (double)free_b);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return RKR_ERR_HIP; }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) { rkr_destroy(c); return RKR_ERR_HIP; }
+    *out = c;
+    return RKR_OK;
+}
+
+void rkr_destroy(rkr_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_scratch(c);
+    free_data(c);
+    for (auto &e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *rkr_last_error(const rkr_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int rkr_set_option(rkr_ctx *c, int option, int64_t value) {
+    if (!c) return RKR_ERR_INVALID;
+    switch (option) {
+        case RKR_OPT_FORM:
+            if (value != RKR_FORM_PLAIN && value != RKR_FORM_SCREENED)
+                return set_err(c, RKR_ERR_INVALID, "unknown form");
+            c->form = (int)value;
+            return RKR_OK;
+        case RKR_OPT_OUTPUT:
+            if (value != RKR_OUTPUT_EXACT && value != RKR_OUTPUT_REFERENCE)
+                return set_err(c, RKR_ERR_INVALID, "unknown output order");
+            c->output = (int)value;
+            return RKR_OK;
+        case RKR_OPT_QUERY_BATCH:
+            if (value < 0) return set_err(c, RKR_ERR_INVALID, "query batch must be >= 0");
+            c->query_batch = value;
+            return RKR_OK;
+        case RKR_OPT_BUILD_BATCH:
+            if (value < 0) return set_err(c, RKR_ERR_INVALID, "build batch must be >= 0");
+            c->build_batch = value;
+            return RKR_OK;
+        default:
+            return set_err(c, RKR_ERR_INVALID, "unknown option");
+    }
+}
+
+int rkr_get_option(const rkr_ctx *c, int option, int64_t *value) {
+    if (!c || !value) return RKR_ERR_INVALID;
+    switch (option) {
+        case RKR_OPT_FORM: *value = c->form; return RKR_OK;
+        case RKR_OPT_OUTPUT: *value = c->output; return RKR_OK;
+        case RKR_OPT_QUERY_BATCH: *value = c->query_batch; return RKR_OK;
+        case RKR_OPT_BUILD_BATCH: *value = c->build_batch; return RKR_OK;
+        default: return RKR_ERR_INVALID;
+    }
+}
+
+int rkr_sync(rkr_ctx *c) {
+    if (!c) return RKR_ERR_INVALID;
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    return RKR_OK;
+}
+
+int rkr_normalize_users(float *users, int64_t n_user, int32_t dim) {
+    if (!users || n_user < 0 || dim <= 0) return RKR_ERR_INVALID;
+    normalize_rows(users, n_user, dim);
+    return RKR_OK;
+}
+
+int rkr_uniform_sample_ranks(int64_t n_item, int32_t n_sample, int32_t *out) {
+    // FindUniformSample, UniformSample.hpp:17-25
+    if (!out || n_sample <= 0 || (int64_t)n_sample >= n_item) return RKR_ERR_INVALID;
+    const int end_rank = (int)n_item - 1;
+    const double delta = (end_rank - 0) * 1.0 / n_sample;
+    for (int32_t j = 0; j < n_sample; ++j) out[j] = (int32_t)std::floor(j * delta);
+    return RKR_OK;
+}
+
+int rkr_set_data(rkr_ctx *c, const float *users, int64_t n_user, const float *items, int64_t n_item, int32_t dim,
+                 int normalize_users) {
+    if (!c) return RKR_ERR_INVALID;
+    if (!users || !items || n_user <= 0 || n_item <= 0 || dim <= 0)
+        return set_err(c, RKR_ERR_INVALID, "set_data: empty or null input");
+    if (n_user > INT32_MAX || n_item > INT32_MAX) return set_err(c, RKR_ERR_INVALID, "set_data: too many rows");
+    if (dim > 4096) return set_err(c, RKR_ERR_INVALID, "set_data: dim > 4096 unsupported");
+    {
+        const double mu = max_abs(users, (size_t)n_user * dim), mp = max_abs(items, (size_t)n_item * dim);
+        if (mu < 0 || mp < 0) return set_err(c, RKR_ERR_INVALID, "set_data: users and items must be finite");
+        const double mu_n = normalize_users ? 1.0 : mu;  // normalised rows have |u_i| <= 1
+        c->max_abs_user = mu_n;
+        if ((double)dim * mu_n * mp > kScoreLimit)
+            return set_err(c, RKR_ERR_INVALID, "set_data: magnitudes would overflow FP32 scores");
+    }
+    RKR_HIP(c, hipSetDevice(c->device));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    free_scratch(c);
+    free_data(c);
+    const int32_t ld = padded_dim(dim);
+    c->n_user = n_user;
+    c->n_item = n_item;
+    c->dim = dim;
+    c->ld = ld;
+    // host staging with padding
+    c->h_users.assign(users, users + (size_t)n_user * dim);
+    if (normalize_users) normalize_rows(c->h_users.data(), n_user, dim);
+    // upload row-major with rows padded to ld floats (pad columns zero)
+    auto upload = [&](const float *src, int64_t n, float **dst, const char *what) -> int {
+        int rc = dev_alloc_t(c, dst, (size_t)n * ld, what);
+        if (rc) return rc;
+        const int64_t chunk = 65536;
+        std::vector<float> tmp((size_t)std::min<int64_t>(chunk, n) * ld, 0.0f);
+        for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+            const int64_t nr = std::min<int64_t>(chunk, n - r0);
+            for (int64_t r = 0; r < nr; ++r)
+                std::memcpy(&tmp[(size_t)r * ld], src + (size_t)(r0 + r) * dim, sizeof(float) * dim);
+            RKR_HIP(c, hipMemcpy(*dst + r0 * ld, tmp.data(), sizeof(float) * (size_t)nr * ld, hipMemcpyHostToDevice));
+        }
+        return RKR_OK;
+    };
+    int rc = upload(c->h_users.data(), n_user, &c->users, "users");
+    if (rc) { free_data(c); return rc; }
+    rc = upload(items, n_item, &c->items, "items");
+    if (rc) { free_data(c); return rc; }
+    rc = dev_alloc_t(c, &c->usersT, (size_t)((n_user + 63) / 64) * 64 * ld, "user panels");
+    if (rc) { free_data(c); return rc; }
+    RKR_HIP(c, launch_make_panels(c->stream, c->users, n_user, ld, c->usersT));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    return RKR_OK;
+}
+
+int rkr_build_table(rkr_ctx *c, const int32_t *ranks, int32_t S, double *build_ms) {
+    if (!c) return RKR_ERR_INVALID;
+    if (!c->users) return set_err(c, RKR_ERR_STATE, "build_table: no data");
+    if (!ranks || S > kMaxSample || !ranks_valid(ranks, S, c->n_item))
+        return set_err(c, RKR_ERR_INVALID, "build_table: ranks must be strictly increasing in [0, n_item), "
+                                           "0 < n_sample < n_item, n_sample <= " + std::to_string(kMaxSample));
+    RKR_HIP(c, hipSetDevice(c->device));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    free_table(c);
+    int rc = dev_alloc_t(c, &c->ranks, (size_t)S, "ranks");
+    if (rc) return rc;
+    rc = dev_alloc_t(c, &c->table, (size_t)c->n_user * S, "table");
+    if (rc) { free_table(c); return rc; }
+    c->n_sample = S;
+    c->h_ranks.assign(ranks, ranks + S);
+    RKR_HIP(c, hipMemcpyAsync(c->ranks, ranks, sizeof(int32_t) * S, hipMemcpyHostToDevice, c->stream));
+
+    // user batch: the score block [nb x n_item] lives in HBM between the
+    // GEMM and the selection; size it from the remaining budget (the
+    // reference sizes its batch from free memory the same way,
+    // GPUComputeSort.hpp:112-135).
+    const size_t row_bytes = sizeof(float) * (size_t)c->n_item;
+    size_t avail = c->budget > c->held ? c->budget - c->held : 0;
+    int64_t nb = (int64_t)std::min<size_t>(avail / 2 / row_bytes, (size_t)(8ull << 30) / row_bytes);
+    if (c->build_batch > 0) nb = std::min<int64_t>(nb, c->build_batch);
+    nb = std::min<int64_t>(nb, c->n_user);
+    if (nb > 128) nb = nb / 128 * 128;
+    if (nb <= 0) { free_table(c); return set_err(c, RKR_ERR_NOMEM, "build_table: no memory for a score block"); }
+    DeviceBuf &blk = c->misc;
+    rc = ensure(c, blk, (size_t)nb * row_bytes, "score block");
+    if (rc) { free_table(c); return rc; }
+    float *scores = reinterpret_cast<float *>(blk.p);
+    RKR_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    for (int64_t u0 = 0; u0 < c->n_user; u0 += nb) {
+        const int64_t n = std::min<int64_t>(nb, c->n_user - u0);
+        RKR_HIP(c, launch_scores_store(c->stream, c->users + u0 * c->ld, nullptr, n, c->ld, c->items, c->n_item, c->ld,
+                                       c->dim, scores, c->n_item, true));
+        RKR_HIP(c, launch_select_rows(c->stream, scores, n, c->n_item, c->ranks, S, c->table + u0 * S));
+    }
+    RKR_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    if (build_ms) *build_ms = ev_ms(c, 0, 1);
+    return RKR_OK;
+}
+
+int rkr_set_table(rkr_ctx *c, const int32_t *ranks, int32_t S, const float *table, float eps) {
+    if (!c) return RKR_ERR_INVALID;
+    if (!c->users) return set_err(c, RKR_ERR_STATE, "set_table: no data");
+    if (!ranks || !table || S > kMaxSample || !ranks_valid(ranks, S, c->n_item))
+        return set_err(c, RKR_ERR_INVALID, "set_table: bad ranks");
+    if (!(eps >= 0.0f) || std::isinf(eps)) return set_err(c, RKR_ERR_INVALID, "set_table: eps must be finite, >= 0");
+    RKR_HIP(c, hipSetDevice(c->device));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    free_table(c);
+    int rc = dev_alloc_t(c, &c->ranks, (size_t)S, "ranks");
+    if (rc) return rc;
+    rc = dev_alloc_t(c, &c->table, (size_t)c->n_user * S, "table");
+    if (rc) { free_table(c); return rc; }
+    c->n_sample = S;
+    c->h_ranks.assign(ranks, ranks + S);
+    c->table_eps = eps;
+    RKR_HIP(c, hipMemcpy(c->ranks, ranks, sizeof(int32_t) * S, hipMemcpyHostToDevice));
+    RKR_HIP(c, hipMemcpy(c->table, table, sizeof(float) * (size_t)c->n_user * S, hipMemcpyHostToDevice));
+    return RKR_OK;
+}
+
+int rkr_get_table(rkr_ctx *c, int64_t user_begin, int64_t n_rows, float *out) {
+    if (!c || !out) return RKR_ERR_INVALID;
+    if (!c->table) return set_err(c, RKR_ERR_STATE, "get_table: no table");
+    if (user_begin < 0 || n_rows < 0 || user_begin + n_rows > c->n_user)
+        return set_err(c, RKR_ERR_INVALID, "get_table: row range out of bounds");
+    RKR_HIP(c, hipSetDevice(c->device));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    RKR_HIP(c, hipMemcpy(out, c->table + user_begin * c->n_sample, sizeof(float) * (size_t)n_rows * c->n_sample,
+                         hipMemcpyDeviceToHost));
+    return RKR_OK;
+}
+
+int rkr_get_ranks(const rkr_ctx *c, int32_t *out, int32_t *n_sample) {
+    if (!c) return RKR_ERR_INVALID;
+    if (n_sample) *n_sample = c->n_sample;
+    if (out && c->n_sample) std::memcpy(out, c->h_ranks.data(), sizeof(int32_t) * c->n_sample);
+    return RKR_OK;
+}
+
+int rkr_save_table(rkr_ctx *c, const char *path) {
+    // SampleSearch::SaveIndex layout (SampleSearch.hpp:186-194)
+    if (!c || !path) return RKR_ERR_INVALID;
+    if (!c->table) return set_err(c, RKR_ERR_STATE, "save_table: no table");
+    std::vector<float> host((size_t)c->n_user * c->n_sample);
+    int rc = rkr_get_table(c, 0, c->n_user, host.data());
+    if (rc) return rc;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return set_err(c, RKR_ERR_IO, std::string("save_table: cannot open ") + path);
+    const size_t hdr[3] = {(size_t)c->n_sample, (size_t)c->n_item, (size_t)c->n_user};
+    bool ok = std::fwrite(hdr, sizeof(size_t), 3, f) == 3;
+    ok = ok && std::fwrite(c->h_ranks.data(), sizeof(int32_t), c->n_sample, f) == (size_t)c->n_sample;
+    ok = ok && std::fwrite(host.data(), sizeof(float), host.size(), f) == host.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return set_err(c, RKR_ERR_IO, std::string("save_table: short write to ") + path);
+    return RKR_OK;
+}
+
+int rkr_load_table(rkr_ctx *c, const char *path, float eps) {
+    // SampleSearch::LoadIndex layout (SampleSearch.hpp:229-241)
+    if (!c || !path) return RKR_ERR_INVALID;
+    if (!c->users) return set_err(c, RKR_ERR_STATE, "load_table: no data");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return set_err(c, RKR_ERR_IO, std::string("load_table: cannot open ") + path);
+    size_t hdr[3];
+    if (std::fread(hdr, sizeof(size_t), 3, f) != 3) { std::fclose(f); return set_err(c, RKR_ERR_IO, "load_table: short header"); }
+    const size_t S = hdr[0], I = hdr[1], U = hdr[2];
+    if ((int64_t)I != c->n_item || (int64_t)U != c->n_user || S == 0 || S > (size_t)kMaxSample) {
+        std::fclose(f);
+        return set_err(c, RKR_ERR_IO, "load_table: header (n_sample " + std::to_string(S) + ", n_item " +
+                                          std::to_string(I) + ", n_user " + std::to_string(U) +
+                                          ") does not match the data");
+    }
+    std::vector<int32_t> r(S);
+    std::vector<float> t(S * U);
+    bool ok = std::fread(r.data(), sizeof(int32_t), S, f) == S;
+    ok = ok && std::fread(t.data(), sizeof(float), t.size(), f) == t.size();
+    std::fclose(f);
+    if (!ok) return set_err(c, RKR_ERR_IO, "load_table: file too short");
+    return rkr_set_table(c, r.data(), (int32_t)S, t.data(), eps);
+}
+
+int64_t rkr_index_size_bytes(const rkr_ctx *c) {
+    if (!c) return 0;
+    return (int64_t)sizeof(int32_t) * c->n_sample + (int64_t)sizeof(float) * c->n_user * c->n_sample;
+}
+
+int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k,
+              int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
+              rkr_batch_timing *timing) {
+    using clock = std::chrono::steady_clock;
+    const auto t_start = clock::now();
+    if (!c) return RKR_ERR_INVALID;
+    if (!c->table) return set_err(c, RKR_ERR_STATE, "query: no table (build or load one first)");
+    if (n_query < 0 || (n_query > 0 && (!queries || !topk || !out_user || !out_rank)))
+        return set_err(c, RKR_ERR_INVALID, "query: null buffer");
+    if (max_k <= 0 || max_k > kMaxTopk) return set_err(c, RKR_ERR_INVALID, "query: max_k must be in [1, 4096]");
+    for (int32_t i = 0; i < n_query; ++i)
+        if (topk[i] <= 0 || topk[i] > max_k || (int64_t)topk[i] > c->n_user)
+            return set_err(c, RKR_ERR_INVALID, "query: topk[" + std::to_string(i) + "] = " + std::to_string(topk[i]) +
+                                                   " outside [1, min(n_user, max_k)]");
+    if (n_query > 0) {
+        const double mq = max_abs(queries, (size_t)n_query * c->dim);
+        if (mq < 0) return set_err(c, RKR_ERR_INVALID, "query: query vectors must be finite");
+        if ((double)c->dim * mq * c->max_abs_user > kScoreLimit)
+            return set_err(c, RKR_ERR_INVALID, "query: magnitudes would overflow FP32 scores");
+    }
+    if (timing) std::memset(timing, 0, sizeof(*timing));
+    if (n_query == 0) return RKR_OK;
+    RKR_HIP(c, hipSetDevice(c->device));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+
+    const int64_t U = c->n_user, I = c->n_item;
+    const int32_t S = c->n_sample, ld = c->ld, d = c->dim;
+    const int mode = c->output;
+    const float eps = mode == RKR_OUTPUT_EXACT ? c->table_eps : 0.0f;
+
+    // queries per device batch: qip + buckets cost 8 bytes per (query, user)
+    int64_t nqb = c->query_batch > 0 ? c->query_batch : 1024;
+    {
+        size_t avail = c->budget > c->held ? c->budget - c->held : 0;
+        const int64_t by_mem = (int64_t)(avail / 4 / ((size_t)U * 8 + 1));
+        nqb = std::max<int64_t>(1, std::min<int64_t>(nqb, by_mem));
+        nqb = std::min<int64_t>(nqb, n_query);
+        nqb = std::min<int64_t>(nqb, 65535);
+    }
+    int rc;
+    if ((rc = ensure(c, c->q_in, sizeof(float) * (size_t)nqb * ld, "query vectors"))) return rc;
+    if ((rc = ensure(c, c->qip, sizeof(float) * (size_t)nqb * U, "query scores"))) return rc;
+    if ((rc = ensure(c, c->bk, sizeof(uint32_t) * (size_t)nqb * U, "bucket pairs"))) return rc;
+    if ((rc = ensure(c, c->qinfo, sizeof(QueryInfo) * (size_t)nqb + 64, "query info"))) return rc;
+    if ((rc = ensure(c, c->out, (sizeof(int32_t) * 2 + sizeof(float)) * (size_t)nqb * max_k, "outputs"))) return rc;
+
+    std::vector<float> hq((size_t)nqb * ld, 0.0f);
+    std::vector<QueryInfo> hqi((size_t)nqb);
+    std::vector<int32_t> ho_user((size_t)nqb * max_k), ho_rank((size_t)nqb * max_k);
+    std::vector<float> ho_qip((size_t)nqb * max_k);
+    rkr_batch_timing tm{};
+
+    for (int64_t q0 = 0; q0 < n_query; q0 += nqb) {
+        const int32_t nq = (int32_t)std::min<int64_t>(nqb, n_query - q0);
+        float *q_in = reinterpret_cast<float *>(c->q_in.p);
+        float *qip = reinterpret_cast<float *>(c->qip.p);
+        uint32_t *bk = reinterpret_cast<uint32_t *>(c->bk.p);
+        QueryInfo *qi = reinterpret_cast<QueryInfo *>(c->qinfo.p);
+        int32_t *refine_total = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(c->qinfo.p) + sizeof(QueryInfo) * nqb);
+        int32_t *o_user = reinterpret_cast<int32_t *>(c->out.p);
+        int32_t *o_rank = o_user + (size_t)nqb * max_k;
+        float *o_qip = reinterpret_cast<float *>(o_rank + (size_t)nqb * max_k);
+
+        for (int32_t i = 0; i < nq; ++i) {
+            std::memcpy(&hq[(size_t)i * ld], queries + (size_t)(q0 + i) * d, sizeof(float) * d);
+            hqi[i] = QueryInfo{};
+            hqi[i].topk = topk[q0 + i];
+        }
+        RKR_HIP(c, hipEventRecord(c->ev[0], c->stream));
+        RKR_HIP(c, hipMemcpyAsync(q_in, hq.data(), sizeof(float) * (size_t)nq * ld, hipMemcpyHostToDevice, c->stream));
+        RKR_HIP(c, hipMemcpyAsync(qi, hqi.data(), sizeof(QueryInfo) * nq, hipMemcpyHostToDevice, c->stream));
+        RKR_HIP(c, hipEventRecord(c->ev[1], c->stream));
+        // <u, q> for every (query, user): qip[b * U + u]
+        if (nq <= 16 && (size_t)nq * d * sizeof(float) <= 65536)
+            RKR_HIP(c, launch_gemv_panels(c->stream, c->usersT, U, ld, d, q_in, nq, qip));
+        else RKR_HIP(c, launch_scores_store(c->stream, c->users, nullptr, U, ld, q_in, nq, ld, d, qip, U, false));
+        RKR_HIP(c, hipEventRecord(c->ev[2], c->stream));
+        RKR_HIP(c, launch_bounds(c->stream, qip, c->table, U, S, nq, eps, bk));
+        RKR_HIP(c, hipEventRecord(c->ev[3], c->stream));
+        RKR_HIP(c, launch_hist_kth(c->stream, bk, U, S, c->ranks, I, nq, mode, qi));
+        RKR_HIP(c, launch_offsets(c->stream, qi, nq, refine_total));
+        RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, c->stream));
+        RKR_HIP(c, hipStreamSynchronize(c->stream));
+        const int64_t total = hqi[nq - 1].cand_off + hqi[nq - 1].n_settled + hqi[nq - 1].n_cand;
+        int64_t total_cand = 0;
+        for (int32_t i = 0; i < nq; ++i) total_cand += hqi[i].n_cand;
+        if (total >= INT32_MAX) return set_err(c, RKR_ERR_NOMEM, "query: candidate count exceeds 2^31; lower the query batch");
+        const size_t cap = (size_t)std::max<int64_t>(total, 1);
+        const size_t rcap = (size_t)std::max<int64_t>(total_cand, 1);
+        if ((rc = ensure(c, c->cand, sizeof(int32_t) * 2 * cap, "candidates"))) return rc;
+        if ((rc = ensure(c, c->refine_idx, (sizeof(int32_t) * 2 + sizeof(float)) * rcap, "refine rows"))) return rc;
+        if ((rc = ensure(c, c->counts, sizeof(int32_t) * rcap, "refine counts"))) return rc;
+        int32_t *cand_user = reinterpret_cast<int32_t *>(c->cand.p);
+        int32_t *cand_rank = cand_user + cap;
+        int32_t *refine_cand = reinterpret_cast<int32_t *>(c->refine_idx.p);
+        int32_t *refine_user = refine_cand + rcap;
+        float *refine_thr = reinterpret_cast<float *>(refine_user + rcap);
+        int32_t *cnt = reinterpret_cast<int32_t *>(c->counts.p);
+
+        RKR_HIP(c, launch_compact(c->stream, bk, qip, U, S, c->ranks, I, nq, mode, qi, cand_user, cand_rank,
+                                  refine_total, refine_cand, refine_user, refine_thr));
+        RKR_HIP(c, hipEventRecord(c->ev[4], c->stream));
+        RKR_HIP(c, hipMemsetAsync(cnt, 0, sizeof(int32_t) * rcap, c->stream));
+        RKR_HIP(c, launch_refine_count(c->stream, c->users, c->items, I, ld, d, refine_user, refine_thr, refine_total,
+                                       (int64_t)rcap, cnt));
+        RKR_HIP(c, launch_scatter_counts(c->stream, refine_total, refine_cand, cnt, (int64_t)rcap, cand_rank));
+        RKR_HIP(c, hipEventRecord(c->ev[5], c->stream));
+        RKR_HIP(c, launch_topk(c->stream, qi, nq, cand_user, cand_rank, qip, U, max_k, o_user, o_rank, o_qip, nullptr));
+        RKR_HIP(c, hipEventRecord(c->ev[6], c->stream));
+        RKR_HIP(c, hipMemcpyAsync(ho_user.data(), o_user, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));
+        RKR_HIP(c, hipMemcpyAsync(ho_rank.data(), o_rank, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));
+        RKR_HIP(c, hipMemcpyAsync(ho_qip.data(), o_qip, sizeof(float) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));
+        RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, c->stream));
+        int32_t h_refine_total = 0;
+        RKR_HIP(c, hipMemcpyAsync(&h_refine_total, refine_total, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        RKR_HIP(c, hipEventRecord(c->ev[7], c->stream));
+        RKR_HIP(c, hipStreamSynchronize(c->stream));
+
+        std::memcpy(out_user + (size_t)q0 * max_k, ho_user.data(), sizeof(int32_t) * (size_t)nq * max_k);
+        std::memcpy(out_rank + (size_t)q0 * max_k, ho_rank.data(), sizeof(int32_t) * (size_t)nq * max_k);
+        if (out_qip) std::memcpy(out_qip + (size_t)q0 * max_k, ho_qip.data(), sizeof(float) * (size_t)nq * max_k);
+        if (stats) {
+            for (int32_t i = 0; i < nq; ++i) {
+                rkr_query_stats &st = stats[q0 + i];
+                std::memset(&st, 0, sizeof(st));
+                st.query_id = (int32_t)(q0 + i);
+                st.n_candidate = hqi[i].n_cand;
+                st.n_refine_user = hqi[i].n_refine;
+                if (mode == RKR_OUTPUT_EXACT) {
+                    st.n_result_user = hqi[i].n_cand - hqi[i].n_refine;
+                    st.n_prune_user = (int32_t)(U - hqi[i].n_cand);
+                } else {
+                    st.n_result_user = hqi[i].n_settled;
+                    st.n_prune_user = (int32_t)(U - hqi[i].n_settled - hqi[i].n_cand);
+                }
+                st.ip_cost = U + (int64_t)hqi[i].n_refine * I;
+            }
+        }
+        tm.h2d_ms += ev_ms(c, 0, 1);
+        tm.ip_ms += ev_ms(c, 1, 2);
+        tm.bound_ms += ev_ms(c, 2, 3);
+        tm.prune_ms += ev_ms(c, 3, 4);
+        tm.refine_ms += ev_ms(c, 4, 5);
+        tm.topk_ms += ev_ms(c, 5, 6);
+        tm.d2h_ms += ev_ms(c, 6, 7);
+        tm.n_batches += 1;
+        tm.n_refined += h_refine_total;
+    }
+    tm.total_ms = std::chrono::duration<double, std::milli>(clock::now() - t_start).count();
+    if (timing) *timing = tm;
+    return RKR_OK;
+}
+
+}  // extern "C"

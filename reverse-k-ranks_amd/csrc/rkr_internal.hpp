@@ -1,0 +1,125 @@
+// rkr_internal.hpp -- context layout and kernel launchers shared by the host
+// code (rkr_api.cpp) and the HIP kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rkr.h"
+
+namespace rkr {
+
+// Device storage pads each vector to a multiple of 4 floats so rows can be
+// read as float4; the score loops still run over exactly `dim` terms.
+inline int32_t padded_dim(int32_t d) { return (d + 3) & ~3; }
+
+// Largest n_sample the query kernels accept: bucket indices are packed as
+// two uint16 values and the per-query histograms live in LDS.
+constexpr int32_t kMaxSample = 4096;
+// Largest k a query may ask for (top-k selection sorts <= kMaxTopk keys in LDS).
+constexpr int32_t kMaxTopk = 4096;
+
+struct DeviceBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    size_t budget = 0;      // bytes the context may hold on the device
+    size_t held = 0;        // bytes currently held
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[16] = {};
+    std::string err;
+
+    int form = RKR_FORM_PLAIN;
+    int output = RKR_OUTPUT_EXACT;
+    int64_t query_batch = 0;
+    int64_t build_batch = 0;
+
+    // data
+    int64_t n_user = 0, n_item = 0;
+    int32_t dim = 0, ld = 0;
+    float *users = nullptr;     // n_user x ld, row-major (normalised)
+    float *usersT = nullptr;    // panels of 64 users: [panel][ld][64]
+    float *items = nullptr;     // n_item x ld, row-major (original order)
+    std::vector<float> h_users; // host copy (normalised)
+    double max_abs_user = 1.0;  // bound on |u_i| after normalisation
+
+    // table
+    int32_t n_sample = 0;
+    int32_t *ranks = nullptr;   // n_sample
+    float *table = nullptr;     // n_user x n_sample
+    float table_eps = 0.0f;
+    std::vector<int32_t> h_ranks;
+
+    // query scratch (grown on demand)
+    DeviceBuf q_in, qip, bk, qinfo, cand, refine_idx, counts, out, misc;
+};
+
+// ---- error helpers -------------------------------------------------------
+int set_err(Ctx *c, int code, const std::string &msg);
+#define RKR_HIP(ctx, call)                                                            \
+    do {                                                                              \
+        hipError_t e__ = (call);                                                      \
+        if (e__ != hipSuccess)                                                        \
+            return ::rkr::set_err((ctx), RKR_ERR_HIP,                                 \
+                                  std::string(#call) + ": " + hipGetErrorString(e__)); \
+    } while (0)
+
+// ---- kernel launchers (k_gemm.hip) ----------------------------------------
+// C = A * B^T with the sequential FP32 score rule.  A rows are gathered
+// through a_rows when non-null.  Store epilogue: C[n * ldc_n + m] (query-major
+// when A = users, B = queries).
+hipError_t launch_scores_store(hipStream_t s, const float *A, const int32_t *a_rows, int64_t M, int32_t lda,
+                               const float *B, int64_t N, int32_t ldb, int32_t K, float *C, int64_t ldc,
+                               bool c_row_major_m);
+// Small-batch <u,q>: qip[b * n_user + u] for b < nq (nq <= 16), users in
+// 64-user panels.
+hipError_t launch_gemv_panels(hipStream_t s, const float *usersT, int64_t n_user, int32_t ld, int32_t K,
+                              const float *q, int32_t nq, float *qip);
+// Refinement count: cnt[r] = #{item p : score(user(r), p) > thr(r)} for every
+// refine row r < *n_rows_dev (device-side count, persistent grid).
+hipError_t launch_refine_count(hipStream_t s, const float *users, const float *items, int64_t n_item,
+                               int32_t ld, int32_t K, const int32_t *rows_user, const float *rows_thr,
+                               const int32_t *n_rows_dev, int64_t max_rows, int32_t *cnt);
+// Transpose users into 64-user panels.
+hipError_t launch_make_panels(hipStream_t s, const float *users, int64_t n_user, int32_t ld, float *usersT);
+
+// ---- selection (k_select.hip) ---------------------------------------------
+// table[(u0 + b) * S + j] = r_j-th largest of scores[b * n_item + ...], b < nb.
+hipError_t launch_select_rows(hipStream_t s, const float *scores, int64_t nb, int64_t n_item,
+                              const int32_t *ranks, int32_t S, float *table_out);
+
+// ---- query pipeline (k_query.hip) -------------------------------------------
+struct QueryInfo {          // one per query of a device batch
+    int32_t topk;
+    int32_t beta;           // bucket index of the k-th smallest lower bound
+    int32_t gamma;          // largest upper-bound bucket that stays a candidate
+    int32_t n_cand;         // candidates kept
+    int32_t n_settled;      // reference output: users settled by the bounds
+    int32_t n_refine;       // candidates whose rank is counted
+    int32_t mode;           // classification case (reference output)
+    int32_t pad;
+    int64_t cand_off;       // offset of this query's candidates
+};
+
+hipError_t launch_bounds(hipStream_t s, const float *qip, const float *table, int64_t n_user, int32_t S,
+                         int32_t nq, float eps, uint32_t *bk);
+hipError_t launch_hist_kth(hipStream_t s, const uint32_t *bk, int64_t n_user, int32_t S, const int32_t *ranks,
+                           int64_t n_item, int32_t nq, int output_mode, QueryInfo *qi);
+hipError_t launch_offsets(hipStream_t s, QueryInfo *qi, int32_t nq, int32_t *refine_total);
+hipError_t launch_compact(hipStream_t s, const uint32_t *bk, const float *qip, int64_t n_user, int32_t S,
+                          const int32_t *ranks, int64_t n_item, int32_t nq, int output_mode, QueryInfo *qi,
+                          int32_t *cand_user, int32_t *cand_rank, int32_t *refine_total, int32_t *refine_cand,
+                          int32_t *refine_user, float *refine_thr);
+hipError_t launch_scatter_counts(hipStream_t s, const int32_t *refine_total, const int32_t *refine_cand,
+                                 const int32_t *cnt, int64_t max_rows, int32_t *cand_rank);
+hipError_t launch_topk(hipStream_t s, const QueryInfo *qi, int32_t nq, const int32_t *cand_user,
+                       const int32_t *cand_rank, const float *qip, int64_t n_user, int32_t max_k,
+                       int32_t *out_user, int32_t *out_rank, float *out_qip, int64_t *scratch_keys);
+
+}  // namespace rkr

@@ -1,0 +1,199 @@
+"""GPU parity: librkr.so (HIP, gfx950) against the CPU oracle.
+
+Integer outputs (user IDs, ranks) must be bit-exact; scores (table entries,
+<u,q>) follow the reference's sequential FP32 InnerProduct rule and must be
+bit-identical too (compared as floats, so -0.0 == +0.0).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def _normalized(orc, users):
+    return orc.normalize_users(users)
+
+
+def test_table_fake30d_matches_oracle(orc, fake30d, gpu_index_factory):
+    users, items, _ = fake30d
+    _case("table fake-normal-30d S=128")
+    idx, ranks = gpu_index_factory(users, items, 128)
+    T = idx.get_table()
+    T_ref = orc.build_table(_normalized(orc, users), items, ranks)
+    assert T.shape == T_ref.shape
+    assert np.array_equal(T, T_ref)
+
+
+@pytest.mark.parametrize("U,I,d,S,seed", [
+    (300, 1000, 30, 16, 1),      # small
+    (257, 4099, 7, 100, 2),      # ragged: I, d not multiples of 4/128
+    (130, 20000, 150, 512, 3),   # plain-GPU row width
+    (64, 50001, 33, 512, 4),     # row larger than one LDS gather pass
+    (5, 3, 4, 2, 5),             # tiny
+])
+def test_table_random_matches_oracle(orc, gpu_index_factory, U, I, d, S, seed):
+    _case(f"table U={U} I={I} d={d} S={S}")
+    rng = np.random.default_rng(seed)
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = rng.standard_normal((I, d), dtype=np.float32) * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32)
+    idx, ranks = gpu_index_factory(users, items, S)
+    T = idx.get_table()
+    T_ref = orc.build_table(_normalized(orc, users), items, ranks)
+    assert np.array_equal(T, T_ref)
+
+
+def test_table_ties_and_degenerate_rows(orc, gpu_index_factory):
+    """Duplicate items (score ties), a zero user (all scores 0), a one-hot user
+    (mostly equal scores) and heavy value clustering."""
+    _case("table ties")
+    rng = np.random.default_rng(7)
+    d, I = 12, 3000
+    base = rng.standard_normal((40, d), dtype=np.float32)
+    items = base[rng.integers(0, 40, I)]                     # only 40 distinct items
+    items[:500] = 0.0                                        # many zero scores
+    users = rng.standard_normal((50, d), dtype=np.float32)
+    users[0] = 0.0                                           # zero user: norm <= 0.001, left unnormalised
+    users[1] = 0.0
+    users[1, 3] = 1.0                                        # one-hot
+    idx, ranks = gpu_index_factory(users, items, 300)
+    T = idx.get_table()
+    T_ref = orc.build_table(_normalized(orc, users), items, ranks)
+    assert np.array_equal(T, T_ref)
+
+
+def _check_exact(orc, users, items, queries, res, ks):
+    ub, rb, qb = orc.bruteforce(users, items, queries, int(max(ks)))
+    for i, k in enumerate(ks):
+        assert np.array_equal(res.users[i, :k], ub[i, :k]), f"query {i}: users differ"
+        assert np.array_equal(res.ranks[i, :k], rb[i, :k]), f"query {i}: ranks differ"
+        assert np.array_equal(res.qips[i, :k], qb[i, :k]), f"query {i}: <u,q> differ"
+
+
+def test_query_exact_fake30d(orc, fake30d, gpu_index_factory):
+    users, items, queries = fake30d
+    _case("query exact fake-normal-30d k=10")
+    idx, ranks = gpu_index_factory(users, items, 128)
+    res = idx.query(queries, 10)
+    _check_exact(orc, _normalized(orc, users), items, queries, res, [10] * len(queries))
+
+
+def test_query_reference_output_fake30d(orc, fake30d, gpu_index_factory):
+    """RKR_OUTPUT_REFERENCE reproduces US::Index::Retrieval's answers."""
+    users, items, queries = fake30d
+    _case("query reference-output fake-normal-30d k=10")
+    idx, ranks = gpu_index_factory(users, items, 128)
+    idx.set_output("reference")
+    res = idx.query(queries, 10)
+    un = _normalized(orc, users)
+    T = orc.build_table(un, items, ranks)
+    ou, orank, oq = orc.us_retrieval(un, items, T, ranks, queries, 10)
+    assert np.array_equal(res.users, ou)
+    assert np.array_equal(res.ranks, orank)
+    assert np.array_equal(res.qips, oq)
+
+
+@pytest.mark.parametrize("nq", [1, 3, 16, 17, 100])
+def test_query_batch_shapes_agree(orc, fake30d, gpu_index_factory, nq):
+    """The GEMV (<= 16 queries) and GEMM paths give identical answers."""
+    users, items, queries = fake30d
+    _case(f"query batch nq={nq}")
+    idx, _ = gpu_index_factory(users, items, 128)
+    res = idx.query(queries[:nq], 25)
+    _check_exact(orc, _normalized(orc, users), items, queries[:nq], res, [25] * nq)
+
+
+def test_query_mixed_k_and_query_batches(orc, gpu_index_factory):
+    _case("query mixed k, device batches of 7")
+    rng = np.random.default_rng(11)
+    U, I, d = 3000, 2500, 24
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = rng.standard_normal((I, d), dtype=np.float32) * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32)
+    queries = items[rng.choice(I, 30, replace=False)]
+    ks = rng.integers(1, 300, 30).astype(np.int32)
+    idx, _ = gpu_index_factory(users, items, 64)
+    idx.set_option(3, 7)  # RKR_OPT_QUERY_BATCH
+    res = idx.query(queries, ks, max_k=300)
+    _check_exact(orc, _normalized(orc, users), items, queries, res, ks)
+    for i, k in enumerate(ks):
+        assert np.all(res.users[i, k:] == -1) and np.all(res.ranks[i, k:] == -1)
+
+
+def test_query_many_candidates(orc, gpu_index_factory):
+    """Coarse table (S=4) so thousands of users survive pruning: exercises the
+    radix-select top-k path (> 4096 candidates) and large refine batches."""
+    _case("query many candidates")
+    rng = np.random.default_rng(12)
+    U, I, d = 20000, 600, 16
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = rng.standard_normal((I, d), dtype=np.float32)
+    queries = items[:4]
+    idx, _ = gpu_index_factory(users, items, 4)
+    res = idx.query(queries, 200)
+    assert max(s["n_candidate"] for s in res.stats) > 4096
+    _check_exact(orc, _normalized(orc, users), items, queries, res, [200] * 4)
+
+
+def test_query_ties(orc, gpu_index_factory):
+    """Queries equal to items, duplicate items and zero users: the strict
+    count and (rank, userID) order decide every tie."""
+    _case("query ties")
+    rng = np.random.default_rng(13)
+    d, I, U = 10, 800, 600
+    base = rng.standard_normal((60, d), dtype=np.float32)
+    items = base[rng.integers(0, 60, I)]
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    users[:5] = 0.0
+    queries = np.concatenate([items[:6], np.zeros((1, d), np.float32)])
+    idx, _ = gpu_index_factory(users, items, 50)
+    res = idx.query(queries, 40)
+    _check_exact(orc, _normalized(orc, users), items, queries, res, [40] * len(queries))
+
+
+def test_query_k_equals_n_user(orc, gpu_index_factory):
+    _case("query k = n_user")
+    rng = np.random.default_rng(14)
+    users = rng.standard_normal((150, 8), dtype=np.float32)
+    items = rng.standard_normal((400, 8), dtype=np.float32)
+    idx, _ = gpu_index_factory(users, items, 20)
+    res = idx.query(items[:3], 150)
+    _check_exact(orc, _normalized(orc, users), items, items[:3], res, [150] * 3)
+
+
+def test_loaded_table_with_eps(orc, tmp_path, gpu_index_factory):
+    """A table perturbed within eps and loaded with that eps still gives exact answers."""
+    _case("query perturbed table with eps")
+    rng = np.random.default_rng(15)
+    U, I, d = 2000, 3000, 20
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = rng.standard_normal((I, d), dtype=np.float32)
+    idx, ranks = gpu_index_factory(users, items, 64)
+    T = idx.get_table()
+    eps = 1e-3
+    Tp = np.sort(T + rng.uniform(-eps / 2, eps / 2, T.shape).astype(np.float32), axis=1)[:, ::-1].copy()
+    from rkr import formats
+    p = str(tmp_path / "t.index")
+    formats.write_table(p, ranks, Tp, I)
+    idx.load_table(p, eps=eps)
+    queries = items[:20]
+    res = idx.query(queries, 30)
+    _check_exact(orc, _normalized(orc, users), items, queries, res, [30] * 20)
+
+
+def test_save_load_roundtrip(tmp_path, gpu_index_factory):
+    _case("table save/load")
+    rng = np.random.default_rng(16)
+    users = rng.standard_normal((500, 12), dtype=np.float32)
+    items = rng.standard_normal((900, 12), dtype=np.float32)
+    idx, ranks = gpu_index_factory(users, items, 32)
+    T = idx.get_table()
+    p = str(tmp_path / "US-x-n_sample_32.index")
+    idx.save_table(p)
+    from rkr import formats
+    r2, T2, n_item = formats.read_table(p)
+    assert n_item == 900 and np.array_equal(r2, ranks) and np.array_equal(T2, T)
+    idx.load_table(p)
+    assert np.array_equal(idx.get_table(), T)
