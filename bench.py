@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark of the reverse k-ranks US path on one MI355X.
+
+Workload (default): 100k users x 20k items x d=150 (users N(0,1)/sqrt(d),
+items N(0,1)/sqrt(d) x lognormal(0,0.5) norms, seeded), uniform sample
+table of 512 ranks built on the device, 1,000 queries drawn from the items,
+k=50, exact ranks.  One step = one rkr_query call over the 1,000-query
+batch with every input resident in HBM (only the query vectors and the
+answers cross PCIe).
+
+Prints one JSON line: queries/s (value), single-query p50/p99 latency, table
+build time, the roofline of the dominant kernel (HIP-event time of that
+kernel inside the timed region) and the CPU baseline (the oracle's
+restatement of the reference's OpenMP US query path, timed on a bounded
+sample of the same queries against the same table).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "reverse-k-ranks_amd"))
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+
+WORKLOADS = {
+    # name: (synth config, k, label)
+    "plain_gpu": ("plain_gpu", 50, "plain_gpu: 100k users x 20k items x d=150, S=512, 1000 queries, k=50"),
+    "precision": ("precision", 50, "precision: 1M users x 50k items x d=150, S=512, 1000 queries, k=50"),
+    "batching": ("batching", None, "batching: 1M users x 50k items x d=150, S=512, 10000 Zipf(1.1) queries, k in [10,200]"),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="plain_gpu", choices=sorted(WORKLOADS))
+    ap.add_argument("--form", default="plain", choices=["plain", "screened"])
+    ap.add_argument("--n-query", type=int, default=None)
+    ap.add_argument("--latency-queries", type=int, default=200)
+    ap.add_argument("--cpu-sample", type=int, default=None, help="queries in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_sample_q):
+    """The oracle's US query path (reference algorithm, OpenMP over users as the
+    reference runs it) on the first n_sample_q queries, same table."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.build()
+    cores = min(os.cpu_count() or 1, 16)
+    oracle.set_num_threads(cores)
+    q = wl.queries[:n_sample_q]
+    t0 = time.perf_counter()
+    ou, orank, oq = oracle.us_retrieval(users_norm, wl.items, table, ranks, q, k)
+    dt = time.perf_counter() - t0
+    agree = float(np.mean([set(ou[i]) == set(gpu_users[i, :k]) for i in range(len(q))]))
+    return {"value": round(len(q) / dt, 4), "unit": "queries/s", "cores": cores, "kind": "port",
+            "sample": f"first {len(q)} of the same queries, same table (US path, OpenMP, -O2 no fast-math)",
+            "seconds": round(dt, 3), "user_set_agreement": agree}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
+                                   "design note only)"}))
+        sys.exit(2)
+    import rkr
+    from rkr import synth
+
+    cfg, k_fixed, label = WORKLOADS[args.workload]
+    wl = synth.make(cfg, seed=args.seed, n_query=args.n_query)
+    ks = wl.topk if k_fixed is None else np.full(len(wl.queries), k_fixed, np.int32)
+    U, I, d = wl.users.shape[0], wl.items.shape[0], wl.users.shape[1]
+    nq = len(wl.queries)
+
+    idx = rkr.Index(0)
+    idx.set_form(args.form)
+    idx.set_data(wl.users, wl.items, normalize_users=True)
+    ranks = rkr.Index.uniform_sample_ranks(I, wl.n_sample)
+    t0 = time.perf_counter()
+    build_ms = idx.build_table(ranks)
+    build_wall = time.perf_counter() - t0
+
+    max_k = int(ks.max())
+    for _ in range(args.warmup):
+        idx.query(wl.queries, ks, max_k=max_k)
+    idx.lib.rkr_sync(idx.ctx)
+    timings = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = idx.query(wl.queries, ks, max_k=max_k)
+        timings.append(res.timing)
+    idx.lib.rkr_sync(idx.ctx)
+    elapsed = time.perf_counter() - t0
+    qps = args.steps * nq / elapsed
+
+    # single-query latency (batch of one, host wall time per call)
+    lat = []
+    for i in range(min(args.latency_queries, nq)):
+        s = time.perf_counter()
+        idx.query(wl.queries[i:i + 1], ks[i:i + 1], max_k=max_k)
+        lat.append((time.perf_counter() - s) * 1e3)
+    lat = np.array(lat)
+
+    # stage times (device, HIP events on the library's stream)
+    tot = {key: sum(t[key] for t in timings) for key in timings[0]}
+    launches = max(1, int(tot["n_batches"]))
+    stages = {"ip": tot["ip_ms"], "bound": tot["bound_ms"], "prune": tot["prune_ms"],
+              "refine": tot["refine_ms"], "topk": tot["topk_ms"]}
+    st = res.stats
+    n_ref_per_step = sum(s["n_refine_user"] for s in st)
+    n_cand_per_step = sum(s["n_candidate"] for s in st)
+    # dominant kernel and its algorithmic work
+    dom = max(("refine", "ip", "bound"), key=lambda s: stages[s])
+    if dom == "refine":
+        flops = 2.0 * I * d * tot["n_refined"]
+        ach = flops / (tot["refine_ms"] * 1e-3) / 1e12
+        roof = {"kernel": "k_refine_count", "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "avg_launch_ms": round(tot["refine_ms"] / launches, 4),
+                "work_per_launch": f"2*I*d flops x {tot['n_refined'] / launches:.0f} refined (query,user) pairs"}
+    elif dom == "ip":
+        flops = 2.0 * U * d * nq * args.steps
+        ach = flops / (tot["ip_ms"] * 1e-3) / 1e12
+        roof = {"kernel": "k_scores_store", "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "avg_launch_ms": round(tot["ip_ms"] / launches, 4), "work_per_launch": "2*U*d*B flops"}
+    else:
+        import math
+        byts = (U * math.ceil(math.log2(wl.n_sample + 1)) * 4 + U * 4 * 2) * nq * args.steps
+        ach = byts / (tot["bound_ms"] * 1e-3) / 1e9
+        roof = {"kernel": "k_bounds", "bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(tot["bound_ms"] / launches, 4),
+                "work_per_launch": "U*(ceil(log2(S+1))*4 + 8) bytes per query"}
+
+    cpu = None
+    if not args.no_cpu:
+        T = idx.get_table()
+        users_norm = np.ascontiguousarray(wl.users, np.float32).copy()
+        rkr.load().rkr_normalize_users(users_norm.ctypes.data, U, d)
+        n_s = args.cpu_sample or 20
+        k_cpu = int(ks[0])
+        # CPU sample runs one k (the reference API takes a single topk)
+        ref_users = idx.query(wl.queries[:n_s], np.full(n_s, k_cpu, np.int32)).users
+        cpu = cpu_baseline(wl, T, ranks, users_norm, ref_users, k_cpu, n_s)
+
+    out = {
+        "metric": "queries/s at k=50, exact ranks (plus p50/p99 latency and score-table build time)",
+        "value": round(qps, 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded), inputs resident in HBM",
+        "config": {"workload": label, "form": args.form, "queries_per_step": nq, "n_user": U, "n_item": I,
+                   "dim": d, "n_sample": wl.n_sample},
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "latency_note": "single-query calls (batch of 1), host wall time incl. PCIe copies",
+        "table_build_ms": round(build_ms, 2),
+        "table_build_wall_s": round(build_wall, 3),
+        "stage_ms_per_step": {k_: round(v / args.steps, 3) for k_, v in stages.items()},
+        "candidates_per_query": round(n_cand_per_step / nq, 1),
+        "refined_per_query": round(n_ref_per_step / nq, 1),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    idx.close()
+
+
+if __name__ == "__main__":
+    main()

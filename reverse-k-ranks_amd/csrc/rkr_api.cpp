@@ -537,12 +537,13 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
 
         RKR_HIP(c, launch_compact(c->stream, bk, qip, U, S, c->ranks, I, nq, mode, qi, cand_user, cand_rank,
                                   refine_total, refine_cand, refine_user, refine_thr));
-        RKR_HIP(c, hipEventRecord(c->ev[4], c->stream));
         RKR_HIP(c, hipMemsetAsync(cnt, 0, sizeof(int32_t) * rcap, c->stream));
+        RKR_HIP(c, hipEventRecord(c->ev[4], c->stream));
+        // events 4..5 bracket exactly the refinement kernel
         RKR_HIP(c, launch_refine_count(c->stream, c->users, c->items, I, ld, d, refine_user, refine_thr, refine_total,
                                        (int64_t)rcap, cnt));
-        RKR_HIP(c, launch_scatter_counts(c->stream, refine_total, refine_cand, cnt, (int64_t)rcap, cand_rank));
         RKR_HIP(c, hipEventRecord(c->ev[5], c->stream));
+        RKR_HIP(c, launch_scatter_counts(c->stream, refine_total, refine_cand, cnt, (int64_t)rcap, cand_rank));
         RKR_HIP(c, launch_topk(c->stream, qi, nq, cand_user, cand_rank, qip, U, max_k, o_user, o_rank, o_qip, nullptr));
         RKR_HIP(c, hipEventRecord(c->ev[6], c->stream));
         RKR_HIP(c, hipMemcpyAsync(ho_user.data(), o_user, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));

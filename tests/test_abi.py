@@ -1,0 +1,125 @@
+"""The C ABI without a GPU: librkr.so loads, exports every entry point that
+include/rkr.h declares, and its host-only helpers and argument checks behave
+(no kernel runs here)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "rkr.h")
+
+
+def _declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(rkr_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_lists_match_binding():
+    from rkr import _lib
+    assert _declared() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    from rkr import _lib
+    L = _lib.load()
+    for name in _declared():
+        assert hasattr(L, name), name
+    assert L.rkr_version().startswith(b"rkr")
+
+
+def test_nm_shows_c_linkage():
+    import subprocess
+    from rkr import _lib
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    syms = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    for name in _declared():
+        assert name in syms, name
+
+
+def test_uniform_sample_ranks_host_helper(orc):
+    from rkr import Index
+    for I, S in [(5000, 128), (20000, 512), (50000, 512), (7, 3)]:
+        assert np.array_equal(Index.uniform_sample_ranks(I, S), orc.uniform_sample(I, S))
+    from rkr import RkrError
+    for I, S in [(100, 0), (100, 100), (5, 9)]:
+        with pytest.raises(RkrError):
+            Index.uniform_sample_ranks(I, S)
+
+
+def test_normalize_users_host_helper_matches_oracle(orc, fake30d):
+    from rkr import _lib
+    users = fake30d[0].copy()
+    users[5] = 1e-6
+    a = users.copy()
+    assert _lib.load().rkr_normalize_users(a.ctypes.data, a.shape[0], a.shape[1]) == 0
+    assert np.array_equal(a.view(np.int32), orc.normalize_users(users).view(np.int32))
+
+
+def test_null_and_invalid_arguments():
+    from rkr import _lib
+    L = _lib.load()
+    assert L.rkr_create(0, 0, None) == _lib.RKR_ERR_INVALID
+    assert L.rkr_set_option(None, 1, 0) == _lib.RKR_ERR_INVALID
+    assert L.rkr_sync(None) == _lib.RKR_ERR_INVALID
+    assert L.rkr_last_error(None) == b"null context"
+    assert L.rkr_index_size_bytes(None) == 0
+    assert L.rkr_normalize_users(None, 3, 3) == _lib.RKR_ERR_INVALID
+    assert L.rkr_uniform_sample_ranks(10, 3, None) == _lib.RKR_ERR_INVALID
+    L.rkr_destroy(None)  # no-op
+
+
+def test_create_without_gpu_reports_nodevice():
+    """In a container without a GPU the library reports it instead of exiting."""
+    import torch
+    from rkr import _lib
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    ctx = C.c_void_p()
+    assert _lib.load().rkr_create(0, 0, C.byref(ctx)) == _lib.RKR_ERR_NODEVICE
+    assert not ctx.value
+
+
+def test_formats_roundtrip(tmp_path):
+    from rkr import formats
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((17, 5), dtype=np.float32)
+    p = str(tmp_path / "a.fvecs")
+    formats.write_fvecs(p, x)
+    assert np.array_equal(formats.read_fvecs(p), x)
+    ranks = np.array([0, 3, 9], np.int32)
+    T = -np.sort(-rng.standard_normal((4, 3), dtype=np.float32), axis=1)
+    tp = str(tmp_path / "t.index")
+    formats.write_table(tp, ranks, T, 10)
+    raw = open(tp, "rb").read()
+    assert len(raw) == 3 * 8 + 3 * 4 + 12 * 4  # size_t x3, int32 ranks, float table
+    r2, T2, n_item = formats.read_table(tp)
+    assert n_item == 10 and np.array_equal(r2, ranks) and np.array_equal(T2, T)
+    paths = formats.write_rank_result(str(tmp_path), "ds", "US", "top2", [[1, 2], [3, 4]], [[0, 5], [1, 1]],
+                                      [[0.5, 0.25], [1.0, 2.0]])
+    assert open(paths[0]).read() == "1,2\n3,4\n"
+    assert open(paths[1]).read() == "0,5\n1,1\n"
+
+
+def test_reference_dataset_fixture_layout(fake30d):
+    users, items, queries = fake30d
+    assert users.shape == (1000, 30) and items.shape == (5000, 30) and queries.shape == (100, 30)
+
+
+def test_synth_shapes_and_seeds():
+    from rkr import synth
+    a = synth.make("plain_cpu", seed=3)
+    b = synth.make("plain_cpu", seed=3)
+    assert a.users.shape == (1000, 30) and a.items.shape == (5000, 30) and a.queries.shape == (100, 30)
+    assert np.array_equal(a.users, b.users) and np.array_equal(a.queries, b.queries)
+    assert np.array_equal(a.queries, a.items[a.query_items])
+    z = synth.make("batching", seed=1, n_user=100, n_item=2000, n_query=5000)
+    assert z.topk.min() >= 10 and z.topk.max() <= 200
+    norms = np.linalg.norm(z.items.astype(np.float64), axis=1)
+    # Zipf over norm-sorted items: the largest-norm item is the most frequent query
+    top = np.bincount(z.query_items, minlength=2000).argmax()
+    assert top == int(np.argmax(norms))
