@@ -76,7 +76,8 @@ typedef struct rkr_batch_timing {
     double ip_ms;           /* <u,q> GEMV/GEMM                                 */
     double bound_ms;        /* table binary search                             */
     double prune_ms;        /* k-th bound, candidate compaction                */
-    double refine_ms;       /* exact-rank refinement                           */
+    double refine_ms;       /* exact-rank refinement (screening kernel when screened) */
+    double recheck_ms;      /* screened form: exact recheck of undecided items */
     double topk_ms;         /* top-k selection                                 */
     double h2d_ms, d2h_ms;  /* copies of queries and results                   */
     int64_t n_batches;

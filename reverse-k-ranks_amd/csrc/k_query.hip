@@ -70,6 +70,42 @@ __global__ void __launch_bounds__(256) k_bounds(const float *__restrict__ qip, c
     bk[b * n_user + u] = lo | (hi << 16);
 }
 
+// Batched form: a workgroup stages UT consecutive table rows in LDS and runs
+// all nq searches of those users from there (each row is read from HBM once
+// per batch instead of once per query).  Thread t serves user t % UT.
+__device__ __forceinline__ void bucket_pair(const float *row, int32_t S, float q, float eps, uint32_t &lo,
+                                            uint32_t &hi) {
+    if (eps > 0.0f) {
+        lo = (uint32_t)bucket_ge(row, S, __double2float_rd((double)q - (double)eps));
+        hi = (uint32_t)bucket_gt(row, S, __double2float_ru((double)q + (double)eps));
+    } else {
+        lo = (uint32_t)bucket_ge(row, S, q);
+        hi = lo;
+        while (hi > 0 && row[hi - 1] == q) --hi;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bounds_lds(const float *__restrict__ qip, const float *__restrict__ table,
+                                                    int64_t n_user, int32_t S, int32_t nq, int32_t ut, float eps,
+                                                    uint32_t *__restrict__ bk) {
+    extern __shared__ float rows[];  // ut x S
+    const int64_t u0 = (int64_t)blockIdx.x * ut;
+    const int64_t nu = min((int64_t)ut, n_user - u0);
+    const float *src = table + u0 * S;
+    for (int64_t i = threadIdx.x; i < nu * S; i += 256) rows[i] = src[i];
+    __syncthreads();
+    const int ul = threadIdx.x % ut;
+    const int lanes_q = 256 / ut;
+    if (ul >= nu) return;
+    const int64_t u = u0 + ul;
+    const float *row = rows + (int64_t)ul * S;
+    for (int32_t b = threadIdx.x / ut; b < nq; b += lanes_q) {
+        uint32_t lo, hi;
+        bucket_pair(row, S, qip[(int64_t)b * n_user + u], eps, lo, hi);
+        bk[(int64_t)b * n_user + u] = lo | (hi << 16);
+    }
+}
+
 // Reference bounds of a bucket (SampleSearch.hpp:107-125).
 __device__ __forceinline__ int32_t ref_lb(int32_t b, int32_t S, const int32_t *r, int64_t I) {
     return b == S ? (int32_t)I : r[b];
@@ -191,7 +227,8 @@ __global__ void __launch_bounds__(NT) k_compact(const uint32_t *__restrict__ bk,
                                                 int64_t n_item, QueryInfo *__restrict__ qi,
                                                 int32_t *__restrict__ cand_user, int32_t *__restrict__ cand_rank,
                                                 int32_t *__restrict__ refine_total, int32_t *__restrict__ refine_cand,
-                                                int32_t *__restrict__ refine_user, float *__restrict__ refine_thr) {
+                                                int32_t *__restrict__ refine_user, int32_t *__restrict__ refine_query,
+                                                float *__restrict__ refine_thr) {
     __shared__ uint32_t red[NW + 1];
     __shared__ int32_t n_refine;
     const int64_t b = blockIdx.x;
@@ -247,6 +284,7 @@ __global__ void __launch_bounds__(NT) k_compact(const uint32_t *__restrict__ bk,
                 const int32_t r = atomicAdd(refine_total, 1);
                 refine_cand[r] = (int32_t)slot;
                 refine_user[r] = (int32_t)u;
+                refine_query[r] = (int32_t)b;
                 refine_thr[r] = qb[u];
             }
         }
@@ -380,6 +418,14 @@ __global__ void __launch_bounds__(NT) k_topk(const QueryInfo *__restrict__ qi, c
 hipError_t launch_bounds(hipStream_t s, const float *qip, const float *table, int64_t n_user, int32_t S, int32_t nq,
                          float eps, uint32_t *bk) {
     if (n_user <= 0 || nq <= 0) return hipSuccess;
+    if (nq >= 8) {
+        int ut = 64;
+        while (ut > 1 && (int64_t)ut * S * 4 > 65536) ut >>= 1;
+        const unsigned grid = (unsigned)((n_user + ut - 1) / ut);
+        hipLaunchKernelGGL(k_bounds_lds, dim3(grid), dim3(256), (size_t)ut * S * 4, s, qip, table, n_user, S, nq, ut,
+                           eps, bk);
+        return hipGetLastError();
+    }
     dim3 grid((unsigned)((n_user + 255) / 256), (unsigned)nq);
     hipLaunchKernelGGL(k_bounds, grid, dim3(256), 0, s, qip, table, n_user, S, eps, bk);
     return hipGetLastError();
@@ -400,11 +446,11 @@ hipError_t launch_offsets(hipStream_t s, QueryInfo *qi, int32_t nq, int32_t *ref
 hipError_t launch_compact(hipStream_t s, const uint32_t *bk, const float *qip, int64_t n_user, int32_t S,
                           const int32_t *ranks, int64_t n_item, int32_t nq, int output_mode, QueryInfo *qi,
                           int32_t *cand_user, int32_t *cand_rank, int32_t *refine_total, int32_t *refine_cand,
-                          int32_t *refine_user, float *refine_thr) {
+                          int32_t *refine_user, int32_t *refine_query, float *refine_thr) {
     (void)output_mode;
     if (nq <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_compact, dim3((unsigned)nq), dim3(NT), 0, s, bk, qip, n_user, S, ranks, n_item, qi, cand_user,
-                       cand_rank, refine_total, refine_cand, refine_user, refine_thr);
+                       cand_rank, refine_total, refine_cand, refine_user, refine_query, refine_thr);
     return hipGetLastError();
 }
 

@@ -1,0 +1,340 @@
+// k_screen.hip -- screened exact-rank refinement (RKR_FORM_SCREENED).
+//
+// For each refine row r (a (query, user) pair with threshold thr_r = <u,q>)
+// the rank is #{items p : s(u,p) > thr_r}, with s the sequential FP32 score
+// rule.  The screened form computes an approximation s^ of every s(u,p) on
+// the matrix cores and decides most items from it:
+//
+//   x = x_hi + x_lo + x_res with x_hi = bf16(x), x_lo = bf16(x - x_hi)
+//   s^ = sum_k u_hi p_hi + u_hi p_lo + u_lo p_hi     (v_mfma_f32_16x16x32_bf16,
+//                                                      FP32 accumulation)
+//   |s^ - s| <= e(u,p) = C * ||u|| * ||p|| + abs_floor
+//
+// C covers the dropped terms (u_lo p_lo and the residues, <= 3.2*2^-16 of
+// sum|u_k p_k|), the FP32 accumulation of the 3*Kp exact products (any
+// order, <= 2^-23 per addition), the rounding of the sequential rule itself
+// (gamma_d), and Cauchy-Schwarz sum|u_k p_k| <= ||u|| ||p||; abs_floor covers
+// denormals a matrix core may flush.  Then
+//   s^ - e > thr  => counted (s > thr);   s^ + e <= thr => not counted;
+// anything in between is appended to a recheck list and decided by the exact
+// sequential FP32 score (k_recheck), so every rank stays bit-exact.
+//
+// Work unit = 128 refine rows (4 waves x 2 blocks of 16 rows; A fragments
+// held in VGPRs) x a chunk of items streamed through LDS 64 items at a time.
+#include "rkr_internal.hpp"
+
+namespace rkr {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int NT = 256;
+constexpr int ROWS_WG = 128;
+constexpr int ITEMS_STAGE = 64;
+constexpr int KPAD_LDS = 8;  // bf16 pad per LDS row
+
+// Round-to-nearest-even FP32 -> BF16 (finite inputs).
+__device__ __forceinline__ uint16_t bf16_bits(float x) {
+    const uint32_t b = __float_as_uint(x);
+    return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+union Frag {
+    bf16x8 v;
+    uint16_t h[8];
+    uint4 q;
+};
+
+// Splits every item row into hi/lo BF16 rows (Kp columns, zero padded) and
+// stores an upper bound of each item's L2 norm (FP64 sum, rounded up).
+__global__ void k_prep_items(const float *__restrict__ items, int64_t n_item, int32_t ld, int32_t K, int32_t Kp,
+                             uint16_t *__restrict__ hi, uint16_t *__restrict__ lo, float *__restrict__ norm) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.y + threadIdx.y;
+    if (p >= n_item) return;
+    const float *x = items + p * ld;
+    double ss = 0.0;
+    for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
+        const float v = k < K ? x[k] : 0.0f;
+        const uint16_t h = bf16_bits(v);
+        const uint16_t l = bf16_bits(v - bf16_to_f32(h));
+        hi[p * Kp + k] = h;
+        lo[p * Kp + k] = l;
+        ss += (double)v * (double)v;
+    }
+    for (int o = 16; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 32);
+    if (threadIdx.x == 0) norm[p] = __double2float_ru(sqrt(ss));
+}
+
+// Upper bound of each user's L2 norm.
+__global__ void k_user_norms(const float *__restrict__ users, int64_t n_user, int32_t ld, int32_t K,
+                             float *__restrict__ norm) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n_user) return;
+    double ss = 0.0;
+    for (int k = 0; k < K; ++k) {
+        const double v = users[u * ld + k];
+        ss += v * v;
+    }
+    norm[u] = __double2float_ru(sqrt(ss));
+}
+
+struct ScreenParams {
+    const float *users;        // n_user x ld FP32
+    const float *user_norm;    // n_user
+    const uint16_t *item_hi;   // n_item x Kp
+    const uint16_t *item_lo;
+    const float *item_norm;    // n_item
+    int64_t n_item;
+    int32_t ld, K, Kp;
+    const int32_t *rows_user;  // refine rows
+    const float *rows_thr;
+    const int32_t *n_rows_dev;
+    int64_t max_rows;
+    int32_t n_chunks;          // item chunks per row block
+    float C;                   // relative error factor
+    float abs_floor;           // absolute error floor
+    int32_t *cnt;              // per refine row
+    int32_t *recheck_n;        // appended rechecks (may exceed capacity)
+    int4 *recheck;             // (row, item, s^ bits, 0)
+    int64_t recheck_cap;
+};
+
+template <int KS>
+__global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
+    __shared__ uint4 sB[2][ITEMS_STAGE * (KS * 32 + KPAD_LDS) / 8];  // [hi|lo][item][k] as 16-byte chunks
+    constexpr int ROWQ = (KS * 32 + KPAD_LDS) / 8;                   // uint4 per LDS item row
+    const int64_t n_rows = min((int64_t)*P.n_rows_dev, P.max_rows);
+    const int64_t n_rb = (n_rows + ROWS_WG - 1) / ROWS_WG;
+    const int64_t unit = blockIdx.x;
+    const int64_t rb = unit / P.n_chunks;
+    const int chunk = (int)(unit % P.n_chunks);
+    if (rb >= n_rb) return;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int64_t row_base = rb * ROWS_WG + wave * 32;
+
+    // A fragments: row (row_base + b*16 + lr), k = ks*32 + 8*lq + i
+    Frag ah[2][KS], al[2][KS];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const int64_t r = row_base + b * 16 + lr;
+        const float *urow = r < n_rows ? P.users + (int64_t)P.rows_user[r] * P.ld : nullptr;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = ks * 32 + 8 * lq + i;
+                const float v = (urow && k < P.K) ? urow[k] : 0.0f;
+                const uint16_t h = bf16_bits(v);
+                ah[b][ks].h[i] = h;
+                al[b][ks].h[i] = bf16_bits(v - bf16_to_f32(h));
+            }
+        }
+    }
+    // epilogue constants of the 4 rows this lane's accumulator covers:
+    // D[4*lq + i][lr] of each 16x16 block
+    float thr_hi[2][4], thr_lo[2][4], Ar[2][4];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t r = row_base + b * 16 + 4 * lq + i;
+            if (r < n_rows) {
+                const float t = P.rows_thr[r];
+                thr_hi[b][i] = t - P.abs_floor;  // s^ + e > thr - floor  <=>  may count
+                thr_lo[b][i] = t + P.abs_floor;  // s^ - e > thr + floor  =>  counts
+                Ar[b][i] = P.C * P.user_norm[P.rows_user[r]];
+            } else {
+                thr_hi[b][i] = INFINITY;
+                thr_lo[b][i] = INFINITY;
+                Ar[b][i] = 0.0f;
+            }
+        }
+    int32_t cnt[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+
+    const int64_t n_stage_total = (P.n_item + ITEMS_STAGE - 1) / ITEMS_STAGE;
+    const int64_t st_per_chunk = (n_stage_total + P.n_chunks - 1) / P.n_chunks;
+    const int64_t st0 = chunk * st_per_chunk;
+    const int64_t st1 = min(n_stage_total, st0 + st_per_chunk);
+    const int kq_row = P.Kp / 8;  // uint4 per global item row
+
+    for (int64_t st = st0; st < st1; ++st) {
+        const int64_t p0 = st * ITEMS_STAGE;
+        // stage 64 items (hi and lo) into LDS
+        for (int c = threadIdx.x; c < 2 * ITEMS_STAGE * kq_row; c += NT) {
+            const int half = c / (ITEMS_STAGE * kq_row);
+            const int rem = c % (ITEMS_STAGE * kq_row);
+            const int it = rem / kq_row, q = rem % kq_row;
+            const int64_t p = p0 + it;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (p < P.n_item)
+                v = reinterpret_cast<const uint4 *>(half ? P.item_lo : P.item_hi)[p * kq_row + q];
+            sB[half][it * ROWQ + q] = v;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int g = 0; g < ITEMS_STAGE / 16; ++g) {
+            const int it = g * 16 + lr;
+            const int64_t p = p0 + it;
+            Frag bh[KS], bl[KS];
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                bh[ks].q = sB[0][it * ROWQ + ks * 4 + lq];
+                bl[ks].q = sB[1][it * ROWQ + ks * 4 + lq];
+            }
+            const float np = p < P.n_item ? P.item_norm[p] : -INFINITY;
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                f32x4 acc_hh = {0.f, 0.f, 0.f, 0.f}, acc_hl = acc_hh, acc_lh = acc_hh;
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) {
+                    acc_hh = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bh[ks].v, acc_hh, 0, 0, 0);
+                    acc_hl = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bl[ks].v, acc_hl, 0, 0, 0);
+                    acc_lh = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[b][ks].v, bh[ks].v, acc_lh, 0, 0, 0);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float s = (acc_hh[i] + acc_hl[i]) + acc_lh[i];
+                    const float hi = __builtin_fmaf(Ar[b][i], np, s);
+                    if (hi > thr_hi[b][i]) {
+                        const float lo = __builtin_fmaf(-Ar[b][i], np, s);
+                        if (lo > thr_lo[b][i]) {
+                            ++cnt[b][i];
+                        } else {
+                            const int64_t r = row_base + b * 16 + 4 * lq + i;
+                            const int32_t slot = atomicAdd(P.recheck_n, 1);
+                            if (slot < P.recheck_cap)
+                                P.recheck[slot] = make_int4((int32_t)r, (int32_t)p, __float_as_int(s), 0);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // reduce each row's count over the 16 lanes (columns) that share it
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int32_t v = cnt[b][i];
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+            const int64_t r = row_base + b * 16 + 4 * lq + i;
+            if (lr == 0 && r < n_rows && v) atomicAdd(&P.cnt[r], v);
+        }
+}
+
+// Exact decision of the rechecked (row, item) pairs with the sequential FP32
+// rule; also records the largest |s^ - s_fp64| seen (as float bits).
+__global__ void k_recheck(const float *__restrict__ users, const float *__restrict__ items, int32_t ld, int32_t K,
+                          const int32_t *__restrict__ rows_user, const float *__restrict__ rows_thr,
+                          const int32_t *__restrict__ rows_query, const int32_t *__restrict__ recheck_n,
+                          const int4 *__restrict__ recheck, int64_t cap, int32_t *__restrict__ cnt,
+                          int32_t *__restrict__ q_recheck, uint32_t *__restrict__ max_err_bits) {
+    const int64_t n = min((int64_t)*recheck_n, cap);
+    float worst = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int4 rc = recheck[i];
+        const float *u = users + (int64_t)rows_user[rc.x] * ld;
+        const float *p = items + (int64_t)rc.y * ld;
+        float s = 0.0f;
+        double s64 = 0.0;
+        for (int k = 0; k < K; k += 4) {
+            const float4 a = *reinterpret_cast<const float4 *>(u + k);
+            const float4 b = *reinterpret_cast<const float4 *>(p + k);
+            const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (k + j < K) {
+                    const float prod = av[j] * bv[j];
+                    s = s + prod;
+                    s64 += (double)av[j] * (double)bv[j];
+                }
+            }
+        }
+        if (s > rows_thr[rc.x]) atomicAdd(&cnt[rc.x], 1);
+        atomicAdd(&q_recheck[rows_query[rc.x]], 1);
+        const float err = (float)fabs((double)__int_as_float(rc.z) - s64);
+        worst = fmaxf(worst, err);
+    }
+    if (worst > 0.0f) atomicMax(max_err_bits, __float_as_uint(worst));
+}
+
+}  // namespace
+
+hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, int32_t ld, int32_t K, int32_t Kp,
+                             uint16_t *hi, uint16_t *lo, float *norm) {
+    if (n_item <= 0) return hipSuccess;
+    dim3 block(32, 8);
+    hipLaunchKernelGGL(k_prep_items, dim3((unsigned)((n_item + 7) / 8)), block, 0, s, items, n_item, ld, K, Kp, hi, lo,
+                       norm);
+    return hipGetLastError();
+}
+
+hipError_t launch_user_norms(hipStream_t s, const float *users, int64_t n_user, int32_t ld, int32_t K, float *norm) {
+    if (n_user <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_user_norms, dim3((unsigned)((n_user + 255) / 256)), dim3(256), 0, s, users, n_user, ld, K,
+                       norm);
+    return hipGetLastError();
+}
+
+int screen_max_kp() { return 256; }
+
+hipError_t launch_refine_screen(hipStream_t s, const ScreenLaunch &L) {
+    if (L.max_rows <= 0 || L.n_item <= 0) return hipSuccess;
+    ScreenParams P;
+    P.users = L.users;
+    P.user_norm = L.user_norm;
+    P.item_hi = L.item_hi;
+    P.item_lo = L.item_lo;
+    P.item_norm = L.item_norm;
+    P.n_item = L.n_item;
+    P.ld = L.ld;
+    P.K = L.K;
+    P.Kp = L.Kp;
+    P.rows_user = L.rows_user;
+    P.rows_thr = L.rows_thr;
+    P.n_rows_dev = L.n_rows_dev;
+    P.max_rows = L.max_rows;
+    P.C = L.C;
+    P.abs_floor = L.abs_floor;
+    P.cnt = L.cnt;
+    P.recheck_n = L.recheck_n;
+    P.recheck = reinterpret_cast<int4 *>(L.recheck);
+    P.recheck_cap = L.recheck_cap;
+    const int64_t n_rb = (L.max_rows + ROWS_WG - 1) / ROWS_WG;
+    const int64_t n_stage = (L.n_item + ITEMS_STAGE - 1) / ITEMS_STAGE;
+    // enough work units to fill the chip even for a handful of rows
+    int64_t chunks = (L.target_units + n_rb - 1) / n_rb;
+    chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, n_stage));
+    P.n_chunks = (int32_t)chunks;
+    const dim3 grid((unsigned)(n_rb * chunks));
+    switch (L.Kp / 32) {
+        case 1: hipLaunchKernelGGL(k_refine_screen<1>, grid, dim3(NT), 0, s, P); break;
+        case 2: hipLaunchKernelGGL(k_refine_screen<2>, grid, dim3(NT), 0, s, P); break;
+        case 3: hipLaunchKernelGGL(k_refine_screen<3>, grid, dim3(NT), 0, s, P); break;
+        case 4: hipLaunchKernelGGL(k_refine_screen<4>, grid, dim3(NT), 0, s, P); break;
+        case 5: hipLaunchKernelGGL(k_refine_screen<5>, grid, dim3(NT), 0, s, P); break;
+        case 6: hipLaunchKernelGGL(k_refine_screen<6>, grid, dim3(NT), 0, s, P); break;
+        case 7: hipLaunchKernelGGL(k_refine_screen<7>, grid, dim3(NT), 0, s, P); break;
+        case 8: hipLaunchKernelGGL(k_refine_screen<8>, grid, dim3(NT), 0, s, P); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_recheck(hipStream_t s, const float *users, const float *items, int32_t ld, int32_t K,
+                          const int32_t *rows_user, const float *rows_thr, const int32_t *rows_query,
+                          const int32_t *recheck_n, const void *recheck, int64_t cap, int32_t *cnt,
+                          int32_t *q_recheck, uint32_t *max_err_bits) {
+    if (cap <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_recheck, dim3(2048), dim3(256), 0, s, users, items, ld, K, rows_user, rows_thr, rows_query,
+                       recheck_n, reinterpret_cast<const int4 *>(recheck), cap, cnt, q_recheck, max_err_bits);
+    return hipGetLastError();
+}
+
+}  // namespace rkr

@@ -85,6 +85,13 @@ void free_data(Ctx *c) {
     dev_free(c, c->users, sizeof(float) * (size_t)c->n_user * ld);
     dev_free(c, c->usersT, sizeof(float) * (size_t)((c->n_user + 63) / 64) * 64 * ld);
     dev_free(c, c->items, sizeof(float) * (size_t)c->n_item * ld);
+    dev_free(c, c->item_hi, sizeof(uint16_t) * (size_t)c->n_item * c->kp);
+    dev_free(c, c->item_lo, sizeof(uint16_t) * (size_t)c->n_item * c->kp);
+    dev_free(c, c->item_norm, sizeof(float) * (size_t)c->n_item);
+    dev_free(c, c->user_norm, sizeof(float) * (size_t)c->n_user);
+    c->item_hi = c->item_lo = nullptr;
+    c->item_norm = c->user_norm = nullptr;
+    c->kp = 0;
     c->users = c->usersT = c->items = nullptr;
     c->n_user = c->n_item = 0;
     c->dim = c->ld = 0;
@@ -93,7 +100,7 @@ void free_data(Ctx *c) {
 
 void free_scratch(Ctx *c) {
     for (DeviceBuf *b : {&c->q_in, &c->qip, &c->bk, &c->qinfo, &c->cand, &c->refine_idx, &c->counts, &c->out,
-                         &c->misc}) {
+                         &c->misc, &c->recheck, &c->small}) {
         dev_free(c, b->p, b->bytes);
         b->p = nullptr;
         b->bytes = 0;
@@ -173,6 +180,9 @@ int rkr_create(int device, size_t mem_budget_bytes, rkr_ctx **out) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { delete c; return RKR_ERR_HIP; }
     c->budget = mem_budget_bytes ? std::min(mem_budget_bytes, free_b) : (size_t)(0.9 * (double)free_b);
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && n_cu > 0)
+        c->n_cu = n_cu;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return RKR_ERR_HIP; }
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) { rkr_destroy(c); return RKR_ERR_HIP; }
@@ -264,6 +274,8 @@ int rkr_set_data(rkr_ctx *c, const float *users, int64_t n_user, const float *it
         if (mu < 0 || mp < 0) return set_err(c, RKR_ERR_INVALID, "set_data: users and items must be finite");
         const double mu_n = normalize_users ? 1.0 : mu;  // normalised rows have |u_i| <= 1
         c->max_abs_user = mu_n;
+        c->max_abs_item = mp;
+        c->max_abs_user_raw = mu;
         if ((double)dim * mu_n * mp > kScoreLimit)
             return set_err(c, RKR_ERR_INVALID, "set_data: magnitudes would overflow FP32 scores");
     }
@@ -300,6 +312,32 @@ int rkr_set_data(rkr_ctx *c, const float *users, int64_t n_user, const float *it
     rc = dev_alloc_t(c, &c->usersT, (size_t)((n_user + 63) / 64) * 64 * ld, "user panels");
     if (rc) { free_data(c); return rc; }
     RKR_HIP(c, launch_make_panels(c->stream, c->users, n_user, ld, c->usersT));
+    // screened form: hi/lo BF16 item rows and norm bounds (d <= 256)
+    const int32_t kp = (dim + 31) / 32 * 32;
+    if (kp <= screen_max_kp()) {
+        c->kp = kp;
+        // |s^ - s| <= C ||u|| ||p|| + abs: dropped split terms (3.2 * 2^-16),
+        // FP32 accumulation of 3*kp exact products at <= 2^-23 each (x1.03 for
+        // the split magnitudes), the sequential rule's own rounding
+        // (gamma_dim at 2^-24), times a 1% margin that also absorbs the FP32
+        // evaluation of the bound itself.
+        const double n3 = 3.0 * kp, e23 = std::ldexp(1.0, -23), e24 = std::ldexp(1.0, -24);
+        const double g_mfma = n3 * e23 / (1.0 - n3 * e23), g_rule = dim * e24 / (1.0 - dim * e24);
+        const double C = (3.2 * std::ldexp(1.0, -16) + 1.03 * g_mfma + g_rule) * 1.01;
+        c->screen_C = std::nextafter((float)C, INFINITY);
+        // flushed denormal products: each < 2^-126 * max|other factor|
+        const double absf = 2.0 * n3 * std::ldexp(1.0, -126) * (1.0 + c->max_abs_user) * (1.0 + c->max_abs_item);
+        c->screen_abs = std::nextafter((float)absf, INFINITY);
+        if ((rc = dev_alloc_t(c, &c->item_hi, (size_t)n_item * kp, "item hi")) ||
+            (rc = dev_alloc_t(c, &c->item_lo, (size_t)n_item * kp, "item lo")) ||
+            (rc = dev_alloc_t(c, &c->item_norm, (size_t)n_item, "item norms")) ||
+            (rc = dev_alloc_t(c, &c->user_norm, (size_t)n_user, "user norms"))) {
+            free_data(c);
+            return rc;
+        }
+        RKR_HIP(c, launch_prep_items(c->stream, c->items, n_item, ld, dim, kp, c->item_hi, c->item_lo, c->item_norm));
+        RKR_HIP(c, launch_user_norms(c->stream, c->users, n_user, ld, dim, c->user_norm));
+    }
     RKR_HIP(c, hipStreamSynchronize(c->stream));
     return RKR_OK;
 }
@@ -526,34 +564,82 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
         const size_t cap = (size_t)std::max<int64_t>(total, 1);
         const size_t rcap = (size_t)std::max<int64_t>(total_cand, 1);
         if ((rc = ensure(c, c->cand, sizeof(int32_t) * 2 * cap, "candidates"))) return rc;
-        if ((rc = ensure(c, c->refine_idx, (sizeof(int32_t) * 2 + sizeof(float)) * rcap, "refine rows"))) return rc;
+        if ((rc = ensure(c, c->refine_idx, (sizeof(int32_t) * 3 + sizeof(float)) * rcap, "refine rows"))) return rc;
         if ((rc = ensure(c, c->counts, sizeof(int32_t) * rcap, "refine counts"))) return rc;
+        if ((rc = ensure(c, c->small, sizeof(int32_t) * ((size_t)nqb + 8), "query counters"))) return rc;
         int32_t *cand_user = reinterpret_cast<int32_t *>(c->cand.p);
         int32_t *cand_rank = cand_user + cap;
         int32_t *refine_cand = reinterpret_cast<int32_t *>(c->refine_idx.p);
         int32_t *refine_user = refine_cand + rcap;
-        float *refine_thr = reinterpret_cast<float *>(refine_user + rcap);
+        int32_t *refine_query = refine_user + rcap;
+        float *refine_thr = reinterpret_cast<float *>(refine_query + rcap);
         int32_t *cnt = reinterpret_cast<int32_t *>(c->counts.p);
+        int32_t *q_recheck = reinterpret_cast<int32_t *>(c->small.p);  // per query
+        int32_t *recheck_n = q_recheck + nqb;
+        uint32_t *max_err_bits = reinterpret_cast<uint32_t *>(recheck_n + 1);
+        const bool screened = c->form == RKR_FORM_SCREENED;
 
         RKR_HIP(c, launch_compact(c->stream, bk, qip, U, S, c->ranks, I, nq, mode, qi, cand_user, cand_rank,
-                                  refine_total, refine_cand, refine_user, refine_thr));
-        RKR_HIP(c, hipMemsetAsync(cnt, 0, sizeof(int32_t) * rcap, c->stream));
-        RKR_HIP(c, hipEventRecord(c->ev[4], c->stream));
-        // events 4..5 bracket exactly the refinement kernel
-        RKR_HIP(c, launch_refine_count(c->stream, c->users, c->items, I, ld, d, refine_user, refine_thr, refine_total,
-                                       (int64_t)rcap, cnt));
-        RKR_HIP(c, hipEventRecord(c->ev[5], c->stream));
-        RKR_HIP(c, launch_scatter_counts(c->stream, refine_total, refine_cand, cnt, (int64_t)rcap, cand_rank));
-        RKR_HIP(c, launch_topk(c->stream, qi, nq, cand_user, cand_rank, qip, U, max_k, o_user, o_rank, o_qip, nullptr));
-        RKR_HIP(c, hipEventRecord(c->ev[6], c->stream));
-        RKR_HIP(c, hipMemcpyAsync(ho_user.data(), o_user, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));
-        RKR_HIP(c, hipMemcpyAsync(ho_rank.data(), o_rank, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));
-        RKR_HIP(c, hipMemcpyAsync(ho_qip.data(), o_qip, sizeof(float) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));
-        RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, c->stream));
-        int32_t h_refine_total = 0;
-        RKR_HIP(c, hipMemcpyAsync(&h_refine_total, refine_total, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        RKR_HIP(c, hipEventRecord(c->ev[7], c->stream));
-        RKR_HIP(c, hipStreamSynchronize(c->stream));
+                                  refine_total, refine_cand, refine_user, refine_query, refine_thr));
+        int32_t h_refine_total = 0, h_recheck_n = 0;
+        uint32_t h_err_bits = 0;
+        std::vector<int32_t> h_qrecheck((size_t)nq, 0);
+        // refinement + top-k; re-run when the recheck list overflowed
+        for (int attempt = 0; attempt < 3; ++attempt) {
+            int64_t rec_cap = 0;
+            if (screened) {
+                const int64_t want = std::max<int64_t>(1 << 20, (int64_t)c->recheck.bytes / 16);
+                if ((rc = ensure(c, c->recheck, 16 * (size_t)want, "recheck list"))) return rc;
+                rec_cap = (int64_t)(c->recheck.bytes / 16);
+                RKR_HIP(c, hipMemsetAsync(c->small.p, 0, sizeof(int32_t) * ((size_t)nqb + 8), c->stream));
+            }
+            RKR_HIP(c, hipMemsetAsync(cnt, 0, sizeof(int32_t) * rcap, c->stream));
+            RKR_HIP(c, hipEventRecord(c->ev[4], c->stream));
+            // events 4..8 bracket exactly the refinement (or screening) kernel
+            if (screened) {
+                ScreenLaunch L{};
+                L.users = c->users; L.user_norm = c->user_norm;
+                L.item_hi = c->item_hi; L.item_lo = c->item_lo; L.item_norm = c->item_norm;
+                L.n_item = I; L.ld = ld; L.K = d; L.Kp = c->kp;
+                L.rows_user = refine_user; L.rows_thr = refine_thr; L.n_rows_dev = refine_total;
+                L.max_rows = (int64_t)rcap; L.C = c->screen_C; L.abs_floor = c->screen_abs;
+                L.cnt = cnt; L.recheck_n = recheck_n; L.recheck = c->recheck.p; L.recheck_cap = rec_cap;
+                L.target_units = (int64_t)c->n_cu * 8;
+                RKR_HIP(c, launch_refine_screen(c->stream, L));
+                RKR_HIP(c, hipEventRecord(c->ev[8], c->stream));
+                RKR_HIP(c, launch_recheck(c->stream, c->users, c->items, ld, d, refine_user, refine_thr, refine_query,
+                                          recheck_n, c->recheck.p, rec_cap, cnt, q_recheck, max_err_bits));
+            } else {
+                RKR_HIP(c, launch_refine_count(c->stream, c->users, c->items, I, ld, d, refine_user, refine_thr,
+                                               refine_total, (int64_t)rcap, cnt));
+                RKR_HIP(c, hipEventRecord(c->ev[8], c->stream));
+            }
+            RKR_HIP(c, hipEventRecord(c->ev[5], c->stream));
+            RKR_HIP(c, launch_scatter_counts(c->stream, refine_total, refine_cand, cnt, (int64_t)rcap, cand_rank));
+            RKR_HIP(c, launch_topk(c->stream, qi, nq, cand_user, cand_rank, qip, U, max_k, o_user, o_rank, o_qip,
+                                   nullptr));
+            RKR_HIP(c, hipEventRecord(c->ev[6], c->stream));
+            RKR_HIP(c, hipMemcpyAsync(ho_user.data(), o_user, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));
+            RKR_HIP(c, hipMemcpyAsync(ho_rank.data(), o_rank, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));
+            RKR_HIP(c, hipMemcpyAsync(ho_qip.data(), o_qip, sizeof(float) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));
+            RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, c->stream));
+            RKR_HIP(c, hipMemcpyAsync(&h_refine_total, refine_total, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+            if (screened) {
+                RKR_HIP(c, hipMemcpyAsync(&h_recheck_n, recheck_n, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+                RKR_HIP(c, hipMemcpyAsync(&h_err_bits, max_err_bits, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+                RKR_HIP(c, hipMemcpyAsync(h_qrecheck.data(), q_recheck, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, c->stream));
+            }
+            RKR_HIP(c, hipEventRecord(c->ev[7], c->stream));
+            RKR_HIP(c, hipStreamSynchronize(c->stream));
+            if (!screened || (int64_t)h_recheck_n <= rec_cap) break;
+            // the list overflowed: grow it and run the refinement again
+            const size_t need = 16 * ((size_t)h_recheck_n + (size_t)h_recheck_n / 4 + 1024);
+            dev_free(c, c->recheck.p, c->recheck.bytes);
+            c->recheck.p = nullptr;
+            c->recheck.bytes = 0;
+            if ((rc = ensure(c, c->recheck, need, "recheck list"))) return rc;
+            if (attempt == 2) return set_err(c, RKR_ERR_NOMEM, "query: recheck list kept overflowing");
+        }
 
         std::memcpy(out_user + (size_t)q0 * max_k, ho_user.data(), sizeof(int32_t) * (size_t)nq * max_k);
         std::memcpy(out_rank + (size_t)q0 * max_k, ho_rank.data(), sizeof(int32_t) * (size_t)nq * max_k);
@@ -573,17 +659,29 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
                     st.n_prune_user = (int32_t)(U - hqi[i].n_settled - hqi[i].n_cand);
                 }
                 st.ip_cost = U + (int64_t)hqi[i].n_refine * I;
+                if (screened) {
+                    st.n_screened = (int64_t)hqi[i].n_refine * I;
+                    st.n_rechecked = h_qrecheck[i];
+                }
             }
         }
         tm.h2d_ms += ev_ms(c, 0, 1);
         tm.ip_ms += ev_ms(c, 1, 2);
         tm.bound_ms += ev_ms(c, 2, 3);
         tm.prune_ms += ev_ms(c, 3, 4);
-        tm.refine_ms += ev_ms(c, 4, 5);
+        tm.refine_ms += ev_ms(c, 4, 8);
+        tm.recheck_ms += ev_ms(c, 8, 5);
         tm.topk_ms += ev_ms(c, 5, 6);
         tm.d2h_ms += ev_ms(c, 6, 7);
         tm.n_batches += 1;
         tm.n_refined += h_refine_total;
+        if (screened) {
+            tm.n_rechecked += h_recheck_n;
+            tm.n_screened += (int64_t)h_refine_total * I;
+            float e;
+            std::memcpy(&e, &h_err_bits, sizeof(e));
+            tm.max_screen_err = std::max(tm.max_screen_err, (double)e);
+        }
     }
     tm.total_ms = std::chrono::duration<double, std::milli>(clock::now() - t_start).count();
     if (timing) *timing = tm;

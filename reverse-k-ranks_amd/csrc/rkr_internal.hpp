@@ -56,8 +56,16 @@ struct Ctx {
     float table_eps = 0.0f;
     std::vector<int32_t> h_ranks;
 
+    // screened form: hi/lo BF16 item rows (n_item x kp), norm bounds
+    int32_t kp = 0;
+    uint16_t *item_hi = nullptr, *item_lo = nullptr;
+    float *item_norm = nullptr, *user_norm = nullptr;
+    double max_abs_item = 0.0, max_abs_user_raw = 0.0;
+    float screen_C = 0.0f, screen_abs = 0.0f;
+    int n_cu = 256;
+
     // query scratch (grown on demand)
-    DeviceBuf q_in, qip, bk, qinfo, cand, refine_idx, counts, out, misc;
+    DeviceBuf q_in, qip, bk, qinfo, cand, refine_idx, counts, out, misc, recheck, small;
 };
 
 // ---- error helpers -------------------------------------------------------
@@ -89,6 +97,34 @@ hipError_t launch_refine_count(hipStream_t s, const float *users, const float *i
 // Transpose users into 64-user panels.
 hipError_t launch_make_panels(hipStream_t s, const float *users, int64_t n_user, int32_t ld, float *usersT);
 
+// ---- screened refinement (k_screen.hip) -------------------------------------
+struct ScreenLaunch {
+    const float *users, *user_norm;
+    const uint16_t *item_hi, *item_lo;
+    const float *item_norm;
+    int64_t n_item;
+    int32_t ld, K, Kp;
+    const int32_t *rows_user;
+    const float *rows_thr;
+    const int32_t *n_rows_dev;
+    int64_t max_rows;
+    float C, abs_floor;
+    int32_t *cnt;
+    int32_t *recheck_n;
+    void *recheck;          // int4 records (row, item, screened score bits, 0)
+    int64_t recheck_cap;
+    int64_t target_units;   // work units wanted to fill the device
+};
+hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, int32_t ld, int32_t K, int32_t Kp,
+                             uint16_t *hi, uint16_t *lo, float *norm);
+hipError_t launch_user_norms(hipStream_t s, const float *users, int64_t n_user, int32_t ld, int32_t K, float *norm);
+hipError_t launch_refine_screen(hipStream_t s, const ScreenLaunch &L);
+hipError_t launch_recheck(hipStream_t s, const float *users, const float *items, int32_t ld, int32_t K,
+                          const int32_t *rows_user, const float *rows_thr, const int32_t *rows_query,
+                          const int32_t *recheck_n, const void *recheck, int64_t cap, int32_t *cnt,
+                          int32_t *q_recheck, uint32_t *max_err_bits);
+int screen_max_kp();
+
 // ---- selection (k_select.hip) ---------------------------------------------
 // table[(u0 + b) * S + j] = r_j-th largest of scores[b * n_item + ...], b < nb.
 hipError_t launch_select_rows(hipStream_t s, const float *scores, int64_t nb, int64_t n_item,
@@ -115,7 +151,7 @@ hipError_t launch_offsets(hipStream_t s, QueryInfo *qi, int32_t nq, int32_t *ref
 hipError_t launch_compact(hipStream_t s, const uint32_t *bk, const float *qip, int64_t n_user, int32_t S,
                           const int32_t *ranks, int64_t n_item, int32_t nq, int output_mode, QueryInfo *qi,
                           int32_t *cand_user, int32_t *cand_rank, int32_t *refine_total, int32_t *refine_cand,
-                          int32_t *refine_user, float *refine_thr);
+                          int32_t *refine_user, int32_t *refine_query, float *refine_thr);
 hipError_t launch_scatter_counts(hipStream_t s, const int32_t *refine_total, const int32_t *refine_cand,
                                  const int32_t *cnt, int64_t max_rows, int32_t *cand_rank);
 hipError_t launch_topk(hipStream_t s, const QueryInfo *qi, int32_t nq, const int32_t *cand_user,

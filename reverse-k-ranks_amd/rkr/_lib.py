@@ -53,7 +53,8 @@ class QueryStats(C.Structure):
 
 class BatchTiming(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("ip_ms", C.c_double), ("bound_ms", C.c_double),
-                ("prune_ms", C.c_double), ("refine_ms", C.c_double), ("topk_ms", C.c_double),
+                ("prune_ms", C.c_double), ("refine_ms", C.c_double), ("recheck_ms", C.c_double),
+                ("topk_ms", C.c_double),
                 ("h2d_ms", C.c_double), ("d2h_ms", C.c_double), ("n_batches", C.c_int64),
                 ("n_refined", C.c_int64), ("n_rechecked", C.c_int64), ("n_screened", C.c_int64),
                 ("max_screen_err", C.c_double)]

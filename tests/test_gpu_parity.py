@@ -4,6 +4,8 @@ Integer outputs (user IDs, ranks) must be bit-exact; scores (table entries,
 <u,q>) follow the reference's sequential FP32 InnerProduct rule and must be
 bit-identical too (compared as floats, so -0.0 == +0.0).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -197,3 +199,106 @@ def test_save_load_roundtrip(tmp_path, gpu_index_factory):
     assert n_item == 900 and np.array_equal(r2, ranks) and np.array_equal(T2, T)
     idx.load_table(p)
     assert np.array_equal(idx.get_table(), T)
+
+
+# ---- screened form: BF16 MFMA screen + exact recheck -------------------------
+
+def _screened(idx):
+    idx.set_form("screened")
+    return idx
+
+
+def test_screened_fake30d_matches_bruteforce(orc, fake30d, gpu_index_factory):
+    users, items, queries = fake30d
+    _case("screened fake-normal-30d k=10")
+    idx, _ = gpu_index_factory(users, items, 128)
+    res = _screened(idx).query(queries, 10)
+    _check_exact(orc, _normalized(orc, users), items, queries, res, [10] * len(queries))
+    # queries are not items here, so only near-threshold items are rechecked
+    assert res.timing["n_screened"] > 0
+    assert res.timing["n_rechecked"] < 0.01 * res.timing["n_screened"]
+
+
+@pytest.mark.parametrize("nq,k", [(1, 50), (16, 7), (40, 120)])
+def test_screened_equals_plain(orc, gpu_index_factory, nq, k):
+    _case(f"screened vs plain nq={nq} k={k}")
+    rng = np.random.default_rng(21 + nq)
+    U, I, d = 6000, 3000, 150
+    users = rng.standard_normal((U, d), dtype=np.float32) / np.float32(np.sqrt(d))
+    items = (rng.standard_normal((I, d), dtype=np.float32) / np.float32(np.sqrt(d))
+             * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32))
+    queries = items[rng.choice(I, nq, replace=False)]
+    idx, _ = gpu_index_factory(users, items, 64)
+    plain = idx.query(queries, k)
+    scr = _screened(idx).query(queries, k)
+    assert np.array_equal(plain.users, scr.users) and np.array_equal(plain.ranks, scr.ranks)
+    _check_exact(orc, _normalized(orc, users), items, queries, scr, [k] * nq)
+    # q is an item: <u,q> ties with that item's own score, so it must be rechecked
+    assert scr.timing["n_rechecked"] >= plain.timing["n_refined"]
+    # the observed screening error stays inside the analytic bound's scale
+    assert 0 < scr.timing["max_screen_err"] < 1e-3
+
+
+def test_screened_ties_and_many_candidates(orc, gpu_index_factory):
+    _case("screened ties + many candidates")
+    rng = np.random.default_rng(23)
+    d, I, U = 10, 800, 20000
+    base = rng.standard_normal((60, d), dtype=np.float32)
+    items = base[rng.integers(0, 60, I)]
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    users[:5] = 0.0
+    queries = np.concatenate([items[:6], np.zeros((1, d), np.float32)])
+    idx, _ = gpu_index_factory(users, items, 8)
+    res = _screened(idx).query(queries, 300)
+    assert max(s["n_candidate"] for s in res.stats) > 4096
+    _check_exact(orc, _normalized(orc, users), items, queries, res, [300] * len(queries))
+
+
+def test_screened_reference_output(orc, fake30d, gpu_index_factory):
+    users, items, queries = fake30d
+    _case("screened reference-output")
+    idx, ranks = gpu_index_factory(users, items, 128)
+    idx.set_output("reference")
+    res = _screened(idx).query(queries, 10)
+    un = _normalized(orc, users)
+    T = orc.build_table(un, items, ranks)
+    ou, orank, oq = orc.us_retrieval(un, items, T, ranks, queries, 10)
+    assert np.array_equal(res.users, ou) and np.array_equal(res.ranks, orank)
+
+
+def test_screened_large_magnitudes(orc, gpu_index_factory):
+    """Unnormalised users and items with large and tiny magnitudes (denormal
+    BF16 splits): the bound's absolute floor and scaling keep answers exact."""
+    _case("screened magnitudes")
+    rng = np.random.default_rng(24)
+    U, I, d = 2000, 1500, 40
+    users = rng.standard_normal((U, d), dtype=np.float32) * np.float32(1e3)
+    items = rng.standard_normal((I, d), dtype=np.float32) * rng.choice(
+        np.array([1e-20, 1e-3, 1.0, 1e4], np.float32), (I, 1))
+    queries = items[:12]
+    idx = __import__("rkr").Index(0)
+    idx.set_data(users, items, normalize_users=False)
+    ranks = idx.uniform_sample_ranks(I, 40)
+    idx.build_table(ranks)
+    res = _screened(idx).query(queries, 25)
+    _check_exact(orc, users, items, queries, res, [25] * 12)
+
+
+def test_cli_pipeline_fake30d(orc, fake30d, tmp_path):
+    """fsr -> bsibc -> rri through the C++ driver, as the reference's run_rkr.py
+    chains its binaries; the rank CSVs hold the exact answers."""
+    import subprocess
+    from conftest import GOLDEN, ROOT
+    _case("cli fsr/bsibc/rri")
+    cli = os.path.join(ROOT, "reverse-k-ranks_amd", "rkr_cli")
+    common = ["--dataset_dir", GOLDEN, "--dataset_name", "fake-normal-30d", "--index_dir", str(tmp_path / "index"),
+              "--method_name", "US", "--n_sample", "128"]
+    for cmd in (["fsr"], ["bsibc"], ["rri", "--topk", "10", "--result_dir", str(tmp_path / "result")]):
+        r = subprocess.run([cli] + cmd + common, capture_output=True, text=True, timeout=100)
+        assert r.returncode == 0, r.stderr
+    stem = tmp_path / "result" / "rank" / "fake-normal-30d-US-top10-n_sample_128-n_thread_-1"
+    users = np.loadtxt(f"{stem}-userID.csv", delimiter=",", dtype=np.int64)
+    ranks = np.loadtxt(f"{stem}-rank.csv", delimiter=",", dtype=np.int64)
+    u, _, q = fake30d
+    bu, br, _ = orc.bruteforce(orc.normalize_users(u), fake30d[1], q, 10)
+    assert np.array_equal(users, bu) and np.array_equal(ranks, br)
