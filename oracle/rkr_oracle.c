@@ -364,9 +364,10 @@ int orc_us_retrieval(const float *users, int64_t n_user, const float *items, int
 /* GPU path implements).  Same score rule, table and tie rule as the    */
 /* reference (strict count RefineCPU.hpp:42; order UserRankElement.hpp:59-64), */
 /* with bounds that stay valid under score ties:                        */
-/*   b  = #{j : T[j] >= q - eps},  b2 = #{j : T[j] > q + eps}           */
-/*   lbs = (b == S) ? I : r[b]     (no item below the sample can beat q) */
-/*   ubs = (b2 == 0) ? 0 : r[b2-1] + 1                                  */
+/*   b  = #{j : T[j] > q - eps},   b2 = #{j : T[j] > q + eps}           */
+/*   lbs = (b == S) ? I : r[b]     (items at or after sample position  */
+/*                                   r[b] score <= T[b] <= q: none beats q) */
+/*   ubs = (b2 == 0) ? 0 : r[b2-1] + 1  (items up to r[b2-1] beat q)     */
 /* eps bounds |T - exact order statistic| (0 for tables built with this */
 /* score rule).  L = topk-th smallest lbs; candidates are users with    */
 /* ubs <= L; each candidate's rank is lbs when lbs == ubs, else counted. */
@@ -380,7 +381,7 @@ void orc_safe_bounds(const float *row, int32_t S, const int32_t *ranks, int64_t 
         qhi = (float)((double)qip + (double)eps);
         if ((double)qhi < (double)qip + (double)eps) qhi = nextafterf(qhi, INFINITY);
     }
-    const int32_t b = bucket_ge(row, S, qlo);
+    const int32_t b = bucket_gt(row, S, qlo);
     const int32_t b2 = bucket_gt(row, S, qhi);
     *lbs = b == S ? (int32_t)n_item : ranks[b];
     *ubs = b2 == 0 ? 0 : ranks[b2 - 1] + 1;

@@ -9,12 +9,16 @@
 // (S+1)-bin histogram of buckets instead of the reference's two max-heaps
 // (PruneCandidateByBound.hpp:35-77).
 //
-// Exact output (RKR_OUTPUT_EXACT): with b_lo = #{T >= q - eps} and
-// b_hi = #{T > q + eps},
-//   rank <= lbs = (b_lo == S ? I : r[b_lo]),
-//   rank >= ubs = (b_hi == 0 ? 0 : r[b_hi - 1] + 1),
-// which hold under score ties too.  beta = bucket of the k-th smallest lbs;
-// a user can be in the answer only if ubs <= r[beta], i.e. b_hi <= beta.
+// Exact output (RKR_OUTPUT_EXACT): with b_lo = #{T > q - eps} and
+// b_hi = #{T > q + eps} (eps bounds |T - exact order statistic|, 0 for tables
+// built with the score rule),
+//   rank <= lbs = (b_lo == S ? I : r[b_lo])       (no item at or below sample
+//                                                   position r[b_lo] beats q),
+//   rank >= ubs = (b_hi == 0 ? 0 : r[b_hi - 1] + 1) (items up to r[b_hi-1] do),
+// which hold under score ties too (q equal to a table entry: the reference's
+// #{T >= q} bucket is also valid but looser).  beta = bucket of the k-th
+// smallest lbs; a user can be in the answer only if ubs <= r[beta], i.e.
+// b_hi <= beta.
 //
 // Reference output (RKR_OUTPUT_REFERENCE): lb/ub and the result / prune /
 // refine classification of PruneCandidateByBound.hpp:83-150, and the
@@ -47,47 +51,40 @@ __device__ __forceinline__ int32_t bucket_gt(const float *__restrict__ row, int3
     return lo;
 }
 
+// Bucket pair of one (user, query).  Reference output keeps the reference's
+// bucket #{T >= q} (SampleSearch.hpp:99-102) in both halves.
+__device__ __forceinline__ void bucket_pair(const float *row, int32_t S, float q, float eps, int ref_mode,
+                                            uint32_t &lo, uint32_t &hi) {
+    if (ref_mode) {
+        lo = hi = (uint32_t)bucket_ge(row, S, q);
+    } else if (eps > 0.0f) {
+        lo = (uint32_t)bucket_gt(row, S, __double2float_rd((double)q - (double)eps));
+        hi = (uint32_t)bucket_gt(row, S, __double2float_ru((double)q + (double)eps));
+    } else {
+        lo = hi = (uint32_t)bucket_gt(row, S, q);
+    }
+}
+
 // grid: (user blocks, queries).  bk[b * U + u] = b_lo | b_hi << 16.
 __global__ void __launch_bounds__(256) k_bounds(const float *__restrict__ qip, const float *__restrict__ table,
-                                                int64_t n_user, int32_t S, float eps, uint32_t *__restrict__ bk) {
+                                                int64_t n_user, int32_t S, float eps, int ref_mode,
+                                                uint32_t *__restrict__ bk) {
     const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t b = blockIdx.y;
     if (u >= n_user) return;
     const float q = qip[b * n_user + u];
     const float *row = table + u * (int64_t)S;
     uint32_t lo, hi;
-    if (eps > 0.0f) {
-        const float qlo = __double2float_rd((double)q - (double)eps);
-        const float qhi = __double2float_ru((double)q + (double)eps);
-        lo = (uint32_t)bucket_ge(row, S, qlo);
-        hi = (uint32_t)bucket_gt(row, S, qhi);
-    } else {
-        lo = (uint32_t)bucket_ge(row, S, q);
-        // b_hi = b_lo minus the trailing run of entries equal to q
-        hi = lo;
-        while (hi > 0 && row[hi - 1] == q) --hi;
-    }
+    bucket_pair(row, S, q, eps, ref_mode, lo, hi);
     bk[b * n_user + u] = lo | (hi << 16);
 }
 
 // Batched form: a workgroup stages UT consecutive table rows in LDS and runs
 // all nq searches of those users from there (each row is read from HBM once
 // per batch instead of once per query).  Thread t serves user t % UT.
-__device__ __forceinline__ void bucket_pair(const float *row, int32_t S, float q, float eps, uint32_t &lo,
-                                            uint32_t &hi) {
-    if (eps > 0.0f) {
-        lo = (uint32_t)bucket_ge(row, S, __double2float_rd((double)q - (double)eps));
-        hi = (uint32_t)bucket_gt(row, S, __double2float_ru((double)q + (double)eps));
-    } else {
-        lo = (uint32_t)bucket_ge(row, S, q);
-        hi = lo;
-        while (hi > 0 && row[hi - 1] == q) --hi;
-    }
-}
-
 __global__ void __launch_bounds__(256) k_bounds_lds(const float *__restrict__ qip, const float *__restrict__ table,
                                                     int64_t n_user, int32_t S, int32_t nq, int32_t ut, float eps,
-                                                    uint32_t *__restrict__ bk) {
+                                                    int ref_mode, uint32_t *__restrict__ bk) {
     extern __shared__ float rows[];  // ut x S
     const int64_t u0 = (int64_t)blockIdx.x * ut;
     const int64_t nu = min((int64_t)ut, n_user - u0);
@@ -101,7 +98,7 @@ __global__ void __launch_bounds__(256) k_bounds_lds(const float *__restrict__ qi
     const float *row = rows + (int64_t)ul * S;
     for (int32_t b = threadIdx.x / ut; b < nq; b += lanes_q) {
         uint32_t lo, hi;
-        bucket_pair(row, S, qip[(int64_t)b * n_user + u], eps, lo, hi);
+        bucket_pair(row, S, qip[(int64_t)b * n_user + u], eps, ref_mode, lo, hi);
         bk[(int64_t)b * n_user + u] = lo | (hi << 16);
     }
 }
@@ -416,18 +413,18 @@ __global__ void __launch_bounds__(NT) k_topk(const QueryInfo *__restrict__ qi, c
 }  // namespace
 
 hipError_t launch_bounds(hipStream_t s, const float *qip, const float *table, int64_t n_user, int32_t S, int32_t nq,
-                         float eps, uint32_t *bk) {
+                         float eps, int ref_mode, uint32_t *bk) {
     if (n_user <= 0 || nq <= 0) return hipSuccess;
     if (nq >= 8) {
         int ut = 64;
         while (ut > 1 && (int64_t)ut * S * 4 > 65536) ut >>= 1;
         const unsigned grid = (unsigned)((n_user + ut - 1) / ut);
         hipLaunchKernelGGL(k_bounds_lds, dim3(grid), dim3(256), (size_t)ut * S * 4, s, qip, table, n_user, S, nq, ut,
-                           eps, bk);
+                           eps, ref_mode, bk);
         return hipGetLastError();
     }
     dim3 grid((unsigned)((n_user + 255) / 256), (unsigned)nq);
-    hipLaunchKernelGGL(k_bounds, grid, dim3(256), 0, s, qip, table, n_user, S, eps, bk);
+    hipLaunchKernelGGL(k_bounds, grid, dim3(256), 0, s, qip, table, n_user, S, eps, ref_mode, bk);
     return hipGetLastError();
 }
 

@@ -551,7 +551,7 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
             RKR_HIP(c, launch_gemv_panels(c->stream, c->usersT, U, ld, d, q_in, nq, qip));
         else RKR_HIP(c, launch_scores_store(c->stream, c->users, nullptr, U, ld, q_in, nq, ld, d, qip, U, false));
         RKR_HIP(c, hipEventRecord(c->ev[2], c->stream));
-        RKR_HIP(c, launch_bounds(c->stream, qip, c->table, U, S, nq, eps, bk));
+        RKR_HIP(c, launch_bounds(c->stream, qip, c->table, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
         RKR_HIP(c, hipEventRecord(c->ev[3], c->stream));
         RKR_HIP(c, launch_hist_kth(c->stream, bk, U, S, c->ranks, I, nq, mode, qi));
         RKR_HIP(c, launch_offsets(c->stream, qi, nq, refine_total));

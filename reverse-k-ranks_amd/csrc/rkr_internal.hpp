@@ -144,7 +144,7 @@ struct QueryInfo {          // one per query of a device batch
 };
 
 hipError_t launch_bounds(hipStream_t s, const float *qip, const float *table, int64_t n_user, int32_t S,
-                         int32_t nq, float eps, uint32_t *bk);
+                         int32_t nq, float eps, int ref_mode, uint32_t *bk);
 hipError_t launch_hist_kth(hipStream_t s, const uint32_t *bk, int64_t n_user, int32_t S, const int32_t *ranks,
                            int64_t n_item, int32_t nq, int output_mode, QueryInfo *qi);
 hipError_t launch_offsets(hipStream_t s, QueryInfo *qi, int32_t nq, int32_t *refine_total);

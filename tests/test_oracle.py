@@ -289,3 +289,21 @@ def test_table_golden_checksum(orc, fake30d):
     g = np.load(os.path.join(GOLDEN, "fake30d_us_top10.npz"))
     assert hashlib.sha256(np.ascontiguousarray(T + np.float32(0)).tobytes()).hexdigest() == str(g["table_sha256"])
     assert np.array_equal(T[:8], g["table_head"])
+
+
+def test_safe_bounds_fix_rank_zero_and_sampled_positions(orc):
+    """q equal to the user's top score has rank 0 and the bounds say so (the
+    reference's #{T >= q} bucket would give [0, r_1]); q strictly between
+    samples is bracketed by (r[b-1], r[b]]."""
+    rng = np.random.default_rng(8)
+    I, S = 500, 50
+    ranks = orc.uniform_sample(I, S)
+    scores = rng.standard_normal(I).astype(np.float32)
+    srt = -np.sort(-scores)
+    row = srt[ranks]
+    assert orc.safe_bounds(row, ranks, I, srt[0]) == (0, 0)
+    assert orc.coarse_bsearch(row, ranks, I, srt[0]) == (ranks[1], 0)
+    for j in range(1, S):
+        q = srt[ranks[j]]  # q ties the sampled entry j: r[j] items beat it
+        lbs, ubs = orc.safe_bounds(row, ranks, I, q)
+        assert lbs == ranks[j] and ubs == ranks[j - 1] + 1
