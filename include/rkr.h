@@ -165,6 +165,28 @@ int rkr_query(rkr_ctx *ctx, const float *queries, int32_t n_query, const int32_t
  * SampleSearch::IndexSizeByte (SampleSearch.hpp:246-250). */
 int64_t rkr_index_size_bytes(const rkr_ctx *ctx);
 
+/* ---- updates (exact output only) -------------------------------------------
+ * The table keeps describing the item set it was built from; each user keeps
+ * sorted side arrays of its scores against inserted items and deleted base
+ * items, and query bounds shift by #{inserted > q} - #{deleted > q}
+ * (SampleSearchUpdate, src/include/alg/RankBoundRefinement/SampleSearchUpdate.hpp:72-166,212-279,
+ * behind BaseUpdateIndex, src/include/struct/MethodBaseUpdate.hpp:15-40).
+ * Answers stay exact; rkr_build_table over the current items folds the side
+ * arrays into a new table.  Current item IDs: base items not deleted (in base
+ * order) followed by inserted items not deleted (in insertion order). */
+
+/* Appends n items (row-major FP32).  Replaces SampleSearchUpdate::InsertItem (:111-141). */
+int rkr_insert_items(rkr_ctx *ctx, const float *items, int64_t n);
+/* Removes items by current ID.  Replaces SampleSearchUpdate::DeleteItem (:142-166). */
+int rkr_delete_items(rkr_ctx *ctx, const int64_t *item_ids, int64_t n);
+/* Appends n users (normalised when the data were); their table rows are built
+ * on the device.  Replaces SampleSearchUpdate::InsertUser (:72-87). */
+int rkr_insert_users(rkr_ctx *ctx, const float *users, int64_t n);
+/* Removes users by ID; later users shift down.  Replaces SampleSearchUpdate::DeleteUser (:89-109). */
+int rkr_delete_users(rkr_ctx *ctx, const int64_t *user_ids, int64_t n);
+/* Current sizes: users, items, pending inserted and deleted items. */
+int rkr_get_counts(const rkr_ctx *ctx, int64_t *n_user, int64_t *n_item, int64_t *n_inserted, int64_t *n_deleted);
+
 #ifdef __cplusplus
 }
 #endif

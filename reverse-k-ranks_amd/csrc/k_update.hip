@@ -1,0 +1,381 @@
+// k_update.hip -- kernels of the update path (inserted / deleted items and
+// users).
+//
+// After items are inserted or deleted, the table T still holds the order
+// statistics of the base item set it was built from.  As the reference does
+// (SampleSearchUpdate::InsertItem / DeleteItem / SearchInsertDelete,
+// src/include/alg/RankBoundRefinement/SampleSearchUpdate.hpp:111-166,212-243),
+// each user keeps two sorted side arrays: its scores against the inserted
+// items and against the deleted base items.  The exact rank over the current
+// item set is
+//   rank = rank_base + #{inserted : s > q} - #{deleted : s > q},
+// so the base bounds shift by that exact offset (the reference uses
+// #{s >= q}; the strict count keeps the bounds exact under ties).  Bounds then
+// vary per user, so the k-th smallest upper bound is found by a two-level
+// histogram over bound values instead of over buckets.
+#include "rkr_internal.hpp"
+
+namespace rkr {
+namespace {
+
+constexpr int NT = 1024;
+constexpr int NW = NT / 64;
+
+// Sorts each row of `x` (rows of n <= 1024 floats, stride ld) descending in
+// place.  One single-wavefront workgroup per row, bitonic sort in LDS.
+__global__ void __launch_bounds__(64) k_sort_rows_desc(float *__restrict__ x, int64_t n_rows, int32_t n, int64_t ld) {
+    __shared__ float b[1024];
+    const int lane = threadIdx.x;
+    const int64_t row = blockIdx.x;
+    float *src = x + row * ld;
+    int p2 = 1;
+    while (p2 < n) p2 <<= 1;
+    for (int i = lane; i < p2; i += 64) b[i] = i < n ? src[i] : -INFINITY;
+    __syncthreads();
+    for (int size = 2; size <= p2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = lane; i < p2 / 2; i += 64) {
+                const int lo = 2 * i - (i & (stride - 1));
+                const int hi = lo + stride;
+                const bool desc = ((lo & size) == 0);
+                const float a = b[lo], c = b[hi];
+                if ((a < c) == desc) { b[lo] = c; b[hi] = a; }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = lane; i < n; i += 64) src[i] = b[i];
+}
+
+// Same for rows of up to 16384 floats: one 1024-thread workgroup per row.
+__global__ void __launch_bounds__(1024) k_sort_rows_desc_big(float *__restrict__ x, int64_t n_rows, int32_t n,
+                                                             int64_t ld) {
+    __shared__ float b[16384];
+    const int64_t row = blockIdx.x;
+    float *src = x + row * ld;
+    int p2 = 1;
+    while (p2 < n) p2 <<= 1;
+    for (int i = threadIdx.x; i < p2; i += 1024) b[i] = i < n ? src[i] : -INFINITY;
+    __syncthreads();
+    for (int size = 2; size <= p2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < p2 / 2; i += 1024) {
+                const int lo = 2 * i - (i & (stride - 1));
+                const int hi = lo + stride;
+                const bool desc = ((lo & size) == 0);
+                const float a = b[lo], c = b[hi];
+                if ((a < c) == desc) { b[lo] = c; b[hi] = a; }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += 1024) src[i] = b[i];
+}
+
+// #{j < n : a[j] > v} for a non-increasing row a.
+__device__ __forceinline__ int32_t count_gt(const float *a, int32_t n, float v) {
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (a[mid] > v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ int32_t count_ge(const float *a, int32_t n, float v) {
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (a[mid] >= v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// out[u] = merge of two non-increasing rows a[u] (na) and b[u] (nb).  Each
+// output position i finds its split by a co-rank search: element a[j] goes
+// to j + #{b > a[j]} and b[j] to j + #{a >= b[j]}, so ties keep a first and
+// every position is written once.
+__global__ void k_merge_rows(const float *__restrict__ a, int64_t lda, int32_t na, const float *__restrict__ b,
+                             int64_t ldb, int32_t nb, float *__restrict__ out, int64_t ldo, int64_t n_rows) {
+    const int64_t row = blockIdx.x;
+    if (row >= n_rows) return;
+    const float *ar = a + row * lda;
+    const float *br = b + row * ldb;
+    float *o = out + row * ldo;
+    for (int64_t i = threadIdx.x; i < (int64_t)na + nb; i += blockDim.x) {
+        if (i < na) {
+            const float v = ar[i];
+            o[i + count_gt(br, nb, v)] = v;
+        } else {
+            const int64_t j = i - na;
+            const float v = br[j];
+            o[j + count_ge(ar, na, v)] = v;
+        }
+    }
+}
+
+// out[u] = a[u] minus the multiset b[u] (both non-increasing; b is a
+// sub-multiset of a).  The t-th copy of a value in b removes the t-th copy of
+// that value in a; kept entries are compacted in order by one workgroup per row.
+__global__ void __launch_bounds__(256) k_subtract_rows(const float *__restrict__ a, int64_t lda, int32_t na,
+                                                       const float *__restrict__ b, int64_t ldb, int32_t nb,
+                                                       float *__restrict__ out, int64_t ldo, int64_t n_rows) {
+    __shared__ uint32_t red[5];
+    const int64_t row = blockIdx.x;
+    if (row >= n_rows) return;
+    const float *ar = a + row * lda;
+    const float *br = b + row * ldb;
+    float *o = out + row * ldo;
+    uint32_t base = 0;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int32_t i0 = 0; i0 < na; i0 += 256) {
+        const int32_t i = i0 + threadIdx.x;
+        bool keep = false;
+        float v = 0.f;
+        if (i < na) {
+            v = ar[i];
+            // copies of v before i in a, and copies of v in b
+            const int32_t first_a = count_gt(ar, na, v);
+            const int32_t nb_v = count_ge(br, nb, v) - count_gt(br, nb, v);
+            keep = (i - first_a) >= nb_v;
+        }
+        const uint64_t ball = __ballot(keep);
+        const uint32_t before = __popcll(ball & ((1ull << lane) - 1ull));
+        if (lane == 0) red[w] = __popcll(ball);
+        __syncthreads();
+        uint32_t wbase = base;
+        for (int k = 0; k < w; ++k) wbase += red[k];
+        const uint32_t tot = red[0] + red[1] + red[2] + red[3];
+        if (keep) o[wbase + before] = v;
+        __syncthreads();
+        base += tot;
+    }
+}
+
+// Bounds with side-array offsets (exact output).  lbs/ubs as values:
+//   off = #{ins > q} - #{del > q};  lbs' = lbs + off;  ubs' = ubs + off.
+__global__ void __launch_bounds__(256) k_bounds_upd(const float *__restrict__ qip, const float *__restrict__ table,
+                                                    int64_t n_user, int32_t S, const int32_t *__restrict__ ranks,
+                                                    int64_t n_item_base, float eps, const float *__restrict__ ins,
+                                                    int32_t n_ins, const float *__restrict__ del, int32_t n_del,
+                                                    int32_t *__restrict__ lbs, int32_t *__restrict__ ubs) {
+    const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t b = blockIdx.y;
+    if (u >= n_user) return;
+    const float q = qip[b * n_user + u];
+    const float *row = table + u * (int64_t)S;
+    float qlo = q, qhi = q;
+    if (eps > 0.0f) {
+        qlo = __double2float_rd((double)q - (double)eps);
+        qhi = __double2float_ru((double)q + (double)eps);
+    }
+    const int32_t blo = count_gt(row, S, qlo), bhi = count_gt(row, S, qhi);
+    int32_t off = 0;
+    if (n_ins) off += count_gt(ins + u * (int64_t)n_ins, n_ins, q);
+    if (n_del) off -= count_gt(del + u * (int64_t)n_del, n_del, q);
+    const int32_t lb = (blo == S ? (int32_t)n_item_base : ranks[blo]) + off;
+    const int32_t ub = (bhi == 0 ? 0 : ranks[bhi - 1] + 1) + off;
+    lbs[b * n_user + u] = lb;
+    ubs[b * n_user + u] = ub;
+}
+
+__device__ uint32_t wg_excl_scan(uint32_t v, uint32_t *red, uint32_t *total) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) red[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int i = 0; i < NW; ++i) { const uint32_t t = red[i]; red[i] = s; s += t; }
+        red[NW] = s;
+    }
+    __syncthreads();
+    const uint32_t r = red[w] + x - v;
+    *total = red[NW];
+    __syncthreads();
+    return r;
+}
+
+// One workgroup per query: L = topk-th smallest lbs' (two-level histogram:
+// value >> 11, then value & 2047; bound values stay below 2^22, which the
+// host enforces) and the candidate count #{ubs' <= L}.
+__global__ void __launch_bounds__(NT) k_kth_values(const int32_t *__restrict__ lbs, const int32_t *__restrict__ ubs,
+                                                   int64_t n_user, QueryInfo *__restrict__ qi) {
+    __shared__ uint32_t h[2048];
+    __shared__ int32_t sel_hi, sel_lo;
+    __shared__ uint32_t sel_rem, n_cand;
+    const int64_t b = blockIdx.x;
+    const int32_t *lv = lbs + b * n_user;
+    const int32_t *uv = ubs + b * n_user;
+    const uint32_t k = (uint32_t)qi[b].topk;
+    for (int i = threadIdx.x; i < 2048; i += NT) h[i] = 0;
+    __syncthreads();
+    for (int64_t u = threadIdx.x; u < n_user; u += NT) atomicAdd(&h[((uint32_t)lv[u] >> 11) & 2047u], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t cum = 0, j = 0;
+        for (; j < 2047; ++j) { if (cum + h[j] >= k) break; cum += h[j]; }
+        sel_hi = (int32_t)j;
+        sel_rem = k - cum;  // 1-based rank inside the bin
+    }
+    __syncthreads();
+    const int32_t hi = sel_hi;
+    for (int i = threadIdx.x; i < 2048; i += NT) h[i] = 0;
+    __syncthreads();
+    for (int64_t u = threadIdx.x; u < n_user; u += NT) {
+        const uint32_t v = (uint32_t)lv[u];
+        if ((int32_t)((v >> 11) & 2047u) == hi) atomicAdd(&h[v & 2047u], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t cum = 0, j = 0;
+        for (; j < 2047; ++j) { if (cum + h[j] >= sel_rem) break; cum += h[j]; }
+        sel_lo = (int32_t)j;
+        n_cand = 0;
+    }
+    __syncthreads();
+    const int32_t L = (hi << 11) + sel_lo;
+    uint32_t c = 0;
+    for (int64_t u = threadIdx.x; u < n_user; u += NT) c += uv[u] <= L ? 1u : 0u;
+    atomicAdd(&n_cand, c);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        QueryInfo q = qi[b];
+        q.mode = 0;
+        q.beta = -1;
+        q.gamma = L;
+        q.n_cand = (int32_t)n_cand;
+        q.n_settled = 0;
+        qi[b] = q;
+    }
+}
+
+// Ordered compaction of the candidates ubs' <= L (rank known when lbs' == ubs').
+__global__ void __launch_bounds__(NT) k_compact_values(const int32_t *__restrict__ lbs, const int32_t *__restrict__ ubs,
+                                                       const float *__restrict__ qip, int64_t n_user,
+                                                       QueryInfo *__restrict__ qi, int32_t *__restrict__ cand_user,
+                                                       int32_t *__restrict__ cand_rank, int32_t *__restrict__ refine_total,
+                                                       int32_t *__restrict__ refine_cand, int32_t *__restrict__ refine_user,
+                                                       int32_t *__restrict__ refine_query, float *__restrict__ refine_thr) {
+    __shared__ uint32_t red[NW + 1];
+    __shared__ int32_t n_refine;
+    const int64_t b = blockIdx.x;
+    const QueryInfo q = qi[b];
+    const int32_t *lv = lbs + b * n_user;
+    const int32_t *uv = ubs + b * n_user;
+    const float *qb = qip + b * n_user;
+    const int32_t L = q.gamma;
+    uint32_t n_can = 0;
+    int32_t my_refine = 0;
+    if (threadIdx.x == 0) n_refine = 0;
+    for (int64_t u0 = 0; u0 < n_user; u0 += NT) {
+        const int64_t u = u0 + threadIdx.x;
+        bool cand = false;
+        int32_t rank = -1;
+        if (u < n_user && uv[u] <= L) {
+            cand = true;
+            if (lv[u] == uv[u]) rank = lv[u];
+        }
+        uint32_t tot;
+        const uint32_t e = wg_excl_scan(cand ? 1u : 0u, red, &tot);
+        if (cand) {
+            const int64_t slot = q.cand_off + n_can + e;
+            cand_user[slot] = (int32_t)u;
+            cand_rank[slot] = rank;
+            if (rank < 0) {
+                ++my_refine;
+                const int32_t r = atomicAdd(refine_total, 1);
+                refine_cand[r] = (int32_t)slot;
+                refine_user[r] = (int32_t)u;
+                refine_query[r] = (int32_t)b;
+                refine_thr[r] = qb[u];
+            }
+        }
+        n_can += tot;
+    }
+    if (my_refine) atomicAdd(&n_refine, my_refine);
+    __syncthreads();
+    if (threadIdx.x == 0) qi[b].n_refine = n_refine;
+}
+
+__global__ void k_gather_rows(const float *__restrict__ src, int64_t lds, const int64_t *__restrict__ idx,
+                              int64_t n_rows, int64_t width, float *__restrict__ dst, int64_t ldd) {
+    const int64_t r = blockIdx.y;
+    if (r >= n_rows) return;
+    const float *s = src + idx[r] * lds;
+    float *d = dst + r * ldd;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < width; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
+
+}  // namespace
+
+hipError_t launch_sort_rows_desc(hipStream_t s, float *x, int64_t n_rows, int32_t n, int64_t ld) {
+    if (n_rows <= 0 || n <= 1) return hipSuccess;
+    if (n > 16384) return hipErrorInvalidValue;
+    if (n > 1024)
+        hipLaunchKernelGGL(k_sort_rows_desc_big, dim3((unsigned)n_rows), dim3(1024), 0, s, x, n_rows, n, ld);
+    else
+        hipLaunchKernelGGL(k_sort_rows_desc, dim3((unsigned)n_rows), dim3(64), 0, s, x, n_rows, n, ld);
+    return hipGetLastError();
+}
+
+hipError_t launch_merge_rows(hipStream_t s, const float *a, int64_t lda, int32_t na, const float *b, int64_t ldb,
+                             int32_t nb, float *out, int64_t ldo, int64_t n_rows) {
+    if (n_rows <= 0 || na + nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_merge_rows, dim3((unsigned)n_rows), dim3(256), 0, s, a, lda, na, b, ldb, nb, out, ldo, n_rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_subtract_rows(hipStream_t s, const float *a, int64_t lda, int32_t na, const float *b, int64_t ldb,
+                                int32_t nb, float *out, int64_t ldo, int64_t n_rows) {
+    if (n_rows <= 0 || na <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_subtract_rows, dim3((unsigned)n_rows), dim3(256), 0, s, a, lda, na, b, ldb, nb, out, ldo,
+                       n_rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_bounds_upd(hipStream_t s, const float *qip, const float *table, int64_t n_user, int32_t S,
+                             const int32_t *ranks, int64_t n_item_base, float eps, const float *ins, int32_t n_ins,
+                             const float *del, int32_t n_del, int32_t nq, int32_t *lbs, int32_t *ubs) {
+    if (n_user <= 0 || nq <= 0) return hipSuccess;
+    dim3 grid((unsigned)((n_user + 255) / 256), (unsigned)nq);
+    hipLaunchKernelGGL(k_bounds_upd, grid, dim3(256), 0, s, qip, table, n_user, S, ranks, n_item_base, eps, ins, n_ins,
+                       del, n_del, lbs, ubs);
+    return hipGetLastError();
+}
+
+hipError_t launch_kth_values(hipStream_t s, const int32_t *lbs, const int32_t *ubs, int64_t n_user, int32_t nq,
+                             QueryInfo *qi) {
+    if (nq <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_kth_values, dim3((unsigned)nq), dim3(NT), 0, s, lbs, ubs, n_user, qi);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_values(hipStream_t s, const int32_t *lbs, const int32_t *ubs, const float *qip,
+                                 int64_t n_user, int32_t nq, QueryInfo *qi, int32_t *cand_user, int32_t *cand_rank,
+                                 int32_t *refine_total, int32_t *refine_cand, int32_t *refine_user,
+                                 int32_t *refine_query, float *refine_thr) {
+    if (nq <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_compact_values, dim3((unsigned)nq), dim3(NT), 0, s, lbs, ubs, qip, n_user, qi, cand_user,
+                       cand_rank, refine_total, refine_cand, refine_user, refine_query, refine_thr);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(hipStream_t s, const float *src, int64_t lds, const int64_t *idx, int64_t n_rows,
+                              int64_t width, float *dst, int64_t ldd) {
+    if (n_rows <= 0 || width <= 0) return hipSuccess;
+    const unsigned gx = (unsigned)std::min<int64_t>((width + 255) / 256, 16);
+    for (int64_t r0 = 0; r0 < n_rows; r0 += 65535) {
+        const int64_t nr = std::min<int64_t>(65535, n_rows - r0);
+        hipLaunchKernelGGL(k_gather_rows, dim3(gx, (unsigned)nr), dim3(256), 0, s, src, lds, idx + r0, nr, width,
+                           dst + r0 * ldd, ldd);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace rkr

@@ -21,8 +21,6 @@ int set_err(Ctx *c, int code, const std::string &msg) {
     return code;
 }
 
-namespace {
-
 const char *kVersion = "rkr 0.1 (gfx950)";
 
 int dev_alloc(Ctx *c, void **p, size_t bytes, const char *what) {
@@ -47,11 +45,6 @@ void dev_free(Ctx *c, void *p, size_t bytes) {
     c->held -= std::min(c->held, bytes);
 }
 
-template <class T>
-int dev_alloc_t(Ctx *c, T **p, size_t n, const char *what) {
-    return dev_alloc(c, reinterpret_cast<void **>(p), n * sizeof(T), what);
-}
-
 // Grow-only scratch buffer.
 int ensure(Ctx *c, DeviceBuf &b, size_t bytes, const char *what) {
     if (b.bytes >= bytes) return RKR_OK;
@@ -69,6 +62,23 @@ int ensure(Ctx *c, DeviceBuf &b, size_t bytes, const char *what) {
     return RKR_OK;
 }
 
+namespace {
+
+void free_update_state(Ctx *c) {
+    dev_free(c, c->items_ins, sizeof(float) * (size_t)c->n_ins * c->ld);
+    dev_free(c, c->items_del, sizeof(float) * (size_t)c->n_del * c->ld);
+    dev_free(c, c->side_ins, sizeof(float) * (size_t)std::max<int32_t>(c->n_ins, 1) * c->n_user);
+    dev_free(c, c->side_del, sizeof(float) * (size_t)std::max<int32_t>(c->n_del, 1) * c->n_user);
+    c->items_ins = c->items_del = c->side_ins = c->side_del = nullptr;
+    c->n_ins = c->n_del = 0;
+    if (c->items_base && c->items_base != c->items)
+        dev_free(c, c->items_base, sizeof(float) * (size_t)c->n_item_base * c->ld);
+    c->items_base = c->items;
+    c->n_item_base = c->n_item;
+    c->cur_src.clear();
+    c->base_deleted.clear();
+}
+
 void free_table(Ctx *c) {
     dev_free(c, c->ranks, sizeof(int32_t) * (size_t)c->n_sample);
     dev_free(c, c->table, sizeof(float) * (size_t)c->n_user * (size_t)c->n_sample);
@@ -80,6 +90,7 @@ void free_table(Ctx *c) {
 }
 
 void free_data(Ctx *c) {
+    free_update_state(c);
     free_table(c);
     const size_t ld = (size_t)c->ld;
     dev_free(c, c->users, sizeof(float) * (size_t)c->n_user * ld);
@@ -286,6 +297,7 @@ int rkr_set_data(rkr_ctx *c, const float *users, int64_t n_user, const float *it
     const int32_t ld = padded_dim(dim);
     c->n_user = n_user;
     c->n_item = n_item;
+    c->normalize_users = normalize_users != 0;
     c->dim = dim;
     c->ld = ld;
     // host staging with padding
@@ -338,6 +350,8 @@ int rkr_set_data(rkr_ctx *c, const float *users, int64_t n_user, const float *it
         RKR_HIP(c, launch_prep_items(c->stream, c->items, n_item, ld, dim, kp, c->item_hi, c->item_lo, c->item_norm));
         RKR_HIP(c, launch_user_norms(c->stream, c->users, n_user, ld, dim, c->user_norm));
     }
+    c->items_base = c->items;
+    c->n_item_base = n_item;
     RKR_HIP(c, hipStreamSynchronize(c->stream));
     return RKR_OK;
 }
@@ -351,6 +365,7 @@ int rkr_build_table(rkr_ctx *c, const int32_t *ranks, int32_t S, double *build_m
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
     free_table(c);
+    free_update_state(c);  // the new table describes the current item set
     int rc = dev_alloc_t(c, &c->ranks, (size_t)S, "ranks");
     if (rc) return rc;
     rc = dev_alloc_t(c, &c->table, (size_t)c->n_user * S, "table");
@@ -396,6 +411,7 @@ int rkr_set_table(rkr_ctx *c, const int32_t *ranks, int32_t S, const float *tabl
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
     free_table(c);
+    free_update_state(c);  // the installed table describes the current item set
     int rc = dev_alloc_t(c, &c->ranks, (size_t)S, "ranks");
     if (rc) return rc;
     rc = dev_alloc_t(c, &c->table, (size_t)c->n_user * S, "table");
@@ -503,12 +519,17 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
     const int32_t S = c->n_sample, ld = c->ld, d = c->dim;
     const int mode = c->output;
     const float eps = mode == RKR_OUTPUT_EXACT ? c->table_eps : 0.0f;
+    // pending item inserts/deletes: bounds become per-user values (k_update.hip)
+    const bool upd = c->n_ins > 0 || c->n_del > 0;
+    if (upd && mode == RKR_OUTPUT_REFERENCE)
+        return set_err(c, RKR_ERR_STATE, "query: reference output has no update path; rebuild the table or use exact output");
+    const size_t pair_bytes = upd ? 12 : 8;  // <u,q> + buckets, or + two bound values
 
-    // queries per device batch: qip + buckets cost 8 bytes per (query, user)
+    // queries per device batch
     int64_t nqb = c->query_batch > 0 ? c->query_batch : 1024;
     {
         size_t avail = c->budget > c->held ? c->budget - c->held : 0;
-        const int64_t by_mem = (int64_t)(avail / 4 / ((size_t)U * 8 + 1));
+        const int64_t by_mem = (int64_t)(avail / 4 / ((size_t)U * pair_bytes + 1));
         nqb = std::max<int64_t>(1, std::min<int64_t>(nqb, by_mem));
         nqb = std::min<int64_t>(nqb, n_query);
         nqb = std::min<int64_t>(nqb, 65535);
@@ -516,7 +537,7 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
     int rc;
     if ((rc = ensure(c, c->q_in, sizeof(float) * (size_t)nqb * ld, "query vectors"))) return rc;
     if ((rc = ensure(c, c->qip, sizeof(float) * (size_t)nqb * U, "query scores"))) return rc;
-    if ((rc = ensure(c, c->bk, sizeof(uint32_t) * (size_t)nqb * U, "bucket pairs"))) return rc;
+    if ((rc = ensure(c, c->bk, (pair_bytes - 4) * (size_t)nqb * U, "bucket pairs"))) return rc;
     if ((rc = ensure(c, c->qinfo, sizeof(QueryInfo) * (size_t)nqb + 64, "query info"))) return rc;
     if ((rc = ensure(c, c->out, (sizeof(int32_t) * 2 + sizeof(float)) * (size_t)nqb * max_k, "outputs"))) return rc;
 
@@ -551,9 +572,16 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
             RKR_HIP(c, launch_gemv_panels(c->stream, c->usersT, U, ld, d, q_in, nq, qip));
         else RKR_HIP(c, launch_scores_store(c->stream, c->users, nullptr, U, ld, q_in, nq, ld, d, qip, U, false));
         RKR_HIP(c, hipEventRecord(c->ev[2], c->stream));
-        RKR_HIP(c, launch_bounds(c->stream, qip, c->table, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
+        int32_t *lbv = reinterpret_cast<int32_t *>(bk);
+        int32_t *ubv = lbv + (size_t)nq * U;
+        if (upd)
+            RKR_HIP(c, launch_bounds_upd(c->stream, qip, c->table, U, S, c->ranks, c->n_item_base, eps, c->side_ins,
+                                         c->n_ins, c->side_del, c->n_del, nq, lbv, ubv));
+        else
+            RKR_HIP(c, launch_bounds(c->stream, qip, c->table, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
         RKR_HIP(c, hipEventRecord(c->ev[3], c->stream));
-        RKR_HIP(c, launch_hist_kth(c->stream, bk, U, S, c->ranks, I, nq, mode, qi));
+        if (upd) RKR_HIP(c, launch_kth_values(c->stream, lbv, ubv, U, nq, qi));
+        else RKR_HIP(c, launch_hist_kth(c->stream, bk, U, S, c->ranks, I, nq, mode, qi));
         RKR_HIP(c, launch_offsets(c->stream, qi, nq, refine_total));
         RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, c->stream));
         RKR_HIP(c, hipStreamSynchronize(c->stream));
@@ -579,8 +607,12 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
         uint32_t *max_err_bits = reinterpret_cast<uint32_t *>(recheck_n + 1);
         const bool screened = c->form == RKR_FORM_SCREENED;
 
-        RKR_HIP(c, launch_compact(c->stream, bk, qip, U, S, c->ranks, I, nq, mode, qi, cand_user, cand_rank,
-                                  refine_total, refine_cand, refine_user, refine_query, refine_thr));
+        if (upd)
+            RKR_HIP(c, launch_compact_values(c->stream, lbv, ubv, qip, U, nq, qi, cand_user, cand_rank, refine_total,
+                                             refine_cand, refine_user, refine_query, refine_thr));
+        else
+            RKR_HIP(c, launch_compact(c->stream, bk, qip, U, S, c->ranks, I, nq, mode, qi, cand_user, cand_rank,
+                                      refine_total, refine_cand, refine_user, refine_query, refine_thr));
         int32_t h_refine_total = 0, h_recheck_n = 0;
         uint32_t h_err_bits = 0;
         std::vector<int32_t> h_qrecheck((size_t)nq, 0);

@@ -49,6 +49,19 @@ struct Ctx {
     std::vector<float> h_users; // host copy (normalised)
     double max_abs_user = 1.0;  // bound on |u_i| after normalisation
 
+    // update path: the table describes the base item set; inserted items and
+    // deleted base items are tracked per user in sorted side arrays
+    bool normalize_users = true;
+    float *items_base = nullptr;       // aliases `items` until the first item update
+    int64_t n_item_base = 0;
+    float *items_ins = nullptr;        // inserted items [n_ins][ld]
+    float *items_del = nullptr;        // deleted base items [n_del][ld]
+    int32_t n_ins = 0, n_del = 0;
+    float *side_ins = nullptr;         // [n_user][n_ins], rows non-increasing
+    float *side_del = nullptr;         // [n_user][n_del]
+    std::vector<int64_t> cur_src;      // current item -> base id (>= 0) or -(1 + inserted index)
+    std::vector<uint8_t> base_deleted; // per base item
+
     // table
     int32_t n_sample = 0;
     int32_t *ranks = nullptr;   // n_sample
@@ -68,8 +81,15 @@ struct Ctx {
     DeviceBuf q_in, qip, bk, qinfo, cand, refine_idx, counts, out, misc, recheck, small;
 };
 
-// ---- error helpers -------------------------------------------------------
+// ---- error and memory helpers (rkr_api.cpp) --------------------------------
 int set_err(Ctx *c, int code, const std::string &msg);
+int dev_alloc(Ctx *c, void **p, size_t bytes, const char *what);
+void dev_free(Ctx *c, void *p, size_t bytes);
+int ensure(Ctx *c, DeviceBuf &b, size_t bytes, const char *what);
+template <class T>
+int dev_alloc_t(Ctx *c, T **p, size_t n, const char *what) {
+    return dev_alloc(c, reinterpret_cast<void **>(p), n * sizeof(T), what);
+}
 #define RKR_HIP(ctx, call)                                                            \
     do {                                                                              \
         hipError_t e__ = (call);                                                      \
@@ -157,5 +177,23 @@ hipError_t launch_scatter_counts(hipStream_t s, const int32_t *refine_total, con
 hipError_t launch_topk(hipStream_t s, const QueryInfo *qi, int32_t nq, const int32_t *cand_user,
                        const int32_t *cand_rank, const float *qip, int64_t n_user, int32_t max_k,
                        int32_t *out_user, int32_t *out_rank, float *out_qip, int64_t *scratch_keys);
+
+// ---- update path (k_update.hip) ---------------------------------------------
+hipError_t launch_sort_rows_desc(hipStream_t s, float *x, int64_t n_rows, int32_t n, int64_t ld);
+hipError_t launch_merge_rows(hipStream_t s, const float *a, int64_t lda, int32_t na, const float *b, int64_t ldb,
+                             int32_t nb, float *out, int64_t ldo, int64_t n_rows);
+hipError_t launch_subtract_rows(hipStream_t s, const float *a, int64_t lda, int32_t na, const float *b, int64_t ldb,
+                                int32_t nb, float *out, int64_t ldo, int64_t n_rows);
+hipError_t launch_bounds_upd(hipStream_t s, const float *qip, const float *table, int64_t n_user, int32_t S,
+                             const int32_t *ranks, int64_t n_item_base, float eps, const float *ins, int32_t n_ins,
+                             const float *del, int32_t n_del, int32_t nq, int32_t *lbs, int32_t *ubs);
+hipError_t launch_kth_values(hipStream_t s, const int32_t *lbs, const int32_t *ubs, int64_t n_user, int32_t nq,
+                             QueryInfo *qi);
+hipError_t launch_compact_values(hipStream_t s, const int32_t *lbs, const int32_t *ubs, const float *qip,
+                                 int64_t n_user, int32_t nq, QueryInfo *qi, int32_t *cand_user, int32_t *cand_rank,
+                                 int32_t *refine_total, int32_t *refine_cand, int32_t *refine_user,
+                                 int32_t *refine_query, float *refine_thr);
+hipError_t launch_gather_rows(hipStream_t s, const float *src, int64_t lds, const int64_t *idx, int64_t n_rows,
+                              int64_t width, float *dst, int64_t ldd);
 
 }  // namespace rkr

@@ -40,7 +40,8 @@ EXPORTS = (
     "rkr_create", "rkr_destroy", "rkr_last_error", "rkr_set_option", "rkr_get_option", "rkr_sync",
     "rkr_version", "rkr_normalize_users", "rkr_set_data", "rkr_uniform_sample_ranks", "rkr_build_table",
     "rkr_set_table", "rkr_get_table", "rkr_get_ranks", "rkr_save_table", "rkr_load_table", "rkr_query",
-    "rkr_index_size_bytes",
+    "rkr_index_size_bytes", "rkr_insert_items", "rkr_delete_items", "rkr_insert_users", "rkr_delete_users",
+    "rkr_get_counts",
 )
 
 
@@ -99,6 +100,11 @@ def load():
     L.rkr_load_table.argtypes = [vp, C.c_char_p, C.c_float]
     L.rkr_query.argtypes = [vp, f32p, i32, i32p, i32, i32p, i32p, f32p, C.POINTER(QueryStats),
                             C.POINTER(BatchTiming)]
+    L.rkr_insert_items.argtypes = [vp, f32p, i64]
+    L.rkr_delete_items.argtypes = [vp, C.c_void_p, i64]
+    L.rkr_insert_users.argtypes = [vp, f32p, i64]
+    L.rkr_delete_users.argtypes = [vp, C.c_void_p, i64]
+    L.rkr_get_counts.argtypes = [vp] + [C.POINTER(C.c_int64)] * 4
     L.rkr_index_size_bytes.argtypes = [vp]
     L.rkr_index_size_bytes.restype = C.c_int64
     for name in EXPORTS:

@@ -141,6 +141,36 @@ class Index:
     def load_table(self, path: str, eps: float = 0.0):
         _check(self.lib, self.ctx, self.lib.rkr_load_table(self.ctx, path.encode(), eps))
 
+    # ---- updates (BaseUpdateIndex, src/include/struct/MethodBaseUpdate.hpp:15-40) ----
+    def insert_items(self, items):
+        items = _f32(items)
+        _check(self.lib, self.ctx, self.lib.rkr_insert_items(self.ctx, _ptr(items), items.shape[0]))
+        self._refresh_counts()
+
+    def delete_items(self, ids):
+        ids = np.ascontiguousarray(ids, np.int64)
+        _check(self.lib, self.ctx, self.lib.rkr_delete_items(self.ctx, _ptr(ids), ids.shape[0]))
+        self._refresh_counts()
+
+    def insert_users(self, users):
+        users = _f32(users)
+        _check(self.lib, self.ctx, self.lib.rkr_insert_users(self.ctx, _ptr(users), users.shape[0]))
+        self._refresh_counts()
+
+    def delete_users(self, ids):
+        ids = np.ascontiguousarray(ids, np.int64)
+        _check(self.lib, self.ctx, self.lib.rkr_delete_users(self.ctx, _ptr(ids), ids.shape[0]))
+        self._refresh_counts()
+
+    def counts(self):
+        v = [C.c_int64() for _ in range(4)]
+        _check(self.lib, self.ctx, self.lib.rkr_get_counts(self.ctx, *[C.byref(x) for x in v]))
+        return dict(zip(("n_user", "n_item", "n_inserted", "n_deleted"), (x.value for x in v)))
+
+    def _refresh_counts(self):
+        c = self.counts()
+        self.n_user, self.n_item = c["n_user"], c["n_item"]
+
     def index_size_bytes(self) -> int:
         return int(self.lib.rkr_index_size_bytes(self.ctx))
 

@@ -1,0 +1,146 @@
+"""GPU parity of the update path: inserts and deletes of items and users keep
+every answer equal to brute force over the current users and items."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+class Mirror:
+    """Host copy of the current user and item sets, in the library's order:
+    base items not deleted, then inserted items not deleted."""
+
+    def __init__(self, orc, users, items):
+        self.orc = orc
+        self.users = orc.normalize_users(users)
+        self.base = items.copy()
+        self.base_alive = np.ones(len(items), bool)
+        self.ins = np.zeros((0, items.shape[1]), np.float32)
+
+    def items(self):
+        return np.concatenate([self.base[self.base_alive], self.ins])
+
+    def insert_items(self, x):
+        self.ins = np.concatenate([self.ins, x])
+
+    def delete_items(self, ids):
+        cur_base = np.flatnonzero(self.base_alive)
+        nb = len(cur_base)
+        ins_keep = np.ones(len(self.ins), bool)
+        for i in ids:
+            if i < nb:
+                self.base_alive[cur_base[i]] = False
+            else:
+                ins_keep[i - nb] = False
+        self.ins = self.ins[ins_keep]
+
+    def insert_users(self, u):
+        self.users = np.concatenate([self.users, self.orc.normalize_users(u)])
+
+    def delete_users(self, ids):
+        keep = np.ones(len(self.users), bool)
+        keep[ids] = False
+        self.users = self.users[keep]
+
+    def check(self, idx, queries, k, form="plain"):
+        idx.set_form(form)
+        res = idx.query(queries, k)
+        items = self.items()
+        assert idx.counts()["n_item"] == len(items) and idx.counts()["n_user"] == len(self.users)
+        bu, br, bq = self.orc.bruteforce(self.users, items, queries, k)
+        for i in range(len(queries)):
+            assert np.array_equal(res.users[i], bu[i]), f"query {i}: users differ ({form})"
+            assert np.array_equal(res.ranks[i], br[i]), f"query {i}: ranks differ ({form})"
+        return res
+
+
+def _data(seed, U=3000, I=2000, d=24):
+    rng = np.random.default_rng(seed)
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = rng.standard_normal((I, d), dtype=np.float32) * rng.lognormal(0, 0.4, (I, 1)).astype(np.float32)
+    return rng, users, items
+
+
+@pytest.mark.parametrize("form", ["plain", "screened"])
+def test_insert_and_delete_items(orc, gpu_index_factory, form):
+    _case(f"item inserts/deletes ({form})")
+    rng, users, items = _data(31)
+    idx, _ = gpu_index_factory(users, items, 40)
+    m = Mirror(orc, users, items)
+    queries = items[rng.choice(len(items), 12, replace=False)]
+    m.check(idx, queries, 30, form)
+    new = rng.standard_normal((150, items.shape[1]), dtype=np.float32) * np.float32(1.5)
+    idx.insert_items(new)
+    m.insert_items(new)
+    q2 = np.concatenate([queries, new[:6]])  # queries equal to inserted items
+    m.check(idx, q2, 30, form)
+    # delete base and inserted items in one call
+    dels = np.array([0, 5, 17, len(items) + 3, len(items) + 40], np.int64)
+    idx.delete_items(dels)
+    m.delete_items(dels)
+    m.check(idx, q2, 30, form)
+    # a second insert batch merges into the sorted side arrays
+    new2 = rng.standard_normal((1100, items.shape[1]), dtype=np.float32)
+    idx.insert_items(new2)
+    m.insert_items(new2)
+    m.check(idx, q2, 60, form)
+
+
+def test_insert_and_delete_users(orc, gpu_index_factory):
+    _case("user inserts/deletes")
+    rng, users, items = _data(32)
+    idx, _ = gpu_index_factory(users, items, 40)
+    m = Mirror(orc, users, items)
+    new_items = rng.standard_normal((70, items.shape[1]), dtype=np.float32)
+    idx.insert_items(new_items)
+    m.insert_items(new_items)
+    idx.delete_items(np.array([2, 3], np.int64))
+    m.delete_items([2, 3])
+    queries = items[rng.choice(len(items), 10, replace=False)]
+    new_users = rng.standard_normal((500, items.shape[1]), dtype=np.float32)
+    new_users[0] *= 0  # a zero user (left unnormalised) has rank 0 everywhere
+    idx.insert_users(new_users)
+    m.insert_users(new_users)
+    m.check(idx, queries, 25)
+    m.check(idx, queries, 25, "screened")
+    gone = np.array([0, 1, 999, 3100, 3499], np.int64)
+    idx.delete_users(gone)
+    m.delete_users(gone)
+    m.check(idx, queries, 25)
+
+
+def test_rebuild_after_updates_matches_oracle_table(orc, gpu_index_factory):
+    _case("rebuild table after updates")
+    rng, users, items = _data(33, U=800, I=1500)
+    idx, _ = gpu_index_factory(users, items, 30)
+    m = Mirror(orc, users, items)
+    new = rng.standard_normal((300, items.shape[1]), dtype=np.float32)
+    idx.insert_items(new)
+    m.insert_items(new)
+    idx.delete_items(np.array([10, 11, 1600], np.int64))
+    m.delete_items([10, 11, 1600])
+    cur = m.items()
+    ranks = idx.uniform_sample_ranks(len(cur), 30)
+    idx.build_table(ranks)
+    assert idx.counts()["n_inserted"] == 0 and idx.counts()["n_deleted"] == 0
+    assert np.array_equal(idx.get_table(), orc.build_table(m.users, cur, ranks))
+    m.check(idx, cur[:8], 20)
+
+
+def test_reference_output_rejects_updates(gpu_index_factory):
+    _case("reference output + updates")
+    import rkr
+    rng, users, items = _data(34, U=200, I=300)
+    idx, _ = gpu_index_factory(users, items, 10)
+    idx.set_output("reference")
+    with pytest.raises(rkr.RkrError):
+        idx.insert_items(items[:3])
+    idx.set_output("exact")
+    idx.insert_items(items[:3])
+    idx.set_output("reference")
+    with pytest.raises(rkr.RkrError):
+        idx.query(items[:2], 5)
