@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(256) k_subtract_rows(const float *__restrict__
         uint32_t wbase = base;
         for (int k = 0; k < w; ++k) wbase += red[k];
         const uint32_t tot = red[0] + red[1] + red[2] + red[3];
-        if (keep) o[wbase + before] = v;
+        if (keep && wbase + before < (uint32_t)ldo) o[wbase + before] = v;  // ldo = na - nb kept entries
         __syncthreads();
         base += tot;
     }
