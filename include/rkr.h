@@ -187,6 +187,54 @@ int rkr_delete_users(rkr_ctx *ctx, const int64_t *user_ids, int64_t n);
 /* Current sizes: users, items, pending inserted and deleted items. */
 int rkr_get_counts(const rkr_ctx *ctx, int64_t *n_user, int64_t *n_item, int64_t *n_inserted, int64_t *n_deleted);
 
+/* ---- serving loop -------------------------------------------------------------
+ * Continuous batching on one context: requests (queries with their own k, and
+ * user / item updates) enter one FIFO; worker threads, each with its own HIP
+ * stream and scratch, batch the queries at the front of the queue, so batches
+ * overlap on the device and requests that arrive meanwhile join the next one.
+ * Updates are barriers: each query sees exactly the updates submitted before
+ * it.  Answers are identical to answering each query alone.  The reference has
+ * no serving loop (its drivers run queries one after another, US.hpp:117);
+ * this replaces that loop for the serving form. */
+typedef struct rkr_server rkr_server;
+
+typedef struct rkr_server_config {
+    int32_t max_batch;   /* queries per device batch (0 = 256); memory may split it */
+    int32_t n_slots;     /* concurrent batches / worker threads (0 = 2, max 4)       */
+} rkr_server_config;
+
+#define RKR_UPDATE_INSERT_USERS 1
+#define RKR_UPDATE_INSERT_ITEMS 2
+#define RKR_UPDATE_DELETE_USERS 3
+#define RKR_UPDATE_DELETE_ITEMS 4
+
+typedef struct rkr_request_info {
+    int32_t status;       /* RKR_OK or the request's error                 */
+    int32_t batch_size;   /* queries in the device batch that answered it  */
+    double t_submit;      /* rkr_now() seconds: admitted to the queue      */
+    double t_start;       /* taken by a worker                             */
+    double t_done;        /* answer ready                                  */
+    rkr_query_stats stats;
+    char error[256];
+} rkr_request_info;
+
+/* Monotonic clock (seconds) used by the request timestamps. */
+double rkr_now(void);
+int rkr_server_start(rkr_ctx *ctx, const rkr_server_config *cfg, rkr_server **out);
+/* Copies q (dim floats); the answer has k entries. */
+int rkr_server_submit_query(rkr_server *srv, const float *q, int32_t k, uint64_t *ticket);
+/* kind = RKR_UPDATE_*; vectors (n x dim) for inserts, ids for deletes (copied). */
+int rkr_server_submit_update(rkr_server *srv, int kind, const float *vectors, const int64_t *ids, int64_t n,
+                             uint64_t *ticket);
+/* Blocks until the request is answered; writes k users / ranks / <u,q> for a
+ * query (buffers may be NULL) and releases the ticket.  Returns the request's status. */
+int rkr_server_wait(rkr_server *srv, uint64_t ticket, int32_t *out_user, int32_t *out_rank, float *out_qip,
+                    rkr_request_info *info);
+int rkr_server_poll(rkr_server *srv, uint64_t ticket, int *done);
+int rkr_server_counters(rkr_server *srv, int64_t *n_batches, int64_t *n_queries, int64_t *n_updates);
+/* Answers everything queued, joins the workers and frees the server. */
+int rkr_server_stop(rkr_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

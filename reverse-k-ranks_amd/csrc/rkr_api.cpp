@@ -17,7 +17,10 @@
 namespace rkr {
 
 int set_err(Ctx *c, int code, const std::string &msg) {
-    if (c) c->err = msg;
+    if (c) {
+        std::lock_guard<std::mutex> g(c->err_mu);
+        c->err = msg;
+    }
     return code;
 }
 
@@ -26,6 +29,7 @@ const char *kVersion = "rkr 0.1 (gfx950)";
 int dev_alloc(Ctx *c, void **p, size_t bytes, const char *what) {
     *p = nullptr;
     if (bytes == 0) return RKR_OK;
+    std::lock_guard<std::mutex> g(c->mem_mu);
     if (c->held + bytes > c->budget)
         return set_err(c, RKR_ERR_NOMEM, std::string("device memory budget exceeded allocating ") + what + " (" +
                                              std::to_string(bytes) + " bytes, budget " + std::to_string(c->budget) +
@@ -41,7 +45,8 @@ int dev_alloc(Ctx *c, void **p, size_t bytes, const char *what) {
 
 void dev_free(Ctx *c, void *p, size_t bytes) {
     if (!p) return;
-    hipFree(p);
+    std::lock_guard<std::mutex> g(c->mem_mu);
+    (void)hipFree(p);
     c->held -= std::min(c->held, bytes);
 }
 
@@ -109,14 +114,44 @@ void free_data(Ctx *c) {
     c->h_users.clear();
 }
 
-void free_scratch(Ctx *c) {
-    for (DeviceBuf *b : {&c->q_in, &c->qip, &c->bk, &c->qinfo, &c->cand, &c->refine_idx, &c->counts, &c->out,
-                         &c->misc, &c->recheck, &c->small}) {
+void free_bufs(Ctx *c, std::initializer_list<DeviceBuf *> bufs) {
+    for (DeviceBuf *b : bufs) {
         dev_free(c, b->p, b->bytes);
         b->p = nullptr;
         b->bytes = 0;
     }
 }
+
+void free_scratch(Ctx *c) {
+    Slot &s = c->main;
+    free_bufs(c, {&s.q_in, &s.qip, &s.bk, &s.qinfo, &s.cand, &s.refine_idx, &s.counts, &s.out, &s.recheck, &s.small,
+                  &c->misc});
+}
+
+}  // namespace
+
+int slot_init(Ctx *c, Slot &sl, bool own_stream) {
+    sl.own_stream = own_stream;
+    if (own_stream) {
+        RKR_HIP(c, hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+    } else {
+        sl.stream = c->stream;
+    }
+    for (auto &e : sl.ev) RKR_HIP(c, hipEventCreate(&e));
+    return RKR_OK;
+}
+
+void slot_free(Ctx *c, Slot &sl) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    free_bufs(c, {&sl.q_in, &sl.qip, &sl.bk, &sl.qinfo, &sl.cand, &sl.refine_idx, &sl.counts, &sl.out, &sl.recheck,
+                  &sl.small});
+    for (auto &e : sl.ev)
+        if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    if (sl.own_stream && sl.stream) (void)hipStreamDestroy(sl.stream);
+    sl.stream = nullptr;
+}
+
+namespace {
 
 // FP32 normalisation exactly as VectorMatrix::vectorNormalize
 // (src/include/struct/VectorMatrix.hpp:51-66).  This file is compiled with
@@ -161,9 +196,9 @@ bool ranks_valid(const int32_t *r, int32_t S, int64_t n_item) {
     return true;
 }
 
-double ev_ms(Ctx *c, int a, int b) {
+double ev_ms(hipEvent_t *ev, int a, int b) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0.0;
+    if (hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) return 0.0;
     return ms;
 }
 
@@ -197,6 +232,7 @@ int rkr_create(int device, size_t mem_budget_bytes, rkr_ctx **out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return RKR_ERR_HIP; }
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) { rkr_destroy(c); return RKR_ERR_HIP; }
+    if (slot_init(c, c->main, false) != RKR_OK) { rkr_destroy(c); return RKR_ERR_HIP; }
     *out = c;
     return RKR_OK;
 }
@@ -206,9 +242,10 @@ void rkr_destroy(rkr_ctx *c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     free_scratch(c);
+    slot_free(c, c->main);
     free_data(c);
     for (auto &e : c->ev)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -398,7 +435,7 @@ int rkr_build_table(rkr_ctx *c, const int32_t *ranks, int32_t S, double *build_m
     }
     RKR_HIP(c, hipEventRecord(c->ev[1], c->stream));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
-    if (build_ms) *build_ms = ev_ms(c, 0, 1);
+    if (build_ms) *build_ms = ev_ms(c->ev, 0, 1);
     return RKR_OK;
 }
 
@@ -493,11 +530,24 @@ int64_t rkr_index_size_bytes(const rkr_ctx *c) {
 int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k,
               int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
               rkr_batch_timing *timing) {
-    using clock = std::chrono::steady_clock;
-    const auto t_start = clock::now();
     if (!c) return RKR_ERR_INVALID;
+    int rc = validate_queries(c, queries, n_query, topk, max_k);
+    if (rc) return rc;
+    if (!out_user || !out_rank) return set_err(c, RKR_ERR_INVALID, "query: null output buffer");
+    if (timing) std::memset(timing, 0, sizeof(*timing));
+    if (n_query == 0) return RKR_OK;
+    RKR_HIP(c, hipSetDevice(c->device));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    return query_on_slot(c, c->main, queries, n_query, topk, max_k, out_user, out_rank, out_qip, stats, timing);
+}
+
+}  // extern "C"
+
+namespace rkr {
+
+int validate_queries(Ctx *c, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k) {
     if (!c->table) return set_err(c, RKR_ERR_STATE, "query: no table (build or load one first)");
-    if (n_query < 0 || (n_query > 0 && (!queries || !topk || !out_user || !out_rank)))
+    if (n_query < 0 || (n_query > 0 && (!queries || !topk)))
         return set_err(c, RKR_ERR_INVALID, "query: null buffer");
     if (max_k <= 0 || max_k > kMaxTopk) return set_err(c, RKR_ERR_INVALID, "query: max_k must be in [1, 4096]");
     for (int32_t i = 0; i < n_query; ++i)
@@ -510,19 +560,22 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
         if ((double)c->dim * mq * c->max_abs_user > kScoreLimit)
             return set_err(c, RKR_ERR_INVALID, "query: magnitudes would overflow FP32 scores");
     }
-    if (timing) std::memset(timing, 0, sizeof(*timing));
-    if (n_query == 0) return RKR_OK;
-    RKR_HIP(c, hipSetDevice(c->device));
-    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->output == RKR_OUTPUT_REFERENCE && (c->n_ins > 0 || c->n_del > 0))
+        return set_err(c, RKR_ERR_STATE, "query: reference output has no update path; rebuild the table or use exact output");
+    return RKR_OK;
+}
 
+int query_on_slot(Ctx *c, Slot &sl, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k,
+                  int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
+                  rkr_batch_timing *timing) {
+    using clock = std::chrono::steady_clock;
+    const auto t_start = clock::now();
     const int64_t U = c->n_user, I = c->n_item;
     const int32_t S = c->n_sample, ld = c->ld, d = c->dim;
     const int mode = c->output;
     const float eps = mode == RKR_OUTPUT_EXACT ? c->table_eps : 0.0f;
     // pending item inserts/deletes: bounds become per-user values (k_update.hip)
     const bool upd = c->n_ins > 0 || c->n_del > 0;
-    if (upd && mode == RKR_OUTPUT_REFERENCE)
-        return set_err(c, RKR_ERR_STATE, "query: reference output has no update path; rebuild the table or use exact output");
     const size_t pair_bytes = upd ? 12 : 8;  // <u,q> + buckets, or + two bound values
 
     // queries per device batch
@@ -535,11 +588,11 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
         nqb = std::min<int64_t>(nqb, 65535);
     }
     int rc;
-    if ((rc = ensure(c, c->q_in, sizeof(float) * (size_t)nqb * ld, "query vectors"))) return rc;
-    if ((rc = ensure(c, c->qip, sizeof(float) * (size_t)nqb * U, "query scores"))) return rc;
-    if ((rc = ensure(c, c->bk, (pair_bytes - 4) * (size_t)nqb * U, "bucket pairs"))) return rc;
-    if ((rc = ensure(c, c->qinfo, sizeof(QueryInfo) * (size_t)nqb + 64, "query info"))) return rc;
-    if ((rc = ensure(c, c->out, (sizeof(int32_t) * 2 + sizeof(float)) * (size_t)nqb * max_k, "outputs"))) return rc;
+    if ((rc = ensure(c, sl.q_in, sizeof(float) * (size_t)nqb * ld, "query vectors"))) return rc;
+    if ((rc = ensure(c, sl.qip, sizeof(float) * (size_t)nqb * U, "query scores"))) return rc;
+    if ((rc = ensure(c, sl.bk, (pair_bytes - 4) * (size_t)nqb * U, "bucket pairs"))) return rc;
+    if ((rc = ensure(c, sl.qinfo, sizeof(QueryInfo) * (size_t)nqb + 64, "query info"))) return rc;
+    if ((rc = ensure(c, sl.out, (sizeof(int32_t) * 2 + sizeof(float)) * (size_t)nqb * max_k, "outputs"))) return rc;
 
     std::vector<float> hq((size_t)nqb * ld, 0.0f);
     std::vector<QueryInfo> hqi((size_t)nqb);
@@ -549,12 +602,12 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
 
     for (int64_t q0 = 0; q0 < n_query; q0 += nqb) {
         const int32_t nq = (int32_t)std::min<int64_t>(nqb, n_query - q0);
-        float *q_in = reinterpret_cast<float *>(c->q_in.p);
-        float *qip = reinterpret_cast<float *>(c->qip.p);
-        uint32_t *bk = reinterpret_cast<uint32_t *>(c->bk.p);
-        QueryInfo *qi = reinterpret_cast<QueryInfo *>(c->qinfo.p);
-        int32_t *refine_total = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(c->qinfo.p) + sizeof(QueryInfo) * nqb);
-        int32_t *o_user = reinterpret_cast<int32_t *>(c->out.p);
+        float *q_in = reinterpret_cast<float *>(sl.q_in.p);
+        float *qip = reinterpret_cast<float *>(sl.qip.p);
+        uint32_t *bk = reinterpret_cast<uint32_t *>(sl.bk.p);
+        QueryInfo *qi = reinterpret_cast<QueryInfo *>(sl.qinfo.p);
+        int32_t *refine_total = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(sl.qinfo.p) + sizeof(QueryInfo) * nqb);
+        int32_t *o_user = reinterpret_cast<int32_t *>(sl.out.p);
         int32_t *o_rank = o_user + (size_t)nqb * max_k;
         float *o_qip = reinterpret_cast<float *>(o_rank + (size_t)nqb * max_k);
 
@@ -563,55 +616,55 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
             hqi[i] = QueryInfo{};
             hqi[i].topk = topk[q0 + i];
         }
-        RKR_HIP(c, hipEventRecord(c->ev[0], c->stream));
-        RKR_HIP(c, hipMemcpyAsync(q_in, hq.data(), sizeof(float) * (size_t)nq * ld, hipMemcpyHostToDevice, c->stream));
-        RKR_HIP(c, hipMemcpyAsync(qi, hqi.data(), sizeof(QueryInfo) * nq, hipMemcpyHostToDevice, c->stream));
-        RKR_HIP(c, hipEventRecord(c->ev[1], c->stream));
+        RKR_HIP(c, hipEventRecord(sl.ev[0], sl.stream));
+        RKR_HIP(c, hipMemcpyAsync(q_in, hq.data(), sizeof(float) * (size_t)nq * ld, hipMemcpyHostToDevice, sl.stream));
+        RKR_HIP(c, hipMemcpyAsync(qi, hqi.data(), sizeof(QueryInfo) * nq, hipMemcpyHostToDevice, sl.stream));
+        RKR_HIP(c, hipEventRecord(sl.ev[1], sl.stream));
         // <u, q> for every (query, user): qip[b * U + u]
         if (nq <= 16 && (size_t)nq * d * sizeof(float) <= 65536)
-            RKR_HIP(c, launch_gemv_panels(c->stream, c->usersT, U, ld, d, q_in, nq, qip));
-        else RKR_HIP(c, launch_scores_store(c->stream, c->users, nullptr, U, ld, q_in, nq, ld, d, qip, U, false));
-        RKR_HIP(c, hipEventRecord(c->ev[2], c->stream));
+            RKR_HIP(c, launch_gemv_panels(sl.stream, c->usersT, U, ld, d, q_in, nq, qip));
+        else RKR_HIP(c, launch_scores_store(sl.stream, c->users, nullptr, U, ld, q_in, nq, ld, d, qip, U, false));
+        RKR_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
         int32_t *lbv = reinterpret_cast<int32_t *>(bk);
         int32_t *ubv = lbv + (size_t)nq * U;
         if (upd)
-            RKR_HIP(c, launch_bounds_upd(c->stream, qip, c->table, U, S, c->ranks, c->n_item_base, eps, c->side_ins,
+            RKR_HIP(c, launch_bounds_upd(sl.stream, qip, c->table, U, S, c->ranks, c->n_item_base, eps, c->side_ins,
                                          c->n_ins, c->side_del, c->n_del, nq, lbv, ubv));
         else
-            RKR_HIP(c, launch_bounds(c->stream, qip, c->table, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
-        RKR_HIP(c, hipEventRecord(c->ev[3], c->stream));
-        if (upd) RKR_HIP(c, launch_kth_values(c->stream, lbv, ubv, U, nq, qi));
-        else RKR_HIP(c, launch_hist_kth(c->stream, bk, U, S, c->ranks, I, nq, mode, qi));
-        RKR_HIP(c, launch_offsets(c->stream, qi, nq, refine_total));
-        RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, c->stream));
-        RKR_HIP(c, hipStreamSynchronize(c->stream));
+            RKR_HIP(c, launch_bounds(sl.stream, qip, c->table, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
+        RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
+        if (upd) RKR_HIP(c, launch_kth_values(sl.stream, lbv, ubv, U, nq, qi));
+        else RKR_HIP(c, launch_hist_kth(sl.stream, bk, U, S, c->ranks, I, nq, mode, qi));
+        RKR_HIP(c, launch_offsets(sl.stream, qi, nq, refine_total));
+        RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
+        RKR_HIP(c, hipStreamSynchronize(sl.stream));
         const int64_t total = hqi[nq - 1].cand_off + hqi[nq - 1].n_settled + hqi[nq - 1].n_cand;
         int64_t total_cand = 0;
         for (int32_t i = 0; i < nq; ++i) total_cand += hqi[i].n_cand;
         if (total >= INT32_MAX) return set_err(c, RKR_ERR_NOMEM, "query: candidate count exceeds 2^31; lower the query batch");
         const size_t cap = (size_t)std::max<int64_t>(total, 1);
         const size_t rcap = (size_t)std::max<int64_t>(total_cand, 1);
-        if ((rc = ensure(c, c->cand, sizeof(int32_t) * 2 * cap, "candidates"))) return rc;
-        if ((rc = ensure(c, c->refine_idx, (sizeof(int32_t) * 3 + sizeof(float)) * rcap, "refine rows"))) return rc;
-        if ((rc = ensure(c, c->counts, sizeof(int32_t) * rcap, "refine counts"))) return rc;
-        if ((rc = ensure(c, c->small, sizeof(int32_t) * ((size_t)nqb + 8), "query counters"))) return rc;
-        int32_t *cand_user = reinterpret_cast<int32_t *>(c->cand.p);
+        if ((rc = ensure(c, sl.cand, sizeof(int32_t) * 2 * cap, "candidates"))) return rc;
+        if ((rc = ensure(c, sl.refine_idx, (sizeof(int32_t) * 3 + sizeof(float)) * rcap, "refine rows"))) return rc;
+        if ((rc = ensure(c, sl.counts, sizeof(int32_t) * rcap, "refine counts"))) return rc;
+        if ((rc = ensure(c, sl.small, sizeof(int32_t) * ((size_t)nqb + 8), "query counters"))) return rc;
+        int32_t *cand_user = reinterpret_cast<int32_t *>(sl.cand.p);
         int32_t *cand_rank = cand_user + cap;
-        int32_t *refine_cand = reinterpret_cast<int32_t *>(c->refine_idx.p);
+        int32_t *refine_cand = reinterpret_cast<int32_t *>(sl.refine_idx.p);
         int32_t *refine_user = refine_cand + rcap;
         int32_t *refine_query = refine_user + rcap;
         float *refine_thr = reinterpret_cast<float *>(refine_query + rcap);
-        int32_t *cnt = reinterpret_cast<int32_t *>(c->counts.p);
-        int32_t *q_recheck = reinterpret_cast<int32_t *>(c->small.p);  // per query
+        int32_t *cnt = reinterpret_cast<int32_t *>(sl.counts.p);
+        int32_t *q_recheck = reinterpret_cast<int32_t *>(sl.small.p);  // per query
         int32_t *recheck_n = q_recheck + nqb;
         uint32_t *max_err_bits = reinterpret_cast<uint32_t *>(recheck_n + 1);
         const bool screened = c->form == RKR_FORM_SCREENED;
 
         if (upd)
-            RKR_HIP(c, launch_compact_values(c->stream, lbv, ubv, qip, U, nq, qi, cand_user, cand_rank, refine_total,
+            RKR_HIP(c, launch_compact_values(sl.stream, lbv, ubv, qip, U, nq, qi, cand_user, cand_rank, refine_total,
                                              refine_cand, refine_user, refine_query, refine_thr));
         else
-            RKR_HIP(c, launch_compact(c->stream, bk, qip, U, S, c->ranks, I, nq, mode, qi, cand_user, cand_rank,
+            RKR_HIP(c, launch_compact(sl.stream, bk, qip, U, S, c->ranks, I, nq, mode, qi, cand_user, cand_rank,
                                       refine_total, refine_cand, refine_user, refine_query, refine_thr));
         int32_t h_refine_total = 0, h_recheck_n = 0;
         uint32_t h_err_bits = 0;
@@ -620,13 +673,13 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
         for (int attempt = 0; attempt < 3; ++attempt) {
             int64_t rec_cap = 0;
             if (screened) {
-                const int64_t want = std::max<int64_t>(1 << 20, (int64_t)c->recheck.bytes / 16);
-                if ((rc = ensure(c, c->recheck, 16 * (size_t)want, "recheck list"))) return rc;
-                rec_cap = (int64_t)(c->recheck.bytes / 16);
-                RKR_HIP(c, hipMemsetAsync(c->small.p, 0, sizeof(int32_t) * ((size_t)nqb + 8), c->stream));
+                const int64_t want = std::max<int64_t>(1 << 20, (int64_t)sl.recheck.bytes / 16);
+                if ((rc = ensure(c, sl.recheck, 16 * (size_t)want, "recheck list"))) return rc;
+                rec_cap = (int64_t)(sl.recheck.bytes / 16);
+                RKR_HIP(c, hipMemsetAsync(sl.small.p, 0, sizeof(int32_t) * ((size_t)nqb + 8), sl.stream));
             }
-            RKR_HIP(c, hipMemsetAsync(cnt, 0, sizeof(int32_t) * rcap, c->stream));
-            RKR_HIP(c, hipEventRecord(c->ev[4], c->stream));
+            RKR_HIP(c, hipMemsetAsync(cnt, 0, sizeof(int32_t) * rcap, sl.stream));
+            RKR_HIP(c, hipEventRecord(sl.ev[4], sl.stream));
             // events 4..8 bracket exactly the refinement (or screening) kernel
             if (screened) {
                 ScreenLaunch L{};
@@ -635,41 +688,41 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
                 L.n_item = I; L.ld = ld; L.K = d; L.Kp = c->kp;
                 L.rows_user = refine_user; L.rows_thr = refine_thr; L.n_rows_dev = refine_total;
                 L.max_rows = (int64_t)rcap; L.C = c->screen_C; L.abs_floor = c->screen_abs;
-                L.cnt = cnt; L.recheck_n = recheck_n; L.recheck = c->recheck.p; L.recheck_cap = rec_cap;
+                L.cnt = cnt; L.recheck_n = recheck_n; L.recheck = sl.recheck.p; L.recheck_cap = rec_cap;
                 L.target_units = (int64_t)c->n_cu * 8;
-                RKR_HIP(c, launch_refine_screen(c->stream, L));
-                RKR_HIP(c, hipEventRecord(c->ev[8], c->stream));
-                RKR_HIP(c, launch_recheck(c->stream, c->users, c->items, ld, d, refine_user, refine_thr, refine_query,
-                                          recheck_n, c->recheck.p, rec_cap, cnt, q_recheck, max_err_bits));
+                RKR_HIP(c, launch_refine_screen(sl.stream, L));
+                RKR_HIP(c, hipEventRecord(sl.ev[8], sl.stream));
+                RKR_HIP(c, launch_recheck(sl.stream, c->users, c->items, ld, d, refine_user, refine_thr, refine_query,
+                                          recheck_n, sl.recheck.p, rec_cap, cnt, q_recheck, max_err_bits));
             } else {
-                RKR_HIP(c, launch_refine_count(c->stream, c->users, c->items, I, ld, d, refine_user, refine_thr,
+                RKR_HIP(c, launch_refine_count(sl.stream, c->users, c->items, I, ld, d, refine_user, refine_thr,
                                                refine_total, (int64_t)rcap, cnt));
-                RKR_HIP(c, hipEventRecord(c->ev[8], c->stream));
+                RKR_HIP(c, hipEventRecord(sl.ev[8], sl.stream));
             }
-            RKR_HIP(c, hipEventRecord(c->ev[5], c->stream));
-            RKR_HIP(c, launch_scatter_counts(c->stream, refine_total, refine_cand, cnt, (int64_t)rcap, cand_rank));
-            RKR_HIP(c, launch_topk(c->stream, qi, nq, cand_user, cand_rank, qip, U, max_k, o_user, o_rank, o_qip,
+            RKR_HIP(c, hipEventRecord(sl.ev[5], sl.stream));
+            RKR_HIP(c, launch_scatter_counts(sl.stream, refine_total, refine_cand, cnt, (int64_t)rcap, cand_rank));
+            RKR_HIP(c, launch_topk(sl.stream, qi, nq, cand_user, cand_rank, qip, U, max_k, o_user, o_rank, o_qip,
                                    nullptr));
-            RKR_HIP(c, hipEventRecord(c->ev[6], c->stream));
-            RKR_HIP(c, hipMemcpyAsync(ho_user.data(), o_user, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));
-            RKR_HIP(c, hipMemcpyAsync(ho_rank.data(), o_rank, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));
-            RKR_HIP(c, hipMemcpyAsync(ho_qip.data(), o_qip, sizeof(float) * (size_t)nq * max_k, hipMemcpyDeviceToHost, c->stream));
-            RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, c->stream));
-            RKR_HIP(c, hipMemcpyAsync(&h_refine_total, refine_total, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+            RKR_HIP(c, hipEventRecord(sl.ev[6], sl.stream));
+            RKR_HIP(c, hipMemcpyAsync(ho_user.data(), o_user, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, sl.stream));
+            RKR_HIP(c, hipMemcpyAsync(ho_rank.data(), o_rank, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, sl.stream));
+            RKR_HIP(c, hipMemcpyAsync(ho_qip.data(), o_qip, sizeof(float) * (size_t)nq * max_k, hipMemcpyDeviceToHost, sl.stream));
+            RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
+            RKR_HIP(c, hipMemcpyAsync(&h_refine_total, refine_total, sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream));
             if (screened) {
-                RKR_HIP(c, hipMemcpyAsync(&h_recheck_n, recheck_n, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-                RKR_HIP(c, hipMemcpyAsync(&h_err_bits, max_err_bits, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-                RKR_HIP(c, hipMemcpyAsync(h_qrecheck.data(), q_recheck, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, c->stream));
+                RKR_HIP(c, hipMemcpyAsync(&h_recheck_n, recheck_n, sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream));
+                RKR_HIP(c, hipMemcpyAsync(&h_err_bits, max_err_bits, sizeof(uint32_t), hipMemcpyDeviceToHost, sl.stream));
+                RKR_HIP(c, hipMemcpyAsync(h_qrecheck.data(), q_recheck, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, sl.stream));
             }
-            RKR_HIP(c, hipEventRecord(c->ev[7], c->stream));
-            RKR_HIP(c, hipStreamSynchronize(c->stream));
+            RKR_HIP(c, hipEventRecord(sl.ev[7], sl.stream));
+            RKR_HIP(c, hipStreamSynchronize(sl.stream));
             if (!screened || (int64_t)h_recheck_n <= rec_cap) break;
             // the list overflowed: grow it and run the refinement again
             const size_t need = 16 * ((size_t)h_recheck_n + (size_t)h_recheck_n / 4 + 1024);
-            dev_free(c, c->recheck.p, c->recheck.bytes);
-            c->recheck.p = nullptr;
-            c->recheck.bytes = 0;
-            if ((rc = ensure(c, c->recheck, need, "recheck list"))) return rc;
+            dev_free(c, sl.recheck.p, sl.recheck.bytes);
+            sl.recheck.p = nullptr;
+            sl.recheck.bytes = 0;
+            if ((rc = ensure(c, sl.recheck, need, "recheck list"))) return rc;
             if (attempt == 2) return set_err(c, RKR_ERR_NOMEM, "query: recheck list kept overflowing");
         }
 
@@ -697,14 +750,14 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
                 }
             }
         }
-        tm.h2d_ms += ev_ms(c, 0, 1);
-        tm.ip_ms += ev_ms(c, 1, 2);
-        tm.bound_ms += ev_ms(c, 2, 3);
-        tm.prune_ms += ev_ms(c, 3, 4);
-        tm.refine_ms += ev_ms(c, 4, 8);
-        tm.recheck_ms += ev_ms(c, 8, 5);
-        tm.topk_ms += ev_ms(c, 5, 6);
-        tm.d2h_ms += ev_ms(c, 6, 7);
+        tm.h2d_ms += ev_ms(sl.ev, 0, 1);
+        tm.ip_ms += ev_ms(sl.ev, 1, 2);
+        tm.bound_ms += ev_ms(sl.ev, 2, 3);
+        tm.prune_ms += ev_ms(sl.ev, 3, 4);
+        tm.refine_ms += ev_ms(sl.ev, 4, 8);
+        tm.recheck_ms += ev_ms(sl.ev, 8, 5);
+        tm.topk_ms += ev_ms(sl.ev, 5, 6);
+        tm.d2h_ms += ev_ms(sl.ev, 6, 7);
         tm.n_batches += 1;
         tm.n_refined += h_refine_total;
         if (screened) {
@@ -720,4 +773,4 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
     return RKR_OK;
 }
 
-}  // extern "C"
+}  // namespace rkr

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -27,8 +28,20 @@ struct DeviceBuf {
     size_t bytes = 0;
 };
 
+// One in-flight query batch: its stream, timing events and scratch buffers.
+// rkr_query uses the context's main slot; the serving loop gives each worker
+// its own slot so batches overlap on the device.
+struct Slot {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev[16] = {};
+    DeviceBuf q_in, qip, bk, qinfo, cand, refine_idx, counts, out, recheck, small;
+};
+
 struct Ctx {
     int device = 0;
+    std::mutex mem_mu;       // guards budget accounting (worker threads allocate)
+    std::mutex err_mu;
     size_t budget = 0;      // bytes the context may hold on the device
     size_t held = 0;        // bytes currently held
     hipStream_t stream = nullptr;
@@ -77,8 +90,9 @@ struct Ctx {
     float screen_C = 0.0f, screen_abs = 0.0f;
     int n_cu = 256;
 
-    // query scratch (grown on demand)
-    DeviceBuf q_in, qip, bk, qinfo, cand, refine_idx, counts, out, misc, recheck, small;
+    // query scratch of rkr_query (grown on demand) and build scratch
+    Slot main;
+    DeviceBuf misc;
 };
 
 // ---- error and memory helpers (rkr_api.cpp) --------------------------------
@@ -86,6 +100,13 @@ int set_err(Ctx *c, int code, const std::string &msg);
 int dev_alloc(Ctx *c, void **p, size_t bytes, const char *what);
 void dev_free(Ctx *c, void *p, size_t bytes);
 int ensure(Ctx *c, DeviceBuf &b, size_t bytes, const char *what);
+int slot_init(Ctx *c, Slot &sl, bool own_stream);
+void slot_free(Ctx *c, Slot &sl);
+// Runs validated queries through one slot (batches sized by free memory).
+int query_on_slot(Ctx *c, Slot &sl, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k,
+                  int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
+                  rkr_batch_timing *timing);
+int validate_queries(Ctx *c, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k);
 template <class T>
 int dev_alloc_t(Ctx *c, T **p, size_t n, const char *what) {
     return dev_alloc(c, reinterpret_cast<void **>(p), n * sizeof(T), what);

@@ -1,0 +1,336 @@
+// rkr_server.cpp -- the serving loop: continuous batching of reverse k-ranks
+// queries (mixed k) interleaved with user / item updates, on one context.
+//
+// Requests enter one FIFO.  Worker threads, each owning a Slot (its own HIP
+// stream and scratch buffers), take the longest run of queries at the front
+// of the queue (up to max_batch), so requests that arrive while a batch is on
+// the device are admitted by the next free worker and the batches overlap on
+// the GPU.  Batches are further split by free device memory inside
+// query_on_slot.  An update request is a barrier: it starts only when no
+// batch is in flight, and no batch starts while it runs, so every query sees
+// exactly the updates submitted before it.  Answers do not depend on how
+// queries are batched (every kernel decides each (query, user) pair on its
+// own), which the tests check by comparing answers alone and in the mix.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "rkr_internal.hpp"
+
+using namespace rkr;
+
+struct rkr_ctx : public rkr::Ctx {};
+
+namespace {
+
+enum Kind { Q_QUERY = 0, Q_INSERT_USERS = 1, Q_INSERT_ITEMS = 2, Q_DELETE_USERS = 3, Q_DELETE_ITEMS = 4 };
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    uint64_t id = 0;
+    int kind = Q_QUERY;
+    std::vector<float> data;
+    std::vector<int64_t> ids;
+    int64_t n = 0;
+    int32_t k = 0;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    int status = RKR_OK;
+    int32_t batch_size = 0;
+    bool done = false;
+    std::string err;
+    std::vector<int32_t> users, ranks;
+    std::vector<float> qips;
+    rkr_query_stats stats{};
+};
+
+}  // namespace
+
+struct rkr_server {
+    Ctx *c = nullptr;
+    rkr_server_config cfg{};
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> queue;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    uint64_t next_id = 1;
+    int busy = 0;            // query batches in flight
+    bool exclusive = false;  // an update is running
+    bool stopping = false;
+    int64_t n_batches = 0, n_queries = 0, n_updates = 0;
+
+    bool can_start() const {
+        if (exclusive || queue.empty()) return false;
+        return queue.front()->kind == Q_QUERY || busy == 0;
+    }
+
+    int apply_update(Request *r) {
+        switch (r->kind) {
+            case Q_INSERT_USERS: return rkr_insert_users(static_cast<rkr_ctx *>(c), r->data.data(), r->n);
+            case Q_INSERT_ITEMS: return rkr_insert_items(static_cast<rkr_ctx *>(c), r->data.data(), r->n);
+            case Q_DELETE_USERS: return rkr_delete_users(static_cast<rkr_ctx *>(c), r->ids.data(), r->n);
+            case Q_DELETE_ITEMS: return rkr_delete_items(static_cast<rkr_ctx *>(c), r->ids.data(), r->n);
+            default: return RKR_ERR_INVALID;
+        }
+    }
+
+    void run_batch(Slot &sl, std::vector<Request *> &all) {
+        // a request that fails validation is answered alone with its error,
+        // so it cannot fail the other queries of its batch
+        std::vector<Request *> batch;
+        for (Request *r : all) {
+            const int32_t k = r->k;
+            const int rc = validate_queries(c, r->data.data(), 1, &k, std::max(k, 1));
+            if (!rc) {
+                batch.push_back(r);
+                continue;
+            }
+            std::string err;
+            {
+                std::lock_guard<std::mutex> g(c->err_mu);
+                err = c->err;
+            }
+            std::lock_guard<std::mutex> g(mu);
+            r->status = rc;
+            r->err = err;
+            r->batch_size = 1;
+            r->t_done = now_s();
+            r->done = true;
+        }
+        if (batch.empty()) return;
+        const int32_t nq = (int32_t)batch.size();
+        const int32_t d = c->dim;
+        int32_t max_k = 1;
+        std::vector<float> q((size_t)nq * d);
+        std::vector<int32_t> ks((size_t)nq);
+        for (int32_t i = 0; i < nq; ++i) {
+            std::memcpy(&q[(size_t)i * d], batch[i]->data.data(), sizeof(float) * d);
+            ks[i] = batch[i]->k;
+            max_k = std::max(max_k, ks[i]);
+        }
+        std::vector<int32_t> ou((size_t)nq * max_k), orank((size_t)nq * max_k);
+        std::vector<float> oq((size_t)nq * max_k);
+        std::vector<rkr_query_stats> st((size_t)nq);
+        rkr_batch_timing tm{};
+        const int rc = query_on_slot(c, sl, q.data(), nq, ks.data(), max_k, ou.data(), orank.data(), oq.data(),
+                                     st.data(), &tm);
+        std::string err;
+        if (rc) {
+            std::lock_guard<std::mutex> g(c->err_mu);
+            err = c->err;
+        }
+        const double t = now_s();
+        std::lock_guard<std::mutex> g(mu);
+        for (int32_t i = 0; i < nq; ++i) {
+            Request *r = batch[i];
+            r->status = rc;
+            r->err = err;
+            r->batch_size = nq;
+            if (!rc) {
+                r->users.assign(ou.begin() + (size_t)i * max_k, ou.begin() + (size_t)i * max_k + r->k);
+                r->ranks.assign(orank.begin() + (size_t)i * max_k, orank.begin() + (size_t)i * max_k + r->k);
+                r->qips.assign(oq.begin() + (size_t)i * max_k, oq.begin() + (size_t)i * max_k + r->k);
+                r->stats = st[i];
+            }
+            r->t_done = t;
+            r->done = true;
+        }
+        ++n_batches;
+        n_queries += nq;
+    }
+
+    void worker(int w) {
+        (void)hipSetDevice(c->device);
+        Slot &sl = *slots[w];
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+            cv_work.wait(lk, [&] { return (stopping && queue.empty()) || can_start(); });
+            if (stopping && queue.empty()) break;
+            Request *front = queue.front();
+            if (front->kind != Q_QUERY) {  // barrier update
+                queue.pop_front();
+                exclusive = true;
+                front->t_start = now_s();
+                lk.unlock();
+                const int rc = apply_update(front);
+                std::string err;
+                if (rc) {
+                    std::lock_guard<std::mutex> g(c->err_mu);
+                    err = c->err;
+                }
+                lk.lock();
+                front->status = rc;
+                front->err = err;
+                front->t_done = now_s();
+                front->done = true;
+                exclusive = false;
+                ++n_updates;
+                cv_done.notify_all();
+                cv_work.notify_all();
+                continue;
+            }
+            std::vector<Request *> batch;
+            const double t = now_s();
+            while (!queue.empty() && queue.front()->kind == Q_QUERY && (int32_t)batch.size() < cfg.max_batch) {
+                queue.front()->t_start = t;
+                batch.push_back(queue.front());
+                queue.pop_front();
+            }
+            ++busy;
+            lk.unlock();
+            run_batch(sl, batch);
+            lk.lock();
+            --busy;
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+
+    int submit(std::unique_ptr<Request> r, uint64_t *ticket) {
+        std::lock_guard<std::mutex> g(mu);
+        if (stopping) return RKR_ERR_STATE;
+        r->id = next_id++;
+        r->t_submit = now_s();
+        *ticket = r->id;
+        queue.push_back(r.get());
+        reqs.emplace(r->id, std::move(r));
+        cv_work.notify_one();
+        return RKR_OK;
+    }
+};
+
+extern "C" {
+
+double rkr_now(void) { return now_s(); }
+
+int rkr_server_start(rkr_ctx *c, const rkr_server_config *cfg, rkr_server **out) {
+    if (!c || !out) return RKR_ERR_INVALID;
+    *out = nullptr;
+    if (!c->table) return set_err(c, RKR_ERR_STATE, "server: no table (build or load one first)");
+    auto *s = new (std::nothrow) rkr_server();
+    if (!s) return RKR_ERR_NOMEM;
+    s->c = c;
+    s->cfg.max_batch = cfg && cfg->max_batch > 0 ? cfg->max_batch : 256;
+    s->cfg.n_slots = cfg && cfg->n_slots > 0 ? std::min(cfg->n_slots, 4) : 2;
+    (void)hipSetDevice(c->device);
+    for (int i = 0; i < s->cfg.n_slots; ++i) {
+        s->slots.emplace_back(new Slot());
+        const int rc = slot_init(c, *s->slots.back(), true);
+        if (rc) {
+            for (auto &sl : s->slots) slot_free(c, *sl);
+            delete s;
+            return rc;
+        }
+    }
+    for (int i = 0; i < s->cfg.n_slots; ++i) s->workers.emplace_back(&rkr_server::worker, s, i);
+    *out = s;
+    return RKR_OK;
+}
+
+int rkr_server_submit_query(rkr_server *s, const float *q, int32_t k, uint64_t *ticket) {
+    if (!s || !q || !ticket || k <= 0) return RKR_ERR_INVALID;
+    auto r = std::make_unique<Request>();
+    r->kind = Q_QUERY;
+    r->data.assign(q, q + s->c->dim);
+    r->k = k;
+    return s->submit(std::move(r), ticket);
+}
+
+int rkr_server_submit_update(rkr_server *s, int kind, const float *vectors, const int64_t *ids, int64_t n,
+                             uint64_t *ticket) {
+    if (!s || !ticket || n < 0) return RKR_ERR_INVALID;
+    auto r = std::make_unique<Request>();
+    r->n = n;
+    switch (kind) {
+        case RKR_UPDATE_INSERT_USERS: r->kind = Q_INSERT_USERS; break;
+        case RKR_UPDATE_INSERT_ITEMS: r->kind = Q_INSERT_ITEMS; break;
+        case RKR_UPDATE_DELETE_USERS: r->kind = Q_DELETE_USERS; break;
+        case RKR_UPDATE_DELETE_ITEMS: r->kind = Q_DELETE_ITEMS; break;
+        default: return RKR_ERR_INVALID;
+    }
+    if (r->kind == Q_INSERT_USERS || r->kind == Q_INSERT_ITEMS) {
+        if (n > 0 && !vectors) return RKR_ERR_INVALID;
+        r->data.assign(vectors, vectors + (size_t)n * s->c->dim);
+    } else {
+        if (n > 0 && !ids) return RKR_ERR_INVALID;
+        r->ids.assign(ids, ids + n);
+    }
+    return s->submit(std::move(r), ticket);
+}
+
+int rkr_server_wait(rkr_server *s, uint64_t ticket, int32_t *out_user, int32_t *out_rank, float *out_qip,
+                    rkr_request_info *info) {
+    if (!s) return RKR_ERR_INVALID;
+    std::unique_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) return RKR_ERR_INVALID;
+        Request *p = it->second.get();
+        s->cv_done.wait(lk, [&] { return p->done; });
+        r = std::move(it->second);
+        s->reqs.erase(it);
+    }
+    if (!r->status && r->kind == Q_QUERY) {
+        if (out_user) std::memcpy(out_user, r->users.data(), sizeof(int32_t) * r->users.size());
+        if (out_rank) std::memcpy(out_rank, r->ranks.data(), sizeof(int32_t) * r->ranks.size());
+        if (out_qip) std::memcpy(out_qip, r->qips.data(), sizeof(float) * r->qips.size());
+    }
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->status = r->status;
+        info->batch_size = r->batch_size;
+        info->t_submit = r->t_submit;
+        info->t_start = r->t_start;
+        info->t_done = r->t_done;
+        info->stats = r->stats;
+        std::strncpy(info->error, r->err.c_str(), sizeof(info->error) - 1);
+    }
+    return r->status;
+}
+
+int rkr_server_poll(rkr_server *s, uint64_t ticket, int *done) {
+    if (!s || !done) return RKR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return RKR_ERR_INVALID;
+    *done = it->second->done ? 1 : 0;
+    return RKR_OK;
+}
+
+int rkr_server_counters(rkr_server *s, int64_t *n_batches, int64_t *n_queries, int64_t *n_updates) {
+    if (!s) return RKR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (n_batches) *n_batches = s->n_batches;
+    if (n_queries) *n_queries = s->n_queries;
+    if (n_updates) *n_updates = s->n_updates;
+    return RKR_OK;
+}
+
+int rkr_server_stop(rkr_server *s) {
+    if (!s) return RKR_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stopping = true;
+        s->cv_work.notify_all();
+    }
+    for (auto &t : s->workers) t.join();
+    for (auto &sl : s->slots) slot_free(s->c, *sl);
+    delete s;
+    return RKR_OK;
+}
+
+}  // extern "C"

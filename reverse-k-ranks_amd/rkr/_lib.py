@@ -41,8 +41,14 @@ EXPORTS = (
     "rkr_version", "rkr_normalize_users", "rkr_set_data", "rkr_uniform_sample_ranks", "rkr_build_table",
     "rkr_set_table", "rkr_get_table", "rkr_get_ranks", "rkr_save_table", "rkr_load_table", "rkr_query",
     "rkr_index_size_bytes", "rkr_insert_items", "rkr_delete_items", "rkr_insert_users", "rkr_delete_users",
-    "rkr_get_counts",
+    "rkr_get_counts", "rkr_now", "rkr_server_start", "rkr_server_submit_query", "rkr_server_submit_update",
+    "rkr_server_wait", "rkr_server_poll", "rkr_server_counters", "rkr_server_stop",
 )
+
+RKR_UPDATE_INSERT_USERS = 1
+RKR_UPDATE_INSERT_ITEMS = 2
+RKR_UPDATE_DELETE_USERS = 3
+RKR_UPDATE_DELETE_ITEMS = 4
 
 
 class QueryStats(C.Structure):
@@ -59,6 +65,15 @@ class BatchTiming(C.Structure):
                 ("h2d_ms", C.c_double), ("d2h_ms", C.c_double), ("n_batches", C.c_int64),
                 ("n_refined", C.c_int64), ("n_rechecked", C.c_int64), ("n_screened", C.c_int64),
                 ("max_screen_err", C.c_double)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("max_batch", C.c_int32), ("n_slots", C.c_int32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("batch_size", C.c_int32), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_done", C.c_double), ("stats", QueryStats), ("error", C.c_char * 256)]
 
 
 class RkrError(RuntimeError):
@@ -105,10 +120,18 @@ def load():
     L.rkr_insert_users.argtypes = [vp, f32p, i64]
     L.rkr_delete_users.argtypes = [vp, C.c_void_p, i64]
     L.rkr_get_counts.argtypes = [vp] + [C.POINTER(C.c_int64)] * 4
+    L.rkr_now.restype = C.c_double
+    L.rkr_server_start.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
+    L.rkr_server_submit_query.argtypes = [vp, f32p, i32, C.POINTER(C.c_uint64)]
+    L.rkr_server_submit_update.argtypes = [vp, C.c_int, f32p, C.c_void_p, i64, C.POINTER(C.c_uint64)]
+    L.rkr_server_wait.argtypes = [vp, C.c_uint64, i32p, i32p, f32p, C.POINTER(RequestInfo)]
+    L.rkr_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(C.c_int)]
+    L.rkr_server_counters.argtypes = [vp] + [C.POINTER(C.c_int64)] * 3
+    L.rkr_server_stop.argtypes = [vp]
     L.rkr_index_size_bytes.argtypes = [vp]
     L.rkr_index_size_bytes.restype = C.c_int64
     for name in EXPORTS:
-        if name not in ("rkr_destroy", "rkr_last_error", "rkr_version", "rkr_index_size_bytes"):
+        if name not in ("rkr_destroy", "rkr_last_error", "rkr_version", "rkr_index_size_bytes", "rkr_now"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
