@@ -36,6 +36,8 @@ WORKLOADS = {
     "plain_gpu": ("plain_gpu", 50, "plain_gpu: 100k users x 20k items x d=150, S=512, 1000 queries, k=50"),
     "precision": ("precision", 50, "precision: 1M users x 50k items x d=150, S=512, 1000 queries, k=50"),
     "batching": ("batching", None, "batching: 1M users x 50k items x d=150, S=512, 10000 Zipf(1.1) queries, k in [10,200]"),
+    "serving": ("serving", None, "serving: 2M users x 100k items x d=128, S=512, Poisson arrivals, k in [10,200], "
+                                 "+1% users and +0.5% items every 1 s"),
 }
 
 
@@ -51,7 +53,103 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=None, help="queries in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--duration", type=float, default=10.0, help="serving: open-loop seconds")
+    ap.add_argument("--qps", type=float, default=0.0, help="serving: arrival rate (0 = half the saturating rate)")
+    ap.add_argument("--n-user", type=int, default=None)
+    ap.add_argument("--n-item", type=int, default=None)
     return ap.parse_args()
+
+
+def serving(args, idx, wl, label):
+    """Open-loop serving: Poisson query arrivals (k uniform in [10, 200]) with an
+    insert batch of 1% new users and 0.5% new items every second; latency is
+    admission -> answer, measured by the library's monotonic clock."""
+    import threading
+    import rkr
+    rng = np.random.default_rng(args.seed + 7)
+    U0, I0, d = wl.users.shape[0], wl.items.shape[0], wl.users.shape[1]
+    qpool = wl.items
+    srv = rkr.Server(idx, max_batch=256, n_slots=2)
+    # saturating rate: a closed burst of 2048 queries
+    n_burst = 2048
+    ks = rng.integers(10, 201, n_burst)
+    t0 = time.perf_counter()
+    tk = [srv.submit_query(qpool[rng.integers(0, I0)], int(k)) for k in ks]
+    for t, k in zip(tk, ks):
+        srv.wait(t, int(k))
+    sat = n_burst / (time.perf_counter() - t0)
+    rate = args.qps if args.qps > 0 else sat / 2
+    # open loop
+    n_arr = int(rate * args.duration)
+    gaps = rng.exponential(1.0 / rate, n_arr)
+    arrivals = np.cumsum(gaps)
+    kq = rng.integers(10, 201, n_arr)
+    qi = rng.integers(0, I0, n_arr)
+    n_new_u = max(1, U0 // 100)
+    n_new_i = max(1, I0 // 200)
+    tickets = []
+    upd = []
+    start = time.perf_counter()
+
+    def submitter():
+        next_upd = 1.0
+        for j in range(n_arr):
+            while True:
+                el = time.perf_counter() - start
+                if el >= next_upd:
+                    nu = (rng.standard_normal((n_new_u, d), dtype=np.float32) / np.float32(np.sqrt(d)))
+                    ni = (rng.standard_normal((n_new_i, d), dtype=np.float32) / np.float32(np.sqrt(d)))
+                    upd.append(srv.submit_update("insert_users", vectors=nu))
+                    upd.append(srv.submit_update("insert_items", vectors=ni))
+                    next_upd += 1.0
+                    continue
+                if el >= arrivals[j]:
+                    break
+                time.sleep(min(arrivals[j] - el, 0.0005))
+            tickets.append((srv.submit_query(qpool[qi[j]], int(kq[j])), int(kq[j])))
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat, bsz, done = [], [], 0
+    while True:
+        if done < len(tickets):
+            t, k = tickets[done]
+            _, _, _, info = srv.wait(t, k)
+            lat.append((info["t_done"] - info["t_submit"]) * 1e3)
+            bsz.append(info["batch_size"])
+            done += 1
+        elif not th.is_alive():
+            break
+        else:
+            time.sleep(0.0002)
+    th.join()
+    while done < len(tickets):
+        t, k = tickets[done]
+        _, _, _, info = srv.wait(t, k)
+        lat.append((info["t_done"] - info["t_submit"]) * 1e3)
+        bsz.append(info["batch_size"])
+        done += 1
+    ulat = []
+    for t in upd:
+        _, _, _, info = srv.wait(t)
+        ulat.append((info["t_done"] - info["t_start"]) * 1e3)
+    wall = time.perf_counter() - start
+    cnt = srv.counters()
+    srv.stop()
+    lat = np.array(lat)
+    out = {
+        "metric": "serving queries/s with p50/p99 latency (exact ranks), inserts every 1 s",
+        "value": round(len(lat) / wall, 2), "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic (seeded), open-loop Poisson arrivals",
+        "config": {"workload": label, "form": args.form, "arrival_rate": round(rate, 1),
+                   "saturating_rate": round(sat, 1), "duration_s": args.duration},
+        "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_batch": round(float(np.mean(bsz)), 1), "queries": len(lat), "batches": cnt["n_batches"],
+        "update_batches": len(ulat), "update_ms_p50": round(float(np.percentile(ulat, 50)), 2) if ulat else None,
+        "update_ms_max": round(float(np.max(ulat)), 2) if ulat else None,
+        "final_counts": idx.counts(),
+    }
+    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_sample_q):
@@ -82,7 +180,7 @@ def main():
     from rkr import synth
 
     cfg, k_fixed, label = WORKLOADS[args.workload]
-    wl = synth.make(cfg, seed=args.seed, n_query=args.n_query)
+    wl = synth.make(cfg, seed=args.seed, n_query=args.n_query, n_user=args.n_user, n_item=args.n_item)
     ks = wl.topk if k_fixed is None else np.full(len(wl.queries), k_fixed, np.int32)
     U, I, d = wl.users.shape[0], wl.items.shape[0], wl.users.shape[1]
     nq = len(wl.queries)
@@ -94,6 +192,11 @@ def main():
     t0 = time.perf_counter()
     build_ms = idx.build_table(ranks)
     build_wall = time.perf_counter() - t0
+    if args.workload == "serving":
+        print(json.dumps({"table_build_ms": round(build_ms, 2), "n_user": U, "n_item": I}), file=sys.stderr, flush=True)
+        serving(args, idx, wl, label)
+        idx.close()
+        return
 
     max_k = int(ks.max())
     for _ in range(args.warmup):

@@ -44,7 +44,7 @@ struct SelSmem {
     uint16_t coop[SMAX];          // fb indices needing wg_select in this group
     float red_f[2 * NW];
     uint32_t red_u[NW + 1];
-    int32_t nfb, ngrp, ncoop, sel_b;
+    int32_t nfb, ngrp, ncoop, sel_b, one_pass;
     float gbuf[GCAP];
 };
 
@@ -195,27 +195,68 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
     for (int j = tid; j < S; j += NT) sm.tb[j] = (uint16_t)find_bin(sm.incl, NB, (uint32_t)ranks[j]);
     __syncthreads();
 
-    // 3. distinct target bins, greedily packed into passes of <= GCAP floats
-    if (tid == 0) {
-        int nfb = 0, ngrp = 0;
-        uint32_t fill = 0;
-        for (int j = 0; j < S; ++j) {
-            if (j > 0 && sm.tb[j] == sm.tb[j - 1]) continue;
-            const int b = sm.tb[j];
-            const uint32_t h = sm.incl[b] - sm.incl[b + 1];
-            sm.fb[nfb] = (uint16_t)j;
-            if (h > GCAP) {
-                sm.grp[ngrp++] = (uint16_t)(nfb | DEEP);
-                fill = GCAP;  // next bin starts a new group
-            } else {
-                if (ngrp == 0 || fill + h > GCAP) { sm.grp[ngrp++] = (uint16_t)nfb; fill = 0; }
-                fill += h;
+    // 3. distinct target bins (parallel scans).  fb[] lists the first target of
+    // each distinct bin; seg[] its gather offset when all bins fit one pass.
+    {
+        constexpr int PER = SMAX / NT;  // targets per thread (S <= SMAX)
+        uint32_t nf = 0, wsum = 0;
+        uint32_t fl[PER], wt[PER];
+        for (int i = 0; i < PER; ++i) {
+            const int j = tid * PER + i;
+            fl[i] = wt[i] = 0;
+            if (j < S && (j == 0 || sm.tb[j] != sm.tb[j - 1])) {
+                const int b = sm.tb[j];
+                fl[i] = 1;
+                wt[i] = sm.incl[b] - sm.incl[b + 1];
             }
-            ++nfb;
+            nf += fl[i];
+            wsum += wt[i];
         }
-        sm.grp[ngrp] = (uint16_t)nfb;
-        sm.nfb = nfb;
-        sm.ngrp = ngrp;
+        uint32_t nf_tot, w_tot;
+        uint32_t fpos = wg_excl_scan(sm, nf, &nf_tot);
+        uint32_t wpos = wg_excl_scan(sm, wsum, &w_tot);
+        for (int i = 0; i < PER; ++i) {
+            if (!fl[i]) continue;
+            const int j = tid * PER + i;
+            sm.fb[fpos] = (uint16_t)j;
+            sm.cur[sm.tb[j]] = wpos;  // used only by the one-pass case below
+            ++fpos;
+            wpos += wt[i];
+        }
+        if (tid == 0) {
+            sm.nfb = (int)nf_tot;
+            sm.one_pass = w_tot <= GCAP;
+        }
+    }
+    __syncthreads();
+    if (!sm.one_pass) {
+        // rows too wide for one pass: clear the offsets and pack bins greedily
+        // into passes of <= GCAP floats (a bin larger than GCAP is narrowed
+        // from global memory on its own)
+        for (int f = tid; f < sm.nfb; f += NT) sm.cur[sm.tb[sm.fb[f]]] = NONE;
+        __syncthreads();
+        if (tid == 0) {
+            const int nfb = sm.nfb;
+            int ngrp = 0;
+            uint32_t fill = 0;
+            for (int f = 0; f < nfb; ++f) {
+                const int b = sm.tb[sm.fb[f]];
+                const uint32_t h = sm.incl[b] - sm.incl[b + 1];
+                if (h > GCAP) {
+                    sm.grp[ngrp++] = (uint16_t)(f | DEEP);
+                    fill = GCAP;  // next bin starts a new group
+                } else {
+                    if (ngrp == 0 || fill + h > GCAP) { sm.grp[ngrp++] = (uint16_t)f; fill = 0; }
+                    fill += h;
+                }
+            }
+            sm.grp[ngrp] = (uint16_t)nfb;
+            sm.ngrp = ngrp;
+        }
+    } else if (tid == 0) {
+        sm.grp[0] = 0;
+        sm.grp[1] = (uint16_t)sm.nfb;
+        sm.ngrp = 1;
     }
     __syncthreads();
     const int nfb = sm.nfb, ngrp = sm.ngrp;
@@ -242,13 +283,15 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
             }
             continue;
         }
-        // segment offsets of the member bins
+        // segment offsets of the member bins (already set in the one-pass case)
         if (tid == 0) {
-            uint32_t off = 0;
-            for (int f = f0; f < f1; ++f) {
-                const int b = sm.tb[sm.fb[f]];
-                sm.cur[b] = off;
-                off += sm.incl[b] - sm.incl[b + 1];
+            if (!sm.one_pass) {
+                uint32_t off = 0;
+                for (int f = f0; f < f1; ++f) {
+                    const int b = sm.tb[sm.fb[f]];
+                    sm.cur[b] = off;
+                    off += sm.incl[b] - sm.incl[b + 1];
+                }
             }
             sm.ncoop = 0;
         }
@@ -304,8 +347,7 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
         }
         // release the member bins
         __syncthreads();
-        if (tid == 0)
-            for (int f = f0; f < f1; ++f) sm.cur[sm.tb[sm.fb[f]]] = NONE;
+        for (int f = f0 + tid; f < f1; f += NT) sm.cur[sm.tb[sm.fb[f]]] = NONE;
         __syncthreads();
     }
 }
