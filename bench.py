@@ -57,6 +57,10 @@ def parse():
     ap.add_argument("--qps", type=float, default=0.0, help="serving: arrival rate (0 = half the saturating rate)")
     ap.add_argument("--n-user", type=int, default=None)
     ap.add_argument("--n-item", type=int, default=None)
+    ap.add_argument("--pre-updates", type=int, default=0,
+                    help="insert rounds (+1%% users, +0.5%% items each) applied before timing")
+    ap.add_argument("--query-batch", type=int, default=0)
+    ap.add_argument("--no-dedup", action="store_true", help="compute byte-equal queries of a call separately")
     return ap.parse_args()
 
 
@@ -192,6 +196,15 @@ def main():
     t0 = time.perf_counter()
     build_ms = idx.build_table(ranks)
     build_wall = time.perf_counter() - t0
+    if args.query_batch:
+        idx.set_option(3, args.query_batch)
+    idx.set_option(6, 0 if args.no_dedup else 1)  # RKR_OPT_DEDUP
+    if args.pre_updates:
+        rng_u = np.random.default_rng(args.seed + 11)
+        for _ in range(args.pre_updates):
+            idx.insert_users(rng_u.standard_normal((max(1, U // 100), d), dtype=np.float32) / np.float32(np.sqrt(d)))
+            idx.insert_items(rng_u.standard_normal((max(1, I // 200), d), dtype=np.float32) / np.float32(np.sqrt(d)))
+        U, I = idx.n_user, idx.n_item
     if args.workload == "serving":
         print(json.dumps({"table_build_ms": round(build_ms, 2), "n_user": U, "n_item": I}), file=sys.stderr, flush=True)
         serving(args, idx, wl, label)
@@ -288,8 +301,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded), inputs resident in HBM",
-        "config": {"workload": label, "form": args.form, "queries_per_step": nq, "n_user": U, "n_item": I,
-                   "dim": d, "n_sample": wl.n_sample},
+        "config": {"workload": label, "form": args.form, "queries_per_step": nq,
+                   "distinct_queries_per_step": int(len(np.unique(wl.queries, axis=0))),
+                   "dedup": not args.no_dedup, "n_user": U, "n_item": I, "dim": d, "n_sample": wl.n_sample},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "latency_note": "single-query calls (batch of 1), host wall time incl. PCIe copies",

@@ -55,6 +55,9 @@ typedef struct rkr_ctx rkr_ctx;
 #define RKR_OPT_OUTPUT 2
 #define RKR_OPT_QUERY_BATCH 3   /* max queries per device batch (0 = auto)      */
 #define RKR_OPT_BUILD_BATCH 4   /* max users per build batch (0 = auto)         */
+#define RKR_OPT_SCREEN_CHUNK 5  /* 64-item stages per screened work unit (0 = 32) */
+#define RKR_OPT_DEDUP 6         /* 1 (default): byte-equal queries of one rkr_query
+                                   call are answered once, with their largest k   */
 
 /* Per-query counters; mirrors SingleQueryPerformance
  * (src/include/util/FileIO.hpp:150-173) plus the screening figures. */
@@ -154,7 +157,8 @@ int rkr_load_table(rkr_ctx *ctx, const char *path, float eps);
  * FP32, caller-owned).  topk[i] is query i's k (1 <= k <= min(n_user, max_k)).
  * Outputs are n_query x max_k, row-major; entries past topk[i] are -1.
  * out_qip receives <u,q> of each returned user (may be NULL); stats
- * (n_query entries) and timing may be NULL.  Replaces US::Index::Retrieval
+ * (n_query entries) and timing may be NULL.  With RKR_OPT_DEDUP on, byte-equal
+ * query vectors are computed once and share that run's stats.  Replaces US::Index::Retrieval
  * (src/impl/US.hpp:96-213), which BaseIndex::Retrieval declares
  * (src/include/struct/MethodBase.hpp:14-30). */
 int rkr_query(rkr_ctx *ctx, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k,

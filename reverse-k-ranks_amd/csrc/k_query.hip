@@ -136,20 +136,48 @@ __device__ uint32_t wg_excl_scan(uint32_t v, uint32_t *red, uint32_t *total) {
     return r;
 }
 
-// One workgroup per query: bucket histograms, k-th bound, class counts.
-__global__ void __launch_bounds__(NT) k_hist_kth(const uint32_t *__restrict__ bk, int64_t n_user, int32_t S,
+// grid (chunks, queries): bucket histograms of the chunk's users, added into
+// the query's global histograms ghist[b] = [h_lo | h_hi], (S + 1) bins each.
+// same != 0: b_lo == b_hi for every user (exact output, eps = 0), so only
+// h_lo is built and the selection reads it for both.
+__global__ void __launch_bounds__(NT) k_hist_chunks(const uint32_t *__restrict__ bk, int64_t n_user, int32_t S,
+                                                    int same, int64_t cu, uint32_t *__restrict__ ghist) {
+    __shared__ uint32_t h_lo[SMAXQ + 1];
+    __shared__ uint32_t h_hi[SMAXQ + 1];
+    const int64_t b = blockIdx.y;
+    const int64_t u0 = (int64_t)blockIdx.x * cu, u1 = min(n_user, u0 + cu);
+    for (int i = threadIdx.x; i <= S; i += NT) { h_lo[i] = 0; h_hi[i] = 0; }
+    __syncthreads();
+    const uint32_t *v = bk + b * n_user;
+    if (same) {
+        for (int64_t u = u0 + threadIdx.x; u < u1; u += NT) atomicAdd(&h_lo[v[u] & 0xFFFFu], 1u);
+    } else {
+        for (int64_t u = u0 + threadIdx.x; u < u1; u += NT) {
+            const uint32_t x = v[u];
+            atomicAdd(&h_lo[x & 0xFFFFu], 1u);
+            atomicAdd(&h_hi[x >> 16], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t *g = ghist + b * 2 * (int64_t)(S + 1);
+    for (int i = threadIdx.x; i <= S; i += NT) {
+        if (h_lo[i]) atomicAdd(&g[i], h_lo[i]);
+        if (!same && h_hi[i]) atomicAdd(&g[S + 1 + i], h_hi[i]);
+    }
+}
+
+// One workgroup per query: k-th bound and class counts from the histograms.
+__global__ void __launch_bounds__(NT) k_hist_kth(const uint32_t *__restrict__ ghist, int32_t S, int same,
                                                  const int32_t *__restrict__ ranks, int64_t n_item, int output_mode,
                                                  QueryInfo *__restrict__ qi) {
     __shared__ uint32_t h_lo[SMAXQ + 1];
     __shared__ uint32_t h_hi[SMAXQ + 1];
     const int64_t b = blockIdx.x;
-    for (int i = threadIdx.x; i <= S; i += NT) { h_lo[i] = 0; h_hi[i] = 0; }
-    __syncthreads();
-    const uint32_t *v = bk + b * n_user;
-    for (int64_t u = threadIdx.x; u < n_user; u += NT) {
-        const uint32_t x = v[u];
-        atomicAdd(&h_lo[x & 0xFFFFu], 1u);
-        atomicAdd(&h_hi[x >> 16], 1u);
+    const uint32_t *g = ghist + b * 2 * (int64_t)(S + 1);
+    for (int i = threadIdx.x; i <= S; i += NT) {
+        const uint32_t lo = g[i];
+        h_lo[i] = lo;
+        h_hi[i] = same ? lo : g[S + 1 + i];
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
@@ -162,6 +190,8 @@ __global__ void __launch_bounds__(NT) k_hist_kth(const uint32_t *__restrict__ bk
         if (cum >= k) { beta = j; break; }
     }
     q.beta = beta;
+    q.n_refine = 0;
+    q.fill = 0;
     if (output_mode == RKR_OUTPUT_EXACT) {
         int64_t nc = 0;
         for (int32_t j = 0; j <= beta; ++j) nc += h_hi[j];
@@ -291,6 +321,68 @@ __global__ void __launch_bounds__(NT) k_compact(const uint32_t *__restrict__ bk,
     if (my_refine) atomicAdd(&n_refine, my_refine);
     __syncthreads();
     if (threadIdx.x == 0) qi[b].n_refine = n_refine;
+}
+
+// Exact output, grid (chunks, queries): the candidates b_hi <= beta of the
+// chunk go to ranges reserved with one atomic per 1024 users on the query's
+// cursor, so the slice is in no fixed order (the top-k selection orders by
+// (rank, userID)); refine rows likewise.
+__global__ void __launch_bounds__(NT) k_compact_chunks(const uint32_t *__restrict__ bk, const float *__restrict__ qip,
+                                                       int64_t n_user, int32_t S, const int32_t *__restrict__ ranks,
+                                                       int64_t n_item, int64_t cu, QueryInfo *__restrict__ qi,
+                                                       int32_t *__restrict__ cand_user, int32_t *__restrict__ cand_rank,
+                                                       int32_t *__restrict__ refine_total,
+                                                       int32_t *__restrict__ refine_cand,
+                                                       int32_t *__restrict__ refine_user,
+                                                       int32_t *__restrict__ refine_query,
+                                                       float *__restrict__ refine_thr) {
+    __shared__ uint32_t red[NW + 1];
+    __shared__ int32_t s_can, s_ref;
+    const int64_t b = blockIdx.y;
+    const int32_t beta = qi[b].beta;
+    const int64_t off = qi[b].cand_off;
+    const uint32_t *v = bk + b * n_user;
+    const float *qb = qip + b * n_user;
+    const int64_t u0 = (int64_t)blockIdx.x * cu, u1 = min(n_user, u0 + cu);
+    for (int64_t ub0 = u0; ub0 < u1; ub0 += NT) {
+        const int64_t u = ub0 + threadIdx.x;
+        bool cand = false;
+        int32_t rank = -1;
+        if (u < u1) {
+            const uint32_t x = v[u];
+            const int32_t blo = (int32_t)(x & 0xFFFFu), bhi = (int32_t)(x >> 16);
+            if (bhi <= beta) {
+                cand = true;
+                const int32_t lbs = blo == S ? (int32_t)n_item : ranks[blo];
+                const int32_t ubs = bhi == 0 ? 0 : ranks[bhi - 1] + 1;
+                if (lbs == ubs) rank = lbs;
+            }
+        }
+        const bool refine = cand && rank < 0;
+        uint32_t tot;
+        const uint32_t e = wg_excl_scan((cand ? 1u : 0u) | (refine ? 0x10000u : 0u), red, &tot);
+        if (tot == 0) continue;
+        if (threadIdx.x == 0) {
+            s_can = atomicAdd(&qi[b].fill, (int32_t)(tot & 0xFFFFu));
+            if (tot >> 16) {
+                s_ref = atomicAdd(refine_total, (int32_t)(tot >> 16));
+                atomicAdd(&qi[b].n_refine, (int32_t)(tot >> 16));
+            }
+        }
+        __syncthreads();
+        if (cand) {
+            const int64_t slot = off + s_can + (e & 0xFFFFu);
+            cand_user[slot] = (int32_t)u;
+            cand_rank[slot] = rank;
+            if (refine) {
+                const int32_t r = s_ref + (int32_t)(e >> 16);
+                refine_cand[r] = (int32_t)slot;
+                refine_user[r] = (int32_t)u;
+                refine_query[r] = (int32_t)b;
+                refine_thr[r] = qb[u];
+            }
+        }
+    }
 }
 
 __global__ void k_scatter_counts(const int32_t *__restrict__ refine_total, const int32_t *__restrict__ refine_cand,
@@ -429,9 +521,15 @@ hipError_t launch_bounds(hipStream_t s, const float *qip, const float *table, in
 }
 
 hipError_t launch_hist_kth(hipStream_t s, const uint32_t *bk, int64_t n_user, int32_t S, const int32_t *ranks,
-                           int64_t n_item, int32_t nq, int output_mode, QueryInfo *qi) {
+                           int64_t n_item, int32_t nq, int output_mode, int same, int64_t cu, uint32_t *ghist,
+                           QueryInfo *qi) {
     if (nq <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_hist_kth, dim3((unsigned)nq), dim3(NT), 0, s, bk, n_user, S, ranks, n_item, output_mode, qi);
+    hipError_t e = hipMemsetAsync(ghist, 0, sizeof(uint32_t) * 2 * (size_t)(S + 1) * nq, s);
+    if (e != hipSuccess) return e;
+    const unsigned n_chunk = (unsigned)((n_user + cu - 1) / cu);
+    if (n_user > 0)
+        hipLaunchKernelGGL(k_hist_chunks, dim3(n_chunk, (unsigned)nq), dim3(NT), 0, s, bk, n_user, S, same, cu, ghist);
+    hipLaunchKernelGGL(k_hist_kth, dim3((unsigned)nq), dim3(NT), 0, s, ghist, S, same, ranks, n_item, output_mode, qi);
     return hipGetLastError();
 }
 
@@ -441,13 +539,21 @@ hipError_t launch_offsets(hipStream_t s, QueryInfo *qi, int32_t nq, int32_t *ref
 }
 
 hipError_t launch_compact(hipStream_t s, const uint32_t *bk, const float *qip, int64_t n_user, int32_t S,
-                          const int32_t *ranks, int64_t n_item, int32_t nq, int output_mode, QueryInfo *qi,
-                          int32_t *cand_user, int32_t *cand_rank, int32_t *refine_total, int32_t *refine_cand,
-                          int32_t *refine_user, int32_t *refine_query, float *refine_thr) {
-    (void)output_mode;
-    if (nq <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_compact, dim3((unsigned)nq), dim3(NT), 0, s, bk, qip, n_user, S, ranks, n_item, qi, cand_user,
-                       cand_rank, refine_total, refine_cand, refine_user, refine_query, refine_thr);
+                          const int32_t *ranks, int64_t n_item, int32_t nq, int output_mode, int64_t cu,
+                          QueryInfo *qi, int32_t *cand_user, int32_t *cand_rank, int32_t *refine_total,
+                          int32_t *refine_cand, int32_t *refine_user, int32_t *refine_query, float *refine_thr) {
+    if (nq <= 0 || n_user <= 0) return hipSuccess;
+    if (output_mode == RKR_OUTPUT_EXACT) {
+        const unsigned n_chunk = (unsigned)((n_user + cu - 1) / cu);
+        hipLaunchKernelGGL(k_compact_chunks, dim3(n_chunk, (unsigned)nq), dim3(NT), 0, s, bk, qip, n_user, S, ranks,
+                           n_item, cu, qi, cand_user, cand_rank, refine_total, refine_cand, refine_user, refine_query,
+                           refine_thr);
+    } else {
+        // reference output keeps the users in ascending order (settled users
+        // are taken by userID, US.hpp:173-186)
+        hipLaunchKernelGGL(k_compact, dim3((unsigned)nq), dim3(NT), 0, s, bk, qip, n_user, S, ranks, n_item, qi,
+                           cand_user, cand_rank, refine_total, refine_cand, refine_user, refine_query, refine_thr);
+    }
     return hipGetLastError();
 }
 

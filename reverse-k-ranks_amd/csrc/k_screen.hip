@@ -21,6 +21,8 @@
 //
 // Work unit = 128 refine rows (4 waves x 2 blocks of 16 rows; A fragments
 // held in VGPRs) x a chunk of items streamed through LDS 64 items at a time.
+// Units are ordered chunk-major, so the workgroups resident at one time all
+// stream the same item chunk, which stays in each XCD's L2.
 #include "rkr_internal.hpp"
 
 namespace rkr {
@@ -92,7 +94,8 @@ struct ScreenParams {
     const float *rows_thr;
     const int32_t *n_rows_dev;
     int64_t max_rows;
-    int32_t n_chunks;          // item chunks per row block
+    int64_t target_units;      // work units wanted to fill the device
+    int32_t stages_per_chunk;  // most 64-item stages per item chunk
     float C;                   // relative error factor
     float abs_floor;           // absolute error floor
     int32_t *cnt;              // per refine row
@@ -105,127 +108,135 @@ template <int KS>
 __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
     __shared__ uint4 sB[2][ITEMS_STAGE * (KS * 32 + KPAD_LDS) / 8];  // [hi|lo][item][k] as 16-byte chunks
     constexpr int ROWQ = (KS * 32 + KPAD_LDS) / 8;                   // uint4 per LDS item row
+    // Persistent grid: the work split is sized here from the refine count the
+    // compaction produced, so the host launches without reading it back.
     const int64_t n_rows = min((int64_t)*P.n_rows_dev, P.max_rows);
     const int64_t n_rb = (n_rows + ROWS_WG - 1) / ROWS_WG;
-    const int64_t unit = blockIdx.x;
-    const int64_t rb = unit / P.n_chunks;
-    const int chunk = (int)(unit % P.n_chunks);
-    if (rb >= n_rb) return;
+    const int64_t n_stage_total = (P.n_item + ITEMS_STAGE - 1) / ITEMS_STAGE;
+    const int64_t fill_chunks = n_rb > 0 ? (P.target_units + n_rb - 1) / n_rb : 1;
+    const int64_t st_per_chunk =
+        max((int64_t)1, min((int64_t)P.stages_per_chunk, (n_stage_total + fill_chunks - 1) / fill_chunks));
+    const int64_t n_chunks = (n_stage_total + st_per_chunk - 1) / st_per_chunk;
+    const int64_t n_units = n_rb * n_chunks;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
-    const int64_t row_base = rb * ROWS_WG + wave * 32;
-
-    // A fragments: row (row_base + b*16 + lr), k = ks*32 + 8*lq + i
-    Frag ah[2][KS], al[2][KS];
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-        const int64_t r = row_base + b * 16 + lr;
-        const float *urow = r < n_rows ? P.users + (int64_t)P.rows_user[r] * P.ld : nullptr;
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int k = ks * 32 + 8 * lq + i;
-                const float v = (urow && k < P.K) ? urow[k] : 0.0f;
-                const uint16_t h = bf16_bits(v);
-                ah[b][ks].h[i] = h;
-                al[b][ks].h[i] = bf16_bits(v - bf16_to_f32(h));
-            }
-        }
-    }
-    // epilogue constants of the 4 rows this lane's accumulator covers:
-    // D[4*lq + i][lr] of each 16x16 block
-    float thr_hi[2][4], thr_lo[2][4], Ar[2][4];
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int64_t r = row_base + b * 16 + 4 * lq + i;
-            if (r < n_rows) {
-                const float t = P.rows_thr[r];
-                thr_hi[b][i] = t - P.abs_floor;  // s^ + e > thr - floor  <=>  may count
-                thr_lo[b][i] = t + P.abs_floor;  // s^ - e > thr + floor  =>  counts
-                Ar[b][i] = P.C * P.user_norm[P.rows_user[r]];
-            } else {
-                thr_hi[b][i] = INFINITY;
-                thr_lo[b][i] = INFINITY;
-                Ar[b][i] = 0.0f;
-            }
-        }
-    int32_t cnt[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-
-    const int64_t n_stage_total = (P.n_item + ITEMS_STAGE - 1) / ITEMS_STAGE;
-    const int64_t st_per_chunk = (n_stage_total + P.n_chunks - 1) / P.n_chunks;
-    const int64_t st0 = chunk * st_per_chunk;
-    const int64_t st1 = min(n_stage_total, st0 + st_per_chunk);
     const int kq_row = P.Kp / 8;  // uint4 per global item row
 
-    for (int64_t st = st0; st < st1; ++st) {
-        const int64_t p0 = st * ITEMS_STAGE;
-        // stage 64 items (hi and lo) into LDS
-        for (int c = threadIdx.x; c < 2 * ITEMS_STAGE * kq_row; c += NT) {
-            const int half = c / (ITEMS_STAGE * kq_row);
-            const int rem = c % (ITEMS_STAGE * kq_row);
-            const int it = rem / kq_row, q = rem % kq_row;
-            const int64_t p = p0 + it;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (p < P.n_item)
-                v = reinterpret_cast<const uint4 *>(half ? P.item_lo : P.item_hi)[p * kq_row + q];
-            sB[half][it * ROWQ + q] = v;
-        }
-        __syncthreads();
-#pragma unroll 1
-        for (int g = 0; g < ITEMS_STAGE / 16; ++g) {
-            const int it = g * 16 + lr;
-            const int64_t p = p0 + it;
-            Frag bh[KS], bl[KS];
+    for (int64_t unit = blockIdx.x; unit < n_units; unit += gridDim.x) {
+        const int64_t chunk = unit / n_rb;
+        const int64_t rb = unit % n_rb;
+        const int64_t row_base = rb * ROWS_WG + wave * 32;
+
+        // A fragments: row (row_base + b*16 + lr), k = ks*32 + 8*lq + i
+        Frag ah[2][KS], al[2][KS];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int64_t r = row_base + b * 16 + lr;
+            const float *urow = r < n_rows ? P.users + (int64_t)P.rows_user[r] * P.ld : nullptr;
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
-                bh[ks].q = sB[0][it * ROWQ + ks * 4 + lq];
-                bl[ks].q = sB[1][it * ROWQ + ks * 4 + lq];
-            }
-            const float np = p < P.n_item ? P.item_norm[p] : -INFINITY;
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                f32x4 acc_hh = {0.f, 0.f, 0.f, 0.f}, acc_hl = acc_hh, acc_lh = acc_hh;
+                for (int i = 0; i < 8; ++i) {
+                    const int k = ks * 32 + 8 * lq + i;
+                    const float v = (urow && k < P.K) ? urow[k] : 0.0f;
+                    const uint16_t h = bf16_bits(v);
+                    ah[b][ks].h[i] = h;
+                    al[b][ks].h[i] = bf16_bits(v - bf16_to_f32(h));
+                }
+            }
+        }
+        // epilogue constants of the 4 rows this lane's accumulator covers:
+        // D[4*lq + i][lr] of each 16x16 block
+        float thr_hi[2][4], thr_lo[2][4], Ar[2][4];
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t r = row_base + b * 16 + 4 * lq + i;
+                if (r < n_rows) {
+                    const float t = P.rows_thr[r];
+                    thr_hi[b][i] = t - P.abs_floor;  // s^ + e > thr - floor  <=>  may count
+                    thr_lo[b][i] = t + P.abs_floor;  // s^ - e > thr + floor  =>  counts
+                    Ar[b][i] = P.C * P.user_norm[P.rows_user[r]];
+                } else {
+                    thr_hi[b][i] = INFINITY;
+                    thr_lo[b][i] = INFINITY;
+                    Ar[b][i] = 0.0f;
+                }
+            }
+        int32_t cnt[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+
+        const int64_t st0 = chunk * st_per_chunk;
+        const int64_t st1 = min(n_stage_total, st0 + st_per_chunk);
+        for (int64_t st = st0; st < st1; ++st) {
+            const int64_t p0 = st * ITEMS_STAGE;
+            // stage 64 items (hi and lo) into LDS
+            for (int c = threadIdx.x; c < 2 * ITEMS_STAGE * kq_row; c += NT) {
+                const int half = c / (ITEMS_STAGE * kq_row);
+                const int rem = c % (ITEMS_STAGE * kq_row);
+                const int it = rem / kq_row, q = rem % kq_row;
+                const int64_t p = p0 + it;
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (p < P.n_item)
+                    v = reinterpret_cast<const uint4 *>(half ? P.item_lo : P.item_hi)[p * kq_row + q];
+                sB[half][it * ROWQ + q] = v;
+            }
+            __syncthreads();
+#pragma unroll 1
+            for (int g = 0; g < ITEMS_STAGE / 16; ++g) {
+                const int it = g * 16 + lr;
+                const int64_t p = p0 + it;
+                Frag bh[KS], bl[KS];
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks) {
-                    acc_hh = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bh[ks].v, acc_hh, 0, 0, 0);
-                    acc_hl = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bl[ks].v, acc_hl, 0, 0, 0);
-                    acc_lh = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[b][ks].v, bh[ks].v, acc_lh, 0, 0, 0);
+                    bh[ks].q = sB[0][it * ROWQ + ks * 4 + lq];
+                    bl[ks].q = sB[1][it * ROWQ + ks * 4 + lq];
                 }
+                const float np = p < P.n_item ? P.item_norm[p] : -INFINITY;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const float s = (acc_hh[i] + acc_hl[i]) + acc_lh[i];
-                    const float hi = __builtin_fmaf(Ar[b][i], np, s);
-                    if (hi > thr_hi[b][i]) {
-                        const float lo = __builtin_fmaf(-Ar[b][i], np, s);
-                        if (lo > thr_lo[b][i]) {
-                            ++cnt[b][i];
-                        } else {
-                            const int64_t r = row_base + b * 16 + 4 * lq + i;
-                            const int32_t slot = atomicAdd(P.recheck_n, 1);
-                            if (slot < P.recheck_cap)
-                                P.recheck[slot] = make_int4((int32_t)r, (int32_t)p, __float_as_int(s), 0);
+                for (int b = 0; b < 2; ++b) {
+                    // all 3*Kp products into one FP32 accumulator (the bound
+                    // holds for any order of the additions)
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks) {
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bh[ks].v, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bl[ks].v, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[b][ks].v, bh[ks].v, acc, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float s = acc[i];
+                        const float hi = __builtin_fmaf(Ar[b][i], np, s);
+                        if (hi > thr_hi[b][i]) {
+                            const float lo = __builtin_fmaf(-Ar[b][i], np, s);
+                            if (lo > thr_lo[b][i]) {
+                                ++cnt[b][i];
+                            } else {
+                                const int64_t r = row_base + b * 16 + 4 * lq + i;
+                                const int32_t slot = atomicAdd(P.recheck_n, 1);
+                                if (slot < P.recheck_cap)
+                                    P.recheck[slot] = make_int4((int32_t)r, (int32_t)p, __float_as_int(s), 0);
+                            }
                         }
                     }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
+        // reduce each row's count over the 16 lanes (columns) that share it
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                int32_t v = cnt[b][i];
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+                const int64_t r = row_base + b * 16 + 4 * lq + i;
+                if (lr == 0 && r < n_rows && v) atomicAdd(&P.cnt[r], v);
+            }
     }
-    // reduce each row's count over the 16 lanes (columns) that share it
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            int32_t v = cnt[b][i];
-#pragma unroll
-            for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
-            const int64_t r = row_base + b * 16 + 4 * lq + i;
-            if (lr == 0 && r < n_rows && v) atomicAdd(&P.cnt[r], v);
-        }
 }
 
 // Exact decision of the rechecked (row, item) pairs with the sequential FP32
@@ -306,13 +317,14 @@ hipError_t launch_refine_screen(hipStream_t s, const ScreenLaunch &L) {
     P.recheck_n = L.recheck_n;
     P.recheck = reinterpret_cast<int4 *>(L.recheck);
     P.recheck_cap = L.recheck_cap;
-    const int64_t n_rb = (L.max_rows + ROWS_WG - 1) / ROWS_WG;
+    P.target_units = std::max<int64_t>(1, L.target_units);
+    P.stages_per_chunk = std::max(1, L.stages_per_chunk);
+    // persistent grid: enough workgroups to fill the device, never more than
+    // the largest possible number of units
+    const int64_t n_rb_max = (L.max_rows + ROWS_WG - 1) / ROWS_WG;
     const int64_t n_stage = (L.n_item + ITEMS_STAGE - 1) / ITEMS_STAGE;
-    // enough work units to fill the chip even for a handful of rows
-    int64_t chunks = (L.target_units + n_rb - 1) / n_rb;
-    chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, n_stage));
-    P.n_chunks = (int32_t)chunks;
-    const dim3 grid((unsigned)(n_rb * chunks));
+    const int64_t max_units = std::max<int64_t>(n_rb_max, std::min<int64_t>(P.target_units, n_rb_max * n_stage));
+    const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(max_units, L.grid_cap)));
     switch (L.Kp / 32) {
         case 1: hipLaunchKernelGGL(k_refine_screen<1>, grid, dim3(NT), 0, s, P); break;
         case 2: hipLaunchKernelGGL(k_refine_screen<2>, grid, dim3(NT), 0, s, P); break;

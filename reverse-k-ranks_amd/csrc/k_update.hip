@@ -153,6 +153,25 @@ __global__ void __launch_bounds__(256) k_subtract_rows(const float *__restrict__
 
 // Bounds with side-array offsets (exact output).  lbs/ubs as values:
 //   off = #{ins > q} - #{del > q};  lbs' = lbs + off;  ubs' = ubs + off.
+__device__ __forceinline__ void bounds_values(const float *row, int32_t S, float q, float eps,
+                                              const int32_t *__restrict__ ranks, int64_t n_item_base,
+                                              const float *ins_row, int32_t n_ins, const float *del_row,
+                                              int32_t n_del, int32_t &lb, int32_t &ub) {
+    float qlo = q, qhi = q;
+    if (eps > 0.0f) {
+        qlo = __double2float_rd((double)q - (double)eps);
+        qhi = __double2float_ru((double)q + (double)eps);
+    }
+    const int32_t blo = count_gt(row, S, qlo), bhi = eps > 0.0f ? count_gt(row, S, qhi) : blo;
+    int32_t off = 0;
+    if (n_ins) off += count_gt(ins_row, n_ins, q);
+    if (n_del) off -= count_gt(del_row, n_del, q);
+    lb = (blo == S ? (int32_t)n_item_base : ranks[blo]) + off;
+    // rank >= ub + off holds for any off; 0 is the tighter bound when it is negative
+    ub = max((bhi == 0 ? 0 : ranks[bhi - 1] + 1) + off, 0);
+}
+
+// grid (user blocks, queries)
 __global__ void __launch_bounds__(256) k_bounds_upd(const float *__restrict__ qip, const float *__restrict__ table,
                                                     int64_t n_user, int32_t S, const int32_t *__restrict__ ranks,
                                                     int64_t n_item_base, float eps, const float *__restrict__ ins,
@@ -161,21 +180,40 @@ __global__ void __launch_bounds__(256) k_bounds_upd(const float *__restrict__ qi
     const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t b = blockIdx.y;
     if (u >= n_user) return;
-    const float q = qip[b * n_user + u];
-    const float *row = table + u * (int64_t)S;
-    float qlo = q, qhi = q;
-    if (eps > 0.0f) {
-        qlo = __double2float_rd((double)q - (double)eps);
-        qhi = __double2float_ru((double)q + (double)eps);
-    }
-    const int32_t blo = count_gt(row, S, qlo), bhi = count_gt(row, S, qhi);
-    int32_t off = 0;
-    if (n_ins) off += count_gt(ins + u * (int64_t)n_ins, n_ins, q);
-    if (n_del) off -= count_gt(del + u * (int64_t)n_del, n_del, q);
-    const int32_t lb = (blo == S ? (int32_t)n_item_base : ranks[blo]) + off;
-    const int32_t ub = (bhi == 0 ? 0 : ranks[bhi - 1] + 1) + off;
+    int32_t lb, ub;
+    bounds_values(table + u * (int64_t)S, S, qip[b * n_user + u], eps, ranks, n_item_base, ins + u * (int64_t)n_ins,
+                  n_ins, del + u * (int64_t)n_del, n_del, lb, ub);
     lbs[b * n_user + u] = lb;
     ubs[b * n_user + u] = ub;
+}
+
+// Batched form: ut table rows staged in LDS serve all nq queries of those
+// users (thread t serves user t % ut); the side rows are searched in place.
+__global__ void __launch_bounds__(256) k_bounds_upd_lds(const float *__restrict__ qip, const float *__restrict__ table,
+                                                        int64_t n_user, int32_t S, int32_t nq, int32_t ut,
+                                                        const int32_t *__restrict__ ranks, int64_t n_item_base,
+                                                        float eps, const float *__restrict__ ins, int32_t n_ins,
+                                                        const float *__restrict__ del, int32_t n_del,
+                                                        int32_t *__restrict__ lbs, int32_t *__restrict__ ubs) {
+    extern __shared__ float rows[];  // ut x S
+    const int64_t u0 = (int64_t)blockIdx.x * ut;
+    const int64_t nu = min((int64_t)ut, n_user - u0);
+    const float *src = table + u0 * S;
+    for (int64_t i = threadIdx.x; i < nu * S; i += 256) rows[i] = src[i];
+    __syncthreads();
+    const int ul = threadIdx.x % ut;
+    const int lanes_q = 256 / ut;
+    if (ul >= nu) return;
+    const int64_t u = u0 + ul;
+    const float *row = rows + (int64_t)ul * S;
+    const float *ins_row = ins + u * (int64_t)n_ins, *del_row = del + u * (int64_t)n_del;
+    for (int32_t b = threadIdx.x / ut; b < nq; b += lanes_q) {
+        int32_t lb, ub;
+        bounds_values(row, S, qip[(int64_t)b * n_user + u], eps, ranks, n_item_base, ins_row, n_ins, del_row, n_del,
+                      lb, ub);
+        lbs[(int64_t)b * n_user + u] = lb;
+        ubs[(int64_t)b * n_user + u] = ub;
+    }
 }
 
 __device__ uint32_t wg_excl_scan(uint32_t v, uint32_t *red, uint32_t *total) {
@@ -200,106 +238,151 @@ __device__ uint32_t wg_excl_scan(uint32_t v, uint32_t *red, uint32_t *total) {
     return r;
 }
 
-// One workgroup per query: L = topk-th smallest lbs' (two-level histogram:
-// value >> 11, then value & 2047; bound values stay below 2^22, which the
-// host enforces) and the candidate count #{ubs' <= L}.
-__global__ void __launch_bounds__(NT) k_kth_values(const int32_t *__restrict__ lbs, const int32_t *__restrict__ ubs,
-                                                   int64_t n_user, QueryInfo *__restrict__ qi) {
-    __shared__ uint32_t h[2048];
-    __shared__ int32_t sel_hi, sel_lo;
-    __shared__ uint32_t sel_rem, n_cand;
-    const int64_t b = blockIdx.x;
-    const int32_t *lv = lbs + b * n_user;
-    const int32_t *uv = ubs + b * n_user;
-    const uint32_t k = (uint32_t)qi[b].topk;
-    for (int i = threadIdx.x; i < 2048; i += NT) h[i] = 0;
+// k-th smallest lbs' per query by a two-level histogram of the bound values
+// (value >> 11, then value & 2047; the host keeps values below 2^22), and
+// the candidate count #{ubs' <= L} from the same two levels of ubs'.
+// ghist per query: [level-1 lbs | level-1 ubs | level-2 lbs | level-2 ubs],
+// 2048 bins each.  Level kernels run on grid (chunks, queries), the
+// selections on one workgroup per query.
+__global__ void __launch_bounds__(NT) k_vhist1(const int32_t *__restrict__ lbs, const int32_t *__restrict__ ubs,
+                                               int64_t n_user, int64_t cu, uint32_t *__restrict__ ghist) {
+    __shared__ uint32_t hl[2048], hu[2048];
+    const int64_t b = blockIdx.y;
+    const int64_t u0 = (int64_t)blockIdx.x * cu, u1 = min(n_user, u0 + cu);
+    for (int i = threadIdx.x; i < 2048; i += NT) { hl[i] = 0; hu[i] = 0; }
     __syncthreads();
-    for (int64_t u = threadIdx.x; u < n_user; u += NT) atomicAdd(&h[((uint32_t)lv[u] >> 11) & 2047u], 1u);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t cum = 0, j = 0;
-        for (; j < 2047; ++j) { if (cum + h[j] >= k) break; cum += h[j]; }
-        sel_hi = (int32_t)j;
-        sel_rem = k - cum;  // 1-based rank inside the bin
+    const int32_t *lv = lbs + b * n_user, *uv = ubs + b * n_user;
+    for (int64_t u = u0 + threadIdx.x; u < u1; u += NT) {
+        atomicAdd(&hl[((uint32_t)lv[u] >> 11) & 2047u], 1u);
+        atomicAdd(&hu[((uint32_t)uv[u] >> 11) & 2047u], 1u);
     }
     __syncthreads();
-    const int32_t hi = sel_hi;
-    for (int i = threadIdx.x; i < 2048; i += NT) h[i] = 0;
-    __syncthreads();
-    for (int64_t u = threadIdx.x; u < n_user; u += NT) {
-        const uint32_t v = (uint32_t)lv[u];
-        if ((int32_t)((v >> 11) & 2047u) == hi) atomicAdd(&h[v & 2047u], 1u);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t cum = 0, j = 0;
-        for (; j < 2047; ++j) { if (cum + h[j] >= sel_rem) break; cum += h[j]; }
-        sel_lo = (int32_t)j;
-        n_cand = 0;
-    }
-    __syncthreads();
-    const int32_t L = (hi << 11) + sel_lo;
-    uint32_t c = 0;
-    for (int64_t u = threadIdx.x; u < n_user; u += NT) c += uv[u] <= L ? 1u : 0u;
-    atomicAdd(&n_cand, c);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        QueryInfo q = qi[b];
-        q.mode = 0;
-        q.beta = -1;
-        q.gamma = L;
-        q.n_cand = (int32_t)n_cand;
-        q.n_settled = 0;
-        qi[b] = q;
+    uint32_t *g = ghist + b * 4 * 2048;
+    for (int i = threadIdx.x; i < 2048; i += NT) {
+        if (hl[i]) atomicAdd(&g[i], hl[i]);
+        if (hu[i]) atomicAdd(&g[2048 + i], hu[i]);
     }
 }
 
-// Ordered compaction of the candidates ubs' <= L (rank known when lbs' == ubs').
+// Level-1 selection: bin of the k-th smallest lbs' (QueryInfo.beta), the rank
+// left inside it (gamma) and #{ubs' in lower bins} (n_cand, partial).
+__global__ void __launch_bounds__(NT) k_vsel1(const uint32_t *__restrict__ ghist, QueryInfo *__restrict__ qi) {
+    __shared__ uint32_t hl[2048], hu[2048];
+    const int64_t b = blockIdx.x;
+    const uint32_t *g = ghist + b * 4 * 2048;
+    for (int i = threadIdx.x; i < 2048; i += NT) { hl[i] = g[i]; hu[i] = g[2048 + i]; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const uint32_t k = (uint32_t)qi[b].topk;
+    uint32_t cum = 0, j = 0;
+    for (; j < 2047; ++j) { if (cum + hl[j] >= k) break; cum += hl[j]; }
+    uint32_t nc = 0;
+    for (uint32_t i = 0; i < j; ++i) nc += hu[i];
+    QueryInfo q = qi[b];
+    q.beta = (int32_t)j;
+    q.gamma = (int32_t)(k - cum);  // 1-based rank inside the bin
+    q.n_cand = (int32_t)nc;
+    qi[b] = q;
+}
+
+__global__ void __launch_bounds__(NT) k_vhist2(const int32_t *__restrict__ lbs, const int32_t *__restrict__ ubs,
+                                               int64_t n_user, int64_t cu, const QueryInfo *__restrict__ qi,
+                                               uint32_t *__restrict__ ghist) {
+    __shared__ uint32_t hl[2048], hu[2048];
+    const int64_t b = blockIdx.y;
+    const uint32_t hi = (uint32_t)qi[b].beta;
+    const int64_t u0 = (int64_t)blockIdx.x * cu, u1 = min(n_user, u0 + cu);
+    for (int i = threadIdx.x; i < 2048; i += NT) { hl[i] = 0; hu[i] = 0; }
+    __syncthreads();
+    const int32_t *lv = lbs + b * n_user, *uv = ubs + b * n_user;
+    for (int64_t u = u0 + threadIdx.x; u < u1; u += NT) {
+        const uint32_t l = (uint32_t)lv[u], w = (uint32_t)uv[u];
+        if (((l >> 11) & 2047u) == hi) atomicAdd(&hl[l & 2047u], 1u);
+        if (((w >> 11) & 2047u) == hi) atomicAdd(&hu[w & 2047u], 1u);
+    }
+    __syncthreads();
+    uint32_t *g = ghist + b * 4 * 2048 + 2 * 2048;
+    for (int i = threadIdx.x; i < 2048; i += NT) {
+        if (hl[i]) atomicAdd(&g[i], hl[i]);
+        if (hu[i]) atomicAdd(&g[2048 + i], hu[i]);
+    }
+}
+
+// Level-2 selection: L = k-th smallest lbs' (QueryInfo.gamma) and the
+// candidate count #{ubs' <= L}.
+__global__ void __launch_bounds__(NT) k_vsel2(const uint32_t *__restrict__ ghist, QueryInfo *__restrict__ qi) {
+    __shared__ uint32_t hl[2048], hu[2048];
+    const int64_t b = blockIdx.x;
+    const uint32_t *g = ghist + b * 4 * 2048 + 2 * 2048;
+    for (int i = threadIdx.x; i < 2048; i += NT) { hl[i] = g[i]; hu[i] = g[2048 + i]; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    QueryInfo q = qi[b];
+    const uint32_t rem = (uint32_t)q.gamma;
+    uint32_t cum = 0, j = 0;
+    for (; j < 2047; ++j) { if (cum + hl[j] >= rem) break; cum += hl[j]; }
+    uint32_t nc = (uint32_t)q.n_cand;
+    for (uint32_t i = 0; i <= j; ++i) nc += hu[i];
+    q.mode = 0;
+    q.gamma = (q.beta << 11) + (int32_t)j;
+    q.beta = -1;
+    q.n_cand = (int32_t)nc;
+    q.n_settled = 0;
+    q.n_refine = 0;
+    q.fill = 0;
+    qi[b] = q;
+}
+
+// Chunked compaction of the candidates ubs' <= L (rank known when
+// lbs' == ubs'), into ranges reserved on the query's cursor; the top-k
+// selection orders the slice by (rank, userID).
 __global__ void __launch_bounds__(NT) k_compact_values(const int32_t *__restrict__ lbs, const int32_t *__restrict__ ubs,
-                                                       const float *__restrict__ qip, int64_t n_user,
+                                                       const float *__restrict__ qip, int64_t n_user, int64_t cu,
                                                        QueryInfo *__restrict__ qi, int32_t *__restrict__ cand_user,
                                                        int32_t *__restrict__ cand_rank, int32_t *__restrict__ refine_total,
                                                        int32_t *__restrict__ refine_cand, int32_t *__restrict__ refine_user,
                                                        int32_t *__restrict__ refine_query, float *__restrict__ refine_thr) {
     __shared__ uint32_t red[NW + 1];
-    __shared__ int32_t n_refine;
-    const int64_t b = blockIdx.x;
-    const QueryInfo q = qi[b];
-    const int32_t *lv = lbs + b * n_user;
-    const int32_t *uv = ubs + b * n_user;
+    __shared__ int32_t s_can, s_ref;
+    const int64_t b = blockIdx.y;
+    const int32_t L = qi[b].gamma;
+    const int64_t off = qi[b].cand_off;
+    const int32_t *lv = lbs + b * n_user, *uv = ubs + b * n_user;
     const float *qb = qip + b * n_user;
-    const int32_t L = q.gamma;
-    uint32_t n_can = 0;
-    int32_t my_refine = 0;
-    if (threadIdx.x == 0) n_refine = 0;
-    for (int64_t u0 = 0; u0 < n_user; u0 += NT) {
-        const int64_t u = u0 + threadIdx.x;
+    const int64_t u0 = (int64_t)blockIdx.x * cu, u1 = min(n_user, u0 + cu);
+    for (int64_t ub0 = u0; ub0 < u1; ub0 += NT) {
+        const int64_t u = ub0 + threadIdx.x;
         bool cand = false;
         int32_t rank = -1;
-        if (u < n_user && uv[u] <= L) {
+        if (u < u1 && uv[u] <= L) {
             cand = true;
             if (lv[u] == uv[u]) rank = lv[u];
         }
+        const bool refine = cand && rank < 0;
         uint32_t tot;
-        const uint32_t e = wg_excl_scan(cand ? 1u : 0u, red, &tot);
+        const uint32_t e = wg_excl_scan((cand ? 1u : 0u) | (refine ? 0x10000u : 0u), red, &tot);
+        if (tot == 0) continue;
+        if (threadIdx.x == 0) {
+            s_can = atomicAdd(&qi[b].fill, (int32_t)(tot & 0xFFFFu));
+            if (tot >> 16) {
+                s_ref = atomicAdd(refine_total, (int32_t)(tot >> 16));
+                atomicAdd(&qi[b].n_refine, (int32_t)(tot >> 16));
+            }
+        }
+        __syncthreads();
         if (cand) {
-            const int64_t slot = q.cand_off + n_can + e;
+            const int64_t slot = off + s_can + (e & 0xFFFFu);
             cand_user[slot] = (int32_t)u;
             cand_rank[slot] = rank;
-            if (rank < 0) {
-                ++my_refine;
-                const int32_t r = atomicAdd(refine_total, 1);
+            if (refine) {
+                const int32_t r = s_ref + (int32_t)(e >> 16);
                 refine_cand[r] = (int32_t)slot;
                 refine_user[r] = (int32_t)u;
                 refine_query[r] = (int32_t)b;
                 refine_thr[r] = qb[u];
             }
         }
-        n_can += tot;
     }
-    if (my_refine) atomicAdd(&n_refine, my_refine);
-    __syncthreads();
-    if (threadIdx.x == 0) qi[b].n_refine = n_refine;
 }
 
 __global__ void k_gather_rows(const float *__restrict__ src, int64_t lds, const int64_t *__restrict__ idx,
@@ -343,6 +426,14 @@ hipError_t launch_bounds_upd(hipStream_t s, const float *qip, const float *table
                              const int32_t *ranks, int64_t n_item_base, float eps, const float *ins, int32_t n_ins,
                              const float *del, int32_t n_del, int32_t nq, int32_t *lbs, int32_t *ubs) {
     if (n_user <= 0 || nq <= 0) return hipSuccess;
+    if (nq >= 8) {
+        int ut = 64;
+        while (ut > 1 && (int64_t)ut * S * 4 > 65536) ut >>= 1;
+        const unsigned grid = (unsigned)((n_user + ut - 1) / ut);
+        hipLaunchKernelGGL(k_bounds_upd_lds, dim3(grid), dim3(256), (size_t)ut * S * 4, s, qip, table, n_user, S, nq,
+                           ut, ranks, n_item_base, eps, ins, n_ins, del, n_del, lbs, ubs);
+        return hipGetLastError();
+    }
     dim3 grid((unsigned)((n_user + 255) / 256), (unsigned)nq);
     hipLaunchKernelGGL(k_bounds_upd, grid, dim3(256), 0, s, qip, table, n_user, S, ranks, n_item_base, eps, ins, n_ins,
                        del, n_del, lbs, ubs);
@@ -350,19 +441,26 @@ hipError_t launch_bounds_upd(hipStream_t s, const float *qip, const float *table
 }
 
 hipError_t launch_kth_values(hipStream_t s, const int32_t *lbs, const int32_t *ubs, int64_t n_user, int32_t nq,
-                             QueryInfo *qi) {
+                             int64_t cu, uint32_t *ghist, QueryInfo *qi) {
     if (nq <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_kth_values, dim3((unsigned)nq), dim3(NT), 0, s, lbs, ubs, n_user, qi);
+    hipError_t e = hipMemsetAsync(ghist, 0, sizeof(uint32_t) * 4 * 2048 * (size_t)nq, s);
+    if (e != hipSuccess) return e;
+    const dim3 grid((unsigned)((n_user + cu - 1) / cu), (unsigned)nq);
+    if (n_user > 0) hipLaunchKernelGGL(k_vhist1, grid, dim3(NT), 0, s, lbs, ubs, n_user, cu, ghist);
+    hipLaunchKernelGGL(k_vsel1, dim3((unsigned)nq), dim3(NT), 0, s, ghist, qi);
+    if (n_user > 0) hipLaunchKernelGGL(k_vhist2, grid, dim3(NT), 0, s, lbs, ubs, n_user, cu, qi, ghist);
+    hipLaunchKernelGGL(k_vsel2, dim3((unsigned)nq), dim3(NT), 0, s, ghist, qi);
     return hipGetLastError();
 }
 
 hipError_t launch_compact_values(hipStream_t s, const int32_t *lbs, const int32_t *ubs, const float *qip,
-                                 int64_t n_user, int32_t nq, QueryInfo *qi, int32_t *cand_user, int32_t *cand_rank,
-                                 int32_t *refine_total, int32_t *refine_cand, int32_t *refine_user,
-                                 int32_t *refine_query, float *refine_thr) {
-    if (nq <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_compact_values, dim3((unsigned)nq), dim3(NT), 0, s, lbs, ubs, qip, n_user, qi, cand_user,
-                       cand_rank, refine_total, refine_cand, refine_user, refine_query, refine_thr);
+                                 int64_t n_user, int32_t nq, int64_t cu, QueryInfo *qi, int32_t *cand_user,
+                                 int32_t *cand_rank, int32_t *refine_total, int32_t *refine_cand,
+                                 int32_t *refine_user, int32_t *refine_query, float *refine_thr) {
+    if (nq <= 0 || n_user <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((n_user + cu - 1) / cu), (unsigned)nq);
+    hipLaunchKernelGGL(k_compact_values, grid, dim3(NT), 0, s, lbs, ubs, qip, n_user, cu, qi, cand_user, cand_rank,
+                       refine_total, refine_cand, refine_user, refine_query, refine_thr);
     return hipGetLastError();
 }
 

@@ -10,6 +10,8 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <string_view>
+#include <unordered_map>
 #include <vector>
 
 #include "rkr_internal.hpp"
@@ -144,7 +146,7 @@ int slot_init(Ctx *c, Slot &sl, bool own_stream) {
 void slot_free(Ctx *c, Slot &sl) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     free_bufs(c, {&sl.q_in, &sl.qip, &sl.bk, &sl.qinfo, &sl.cand, &sl.refine_idx, &sl.counts, &sl.out, &sl.recheck,
-                  &sl.small});
+                  &sl.small, &sl.hist});
     for (auto &e : sl.ev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (sl.own_stream && sl.stream) (void)hipStreamDestroy(sl.stream);
@@ -239,14 +241,14 @@ int rkr_create(int device, size_t mem_budget_bytes, rkr_ctx **out) {
 
 void rkr_destroy(rkr_ctx *c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_scratch(c);
     slot_free(c, c->main);
     free_data(c);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -273,6 +275,14 @@ int rkr_set_option(rkr_ctx *c, int option, int64_t value) {
             if (value < 0) return set_err(c, RKR_ERR_INVALID, "build batch must be >= 0");
             c->build_batch = value;
             return RKR_OK;
+        case RKR_OPT_SCREEN_CHUNK:
+            if (value < 0 || value > (1 << 20)) return set_err(c, RKR_ERR_INVALID, "screen chunk must be in [0, 2^20]");
+            c->screen_chunk = (int32_t)value;
+            return RKR_OK;
+        case RKR_OPT_DEDUP:
+            if (value != 0 && value != 1) return set_err(c, RKR_ERR_INVALID, "dedup must be 0 or 1");
+            c->dedup = (int)value;
+            return RKR_OK;
         default:
             return set_err(c, RKR_ERR_INVALID, "unknown option");
     }
@@ -285,6 +295,8 @@ int rkr_get_option(const rkr_ctx *c, int option, int64_t *value) {
         case RKR_OPT_OUTPUT: *value = c->output; return RKR_OK;
         case RKR_OPT_QUERY_BATCH: *value = c->query_batch; return RKR_OK;
         case RKR_OPT_BUILD_BATCH: *value = c->build_batch; return RKR_OK;
+        case RKR_OPT_SCREEN_CHUNK: *value = c->screen_chunk; return RKR_OK;
+        case RKR_OPT_DEDUP: *value = c->dedup; return RKR_OK;
         default: return RKR_ERR_INVALID;
     }
 }
@@ -538,7 +550,49 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
     if (n_query == 0) return RKR_OK;
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
-    return query_on_slot(c, c->main, queries, n_query, topk, max_k, out_user, out_rank, out_qip, stats, timing);
+    if (!c->dedup || n_query < 2)
+        return query_on_slot(c, c->main, queries, n_query, topk, max_k, out_user, out_rank, out_qip, stats, timing);
+
+    // Identical query vectors (byte-equal) are answered once, with the largest
+    // k asked for them: the answer for k is the first k entries of the answer
+    // for any larger k, since (rank, userID) is a total order.
+    const size_t row = sizeof(float) * (size_t)c->dim;
+    std::unordered_map<std::string_view, int32_t> seen;
+    seen.reserve((size_t)n_query * 2);
+    std::vector<int32_t> rep_of((size_t)n_query), reps;
+    for (int32_t i = 0; i < n_query; ++i) {
+        const std::string_view key(reinterpret_cast<const char *>(queries + (size_t)i * c->dim), row);
+        auto it = seen.emplace(key, (int32_t)reps.size());
+        if (it.second) reps.push_back(i);
+        rep_of[i] = it.first->second;
+    }
+    const int32_t n_rep = (int32_t)reps.size();
+    if (n_rep == n_query)
+        return query_on_slot(c, c->main, queries, n_query, topk, max_k, out_user, out_rank, out_qip, stats, timing);
+    std::vector<float> rq((size_t)n_rep * c->dim);
+    std::vector<int32_t> rk((size_t)n_rep, 0);
+    for (int32_t j = 0; j < n_rep; ++j) std::memcpy(&rq[(size_t)j * c->dim], queries + (size_t)reps[j] * c->dim, row);
+    for (int32_t i = 0; i < n_query; ++i) rk[rep_of[i]] = std::max(rk[rep_of[i]], topk[i]);
+    std::vector<int32_t> ru((size_t)n_rep * max_k), rr((size_t)n_rep * max_k);
+    std::vector<float> rqip((size_t)n_rep * max_k);
+    std::vector<rkr_query_stats> rst(stats ? (size_t)n_rep : 0);
+    rc = query_on_slot(c, c->main, rq.data(), n_rep, rk.data(), max_k, ru.data(), rr.data(), rqip.data(),
+                       stats ? rst.data() : nullptr, timing);
+    if (rc) return rc;
+    for (int32_t i = 0; i < n_query; ++i) {
+        const size_t src = (size_t)rep_of[i] * max_k, dst = (size_t)i * max_k;
+        for (int32_t j = 0; j < max_k; ++j) {
+            const bool keep = j < topk[i];
+            out_user[dst + j] = keep ? ru[src + j] : -1;
+            out_rank[dst + j] = keep ? rr[src + j] : -1;
+            if (out_qip) out_qip[dst + j] = keep ? rqip[src + j] : -1.0f;
+        }
+        if (stats) {
+            stats[i] = rst[rep_of[i]];
+            stats[i].query_id = i;
+        }
+    }
+    return RKR_OK;
 }
 
 }  // extern "C"
@@ -593,6 +647,7 @@ int query_on_slot(Ctx *c, Slot &sl, const float *queries, int32_t n_query, const
     if ((rc = ensure(c, sl.bk, (pair_bytes - 4) * (size_t)nqb * U, "bucket pairs"))) return rc;
     if ((rc = ensure(c, sl.qinfo, sizeof(QueryInfo) * (size_t)nqb + 64, "query info"))) return rc;
     if ((rc = ensure(c, sl.out, (sizeof(int32_t) * 2 + sizeof(float)) * (size_t)nqb * max_k, "outputs"))) return rc;
+    if ((rc = ensure(c, sl.hist, sizeof(uint32_t) * hist_words((int32_t)nqb, S), "bound histograms"))) return rc;
 
     std::vector<float> hq((size_t)nqb * ld, 0.0f);
     std::vector<QueryInfo> hqi((size_t)nqb);
@@ -633,8 +688,12 @@ int query_on_slot(Ctx *c, Slot &sl, const float *queries, int32_t n_query, const
         else
             RKR_HIP(c, launch_bounds(sl.stream, qip, c->table, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
         RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
-        if (upd) RKR_HIP(c, launch_kth_values(sl.stream, lbv, ubv, U, nq, qi));
-        else RKR_HIP(c, launch_hist_kth(sl.stream, bk, U, S, c->ranks, I, nq, mode, qi));
+        const int64_t cu = chunk_users(U, nq, c->n_cu);
+        uint32_t *ghist = reinterpret_cast<uint32_t *>(sl.hist.p);
+        if (upd) RKR_HIP(c, launch_kth_values(sl.stream, lbv, ubv, U, nq, cu, ghist, qi));
+        else
+            RKR_HIP(c, launch_hist_kth(sl.stream, bk, U, S, c->ranks, I, nq, mode,
+                                       (mode == RKR_OUTPUT_EXACT && eps == 0.0f) ? 1 : 0, cu, ghist, qi));
         RKR_HIP(c, launch_offsets(sl.stream, qi, nq, refine_total));
         RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
         RKR_HIP(c, hipStreamSynchronize(sl.stream));
@@ -661,10 +720,10 @@ int query_on_slot(Ctx *c, Slot &sl, const float *queries, int32_t n_query, const
         const bool screened = c->form == RKR_FORM_SCREENED;
 
         if (upd)
-            RKR_HIP(c, launch_compact_values(sl.stream, lbv, ubv, qip, U, nq, qi, cand_user, cand_rank, refine_total,
-                                             refine_cand, refine_user, refine_query, refine_thr));
+            RKR_HIP(c, launch_compact_values(sl.stream, lbv, ubv, qip, U, nq, cu, qi, cand_user, cand_rank,
+                                             refine_total, refine_cand, refine_user, refine_query, refine_thr));
         else
-            RKR_HIP(c, launch_compact(sl.stream, bk, qip, U, S, c->ranks, I, nq, mode, qi, cand_user, cand_rank,
+            RKR_HIP(c, launch_compact(sl.stream, bk, qip, U, S, c->ranks, I, nq, mode, cu, qi, cand_user, cand_rank,
                                       refine_total, refine_cand, refine_user, refine_query, refine_thr));
         int32_t h_refine_total = 0, h_recheck_n = 0;
         uint32_t h_err_bits = 0;
@@ -690,6 +749,8 @@ int query_on_slot(Ctx *c, Slot &sl, const float *queries, int32_t n_query, const
                 L.max_rows = (int64_t)rcap; L.C = c->screen_C; L.abs_floor = c->screen_abs;
                 L.cnt = cnt; L.recheck_n = recheck_n; L.recheck = sl.recheck.p; L.recheck_cap = rec_cap;
                 L.target_units = (int64_t)c->n_cu * 8;
+                L.grid_cap = (int64_t)c->n_cu * 4;
+                L.stages_per_chunk = c->screen_chunk > 0 ? c->screen_chunk : 32;
                 RKR_HIP(c, launch_refine_screen(sl.stream, L));
                 RKR_HIP(c, hipEventRecord(sl.ev[8], sl.stream));
                 RKR_HIP(c, launch_recheck(sl.stream, c->users, c->items, ld, d, refine_user, refine_thr, refine_query,

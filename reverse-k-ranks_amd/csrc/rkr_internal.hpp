@@ -35,7 +35,7 @@ struct Slot {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     hipEvent_t ev[16] = {};
-    DeviceBuf q_in, qip, bk, qinfo, cand, refine_idx, counts, out, recheck, small;
+    DeviceBuf q_in, qip, bk, qinfo, cand, refine_idx, counts, out, recheck, small, hist;
 };
 
 struct Ctx {
@@ -52,6 +52,8 @@ struct Ctx {
     int output = RKR_OUTPUT_EXACT;
     int64_t query_batch = 0;
     int64_t build_batch = 0;
+    int32_t screen_chunk = 0;   // 64-item stages per screened work unit (0 = 32)
+    int dedup = 1;              // answer byte-equal queries of one call once
 
     // data
     int64_t n_user = 0, n_item = 0;
@@ -155,6 +157,8 @@ struct ScreenLaunch {
     void *recheck;          // int4 records (row, item, screened score bits, 0)
     int64_t recheck_cap;
     int64_t target_units;   // work units wanted to fill the device
+    int64_t grid_cap;       // workgroups of the persistent grid
+    int32_t stages_per_chunk;  // most 64-item stages per item chunk (L2-resident slice)
 };
 hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, int32_t ld, int32_t K, int32_t Kp,
                              uint16_t *hi, uint16_t *lo, float *norm);
@@ -181,18 +185,39 @@ struct QueryInfo {          // one per query of a device batch
     int32_t n_refine;       // candidates whose rank is counted
     int32_t mode;           // classification case (reference output)
     int32_t pad;
+    int32_t fill;           // candidates written so far (chunked compaction)
+    int32_t pad2;
     int64_t cand_off;       // offset of this query's candidates
 };
+
+// Users per chunk of the chunked prune kernels: enough workgroups per query
+// to fill the device for small batches, one chunk per query for large ones.
+inline int64_t chunk_users(int64_t n_user, int32_t nq, int n_cu) {
+    const int64_t target = (int64_t)(n_cu > 0 ? n_cu : 1) * 4;
+    int64_t n_chunk = (target + nq - 1) / (nq > 0 ? nq : 1);
+    n_chunk = n_chunk < 1 ? 1 : n_chunk;
+    const int64_t max_chunk = (n_user + 4095) / 4096;
+    if (n_chunk > max_chunk) n_chunk = max_chunk < 1 ? 1 : max_chunk;
+    int64_t cu = (n_user + n_chunk - 1) / n_chunk;
+    cu = (cu + 1023) / 1024 * 1024;
+    return cu < 1024 ? 1024 : cu;
+}
+// uint32 words of the global histograms of a batch of nq queries
+inline size_t hist_words(int32_t nq, int32_t S) {
+    const size_t a = (size_t)nq * 2 * (size_t)(S + 1), b = (size_t)nq * 4 * 2048;
+    return a > b ? a : b;
+}
 
 hipError_t launch_bounds(hipStream_t s, const float *qip, const float *table, int64_t n_user, int32_t S,
                          int32_t nq, float eps, int ref_mode, uint32_t *bk);
 hipError_t launch_hist_kth(hipStream_t s, const uint32_t *bk, int64_t n_user, int32_t S, const int32_t *ranks,
-                           int64_t n_item, int32_t nq, int output_mode, QueryInfo *qi);
+                           int64_t n_item, int32_t nq, int output_mode, int same, int64_t cu, uint32_t *ghist,
+                           QueryInfo *qi);
 hipError_t launch_offsets(hipStream_t s, QueryInfo *qi, int32_t nq, int32_t *refine_total);
 hipError_t launch_compact(hipStream_t s, const uint32_t *bk, const float *qip, int64_t n_user, int32_t S,
-                          const int32_t *ranks, int64_t n_item, int32_t nq, int output_mode, QueryInfo *qi,
-                          int32_t *cand_user, int32_t *cand_rank, int32_t *refine_total, int32_t *refine_cand,
-                          int32_t *refine_user, int32_t *refine_query, float *refine_thr);
+                          const int32_t *ranks, int64_t n_item, int32_t nq, int output_mode, int64_t cu,
+                          QueryInfo *qi, int32_t *cand_user, int32_t *cand_rank, int32_t *refine_total,
+                          int32_t *refine_cand, int32_t *refine_user, int32_t *refine_query, float *refine_thr);
 hipError_t launch_scatter_counts(hipStream_t s, const int32_t *refine_total, const int32_t *refine_cand,
                                  const int32_t *cnt, int64_t max_rows, int32_t *cand_rank);
 hipError_t launch_topk(hipStream_t s, const QueryInfo *qi, int32_t nq, const int32_t *cand_user,
@@ -209,11 +234,11 @@ hipError_t launch_bounds_upd(hipStream_t s, const float *qip, const float *table
                              const int32_t *ranks, int64_t n_item_base, float eps, const float *ins, int32_t n_ins,
                              const float *del, int32_t n_del, int32_t nq, int32_t *lbs, int32_t *ubs);
 hipError_t launch_kth_values(hipStream_t s, const int32_t *lbs, const int32_t *ubs, int64_t n_user, int32_t nq,
-                             QueryInfo *qi);
+                             int64_t cu, uint32_t *ghist, QueryInfo *qi);
 hipError_t launch_compact_values(hipStream_t s, const int32_t *lbs, const int32_t *ubs, const float *qip,
-                                 int64_t n_user, int32_t nq, QueryInfo *qi, int32_t *cand_user, int32_t *cand_rank,
-                                 int32_t *refine_total, int32_t *refine_cand, int32_t *refine_user,
-                                 int32_t *refine_query, float *refine_thr);
+                                 int64_t n_user, int32_t nq, int64_t cu, QueryInfo *qi, int32_t *cand_user,
+                                 int32_t *cand_rank, int32_t *refine_total, int32_t *refine_cand,
+                                 int32_t *refine_user, int32_t *refine_query, float *refine_thr);
 hipError_t launch_gather_rows(hipStream_t s, const float *src, int64_t lds, const int64_t *idx, int64_t n_rows,
                               int64_t width, float *dst, int64_t ldd);
 

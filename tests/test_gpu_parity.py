@@ -124,6 +124,28 @@ def test_query_mixed_k_and_query_batches(orc, gpu_index_factory):
         assert np.all(res.users[i, k:] == -1) and np.all(res.ranks[i, k:] == -1)
 
 
+def test_query_duplicates_dedup(orc, gpu_index_factory):
+    """Byte-equal queries with different k are answered once (largest k) and
+    sliced; the answers equal the ones computed without deduplication and the
+    brute-force oracle.  U spans several prune chunks."""
+    _case("query duplicates, dedup on/off")
+    rng = np.random.default_rng(15)
+    U, I, d = 10000, 1500, 20
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = rng.standard_normal((I, d), dtype=np.float32) * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32)
+    pick = rng.integers(0, 5, 40)  # 40 queries over 5 distinct vectors
+    queries = items[pick]
+    ks = rng.integers(1, 150, 40).astype(np.int32)
+    idx, _ = gpu_index_factory(users, items, 48)
+    on = idx.query(queries, ks, max_k=150)
+    idx.set_option(6, 0)  # RKR_OPT_DEDUP off
+    off = idx.query(queries, ks, max_k=150)
+    assert np.array_equal(on.users, off.users) and np.array_equal(on.ranks, off.ranks)
+    assert np.array_equal(on.qips, off.qips)
+    assert [s["query_id"] for s in on.stats] == list(range(40))
+    _check_exact(orc, _normalized(orc, users), items, queries, on, ks)
+
+
 def test_query_many_candidates(orc, gpu_index_factory):
     """Coarse table (S=4) so thousands of users survive pruning: exercises the
     radix-select top-k path (> 4096 candidates) and large refine batches."""

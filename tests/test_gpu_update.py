@@ -113,6 +113,25 @@ def test_insert_and_delete_users(orc, gpu_index_factory):
     m.check(idx, queries, 25)
 
 
+@pytest.mark.parametrize("nq", [1, 12])
+def test_updates_many_users_chunked(orc, gpu_index_factory, nq):
+    """Enough users that the prune kernels split each query over several
+    chunks, with deletes that make some shifted upper bounds negative."""
+    _case(f"updates, chunked prune, nq={nq}")
+    rng, users, items = _data(35, U=13000, I=900, d=16)
+    idx, _ = gpu_index_factory(users, items, 24)
+    m = Mirror(orc, users, items)
+    new = rng.standard_normal((40, items.shape[1]), dtype=np.float32) * np.float32(1.4)
+    idx.insert_items(new)
+    m.insert_items(new)
+    top = np.argsort(-np.linalg.norm(items, axis=1))[:25].astype(np.int64)
+    idx.delete_items(np.sort(top))
+    m.delete_items(np.sort(top))
+    queries = m.items()[rng.choice(len(m.items()), nq, replace=False)]
+    m.check(idx, queries, 60)
+    m.check(idx, queries, 60, "screened")
+
+
 def test_rebuild_after_updates_matches_oracle_table(orc, gpu_index_factory):
     _case("rebuild table after updates")
     rng, users, items = _data(33, U=800, I=1500)
