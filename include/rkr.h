@@ -55,7 +55,7 @@ typedef struct rkr_ctx rkr_ctx;
 #define RKR_OPT_OUTPUT 2
 #define RKR_OPT_QUERY_BATCH 3   /* max queries per device batch (0 = auto)      */
 #define RKR_OPT_BUILD_BATCH 4   /* max users per build batch (0 = auto)         */
-#define RKR_OPT_SCREEN_CHUNK 5  /* 64-item stages per screened work unit (0 = 32) */
+#define RKR_OPT_SCREEN_CHUNK 5  /* items per screened work-unit chunk (0 = no limit) */
 #define RKR_OPT_DEDUP 6         /* 1 (default): byte-equal queries of one rkr_query
                                    call are answered once, with their largest k   */
 

@@ -79,23 +79,24 @@ __global__ void __launch_bounds__(256) k_bounds(const float *__restrict__ qip, c
     bk[b * n_user + u] = lo | (hi << 16);
 }
 
-// Batched form: a workgroup stages UT consecutive table rows in LDS and runs
+// Batched form: a workgroup stages ut consecutive table rows in LDS and runs
 // all nq searches of those users from there (each row is read from HBM once
-// per batch instead of once per query).  Thread t serves user t % UT.
+// per batch instead of once per query).  Thread t serves user t % ut.  Rows
+// sit at a stride of S + 1 floats, so the ut searches of one step, which
+// probe the same index of different rows, fall in different LDS banks.
 __global__ void __launch_bounds__(256) k_bounds_lds(const float *__restrict__ qip, const float *__restrict__ table,
                                                     int64_t n_user, int32_t S, int32_t nq, int32_t ut, float eps,
                                                     int ref_mode, uint32_t *__restrict__ bk) {
-    extern __shared__ float rows[];  // ut x S
+    extern __shared__ float rows[];  // ut x (S + 1)
     const int64_t u0 = (int64_t)blockIdx.x * ut;
     const int64_t nu = min((int64_t)ut, n_user - u0);
-    const float *src = table + u0 * S;
-    for (int64_t i = threadIdx.x; i < nu * S; i += 256) rows[i] = src[i];
+    stage_rows(rows, table + u0 * S, nu, S);
     __syncthreads();
     const int ul = threadIdx.x % ut;
     const int lanes_q = 256 / ut;
     if (ul >= nu) return;
     const int64_t u = u0 + ul;
-    const float *row = rows + (int64_t)ul * S;
+    const float *row = rows + (int64_t)ul * (S + 1);
     for (int32_t b = threadIdx.x / ut; b < nq; b += lanes_q) {
         uint32_t lo, hi;
         bucket_pair(row, S, qip[(int64_t)b * n_user + u], eps, ref_mode, lo, hi);
@@ -507,12 +508,11 @@ __global__ void __launch_bounds__(NT) k_topk(const QueryInfo *__restrict__ qi, c
 hipError_t launch_bounds(hipStream_t s, const float *qip, const float *table, int64_t n_user, int32_t S, int32_t nq,
                          float eps, int ref_mode, uint32_t *bk) {
     if (n_user <= 0 || nq <= 0) return hipSuccess;
-    if (nq >= 8) {
-        int ut = 64;
-        while (ut > 1 && (int64_t)ut * S * 4 > 65536) ut >>= 1;
+    const int ut = bounds_rows_per_wg(S);
+    if (nq >= 4 && ut > 0) {
         const unsigned grid = (unsigned)((n_user + ut - 1) / ut);
-        hipLaunchKernelGGL(k_bounds_lds, dim3(grid), dim3(256), (size_t)ut * S * 4, s, qip, table, n_user, S, nq, ut,
-                           eps, ref_mode, bk);
+        hipLaunchKernelGGL(k_bounds_lds, dim3(grid), dim3(256), (size_t)ut * (S + 1) * 4, s, qip, table, n_user, S, nq,
+                           ut, eps, ref_mode, bk);
         return hipGetLastError();
     }
     dim3 grid((unsigned)((n_user + 255) / 256), (unsigned)nq);

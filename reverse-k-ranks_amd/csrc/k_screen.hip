@@ -22,7 +22,8 @@
 // Work unit = 128 refine rows (4 waves x 2 blocks of 16 rows; A fragments
 // held in VGPRs) x a chunk of items streamed through LDS 64 items at a time.
 // Units are ordered chunk-major, so the workgroups resident at one time all
-// stream the same item chunk, which stays in each XCD's L2.
+// stream the same item chunk, which stays in each XCD's L2.  The next stage
+// is loaded into registers while the current one is computed from LDS.
 #include "rkr_internal.hpp"
 
 namespace rkr {
@@ -95,7 +96,7 @@ struct ScreenParams {
     const int32_t *n_rows_dev;
     int64_t max_rows;
     int64_t target_units;      // work units wanted to fill the device
-    int32_t stages_per_chunk;  // most 64-item stages per item chunk
+    int32_t chunk_items;       // most items per work-unit chunk
     float C;                   // relative error factor
     float abs_floor;           // absolute error floor
     int32_t *cnt;              // per refine row
@@ -106,22 +107,58 @@ struct ScreenParams {
 
 template <int KS>
 __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
-    __shared__ uint4 sB[2][ITEMS_STAGE * (KS * 32 + KPAD_LDS) / 8];  // [hi|lo][item][k] as 16-byte chunks
+    // items per LDS stage: 32 for long rows keeps the prefetch registers
+    // within the 256-VGPR budget of two waves per SIMD
+    constexpr int IST = KS >= 5 ? 32 : ITEMS_STAGE;
+    __shared__ uint4 sB[2][IST * (KS * 32 + KPAD_LDS) / 8];  // [hi|lo][item][k] as 16-byte chunks
+    __shared__ float sN[IST];                                // item norm bounds of the stage
     constexpr int ROWQ = (KS * 32 + KPAD_LDS) / 8;                   // uint4 per LDS item row
+    constexpr int KQ = KS * 4;                                       // uint4 per global item row (Kp = 32 KS)
+    constexpr int PER_T = 2 * IST * KQ / NT;                 // uint4 each thread stages per stage
+    static_assert(2 * IST * KQ % NT == 0, "stage must split evenly over the workgroup");
     // Persistent grid: the work split is sized here from the refine count the
     // compaction produced, so the host launches without reading it back.
     const int64_t n_rows = min((int64_t)*P.n_rows_dev, P.max_rows);
     const int64_t n_rb = (n_rows + ROWS_WG - 1) / ROWS_WG;
-    const int64_t n_stage_total = (P.n_item + ITEMS_STAGE - 1) / ITEMS_STAGE;
+    const int64_t n_stage_total = (P.n_item + IST - 1) / IST;
     const int64_t fill_chunks = n_rb > 0 ? (P.target_units + n_rb - 1) / n_rb : 1;
     const int64_t st_per_chunk =
-        max((int64_t)1, min((int64_t)P.stages_per_chunk, (n_stage_total + fill_chunks - 1) / fill_chunks));
+        max((int64_t)1, min((int64_t)max(1, P.chunk_items / IST), (n_stage_total + fill_chunks - 1) / fill_chunks));
     const int64_t n_chunks = (n_stage_total + st_per_chunk - 1) / st_per_chunk;
     const int64_t n_units = n_rb * n_chunks;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
-    const int kq_row = P.Kp / 8;  // uint4 per global item row
+    // Stage st+1 is loaded into registers while stage st is computed from
+    // LDS, then written to LDS between two barriers.
+    uint4 pre[PER_T];
+    float pre_n = -INFINITY;
+    auto load_stage = [&](int64_t st) {
+        const int64_t p0 = st * IST;
+#pragma unroll
+        for (int j = 0; j < PER_T; ++j) {
+            const int c = threadIdx.x + j * NT;
+            const int half = c / (IST * KQ);
+            const int rem = c % (IST * KQ);
+            const int64_t p = p0 + rem / KQ;
+            pre[j] = p < P.n_item ? reinterpret_cast<const uint4 *>(half ? P.item_lo : P.item_hi)[p * KQ + rem % KQ]
+                                  : make_uint4(0, 0, 0, 0);
+        }
+        if (threadIdx.x < IST) {
+            const int64_t p = p0 + threadIdx.x;
+            pre_n = p < P.n_item ? P.item_norm[p] : -INFINITY;
+        }
+    };
+    auto store_stage = [&]() {
+#pragma unroll
+        for (int j = 0; j < PER_T; ++j) {
+            const int c = threadIdx.x + j * NT;
+            const int half = c / (IST * KQ);
+            const int rem = c % (IST * KQ);
+            sB[half][(rem / KQ) * ROWQ + rem % KQ] = pre[j];
+        }
+        if (threadIdx.x < IST) sN[threadIdx.x] = pre_n;
+    };
 
     for (int64_t unit = blockIdx.x; unit < n_units; unit += gridDim.x) {
         const int64_t chunk = unit / n_rb;
@@ -148,20 +185,18 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
         }
         // epilogue constants of the 4 rows this lane's accumulator covers:
         // D[4*lq + i][lr] of each 16x16 block
-        float thr_hi[2][4], thr_lo[2][4], Ar[2][4];
+        // (the row's threshold itself is re-read on the rare undecided path)
+        float thr_hi[2][4], Ar[2][4];
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int64_t r = row_base + b * 16 + 4 * lq + i;
                 if (r < n_rows) {
-                    const float t = P.rows_thr[r];
-                    thr_hi[b][i] = t - P.abs_floor;  // s^ + e > thr - floor  <=>  may count
-                    thr_lo[b][i] = t + P.abs_floor;  // s^ - e > thr + floor  =>  counts
+                    thr_hi[b][i] = P.rows_thr[r] - P.abs_floor;  // s^ + e > thr - floor  <=>  may count
                     Ar[b][i] = P.C * P.user_norm[P.rows_user[r]];
                 } else {
                     thr_hi[b][i] = INFINITY;
-                    thr_lo[b][i] = INFINITY;
                     Ar[b][i] = 0.0f;
                 }
             }
@@ -169,52 +204,47 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
 
         const int64_t st0 = chunk * st_per_chunk;
         const int64_t st1 = min(n_stage_total, st0 + st_per_chunk);
+        if (st0 < st1) load_stage(st0);
         for (int64_t st = st0; st < st1; ++st) {
-            const int64_t p0 = st * ITEMS_STAGE;
-            // stage 64 items (hi and lo) into LDS
-            for (int c = threadIdx.x; c < 2 * ITEMS_STAGE * kq_row; c += NT) {
-                const int half = c / (ITEMS_STAGE * kq_row);
-                const int rem = c % (ITEMS_STAGE * kq_row);
-                const int it = rem / kq_row, q = rem % kq_row;
-                const int64_t p = p0 + it;
-                uint4 v = make_uint4(0, 0, 0, 0);
-                if (p < P.n_item)
-                    v = reinterpret_cast<const uint4 *>(half ? P.item_lo : P.item_hi)[p * kq_row + q];
-                sB[half][it * ROWQ + q] = v;
-            }
+            const int64_t p0 = st * IST;
+            __syncthreads();  // every wave is done with the previous stage
+            store_stage();
             __syncthreads();
+            if (st + 1 < st1) load_stage(st + 1);
 #pragma unroll 1
-            for (int g = 0; g < ITEMS_STAGE / 16; ++g) {
+            for (int g = 0; g < IST / 16; ++g) {
                 const int it = g * 16 + lr;
                 const int64_t p = p0 + it;
-                Frag bh[KS], bl[KS];
+                // all 3*Kp products of a 16x16 block into one FP32 accumulator
+                // (the bound holds for any order of the additions); B
+                // fragments are read from LDS one k-step at a time
+                f32x4 accs[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks) {
-                    bh[ks].q = sB[0][it * ROWQ + ks * 4 + lq];
-                    bl[ks].q = sB[1][it * ROWQ + ks * 4 + lq];
+                    Frag bh, bl;
+                    bh.q = sB[0][it * ROWQ + ks * 4 + lq];
+                    bl.q = sB[1][it * ROWQ + ks * 4 + lq];
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        accs[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bh.v, accs[b], 0, 0, 0);
+                        accs[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bl.v, accs[b], 0, 0, 0);
+                        accs[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[b][ks].v, bh.v, accs[b], 0, 0, 0);
+                    }
                 }
-                const float np = p < P.n_item ? P.item_norm[p] : -INFINITY;
+                const float np = sN[it];
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    // all 3*Kp products into one FP32 accumulator (the bound
-                    // holds for any order of the additions)
-                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int ks = 0; ks < KS; ++ks) {
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bh[ks].v, acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bl[ks].v, acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[b][ks].v, bh[ks].v, acc, 0, 0, 0);
-                    }
+                    const f32x4 acc = accs[b];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const float s = acc[i];
                         const float hi = __builtin_fmaf(Ar[b][i], np, s);
                         if (hi > thr_hi[b][i]) {
+                            const int64_t r = row_base + b * 16 + 4 * lq + i;
                             const float lo = __builtin_fmaf(-Ar[b][i], np, s);
-                            if (lo > thr_lo[b][i]) {
+                            if (lo > P.rows_thr[r] + P.abs_floor) {  // s^ - e > thr + floor  =>  counts
                                 ++cnt[b][i];
                             } else {
-                                const int64_t r = row_base + b * 16 + 4 * lq + i;
                                 const int32_t slot = atomicAdd(P.recheck_n, 1);
                                 if (slot < P.recheck_cap)
                                     P.recheck[slot] = make_int4((int32_t)r, (int32_t)p, __float_as_int(s), 0);
@@ -223,7 +253,6 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
                     }
                 }
             }
-            __syncthreads();
         }
         // reduce each row's count over the 16 lanes (columns) that share it
 #pragma unroll
@@ -318,7 +347,7 @@ hipError_t launch_refine_screen(hipStream_t s, const ScreenLaunch &L) {
     P.recheck = reinterpret_cast<int4 *>(L.recheck);
     P.recheck_cap = L.recheck_cap;
     P.target_units = std::max<int64_t>(1, L.target_units);
-    P.stages_per_chunk = std::max(1, L.stages_per_chunk);
+    P.chunk_items = std::max(1, L.chunk_items);
     // persistent grid: enough workgroups to fill the device, never more than
     // the largest possible number of units
     const int64_t n_rb_max = (L.max_rows + ROWS_WG - 1) / ROWS_WG;

@@ -195,17 +195,16 @@ __global__ void __launch_bounds__(256) k_bounds_upd_lds(const float *__restrict_
                                                         float eps, const float *__restrict__ ins, int32_t n_ins,
                                                         const float *__restrict__ del, int32_t n_del,
                                                         int32_t *__restrict__ lbs, int32_t *__restrict__ ubs) {
-    extern __shared__ float rows[];  // ut x S
+    extern __shared__ float rows[];  // ut x (S + 1)
     const int64_t u0 = (int64_t)blockIdx.x * ut;
     const int64_t nu = min((int64_t)ut, n_user - u0);
-    const float *src = table + u0 * S;
-    for (int64_t i = threadIdx.x; i < nu * S; i += 256) rows[i] = src[i];
+    stage_rows(rows, table + u0 * S, nu, S);
     __syncthreads();
     const int ul = threadIdx.x % ut;
     const int lanes_q = 256 / ut;
     if (ul >= nu) return;
     const int64_t u = u0 + ul;
-    const float *row = rows + (int64_t)ul * S;
+    const float *row = rows + (int64_t)ul * (S + 1);
     const float *ins_row = ins + u * (int64_t)n_ins, *del_row = del + u * (int64_t)n_del;
     for (int32_t b = threadIdx.x / ut; b < nq; b += lanes_q) {
         int32_t lb, ub;
@@ -426,11 +425,10 @@ hipError_t launch_bounds_upd(hipStream_t s, const float *qip, const float *table
                              const int32_t *ranks, int64_t n_item_base, float eps, const float *ins, int32_t n_ins,
                              const float *del, int32_t n_del, int32_t nq, int32_t *lbs, int32_t *ubs) {
     if (n_user <= 0 || nq <= 0) return hipSuccess;
-    if (nq >= 8) {
-        int ut = 64;
-        while (ut > 1 && (int64_t)ut * S * 4 > 65536) ut >>= 1;
+    const int ut = bounds_rows_per_wg(S);
+    if (nq >= 4 && ut > 0) {
         const unsigned grid = (unsigned)((n_user + ut - 1) / ut);
-        hipLaunchKernelGGL(k_bounds_upd_lds, dim3(grid), dim3(256), (size_t)ut * S * 4, s, qip, table, n_user, S, nq,
+        hipLaunchKernelGGL(k_bounds_upd_lds, dim3(grid), dim3(256), (size_t)ut * (S + 1) * 4, s, qip, table, n_user, S, nq,
                            ut, ranks, n_item_base, eps, ins, n_ins, del, n_del, lbs, ubs);
         return hipGetLastError();
     }

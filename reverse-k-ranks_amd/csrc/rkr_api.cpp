@@ -276,7 +276,7 @@ int rkr_set_option(rkr_ctx *c, int option, int64_t value) {
             c->build_batch = value;
             return RKR_OK;
         case RKR_OPT_SCREEN_CHUNK:
-            if (value < 0 || value > (1 << 20)) return set_err(c, RKR_ERR_INVALID, "screen chunk must be in [0, 2^20]");
+            if (value < 0 || value > (1 << 30)) return set_err(c, RKR_ERR_INVALID, "screen chunk must be in [0, 2^30]");
             c->screen_chunk = (int32_t)value;
             return RKR_OK;
         case RKR_OPT_DEDUP:
@@ -750,7 +750,7 @@ int query_on_slot(Ctx *c, Slot &sl, const float *queries, int32_t n_query, const
                 L.cnt = cnt; L.recheck_n = recheck_n; L.recheck = sl.recheck.p; L.recheck_cap = rec_cap;
                 L.target_units = (int64_t)c->n_cu * 8;
                 L.grid_cap = (int64_t)c->n_cu * 4;
-                L.stages_per_chunk = c->screen_chunk > 0 ? c->screen_chunk : 32;
+                L.chunk_items = c->screen_chunk > 0 ? c->screen_chunk : (1 << 30);
                 RKR_HIP(c, launch_refine_screen(sl.stream, L));
                 RKR_HIP(c, hipEventRecord(sl.ev[8], sl.stream));
                 RKR_HIP(c, launch_recheck(sl.stream, c->users, c->items, ld, d, refine_user, refine_thr, refine_query,

@@ -52,7 +52,7 @@ struct Ctx {
     int output = RKR_OUTPUT_EXACT;
     int64_t query_batch = 0;
     int64_t build_batch = 0;
-    int32_t screen_chunk = 0;   // 64-item stages per screened work unit (0 = 32)
+    int32_t screen_chunk = 0;   // items per screened work-unit chunk (0 = no limit)
     int dedup = 1;              // answer byte-equal queries of one call once
 
     // data
@@ -158,7 +158,7 @@ struct ScreenLaunch {
     int64_t recheck_cap;
     int64_t target_units;   // work units wanted to fill the device
     int64_t grid_cap;       // workgroups of the persistent grid
-    int32_t stages_per_chunk;  // most 64-item stages per item chunk (L2-resident slice)
+    int32_t chunk_items;    // most items per work-unit chunk (L2-resident slice)
 };
 hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, int32_t ld, int32_t K, int32_t Kp,
                              uint16_t *hi, uint16_t *lo, float *norm);
@@ -189,6 +189,34 @@ struct QueryInfo {          // one per query of a device batch
     int32_t pad2;
     int64_t cand_off;       // offset of this query's candidates
 };
+
+// Table rows staged per workgroup by the batched bounds kernels: 16 rows of
+// S + 1 floats (<= 32 KB at S = 511), fewer for long rows; 0 when one row
+// does not fit in 64 KB.
+inline int bounds_rows_per_wg(int32_t S) {
+    int ut = 16;
+    while (ut > 1 && (int64_t)ut * (S + 1) * 4 > 40960) ut >>= 1;
+    return (int64_t)ut * (S + 1) * 4 <= 65536 ? ut : 0;
+}
+
+#ifdef __HIPCC__
+// Copies nu rows of S floats (contiguous in HBM) to LDS rows of stride S + 1,
+// 16-byte loads where the source row allows (S % 4 == 0).
+__device__ __forceinline__ void stage_rows(float *rows, const float *__restrict__ src, int64_t nu, int32_t S) {
+    if ((S & 3) == 0) {
+        const int32_t q = S >> 2;
+        const float4 *s4 = reinterpret_cast<const float4 *>(src);
+        for (int64_t i = threadIdx.x; i < nu * q; i += blockDim.x) {
+            const float4 v = s4[i];
+            const int64_t r = i / q, c = (i % q) * 4;
+            float *d = rows + r * (S + 1) + c;
+            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+        }
+    } else {
+        for (int64_t i = threadIdx.x; i < nu * S; i += blockDim.x) rows[(i / S) * (S + 1) + i % S] = src[i];
+    }
+}
+#endif
 
 // Users per chunk of the chunked prune kernels: enough workgroups per query
 // to fill the device for small batches, one chunk per query for large ones.
