@@ -61,52 +61,71 @@ def parse():
                     help="insert rounds (+1%% users, +0.5%% items each) applied before timing")
     ap.add_argument("--query-batch", type=int, default=0)
     ap.add_argument("--no-dedup", action="store_true", help="compute byte-equal queries of a call separately")
+    ap.add_argument("--update-mode", default="snapshot", choices=["snapshot", "barrier"],
+                    help="serving: updates prepared beside running batches, or exclusive barriers")
+    ap.add_argument("--sat-seconds", type=float, default=3.0, help="serving: closed-loop saturation run length")
     return ap.parse_args()
 
 
 def serving(args, idx, wl, label):
-    """Open-loop serving: Poisson query arrivals (k uniform in [10, 200]) with an
-    insert batch of 1% new users and 0.5% new items every second; latency is
-    admission -> answer, measured by the library's monotonic clock."""
+    """Serving form: Poisson query arrivals (k uniform in [10, 200]) while an
+    insert batch of 1% new users and 0.5% new items arrives every second.
+    The saturating rate is measured first, closed-loop, under the same insert
+    stream; the open loop then runs at half of it (or --qps).  Latency is
+    submit -> answer on the library's monotonic clock."""
     import threading
     import rkr
     rng = np.random.default_rng(args.seed + 7)
     U0, I0, d = wl.users.shape[0], wl.items.shape[0], wl.users.shape[1]
     qpool = wl.items
-    srv = rkr.Server(idx, max_batch=256, n_slots=2)
-    # saturating rate: a closed burst of 2048 queries
-    n_burst = 2048
-    ks = rng.integers(10, 201, n_burst)
+    n_new_u = max(1, U0 // 100)
+    n_new_i = max(1, I0 // 200)
+    srv = rkr.Server(idx, max_batch=256, n_slots=2, update_mode=args.update_mode)
+    upd, stop_upd = [], threading.Event()
+    urng = np.random.default_rng(args.seed + 8)
+
+    def updater():
+        t_next = time.perf_counter() + 1.0
+        while not stop_upd.wait(max(0.0, t_next - time.perf_counter())):
+            nu = urng.standard_normal((n_new_u, d), dtype=np.float32) / np.float32(np.sqrt(d))
+            ni = urng.standard_normal((n_new_i, d), dtype=np.float32) / np.float32(np.sqrt(d))
+            upd.append(srv.submit_update("insert_users", vectors=nu))
+            upd.append(srv.submit_update("insert_items", vectors=ni))
+            t_next += 1.0
+
+    uth = threading.Thread(target=updater)
+    uth.start()
+    # saturating rate: closed loop with 2 x max_batch queries outstanding
+    window = 512
+    pending = []
     t0 = time.perf_counter()
-    tk = [srv.submit_query(qpool[rng.integers(0, I0)], int(k)) for k in ks]
-    for t, k in zip(tk, ks):
-        srv.wait(t, int(k))
-    sat = n_burst / (time.perf_counter() - t0)
+    n_sat = 0
+    while True:
+        while len(pending) < window:
+            k = int(rng.integers(10, 201))
+            pending.append((srv.submit_query(qpool[rng.integers(0, I0)], k), k))
+        t, k = pending.pop(0)
+        srv.wait(t, k)
+        n_sat += 1
+        if time.perf_counter() - t0 >= args.sat_seconds:
+            break
+    for t, k in pending:
+        srv.wait(t, k)
+        n_sat += 1
+    sat = n_sat / (time.perf_counter() - t0)
     rate = args.qps if args.qps > 0 else sat / 2
     # open loop
     n_arr = int(rate * args.duration)
-    gaps = rng.exponential(1.0 / rate, n_arr)
-    arrivals = np.cumsum(gaps)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_arr))
     kq = rng.integers(10, 201, n_arr)
     qi = rng.integers(0, I0, n_arr)
-    n_new_u = max(1, U0 // 100)
-    n_new_i = max(1, I0 // 200)
     tickets = []
-    upd = []
     start = time.perf_counter()
 
     def submitter():
-        next_upd = 1.0
         for j in range(n_arr):
             while True:
                 el = time.perf_counter() - start
-                if el >= next_upd:
-                    nu = (rng.standard_normal((n_new_u, d), dtype=np.float32) / np.float32(np.sqrt(d)))
-                    ni = (rng.standard_normal((n_new_i, d), dtype=np.float32) / np.float32(np.sqrt(d)))
-                    upd.append(srv.submit_update("insert_users", vectors=nu))
-                    upd.append(srv.submit_update("insert_items", vectors=ni))
-                    next_upd += 1.0
-                    continue
                 if el >= arrivals[j]:
                     break
                 time.sleep(min(arrivals[j] - el, 0.0005))
@@ -115,29 +134,23 @@ def serving(args, idx, wl, label):
     th = threading.Thread(target=submitter)
     th.start()
     lat, bsz, done = [], [], 0
-    while True:
+    while done < n_arr:
         if done < len(tickets):
             t, k = tickets[done]
             _, _, _, info = srv.wait(t, k)
             lat.append((info["t_done"] - info["t_submit"]) * 1e3)
             bsz.append(info["batch_size"])
             done += 1
-        elif not th.is_alive():
-            break
         else:
             time.sleep(0.0002)
     th.join()
-    while done < len(tickets):
-        t, k = tickets[done]
-        _, _, _, info = srv.wait(t, k)
-        lat.append((info["t_done"] - info["t_submit"]) * 1e3)
-        bsz.append(info["batch_size"])
-        done += 1
-    ulat = []
+    wall = time.perf_counter() - start
+    stop_upd.set()
+    uth.join()
+    ulat, ustall = [], []
     for t in upd:
         _, _, _, info = srv.wait(t)
         ulat.append((info["t_done"] - info["t_start"]) * 1e3)
-    wall = time.perf_counter() - start
     cnt = srv.counters()
     srv.stop()
     lat = np.array(lat)
@@ -145,9 +158,12 @@ def serving(args, idx, wl, label):
         "metric": "serving queries/s with p50/p99 latency (exact ranks), inserts every 1 s",
         "value": round(len(lat) / wall, 2), "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32", "data": "synthetic (seeded), open-loop Poisson arrivals",
-        "config": {"workload": label, "form": args.form, "arrival_rate": round(rate, 1),
-                   "saturating_rate": round(sat, 1), "duration_s": args.duration},
+        "config": {"workload": label, "form": args.form, "update_mode": args.update_mode,
+                   "arrival_rate": round(rate, 1), "saturating_rate": round(sat, 1),
+                   "saturation": f"closed loop, {window} outstanding, {args.sat_seconds:g} s, same insert stream",
+                   "duration_s": args.duration},
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "p999_ms": round(float(np.percentile(lat, 99.9)), 3),
         "mean_batch": round(float(np.mean(bsz)), 1), "queries": len(lat), "batches": cnt["n_batches"],
         "update_batches": len(ulat), "update_ms_p50": round(float(np.percentile(ulat, 50)), 2) if ulat else None,
         "update_ms_max": round(float(np.max(ulat)), 2) if ulat else None,

@@ -196,8 +196,10 @@ int rkr_get_counts(const rkr_ctx *ctx, int64_t *n_user, int64_t *n_item, int64_t
  * user / item updates) enter one FIFO; worker threads, each with its own HIP
  * stream and scratch, batch the queries at the front of the queue, so batches
  * overlap on the device and requests that arrive meanwhile join the next one.
- * Updates are barriers: each query sees exactly the updates submitted before
- * it.  Answers are identical to answering each query alone.  The reference has
+ * Updates are barriers (each query sees exactly the updates submitted before
+ * it) or, in snapshot mode, are prepared while batches run (see
+ * rkr_server_config).  Answers are identical to answering each query alone
+ * against the state it reads.  The reference has
  * no serving loop (its drivers run queries one after another, US.hpp:117);
  * this replaces that loop for the serving form. */
 typedef struct rkr_server rkr_server;
@@ -205,7 +207,18 @@ typedef struct rkr_server rkr_server;
 typedef struct rkr_server_config {
     int32_t max_batch;   /* queries per device batch (0 = 256); memory may split it */
     int32_t n_slots;     /* concurrent batches / worker threads (0 = 2, max 4)       */
+    int32_t update_mode; /* RKR_SERVE_BARRIER (0) or RKR_SERVE_SNAPSHOT              */
 } rkr_server_config;
+
+/* Barrier: an update waits for the batches in flight and blocks new ones, so
+ * each query sees exactly the updates submitted before it.
+ * Snapshot: updates are applied in submission order by their own thread while
+ * batches keep running; a batch reads the state of the updates committed
+ * before it started (an update's ticket completes at its commit), and the
+ * buffers an update replaces are freed once the batches that may read them
+ * have finished. */
+#define RKR_SERVE_BARRIER 0
+#define RKR_SERVE_SNAPSHOT 1
 
 #define RKR_UPDATE_INSERT_USERS 1
 #define RKR_UPDATE_INSERT_ITEMS 2

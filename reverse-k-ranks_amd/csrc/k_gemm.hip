@@ -254,8 +254,8 @@ hipError_t launch_refine_count(hipStream_t s, const float *users, const float *i
                                int64_t max_rows, int32_t *cnt) {
     if (max_rows <= 0 || n_item <= 0) return hipSuccess;
     int dev = 0, n_cu = 256;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
     const int64_t n_it = (n_item + BN - 1) / BN;
     const int64_t max_units = ((max_rows + BM - 1) / BM) * ((n_it + kGroup - 1) / kGroup);
     const int64_t grid = std::min<int64_t>(max_units, (int64_t)n_cu * 4);

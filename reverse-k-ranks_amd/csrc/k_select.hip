@@ -68,6 +68,31 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// Calls f(v) for every element of row x[0..n), in no particular order.
+// Aligned rows use 16-byte loads with four in flight per thread, so a pass
+// over a row is not a chain of dependent load latencies.
+template <class F>
+__device__ __forceinline__ void row_for_each(const float *__restrict__ x, int64_t n, F &&f) {
+    if ((n & 3) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+        const float4 *x4 = reinterpret_cast<const float4 *>(x);
+        const int64_t n4 = n >> 2;
+        int64_t i = threadIdx.x;
+        for (; i + 3 * NT < n4; i += 4 * NT) {
+            const float4 a = x4[i], b = x4[i + NT], c = x4[i + 2 * NT], d = x4[i + 3 * NT];
+            f(a.x); f(a.y); f(a.z); f(a.w);
+            f(b.x); f(b.y); f(b.z); f(b.w);
+            f(c.x); f(c.y); f(c.z); f(c.w);
+            f(d.x); f(d.y); f(d.z); f(d.w);
+        }
+        for (; i < n4; i += NT) {
+            const float4 a = x4[i];
+            f(a.x); f(a.y); f(a.z); f(a.w);
+        }
+    } else {
+        for (int64_t i = threadIdx.x; i < n; i += NT) f(x[i]);
+    }
+}
+
 // Workgroup min/max; every thread gets the result.
 __device__ void wg_minmax(SelSmem &sm, float &mn, float &mx) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -175,7 +200,7 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
 
     // 1. range
     float mn = INFINITY, mx = -INFINITY;
-    for (int64_t i = tid; i < n_item; i += NT) { const float v = x[i]; mn = fminf(mn, v); mx = fmaxf(mx, v); }
+    row_for_each(x, n_item, [&](float v) { mn = fminf(mn, v); mx = fmaxf(mx, v); });
     wg_minmax(sm, mn, mx);
     const float lo0 = mn, hi0 = mx;
     if (!(lo0 < hi0) || isinf(lo0) || isinf(hi0)) {
@@ -187,7 +212,7 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
     // 2. histogram + reverse prefix sums
     for (int i = tid; i < NB; i += NT) { sm.incl[i] = 0; sm.cur[i] = NONE; }
     __syncthreads();
-    for (int64_t i = tid; i < n_item; i += NT) atomicAdd(&sm.incl[bin_of(x[i], lo0, scale0, NB)], 1u);
+    row_for_each(x, n_item, [&](float v) { atomicAdd(&sm.incl[bin_of(v, lo0, scale0, NB)], 1u); });
     __syncthreads();
     wg_reverse_incl(sm, sm.incl, NB);
 
@@ -271,10 +296,9 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
             const int j1 = (f0 + 1 < nfb) ? sm.fb[f0 + 1] : S;
             const int b = sm.tb[j0];
             float bmn = INFINITY, bmx = -INFINITY;
-            for (int64_t i = tid; i < n_item; i += NT) {
-                const float v = x[i];
+            row_for_each(x, n_item, [&](float v) {
                 if (bin_of(v, lo0, scale0, NB) == b) { bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
-            }
+            });
             wg_minmax(sm, bmn, bmx);
             for (int j = j0; j < j1; ++j) {
                 const uint32_t t = (uint32_t)ranks[j] - sm.incl[b + 1];
@@ -296,11 +320,10 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
             sm.ncoop = 0;
         }
         __syncthreads();
-        for (int64_t i = tid; i < n_item; i += NT) {
-            const float v = x[i];
+        row_for_each(x, n_item, [&](float v) {
             const int b = bin_of(v, lo0, scale0, NB);
             if (sm.cur[b] != NONE) sm.gbuf[atomicAdd(&sm.cur[b], 1u)] = v;
-        }
+        });
         __syncthreads();
         // one wavefront per bin: position of each element in descending order
         const int w = tid >> 6, lane = tid & 63;

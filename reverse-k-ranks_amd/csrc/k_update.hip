@@ -188,24 +188,34 @@ __global__ void __launch_bounds__(256) k_bounds_upd(const float *__restrict__ qi
 }
 
 // Batched form: ut table rows staged in LDS serve all nq queries of those
-// users (thread t serves user t % ut); the side rows are searched in place.
+// users (thread t serves user t % ut).  With stage_side, the users' side rows
+// are staged too; otherwise they are searched in place.  LDS rows carry one
+// pad float so the searches of one step spread over the banks.
 __global__ void __launch_bounds__(256) k_bounds_upd_lds(const float *__restrict__ qip, const float *__restrict__ table,
                                                         int64_t n_user, int32_t S, int32_t nq, int32_t ut,
-                                                        const int32_t *__restrict__ ranks, int64_t n_item_base,
-                                                        float eps, const float *__restrict__ ins, int32_t n_ins,
-                                                        const float *__restrict__ del, int32_t n_del,
+                                                        int stage_side, const int32_t *__restrict__ ranks,
+                                                        int64_t n_item_base, float eps, const float *__restrict__ ins,
+                                                        int32_t n_ins, const float *__restrict__ del, int32_t n_del,
                                                         int32_t *__restrict__ lbs, int32_t *__restrict__ ubs) {
-    extern __shared__ float rows[];  // ut x (S + 1)
+    // ut x (S + 1), then (stage_side) ut x (n_ins + 1) if n_ins, ut x (n_del + 1) if n_del
+    extern __shared__ float rows[];
+    float *rows_ins = rows + (int64_t)ut * (S + 1);
+    float *rows_del = rows_ins + (n_ins ? (int64_t)ut * (n_ins + 1) : 0);
     const int64_t u0 = (int64_t)blockIdx.x * ut;
     const int64_t nu = min((int64_t)ut, n_user - u0);
     stage_rows(rows, table + u0 * S, nu, S);
+    if (stage_side) {
+        if (n_ins) stage_rows(rows_ins, ins + u0 * n_ins, nu, n_ins);
+        if (n_del) stage_rows(rows_del, del + u0 * n_del, nu, n_del);
+    }
     __syncthreads();
     const int ul = threadIdx.x % ut;
     const int lanes_q = 256 / ut;
     if (ul >= nu) return;
     const int64_t u = u0 + ul;
     const float *row = rows + (int64_t)ul * (S + 1);
-    const float *ins_row = ins + u * (int64_t)n_ins, *del_row = del + u * (int64_t)n_del;
+    const float *ins_row = stage_side ? rows_ins + (int64_t)ul * (n_ins + 1) : ins + u * (int64_t)n_ins;
+    const float *del_row = stage_side ? rows_del + (int64_t)ul * (n_del + 1) : del + u * (int64_t)n_del;
     for (int32_t b = threadIdx.x / ut; b < nq; b += lanes_q) {
         int32_t lb, ub;
         bounds_values(row, S, qip[(int64_t)b * n_user + u], eps, ranks, n_item_base, ins_row, n_ins, del_row, n_del,
@@ -425,12 +435,25 @@ hipError_t launch_bounds_upd(hipStream_t s, const float *qip, const float *table
                              const int32_t *ranks, int64_t n_item_base, float eps, const float *ins, int32_t n_ins,
                              const float *del, int32_t n_del, int32_t nq, int32_t *lbs, int32_t *ubs) {
     if (n_user <= 0 || nq <= 0) return hipSuccess;
-    const int ut = bounds_rows_per_wg(S);
-    if (nq >= 4 && ut > 0) {
-        const unsigned grid = (unsigned)((n_user + ut - 1) / ut);
-        hipLaunchKernelGGL(k_bounds_upd_lds, dim3(grid), dim3(256), (size_t)ut * (S + 1) * 4, s, qip, table, n_user, S, nq,
-                           ut, ranks, n_item_base, eps, ins, n_ins, del, n_del, lbs, ubs);
-        return hipGetLastError();
+    if (nq >= 4) {
+        // users per workgroup: with their side rows when they fit 40 KB
+        const int64_t per_user = (int64_t)(S + 1) + (n_ins ? n_ins + 1 : 0) + (n_del ? n_del + 1 : 0);
+        int ut = 16;
+        while (ut > 1 && (int64_t)ut * per_user * 4 > 40960) ut >>= 1;
+        // staging every side row costs a full pass over them: worth it for
+        // larger batches only
+        int stage_side = nq >= 32 && (int64_t)ut * per_user * 4 <= 65536;
+        size_t lds = (size_t)ut * per_user * 4;
+        if (!stage_side) {
+            ut = bounds_rows_per_wg(S);
+            lds = (size_t)ut * (S + 1) * 4;
+        }
+        if (ut > 0) {
+            const unsigned grid = (unsigned)((n_user + ut - 1) / ut);
+            hipLaunchKernelGGL(k_bounds_upd_lds, dim3(grid), dim3(256), lds, s, qip, table, n_user, S, nq, ut,
+                               stage_side, ranks, n_item_base, eps, ins, n_ins, del, n_del, lbs, ubs);
+            return hipGetLastError();
+        }
     }
     dim3 grid((unsigned)((n_user + 255) / 256), (unsigned)nq);
     hipLaunchKernelGGL(k_bounds_upd, grid, dim3(256), 0, s, qip, table, n_user, S, ranks, n_item_base, eps, ins, n_ins,

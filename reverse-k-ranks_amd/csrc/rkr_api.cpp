@@ -45,11 +45,52 @@ int dev_alloc(Ctx *c, void **p, size_t bytes, const char *what) {
     return RKR_OK;
 }
 
+namespace {
+thread_local bool tl_defer_frees = false;
+}
+
+void set_defer_frees(bool on) { tl_defer_frees = on; }
+
 void dev_free(Ctx *c, void *p, size_t bytes) {
     if (!p) return;
     std::lock_guard<std::mutex> g(c->mem_mu);
+    if (tl_defer_frees) {
+        c->retired.push_back({p, bytes});
+        return;
+    }
     (void)hipFree(p);
     c->held -= std::min(c->held, bytes);
+}
+
+void release_retired(Ctx *c) {
+    std::lock_guard<std::mutex> g(c->mem_mu);
+    for (const auto &r : c->retired) {
+        (void)hipFree(r.p);
+        c->held -= std::min(c->held, r.bytes);
+    }
+    c->retired.clear();
+}
+
+DataSnap snap_of(const Ctx *c) {
+    DataSnap d;
+    d.n_user = c->n_user;
+    d.n_item = c->n_item;
+    d.n_item_base = c->n_item_base;
+    d.users = c->users;
+    d.usersT = c->usersT;
+    d.items = c->items;
+    d.user_norm = c->user_norm;
+    d.item_norm = c->item_norm;
+    d.item_hi = c->item_hi;
+    d.item_lo = c->item_lo;
+    d.table = c->table;
+    d.side_ins = c->side_ins;
+    d.side_del = c->side_del;
+    d.n_ins = c->n_ins;
+    d.n_del = c->n_del;
+    d.max_abs_user = c->max_abs_user;
+    d.screen_abs = c->screen_abs;
+    return d;
 }
 
 // Grow-only scratch buffer.
@@ -172,22 +213,6 @@ void normalize_rows(float *u, int64_t n, int32_t d) {
         for (int32_t j = 0; j < d; ++j) row[j] = row[j] / l2;
     }
 }
-
-// Largest |x| over a row-major matrix; -1 when a value is NaN or infinite.
-double max_abs(const float *x, size_t n) {
-    double m = 0.0;
-    for (size_t i = 0; i < n; ++i) {
-        const float v = x[i];
-        if (!std::isfinite(v)) return -1.0;
-        const double a = std::fabs((double)v);
-        if (a > m) m = a;
-    }
-    return m;
-}
-
-// Every partial sum of a score is bounded by dim * max|a| * max|b|; keep it
-// well inside FP32 range so no score overflows.
-constexpr double kScoreLimit = 1e37;
 
 bool ranks_valid(const int32_t *r, int32_t S, int64_t n_item) {
     if (S <= 0 || (int64_t)S >= n_item) return false;
@@ -386,9 +411,7 @@ int rkr_set_data(rkr_ctx *c, const float *users, int64_t n_user, const float *it
         const double g_mfma = n3 * e23 / (1.0 - n3 * e23), g_rule = dim * e24 / (1.0 - dim * e24);
         const double C = (3.2 * std::ldexp(1.0, -16) + 1.03 * g_mfma + g_rule) * 1.01;
         c->screen_C = std::nextafter((float)C, INFINITY);
-        // flushed denormal products: each < 2^-126 * max|other factor|
-        const double absf = 2.0 * n3 * std::ldexp(1.0, -126) * (1.0 + c->max_abs_user) * (1.0 + c->max_abs_item);
-        c->screen_abs = std::nextafter((float)absf, INFINITY);
+        c->screen_abs = screen_abs_floor(kp, c->max_abs_user, c->max_abs_item);
         if ((rc = dev_alloc_t(c, &c->item_hi, (size_t)n_item * kp, "item hi")) ||
             (rc = dev_alloc_t(c, &c->item_lo, (size_t)n_item * kp, "item lo")) ||
             (rc = dev_alloc_t(c, &c->item_norm, (size_t)n_item, "item norms")) ||
@@ -543,7 +566,8 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
               int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
               rkr_batch_timing *timing) {
     if (!c) return RKR_ERR_INVALID;
-    int rc = validate_queries(c, queries, n_query, topk, max_k);
+    const DataSnap snap = snap_of(c);
+    int rc = validate_queries(c, snap, queries, n_query, topk, max_k);
     if (rc) return rc;
     if (!out_user || !out_rank) return set_err(c, RKR_ERR_INVALID, "query: null output buffer");
     if (timing) std::memset(timing, 0, sizeof(*timing));
@@ -551,7 +575,8 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
     if (!c->dedup || n_query < 2)
-        return query_on_slot(c, c->main, queries, n_query, topk, max_k, out_user, out_rank, out_qip, stats, timing);
+        return query_on_slot(c, c->main, snap, queries, n_query, topk, max_k, out_user, out_rank, out_qip, stats,
+                             timing);
 
     // Identical query vectors (byte-equal) are answered once, with the largest
     // k asked for them: the answer for k is the first k entries of the answer
@@ -568,7 +593,8 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
     }
     const int32_t n_rep = (int32_t)reps.size();
     if (n_rep == n_query)
-        return query_on_slot(c, c->main, queries, n_query, topk, max_k, out_user, out_rank, out_qip, stats, timing);
+        return query_on_slot(c, c->main, snap, queries, n_query, topk, max_k, out_user, out_rank, out_qip, stats,
+                             timing);
     std::vector<float> rq((size_t)n_rep * c->dim);
     std::vector<int32_t> rk((size_t)n_rep, 0);
     for (int32_t j = 0; j < n_rep; ++j) std::memcpy(&rq[(size_t)j * c->dim], queries + (size_t)reps[j] * c->dim, row);
@@ -576,7 +602,7 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
     std::vector<int32_t> ru((size_t)n_rep * max_k), rr((size_t)n_rep * max_k);
     std::vector<float> rqip((size_t)n_rep * max_k);
     std::vector<rkr_query_stats> rst(stats ? (size_t)n_rep : 0);
-    rc = query_on_slot(c, c->main, rq.data(), n_rep, rk.data(), max_k, ru.data(), rr.data(), rqip.data(),
+    rc = query_on_slot(c, c->main, snap, rq.data(), n_rep, rk.data(), max_k, ru.data(), rr.data(), rqip.data(),
                        stats ? rst.data() : nullptr, timing);
     if (rc) return rc;
     for (int32_t i = 0; i < n_query; ++i) {
@@ -599,37 +625,38 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
 
 namespace rkr {
 
-int validate_queries(Ctx *c, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k) {
-    if (!c->table) return set_err(c, RKR_ERR_STATE, "query: no table (build or load one first)");
+int validate_queries(Ctx *c, const DataSnap &d, const float *queries, int32_t n_query, const int32_t *topk,
+                     int32_t max_k) {
+    if (!d.table) return set_err(c, RKR_ERR_STATE, "query: no table (build or load one first)");
     if (n_query < 0 || (n_query > 0 && (!queries || !topk)))
         return set_err(c, RKR_ERR_INVALID, "query: null buffer");
     if (max_k <= 0 || max_k > kMaxTopk) return set_err(c, RKR_ERR_INVALID, "query: max_k must be in [1, 4096]");
     for (int32_t i = 0; i < n_query; ++i)
-        if (topk[i] <= 0 || topk[i] > max_k || (int64_t)topk[i] > c->n_user)
+        if (topk[i] <= 0 || topk[i] > max_k || (int64_t)topk[i] > d.n_user)
             return set_err(c, RKR_ERR_INVALID, "query: topk[" + std::to_string(i) + "] = " + std::to_string(topk[i]) +
                                                    " outside [1, min(n_user, max_k)]");
     if (n_query > 0) {
         const double mq = max_abs(queries, (size_t)n_query * c->dim);
         if (mq < 0) return set_err(c, RKR_ERR_INVALID, "query: query vectors must be finite");
-        if ((double)c->dim * mq * c->max_abs_user > kScoreLimit)
+        if ((double)c->dim * mq * d.max_abs_user > kScoreLimit)
             return set_err(c, RKR_ERR_INVALID, "query: magnitudes would overflow FP32 scores");
     }
-    if (c->output == RKR_OUTPUT_REFERENCE && (c->n_ins > 0 || c->n_del > 0))
+    if (c->output == RKR_OUTPUT_REFERENCE && (d.n_ins > 0 || d.n_del > 0))
         return set_err(c, RKR_ERR_STATE, "query: reference output has no update path; rebuild the table or use exact output");
     return RKR_OK;
 }
 
-int query_on_slot(Ctx *c, Slot &sl, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k,
-                  int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
+int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int32_t n_query, const int32_t *topk,
+                  int32_t max_k, int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
                   rkr_batch_timing *timing) {
     using clock = std::chrono::steady_clock;
     const auto t_start = clock::now();
-    const int64_t U = c->n_user, I = c->n_item;
+    const int64_t U = dv.n_user, I = dv.n_item;
     const int32_t S = c->n_sample, ld = c->ld, d = c->dim;
     const int mode = c->output;
     const float eps = mode == RKR_OUTPUT_EXACT ? c->table_eps : 0.0f;
     // pending item inserts/deletes: bounds become per-user values (k_update.hip)
-    const bool upd = c->n_ins > 0 || c->n_del > 0;
+    const bool upd = dv.n_ins > 0 || dv.n_del > 0;
     const size_t pair_bytes = upd ? 12 : 8;  // <u,q> + buckets, or + two bound values
 
     // queries per device batch
@@ -677,16 +704,16 @@ int query_on_slot(Ctx *c, Slot &sl, const float *queries, int32_t n_query, const
         RKR_HIP(c, hipEventRecord(sl.ev[1], sl.stream));
         // <u, q> for every (query, user): qip[b * U + u]
         if (nq <= 16 && (size_t)nq * d * sizeof(float) <= 65536)
-            RKR_HIP(c, launch_gemv_panels(sl.stream, c->usersT, U, ld, d, q_in, nq, qip));
-        else RKR_HIP(c, launch_scores_store(sl.stream, c->users, nullptr, U, ld, q_in, nq, ld, d, qip, U, false));
+            RKR_HIP(c, launch_gemv_panels(sl.stream, dv.usersT, U, ld, d, q_in, nq, qip));
+        else RKR_HIP(c, launch_scores_store(sl.stream, dv.users, nullptr, U, ld, q_in, nq, ld, d, qip, U, false));
         RKR_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
         int32_t *lbv = reinterpret_cast<int32_t *>(bk);
         int32_t *ubv = lbv + (size_t)nq * U;
         if (upd)
-            RKR_HIP(c, launch_bounds_upd(sl.stream, qip, c->table, U, S, c->ranks, c->n_item_base, eps, c->side_ins,
-                                         c->n_ins, c->side_del, c->n_del, nq, lbv, ubv));
+            RKR_HIP(c, launch_bounds_upd(sl.stream, qip, dv.table, U, S, c->ranks, dv.n_item_base, eps, dv.side_ins,
+                                         dv.n_ins, dv.side_del, dv.n_del, nq, lbv, ubv));
         else
-            RKR_HIP(c, launch_bounds(sl.stream, qip, c->table, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
+            RKR_HIP(c, launch_bounds(sl.stream, qip, dv.table, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
         RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
         const int64_t cu = chunk_users(U, nq, c->n_cu);
         uint32_t *ghist = reinterpret_cast<uint32_t *>(sl.hist.p);
@@ -742,21 +769,21 @@ int query_on_slot(Ctx *c, Slot &sl, const float *queries, int32_t n_query, const
             // events 4..8 bracket exactly the refinement (or screening) kernel
             if (screened) {
                 ScreenLaunch L{};
-                L.users = c->users; L.user_norm = c->user_norm;
-                L.item_hi = c->item_hi; L.item_lo = c->item_lo; L.item_norm = c->item_norm;
+                L.users = dv.users; L.user_norm = dv.user_norm;
+                L.item_hi = dv.item_hi; L.item_lo = dv.item_lo; L.item_norm = dv.item_norm;
                 L.n_item = I; L.ld = ld; L.K = d; L.Kp = c->kp;
                 L.rows_user = refine_user; L.rows_thr = refine_thr; L.n_rows_dev = refine_total;
-                L.max_rows = (int64_t)rcap; L.C = c->screen_C; L.abs_floor = c->screen_abs;
+                L.max_rows = (int64_t)rcap; L.C = c->screen_C; L.abs_floor = dv.screen_abs;
                 L.cnt = cnt; L.recheck_n = recheck_n; L.recheck = sl.recheck.p; L.recheck_cap = rec_cap;
                 L.target_units = (int64_t)c->n_cu * 8;
                 L.grid_cap = (int64_t)c->n_cu * 4;
                 L.chunk_items = c->screen_chunk > 0 ? c->screen_chunk : (1 << 30);
                 RKR_HIP(c, launch_refine_screen(sl.stream, L));
                 RKR_HIP(c, hipEventRecord(sl.ev[8], sl.stream));
-                RKR_HIP(c, launch_recheck(sl.stream, c->users, c->items, ld, d, refine_user, refine_thr, refine_query,
+                RKR_HIP(c, launch_recheck(sl.stream, dv.users, dv.items, ld, d, refine_user, refine_thr, refine_query,
                                           recheck_n, sl.recheck.p, rec_cap, cnt, q_recheck, max_err_bits));
             } else {
-                RKR_HIP(c, launch_refine_count(sl.stream, c->users, c->items, I, ld, d, refine_user, refine_thr,
+                RKR_HIP(c, launch_refine_count(sl.stream, dv.users, dv.items, I, ld, d, refine_user, refine_thr,
                                                refine_total, (int64_t)rcap, cnt));
                 RKR_HIP(c, hipEventRecord(sl.ev[8], sl.stream));
             }

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cmath>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -95,7 +96,60 @@ struct Ctx {
     // query scratch of rkr_query (grown on demand) and build scratch
     Slot main;
     DeviceBuf misc;
+
+    // buffers an update thread freed while batches may still read them
+    // (snapshot-mode server): released once every batch that started before
+    // the update committed has finished (guarded by mem_mu)
+    struct Retired {
+        void *p;
+        size_t bytes;
+    };
+    std::vector<Retired> retired;
 };
+
+// Frees made by this thread go to Ctx::retired instead of hipFree.
+void set_defer_frees(bool on);
+// Releases the retired buffers (call only when no batch can still read them).
+void release_retired(Ctx *c);
+
+// Largest |x| over a row-major matrix; -1 when a value is NaN or infinite.
+inline double max_abs(const float *x, size_t n) {
+    double m = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const float v = x[i];
+        if (!std::isfinite(v)) return -1.0;
+        const double a = std::fabs((double)v);
+        if (a > m) m = a;
+    }
+    return m;
+}
+
+// Every partial sum of a score is bounded by dim * max|a| * max|b|; keep it
+// well inside FP32 range so no score overflows.
+constexpr double kScoreLimit = 1e37;
+
+// Absolute error floor of the screened form: products a matrix core may
+// flush as denormals, each < 2^-126 * max|other factor|, over 3*kp terms.
+inline float screen_abs_floor(int32_t kp, double max_abs_user, double max_abs_item) {
+    const double absf = 2.0 * 3.0 * kp * std::ldexp(1.0, -126) * (1.0 + max_abs_user) * (1.0 + max_abs_item);
+    return std::nextafter((float)absf, INFINITY);
+}
+
+// Everything a query batch reads from the data and table state, captured
+// when the batch starts.  Outside a snapshot-mode server it is the context's
+// current state; in one, the last committed update's state, so updates can
+// be prepared while batches run.
+struct DataSnap {
+    int64_t n_user = 0, n_item = 0, n_item_base = 0;
+    const float *users = nullptr, *usersT = nullptr, *items = nullptr;
+    const float *user_norm = nullptr, *item_norm = nullptr;
+    const uint16_t *item_hi = nullptr, *item_lo = nullptr;
+    const float *table = nullptr, *side_ins = nullptr, *side_del = nullptr;
+    int32_t n_ins = 0, n_del = 0;
+    double max_abs_user = 1.0;
+    float screen_abs = 0.0f;
+};
+DataSnap snap_of(const Ctx *c);
 
 // ---- error and memory helpers (rkr_api.cpp) --------------------------------
 int set_err(Ctx *c, int code, const std::string &msg);
@@ -105,10 +159,11 @@ int ensure(Ctx *c, DeviceBuf &b, size_t bytes, const char *what);
 int slot_init(Ctx *c, Slot &sl, bool own_stream);
 void slot_free(Ctx *c, Slot &sl);
 // Runs validated queries through one slot (batches sized by free memory).
-int query_on_slot(Ctx *c, Slot &sl, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k,
-                  int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
+int query_on_slot(Ctx *c, Slot &sl, const DataSnap &d, const float *queries, int32_t n_query, const int32_t *topk,
+                  int32_t max_k, int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
                   rkr_batch_timing *timing);
-int validate_queries(Ctx *c, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k);
+int validate_queries(Ctx *c, const DataSnap &d, const float *queries, int32_t n_query, const int32_t *topk,
+                     int32_t max_k);
 template <class T>
 int dev_alloc_t(Ctx *c, T **p, size_t n, const char *what) {
     return dev_alloc(c, reinterpret_cast<void **>(p), n * sizeof(T), what);

@@ -1,22 +1,33 @@
 // rkr_server.cpp -- the serving loop: continuous batching of reverse k-ranks
 // queries (mixed k) interleaved with user / item updates, on one context.
 //
-// Requests enter one FIFO.  Worker threads, each owning a Slot (its own HIP
+// Queries enter a FIFO.  Worker threads, each owning a Slot (its own HIP
 // stream and scratch buffers), take the longest run of queries at the front
 // of the queue (up to max_batch), so requests that arrive while a batch is on
 // the device are admitted by the next free worker and the batches overlap on
 // the GPU.  Batches are further split by free device memory inside
-// query_on_slot.  An update request is a barrier: it starts only when no
-// batch is in flight, and no batch starts while it runs, so every query sees
-// exactly the updates submitted before it.  Answers do not depend on how
-// queries are batched (every kernel decides each (query, user) pair on its
-// own), which the tests check by comparing answers alone and in the mix.
+// query_on_slot.  Answers do not depend on how queries are batched (every
+// kernel decides each (query, user) pair on its own), which the tests check
+// by comparing answers alone and in the mix.
+//
+// Updates, in one of two modes:
+//  - barrier (default): an update waits in the same FIFO, starts when no
+//    batch is in flight, and no batch starts while it runs, so every query
+//    sees exactly the updates submitted before it;
+//  - snapshot: an updater thread applies updates in submission order while
+//    batches keep running.  A batch reads the state the last committed update
+//    left (captured when it starts); the update's frees are deferred until
+//    every batch that started before its commit has finished.  A query then
+//    sees every update that committed before its batch started.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <string>
 #include <thread>
@@ -63,19 +74,33 @@ struct rkr_server {
     std::vector<std::unique_ptr<Slot>> slots;
     std::vector<std::thread> workers;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::deque<Request *> queue;
+    std::condition_variable cv_work, cv_done, cv_upd;
+    std::deque<Request *> queue;   // queries (and updates in barrier mode)
+    std::deque<Request *> uqueue;  // updates (snapshot mode)
+    std::thread updater;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     uint64_t next_id = 1;
     int busy = 0;            // query batches in flight
-    bool exclusive = false;  // an update is running
+    bool exclusive = false;  // an update is running (barrier mode)
     bool stopping = false;
     int64_t n_batches = 0, n_queries = 0, n_updates = 0;
+    // snapshot mode: the committed state, its version, and the versions the
+    // batches in flight started from (version -> batches)
+    DataSnap published;
+    uint64_t version = 0;
+    std::map<uint64_t, int> inflight;
+
+    bool trace = false;      // RKR_SERVER_TRACE=1: one stderr line per snapshot update
+
+    bool snapshot_mode() const { return cfg.update_mode == RKR_SERVE_SNAPSHOT; }
 
     bool can_start() const {
         if (exclusive || queue.empty()) return false;
         return queue.front()->kind == Q_QUERY || busy == 0;
     }
+
+    // no batch in flight started before version v
+    bool drained_before(uint64_t v) const { return inflight.empty() || inflight.begin()->first >= v; }
 
     int apply_update(Request *r) {
         switch (r->kind) {
@@ -87,13 +112,13 @@ struct rkr_server {
         }
     }
 
-    void run_batch(Slot &sl, std::vector<Request *> &all) {
+    void run_batch(Slot &sl, const DataSnap &snap, std::vector<Request *> &all) {
         // a request that fails validation is answered alone with its error,
         // so it cannot fail the other queries of its batch
         std::vector<Request *> batch;
         for (Request *r : all) {
             const int32_t k = r->k;
-            const int rc = validate_queries(c, r->data.data(), 1, &k, std::max(k, 1));
+            const int rc = validate_queries(c, snap, r->data.data(), 1, &k, std::max(k, 1));
             if (!rc) {
                 batch.push_back(r);
                 continue;
@@ -125,7 +150,7 @@ struct rkr_server {
         std::vector<float> oq((size_t)nq * max_k);
         std::vector<rkr_query_stats> st((size_t)nq);
         rkr_batch_timing tm{};
-        const int rc = query_on_slot(c, sl, q.data(), nq, ks.data(), max_k, ou.data(), orank.data(), oq.data(),
+        const int rc = query_on_slot(c, sl, snap, q.data(), nq, ks.data(), max_k, ou.data(), orank.data(), oq.data(),
                                      st.data(), &tm);
         std::string err;
         if (rc) {
@@ -190,12 +215,64 @@ struct rkr_server {
                 queue.pop_front();
             }
             ++busy;
+            const uint64_t v = version;
+            const DataSnap snap = snapshot_mode() ? published : snap_of(c);
+            if (snapshot_mode()) ++inflight[v];
             lk.unlock();
-            run_batch(sl, batch);
+            run_batch(sl, snap, batch);
             lk.lock();
             --busy;
+            if (snapshot_mode() && --inflight[v] == 0) inflight.erase(v);
             cv_done.notify_all();
             cv_work.notify_all();
+        }
+    }
+
+    // Snapshot mode: applies updates in order while batches run, commits each
+    // by publishing the new state, then frees what it replaced once the
+    // batches that may read the old state have finished.
+    void update_loop() {
+        (void)hipSetDevice(c->device);
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+            cv_upd.wait(lk, [&] { return stopping || !uqueue.empty(); });
+            if (uqueue.empty()) break;  // stopping with nothing left to apply
+            Request *r = uqueue.front();
+            uqueue.pop_front();
+            r->t_start = now_s();
+            lk.unlock();
+            set_defer_frees(true);
+            const int rc = apply_update(r);
+            set_defer_frees(false);
+            const double t_applied = now_s();
+            std::string err;
+            if (rc) {
+                std::lock_guard<std::mutex> g(c->err_mu);
+                err = c->err;
+            }
+            lk.lock();
+            published = snap_of(c);
+            const uint64_t v = ++version;
+            r->status = rc;
+            r->err = err;
+            r->t_done = now_s();
+            r->done = true;
+            ++n_updates;
+            cv_done.notify_all();
+            cv_done.wait(lk, [&] { return drained_before(v); });
+            const double t_drained = now_s();
+            lk.unlock();
+            size_t freed = 0;
+            {
+                std::lock_guard<std::mutex> g(c->mem_mu);
+                for (const auto &x : c->retired) freed += x.bytes;
+            }
+            release_retired(c);
+            if (trace)
+                std::fprintf(stderr, "[rkr-server] update kind=%d n=%lld apply_ms=%.2f drain_ms=%.2f free_ms=%.2f freed_mb=%.1f\n",
+                             r->kind, (long long)r->n, 1e3 * (t_applied - r->t_start), 1e3 * (t_drained - t_applied),
+                             1e3 * (now_s() - t_drained), freed / 1048576.0);
+            lk.lock();
         }
     }
 
@@ -205,9 +282,14 @@ struct rkr_server {
         r->id = next_id++;
         r->t_submit = now_s();
         *ticket = r->id;
-        queue.push_back(r.get());
+        if (snapshot_mode() && r->kind != Q_QUERY) {
+            uqueue.push_back(r.get());
+            cv_upd.notify_one();
+        } else {
+            queue.push_back(r.get());
+            cv_work.notify_one();
+        }
         reqs.emplace(r->id, std::move(r));
-        cv_work.notify_one();
         return RKR_OK;
     }
 };
@@ -225,6 +307,14 @@ int rkr_server_start(rkr_ctx *c, const rkr_server_config *cfg, rkr_server **out)
     s->c = c;
     s->cfg.max_batch = cfg && cfg->max_batch > 0 ? cfg->max_batch : 256;
     s->cfg.n_slots = cfg && cfg->n_slots > 0 ? std::min(cfg->n_slots, 4) : 2;
+    s->cfg.update_mode = cfg ? cfg->update_mode : RKR_SERVE_BARRIER;
+    if (s->cfg.update_mode != RKR_SERVE_BARRIER && s->cfg.update_mode != RKR_SERVE_SNAPSHOT) {
+        delete s;
+        return set_err(c, RKR_ERR_INVALID, "server: unknown update mode");
+    }
+    s->published = snap_of(c);
+    const char *tr = std::getenv("RKR_SERVER_TRACE");
+    s->trace = tr && tr[0] == '1';
     (void)hipSetDevice(c->device);
     for (int i = 0; i < s->cfg.n_slots; ++i) {
         s->slots.emplace_back(new Slot());
@@ -236,6 +326,7 @@ int rkr_server_start(rkr_ctx *c, const rkr_server_config *cfg, rkr_server **out)
         }
     }
     for (int i = 0; i < s->cfg.n_slots; ++i) s->workers.emplace_back(&rkr_server::worker, s, i);
+    if (s->snapshot_mode()) s->updater = std::thread(&rkr_server::update_loop, s);
     *out = s;
     return RKR_OK;
 }
@@ -326,8 +417,11 @@ int rkr_server_stop(rkr_server *s) {
         std::lock_guard<std::mutex> g(s->mu);
         s->stopping = true;
         s->cv_work.notify_all();
+        s->cv_upd.notify_all();
     }
     for (auto &t : s->workers) t.join();
+    if (s->updater.joinable()) s->updater.join();
+    release_retired(s->c);
     for (auto &sl : s->slots) slot_free(s->c, *sl);
     delete s;
     return RKR_OK;

@@ -40,12 +40,6 @@ int upload_padded(Ctx *c, const float *src, int64_t n, float **dst, const char *
     return RKR_OK;
 }
 
-int check_finite(Ctx *c, const float *x, size_t n, const char *what) {
-    for (size_t i = 0; i < n; ++i)
-        if (!std::isfinite(x[i])) return set_err(c, RKR_ERR_INVALID, std::string(what) + " must be finite");
-    return RKR_OK;
-}
-
 int check_updatable(Ctx *c) {
     if (!c->users) return set_err(c, RKR_ERR_STATE, "update: no data");
     if (!c->table) return set_err(c, RKR_ERR_STATE, "update: no table (build or load one first)");
@@ -162,8 +156,13 @@ int append_rows(Ctx *c, float **arr, int64_t n_old, int64_t w, const float *dev_
     float *next = nullptr;
     int rc = dev_alloc_t(c, &next, (size_t)(n_old + n_new) * w, what);
     if (rc) return rc;
-    if (n_old) RKR_HIP(c, hipMemcpy(next, *arr, sizeof(float) * (size_t)n_old * w, hipMemcpyDeviceToDevice));
-    if (n_new) RKR_HIP(c, hipMemcpy(next + n_old * w, dev_rows, sizeof(float) * (size_t)n_new * w, hipMemcpyDeviceToDevice));
+    // on the context stream, after the kernels that produced dev_rows
+    if (n_old)
+        RKR_HIP(c, hipMemcpyAsync(next, *arr, sizeof(float) * (size_t)n_old * w, hipMemcpyDeviceToDevice, c->stream));
+    if (n_new)
+        RKR_HIP(c, hipMemcpyAsync(next + n_old * w, dev_rows, sizeof(float) * (size_t)n_new * w,
+                                  hipMemcpyDeviceToDevice, c->stream));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
     dev_free(c, *arr, sizeof(float) * (size_t)std::max<int64_t>(n_old, 1) * w);
     *arr = next;
     return RKR_OK;
@@ -200,12 +199,18 @@ int rkr_insert_items(rkr_ctx *c, const float *items, int64_t n) {
     if (c->output == RKR_OUTPUT_REFERENCE)
         return set_err(c, RKR_ERR_STATE, "insert_items: the reference's US method has no update path; use exact output");
     if (n == 0) return RKR_OK;
-    if ((rc = check_finite(c, items, (size_t)n * c->dim, "insert_items: items"))) return rc;
+    const double mp = max_abs(items, (size_t)n * c->dim);
+    if (mp < 0) return set_err(c, RKR_ERR_INVALID, "insert_items: items must be finite");
+    const double mp_new = std::max(c->max_abs_item, mp);
+    if ((double)c->dim * c->max_abs_user * mp_new > kScoreLimit)
+        return set_err(c, RKR_ERR_INVALID, "insert_items: magnitudes would overflow FP32 scores");
     if ((int64_t)c->n_ins + n > kMaxSideItems)
         return set_err(c, RKR_ERR_NOMEM, "insert_items: too many pending inserts; rebuild the table");
     if (c->n_item_base + c->n_ins + n >= kMaxBoundValue) return set_err(c, RKR_ERR_INVALID, "insert_items: too many items");
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
+    c->max_abs_item = mp_new;
+    if (c->kp) c->screen_abs = screen_abs_floor(c->kp, c->max_abs_user, c->max_abs_item);
     float *X = nullptr;
     if ((rc = upload_padded(c, items, n, &X, "inserted items"))) return rc;
     rc = update_side(c, &c->side_ins, &c->n_ins, X, n, +1);
@@ -292,16 +297,21 @@ int rkr_insert_users(rkr_ctx *c, const float *users, int64_t n) {
     int rc = check_updatable(c);
     if (rc) return rc;
     if (n == 0) return RKR_OK;
-    if ((rc = check_finite(c, users, (size_t)n * c->dim, "insert_users: users"))) return rc;
+    const double mu = max_abs(users, (size_t)n * c->dim);
+    if (mu < 0) return set_err(c, RKR_ERR_INVALID, "insert_users: users must be finite");
     if (c->n_user + n > INT32_MAX) return set_err(c, RKR_ERR_INVALID, "insert_users: too many users");
+    // normalised rows have |u_i| <= 1
+    const double mu_raw = std::max(c->max_abs_user_raw, mu);
+    const double mu_new = c->normalize_users ? c->max_abs_user : mu_raw;
+    if ((double)c->dim * mu_new * c->max_abs_item > kScoreLimit)
+        return set_err(c, RKR_ERR_INVALID, "insert_users: magnitudes would overflow FP32 scores");
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
+    c->max_abs_user_raw = mu_raw;
+    c->max_abs_user = mu_new;
+    if (c->kp) c->screen_abs = screen_abs_floor(c->kp, c->max_abs_user, c->max_abs_item);
     std::vector<float> h(users, users + (size_t)n * c->dim);
     if (c->normalize_users) rkr_normalize_users(h.data(), n, c->dim);  // as readData does on load
-    if (!c->normalize_users) {
-        for (float v : h) c->max_abs_user_raw = std::max(c->max_abs_user_raw, (double)std::fabs(v));
-        c->max_abs_user = c->max_abs_user_raw;
-    }
     const int64_t U0 = c->n_user, ld = c->ld, S = c->n_sample;
     float *X = nullptr;
     if ((rc = upload_padded(c, h.data(), n, &X, "inserted users"))) return rc;

@@ -70,7 +70,11 @@ class BatchTiming(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("max_batch", C.c_int32), ("n_slots", C.c_int32)]
+    _fields_ = [("max_batch", C.c_int32), ("n_slots", C.c_int32), ("update_mode", C.c_int32)]
+
+
+RKR_SERVE_BARRIER = 0
+RKR_SERVE_SNAPSHOT = 1
 
 
 class RequestInfo(C.Structure):

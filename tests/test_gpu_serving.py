@@ -102,3 +102,45 @@ def test_bad_request_does_not_stop_the_loop():
             srv.wait(bad, 5000)
         u, r, _, info = srv.wait(ok, 5)
         assert info["status"] == 0 and len(u) == 5
+
+
+def test_snapshot_updates(orc):
+    """Snapshot mode: queries keep running while an update is applied; each
+    answer equals brute force over the state before or after the update, and
+    queries submitted after the update committed see it."""
+    _case("serving snapshot updates")
+    import rkr
+    rng, users, items, idx = _setup(seed=44, U=5000, I=2200)
+    qs = items[rng.choice(len(items), 24, replace=False)]
+    k = 35
+    new_users = rng.standard_normal((800, users.shape[1]), dtype=np.float32)
+    new_items = rng.standard_normal((300, items.shape[1]), dtype=np.float32) * np.float32(1.2)
+    un0 = orc.normalize_users(users)
+    un1 = np.concatenate([un0, orc.normalize_users(new_users)])
+    items2 = np.concatenate([items, new_items])
+    states = [(un0, items), (un1, items), (un1, items2)]
+    truth = []
+    for U_, I_ in states:
+        bu, br, _ = orc.bruteforce(U_, I_, qs, k)
+        truth.append((bu, br))
+
+    def which(u, r, i):
+        return [s for s, (bu, br) in enumerate(truth) if np.array_equal(u, bu[i]) and np.array_equal(r, br[i])]
+
+    with rkr.Server(idx, max_batch=8, n_slots=2, update_mode="snapshot") as srv:
+        t_u = srv.submit_update("insert_users", vectors=new_users)
+        during = [srv.submit_query(q, k) for q in qs]
+        t_i = srv.submit_update("insert_items", vectors=new_items)
+        during += [srv.submit_query(q, k) for q in qs]
+        for j, t in enumerate(during):
+            u, r, _, info = srv.wait(t, k)
+            assert info["status"] == 0
+            assert which(u, r, j % len(qs)), f"query {j}: answer matches no committed state"
+        assert srv.wait(t_u)[3]["status"] == 0 and srv.wait(t_i)[3]["status"] == 0
+        after = [srv.submit_query(q, k) for q in qs]
+        for j, t in enumerate(after):
+            u, r, _, _ = srv.wait(t, k)
+            assert which(u, r, j) == [2], f"query {j} after both commits does not see them"
+        cnt = srv.counters()
+        assert cnt["n_updates"] == 2
+    assert idx.counts()["n_user"] == 5800 and idx.counts()["n_item"] == 2500

@@ -113,17 +113,19 @@ def test_insert_and_delete_users(orc, gpu_index_factory):
     m.check(idx, queries, 25)
 
 
-@pytest.mark.parametrize("nq", [1, 12])
-def test_updates_many_users_chunked(orc, gpu_index_factory, nq):
+@pytest.mark.parametrize("nq,inserts", [(1, True), (12, True), (6, False)])
+def test_updates_many_users_chunked(orc, gpu_index_factory, nq, inserts):
     """Enough users that the prune kernels split each query over several
-    chunks, with deletes that make some shifted upper bounds negative."""
-    _case(f"updates, chunked prune, nq={nq}")
+    chunks, with deletes that make some shifted upper bounds negative (and a
+    delete-only case: side arrays of deletes without inserts)."""
+    _case(f"updates, chunked prune, nq={nq}, inserts={inserts}")
     rng, users, items = _data(35, U=13000, I=900, d=16)
     idx, _ = gpu_index_factory(users, items, 24)
     m = Mirror(orc, users, items)
     new = rng.standard_normal((40, items.shape[1]), dtype=np.float32) * np.float32(1.4)
-    idx.insert_items(new)
-    m.insert_items(new)
+    if inserts:
+        idx.insert_items(new)
+        m.insert_items(new)
     top = np.argsort(-np.linalg.norm(items, axis=1))[:25].astype(np.int64)
     idx.delete_items(np.sort(top))
     m.delete_items(np.sort(top))
