@@ -34,6 +34,7 @@ PEAK_BF16_TFLOPS = 2500.0
 WORKLOADS = {
     # name: (synth config, k, label)
     "plain_gpu": ("plain_gpu", 50, "plain_gpu: 100k users x 20k items x d=150, S=512, 1000 queries, k=50"),
+    "plain_cpu": ("plain_cpu", 10, "plain_cpu: 1k users x 5k items x d=30, S=128, 100 queries, k=10"),
     "precision": ("precision", 50, "precision: 1M users x 50k items x d=150, S=512, 1000 queries, k=50"),
     "batching": ("batching", None, "batching: 1M users x 50k items x d=150, S=512, 10000 Zipf(1.1) queries, k in [10,200]"),
     "serving": ("serving", None, "serving: 2M users x 100k items x d=128, S=512, Poisson arrivals, k in [10,200], "
@@ -47,7 +48,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plain_gpu", choices=sorted(WORKLOADS))
-    ap.add_argument("--form", default="plain", choices=["plain", "screened"])
+    ap.add_argument("--form", default="screened", choices=["plain", "screened"],
+                    help="screened: BF16 matrix-core screen + exact recheck (same answers as plain)")
     ap.add_argument("--n-query", type=int, default=None)
     ap.add_argument("--latency-queries", type=int, default=200)
     ap.add_argument("--cpu-sample", type=int, default=None, help="queries in the CPU baseline sample")
@@ -157,7 +159,7 @@ def serving(args, idx, wl, label):
     out = {
         "metric": "serving queries/s with p50/p99 latency (exact ranks), inserts every 1 s",
         "value": round(len(lat) / wall, 2), "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "f32", "data": "synthetic (seeded), open-loop Poisson arrivals",
+        "dtype": "bf16" if args.form == "screened" else "f32", "data": "synthetic (seeded), open-loop Poisson arrivals",
         "config": {"workload": label, "form": args.form, "update_mode": args.update_mode,
                    "arrival_rate": round(rate, 1), "saturating_rate": round(sat, 1),
                    "saturation": f"closed loop, {window} outstanding, {args.sat_seconds:g} s, same insert stream",
@@ -315,7 +317,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16" if args.form == "screened" else "f32",
+        "dtype_note": ("refinement screen on BF16 matrix cores (3-product hi/lo split, FP32 accumulation), "
+                       "undecided items rechecked with the exact FP32 rule; answers identical to the f32 form"
+                       if args.form == "screened" else "exact sequential FP32 score rule"),
         "data": "synthetic (seeded), inputs resident in HBM",
         "config": {"workload": label, "form": args.form, "queries_per_step": nq,
                    "distinct_queries_per_step": int(len(np.unique(wl.queries, axis=0))),
