@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) k_bounds_lds(const float *__restrict__ qi
     extern __shared__ float rows[];  // ut x (S + 1)
     const int64_t u0 = (int64_t)blockIdx.x * ut;
     const int64_t nu = min((int64_t)ut, n_user - u0);
-    stage_rows(rows, table + u0 * S, nu, S);
+    stage_rows(rows, table + u0 * S, nu, S, S);
     __syncthreads();
     const int ul = threadIdx.x % ut;
     const int lanes_q = 256 / ut;

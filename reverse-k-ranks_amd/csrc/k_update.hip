@@ -171,18 +171,19 @@ __device__ __forceinline__ void bounds_values(const float *row, int32_t S, float
     ub = max((bhi == 0 ? 0 : ranks[bhi - 1] + 1) + off, 0);
 }
 
-// grid (user blocks, queries)
+// grid (user blocks, queries); side rows at strides ld_ins / ld_del
 __global__ void __launch_bounds__(256) k_bounds_upd(const float *__restrict__ qip, const float *__restrict__ table,
                                                     int64_t n_user, int32_t S, const int32_t *__restrict__ ranks,
                                                     int64_t n_item_base, float eps, const float *__restrict__ ins,
-                                                    int32_t n_ins, const float *__restrict__ del, int32_t n_del,
-                                                    int32_t *__restrict__ lbs, int32_t *__restrict__ ubs) {
+                                                    int32_t n_ins, int32_t ld_ins, const float *__restrict__ del,
+                                                    int32_t n_del, int32_t ld_del, int32_t *__restrict__ lbs,
+                                                    int32_t *__restrict__ ubs) {
     const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t b = blockIdx.y;
     if (u >= n_user) return;
     int32_t lb, ub;
-    bounds_values(table + u * (int64_t)S, S, qip[b * n_user + u], eps, ranks, n_item_base, ins + u * (int64_t)n_ins,
-                  n_ins, del + u * (int64_t)n_del, n_del, lb, ub);
+    bounds_values(table + u * (int64_t)S, S, qip[b * n_user + u], eps, ranks, n_item_base, ins + u * (int64_t)ld_ins,
+                  n_ins, del + u * (int64_t)ld_del, n_del, lb, ub);
     lbs[b * n_user + u] = lb;
     ubs[b * n_user + u] = ub;
 }
@@ -195,18 +196,19 @@ __global__ void __launch_bounds__(256) k_bounds_upd_lds(const float *__restrict_
                                                         int64_t n_user, int32_t S, int32_t nq, int32_t ut,
                                                         int stage_side, const int32_t *__restrict__ ranks,
                                                         int64_t n_item_base, float eps, const float *__restrict__ ins,
-                                                        int32_t n_ins, const float *__restrict__ del, int32_t n_del,
-                                                        int32_t *__restrict__ lbs, int32_t *__restrict__ ubs) {
+                                                        int32_t n_ins, int32_t ld_ins, const float *__restrict__ del,
+                                                        int32_t n_del, int32_t ld_del, int32_t *__restrict__ lbs,
+                                                        int32_t *__restrict__ ubs) {
     // ut x (S + 1), then (stage_side) ut x (n_ins + 1) if n_ins, ut x (n_del + 1) if n_del
     extern __shared__ float rows[];
     float *rows_ins = rows + (int64_t)ut * (S + 1);
     float *rows_del = rows_ins + (n_ins ? (int64_t)ut * (n_ins + 1) : 0);
     const int64_t u0 = (int64_t)blockIdx.x * ut;
     const int64_t nu = min((int64_t)ut, n_user - u0);
-    stage_rows(rows, table + u0 * S, nu, S);
+    stage_rows(rows, table + u0 * S, nu, S, S);
     if (stage_side) {
-        if (n_ins) stage_rows(rows_ins, ins + u0 * n_ins, nu, n_ins);
-        if (n_del) stage_rows(rows_del, del + u0 * n_del, nu, n_del);
+        if (n_ins) stage_rows(rows_ins, ins + u0 * ld_ins, nu, n_ins, ld_ins);
+        if (n_del) stage_rows(rows_del, del + u0 * ld_del, nu, n_del, ld_del);
     }
     __syncthreads();
     const int ul = threadIdx.x % ut;
@@ -214,8 +216,8 @@ __global__ void __launch_bounds__(256) k_bounds_upd_lds(const float *__restrict_
     if (ul >= nu) return;
     const int64_t u = u0 + ul;
     const float *row = rows + (int64_t)ul * (S + 1);
-    const float *ins_row = stage_side ? rows_ins + (int64_t)ul * (n_ins + 1) : ins + u * (int64_t)n_ins;
-    const float *del_row = stage_side ? rows_del + (int64_t)ul * (n_del + 1) : del + u * (int64_t)n_del;
+    const float *ins_row = stage_side ? rows_ins + (int64_t)ul * (n_ins + 1) : ins + u * (int64_t)ld_ins;
+    const float *del_row = stage_side ? rows_del + (int64_t)ul * (n_del + 1) : del + u * (int64_t)ld_del;
     for (int32_t b = threadIdx.x / ut; b < nq; b += lanes_q) {
         int32_t lb, ub;
         bounds_values(row, S, qip[(int64_t)b * n_user + u], eps, ranks, n_item_base, ins_row, n_ins, del_row, n_del,
@@ -433,7 +435,8 @@ hipError_t launch_subtract_rows(hipStream_t s, const float *a, int64_t lda, int3
 
 hipError_t launch_bounds_upd(hipStream_t s, const float *qip, const float *table, int64_t n_user, int32_t S,
                              const int32_t *ranks, int64_t n_item_base, float eps, const float *ins, int32_t n_ins,
-                             const float *del, int32_t n_del, int32_t nq, int32_t *lbs, int32_t *ubs) {
+                             int32_t ld_ins, const float *del, int32_t n_del, int32_t ld_del, int32_t nq,
+                             int32_t *lbs, int32_t *ubs) {
     if (n_user <= 0 || nq <= 0) return hipSuccess;
     if (nq >= 4) {
         // users per workgroup: with their side rows when they fit 40 KB
@@ -451,13 +454,13 @@ hipError_t launch_bounds_upd(hipStream_t s, const float *qip, const float *table
         if (ut > 0) {
             const unsigned grid = (unsigned)((n_user + ut - 1) / ut);
             hipLaunchKernelGGL(k_bounds_upd_lds, dim3(grid), dim3(256), lds, s, qip, table, n_user, S, nq, ut,
-                               stage_side, ranks, n_item_base, eps, ins, n_ins, del, n_del, lbs, ubs);
+                               stage_side, ranks, n_item_base, eps, ins, n_ins, ld_ins, del, n_del, ld_del, lbs, ubs);
             return hipGetLastError();
         }
     }
     dim3 grid((unsigned)((n_user + 255) / 256), (unsigned)nq);
     hipLaunchKernelGGL(k_bounds_upd, grid, dim3(256), 0, s, qip, table, n_user, S, ranks, n_item_base, eps, ins, n_ins,
-                       del, n_del, lbs, ubs);
+                       ld_ins, del, n_del, ld_del, lbs, ubs);
     return hipGetLastError();
 }
 

@@ -42,6 +42,7 @@ int dev_alloc(Ctx *c, void **p, size_t bytes, const char *what) {
         return set_err(c, RKR_ERR_NOMEM, std::string("hipMalloc ") + what + ": " + hipGetErrorString(e));
     }
     c->held += bytes;
+    c->alloc_bytes[*p] = bytes;
     return RKR_OK;
 }
 
@@ -51,9 +52,16 @@ thread_local bool tl_defer_frees = false;
 
 void set_defer_frees(bool on) { tl_defer_frees = on; }
 
+// The size recorded at allocation is what returns to the budget; `bytes` is
+// only used for a pointer this context did not allocate.
 void dev_free(Ctx *c, void *p, size_t bytes) {
     if (!p) return;
     std::lock_guard<std::mutex> g(c->mem_mu);
+    auto it = c->alloc_bytes.find(p);
+    if (it != c->alloc_bytes.end()) {
+        bytes = it->second;
+        c->alloc_bytes.erase(it);
+    }
     if (tl_defer_frees) {
         c->retired.push_back({p, bytes});
         return;
@@ -88,6 +96,8 @@ DataSnap snap_of(const Ctx *c) {
     d.side_del = c->side_del;
     d.n_ins = c->n_ins;
     d.n_del = c->n_del;
+    d.ld_ins = c->ld_ins;
+    d.ld_del = c->ld_del;
     d.max_abs_user = c->max_abs_user;
     d.screen_abs = c->screen_abs;
     return d;
@@ -113,12 +123,13 @@ int ensure(Ctx *c, DeviceBuf &b, size_t bytes, const char *what) {
 namespace {
 
 void free_update_state(Ctx *c) {
-    dev_free(c, c->items_ins, sizeof(float) * (size_t)c->n_ins * c->ld);
-    dev_free(c, c->items_del, sizeof(float) * (size_t)c->n_del * c->ld);
-    dev_free(c, c->side_ins, sizeof(float) * (size_t)std::max<int32_t>(c->n_ins, 1) * c->n_user);
-    dev_free(c, c->side_del, sizeof(float) * (size_t)std::max<int32_t>(c->n_del, 1) * c->n_user);
+    for (float *p : {c->items_ins, c->items_del, c->side_ins, c->side_del, c->side_ins_spare, c->side_del_spare})
+        dev_free(c, p, 0);
     c->items_ins = c->items_del = c->side_ins = c->side_del = nullptr;
+    c->side_ins_spare = c->side_del_spare = nullptr;
     c->n_ins = c->n_del = 0;
+    c->ld_ins = c->ld_del = 0;
+    c->ins_cap = c->del_cap = 0;
     if (c->items_base && c->items_base != c->items)
         dev_free(c, c->items_base, sizeof(float) * (size_t)c->n_item_base * c->ld);
     c->items_base = c->items;
@@ -140,6 +151,9 @@ void free_table(Ctx *c) {
 void free_data(Ctx *c) {
     free_update_state(c);
     free_table(c);
+    dev_free(c, c->upd_scratch.p, 0);
+    c->upd_scratch = DeviceBuf{};
+    c->u_cap = c->i_cap = 0;
     const size_t ld = (size_t)c->ld;
     dev_free(c, c->users, sizeof(float) * (size_t)c->n_user * ld);
     dev_free(c, c->usersT, sizeof(float) * (size_t)((c->n_user + 63) / 64) * 64 * ld);
@@ -371,6 +385,8 @@ int rkr_set_data(rkr_ctx *c, const float *users, int64_t n_user, const float *it
     const int32_t ld = padded_dim(dim);
     c->n_user = n_user;
     c->n_item = n_item;
+    c->u_cap = n_user;
+    c->i_cap = n_item;
     c->normalize_users = normalize_users != 0;
     c->dim = dim;
     c->ld = ld;
@@ -440,7 +456,7 @@ int rkr_build_table(rkr_ctx *c, const int32_t *ranks, int32_t S, double *build_m
     free_update_state(c);  // the new table describes the current item set
     int rc = dev_alloc_t(c, &c->ranks, (size_t)S, "ranks");
     if (rc) return rc;
-    rc = dev_alloc_t(c, &c->table, (size_t)c->n_user * S, "table");
+    rc = dev_alloc_t(c, &c->table, (size_t)std::max(c->u_cap, c->n_user) * S, "table");
     if (rc) { free_table(c); return rc; }
     c->n_sample = S;
     c->h_ranks.assign(ranks, ranks + S);
@@ -486,7 +502,7 @@ int rkr_set_table(rkr_ctx *c, const int32_t *ranks, int32_t S, const float *tabl
     free_update_state(c);  // the installed table describes the current item set
     int rc = dev_alloc_t(c, &c->ranks, (size_t)S, "ranks");
     if (rc) return rc;
-    rc = dev_alloc_t(c, &c->table, (size_t)c->n_user * S, "table");
+    rc = dev_alloc_t(c, &c->table, (size_t)std::max(c->u_cap, c->n_user) * S, "table");
     if (rc) { free_table(c); return rc; }
     c->n_sample = S;
     c->h_ranks.assign(ranks, ranks + S);
@@ -711,7 +727,7 @@ int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, in
         int32_t *ubv = lbv + (size_t)nq * U;
         if (upd)
             RKR_HIP(c, launch_bounds_upd(sl.stream, qip, dv.table, U, S, c->ranks, dv.n_item_base, eps, dv.side_ins,
-                                         dv.n_ins, dv.side_del, dv.n_del, nq, lbv, ubv));
+                                         dv.n_ins, dv.ld_ins, dv.side_del, dv.n_del, dv.ld_del, nq, lbv, ubv));
         else
             RKR_HIP(c, launch_bounds(sl.stream, qip, dv.table, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
         RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));

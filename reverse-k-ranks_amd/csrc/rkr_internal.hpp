@@ -8,6 +8,7 @@
 #include <cmath>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/rkr.h"
@@ -73,10 +74,21 @@ struct Ctx {
     float *items_ins = nullptr;        // inserted items [n_ins][ld]
     float *items_del = nullptr;        // deleted base items [n_del][ld]
     int32_t n_ins = 0, n_del = 0;
-    float *side_ins = nullptr;         // [n_user][n_ins], rows non-increasing
-    float *side_del = nullptr;         // [n_user][n_del]
+    // side arrays [u_cap][ld_*], row u holds n_* scores, non-increasing; each
+    // has a spare of the same shape so an update writes the next version
+    // while batches read the current one
+    float *side_ins = nullptr, *side_ins_spare = nullptr;
+    float *side_del = nullptr, *side_del_spare = nullptr;
+    int32_t ld_ins = 0, ld_del = 0;
     std::vector<int64_t> cur_src;      // current item -> base id (>= 0) or -(1 + inserted index)
     std::vector<uint8_t> base_deleted; // per base item
+    // capacities (rows): users, panels, user norms, table rows and side-array
+    // rows hold u_cap users; current items and their BF16 rows hold i_cap
+    // items once an item update gave them their own buffers; items_ins holds
+    // ins_cap rows.  New rows are appended in place: a batch only reads the
+    // rows of its snapshot's counts.
+    int64_t u_cap = 0, i_cap = 0, ins_cap = 0, del_cap = 0;
+    DeviceBuf upd_scratch;             // score blocks of updates
 
     // table
     int32_t n_sample = 0;
@@ -105,6 +117,7 @@ struct Ctx {
         size_t bytes;
     };
     std::vector<Retired> retired;
+    std::unordered_map<void *, size_t> alloc_bytes;  // size of every live allocation (mem_mu)
 };
 
 // Frees made by this thread go to Ctx::retired instead of hipFree.
@@ -145,7 +158,7 @@ struct DataSnap {
     const float *user_norm = nullptr, *item_norm = nullptr;
     const uint16_t *item_hi = nullptr, *item_lo = nullptr;
     const float *table = nullptr, *side_ins = nullptr, *side_del = nullptr;
-    int32_t n_ins = 0, n_del = 0;
+    int32_t n_ins = 0, n_del = 0, ld_ins = 0, ld_del = 0;
     double max_abs_user = 1.0;
     float screen_abs = 0.0f;
 };
@@ -255,20 +268,20 @@ inline int bounds_rows_per_wg(int32_t S) {
 }
 
 #ifdef __HIPCC__
-// Copies nu rows of S floats (contiguous in HBM) to LDS rows of stride S + 1,
-// 16-byte loads where the source row allows (S % 4 == 0).
-__device__ __forceinline__ void stage_rows(float *rows, const float *__restrict__ src, int64_t nu, int32_t S) {
-    if ((S & 3) == 0) {
+// Copies nu rows of S floats (source row stride src_ld) to LDS rows of
+// stride S + 1, 16-byte loads where the rows allow.
+__device__ __forceinline__ void stage_rows(float *rows, const float *__restrict__ src, int64_t nu, int32_t S,
+                                           int64_t src_ld) {
+    if ((S & 3) == 0 && (src_ld & 3) == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
         const int32_t q = S >> 2;
-        const float4 *s4 = reinterpret_cast<const float4 *>(src);
         for (int64_t i = threadIdx.x; i < nu * q; i += blockDim.x) {
-            const float4 v = s4[i];
             const int64_t r = i / q, c = (i % q) * 4;
+            const float4 v = *reinterpret_cast<const float4 *>(src + r * src_ld + c);
             float *d = rows + r * (S + 1) + c;
             d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
         }
     } else {
-        for (int64_t i = threadIdx.x; i < nu * S; i += blockDim.x) rows[(i / S) * (S + 1) + i % S] = src[i];
+        for (int64_t i = threadIdx.x; i < nu * S; i += blockDim.x) rows[(i / S) * (S + 1) + i % S] = src[(i / S) * src_ld + i % S];
     }
 }
 #endif
@@ -315,7 +328,8 @@ hipError_t launch_subtract_rows(hipStream_t s, const float *a, int64_t lda, int3
                                 int32_t nb, float *out, int64_t ldo, int64_t n_rows);
 hipError_t launch_bounds_upd(hipStream_t s, const float *qip, const float *table, int64_t n_user, int32_t S,
                              const int32_t *ranks, int64_t n_item_base, float eps, const float *ins, int32_t n_ins,
-                             const float *del, int32_t n_del, int32_t nq, int32_t *lbs, int32_t *ubs);
+                             int32_t ld_ins, const float *del, int32_t n_del, int32_t ld_del, int32_t nq,
+                             int32_t *lbs, int32_t *ubs);
 hipError_t launch_kth_values(hipStream_t s, const int32_t *lbs, const int32_t *ubs, int64_t n_user, int32_t nq,
                              int64_t cu, uint32_t *ghist, QueryInfo *qi);
 hipError_t launch_compact_values(hipStream_t s, const int32_t *lbs, const int32_t *ubs, const float *qip,

@@ -134,6 +134,36 @@ def test_updates_many_users_chunked(orc, gpu_index_factory, nq, inserts):
     m.check(idx, queries, 60, "screened")
 
 
+def test_update_rounds_reuse_buffers(orc, gpu_index_factory):
+    """Several rounds of user and item inserts (the side arrays alternate with
+    their spares, capacities grow), then a user delete and an item delete;
+    every round's answers equal brute force."""
+    _case("update rounds, spare rotation and capacity growth")
+    rng, users, items = _data(36, U=2500, I=1200, d=20)
+    idx, _ = gpu_index_factory(users, items, 32)
+    m = Mirror(orc, users, items)
+    queries = items[rng.choice(len(items), 9, replace=False)]
+    for r in range(4):
+        nu = rng.standard_normal((700 + 300 * r, users.shape[1]), dtype=np.float32)
+        ni = rng.standard_normal((25 + 10 * r, items.shape[1]), dtype=np.float32) * np.float32(1.1)
+        idx.insert_users(nu)
+        m.insert_users(nu)
+        idx.insert_items(ni)
+        m.insert_items(ni)
+        m.check(idx, queries, 45, "screened" if r % 2 else "plain")
+    gone = np.sort(rng.choice(idx.counts()["n_user"], 400, replace=False)).astype(np.int64)
+    idx.delete_users(gone)
+    m.delete_users(gone)
+    dels = np.array([1, 7, len(items) + 2, len(items) + 30], np.int64)
+    idx.delete_items(dels)
+    m.delete_items(dels)
+    m.check(idx, queries, 45)
+    ni = rng.standard_normal((15, items.shape[1]), dtype=np.float32)
+    idx.insert_items(ni)
+    m.insert_items(ni)
+    m.check(idx, queries, 45, "screened")
+
+
 def test_rebuild_after_updates_matches_oracle_table(orc, gpu_index_factory):
     _case("rebuild table after updates")
     rng, users, items = _data(33, U=800, I=1500)
