@@ -66,6 +66,10 @@ def parse():
     ap.add_argument("--update-mode", default="snapshot", choices=["snapshot", "barrier"],
                     help="serving: updates prepared beside running batches, or exclusive barriers")
     ap.add_argument("--sat-seconds", type=float, default=3.0, help="serving: closed-loop saturation run length")
+    ap.add_argument("--max-batch", type=int, default=32, help="serving: queries per device batch")
+    ap.add_argument("--slots", type=int, default=2, help="serving: concurrent batches")
+    ap.add_argument("--update-cus", type=int, default=0,
+                    help="serving, snapshot mode: CUs for update kernels (0 = a quarter, -1 = all)")
     return ap.parse_args()
 
 
@@ -82,7 +86,11 @@ def serving(args, idx, wl, label):
     qpool = wl.items
     n_new_u = max(1, U0 // 100)
     n_new_i = max(1, I0 // 200)
-    srv = rkr.Server(idx, max_batch=256, n_slots=2, update_mode=args.update_mode)
+    # capacity for every insert round of the run, so updates never allocate
+    rounds = int(np.ceil(args.sat_seconds + args.duration)) + 3
+    idx.reserve(U0 + rounds * n_new_u, rounds * n_new_i)
+    srv = rkr.Server(idx, max_batch=args.max_batch, n_slots=args.slots, update_mode=args.update_mode,
+                     update_cus=args.update_cus)
     upd, stop_upd = [], threading.Event()
     urng = np.random.default_rng(args.seed + 8)
 
@@ -161,6 +169,7 @@ def serving(args, idx, wl, label):
         "value": round(len(lat) / wall, 2), "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "bf16" if args.form == "screened" else "f32", "data": "synthetic (seeded), open-loop Poisson arrivals",
         "config": {"workload": label, "form": args.form, "update_mode": args.update_mode,
+                   "max_batch": args.max_batch, "slots": args.slots, "update_cus": args.update_cus,
                    "arrival_rate": round(rate, 1), "saturating_rate": round(sat, 1),
                    "saturation": f"closed loop, {window} outstanding, {args.sat_seconds:g} s, same insert stream",
                    "duration_s": args.duration},

@@ -190,6 +190,12 @@ int rkr_insert_users(rkr_ctx *ctx, const float *users, int64_t n);
 int rkr_delete_users(rkr_ctx *ctx, const int64_t *user_ids, int64_t n);
 /* Current sizes: users, items, pending inserted and deleted items. */
 int rkr_get_counts(const rkr_ctx *ctx, int64_t *n_user, int64_t *n_item, int64_t *n_inserted, int64_t *n_deleted);
+/* Sizes the update path's storage for up to n_user users and n_inserted
+ * pending inserted items (and the items they add), so that updates within
+ * these sizes neither allocate nor copy the large arrays (a serving loop
+ * then never stalls on device allocation).  Never shrinks.  No counterpart in
+ * the reference, whose updates reallocate host vectors. */
+int rkr_reserve(rkr_ctx *ctx, int64_t n_user, int64_t n_inserted);
 
 /* ---- serving loop -------------------------------------------------------------
  * Continuous batching on one context: requests (queries with their own k, and
@@ -208,6 +214,8 @@ typedef struct rkr_server_config {
     int32_t max_batch;   /* queries per device batch (0 = 256); memory may split it */
     int32_t n_slots;     /* concurrent batches / worker threads (0 = 2, max 4)       */
     int32_t update_mode; /* RKR_SERVE_BARRIER (0) or RKR_SERVE_SNAPSHOT              */
+    int32_t update_cus;  /* snapshot mode: CUs the update kernels may run on, so
+                            batches keep the rest (0 = a quarter, < 0 = all)       */
 } rkr_server_config;
 
 /* Barrier: an update waits for the batches in flight and blocks new ones, so

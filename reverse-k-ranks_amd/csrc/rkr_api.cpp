@@ -187,9 +187,11 @@ void free_scratch(Ctx *c) {
 
 }  // namespace
 
-int slot_init(Ctx *c, Slot &sl, bool own_stream) {
+int slot_init(Ctx *c, Slot &sl, bool own_stream, const std::vector<uint32_t> *cu_mask) {
     sl.own_stream = own_stream;
-    if (own_stream) {
+    if (own_stream && cu_mask) {
+        RKR_HIP(c, hipExtStreamCreateWithCUMask(&sl.stream, (uint32_t)cu_mask->size(), cu_mask->data()));
+    } else if (own_stream) {
         RKR_HIP(c, hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
     } else {
         sl.stream = c->stream;

@@ -169,7 +169,8 @@ int set_err(Ctx *c, int code, const std::string &msg);
 int dev_alloc(Ctx *c, void **p, size_t bytes, const char *what);
 void dev_free(Ctx *c, void *p, size_t bytes);
 int ensure(Ctx *c, DeviceBuf &b, size_t bytes, const char *what);
-int slot_init(Ctx *c, Slot &sl, bool own_stream);
+// cu_mask (optional): the CUs the slot's stream may use
+int slot_init(Ctx *c, Slot &sl, bool own_stream, const std::vector<uint32_t> *cu_mask = nullptr);
 void slot_free(Ctx *c, Slot &sl);
 // Runs validated queries through one slot (batches sized by free memory).
 int query_on_slot(Ctx *c, Slot &sl, const DataSnap &d, const float *queries, int32_t n_query, const int32_t *topk,

@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <string>
@@ -89,6 +90,9 @@ struct rkr_server {
     DataSnap published;
     uint64_t version = 0;
     std::map<uint64_t, int> inflight;
+    // snapshot mode: the update thread's stream, limited to a subset of the
+    // CUs; the context's own stream is restored at stop
+    hipStream_t upd_stream = nullptr, ctx_stream = nullptr;
 
     bool trace = false;      // RKR_SERVER_TRACE=1: one stderr line per snapshot update
 
@@ -179,6 +183,9 @@ struct rkr_server {
 
     void worker(int w) {
         (void)hipSetDevice(c->device);
+        // snapshot mode: scratch a batch outgrows is released by the update
+        // thread, so a worker never waits in hipFree behind update kernels
+        if (snapshot_mode()) set_defer_frees(true);
         Slot &sl = *slots[w];
         std::unique_lock<std::mutex> lk(mu);
         while (true) {
@@ -253,6 +260,10 @@ struct rkr_server {
             lk.lock();
             published = snap_of(c);
             const uint64_t v = ++version;
+            // r belongs to its waiter once done is set
+            const int kind = r->kind;
+            const long long n_rows = (long long)r->n;
+            const double t_started = r->t_start;
             r->status = rc;
             r->err = err;
             r->t_done = now_s();
@@ -270,7 +281,7 @@ struct rkr_server {
             release_retired(c);
             if (trace)
                 std::fprintf(stderr, "[rkr-server] update kind=%d n=%lld apply_ms=%.2f drain_ms=%.2f free_ms=%.2f freed_mb=%.1f\n",
-                             r->kind, (long long)r->n, 1e3 * (t_applied - r->t_start), 1e3 * (t_drained - t_applied),
+                             kind, n_rows, 1e3 * (t_applied - t_started), 1e3 * (t_drained - t_applied),
                              1e3 * (now_s() - t_drained), freed / 1048576.0);
             lk.lock();
         }
@@ -316,9 +327,22 @@ int rkr_server_start(rkr_ctx *c, const rkr_server_config *cfg, rkr_server **out)
     const char *tr = std::getenv("RKR_SERVER_TRACE");
     s->trace = tr && tr[0] == '1';
     (void)hipSetDevice(c->device);
+    // snapshot mode: updates run on the last k CUs and batches on the others
+    std::vector<uint32_t> upd_mask, query_mask;
+    if (s->snapshot_mode()) {
+        const int n_cu = std::max(c->n_cu, 1);
+        int k = cfg && cfg->update_cus != 0 ? cfg->update_cus : n_cu / 4;
+        if (k < 0 || k > n_cu) k = n_cu;
+        k = std::max(k, 1);
+        if (k < n_cu) {
+            upd_mask.assign((size_t)(n_cu + 31) / 32, 0u);
+            query_mask.assign((size_t)(n_cu + 31) / 32, 0u);
+            for (int cu = 0; cu < n_cu; ++cu) (cu >= n_cu - k ? upd_mask : query_mask)[cu / 32] |= 1u << (cu % 32);
+        }
+    }
     for (int i = 0; i < s->cfg.n_slots; ++i) {
         s->slots.emplace_back(new Slot());
-        const int rc = slot_init(c, *s->slots.back(), true);
+        const int rc = slot_init(c, *s->slots.back(), true, query_mask.empty() ? nullptr : &query_mask);
         if (rc) {
             for (auto &sl : s->slots) slot_free(c, *sl);
             delete s;
@@ -326,7 +350,16 @@ int rkr_server_start(rkr_ctx *c, const rkr_server_config *cfg, rkr_server **out)
         }
     }
     for (int i = 0; i < s->cfg.n_slots; ++i) s->workers.emplace_back(&rkr_server::worker, s, i);
-    if (s->snapshot_mode()) s->updater = std::thread(&rkr_server::update_loop, s);
+    if (s->snapshot_mode()) {
+        s->ctx_stream = c->stream;
+        if (!upd_mask.empty()) {
+            if (hipExtStreamCreateWithCUMask(&s->upd_stream, (uint32_t)upd_mask.size(), upd_mask.data()) == hipSuccess)
+                c->stream = s->upd_stream;
+            else
+                s->upd_stream = nullptr;  // updates then share all CUs
+        }
+        s->updater = std::thread(&rkr_server::update_loop, s);
+    }
     *out = s;
     return RKR_OK;
 }
@@ -421,6 +454,11 @@ int rkr_server_stop(rkr_server *s) {
     }
     for (auto &t : s->workers) t.join();
     if (s->updater.joinable()) s->updater.join();
+    if (s->upd_stream) {
+        (void)hipStreamSynchronize(s->upd_stream);
+        s->c->stream = s->ctx_stream;
+        (void)hipStreamDestroy(s->upd_stream);
+    }
     release_retired(s->c);
     for (auto &sl : s->slots) slot_free(s->c, *sl);
     delete s;

@@ -530,6 +530,45 @@ int rkr_delete_users(rkr_ctx *c, const int64_t *ids, int64_t n) {
     return RKR_OK;
 }
 
+int rkr_reserve(rkr_ctx *c, int64_t n_user, int64_t n_inserted) {
+    if (!c || n_user < 0 || n_inserted < 0 || n_inserted > kMaxSideItems)
+        return c ? set_err(c, RKR_ERR_INVALID, "reserve: sizes out of range") : RKR_ERR_INVALID;
+    int rc = check_updatable(c);
+    if (rc) return rc;
+    RKR_HIP(c, hipSetDevice(c->device));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    if ((rc = reserve_users(c, std::max(n_user, c->n_user)))) return rc;
+    const int64_t extra = std::max<int64_t>(0, n_inserted - c->n_ins);
+    if (extra > 0) {
+        init_item_map(c);
+        if ((rc = reserve_items(c, c->n_item + extra)) ||
+            (rc = reserve_list(c, &c->items_ins, &c->ins_cap, c->n_ins, n_inserted, "inserted items")))
+            return rc;
+    }
+    // both side buffers at the reserved row length
+    const int32_t ld = (int32_t)((n_inserted + 63) / 64 * 64);
+    if (ld > c->ld_ins) {
+        float *a = nullptr, *b = nullptr;
+        if ((rc = dev_alloc_t(c, &a, (size_t)c->u_cap * ld, "side array")) ||
+            (rc = dev_alloc_t(c, &b, (size_t)c->u_cap * ld, "side array"))) {
+            dev_free(c, a, 0);
+            return rc;
+        }
+        if ((rc = copy_rows(c, a, ld, c->side_ins, c->ld_ins, c->n_user, c->n_ins))) return rc;
+        RKR_HIP(c, hipStreamSynchronize(c->stream));
+        dev_free(c, c->side_ins, 0);
+        dev_free(c, c->side_ins_spare, 0);
+        c->side_ins = a;
+        c->side_ins_spare = b;
+        c->ld_ins = ld;
+    } else if (!c->side_ins_spare && c->ld_ins > 0) {
+        if ((rc = dev_alloc_t(c, &c->side_ins_spare, (size_t)c->u_cap * c->ld_ins, "side array"))) return rc;
+    }
+    // the merge score block of a full sort chunk over the reserved users
+    const size_t blk = sizeof(float) * (size_t)c->u_cap * (size_t)std::min<int64_t>(kSortChunk, std::max<int64_t>(n_inserted, 1));
+    return ensure(c, c->upd_scratch, blk, "update score block");
+}
+
 int rkr_get_counts(const rkr_ctx *c, int64_t *n_user, int64_t *n_item, int64_t *n_inserted, int64_t *n_deleted) {
     if (!c) return RKR_ERR_INVALID;
     if (n_user) *n_user = c->n_user;

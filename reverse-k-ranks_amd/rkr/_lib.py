@@ -43,7 +43,7 @@ EXPORTS = (
     "rkr_version", "rkr_normalize_users", "rkr_set_data", "rkr_uniform_sample_ranks", "rkr_build_table",
     "rkr_set_table", "rkr_get_table", "rkr_get_ranks", "rkr_save_table", "rkr_load_table", "rkr_query",
     "rkr_index_size_bytes", "rkr_insert_items", "rkr_delete_items", "rkr_insert_users", "rkr_delete_users",
-    "rkr_get_counts", "rkr_now", "rkr_server_start", "rkr_server_submit_query", "rkr_server_submit_update",
+    "rkr_get_counts", "rkr_reserve", "rkr_now", "rkr_server_start", "rkr_server_submit_query", "rkr_server_submit_update",
     "rkr_server_wait", "rkr_server_poll", "rkr_server_counters", "rkr_server_stop",
 )
 
@@ -70,7 +70,8 @@ class BatchTiming(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("max_batch", C.c_int32), ("n_slots", C.c_int32), ("update_mode", C.c_int32)]
+    _fields_ = [("max_batch", C.c_int32), ("n_slots", C.c_int32), ("update_mode", C.c_int32),
+                ("update_cus", C.c_int32)]
 
 
 RKR_SERVE_BARRIER = 0
@@ -126,6 +127,7 @@ def load():
     L.rkr_insert_users.argtypes = [vp, f32p, i64]
     L.rkr_delete_users.argtypes = [vp, C.c_void_p, i64]
     L.rkr_get_counts.argtypes = [vp] + [C.POINTER(C.c_int64)] * 4
+    L.rkr_reserve.argtypes = [vp, i64, i64]
     L.rkr_now.restype = C.c_double
     L.rkr_server_start.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
     L.rkr_server_submit_query.argtypes = [vp, f32p, i32, C.POINTER(C.c_uint64)]

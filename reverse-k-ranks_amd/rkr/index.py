@@ -162,6 +162,11 @@ class Index:
         _check(self.lib, self.ctx, self.lib.rkr_delete_users(self.ctx, _ptr(ids), ids.shape[0]))
         self._refresh_counts()
 
+    def reserve(self, n_user: int, n_inserted: int):
+        """Sizes the update storage for n_user users and n_inserted pending
+        inserted items, so updates within those sizes do not allocate."""
+        _check(self.lib, self.ctx, self.lib.rkr_reserve(self.ctx, int(n_user), int(n_inserted)))
+
     def counts(self):
         v = [C.c_int64() for _ in range(4)]
         _check(self.lib, self.ctx, self.lib.rkr_get_counts(self.ctx, *[C.byref(x) for x in v]))

@@ -11,11 +11,12 @@ from . import _lib as L
 class Server:
     """Continuous-batching request loop on an Index's device context."""
 
-    def __init__(self, index, max_batch: int = 256, n_slots: int = 2, update_mode: str = "barrier"):
+    def __init__(self, index, max_batch: int = 256, n_slots: int = 2, update_mode: str = "barrier",
+                 update_cus: int = 0):
         self.index = index
         self.lib = index.lib
         mode = {"barrier": L.RKR_SERVE_BARRIER, "snapshot": L.RKR_SERVE_SNAPSHOT}[update_mode]
-        cfg = L.ServerConfig(max_batch, n_slots, mode)
+        cfg = L.ServerConfig(max_batch, n_slots, mode, update_cus)
         h = C.c_void_p()
         rc = self.lib.rkr_server_start(index.ctx, C.byref(cfg), C.byref(h))
         if rc != L.RKR_OK:

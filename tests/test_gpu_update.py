@@ -164,6 +164,25 @@ def test_update_rounds_reuse_buffers(orc, gpu_index_factory):
     m.check(idx, queries, 45, "screened")
 
 
+def test_reserved_updates(orc, gpu_index_factory):
+    """Updates within reserved capacity (no reallocation) stay exact."""
+    _case("updates within reserved capacity")
+    rng, users, items = _data(37, U=3000, I=1000, d=16)
+    idx, _ = gpu_index_factory(users, items, 24)
+    idx.reserve(5000, 400)
+    m = Mirror(orc, users, items)
+    queries = items[rng.choice(len(items), 8, replace=False)]
+    for r in range(3):
+        nu = rng.standard_normal((500, users.shape[1]), dtype=np.float32)
+        ni = rng.standard_normal((100, items.shape[1]), dtype=np.float32)
+        idx.insert_items(ni)
+        m.insert_items(ni)
+        idx.insert_users(nu)
+        m.insert_users(nu)
+        m.check(idx, queries, 40, "screened")
+    m.check(idx, queries, 40)
+
+
 def test_rebuild_after_updates_matches_oracle_table(orc, gpu_index_factory):
     _case("rebuild table after updates")
     rng, users, items = _data(33, U=800, I=1500)
