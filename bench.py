@@ -63,6 +63,8 @@ def parse():
                     help="insert rounds (+1%% users, +0.5%% items each) applied before timing")
     ap.add_argument("--query-batch", type=int, default=0)
     ap.add_argument("--no-dedup", action="store_true", help="compute byte-equal queries of a call separately")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain 1 GiB device copy (for scaling rocprofv3 byte counters) and exit")
     ap.add_argument("--update-mode", default="snapshot", choices=["snapshot", "barrier"],
                     help="serving: updates prepared beside running batches, or exclusive barriers")
     ap.add_argument("--sat-seconds", type=float, default=3.0, help="serving: closed-loop saturation run length")
@@ -201,8 +203,34 @@ def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_sample_q):
             "seconds": round(dt, 3), "user_set_agreement": agree}
 
 
+# HBM traffic of the dominant kernel's launch, from the rocprofv3 --pmc runs
+# summarised in profiles/ (FETCH_SIZE x2.0 and WRITE_SIZE x1.0, the factors a
+# plain 1 GiB device copy gives, bench.py --copy-calibration).  Keyed by
+# (workload, form); other configurations report null.
+PROFILED_TRAFFIC = {
+    ("plain_gpu", "screened"): {"bytes": 2.886e9, "fetch_kib": 1331841.1, "write_kib": 21502.3,
+                                "profile": "profiles/k_refine_screen_100k_x_20k_d150_k50.md"},
+}
+
+
+def copy_calibration():
+    """A plain device copy of 1 GiB (three times), for scaling the PMC byte counters."""
+    import torch
+    n = 1 << 28
+    a = torch.empty(n, dtype=torch.float32, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 3}), flush=True)
+
+
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
                                    "design note only)"}))
@@ -303,6 +331,11 @@ def main():
                 "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
                 "avg_launch_ms": round(tot["bound_ms"] / launches, 4),
                 "work_per_launch": "U*(ceil(log2(S+1))*4 + 8) bytes per query"}
+
+    prof = PROFILED_TRAFFIC.get((args.workload, args.form))
+    if prof and roof.get("kernel") == "k_refine_screen":
+        roof["traffic"] = prof["bytes"]
+        roof["traffic_source"] = prof["profile"]
 
     cpu = None
     if not args.no_cpu:
