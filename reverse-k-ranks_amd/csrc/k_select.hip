@@ -15,7 +15,8 @@
 //      reverse prefix sum tells, for every r_j, its bin and its residual rank
 //      inside the bin;
 //   3. the bins holding targets are gathered into LDS (as many per pass as
-//      fit), and each bin is ranked by one wavefront with a counting select;
+//      fit), and each bin is ranked by one wavefront: a counting select for
+//      bins of <= 32 elements, a register bitonic sort up to 256;
 //   4. bins too large for that are narrowed by repeated histograms over
 //      their value range (wg_select) -- this also makes ties and skewed
 //      rows exact.
@@ -90,6 +91,59 @@ __device__ __forceinline__ void row_for_each(const float *__restrict__ x, int64_
         }
     } else {
         for (int64_t i = threadIdx.x; i < n; i += NT) f(x[i]);
+    }
+}
+
+// Sorts the 64*E values a wavefront holds (value v[e] of lane l is element
+// e*64 + l) in descending order with a bitonic network: partners in other
+// lanes are reached by shuffles, partners in the same lane directly.
+template <int E>
+__device__ __forceinline__ void wave_bitonic_desc(float (&v)[E]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int size = 2; size <= 64 * E; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            float nv[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int idx = e * 64 + lane;
+                const float other = stride < 64 ? __shfl_xor(v[e], stride, 64) : v[e ^ (stride >> 6)];
+                // blocks with (idx & size) == 0 run descending; the lower index of a
+                // pair keeps the larger value in a descending block
+                const bool desc = (idx & size) == 0;
+                const bool lower = (idx & stride) == 0;
+                nv[e] = (lower == desc) ? fmaxf(v[e], other) : fminf(v[e], other);
+            }
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = nv[e];
+        }
+    }
+}
+
+// Writes T[j] for the targets j0 <= j < j1 of one bin whose h elements sit at
+// gbuf[seg..seg+h) (h <= 64*E): sorts them in registers, then reads the
+// element at position ranks[j] - above.
+template <int E>
+__device__ __forceinline__ void wave_sort_select(const float *gbuf, uint32_t seg, uint32_t h, uint32_t above,
+                                                 const int32_t *__restrict__ ranks, int j0, int j1, float *T) {
+    const int lane = threadIdx.x & 63;
+    float v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = (uint32_t)(e * 64 + lane);
+        v[e] = i < h ? gbuf[seg + i] : -INFINITY;  // padding sorts last (all scores are finite)
+    }
+    wave_bitonic_desc<E>(v);
+    for (int j = j0; j < j1; ++j) {
+        const uint32_t pos = (uint32_t)ranks[j] - above;  // < h
+        float x = 0.0f;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const float y = __shfl(v[e], (int)(pos & 63u), 64);
+            if ((int)(pos >> 6) == e) x = y;
+        }
+        if (lane == 0) T[j] = x;
     }
 }
 
@@ -338,6 +392,11 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
                 continue;
             }
             const uint32_t above = sm.incl[b + 1];
+            // mid-sized bins: sort in registers, O(h log^2 h)
+            if (h > 32 && h <= 64) { wave_sort_select<1>(sm.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
+            if (h > 64 && h <= 128) { wave_sort_select<2>(sm.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
+            if (h > 128 && h <= 256) { wave_sort_select<4>(sm.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
+            // small (and rare very large) bins: rank each element by counting
             for (uint32_t i = lane; i < h; i += 64) {
                 const float e = sm.gbuf[seg + i];
                 uint32_t pos = 0;
