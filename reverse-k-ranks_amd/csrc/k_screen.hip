@@ -185,18 +185,20 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
         }
         // epilogue constants of the 4 rows this lane's accumulator covers:
         // D[4*lq + i][lr] of each 16x16 block
-        // (the row's threshold itself is re-read on the rare undecided path)
-        float thr_hi[2][4], Ar[2][4];
+        float thr_hi[2][4], thr_lo[2][4], Ar[2][4];
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int64_t r = row_base + b * 16 + 4 * lq + i;
                 if (r < n_rows) {
-                    thr_hi[b][i] = P.rows_thr[r] - P.abs_floor;  // s^ + e > thr - floor  <=>  may count
+                    const float t = P.rows_thr[r];
+                    thr_hi[b][i] = t - P.abs_floor;  // s^ + e > thr - floor  <=>  may count
+                    thr_lo[b][i] = t + P.abs_floor;  // s^ - e > thr + floor  =>  counts
                     Ar[b][i] = P.C * P.user_norm[P.rows_user[r]];
                 } else {
                     thr_hi[b][i] = INFINITY;
+                    thr_lo[b][i] = INFINITY;
                     Ar[b][i] = 0.0f;
                 }
             }
@@ -231,26 +233,31 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
                         accs[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[b][ks].v, bh.v, accs[b], 0, 0, 0);
                     }
                 }
+                // decide the 8 outputs without branches; only undecided ones
+                // (rare) take the recheck path
                 const float np = sN[it];
+                uint32_t undecided = 0;
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    const f32x4 acc = accs[b];
+                for (int b = 0; b < 2; ++b)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const float s = acc[i];
-                        const float hi = __builtin_fmaf(Ar[b][i], np, s);
-                        if (hi > thr_hi[b][i]) {
-                            const int64_t r = row_base + b * 16 + 4 * lq + i;
-                            const float lo = __builtin_fmaf(-Ar[b][i], np, s);
-                            if (lo > P.rows_thr[r] + P.abs_floor) {  // s^ - e > thr + floor  =>  counts
-                                ++cnt[b][i];
-                            } else {
+                        const float sc = accs[b][i];
+                        const bool may = __builtin_fmaf(Ar[b][i], np, sc) > thr_hi[b][i];
+                        const bool sure = __builtin_fmaf(-Ar[b][i], np, sc) > thr_lo[b][i];
+                        cnt[b][i] += (may && sure) ? 1 : 0;
+                        undecided |= (may && !sure) ? (1u << (b * 4 + i)) : 0u;
+                    }
+                if (undecided) {
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            if (undecided & (1u << (b * 4 + i))) {
+                                const int64_t r = row_base + b * 16 + 4 * lq + i;
                                 const int32_t slot = atomicAdd(P.recheck_n, 1);
                                 if (slot < P.recheck_cap)
-                                    P.recheck[slot] = make_int4((int32_t)r, (int32_t)p, __float_as_int(s), 0);
+                                    P.recheck[slot] = make_int4((int32_t)r, (int32_t)p, __float_as_int(accs[b][i]), 0);
                             }
-                        }
-                    }
                 }
             }
         }
