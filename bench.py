@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--pre-updates", type=int, default=0,
                     help="insert rounds (+1%% users, +0.5%% items each) applied before timing")
     ap.add_argument("--query-batch", type=int, default=0)
+    ap.add_argument("--screen-rows", type=int, default=0, choices=[0, 32, 64],
+                    help="refine rows per wavefront of the screen (0 = library default)")
     ap.add_argument("--no-dedup", action="store_true", help="compute byte-equal queries of a call separately")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy (for scaling rocprofv3 byte counters) and exit")
@@ -254,6 +256,8 @@ def main():
     if args.query_batch:
         idx.set_option(3, args.query_batch)
     idx.set_option(6, 0 if args.no_dedup else 1)  # RKR_OPT_DEDUP
+    if args.screen_rows:
+        idx.set_option(7, args.screen_rows)  # RKR_OPT_SCREEN_ROWS
     if args.pre_updates:
         rng_u = np.random.default_rng(args.seed + 11)
         for _ in range(args.pre_updates):

@@ -58,6 +58,8 @@ typedef struct rkr_ctx rkr_ctx;
 #define RKR_OPT_SCREEN_CHUNK 5  /* items per screened work-unit chunk (0 = no limit) */
 #define RKR_OPT_DEDUP 6         /* 1 (default): byte-equal queries of one rkr_query
                                    call are answered once, with their largest k   */
+#define RKR_OPT_SCREEN_ROWS 7   /* refine rows per wavefront of the screen: 32 or 64
+                                   (0 = 32)                                       */
 
 /* Per-query counters; mirrors SingleQueryPerformance
  * (src/include/util/FileIO.hpp:150-173) plus the screening figures. */

@@ -33,7 +33,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int NT = 256;
-constexpr int ROWS_WG = 128;
 constexpr int ITEMS_STAGE = 64;
 constexpr int KPAD_LDS = 8;  // bf16 pad per LDS row
 
@@ -105,8 +104,11 @@ struct ScreenParams {
     int64_t recheck_cap;
 };
 
-template <int KS>
-__global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
+// RB row blocks of 16 per wave: 2 (32 rows, two waves per SIMD) or 4 (64
+// rows, one wave per SIMD, half the LDS reads per matrix-core operation).
+template <int KS, int RB>
+__global__ void __launch_bounds__(NT, RB == 2 ? 2 : 1) k_refine_screen(ScreenParams P) {
+    constexpr int RW = 4 * 16 * RB;  // rows per workgroup
     // items per LDS stage: 32 for long rows keeps the prefetch registers
     // within the 256-VGPR budget of two waves per SIMD
     constexpr int IST = KS >= 5 ? 32 : ITEMS_STAGE;
@@ -119,7 +121,7 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
     // Persistent grid: the work split is sized here from the refine count the
     // compaction produced, so the host launches without reading it back.
     const int64_t n_rows = min((int64_t)*P.n_rows_dev, P.max_rows);
-    const int64_t n_rb = (n_rows + ROWS_WG - 1) / ROWS_WG;
+    const int64_t n_rb = (n_rows + RW - 1) / RW;
     const int64_t n_stage_total = (P.n_item + IST - 1) / IST;
     const int64_t fill_chunks = n_rb > 0 ? (P.target_units + n_rb - 1) / n_rb : 1;
     const int64_t st_per_chunk =
@@ -163,12 +165,12 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
     for (int64_t unit = blockIdx.x; unit < n_units; unit += gridDim.x) {
         const int64_t chunk = unit / n_rb;
         const int64_t rb = unit % n_rb;
-        const int64_t row_base = rb * ROWS_WG + wave * 32;
+        const int64_t row_base = rb * RW + wave * 16 * RB;
 
         // A fragments: row (row_base + b*16 + lr), k = ks*32 + 8*lq + i
-        Frag ah[2][KS], al[2][KS];
+        Frag ah[RB][KS], al[RB][KS];
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < RB; ++b) {
             const int64_t r = row_base + b * 16 + lr;
             const float *urow = r < n_rows ? P.users + (int64_t)P.rows_user[r] * P.ld : nullptr;
 #pragma unroll
@@ -185,9 +187,9 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
         }
         // epilogue constants of the 4 rows this lane's accumulator covers:
         // D[4*lq + i][lr] of each 16x16 block
-        float thr_hi[2][4], thr_lo[2][4], Ar[2][4];
+        float thr_hi[RB][4], thr_lo[RB][4], Ar[RB][4];
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < RB; ++b)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int64_t r = row_base + b * 16 + 4 * lq + i;
@@ -202,7 +204,11 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
                     Ar[b][i] = 0.0f;
                 }
             }
-        int32_t cnt[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+        int32_t cnt[RB][4];
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cnt[b][i] = 0;
 
         const int64_t st0 = chunk * st_per_chunk;
         const int64_t st1 = min(n_stage_total, st0 + st_per_chunk);
@@ -220,14 +226,16 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
                 // all 3*Kp products of a 16x16 block into one FP32 accumulator
                 // (the bound holds for any order of the additions); B
                 // fragments are read from LDS one k-step at a time
-                f32x4 accs[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+                f32x4 accs[RB];
+#pragma unroll
+                for (int b = 0; b < RB; ++b) accs[b] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks) {
                     Frag bh, bl;
                     bh.q = sB[0][it * ROWQ + ks * 4 + lq];
                     bl.q = sB[1][it * ROWQ + ks * 4 + lq];
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) {
+                    for (int b = 0; b < RB; ++b) {
                         accs[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bh.v, accs[b], 0, 0, 0);
                         accs[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bl.v, accs[b], 0, 0, 0);
                         accs[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[b][ks].v, bh.v, accs[b], 0, 0, 0);
@@ -238,7 +246,7 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
                 const float np = sN[it];
                 uint32_t undecided = 0;
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
+                for (int b = 0; b < RB; ++b)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const float sc = accs[b][i];
@@ -249,7 +257,7 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
                     }
                 if (undecided) {
 #pragma unroll
-                    for (int b = 0; b < 2; ++b)
+                    for (int b = 0; b < RB; ++b)
 #pragma unroll
                         for (int i = 0; i < 4; ++i)
                             if (undecided & (1u << (b * 4 + i))) {
@@ -263,7 +271,7 @@ __global__ void __launch_bounds__(NT, 2) k_refine_screen(ScreenParams P) {
         }
         // reduce each row's count over the 16 lanes (columns) that share it
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < RB; ++b)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 int32_t v = cnt[b][i];
@@ -357,21 +365,29 @@ hipError_t launch_refine_screen(hipStream_t s, const ScreenLaunch &L) {
     P.chunk_items = std::max(1, L.chunk_items);
     // persistent grid: enough workgroups to fill the device, never more than
     // the largest possible number of units
-    const int64_t n_rb_max = (L.max_rows + ROWS_WG - 1) / ROWS_WG;
+    const int rb = L.rows_per_wave == 64 ? 4 : 2;
+    const int64_t n_rb_max = (L.max_rows + 64 * rb - 1) / (64 * rb);
     const int64_t n_stage = (L.n_item + ITEMS_STAGE - 1) / ITEMS_STAGE;
     const int64_t max_units = std::max<int64_t>(n_rb_max, std::min<int64_t>(P.target_units, n_rb_max * n_stage));
-    const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(max_units, L.grid_cap)));
+    const int64_t cap = rb == 4 ? L.grid_cap / 2 : L.grid_cap;  // one workgroup per CU at 64 rows per wave
+    const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(max_units, cap)));
+#define RKR_SCREEN_CASE(ks)                                                                              \
+    case ks:                                                                                           \
+        if (rb == 4) hipLaunchKernelGGL((k_refine_screen<ks, 4>), grid, dim3(NT), 0, s, P);             \
+        else hipLaunchKernelGGL((k_refine_screen<ks, 2>), grid, dim3(NT), 0, s, P);                     \
+        break;
     switch (L.Kp / 32) {
-        case 1: hipLaunchKernelGGL(k_refine_screen<1>, grid, dim3(NT), 0, s, P); break;
-        case 2: hipLaunchKernelGGL(k_refine_screen<2>, grid, dim3(NT), 0, s, P); break;
-        case 3: hipLaunchKernelGGL(k_refine_screen<3>, grid, dim3(NT), 0, s, P); break;
-        case 4: hipLaunchKernelGGL(k_refine_screen<4>, grid, dim3(NT), 0, s, P); break;
-        case 5: hipLaunchKernelGGL(k_refine_screen<5>, grid, dim3(NT), 0, s, P); break;
-        case 6: hipLaunchKernelGGL(k_refine_screen<6>, grid, dim3(NT), 0, s, P); break;
-        case 7: hipLaunchKernelGGL(k_refine_screen<7>, grid, dim3(NT), 0, s, P); break;
-        case 8: hipLaunchKernelGGL(k_refine_screen<8>, grid, dim3(NT), 0, s, P); break;
+        RKR_SCREEN_CASE(1)
+        RKR_SCREEN_CASE(2)
+        RKR_SCREEN_CASE(3)
+        RKR_SCREEN_CASE(4)
+        RKR_SCREEN_CASE(5)
+        RKR_SCREEN_CASE(6)
+        RKR_SCREEN_CASE(7)
+        RKR_SCREEN_CASE(8)
         default: return hipErrorInvalidValue;
     }
+#undef RKR_SCREEN_CASE
     return hipGetLastError();
 }
 

@@ -320,6 +320,10 @@ int rkr_set_option(rkr_ctx *c, int option, int64_t value) {
             if (value < 0 || value > (1 << 30)) return set_err(c, RKR_ERR_INVALID, "screen chunk must be in [0, 2^30]");
             c->screen_chunk = (int32_t)value;
             return RKR_OK;
+        case RKR_OPT_SCREEN_ROWS:
+            if (value != 0 && value != 32 && value != 64) return set_err(c, RKR_ERR_INVALID, "screen rows must be 0, 32 or 64");
+            c->screen_rows = (int32_t)value;
+            return RKR_OK;
         case RKR_OPT_DEDUP:
             if (value != 0 && value != 1) return set_err(c, RKR_ERR_INVALID, "dedup must be 0 or 1");
             c->dedup = (int)value;
@@ -338,6 +342,7 @@ int rkr_get_option(const rkr_ctx *c, int option, int64_t *value) {
         case RKR_OPT_BUILD_BATCH: *value = c->build_batch; return RKR_OK;
         case RKR_OPT_SCREEN_CHUNK: *value = c->screen_chunk; return RKR_OK;
         case RKR_OPT_DEDUP: *value = c->dedup; return RKR_OK;
+        case RKR_OPT_SCREEN_ROWS: *value = c->screen_rows; return RKR_OK;
         default: return RKR_ERR_INVALID;
     }
 }
@@ -796,6 +801,7 @@ int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, in
                 L.target_units = (int64_t)c->n_cu * 8;
                 L.grid_cap = (int64_t)c->n_cu * 4;
                 L.chunk_items = c->screen_chunk > 0 ? c->screen_chunk : (1 << 30);
+                L.rows_per_wave = c->screen_rows > 0 ? c->screen_rows : 32;
                 RKR_HIP(c, launch_refine_screen(sl.stream, L));
                 RKR_HIP(c, hipEventRecord(sl.ev[8], sl.stream));
                 RKR_HIP(c, launch_recheck(sl.stream, dv.users, dv.items, ld, d, refine_user, refine_thr, refine_query,

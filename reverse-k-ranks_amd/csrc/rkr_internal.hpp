@@ -55,6 +55,7 @@ struct Ctx {
     int64_t query_batch = 0;
     int64_t build_batch = 0;
     int32_t screen_chunk = 0;   // items per screened work-unit chunk (0 = no limit)
+    int32_t screen_rows = 0;    // refine rows per wavefront of the screen (0 = 32)
     int dedup = 1;              // answer byte-equal queries of one call once
 
     // data
@@ -228,6 +229,7 @@ struct ScreenLaunch {
     int64_t target_units;   // work units wanted to fill the device
     int64_t grid_cap;       // workgroups of the persistent grid
     int32_t chunk_items;    // most items per work-unit chunk (L2-resident slice)
+    int32_t rows_per_wave;  // 32 or 64 refine rows per wavefront
 };
 hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, int32_t ld, int32_t K, int32_t Kp,
                              uint16_t *hi, uint16_t *lo, float *norm);

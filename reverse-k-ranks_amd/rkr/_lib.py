@@ -30,6 +30,7 @@ RKR_OPT_QUERY_BATCH = 3
 RKR_OPT_BUILD_BATCH = 4
 RKR_OPT_SCREEN_CHUNK = 5
 RKR_OPT_DEDUP = 6
+RKR_OPT_SCREEN_ROWS = 7
 
 STATUS_NAMES = {
     RKR_OK: "RKR_OK", RKR_ERR_INVALID: "RKR_ERR_INVALID", RKR_ERR_STATE: "RKR_ERR_STATE",

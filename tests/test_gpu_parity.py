@@ -261,6 +261,29 @@ def test_screened_equals_plain(orc, gpu_index_factory, nq, k):
     assert 0 < scr.timing["max_screen_err"] < 1e-3
 
 
+@pytest.mark.parametrize("d", [150, 64, 20])
+def test_screened_64_rows_per_wave(orc, gpu_index_factory, d):
+    """The 64-rows-per-wavefront screen tile gives the same answers as the
+    32-row tile and the plain form (several row blocks, ragged last block)."""
+    _case(f"screened 64 rows per wave d={d}")
+    rng = np.random.default_rng(40 + d)
+    U, I = 7000, 2600
+    users = rng.standard_normal((U, d), dtype=np.float32) / np.float32(np.sqrt(d))
+    items = (rng.standard_normal((I, d), dtype=np.float32) / np.float32(np.sqrt(d))
+             * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32))
+    queries = items[rng.choice(I, 24, replace=False)]
+    idx, _ = gpu_index_factory(users, items, 48)
+    plain = idx.query(queries, 60)
+    idx.set_form("screened")
+    idx.set_option(7, 64)  # RKR_OPT_SCREEN_ROWS
+    wide = idx.query(queries, 60)
+    idx.set_option(7, 32)
+    narrow = idx.query(queries, 60)
+    for res in (wide, narrow):
+        assert np.array_equal(plain.users, res.users) and np.array_equal(plain.ranks, res.ranks)
+    _check_exact(orc, _normalized(orc, users), items, queries, wide, [60] * 24)
+
+
 def test_screened_ties_and_many_candidates(orc, gpu_index_factory):
     _case("screened ties + many candidates")
     rng = np.random.default_rng(23)
