@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--pre-updates", type=int, default=0,
                     help="insert rounds (+1%% users, +0.5%% items each) applied before timing")
     ap.add_argument("--query-batch", type=int, default=0)
+    ap.add_argument("--autotune", type=int, default=0, metavar="N",
+                    help="screened form: pick tile, work split and batch size on the first N queries before timing")
     ap.add_argument("--screen-rows", type=int, default=0, choices=[0, 32, 64],
                     help="refine rows per wavefront of the screen (0 = library default)")
     ap.add_argument("--no-dedup", action="store_true", help="compute byte-equal queries of a call separately")
@@ -271,6 +273,11 @@ def main():
         return
 
     max_k = int(ks.max())
+    tuned = None
+    if args.autotune and args.form == "screened":
+        from rkr import tune
+        n_s = min(nq, args.autotune)
+        tuned, _ = tune.autotune(idx, wl.queries[:n_s], ks[:n_s], reps=1)
     for _ in range(args.warmup):
         idx.query(wl.queries, ks, max_k=max_k)
     idx.lib.rkr_sync(idx.ctx)
@@ -370,7 +377,8 @@ def main():
         "data": "synthetic (seeded), inputs resident in HBM",
         "config": {"workload": label, "form": args.form, "queries_per_step": nq,
                    "distinct_queries_per_step": int(len(np.unique(wl.queries, axis=0))),
-                   "dedup": not args.no_dedup, "n_user": U, "n_item": I, "dim": d, "n_sample": wl.n_sample},
+                   "dedup": not args.no_dedup, "n_user": U, "n_item": I, "dim": d, "n_sample": wl.n_sample,
+                   "autotuned": tuned},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "latency_note": "single-query calls (batch of 1), host wall time incl. PCIe copies",

@@ -5,5 +5,5 @@ include/rkr.h); this package only binds it.
 """
 from ._lib import RkrError, load  # noqa: F401
 from .index import Index, QueryResult, UserRankElement, build_index  # noqa: F401
-from . import formats, serve, synth  # noqa: F401
+from . import formats, serve, synth, tune  # noqa: F401
 from .serve import Server  # noqa: F401
