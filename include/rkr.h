@@ -60,6 +60,11 @@ typedef struct rkr_ctx rkr_ctx;
                                    call are answered once, with their largest k   */
 #define RKR_OPT_SCREEN_ROWS 7   /* refine rows per wavefront of the screen: 32 or 64
                                    (0 = 32)                                       */
+#define RKR_OPT_DENSE_TOP 8     /* 1 (default): rkr_build_table also keeps a query
+                                   table with dense top ranks (0..255, every 8th
+                                   to 1023, every 32nd to 4095) that exact output
+                                   searches; answers are unchanged, refinement
+                                   shrinks.  The sampled table is unchanged.     */
 
 /* Per-query counters; mirrors SingleQueryPerformance
  * (src/include/util/FileIO.hpp:150-173) plus the screening figures. */

@@ -13,6 +13,8 @@
 // #{s >= q}; the strict count keeps the bounds exact under ties).  Bounds then
 // vary per user, so the k-th smallest upper bound is found by a two-level
 // histogram over bound values instead of over buckets.
+#include <algorithm>
+
 #include "rkr_internal.hpp"
 
 namespace rkr {
@@ -396,6 +398,13 @@ __global__ void __launch_bounds__(NT) k_compact_values(const int32_t *__restrict
     }
 }
 
+__global__ void __launch_bounds__(256) k_gather_cols(const float *__restrict__ src, int64_t ld_src,
+                                                     const int32_t *__restrict__ cols, int32_t n_cols, int64_t n_rows,
+                                                     float *__restrict__ dst, int64_t ld_dst) {
+    for (int64_t r = blockIdx.x; r < n_rows; r += gridDim.x)
+        for (int32_t j = threadIdx.x; j < n_cols; j += blockDim.x) dst[r * ld_dst + j] = src[r * ld_src + cols[j]];
+}
+
 __global__ void k_gather_rows(const float *__restrict__ src, int64_t lds, const int64_t *__restrict__ idx,
                               int64_t n_rows, int64_t width, float *__restrict__ dst, int64_t ldd) {
     const int64_t r = blockIdx.y;
@@ -485,6 +494,14 @@ hipError_t launch_compact_values(hipStream_t s, const int32_t *lbs, const int32_
     const dim3 grid((unsigned)((n_user + cu - 1) / cu), (unsigned)nq);
     hipLaunchKernelGGL(k_compact_values, grid, dim3(NT), 0, s, lbs, ubs, qip, n_user, cu, qi, cand_user, cand_rank,
                        refine_total, refine_cand, refine_user, refine_query, refine_thr);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_cols(hipStream_t s, const float *src, int64_t ld_src, const int32_t *cols, int32_t n_cols,
+                              int64_t n_rows, float *dst, int64_t ld_dst) {
+    if (n_rows <= 0 || n_cols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)std::min<int64_t>(n_rows, 65536)), dim3(256), 0, s, src, ld_src,
+                       cols, n_cols, n_rows, dst, ld_dst);
     return hipGetLastError();
 }
 

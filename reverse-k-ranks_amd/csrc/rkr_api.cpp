@@ -100,7 +100,23 @@ DataSnap snap_of(const Ctx *c) {
     d.ld_del = c->ld_del;
     d.max_abs_user = c->max_abs_user;
     d.screen_abs = c->screen_abs;
+    d.qtable = c->qtable;
+    d.qranks = c->qranks;
+    d.n_qs = c->n_qs;
     return d;
+}
+
+std::vector<int32_t> query_table_ranks(const std::vector<int32_t> &sample, int64_t n_item) {
+    std::vector<int32_t> r(sample);
+    for (int64_t x = 0; x < 256; ++x) r.push_back((int32_t)x);
+    for (int64_t x = 256; x < 1024; x += 8) r.push_back((int32_t)x);
+    for (int64_t x = 1024; x < 4096; x += 32) r.push_back((int32_t)x);
+    std::vector<int32_t> out;
+    std::sort(r.begin(), r.end());
+    for (int32_t x : r)
+        if (x < n_item && (out.empty() || x != out.back())) out.push_back(x);
+    if ((int64_t)out.size() > kMaxSample) out.clear();
+    return out;
 }
 
 // Grow-only scratch buffer.
@@ -139,12 +155,16 @@ void free_update_state(Ctx *c) {
 }
 
 void free_table(Ctx *c) {
-    dev_free(c, c->ranks, sizeof(int32_t) * (size_t)c->n_sample);
-    dev_free(c, c->table, sizeof(float) * (size_t)c->n_user * (size_t)c->n_sample);
-    c->ranks = nullptr;
-    c->table = nullptr;
-    c->n_sample = 0;
+    dev_free(c, c->ranks, 0);
+    dev_free(c, c->table, 0);
+    dev_free(c, c->qranks, 0);
+    dev_free(c, c->qcol, 0);
+    dev_free(c, c->qtable, 0);
+    c->ranks = c->qranks = c->qcol = nullptr;
+    c->table = c->qtable = nullptr;
+    c->n_sample = c->n_qs = 0;
     c->h_ranks.clear();
+    c->h_qranks.clear();
     c->table_eps = 0.0f;
 }
 
@@ -320,6 +340,10 @@ int rkr_set_option(rkr_ctx *c, int option, int64_t value) {
             if (value < 0 || value > (1 << 30)) return set_err(c, RKR_ERR_INVALID, "screen chunk must be in [0, 2^30]");
             c->screen_chunk = (int32_t)value;
             return RKR_OK;
+        case RKR_OPT_DENSE_TOP:
+            if (value != 0 && value != 1) return set_err(c, RKR_ERR_INVALID, "dense top must be 0 or 1");
+            c->dense_top = (int)value;
+            return RKR_OK;
         case RKR_OPT_SCREEN_ROWS:
             if (value != 0 && value != 32 && value != 64) return set_err(c, RKR_ERR_INVALID, "screen rows must be 0, 32 or 64");
             c->screen_rows = (int32_t)value;
@@ -343,6 +367,7 @@ int rkr_get_option(const rkr_ctx *c, int option, int64_t *value) {
         case RKR_OPT_SCREEN_CHUNK: *value = c->screen_chunk; return RKR_OK;
         case RKR_OPT_DEDUP: *value = c->dedup; return RKR_OK;
         case RKR_OPT_SCREEN_ROWS: *value = c->screen_rows; return RKR_OK;
+        case RKR_OPT_DENSE_TOP: *value = c->dense_top; return RKR_OK;
         default: return RKR_ERR_INVALID;
     }
 }
@@ -468,6 +493,31 @@ int rkr_build_table(rkr_ctx *c, const int32_t *ranks, int32_t S, double *build_m
     c->n_sample = S;
     c->h_ranks.assign(ranks, ranks + S);
     RKR_HIP(c, hipMemcpyAsync(c->ranks, ranks, sizeof(int32_t) * S, hipMemcpyHostToDevice, c->stream));
+    if (c->dense_top) {
+        // the query table: select at its ranks, then copy the sampled columns
+        c->h_qranks = query_table_ranks(c->h_ranks, c->n_item);
+        const int32_t nqs = (int32_t)c->h_qranks.size();
+        if (nqs > S) {
+            std::vector<int32_t> col((size_t)S);
+            for (int32_t j = 0, q = 0; j < S; ++j) {
+                while (c->h_qranks[q] != ranks[j]) ++q;
+                col[j] = q;
+            }
+            if ((rc = dev_alloc_t(c, &c->qranks, (size_t)nqs, "query table ranks")) ||
+                (rc = dev_alloc_t(c, &c->qcol, (size_t)S, "query table columns")) ||
+                (rc = dev_alloc_t(c, &c->qtable, (size_t)std::max(c->u_cap, c->n_user) * nqs, "query table"))) {
+                free_table(c);
+                return rc;
+            }
+            c->n_qs = nqs;
+            RKR_HIP(c, hipMemcpyAsync(c->qranks, c->h_qranks.data(), sizeof(int32_t) * nqs, hipMemcpyHostToDevice,
+                                      c->stream));
+            RKR_HIP(c, hipMemcpyAsync(c->qcol, col.data(), sizeof(int32_t) * S, hipMemcpyHostToDevice, c->stream));
+            RKR_HIP(c, hipStreamSynchronize(c->stream));
+        } else {
+            c->h_qranks.clear();
+        }
+    }
 
     // user batch: the score block [nb x n_item] lives in HBM between the
     // GEMM and the selection; size it from the remaining budget (the
@@ -489,7 +539,14 @@ int rkr_build_table(rkr_ctx *c, const int32_t *ranks, int32_t S, double *build_m
         const int64_t n = std::min<int64_t>(nb, c->n_user - u0);
         RKR_HIP(c, launch_scores_store(c->stream, c->users + u0 * c->ld, nullptr, n, c->ld, c->items, c->n_item, c->ld,
                                        c->dim, scores, c->n_item, true));
-        RKR_HIP(c, launch_select_rows(c->stream, scores, n, c->n_item, c->ranks, S, c->table + u0 * S));
+        if (c->qtable) {
+            RKR_HIP(c, launch_select_rows(c->stream, scores, n, c->n_item, c->qranks, c->n_qs,
+                                          c->qtable + u0 * c->n_qs));
+            RKR_HIP(c, launch_gather_cols(c->stream, c->qtable + u0 * c->n_qs, c->n_qs, c->qcol, S, n,
+                                          c->table + u0 * S, S));
+        } else {
+            RKR_HIP(c, launch_select_rows(c->stream, scores, n, c->n_item, c->ranks, S, c->table + u0 * S));
+        }
     }
     RKR_HIP(c, hipEventRecord(c->ev[1], c->stream));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
@@ -675,9 +732,15 @@ int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, in
     using clock = std::chrono::steady_clock;
     const auto t_start = clock::now();
     const int64_t U = dv.n_user, I = dv.n_item;
-    const int32_t S = c->n_sample, ld = c->ld, d = c->dim;
+    const int32_t ld = c->ld, d = c->dim;
     const int mode = c->output;
-    const float eps = mode == RKR_OUTPUT_EXACT ? c->table_eps : 0.0f;
+    // exact output searches the query table when there is one (exact under
+    // the score rule, eps 0); reference output keeps the sampled table
+    const bool use_q = mode == RKR_OUTPUT_EXACT && dv.qtable;
+    const float *T = use_q ? dv.qtable : dv.table;
+    const int32_t *R = use_q ? dv.qranks : c->ranks;
+    const int32_t S = use_q ? dv.n_qs : c->n_sample;
+    const float eps = mode == RKR_OUTPUT_EXACT && !use_q ? c->table_eps : 0.0f;
     // pending item inserts/deletes: bounds become per-user values (k_update.hip)
     const bool upd = dv.n_ins > 0 || dv.n_del > 0;
     const size_t pair_bytes = upd ? 12 : 8;  // <u,q> + buckets, or + two bound values
@@ -733,16 +796,16 @@ int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, in
         int32_t *lbv = reinterpret_cast<int32_t *>(bk);
         int32_t *ubv = lbv + (size_t)nq * U;
         if (upd)
-            RKR_HIP(c, launch_bounds_upd(sl.stream, qip, dv.table, U, S, c->ranks, dv.n_item_base, eps, dv.side_ins,
+            RKR_HIP(c, launch_bounds_upd(sl.stream, qip, T, U, S, R, dv.n_item_base, eps, dv.side_ins,
                                          dv.n_ins, dv.ld_ins, dv.side_del, dv.n_del, dv.ld_del, nq, lbv, ubv));
         else
-            RKR_HIP(c, launch_bounds(sl.stream, qip, dv.table, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
+            RKR_HIP(c, launch_bounds(sl.stream, qip, T, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
         RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
         const int64_t cu = chunk_users(U, nq, c->n_cu);
         uint32_t *ghist = reinterpret_cast<uint32_t *>(sl.hist.p);
         if (upd) RKR_HIP(c, launch_kth_values(sl.stream, lbv, ubv, U, nq, cu, ghist, qi));
         else
-            RKR_HIP(c, launch_hist_kth(sl.stream, bk, U, S, c->ranks, I, nq, mode,
+            RKR_HIP(c, launch_hist_kth(sl.stream, bk, U, S, R, I, nq, mode,
                                        (mode == RKR_OUTPUT_EXACT && eps == 0.0f) ? 1 : 0, cu, ghist, qi));
         RKR_HIP(c, launch_offsets(sl.stream, qi, nq, refine_total));
         RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
@@ -773,7 +836,7 @@ int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, in
             RKR_HIP(c, launch_compact_values(sl.stream, lbv, ubv, qip, U, nq, cu, qi, cand_user, cand_rank,
                                              refine_total, refine_cand, refine_user, refine_query, refine_thr));
         else
-            RKR_HIP(c, launch_compact(sl.stream, bk, qip, U, S, c->ranks, I, nq, mode, cu, qi, cand_user, cand_rank,
+            RKR_HIP(c, launch_compact(sl.stream, bk, qip, U, S, R, I, nq, mode, cu, qi, cand_user, cand_rank,
                                       refine_total, refine_cand, refine_user, refine_query, refine_thr));
         int32_t h_refine_total = 0, h_recheck_n = 0;
         uint32_t h_err_bits = 0;

@@ -97,6 +97,17 @@ struct Ctx {
     float *table = nullptr;     // n_user x n_sample
     float table_eps = 0.0f;
     std::vector<int32_t> h_ranks;
+    // query table of exact output (built with the table): order statistics at
+    // the sample ranks and at dense ranks of every row's top (all of 0..255,
+    // every 8th to 1023, every 32nd to 4095).  Bounds from it settle most
+    // high-ranking users without refinement; the sampled table is the subset
+    // of its columns at qcol.  Absent (n_qs = 0) for installed tables.
+    int32_t n_qs = 0;
+    int32_t *qranks = nullptr;  // n_qs
+    int32_t *qcol = nullptr;    // n_sample
+    float *qtable = nullptr;    // [u_cap][n_qs]
+    std::vector<int32_t> h_qranks;
+    int dense_top = 1;          // build the query table (RKR_OPT_DENSE_TOP)
 
     // screened form: hi/lo BF16 item rows (n_item x kp), norm bounds
     int32_t kp = 0;
@@ -162,8 +173,14 @@ struct DataSnap {
     int32_t n_ins = 0, n_del = 0, ld_ins = 0, ld_del = 0;
     double max_abs_user = 1.0;
     float screen_abs = 0.0f;
+    const float *qtable = nullptr;   // query table (exact output), or null
+    const int32_t *qranks = nullptr;
+    int32_t n_qs = 0;
 };
 DataSnap snap_of(const Ctx *c);
+// Ranks of the query table: the sample ranks and the dense top ranks below
+// n_item, ascending; empty when they would exceed kMaxSample.
+std::vector<int32_t> query_table_ranks(const std::vector<int32_t> &sample, int64_t n_item);
 
 // ---- error and memory helpers (rkr_api.cpp) --------------------------------
 int set_err(Ctx *c, int code, const std::string &msg);
@@ -339,6 +356,9 @@ hipError_t launch_compact_values(hipStream_t s, const int32_t *lbs, const int32_
                                  int64_t n_user, int32_t nq, int64_t cu, QueryInfo *qi, int32_t *cand_user,
                                  int32_t *cand_rank, int32_t *refine_total, int32_t *refine_cand,
                                  int32_t *refine_user, int32_t *refine_query, float *refine_thr);
+// dst[r][j] = src[r][cols[j]] for r < n_rows, j < n_cols
+hipError_t launch_gather_cols(hipStream_t s, const float *src, int64_t ld_src, const int32_t *cols, int32_t n_cols,
+                              int64_t n_rows, float *dst, int64_t ld_dst);
 hipError_t launch_gather_rows(hipStream_t s, const float *src, int64_t lds, const int64_t *idx, int64_t n_rows,
                               int64_t width, float *dst, int64_t ldd);
 

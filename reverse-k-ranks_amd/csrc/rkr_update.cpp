@@ -76,32 +76,37 @@ int reserve_users(Ctx *c, int64_t need) {
     if (need <= c->u_cap) return RKR_OK;
     const int64_t cap = grown(c->u_cap, need), U = c->n_user;
     const int32_t ld = c->ld, S = c->n_sample;
-    float *users = nullptr, *panels = nullptr, *unorm = nullptr, *table = nullptr, *si = nullptr, *sd = nullptr;
+    const int32_t QS = c->n_qs;
+    float *users = nullptr, *panels = nullptr, *unorm = nullptr, *table = nullptr, *qtable = nullptr, *si = nullptr,
+          *sd = nullptr;
     int rc = RKR_OK;
     if ((rc = dev_alloc_t(c, &users, (size_t)cap * ld, "users")) ||
         (rc = dev_alloc_t(c, &panels, (size_t)((cap + 63) / 64) * 64 * ld, "user panels")) ||
         (c->kp && (rc = dev_alloc_t(c, &unorm, (size_t)cap, "user norms"))) ||
         (rc = dev_alloc_t(c, &table, (size_t)cap * S, "table")) ||
+        (QS && (rc = dev_alloc_t(c, &qtable, (size_t)cap * QS, "query table"))) ||
         (c->ld_ins && (rc = dev_alloc_t(c, &si, (size_t)cap * c->ld_ins, "side array"))) ||
         (c->ld_del && (rc = dev_alloc_t(c, &sd, (size_t)cap * c->ld_del, "side array")))) {
-        for (float *p : {users, panels, unorm, table, si, sd}) dev_free(c, p, 0);
+        for (float *p : {users, panels, unorm, table, qtable, si, sd}) dev_free(c, p, 0);
         return rc;
     }
     if ((rc = copy_rows(c, users, ld, c->users, ld, U, ld)) ||
         (rc = copy_rows(c, panels, 64 * ld, c->usersT, 64 * ld, (U + 63) / 64, 64 * ld)) ||
         (c->kp && (rc = copy_rows(c, unorm, 1, c->user_norm, 1, U, 1))) ||
         (rc = copy_rows(c, table, S, c->table, S, U, S)) ||
+        (QS && (rc = copy_rows(c, qtable, QS, c->qtable, QS, U, QS))) ||
         (si && (rc = copy_rows(c, si, c->ld_ins, c->side_ins, c->ld_ins, U, c->n_ins))) ||
         (sd && (rc = copy_rows(c, sd, c->ld_del, c->side_del, c->ld_del, U, c->n_del))))
         return rc;
     RKR_HIP(c, hipStreamSynchronize(c->stream));
-    for (float *p : {c->users, c->usersT, c->user_norm, c->table, c->side_ins, c->side_del, c->side_ins_spare,
-                     c->side_del_spare})
+    for (float *p : {c->users, c->usersT, c->user_norm, c->table, c->qtable, c->side_ins, c->side_del,
+                     c->side_ins_spare, c->side_del_spare})
         dev_free(c, p, 0);
     c->users = users;
     c->usersT = panels;
     c->user_norm = unorm;
     c->table = table;
+    c->qtable = qtable;
     c->side_ins = si;
     c->side_del = sd;
     c->side_ins_spare = c->side_del_spare = nullptr;
@@ -456,7 +461,15 @@ int rkr_insert_users(rkr_ctx *c, const float *users, int64_t n) {
         const int64_t m = std::min<int64_t>(chunk, n - u0);
         RKR_HIP(c, launch_scores_store(c->stream, c->users + (U0 + u0) * ld, nullptr, m, ld, c->items_base,
                                        c->n_item_base, ld, c->dim, block, c->n_item_base, true));
-        RKR_HIP(c, launch_select_rows(c->stream, block, m, c->n_item_base, c->ranks, S, c->table + (U0 + u0) * S));
+        if (c->qtable) {  // query table rows, then their sampled columns
+            const int32_t QS = c->n_qs;
+            RKR_HIP(c, launch_select_rows(c->stream, block, m, c->n_item_base, c->qranks, QS,
+                                          c->qtable + (U0 + u0) * QS));
+            RKR_HIP(c, launch_gather_cols(c->stream, c->qtable + (U0 + u0) * QS, QS, c->qcol, S, m,
+                                          c->table + (U0 + u0) * S, S));
+        } else {
+            RKR_HIP(c, launch_select_rows(c->stream, block, m, c->n_item_base, c->ranks, S, c->table + (U0 + u0) * S));
+        }
     }
     // side rows of the new users, sorted in place at the side arrays' stride
     for (int side = 0; side < 2; ++side) {
@@ -503,6 +516,7 @@ int rkr_delete_users(rkr_ctx *c, const int64_t *ids, int64_t n) {
     // SampleSearchUpdate::DeleteUser (SampleSearchUpdate.hpp:89-109): later users shift down
     rc = keep_user_rows(c, &c->users, ld, ld, d_keep, nk, "users");
     if (!rc) rc = keep_user_rows(c, &c->table, c->n_sample, c->n_sample, d_keep, nk, "table");
+    if (!rc) rc = keep_user_rows(c, &c->qtable, c->n_qs, c->n_qs, d_keep, nk, "query table");
     if (!rc) rc = keep_user_rows(c, &c->side_ins, c->n_ins, c->ld_ins, d_keep, nk, "side array");
     if (!rc) rc = keep_user_rows(c, &c->side_del, c->n_del, c->ld_del, d_keep, nk, "side array");
     dev_free(c, d_keep, 0);

@@ -31,6 +31,7 @@ RKR_OPT_BUILD_BATCH = 4
 RKR_OPT_SCREEN_CHUNK = 5
 RKR_OPT_DEDUP = 6
 RKR_OPT_SCREEN_ROWS = 7
+RKR_OPT_DENSE_TOP = 8
 
 STATUS_NAMES = {
     RKR_OK: "RKR_OK", RKR_ERR_INVALID: "RKR_ERR_INVALID", RKR_ERR_STATE: "RKR_ERR_STATE",

@@ -35,8 +35,10 @@ def fake30d():
 def gpu_index_factory():
     import rkr
 
-    def make(users, items, n_sample, normalize=True):
+    def make(users, items, n_sample, normalize=True, dense_top=None):
         idx = rkr.Index(0)
+        if dense_top is not None:
+            idx.set_option(8, int(dense_top))  # RKR_OPT_DENSE_TOP: query table with dense top ranks
         idx.set_data(users, items, normalize_users=normalize)
         ranks = rkr.Index.uniform_sample_ranks(items.shape[0], n_sample)
         idx.build_table(ranks)

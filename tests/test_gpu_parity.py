@@ -146,18 +146,21 @@ def test_query_duplicates_dedup(orc, gpu_index_factory):
     _check_exact(orc, _normalized(orc, users), items, queries, on, ks)
 
 
-def test_query_many_candidates(orc, gpu_index_factory):
+@pytest.mark.parametrize("dense", [0, 1])
+def test_query_many_candidates(orc, gpu_index_factory, dense):
     """Coarse table (S=4) so thousands of users survive pruning: exercises the
-    radix-select top-k path (> 4096 candidates) and large refine batches."""
-    _case("query many candidates")
+    radix-select top-k path (> 4096 candidates) and large refine batches.
+    With the dense query table the same queries refine far fewer users."""
+    _case(f"query many candidates dense={dense}")
     rng = np.random.default_rng(12)
     U, I, d = 20000, 600, 16
     users = rng.standard_normal((U, d), dtype=np.float32)
     items = rng.standard_normal((I, d), dtype=np.float32)
     queries = items[:4]
-    idx, _ = gpu_index_factory(users, items, 4)
+    idx, _ = gpu_index_factory(users, items, 4, dense_top=dense)
     res = idx.query(queries, 200)
-    assert max(s["n_candidate"] for s in res.stats) > 4096
+    if not dense:
+        assert max(s["n_candidate"] for s in res.stats) > 4096
     _check_exact(orc, _normalized(orc, users), items, queries, res, [200] * 4)
 
 
@@ -233,7 +236,7 @@ def _screened(idx):
 def test_screened_fake30d_matches_bruteforce(orc, fake30d, gpu_index_factory):
     users, items, queries = fake30d
     _case("screened fake-normal-30d k=10")
-    idx, _ = gpu_index_factory(users, items, 128)
+    idx, _ = gpu_index_factory(users, items, 128, dense_top=0)  # sampled table only: users get refined
     res = _screened(idx).query(queries, 10)
     _check_exact(orc, _normalized(orc, users), items, queries, res, [10] * len(queries))
     # queries are not items here, so only near-threshold items are rechecked
@@ -241,20 +244,22 @@ def test_screened_fake30d_matches_bruteforce(orc, fake30d, gpu_index_factory):
     assert res.timing["n_rechecked"] < 0.01 * res.timing["n_screened"]
 
 
-@pytest.mark.parametrize("nq,k", [(1, 50), (16, 7), (40, 120)])
-def test_screened_equals_plain(orc, gpu_index_factory, nq, k):
-    _case(f"screened vs plain nq={nq} k={k}")
+@pytest.mark.parametrize("nq,k,dense", [(1, 50, 0), (16, 7, 0), (40, 120, 0), (40, 120, 1)])
+def test_screened_equals_plain(orc, gpu_index_factory, nq, k, dense):
+    _case(f"screened vs plain nq={nq} k={k} dense={dense}")
     rng = np.random.default_rng(21 + nq)
     U, I, d = 6000, 3000, 150
     users = rng.standard_normal((U, d), dtype=np.float32) / np.float32(np.sqrt(d))
     items = (rng.standard_normal((I, d), dtype=np.float32) / np.float32(np.sqrt(d))
              * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32))
     queries = items[rng.choice(I, nq, replace=False)]
-    idx, _ = gpu_index_factory(users, items, 64)
+    idx, _ = gpu_index_factory(users, items, 64, dense_top=dense)
     plain = idx.query(queries, k)
     scr = _screened(idx).query(queries, k)
     assert np.array_equal(plain.users, scr.users) and np.array_equal(plain.ranks, scr.ranks)
     _check_exact(orc, _normalized(orc, users), items, queries, scr, [k] * nq)
+    if dense:
+        return  # the statistics below need refined users
     # q is an item: <u,q> ties with that item's own score, so it must be rechecked
     assert scr.timing["n_rechecked"] >= plain.timing["n_refined"]
     # the observed screening error stays inside the analytic bound's scale
@@ -284,8 +289,39 @@ def test_screened_64_rows_per_wave(orc, gpu_index_factory, d):
     _check_exact(orc, _normalized(orc, users), items, queries, wide, [60] * 24)
 
 
-def test_screened_ties_and_many_candidates(orc, gpu_index_factory):
-    _case("screened ties + many candidates")
+@pytest.mark.parametrize("form", ["plain", "screened"])
+def test_dense_query_table(orc, form):
+    """The query table (dense top ranks) leaves the sampled table and every
+    answer unchanged and refines fewer users."""
+    _case(f"dense query table ({form})")
+    import rkr
+    rng = np.random.default_rng(50)
+    U, I, d = 20000, 3000, 24
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = rng.standard_normal((I, d), dtype=np.float32) * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32)
+    queries = items[rng.choice(I, 30, replace=False)]
+    ks = rng.integers(1, 120, 30).astype(np.int32)
+    ranks = rkr.Index.uniform_sample_ranks(I, 40)
+    out = {}
+    for dense in (0, 1):
+        idx = rkr.Index(0)
+        idx.set_option(8, dense)  # RKR_OPT_DENSE_TOP
+        idx.set_form(form)
+        idx.set_data(users, items)
+        idx.build_table(ranks)
+        res = idx.query(queries, ks, max_k=120)
+        out[dense] = (idx.get_table(), res)
+        idx.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1].users, out[1][1].users) and np.array_equal(out[0][1].ranks, out[1][1].ranks)
+    refined = [sum(s["n_refine_user"] for s in out[x][1].stats) for x in (0, 1)]
+    assert refined[1] < refined[0]
+    _check_exact(orc, _normalized(orc, users), items, queries, out[1][1], ks)
+
+
+@pytest.mark.parametrize("dense", [0, 1])
+def test_screened_ties_and_many_candidates(orc, gpu_index_factory, dense):
+    _case(f"screened ties + many candidates dense={dense}")
     rng = np.random.default_rng(23)
     d, I, U = 10, 800, 20000
     base = rng.standard_normal((60, d), dtype=np.float32)
@@ -293,9 +329,10 @@ def test_screened_ties_and_many_candidates(orc, gpu_index_factory):
     users = rng.standard_normal((U, d), dtype=np.float32)
     users[:5] = 0.0
     queries = np.concatenate([items[:6], np.zeros((1, d), np.float32)])
-    idx, _ = gpu_index_factory(users, items, 8)
+    idx, _ = gpu_index_factory(users, items, 8, dense_top=dense)
     res = _screened(idx).query(queries, 300)
-    assert max(s["n_candidate"] for s in res.stats) > 4096
+    if not dense:
+        assert max(s["n_candidate"] for s in res.stats) > 4096
     _check_exact(orc, _normalized(orc, users), items, queries, res, [300] * len(queries))
 
 
