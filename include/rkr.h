@@ -65,6 +65,11 @@ typedef struct rkr_ctx rkr_ctx;
                                    to 1023, every 32nd to 4095) that exact output
                                    searches; answers are unchanged, refinement
                                    shrinks.  The sampled table is unchanged.     */
+#define RKR_OPT_COARSE 9        /* 1 (default): screened form, exact output: the
+                                   batch's <u,q> runs on BF16 matrix cores against
+                                   7 guard columns per user, and only the users
+                                   this cannot rule out are scored exactly and
+                                   searched; answers are unchanged.             */
 
 /* Per-query counters; mirrors SingleQueryPerformance
  * (src/include/util/FileIO.hpp:150-173) plus the screening figures. */
@@ -95,6 +100,8 @@ typedef struct rkr_batch_timing {
     int64_t n_rechecked;
     int64_t n_screened;
     double max_screen_err;  /* screened form: max |screen - FP64| observed     */
+    int64_t n_listed;       /* coarse bounds: (query, user) pairs scored exactly */
+    int64_t n_fallback;     /* coarse bounds: queries that took the full path  */
 } rkr_batch_timing;
 
 /* ---- context ------------------------------------------------------------ */

@@ -42,6 +42,14 @@ __device__ __forceinline__ int32_t bucket_ge(const float *__restrict__ row, int3
     }
     return lo;
 }
+// #{T > q} when it is known to lie in [lo, hi] (T[0..lo) > q >= T[hi..S)).
+__device__ __forceinline__ int32_t bucket_gt_in(const float *__restrict__ row, int32_t lo, int32_t hi, float q) {
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (row[mid] > q) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
 __device__ __forceinline__ int32_t bucket_gt(const float *__restrict__ row, int32_t S, float q) {
     int32_t lo = 0, hi = S;
     while (lo < hi) {
@@ -503,6 +511,310 @@ __global__ void __launch_bounds__(NT) k_topk(const QueryInfo *__restrict__ qi, c
     }
 }
 
+
+// ---- coarse bounds (screened form, exact output) ---------------------------
+// k_ip_coarse (k_screen.hip) gives every (query, user) a byte c_lo | c_hi << 4:
+// how many of the user's guard columns j_0 < ... < j_{G-1} of the table lie
+// above the lowest / highest value the score can take.  A guard above the
+// lowest value bounds b_lo = #{T > s - eps} <= j_{c_lo} (S when c_lo = G); a
+// guard above the highest bounds b_hi >= j_{c_hi - 1} + 1 (0 when c_hi = 0).
+// The k-th smallest bound j_{beta_c} over all users is then >= the exact
+// beta, so the users with c_hi <= beta_c hold every exact candidate and the k
+// users of smallest exact b_lo: scoring and searching only them yields the
+// exact path's beta, candidates and ranks.
+
+constexpr int CNT = 256;
+
+// The score rule: FP32 products summed in ascending dimension order.
+__device__ __forceinline__ float rule_dot(const float *__restrict__ a, const float *__restrict__ b, int32_t d) {
+    float s = 0.0f;
+    int32_t k = 0;
+    for (; k + 4 <= d; k += 4) {
+        const float4 x = *reinterpret_cast<const float4 *>(a + k);
+        const float4 y = *reinterpret_cast<const float4 *>(b + k);
+        s = s + x.x * y.x;
+        s = s + x.y * y.y;
+        s = s + x.z * y.z;
+        s = s + x.w * y.w;
+    }
+    for (; k < d; ++k) s = s + a[k] * b[k];
+    return s;
+}
+
+// guards[u * 8 + g] = T[u][col[g]] for g < n, -inf after.
+__global__ void k_guard_rows(const float *__restrict__ table, int64_t n_user, int32_t S, GuardCols gc,
+                             float *__restrict__ guards) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n_user) return;
+    const float *row = table + u * (int64_t)S;
+    float g[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) g[j] = j < gc.n ? row[gc.col[j]] : -INFINITY;
+    float4 *o = reinterpret_cast<float4 *>(guards + u * 8);
+    o[0] = make_float4(g[0], g[1], g[2], g[3]);
+    o[1] = make_float4(g[4], g[5], g[6], g[7]);
+}
+
+__device__ __forceinline__ void coarse_flush(uint64_t alo, uint64_t ahi, uint32_t *h) {
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const uint32_t fl = (uint32_t)(alo >> (8 * j)) & 255u, fh = (uint32_t)(ahi >> (8 * j)) & 255u;
+        if (fl) atomicAdd(&h[j], fl);
+        if (fh) atomicAdd(&h[8 + j], fh);
+    }
+}
+
+// grid (chunks, queries): counts of c_lo and c_hi values 0..6 of the chunk's
+// users, part[(b * n_chunk + chunk) * 16 + {0..6 | 8..14}].  16 users per
+// thread and load (rows of cb are ldc bytes, ldc % 16 == 0, chunk starts at
+// multiples of 1024); per-thread counters are bytes of a 64-bit word,
+// flushed before they can carry.
+__global__ void __launch_bounds__(CNT) k_coarse_hist(const uint8_t *__restrict__ cb, int64_t ldc, int64_t n_user,
+                                                     int64_t cu, int32_t n_chunk, uint32_t *__restrict__ part) {
+    __shared__ uint32_t h[16];
+    const int64_t b = blockIdx.y;
+    if (threadIdx.x < 16) h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint8_t *v = cb + b * ldc;
+    const int64_t u0 = (int64_t)blockIdx.x * cu, u1 = min(n_user, u0 + cu);
+    uint64_t alo = 0, ahi = 0;
+    int n = 0;
+    for (int64_t u = u0 + 16 * (int64_t)threadIdx.x; u < u1; u += 16 * CNT) {
+        const uint4 w = *reinterpret_cast<const uint4 *>(v + u);
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+        const int64_t rem = u1 - u;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (j < rem) {
+                const uint32_t x = ws[j >> 2] >> (8 * (j & 3));
+                alo += 1ull << ((x & 7u) * 8u);
+                ahi += 1ull << (((x >> 4) & 7u) * 8u);
+            }
+        }
+        n += 16;
+        if (n >= 240) {
+            coarse_flush(alo, ahi, h);
+            alo = ahi = 0;
+            n = 0;
+        }
+    }
+    coarse_flush(alo, ahi, h);
+    __syncthreads();
+    if (threadIdx.x < 16) part[((int64_t)b * n_chunk + blockIdx.x) * 16 + threadIdx.x] = h[threadIdx.x];
+}
+
+// One workgroup per query: beta_c = smallest c < n_guard with
+// #{c_lo <= c} >= k (-1 when there is none: the query takes the full path)
+// and the number of users with c_hi <= beta_c.  cq[b] = {beta_c, count, -, 0}.
+__global__ void __launch_bounds__(CNT) k_coarse_kth(const uint32_t *__restrict__ part, int32_t n_chunk,
+                                                    int32_t n_guard, const QueryInfo *__restrict__ qi,
+                                                    int32_t *__restrict__ cq) {
+    __shared__ uint32_t sh[CNT];
+    const int64_t b = blockIdx.x;
+    uint32_t acc = 0;
+    for (int32_t ch = threadIdx.x >> 4; ch < n_chunk; ch += CNT / 16)
+        acc += part[((int64_t)b * n_chunk + ch) * 16 + (threadIdx.x & 15)];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        uint32_t t = 0;
+        for (int j = 0; j < CNT / 16; ++j) t += sh[j * 16 + threadIdx.x];
+        sh[threadIdx.x] = t;  // rows j >= 1 are no longer read
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int64_t k = qi[b].topk;
+    int32_t beta = -1;
+    int64_t cum = 0;
+    for (int32_t c = 0; c < n_guard; ++c) {
+        cum += sh[c];
+        if (cum >= k) { beta = c; break; }
+    }
+    int64_t n = 0;
+    for (int32_t c = 0; c <= beta; ++c) n += sh[8 + c];
+    cq[b * 4 + 0] = beta;
+    cq[b * 4 + 1] = (int32_t)min(n, (int64_t)INT32_MAX);
+    cq[b * 4 + 3] = 0;
+}
+
+// grid = n_chunk workgroups: per chunk, the listed users of every query
+// (c_hi <= beta_c, none when beta_c < 0) from the chunk histograms, their
+// exclusive offsets within the chunk, loc[chunk * nq + b], and the chunk's
+// total, ctot[chunk].
+__global__ void __launch_bounds__(NT) k_coarse_chunk_offsets(const uint32_t *__restrict__ part, int32_t n_chunk,
+                                                             int32_t nq, const int32_t *__restrict__ cq,
+                                                             int32_t *__restrict__ loc, int32_t *__restrict__ ctot) {
+    __shared__ uint32_t red[NW + 1];
+    const int64_t ch = blockIdx.x;
+    uint32_t base = 0;
+    for (int32_t b0 = 0; b0 < nq; b0 += NT) {
+        const int32_t b = b0 + (int32_t)threadIdx.x;
+        uint32_t v = 0;
+        if (b < nq) {
+            const int32_t beta = cq[(int64_t)b * 4];
+            const uint32_t *p = part + ((int64_t)b * n_chunk + ch) * 16 + 8;
+            for (int32_t c = 0; c <= beta; ++c) v += p[c];
+        }
+        uint32_t tot;
+        const uint32_t e = wg_excl_scan(v, red, &tot);
+        if (b < nq) loc[ch * nq + b] = (int32_t)(base + e);
+        base += tot;
+    }
+    if (threadIdx.x == 0) ctot[ch] = (int32_t)base;
+}
+
+// One workgroup: cbase[ch] = exclusive scan of ctot (chunk-major list).
+__global__ void __launch_bounds__(NT) k_coarse_chunk_base(const int32_t *__restrict__ ctot, int32_t n_chunk,
+                                                          int32_t *__restrict__ cbase) {
+    __shared__ uint32_t red[NW + 1];
+    uint32_t base = 0;
+    for (int32_t c0 = 0; c0 < n_chunk; c0 += NT) {
+        const int32_t i = c0 + (int32_t)threadIdx.x;
+        const uint32_t v = i < n_chunk ? (uint32_t)ctot[i] : 0u;
+        uint32_t tot;
+        const uint32_t e = wg_excl_scan(v, red, &tot);
+        if (i < n_chunk) cbase[i] = (int32_t)(base + e);
+        base += tot;
+    }
+}
+
+// grid (chunks, queries): lists the users with c_hi <= beta_c of every query
+// that has one.  The list is chunk-major (all queries' users of chunk 0,
+// then chunk 1, ...), so entries processed together share users' rows; the
+// order within a (chunk, query) slice is arbitrary.  cl_b = b | byte << 16.
+// 16 users per thread and load; a byte-parallel compare marks the listed ones.
+__global__ void __launch_bounds__(CNT) k_coarse_compact(const uint8_t *__restrict__ cb, int64_t ldc, int64_t n_user,
+                                                        int64_t cu, int32_t nq, const int32_t *__restrict__ cq,
+                                                        const int32_t *__restrict__ loc,
+                                                        const int32_t *__restrict__ cbase,
+                                                        int32_t *__restrict__ cl_u, int32_t *__restrict__ cl_b) {
+    __shared__ int32_t s_fill;
+    const int64_t b = blockIdx.y;
+    const int32_t beta = cq[b * 4 + 0];
+    if (beta < 0) return;
+    const int64_t off = (int64_t)cbase[blockIdx.x] + loc[(int64_t)blockIdx.x * nq + b];
+    if (threadIdx.x == 0) s_fill = 0;
+    __syncthreads();
+    const uint8_t *v = cb + b * ldc;
+    const int64_t u0 = (int64_t)blockIdx.x * cu, u1 = min(n_user, u0 + cu);
+    const int lane = threadIdx.x & 63;
+    const uint32_t bb = (uint32_t)(beta + 1) * 0x01010101u;
+    for (int64_t ub = u0; ub < u1; ub += 16 * CNT) {
+        const int64_t u = ub + 16 * (int64_t)threadIdx.x;
+        uint32_t bits = 0;  // bit j: user u + j is listed
+        uint32_t ws[4] = {0u, 0u, 0u, 0u};
+        if (u < u1) {
+            const uint4 w = *reinterpret_cast<const uint4 *>(v + u);
+            ws[0] = w.x; ws[1] = w.y; ws[2] = w.z; ws[3] = w.w;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                // bit 7 of a byte of t is set iff its c_hi >= beta + 1
+                const uint32_t t = (((ws[i] >> 4) & 0x07070707u) | 0x80808080u) - bb;
+                const uint32_t m = ~t & 0x80808080u;
+                bits |= (((m >> 7) & 1u) | ((m >> 14) & 2u) | ((m >> 21) & 4u) | ((m >> 28) & 8u)) << (4 * i);
+            }
+            const int64_t rem = u1 - u;
+            if (rem < 16) bits &= (1u << rem) - 1u;
+        }
+        const uint32_t cnt = (uint32_t)__popc(bits);
+        if (!__ballot(cnt != 0)) continue;
+        uint32_t x = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        int32_t base = 0;
+        if (lane == 63) base = atomicAdd(&s_fill, (int32_t)x);
+        base = __shfl(base, 63, 64);
+        int64_t pos = off + base + (int64_t)(x - cnt);
+        while (bits) {
+            const int j = __ffs(bits) - 1;
+            bits &= bits - 1;
+            cl_u[pos] = (int32_t)(u + j);
+            cl_b[pos] = (int32_t)b | (int32_t)(((ws[j >> 2] >> (8 * (j & 3))) & 0xFFu) << 16);
+            ++pos;
+        }
+    }
+}
+
+// One thread per listed (query, user): the score rule's <u,q>, the exact
+// bucket pair against the full table row, and the query's bucket histograms
+// (ghist[b] = [h_lo | h_hi], S + 1 bins each; h_lo only when same != 0).
+__global__ void __launch_bounds__(CNT) k_coarse_exact(const float *__restrict__ users,
+                                                      const float *__restrict__ queries, int32_t ld, int32_t dim,
+                                                      const float *__restrict__ table, int32_t S, float eps, int same,
+                                                      GuardCols gc, const int32_t *__restrict__ cl_u,
+                                                      const int32_t *__restrict__ cl_b, int64_t n,
+                                                      uint32_t *__restrict__ bk_c, float *__restrict__ s_c,
+                                                      uint32_t *__restrict__ ghist) {
+    const int64_t i = (int64_t)blockIdx.x * CNT + threadIdx.x;
+    if (i >= n) return;
+    const int64_t u = cl_u[i];
+    const uint32_t packed = (uint32_t)cl_b[i];
+    const int64_t b = packed & 0xFFFFu;
+    const int32_t clo = (int32_t)((packed >> 16) & 7u), chi = (int32_t)((packed >> 20) & 7u);
+    const float sc = rule_dot(users + u * ld, queries + b * ld, dim);
+    // both buckets lie in [j_{c_hi - 1} + 1, j_{c_lo}] (0 / S at the ends)
+    int32_t r0 = 0, r1 = S;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (j == chi - 1) r0 = gc.col[j] + 1;
+        if (j == clo && j < gc.n) r1 = gc.col[j];
+    }
+    const float *row = table + u * (int64_t)S;
+    uint32_t lo, hi;
+    if (eps > 0.0f) {
+        lo = (uint32_t)bucket_gt_in(row, r0, r1, __double2float_rd((double)sc - (double)eps));
+        hi = (uint32_t)bucket_gt_in(row, r0, r1, __double2float_ru((double)sc + (double)eps));
+    } else {
+        lo = hi = (uint32_t)bucket_gt_in(row, r0, r1, sc);
+    }
+    bk_c[i] = lo | (hi << 16);
+    s_c[i] = sc;
+    uint32_t *g = ghist + b * 2 * (int64_t)(S + 1);
+    atomicAdd(&g[lo], 1u);
+    if (!same) atomicAdd(&g[S + 1 + hi], 1u);
+}
+
+// One thread per listed (query, user): the candidates (b_hi <= beta) with
+// their rank when the bounds fix it, else -1 and a refine row; <u,q> of each
+// candidate goes to qip for the top-k.
+__global__ void __launch_bounds__(CNT) k_coarse_final(const int32_t *__restrict__ cl_u, const int32_t *__restrict__ cl_b,
+                                                      const uint32_t *__restrict__ bk_c, const float *__restrict__ s_c,
+                                                      int64_t n, int32_t S, const int32_t *__restrict__ ranks,
+                                                      int64_t n_item, int64_t n_user, QueryInfo *__restrict__ qi,
+                                                      int32_t *__restrict__ cand_user, int32_t *__restrict__ cand_rank,
+                                                      int32_t *__restrict__ refine_total,
+                                                      int32_t *__restrict__ refine_cand,
+                                                      int32_t *__restrict__ refine_user,
+                                                      int32_t *__restrict__ refine_query,
+                                                      float *__restrict__ refine_thr, float *__restrict__ qip) {
+    const int64_t i = (int64_t)blockIdx.x * CNT + threadIdx.x;
+    if (i >= n) return;
+    const int32_t b = cl_b[i] & 0xFFFF;
+    const uint32_t x = bk_c[i];
+    const int32_t blo = (int32_t)(x & 0xFFFFu), bhi = (int32_t)(x >> 16);
+    if (bhi > qi[b].beta) return;
+    const int32_t u = cl_u[i];
+    const int32_t lbs = blo == S ? (int32_t)n_item : ranks[blo];
+    const int32_t ubs = bhi == 0 ? 0 : ranks[bhi - 1] + 1;
+    const int32_t rank = lbs == ubs ? lbs : -1;
+    const int64_t slot = qi[b].cand_off + atomicAdd(&qi[b].fill, 1);
+    const float sc = s_c[i];
+    cand_user[slot] = u;
+    cand_rank[slot] = rank;
+    qip[(int64_t)b * n_user + u] = sc;
+    if (rank < 0) {
+        const int32_t r = atomicAdd(refine_total, 1);
+        atomicAdd(&qi[b].n_refine, 1);
+        refine_cand[r] = (int32_t)slot;
+        refine_user[r] = u;
+        refine_query[r] = b;
+        refine_thr[r] = sc;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_bounds(hipStream_t s, const float *qip, const float *table, int64_t n_user, int32_t S, int32_t nq,
@@ -572,6 +884,67 @@ hipError_t launch_topk(hipStream_t s, const QueryInfo *qi, int32_t nq, const int
     if (nq <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_topk, dim3((unsigned)nq), dim3(NT), 0, s, qi, cand_user, cand_rank, qip, n_user, max_k,
                        out_user, out_rank, out_qip);
+    return hipGetLastError();
+}
+
+hipError_t launch_guard_rows(hipStream_t s, const float *table, int64_t n_user, int32_t S, const GuardCols &gc,
+                             float *guards) {
+    if (n_user <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_guard_rows, dim3((unsigned)((n_user + 255) / 256)), dim3(256), 0, s, table, n_user, S, gc,
+                       guards);
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_kth(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq, int64_t cu,
+                             int32_t n_guard, uint32_t *part, const QueryInfo *qi, int32_t *cq) {
+    if (nq <= 0 || n_user <= 0) return hipSuccess;
+    if (ldc % 16 || cu % 1024) return hipErrorInvalidValue;
+    const int32_t n_chunk = (int32_t)((n_user + cu - 1) / cu);
+    hipLaunchKernelGGL(k_coarse_hist, dim3((unsigned)n_chunk, (unsigned)nq), dim3(CNT), 0, s, cb, ldc, n_user, cu,
+                       n_chunk, part);
+    hipLaunchKernelGGL(k_coarse_kth, dim3((unsigned)nq), dim3(CNT), 0, s, part, n_chunk, n_guard, qi, cq);
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_compact(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq,
+                                 int64_t cu, const uint32_t *part, const int32_t *cq, int32_t *loc, int32_t *ctot,
+                                 int32_t *cbase, int32_t *cl_u, int32_t *cl_b) {
+    if (nq <= 0 || n_user <= 0) return hipSuccess;
+    if (ldc % 16 || cu % 1024 || nq > 65535) return hipErrorInvalidValue;
+    const int32_t n_chunk = (int32_t)((n_user + cu - 1) / cu);
+    hipLaunchKernelGGL(k_coarse_chunk_offsets, dim3((unsigned)n_chunk), dim3(NT), 0, s, part, n_chunk, nq, cq, loc,
+                       ctot);
+    hipLaunchKernelGGL(k_coarse_chunk_base, dim3(1), dim3(NT), 0, s, ctot, n_chunk, cbase);
+    hipLaunchKernelGGL(k_coarse_compact, dim3((unsigned)n_chunk, (unsigned)nq), dim3(CNT), 0, s, cb, ldc, n_user, cu,
+                       nq, cq, loc, cbase, cl_u, cl_b);
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_exact(hipStream_t s, const float *users, const float *queries, int32_t ld, int32_t dim,
+                               const float *table, int32_t S, const int32_t *ranks, int64_t n_item, float eps,
+                               const GuardCols &gc, int32_t nq, const int32_t *cl_u, const int32_t *cl_b, int64_t n,
+                               uint32_t *bk_c, float *s_c, uint32_t *ghist, QueryInfo *qi) {
+    if (nq <= 0) return hipSuccess;
+    const int same = eps == 0.0f ? 1 : 0;
+    hipError_t e = hipMemsetAsync(ghist, 0, sizeof(uint32_t) * 2 * (size_t)(S + 1) * nq, s);
+    if (e != hipSuccess) return e;
+    if (n > 0)
+        hipLaunchKernelGGL(k_coarse_exact, dim3((unsigned)((n + CNT - 1) / CNT)), dim3(CNT), 0, s, users, queries, ld,
+                           dim, table, S, eps, same, gc, cl_u, cl_b, n, bk_c, s_c, ghist);
+    hipLaunchKernelGGL(k_hist_kth, dim3((unsigned)nq), dim3(NT), 0, s, ghist, S, same, ranks, n_item,
+                       (int)RKR_OUTPUT_EXACT, qi);
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_final(hipStream_t s, const int32_t *cl_u, const int32_t *cl_b, const uint32_t *bk_c,
+                               const float *s_c, int64_t n, int32_t S, const int32_t *ranks, int64_t n_item,
+                               int64_t n_user, QueryInfo *qi, int32_t *cand_user, int32_t *cand_rank,
+                               int32_t *refine_total, int32_t *refine_cand, int32_t *refine_user,
+                               int32_t *refine_query, float *refine_thr, float *qip) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_coarse_final, dim3((unsigned)((n + CNT - 1) / CNT)), dim3(CNT), 0, s, cl_u, cl_b, bk_c, s_c, n,
+                       S, ranks, n_item, n_user, qi, cand_user, cand_rank, refine_total, refine_cand, refine_user,
+                       refine_query, refine_thr, qip);
     return hipGetLastError();
 }
 

@@ -319,6 +319,123 @@ __global__ void k_recheck(const float *__restrict__ users, const float *__restri
     if (worst > 0.0f) atomicMax(max_err_bits, __float_as_uint(worst));
 }
 
+
+// Coarse buckets of a query batch.  For every (query b, user u) the screened
+// estimate s^ = <u,q_b> (the same BF16 hi/lo three-product form as above,
+// |s^ - s| <= e = C ||u|| ||q|| + abs_floor) is compared with G <= 7 guard
+// values of u's table row (descending; -inf past G):
+//   c_lo = #{g : guard_g > s^ - e},  c_hi = #{g : guard_g > s^ + e}
+// and cb[b * ldc + u] = c_lo | c_hi << 4.  abs_floor also carries the
+// table's own eps.  Queries are the A operand (staged through LDS, 32 per
+// stage), users the B operand held in VGPRs: lane (lr, lq) of a 16x16 tile
+// owns user lr and queries 4 lq .. 4 lq + 3, so its guards stay in registers.
+struct IpCoarseParams {
+    const float *users;           // n_user x ld
+    const float *user_norm;       // n_user
+    const float *guards;          // n_user x 8
+    int64_t n_user;
+    const uint16_t *q_hi, *q_lo;  // nq x Kp
+    const float *q_norm;          // nq
+    int32_t nq;
+    int32_t ld, K, Kp;
+    float C, abs_floor;
+    uint8_t *cb;                  // nq rows of ldc bytes
+    int64_t ldc;
+};
+
+template <int KS>
+__global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
+    constexpr int QST = 32;                                          // queries per LDS stage
+    constexpr int ROWQ = (KS * 32 + KPAD_LDS) / 8;
+    constexpr int KQ = KS * 4;
+    __shared__ uint4 sA[2][QST * ROWQ];
+    __shared__ float sQn[QST];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int64_t n_rb = (P.n_user + 127) / 128;
+    for (int64_t rb = blockIdx.x; rb < n_rb; rb += gridDim.x) {
+        const int64_t row_base = rb * 128 + wave * 32;
+        Frag uh[2][KS], ul[2][KS];
+        float gd[2][8], ar[2];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int64_t r = row_base + b * 16 + lr;
+            const bool live = r < P.n_user;
+            const float *urow = live ? P.users + r * P.ld : nullptr;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int k = ks * 32 + 8 * lq + i;
+                    const float v = (live && k < P.K) ? urow[k] : 0.0f;
+                    const uint16_t h = bf16_bits(v);
+                    uh[b][ks].h[i] = h;
+                    ul[b][ks].h[i] = bf16_bits(v - bf16_to_f32(h));
+                }
+            float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f), g1 = g0;
+            if (live) {
+                g0 = reinterpret_cast<const float4 *>(P.guards + r * 8)[0];
+                g1 = reinterpret_cast<const float4 *>(P.guards + r * 8)[1];
+            }
+            gd[b][0] = g0.x; gd[b][1] = g0.y; gd[b][2] = g0.z; gd[b][3] = g0.w;
+            gd[b][4] = g1.x; gd[b][5] = g1.y; gd[b][6] = g1.z; gd[b][7] = g1.w;
+            ar[b] = live ? P.C * P.user_norm[r] : 0.0f;
+        }
+        for (int32_t q0 = 0; q0 < P.nq; q0 += QST) {
+            __syncthreads();  // every wave is done with the previous stage
+            for (int c = threadIdx.x; c < 2 * QST * KQ; c += NT) {
+                const int half = c / (QST * KQ);
+                const int rem = c % (QST * KQ);
+                const int qq = q0 + rem / KQ;
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (qq < P.nq)
+                    v = reinterpret_cast<const uint4 *>(half ? P.q_lo : P.q_hi)[(int64_t)qq * KQ + rem % KQ];
+                sA[half][(rem / KQ) * ROWQ + rem % KQ] = v;
+            }
+            if (threadIdx.x < QST) sQn[threadIdx.x] = q0 + (int)threadIdx.x < P.nq ? P.q_norm[q0 + threadIdx.x] : 0.f;
+            __syncthreads();
+#pragma unroll
+            for (int g = 0; g < QST / 16; ++g) {
+                const int it = g * 16 + lr;
+                f32x4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) {
+                    Frag qh, ql;
+                    qh.q = sA[0][it * ROWQ + ks * 4 + lq];
+                    ql.q = sA[1][it * ROWQ + ks * 4 + lq];
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qh.v, uh[b][ks].v, acc[b], 0, 0, 0);
+                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qh.v, ul[b][ks].v, acc[b], 0, 0, 0);
+                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ql.v, uh[b][ks].v, acc[b], 0, 0, 0);
+                    }
+                }
+                // D[4 lq + i][lr]: query q0 + g*16 + 4 lq + i, user row_base + b*16 + lr
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int32_t qq = q0 + g * 16 + 4 * lq + i;
+                    if (qq >= P.nq) continue;
+                    const float qn = sQn[g * 16 + 4 * lq + i];
+                    uint8_t *out = P.cb + (int64_t)qq * P.ldc;
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        const int64_t r = row_base + b * 16 + lr;
+                        const float e = ar[b] * qn + P.abs_floor;
+                        const float xl = acc[b][i] - e, xh = acc[b][i] + e;
+                        uint32_t clo = 0, chi = 0;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            clo += gd[b][j] > xl ? 1u : 0u;
+                            chi += gd[b][j] > xh ? 1u : 0u;
+                        }
+                        if (r < P.n_user) out[r] = (uint8_t)(clo | (chi << 4));
+                    }
+                }
+            }
+        }
+    }
+}
+
 }  // namespace
 
 hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, int32_t ld, int32_t K, int32_t Kp,
@@ -398,6 +515,26 @@ hipError_t launch_recheck(hipStream_t s, const float *users, const float *items,
     if (cap <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_recheck, dim3(2048), dim3(256), 0, s, users, items, ld, K, rows_user, rows_thr, rows_query,
                        recheck_n, reinterpret_cast<const int4 *>(recheck), cap, cnt, q_recheck, max_err_bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_ip_coarse(hipStream_t s, const CoarseLaunch &L) {
+    if (L.n_user <= 0 || L.nq <= 0) return hipSuccess;
+    IpCoarseParams P{L.users, L.user_norm, L.guards, L.n_user, L.q_hi, L.q_lo, L.q_norm, L.nq,
+                     L.ld, L.K, L.Kp, L.C, L.abs_floor, L.cb, L.ldc};
+    const int64_t n_rb = (L.n_user + 127) / 128;
+    const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(n_rb, L.grid_cap)));
+    switch (L.Kp / 32) {
+        case 1: hipLaunchKernelGGL(k_ip_coarse<1>, grid, dim3(NT), 0, s, P); break;
+        case 2: hipLaunchKernelGGL(k_ip_coarse<2>, grid, dim3(NT), 0, s, P); break;
+        case 3: hipLaunchKernelGGL(k_ip_coarse<3>, grid, dim3(NT), 0, s, P); break;
+        case 4: hipLaunchKernelGGL(k_ip_coarse<4>, grid, dim3(NT), 0, s, P); break;
+        case 5: hipLaunchKernelGGL(k_ip_coarse<5>, grid, dim3(NT), 0, s, P); break;
+        case 6: hipLaunchKernelGGL(k_ip_coarse<6>, grid, dim3(NT), 0, s, P); break;
+        case 7: hipLaunchKernelGGL(k_ip_coarse<7>, grid, dim3(NT), 0, s, P); break;
+        case 8: hipLaunchKernelGGL(k_ip_coarse<8>, grid, dim3(NT), 0, s, P); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

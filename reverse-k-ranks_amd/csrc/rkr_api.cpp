@@ -103,6 +103,19 @@ DataSnap snap_of(const Ctx *c) {
     d.qtable = c->qtable;
     d.qranks = c->qranks;
     d.n_qs = c->n_qs;
+    // guard columns of the coarse bounds: the table columns nearest below
+    // ranks 4^j - 1 (3, 15, ..., 16383), at most 7, ascending
+    const std::vector<int32_t> &hr = c->qtable ? c->h_qranks : c->h_ranks;
+    d.n_guard = 0;
+    int32_t last = -1;
+    for (int64_t target = 3; target <= 16383 && d.n_guard < 7; target = target * 4 + 3) {
+        const auto it = std::upper_bound(hr.begin(), hr.end(), (int32_t)target);
+        const int32_t j = (int32_t)(it - hr.begin()) - 1;
+        if (j > last) {
+            d.gcol[d.n_guard++] = j;
+            last = j;
+        }
+    }
     return d;
 }
 
@@ -223,7 +236,7 @@ int slot_init(Ctx *c, Slot &sl, bool own_stream, const std::vector<uint32_t> *cu
 void slot_free(Ctx *c, Slot &sl) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     free_bufs(c, {&sl.q_in, &sl.qip, &sl.bk, &sl.qinfo, &sl.cand, &sl.refine_idx, &sl.counts, &sl.out, &sl.recheck,
-                  &sl.small, &sl.hist});
+                  &sl.small, &sl.hist, &sl.q_split, &sl.guards, &sl.coarse_info, &sl.coarse_list});
     for (auto &e : sl.ev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (sl.own_stream && sl.stream) (void)hipStreamDestroy(sl.stream);
@@ -352,6 +365,10 @@ int rkr_set_option(rkr_ctx *c, int option, int64_t value) {
             if (value != 0 && value != 1) return set_err(c, RKR_ERR_INVALID, "dedup must be 0 or 1");
             c->dedup = (int)value;
             return RKR_OK;
+        case RKR_OPT_COARSE:
+            if (value != 0 && value != 1) return set_err(c, RKR_ERR_INVALID, "coarse must be 0 or 1");
+            c->coarse = (int)value;
+            return RKR_OK;
         default:
             return set_err(c, RKR_ERR_INVALID, "unknown option");
     }
@@ -368,6 +385,7 @@ int rkr_get_option(const rkr_ctx *c, int option, int64_t *value) {
         case RKR_OPT_DEDUP: *value = c->dedup; return RKR_OK;
         case RKR_OPT_SCREEN_ROWS: *value = c->screen_rows; return RKR_OK;
         case RKR_OPT_DENSE_TOP: *value = c->dense_top; return RKR_OK;
+        case RKR_OPT_COARSE: *value = c->coarse; return RKR_OK;
         default: return RKR_ERR_INVALID;
     }
 }
@@ -726,11 +744,15 @@ int validate_queries(Ctx *c, const DataSnap &d, const float *queries, int32_t n_
     return RKR_OK;
 }
 
-int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int32_t n_query, const int32_t *topk,
-                  int32_t max_k, int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
-                  rkr_batch_timing *timing) {
-    using clock = std::chrono::steady_clock;
-    const auto t_start = clock::now();
+namespace {
+
+// Device batches of query_on_slot.  allow_coarse: screened-form exact batches
+// take the coarse bounds; queries whose coarse list would be empty of a k-th
+// bound or too long are appended to *fallback (indices into queries) with
+// their outputs left unwritten.
+int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int32_t n_query, const int32_t *topk,
+                int32_t max_k, int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
+                rkr_batch_timing &tm, bool allow_coarse, std::vector<int32_t> *fallback) {
     const int64_t U = dv.n_user, I = dv.n_item;
     const int32_t ld = c->ld, d = c->dim;
     const int mode = c->output;
@@ -743,6 +765,8 @@ int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, in
     const float eps = mode == RKR_OUTPUT_EXACT && !use_q ? c->table_eps : 0.0f;
     // pending item inserts/deletes: bounds become per-user values (k_update.hip)
     const bool upd = dv.n_ins > 0 || dv.n_del > 0;
+    const bool coarse = allow_coarse && c->coarse && c->form == RKR_FORM_SCREENED && mode == RKR_OUTPUT_EXACT && !upd &&
+                        c->kp > 0 && dv.user_norm && dv.n_guard > 0 && U > 0;
     const size_t pair_bytes = upd ? 12 : 8;  // <u,q> + buckets, or + two bound values
 
     // queries per device batch
@@ -757,16 +781,30 @@ int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, in
     int rc;
     if ((rc = ensure(c, sl.q_in, sizeof(float) * (size_t)nqb * ld, "query vectors"))) return rc;
     if ((rc = ensure(c, sl.qip, sizeof(float) * (size_t)nqb * U, "query scores"))) return rc;
-    if ((rc = ensure(c, sl.bk, (pair_bytes - 4) * (size_t)nqb * U, "bucket pairs"))) return rc;
+    const int64_t ldc = (U + 15) / 16 * 16;  // coarse bucket rows
+    if ((rc = ensure(c, sl.bk, std::max<size_t>((pair_bytes - 4) * (size_t)U, (size_t)ldc) * (size_t)nqb,
+                     "bucket pairs")))
+        return rc;
     if ((rc = ensure(c, sl.qinfo, sizeof(QueryInfo) * (size_t)nqb + 64, "query info"))) return rc;
     if ((rc = ensure(c, sl.out, (sizeof(int32_t) * 2 + sizeof(float)) * (size_t)nqb * max_k, "outputs"))) return rc;
     if ((rc = ensure(c, sl.hist, sizeof(uint32_t) * hist_words((int32_t)nqb, S), "bound histograms"))) return rc;
+    GuardCols gc{};
+    if (coarse) {
+        if ((rc = ensure(c, sl.q_split, (sizeof(uint16_t) * 2 * c->kp + sizeof(float)) * (size_t)nqb, "query BF16 rows")))
+            return rc;
+        if ((rc = ensure(c, sl.guards, sizeof(float) * 8 * (size_t)U, "guard values"))) return rc;
+        for (int j = 0; j < dv.n_guard; ++j) gc.col[j] = dv.gcol[j];
+        gc.n = dv.n_guard;
+        RKR_HIP(c, launch_guard_rows(sl.stream, T, U, S, gc, reinterpret_cast<float *>(sl.guards.p)));
+    }
+    // a coarse list longer than this takes the full path instead
+    const int64_t coarse_cap = std::max<int64_t>(4096, U / 4);
 
     std::vector<float> hq((size_t)nqb * ld, 0.0f);
     std::vector<QueryInfo> hqi((size_t)nqb);
     std::vector<int32_t> ho_user((size_t)nqb * max_k), ho_rank((size_t)nqb * max_k);
     std::vector<float> ho_qip((size_t)nqb * max_k);
-    rkr_batch_timing tm{};
+    std::vector<int32_t> hcq;
 
     for (int64_t q0 = 0; q0 < n_query; q0 += nqb) {
         const int32_t nq = (int32_t)std::min<int64_t>(nqb, n_query - q0);
@@ -788,25 +826,86 @@ int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, in
         RKR_HIP(c, hipMemcpyAsync(q_in, hq.data(), sizeof(float) * (size_t)nq * ld, hipMemcpyHostToDevice, sl.stream));
         RKR_HIP(c, hipMemcpyAsync(qi, hqi.data(), sizeof(QueryInfo) * nq, hipMemcpyHostToDevice, sl.stream));
         RKR_HIP(c, hipEventRecord(sl.ev[1], sl.stream));
-        // <u, q> for every (query, user): qip[b * U + u]
-        if (nq <= 16 && (size_t)nq * d * sizeof(float) <= 65536)
-            RKR_HIP(c, launch_gemv_panels(sl.stream, dv.usersT, U, ld, d, q_in, nq, qip));
-        else RKR_HIP(c, launch_scores_store(sl.stream, dv.users, nullptr, U, ld, q_in, nq, ld, d, qip, U, false));
-        RKR_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
         int32_t *lbv = reinterpret_cast<int32_t *>(bk);
         int32_t *ubv = lbv + (size_t)nq * U;
-        if (upd)
-            RKR_HIP(c, launch_bounds_upd(sl.stream, qip, T, U, S, R, dv.n_item_base, eps, dv.side_ins,
-                                         dv.n_ins, dv.ld_ins, dv.side_del, dv.n_del, dv.ld_del, nq, lbv, ubv));
-        else
-            RKR_HIP(c, launch_bounds(sl.stream, qip, T, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
-        RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
         const int64_t cu = chunk_users(U, nq, c->n_cu);
         uint32_t *ghist = reinterpret_cast<uint32_t *>(sl.hist.p);
-        if (upd) RKR_HIP(c, launch_kth_values(sl.stream, lbv, ubv, U, nq, cu, ghist, qi));
-        else
-            RKR_HIP(c, launch_hist_kth(sl.stream, bk, U, S, R, I, nq, mode,
-                                       (mode == RKR_OUTPUT_EXACT && eps == 0.0f) ? 1 : 0, cu, ghist, qi));
+        int32_t *cl_u = nullptr, *cl_b = nullptr;
+        uint32_t *bk_c = nullptr;
+        float *s_c = nullptr;
+        int64_t n_listed = 0;
+        if (coarse) {
+            // coarse buckets on the matrix cores, the coarse k-th bound and
+            // list sizes; exact scores and bucket pairs of the listed users
+            uint16_t *q_hi = reinterpret_cast<uint16_t *>(sl.q_split.p);
+            uint16_t *q_lo = q_hi + (size_t)nqb * c->kp;
+            float *qn = reinterpret_cast<float *>(q_lo + (size_t)nqb * c->kp);
+            RKR_HIP(c, launch_prep_items(sl.stream, q_in, nq, ld, d, c->kp, q_hi, q_lo, qn));
+            const double mq = max_abs(hq.data(), (size_t)nq * ld);
+            const double absf = ((double)screen_abs_floor(c->kp, dv.max_abs_user, mq) + (double)eps) *
+                                (1.0 + std::ldexp(1.0, -18));
+            uint8_t *cb = reinterpret_cast<uint8_t *>(bk);
+            CoarseLaunch L{};
+            L.users = dv.users; L.user_norm = dv.user_norm; L.guards = reinterpret_cast<const float *>(sl.guards.p);
+            L.n_user = U; L.q_hi = q_hi; L.q_lo = q_lo; L.q_norm = qn; L.nq = nq; L.ld = ld; L.K = d; L.Kp = c->kp;
+            L.C = c->screen_C; L.abs_floor = std::nextafter((float)absf, INFINITY); L.cb = cb; L.ldc = ldc;
+            L.grid_cap = (int64_t)c->n_cu * 2;
+            RKR_HIP(c, launch_ip_coarse(sl.stream, L));
+            RKR_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
+            const int64_t n_chunk = (U + cu - 1) / cu;
+            if ((rc = ensure(c, sl.coarse_info, sizeof(uint32_t) * (17 * (size_t)nq * n_chunk + 4 * (size_t)nqb +
+                                                                   2 * (size_t)n_chunk + 64), "coarse counts")))
+                return rc;
+            uint32_t *part = reinterpret_cast<uint32_t *>(sl.coarse_info.p);
+            int32_t *cq = reinterpret_cast<int32_t *>(part + 16 * (size_t)nq * n_chunk);
+            int32_t *loc = cq + 4 * (size_t)nqb;
+            int32_t *ctot = loc + (size_t)nq * n_chunk;
+            int32_t *cbase = ctot + n_chunk;
+            RKR_HIP(c, launch_coarse_kth(sl.stream, cb, ldc, U, nq, cu, dv.n_guard, part, qi, cq));
+            hcq.assign((size_t)nq * 4, 0);
+            RKR_HIP(c, hipMemcpyAsync(hcq.data(), cq, sizeof(int32_t) * 4 * nq, hipMemcpyDeviceToHost, sl.stream));
+            RKR_HIP(c, hipStreamSynchronize(sl.stream));
+            for (int32_t i = 0; i < nq; ++i) {
+                int32_t *e = &hcq[(size_t)i * 4];
+                if (e[0] < 0 || e[1] > coarse_cap) {
+                    if (fallback) fallback->push_back((int32_t)(q0 + i));
+                    tm.n_fallback += 1;
+                    e[0] = -1;
+                    e[1] = 0;
+                }
+                e[2] = (int32_t)n_listed;
+                e[3] = 0;
+                n_listed += e[1];
+            }
+            if (n_listed >= INT32_MAX) return set_err(c, RKR_ERR_NOMEM, "query: coarse list exceeds 2^31; lower the query batch");
+            tm.n_listed += n_listed;
+            if ((rc = ensure(c, sl.coarse_list, 16 * (size_t)std::max<int64_t>(n_listed, 1), "coarse list"))) return rc;
+            cl_u = reinterpret_cast<int32_t *>(sl.coarse_list.p);
+            cl_b = cl_u + std::max<int64_t>(n_listed, 1);
+            bk_c = reinterpret_cast<uint32_t *>(cl_b + std::max<int64_t>(n_listed, 1));
+            s_c = reinterpret_cast<float *>(bk_c + std::max<int64_t>(n_listed, 1));
+            RKR_HIP(c, hipMemcpyAsync(cq, hcq.data(), sizeof(int32_t) * 4 * nq, hipMemcpyHostToDevice, sl.stream));
+            RKR_HIP(c, launch_coarse_compact(sl.stream, cb, ldc, U, nq, cu, part, cq, loc, ctot, cbase, cl_u, cl_b));
+            RKR_HIP(c, launch_coarse_exact(sl.stream, dv.users, q_in, ld, d, T, S, R, I, eps, gc, nq, cl_u, cl_b,
+                                           n_listed, bk_c, s_c, ghist, qi));
+            RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
+        } else {
+            // <u, q> for every (query, user): qip[b * U + u]
+            if (nq <= 16 && (size_t)nq * d * sizeof(float) <= 65536)
+                RKR_HIP(c, launch_gemv_panels(sl.stream, dv.usersT, U, ld, d, q_in, nq, qip));
+            else RKR_HIP(c, launch_scores_store(sl.stream, dv.users, nullptr, U, ld, q_in, nq, ld, d, qip, U, false));
+            RKR_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
+            if (upd)
+                RKR_HIP(c, launch_bounds_upd(sl.stream, qip, T, U, S, R, dv.n_item_base, eps, dv.side_ins,
+                                             dv.n_ins, dv.ld_ins, dv.side_del, dv.n_del, dv.ld_del, nq, lbv, ubv));
+            else
+                RKR_HIP(c, launch_bounds(sl.stream, qip, T, U, S, nq, eps, mode == RKR_OUTPUT_REFERENCE ? 1 : 0, bk));
+            RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
+            if (upd) RKR_HIP(c, launch_kth_values(sl.stream, lbv, ubv, U, nq, cu, ghist, qi));
+            else
+                RKR_HIP(c, launch_hist_kth(sl.stream, bk, U, S, R, I, nq, mode,
+                                           (mode == RKR_OUTPUT_EXACT && eps == 0.0f) ? 1 : 0, cu, ghist, qi));
+        }
         RKR_HIP(c, launch_offsets(sl.stream, qi, nq, refine_total));
         RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
         RKR_HIP(c, hipStreamSynchronize(sl.stream));
@@ -832,7 +931,11 @@ int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, in
         uint32_t *max_err_bits = reinterpret_cast<uint32_t *>(recheck_n + 1);
         const bool screened = c->form == RKR_FORM_SCREENED;
 
-        if (upd)
+        if (coarse)
+            RKR_HIP(c, launch_coarse_final(sl.stream, cl_u, cl_b, bk_c, s_c, n_listed, S, R, I, U, qi, cand_user,
+                                           cand_rank, refine_total, refine_cand, refine_user, refine_query, refine_thr,
+                                           qip));
+        else if (upd)
             RKR_HIP(c, launch_compact_values(sl.stream, lbv, ubv, qip, U, nq, cu, qi, cand_user, cand_rank,
                                              refine_total, refine_cand, refine_user, refine_query, refine_thr));
         else
@@ -941,6 +1044,45 @@ int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, in
             float e;
             std::memcpy(&e, &h_err_bits, sizeof(e));
             tm.max_screen_err = std::max(tm.max_screen_err, (double)e);
+        }
+    }
+    return RKR_OK;
+}
+
+}  // namespace
+
+int query_on_slot(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int32_t n_query, const int32_t *topk,
+                  int32_t max_k, int32_t *out_user, int32_t *out_rank, float *out_qip, rkr_query_stats *stats,
+                  rkr_batch_timing *timing) {
+    using clock = std::chrono::steady_clock;
+    const auto t_start = clock::now();
+    rkr_batch_timing tm{};
+    std::vector<int32_t> fb;
+    int rc = run_batches(c, sl, dv, queries, n_query, topk, max_k, out_user, out_rank, out_qip, stats, tm, true, &fb);
+    if (rc) return rc;
+    if (!fb.empty()) {
+        // queries the coarse bounds could not narrow enough: full path
+        const int32_t n = (int32_t)fb.size(), d = c->dim;
+        std::vector<float> fq((size_t)n * d);
+        std::vector<int32_t> fk(n), fu((size_t)n * max_k), fr((size_t)n * max_k);
+        std::vector<float> fqip((size_t)n * max_k);
+        std::vector<rkr_query_stats> fs(stats ? n : 0);
+        for (int32_t i = 0; i < n; ++i) {
+            std::memcpy(&fq[(size_t)i * d], queries + (size_t)fb[i] * d, sizeof(float) * d);
+            fk[i] = topk[fb[i]];
+        }
+        rc = run_batches(c, sl, dv, fq.data(), n, fk.data(), max_k, fu.data(), fr.data(), fqip.data(),
+                         stats ? fs.data() : nullptr, tm, false, nullptr);
+        if (rc) return rc;
+        for (int32_t i = 0; i < n; ++i) {
+            const size_t o = (size_t)fb[i] * max_k, f = (size_t)i * max_k;
+            std::memcpy(out_user + o, &fu[f], sizeof(int32_t) * max_k);
+            std::memcpy(out_rank + o, &fr[f], sizeof(int32_t) * max_k);
+            if (out_qip) std::memcpy(out_qip + o, &fqip[f], sizeof(float) * max_k);
+            if (stats) {
+                stats[fb[i]] = fs[i];
+                stats[fb[i]].query_id = fb[i];
+            }
         }
     }
     tm.total_ms = std::chrono::duration<double, std::milli>(clock::now() - t_start).count();

@@ -38,6 +38,7 @@ struct Slot {
     bool own_stream = false;
     hipEvent_t ev[16] = {};
     DeviceBuf q_in, qip, bk, qinfo, cand, refine_idx, counts, out, recheck, small, hist;
+    DeviceBuf q_split, guards, coarse_info, coarse_list;  // coarse bounds
 };
 
 struct Ctx {
@@ -57,6 +58,7 @@ struct Ctx {
     int32_t screen_chunk = 0;   // items per screened work-unit chunk (0 = no limit)
     int32_t screen_rows = 0;    // refine rows per wavefront of the screen (0 = 32)
     int dedup = 1;              // answer byte-equal queries of one call once
+    int coarse = 1;             // coarse bounds on matrix cores (RKR_OPT_COARSE)
 
     // data
     int64_t n_user = 0, n_item = 0;
@@ -176,6 +178,8 @@ struct DataSnap {
     const float *qtable = nullptr;   // query table (exact output), or null
     const int32_t *qranks = nullptr;
     int32_t n_qs = 0;
+    int32_t n_guard = 0;             // guard columns of the coarse bounds
+    int32_t gcol[8] = {};
 };
 DataSnap snap_of(const Ctx *c);
 // Ranks of the query table: the sample ranks and the dense top ranks below
@@ -252,6 +256,20 @@ hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, 
                              uint16_t *hi, uint16_t *lo, float *norm);
 hipError_t launch_user_norms(hipStream_t s, const float *users, int64_t n_user, int32_t ld, int32_t K, float *norm);
 hipError_t launch_refine_screen(hipStream_t s, const ScreenLaunch &L);
+// Coarse buckets of a batch (screened <u,q> against <= 7 guard columns of
+// every user's table row): cb[b * n_user + u] = c_lo | c_hi << 4.
+struct CoarseLaunch {
+    const float *users, *user_norm, *guards;  // guards: n_user x 8, descending, -inf padded
+    int64_t n_user;
+    const uint16_t *q_hi, *q_lo;
+    const float *q_norm;
+    int32_t nq, ld, K, Kp;
+    float C, abs_floor;
+    uint8_t *cb;            // nq rows of ldc bytes
+    int64_t ldc;
+    int64_t grid_cap;
+};
+hipError_t launch_ip_coarse(hipStream_t s, const CoarseLaunch &L);
 hipError_t launch_recheck(hipStream_t s, const float *users, const float *items, int32_t ld, int32_t K,
                           const int32_t *rows_user, const float *rows_thr, const int32_t *rows_query,
                           const int32_t *recheck_n, const void *recheck, int64_t cap, int32_t *cnt,
@@ -329,6 +347,32 @@ hipError_t launch_bounds(hipStream_t s, const float *qip, const float *table, in
 hipError_t launch_hist_kth(hipStream_t s, const uint32_t *bk, int64_t n_user, int32_t S, const int32_t *ranks,
                            int64_t n_item, int32_t nq, int output_mode, int same, int64_t cu, uint32_t *ghist,
                            QueryInfo *qi);
+// Coarse-bound pipeline of exact output (k_query.hip): guard values per
+// user, the coarse k-th bound and list size per query (cq[b] = {beta_c,
+// count, offset, fill}), the listed users, their exact scores and bucket
+// pairs with the exact k-th bound (into qi), and the final candidates.
+struct GuardCols {
+    int32_t col[8];
+    int32_t n;
+};
+hipError_t launch_guard_rows(hipStream_t s, const float *table, int64_t n_user, int32_t S, const GuardCols &gc,
+                             float *guards);
+// cb rows are ldc bytes (ldc % 16 == 0); cu % 1024 == 0
+hipError_t launch_coarse_kth(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq, int64_t cu,
+                             int32_t n_guard, uint32_t *part, const QueryInfo *qi, int32_t *cq);
+// chunk-major list: loc (n_chunk x nq), ctot and cbase (n_chunk) are scratch
+hipError_t launch_coarse_compact(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq,
+                                 int64_t cu, const uint32_t *part, const int32_t *cq, int32_t *loc, int32_t *ctot,
+                                 int32_t *cbase, int32_t *cl_u, int32_t *cl_b);
+hipError_t launch_coarse_exact(hipStream_t s, const float *users, const float *queries, int32_t ld, int32_t dim,
+                               const float *table, int32_t S, const int32_t *ranks, int64_t n_item, float eps,
+                               const GuardCols &gc, int32_t nq, const int32_t *cl_u, const int32_t *cl_b, int64_t n,
+                               uint32_t *bk_c, float *s_c, uint32_t *ghist, QueryInfo *qi);
+hipError_t launch_coarse_final(hipStream_t s, const int32_t *cl_u, const int32_t *cl_b, const uint32_t *bk_c,
+                               const float *s_c, int64_t n, int32_t S, const int32_t *ranks, int64_t n_item,
+                               int64_t n_user, QueryInfo *qi, int32_t *cand_user, int32_t *cand_rank,
+                               int32_t *refine_total, int32_t *refine_cand, int32_t *refine_user,
+                               int32_t *refine_query, float *refine_thr, float *qip);
 hipError_t launch_offsets(hipStream_t s, QueryInfo *qi, int32_t nq, int32_t *refine_total);
 hipError_t launch_compact(hipStream_t s, const uint32_t *bk, const float *qip, int64_t n_user, int32_t S,
                           const int32_t *ranks, int64_t n_item, int32_t nq, int output_mode, int64_t cu,

@@ -32,6 +32,7 @@ RKR_OPT_SCREEN_CHUNK = 5
 RKR_OPT_DEDUP = 6
 RKR_OPT_SCREEN_ROWS = 7
 RKR_OPT_DENSE_TOP = 8
+RKR_OPT_COARSE = 9
 
 STATUS_NAMES = {
     RKR_OK: "RKR_OK", RKR_ERR_INVALID: "RKR_ERR_INVALID", RKR_ERR_STATE: "RKR_ERR_STATE",
@@ -68,7 +69,7 @@ class BatchTiming(C.Structure):
                 ("topk_ms", C.c_double),
                 ("h2d_ms", C.c_double), ("d2h_ms", C.c_double), ("n_batches", C.c_int64),
                 ("n_refined", C.c_int64), ("n_rechecked", C.c_int64), ("n_screened", C.c_int64),
-                ("max_screen_err", C.c_double)]
+                ("max_screen_err", C.c_double), ("n_listed", C.c_int64), ("n_fallback", C.c_int64)]
 
 
 class ServerConfig(C.Structure):

@@ -266,6 +266,80 @@ def test_screened_equals_plain(orc, gpu_index_factory, nq, k, dense):
     assert 0 < scr.timing["max_screen_err"] < 1e-3
 
 
+_PRUNE_FIELDS = ("n_candidate", "n_refine_user", "n_prune_user", "n_result_user")
+
+
+def _same_answers(a, b, ks, stats=True):
+    for i, k in enumerate(ks):
+        assert np.array_equal(a.users[i, :k], b.users[i, :k]), f"query {i}: users differ"
+        assert np.array_equal(a.ranks[i, :k], b.ranks[i, :k]), f"query {i}: ranks differ"
+        assert np.array_equal(a.qips[i, :k], b.qips[i, :k]), f"query {i}: <u,q> differ"
+        if stats:
+            for f in _PRUNE_FIELDS:
+                assert a.stats[i][f] == b.stats[i][f], f"query {i}: {f} {a.stats[i][f]} != {b.stats[i][f]}"
+
+
+@pytest.mark.parametrize("d,dense,nq", [(150, 1, 70), (150, 0, 70), (37, 1, 70), (20, 0, 33), (150, 1, 1),
+                                        (64, 1, 5)])
+def test_coarse_bounds(orc, gpu_index_factory, d, dense, nq):
+    """Coarse bounds (screened <u,q> on BF16 matrix cores against guard
+    columns, exact scores and searches for the listed users only) keep the
+    full path's candidates, refinements and answers, bit for bit."""
+    _case(f"coarse bounds d={d} dense={dense} nq={nq}")
+    rng = np.random.default_rng(90 + d + dense + nq)
+    U, I = 5003, 2100
+    users = rng.standard_normal((U, d), dtype=np.float32) * rng.lognormal(0, 0.3, (U, 1)).astype(np.float32)
+    items = (rng.standard_normal((I, d), dtype=np.float32) / np.float32(np.sqrt(d))
+             * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32))
+    # half the queries are items (ties with their own score), half are not
+    queries = np.concatenate([items[rng.choice(I, (nq + 1) // 2, replace=False)],
+                              rng.standard_normal((nq // 2, d), dtype=np.float32) / np.float32(np.sqrt(d))])
+    idx, _ = gpu_index_factory(users, items, 64, dense_top=dense)
+    ks = [int(k) for k in rng.integers(1, 150, nq)]
+    plain = idx.query(queries, ks)
+    idx.set_form("screened")
+    idx.set_option(9, 0)  # RKR_OPT_COARSE
+    full = idx.query(queries, ks)
+    idx.set_option(9, 1)
+    coarse = idx.query(queries, ks)
+    _same_answers(full, coarse, ks)
+    _same_answers(plain, coarse, ks)
+    _check_exact(orc, _normalized(orc, users), items, queries, coarse, ks)
+
+
+def test_coarse_bounds_fallback(orc, gpu_index_factory):
+    """Queries the guards cannot narrow (k beyond every guard's reach, or a
+    list longer than the cap) take the full path inside the same call."""
+    _case("coarse bounds fallback")
+    rng = np.random.default_rng(7)
+    U, I, d = 5003, 2100, 48
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = rng.standard_normal((I, d), dtype=np.float32) / np.float32(np.sqrt(d))
+    # negated items rank near the bottom for most users; k up to 4000
+    queries = np.concatenate([-items[:6], items[6:12], rng.standard_normal((6, d), dtype=np.float32)])
+    ks = [4000, 3000, 2500, 50, 10, 1, 4000, 3500, 200, 100, 20, 5, 4000, 2000, 900, 30, 3, 1]
+    idx, _ = gpu_index_factory(users, items, 64)
+    plain = idx.query(queries, ks)
+    idx.set_form("screened")
+    idx.set_option(9, 0)
+    full = idx.query(queries, ks)
+    idx.set_option(9, 1)
+    coarse = idx.query(queries, ks)
+    _same_answers(full, coarse, ks)
+    _same_answers(plain, coarse, ks)
+    _check_exact(orc, _normalized(orc, users), items, queries, coarse, ks)
+    # positive users and items, negative queries: every query ranks last for
+    # every user, below all guards, so no coarse k-th bound exists
+    users, items = np.abs(users[:800]), np.abs(items[:300])
+    queries = -np.abs(rng.standard_normal((20, d), dtype=np.float32))
+    ks = [int(k) for k in rng.integers(1, 60, 20)]
+    idx, _ = gpu_index_factory(users, items, 32)
+    idx.set_form("screened")
+    coarse = idx.query(queries, ks)
+    _check_exact(orc, _normalized(orc, users), items, queries, coarse, ks)
+    assert all(coarse.ranks[i, 0] == 300 for i in range(20))
+
+
 @pytest.mark.parametrize("d", [150, 64, 20])
 def test_screened_64_rows_per_wave(orc, gpu_index_factory, d):
     """The 64-rows-per-wavefront screen tile gives the same answers as the
