@@ -392,7 +392,8 @@ def main():
                        "recheck_ms_per_step": round(tot["recheck_ms"] / args.steps, 3)}
                       if args.form == "screened" else None),
         "coarse_bounds": ({"listed_per_query": round(tot["n_listed"] / max(1, args.steps) / nq, 1),
-                           "fallback_queries_per_step": round(tot["n_fallback"] / max(1, args.steps), 2)}
+                           "fallback_queries_per_step": round(tot["n_fallback"] / max(1, args.steps), 2),
+                           "second_pass_queries_per_step": round(tot["n_coarse_refined"] / max(1, args.steps), 2)}
                           if tot.get("n_listed", 0) or tot.get("n_fallback", 0) else None),
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -102,6 +102,7 @@ typedef struct rkr_batch_timing {
     double max_screen_err;  /* screened form: max |screen - FP64| observed     */
     int64_t n_listed;       /* coarse bounds: (query, user) pairs scored exactly */
     int64_t n_fallback;     /* coarse bounds: queries that took the full path  */
+    int64_t n_coarse_refined; /* coarse bounds: queries given a second, finer pass */
 } rkr_batch_timing;
 
 /* ---- context ------------------------------------------------------------ */

@@ -541,20 +541,6 @@ __device__ __forceinline__ float rule_dot(const float *__restrict__ a, const flo
     return s;
 }
 
-// guards[u * 8 + g] = T[u][col[g]] for g < n, -inf after.
-__global__ void k_guard_rows(const float *__restrict__ table, int64_t n_user, int32_t S, GuardCols gc,
-                             float *__restrict__ guards) {
-    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= n_user) return;
-    const float *row = table + u * (int64_t)S;
-    float g[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) g[j] = j < gc.n ? row[gc.col[j]] : -INFINITY;
-    float4 *o = reinterpret_cast<float4 *>(guards + u * 8);
-    o[0] = make_float4(g[0], g[1], g[2], g[3]);
-    o[1] = make_float4(g[4], g[5], g[6], g[7]);
-}
-
 __device__ __forceinline__ void coarse_flush(uint64_t alo, uint64_t ahi, uint32_t *h) {
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
@@ -565,14 +551,16 @@ __device__ __forceinline__ void coarse_flush(uint64_t alo, uint64_t ahi, uint32_
 }
 
 // grid (chunks, queries): counts of c_lo and c_hi values 0..6 of the chunk's
-// users, part[(b * n_chunk + chunk) * 16 + {0..6 | 8..14}].  16 users per
+// users, part[(b * n_chunk + chunk) * 16 + {0..6 | 8..14}] (b = qmap[y] when
+// given).  16 users per
 // thread and load (rows of cb are ldc bytes, ldc % 16 == 0, chunk starts at
 // multiples of 1024); per-thread counters are bytes of a 64-bit word,
 // flushed before they can carry.
 __global__ void __launch_bounds__(CNT) k_coarse_hist(const uint8_t *__restrict__ cb, int64_t ldc, int64_t n_user,
-                                                     int64_t cu, int32_t n_chunk, uint32_t *__restrict__ part) {
+                                                     int64_t cu, int32_t n_chunk, const int32_t *__restrict__ qmap,
+                                                     uint32_t *__restrict__ part) {
     __shared__ uint32_t h[16];
-    const int64_t b = blockIdx.y;
+    const int64_t b = qmap ? qmap[blockIdx.y] : blockIdx.y;
     if (threadIdx.x < 16) h[threadIdx.x] = 0;
     __syncthreads();
     const uint8_t *v = cb + b * ldc;
@@ -607,10 +595,10 @@ __global__ void __launch_bounds__(CNT) k_coarse_hist(const uint8_t *__restrict__
 // #{c_lo <= c} >= k (-1 when there is none: the query takes the full path)
 // and the number of users with c_hi <= beta_c.  cq[b] = {beta_c, count, -, 0}.
 __global__ void __launch_bounds__(CNT) k_coarse_kth(const uint32_t *__restrict__ part, int32_t n_chunk,
-                                                    int32_t n_guard, const QueryInfo *__restrict__ qi,
-                                                    int32_t *__restrict__ cq) {
+                                                    int32_t n_guard, const int32_t *__restrict__ qmap,
+                                                    const QueryInfo *__restrict__ qi, int32_t *__restrict__ cq) {
     __shared__ uint32_t sh[CNT];
-    const int64_t b = blockIdx.x;
+    const int64_t b = qmap ? qmap[blockIdx.x] : blockIdx.x;
     uint32_t acc = 0;
     for (int32_t ch = threadIdx.x >> 4; ch < n_chunk; ch += CNT / 16)
         acc += part[((int64_t)b * n_chunk + ch) * 16 + (threadIdx.x & 15)];
@@ -744,7 +732,8 @@ __global__ void __launch_bounds__(CNT) k_coarse_compact(const uint8_t *__restric
 __global__ void __launch_bounds__(CNT) k_coarse_exact(const float *__restrict__ users,
                                                       const float *__restrict__ queries, int32_t ld, int32_t dim,
                                                       const float *__restrict__ table, int32_t S, float eps, int same,
-                                                      GuardCols gc, const int32_t *__restrict__ cl_u,
+                                                      const GuardCols *__restrict__ gsets,
+                                                      const int32_t *__restrict__ gset, const int32_t *__restrict__ cl_u,
                                                       const int32_t *__restrict__ cl_b, int64_t n,
                                                       uint32_t *__restrict__ bk_c, float *__restrict__ s_c,
                                                       uint32_t *__restrict__ ghist) {
@@ -754,8 +743,10 @@ __global__ void __launch_bounds__(CNT) k_coarse_exact(const float *__restrict__ 
     const uint32_t packed = (uint32_t)cl_b[i];
     const int64_t b = packed & 0xFFFFu;
     const int32_t clo = (int32_t)((packed >> 16) & 7u), chi = (int32_t)((packed >> 20) & 7u);
+    const GuardCols gc = gsets[gset[b]];
     const float sc = rule_dot(users + u * ld, queries + b * ld, dim);
-    // both buckets lie in [j_{c_hi - 1} + 1, j_{c_lo}] (0 / S at the ends)
+    // both buckets lie in [j_{c_hi - 1} + 1, j_{c_lo}] (0 / S at the ends),
+    // j the query's guard columns
     int32_t r0 = 0, r1 = S;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -887,22 +878,14 @@ hipError_t launch_topk(hipStream_t s, const QueryInfo *qi, int32_t nq, const int
     return hipGetLastError();
 }
 
-hipError_t launch_guard_rows(hipStream_t s, const float *table, int64_t n_user, int32_t S, const GuardCols &gc,
-                             float *guards) {
-    if (n_user <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_guard_rows, dim3((unsigned)((n_user + 255) / 256)), dim3(256), 0, s, table, n_user, S, gc,
-                       guards);
-    return hipGetLastError();
-}
-
 hipError_t launch_coarse_kth(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq, int64_t cu,
-                             int32_t n_guard, uint32_t *part, const QueryInfo *qi, int32_t *cq) {
+                             int32_t n_guard, const int32_t *qmap, uint32_t *part, const QueryInfo *qi, int32_t *cq) {
     if (nq <= 0 || n_user <= 0) return hipSuccess;
     if (ldc % 16 || cu % 1024) return hipErrorInvalidValue;
     const int32_t n_chunk = (int32_t)((n_user + cu - 1) / cu);
     hipLaunchKernelGGL(k_coarse_hist, dim3((unsigned)n_chunk, (unsigned)nq), dim3(CNT), 0, s, cb, ldc, n_user, cu,
-                       n_chunk, part);
-    hipLaunchKernelGGL(k_coarse_kth, dim3((unsigned)nq), dim3(CNT), 0, s, part, n_chunk, n_guard, qi, cq);
+                       n_chunk, qmap, part);
+    hipLaunchKernelGGL(k_coarse_kth, dim3((unsigned)nq), dim3(CNT), 0, s, part, n_chunk, n_guard, qmap, qi, cq);
     return hipGetLastError();
 }
 
@@ -922,15 +905,16 @@ hipError_t launch_coarse_compact(hipStream_t s, const uint8_t *cb, int64_t ldc, 
 
 hipError_t launch_coarse_exact(hipStream_t s, const float *users, const float *queries, int32_t ld, int32_t dim,
                                const float *table, int32_t S, const int32_t *ranks, int64_t n_item, float eps,
-                               const GuardCols &gc, int32_t nq, const int32_t *cl_u, const int32_t *cl_b, int64_t n,
-                               uint32_t *bk_c, float *s_c, uint32_t *ghist, QueryInfo *qi) {
+                               const GuardCols *gsets, const int32_t *gset, int32_t nq, const int32_t *cl_u,
+                               const int32_t *cl_b, int64_t n, uint32_t *bk_c, float *s_c, uint32_t *ghist,
+                               QueryInfo *qi) {
     if (nq <= 0) return hipSuccess;
     const int same = eps == 0.0f ? 1 : 0;
     hipError_t e = hipMemsetAsync(ghist, 0, sizeof(uint32_t) * 2 * (size_t)(S + 1) * nq, s);
     if (e != hipSuccess) return e;
     if (n > 0)
         hipLaunchKernelGGL(k_coarse_exact, dim3((unsigned)((n + CNT - 1) / CNT)), dim3(CNT), 0, s, users, queries, ld,
-                           dim, table, S, eps, same, gc, cl_u, cl_b, n, bk_c, s_c, ghist);
+                           dim, table, S, eps, same, gsets, gset, cl_u, cl_b, n, bk_c, s_c, ghist);
     hipLaunchKernelGGL(k_hist_kth, dim3((unsigned)nq), dim3(NT), 0, s, ghist, S, same, ranks, n_item,
                        (int)RKR_OUTPUT_EXACT, qi);
     return hipGetLastError();

@@ -323,33 +323,54 @@ __global__ void k_recheck(const float *__restrict__ users, const float *__restri
 // Coarse buckets of a query batch.  For every (query b, user u) the screened
 // estimate s^ = <u,q_b> (the same BF16 hi/lo three-product form as above,
 // |s^ - s| <= e = C ||u|| ||q|| + abs_floor) is compared with G <= 7 guard
-// values of u's table row (descending; -inf past G):
+// values of u's table row, T[u][col_0] > T[u][col_1] > ... (col ascending):
 //   c_lo = #{g : guard_g > s^ - e},  c_hi = #{g : guard_g > s^ + e}
-// and cb[b * ldc + u] = c_lo | c_hi << 4.  abs_floor also carries the
+// and cb[row(b) * ldc + u] = c_lo | c_hi << 4.  abs_floor also carries the
 // table's own eps.  Queries are the A operand (staged through LDS, 32 per
 // stage), users the B operand held in VGPRs: lane (lr, lq) of a 16x16 tile
-// owns user lr and queries 4 lq .. 4 lq + 3, so its guards stay in registers.
+// owns user lr and queries 4 lq .. 4 lq + 3, so its guards stay in registers.  qmap (optional) maps the
+// launch's queries to rows of q_hi / q_lo / q_norm and cb.
 struct IpCoarseParams {
     const float *users;           // n_user x ld
     const float *user_norm;       // n_user
-    const float *guards;          // n_user x 8
+    const float *table;           // n_user x S
     int64_t n_user;
-    const uint16_t *q_hi, *q_lo;  // nq x Kp
-    const float *q_norm;          // nq
+    int32_t S;
+    GuardCols gc;
+    const uint16_t *q_hi, *q_lo;  // rows of Kp
+    const float *q_norm;
+    const int32_t *qmap;          // nq entries, or null
     int32_t nq;
     int32_t ld, K, Kp;
     float C, abs_floor;
-    uint8_t *cb;                  // nq rows of ldc bytes
+    uint8_t *cb;                  // rows of ldc bytes
     int64_t ldc;
+    // per-query guard sets (second pass): query qq uses sets[qset[qq]]
+    const GuardCols *sets;
+    const int32_t *qset;
+    int32_t n_sets;
 };
 
-template <int KS>
+constexpr int kMaxGuardSets = 7;
+
+// #{j < 7 : g[j] > x} (independent compares; -inf padded past G).
+__device__ __forceinline__ uint32_t count_above8(const float (&g)[8], float x) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) c += g[j] > x ? 1u : 0u;
+    return c;
+}
+
+template <int KS, bool SETS>
 __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
     constexpr int QST = 32;                                          // queries per LDS stage
     constexpr int ROWQ = (KS * 32 + KPAD_LDS) / 8;
     constexpr int KQ = KS * 4;
     __shared__ uint4 sA[2][QST * ROWQ];
     __shared__ float sQn[QST];
+    __shared__ int32_t sQr[QST];
+    __shared__ int32_t sQs[QST];
+    __shared__ float4 sG[SETS ? kMaxGuardSets * 128 * 2 : 1];      // [set][user][8 guards]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
     const int64_t n_rb = (P.n_user + 127) / 128;
@@ -357,29 +378,38 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
         const int64_t row_base = rb * 128 + wave * 32;
         Frag uh[2][KS], ul[2][KS];
         float gd[2][8], ar[2];
+        bool live[2];
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             const int64_t r = row_base + b * 16 + lr;
-            const bool live = r < P.n_user;
-            const float *urow = live ? P.users + r * P.ld : nullptr;
+            live[b] = r < P.n_user;
+            const float *urow = live[b] ? P.users + r * P.ld : nullptr;
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     const int k = ks * 32 + 8 * lq + i;
-                    const float v = (live && k < P.K) ? urow[k] : 0.0f;
+                    const float v = (live[b] && k < P.K) ? urow[k] : 0.0f;
                     const uint16_t h = bf16_bits(v);
                     uh[b][ks].h[i] = h;
                     ul[b][ks].h[i] = bf16_bits(v - bf16_to_f32(h));
                 }
-            float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f), g1 = g0;
-            if (live) {
-                g0 = reinterpret_cast<const float4 *>(P.guards + r * 8)[0];
-                g1 = reinterpret_cast<const float4 *>(P.guards + r * 8)[1];
+            if (!SETS) {
+                const float *trow = P.table + (live[b] ? r : 0) * (int64_t)P.S;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) gd[b][j] = (live[b] && j < P.gc.n) ? trow[P.gc.col[j]] : -INFINITY;
             }
-            gd[b][0] = g0.x; gd[b][1] = g0.y; gd[b][2] = g0.z; gd[b][3] = g0.w;
-            gd[b][4] = g1.x; gd[b][5] = g1.y; gd[b][6] = g1.z; gd[b][7] = g1.w;
-            ar[b] = live ? P.C * P.user_norm[r] : 0.0f;
+            ar[b] = live[b] ? P.C * P.user_norm[r] : 0.0f;
+        }
+        if (SETS) {
+            __syncthreads();  // the previous row block is done with sG
+            float *g = reinterpret_cast<float *>(sG);
+            for (int t = threadIdx.x; t < P.n_sets * 128 * 8; t += NT) {
+                const int set = t / (128 * 8), ul = (t / 8) % 128, j = t % 8;
+                const int64_t r = rb * 128 + ul;
+                const GuardCols &gc = P.sets[set];
+                g[t] = (r < P.n_user && j < gc.n) ? P.table[r * P.S + gc.col[j]] : -INFINITY;
+            }
         }
         for (int32_t q0 = 0; q0 < P.nq; q0 += QST) {
             __syncthreads();  // every wave is done with the previous stage
@@ -388,11 +418,19 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
                 const int rem = c % (QST * KQ);
                 const int qq = q0 + rem / KQ;
                 uint4 v = make_uint4(0, 0, 0, 0);
-                if (qq < P.nq)
-                    v = reinterpret_cast<const uint4 *>(half ? P.q_lo : P.q_hi)[(int64_t)qq * KQ + rem % KQ];
+                if (qq < P.nq) {
+                    const int64_t qr = P.qmap ? P.qmap[qq] : qq;
+                    v = reinterpret_cast<const uint4 *>(half ? P.q_lo : P.q_hi)[qr * KQ + rem % KQ];
+                }
                 sA[half][(rem / KQ) * ROWQ + rem % KQ] = v;
             }
-            if (threadIdx.x < QST) sQn[threadIdx.x] = q0 + (int)threadIdx.x < P.nq ? P.q_norm[q0 + threadIdx.x] : 0.f;
+            if (threadIdx.x < QST) {
+                const int qq = q0 + (int)threadIdx.x;
+                const int32_t qr = qq < P.nq ? (P.qmap ? P.qmap[qq] : qq) : 0;
+                sQr[threadIdx.x] = qr;
+                sQn[threadIdx.x] = qq < P.nq ? P.q_norm[qr] : 0.f;
+                if (SETS) sQs[threadIdx.x] = qq < P.nq ? P.qset[qq] : 0;
+            }
             __syncthreads();
 #pragma unroll
             for (int g = 0; g < QST / 16; ++g) {
@@ -413,22 +451,22 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
                 // D[4 lq + i][lr]: query q0 + g*16 + 4 lq + i, user row_base + b*16 + lr
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int32_t qq = q0 + g * 16 + 4 * lq + i;
-                    if (qq >= P.nq) continue;
-                    const float qn = sQn[g * 16 + 4 * lq + i];
-                    uint8_t *out = P.cb + (int64_t)qq * P.ldc;
+                    const int ql_ = g * 16 + 4 * lq + i;
+                    const bool qok = q0 + ql_ < P.nq;
+                    const float qn = sQn[ql_];
+                    uint8_t *out = P.cb + (int64_t)sQr[ql_] * P.ldc + row_base + lr;
 #pragma unroll
                     for (int b = 0; b < 2; ++b) {
-                        const int64_t r = row_base + b * 16 + lr;
-                        const float e = ar[b] * qn + P.abs_floor;
-                        const float xl = acc[b][i] - e, xh = acc[b][i] + e;
-                        uint32_t clo = 0, chi = 0;
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            clo += gd[b][j] > xl ? 1u : 0u;
-                            chi += gd[b][j] > xh ? 1u : 0u;
+                        if (SETS) {
+                            const float4 *gp = sG + (sQs[ql_] * 128 + wave * 32 + b * 16 + lr) * 2;
+                            const float4 x0 = gp[0], x1 = gp[1];
+                            gd[b][0] = x0.x; gd[b][1] = x0.y; gd[b][2] = x0.z; gd[b][3] = x0.w;
+                            gd[b][4] = x1.x; gd[b][5] = x1.y; gd[b][6] = x1.z; gd[b][7] = x1.w;
                         }
-                        if (r < P.n_user) out[r] = (uint8_t)(clo | (chi << 4));
+                        const float e = __fmaf_rn(ar[b], qn, P.abs_floor);
+                        const uint32_t clo = count_above8(gd[b], acc[b][i] - e);
+                        const uint32_t chi = count_above8(gd[b], acc[b][i] + e);
+                        if (qok && live[b]) out[b * 16] = (uint8_t)(clo | (chi << 4));
                     }
                 }
             }
@@ -520,21 +558,30 @@ hipError_t launch_recheck(hipStream_t s, const float *users, const float *items,
 
 hipError_t launch_ip_coarse(hipStream_t s, const CoarseLaunch &L) {
     if (L.n_user <= 0 || L.nq <= 0) return hipSuccess;
-    IpCoarseParams P{L.users, L.user_norm, L.guards, L.n_user, L.q_hi, L.q_lo, L.q_norm, L.nq,
-                     L.ld, L.K, L.Kp, L.C, L.abs_floor, L.cb, L.ldc};
+    const bool sets = L.sets != nullptr;
+    if (sets ? (L.n_sets < 1 || L.n_sets > kMaxGuardSets || !L.qset) : (L.gc.n < 1 || L.gc.n > 7))
+        return hipErrorInvalidValue;
+    IpCoarseParams P{L.users, L.user_norm, L.table, L.n_user, L.S, L.gc, L.q_hi, L.q_lo, L.q_norm, L.qmap, L.nq,
+                     L.ld, L.K, L.Kp, L.C, L.abs_floor, L.cb, L.ldc, L.sets, L.qset, L.n_sets};
     const int64_t n_rb = (L.n_user + 127) / 128;
     const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(n_rb, L.grid_cap)));
+#define RKR_COARSE_CASE(ks)                                                                                   \
+    case ks:                                                                                                  \
+        if (sets) hipLaunchKernelGGL((k_ip_coarse<ks, true>), grid, dim3(NT), 0, s, P);                      \
+        else hipLaunchKernelGGL((k_ip_coarse<ks, false>), grid, dim3(NT), 0, s, P);                          \
+        break;
     switch (L.Kp / 32) {
-        case 1: hipLaunchKernelGGL(k_ip_coarse<1>, grid, dim3(NT), 0, s, P); break;
-        case 2: hipLaunchKernelGGL(k_ip_coarse<2>, grid, dim3(NT), 0, s, P); break;
-        case 3: hipLaunchKernelGGL(k_ip_coarse<3>, grid, dim3(NT), 0, s, P); break;
-        case 4: hipLaunchKernelGGL(k_ip_coarse<4>, grid, dim3(NT), 0, s, P); break;
-        case 5: hipLaunchKernelGGL(k_ip_coarse<5>, grid, dim3(NT), 0, s, P); break;
-        case 6: hipLaunchKernelGGL(k_ip_coarse<6>, grid, dim3(NT), 0, s, P); break;
-        case 7: hipLaunchKernelGGL(k_ip_coarse<7>, grid, dim3(NT), 0, s, P); break;
-        case 8: hipLaunchKernelGGL(k_ip_coarse<8>, grid, dim3(NT), 0, s, P); break;
+        RKR_COARSE_CASE(1)
+        RKR_COARSE_CASE(2)
+        RKR_COARSE_CASE(3)
+        RKR_COARSE_CASE(4)
+        RKR_COARSE_CASE(5)
+        RKR_COARSE_CASE(6)
+        RKR_COARSE_CASE(7)
+        RKR_COARSE_CASE(8)
         default: return hipErrorInvalidValue;
     }
+#undef RKR_COARSE_CASE
     return hipGetLastError();
 }
 

@@ -236,7 +236,7 @@ int slot_init(Ctx *c, Slot &sl, bool own_stream, const std::vector<uint32_t> *cu
 void slot_free(Ctx *c, Slot &sl) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     free_bufs(c, {&sl.q_in, &sl.qip, &sl.bk, &sl.qinfo, &sl.cand, &sl.refine_idx, &sl.counts, &sl.out, &sl.recheck,
-                  &sl.small, &sl.hist, &sl.q_split, &sl.guards, &sl.coarse_info, &sl.coarse_list});
+                  &sl.small, &sl.hist, &sl.q_split, &sl.coarse_info, &sl.coarse_list});
     for (auto &e : sl.ev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (sl.own_stream && sl.stream) (void)hipStreamDestroy(sl.stream);
@@ -746,6 +746,10 @@ int validate_queries(Ctx *c, const DataSnap &d, const float *queries, int32_t n_
 
 namespace {
 
+constexpr int32_t kCoarseRefine = 4096;
+constexpr int64_t kCoarseChunk = 16384;
+constexpr int32_t kCoarseMinBatch = 8;
+
 // Device batches of query_on_slot.  allow_coarse: screened-form exact batches
 // take the coarse bounds; queries whose coarse list would be empty of a k-th
 // bound or too long are appended to *fallback (indices into queries) with
@@ -792,12 +796,13 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
     if (coarse) {
         if ((rc = ensure(c, sl.q_split, (sizeof(uint16_t) * 2 * c->kp + sizeof(float)) * (size_t)nqb, "query BF16 rows")))
             return rc;
-        if ((rc = ensure(c, sl.guards, sizeof(float) * 8 * (size_t)U, "guard values"))) return rc;
         for (int j = 0; j < dv.n_guard; ++j) gc.col[j] = dv.gcol[j];
         gc.n = dv.n_guard;
-        RKR_HIP(c, launch_guard_rows(sl.stream, T, U, S, gc, reinterpret_cast<float *>(sl.guards.p)));
     }
-    // a coarse list longer than this takes the full path instead
+    // ranks of the searched table's columns (guard placement)
+    const std::vector<int32_t> &hrk = use_q ? c->h_qranks : c->h_ranks;
+    // coarse lists longer than kCoarseRefine get a second, finer pass; a
+    // list longer than coarse_cap takes the full path instead
     const int64_t coarse_cap = std::max<int64_t>(4096, U / 4);
 
     std::vector<float> hq((size_t)nqb * ld, 0.0f);
@@ -830,11 +835,14 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
         int32_t *ubv = lbv + (size_t)nq * U;
         const int64_t cu = chunk_users(U, nq, c->n_cu);
         uint32_t *ghist = reinterpret_cast<uint32_t *>(sl.hist.p);
+        // a few queries are answered faster by the full path: the coarse
+        // kernel converts every user row once per launch
+        const bool coarse_b = coarse && nq >= kCoarseMinBatch;
         int32_t *cl_u = nullptr, *cl_b = nullptr;
         uint32_t *bk_c = nullptr;
         float *s_c = nullptr;
         int64_t n_listed = 0;
-        if (coarse) {
+        if (coarse_b) {
             // coarse buckets on the matrix cores, the coarse k-th bound and
             // list sizes; exact scores and bucket pairs of the listed users
             uint16_t *q_hi = reinterpret_cast<uint16_t *>(sl.q_split.p);
@@ -846,25 +854,93 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                                 (1.0 + std::ldexp(1.0, -18));
             uint8_t *cb = reinterpret_cast<uint8_t *>(bk);
             CoarseLaunch L{};
-            L.users = dv.users; L.user_norm = dv.user_norm; L.guards = reinterpret_cast<const float *>(sl.guards.p);
-            L.n_user = U; L.q_hi = q_hi; L.q_lo = q_lo; L.q_norm = qn; L.nq = nq; L.ld = ld; L.K = d; L.Kp = c->kp;
-            L.C = c->screen_C; L.abs_floor = std::nextafter((float)absf, INFINITY); L.cb = cb; L.ldc = ldc;
-            L.grid_cap = (int64_t)c->n_cu * 2;
+            L.users = dv.users; L.user_norm = dv.user_norm; L.table = T; L.n_user = U; L.S = S; L.gc = gc;
+            L.q_hi = q_hi; L.q_lo = q_lo; L.q_norm = qn; L.qmap = nullptr; L.nq = nq; L.ld = ld; L.K = d;
+            L.Kp = c->kp; L.C = c->screen_C; L.abs_floor = std::nextafter((float)absf, INFINITY); L.cb = cb;
+            L.ldc = ldc; L.grid_cap = (int64_t)c->n_cu * 3;
             RKR_HIP(c, launch_ip_coarse(sl.stream, L));
             RKR_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
-            const int64_t n_chunk = (U + cu - 1) / cu;
-            if ((rc = ensure(c, sl.coarse_info, sizeof(uint32_t) * (17 * (size_t)nq * n_chunk + 4 * (size_t)nqb +
-                                                                   2 * (size_t)n_chunk + 64), "coarse counts")))
+            // coarse chunks: a fixed user count, so launches over few queries
+            // still fill the device and the list stays chunk-major
+            const int64_t ccu = kCoarseChunk;
+            const int64_t n_chunk = (U + ccu - 1) / ccu;
+            // part | cq | loc | ctot | cbase | guard sets | gset | qmap | qset
+            const size_t w_sets = sizeof(GuardCols) * 8 / 4;
+            if ((rc = ensure(c, sl.coarse_info, sizeof(uint32_t) * (17 * (size_t)nq * n_chunk + 7 * (size_t)nqb +
+                                                                   2 * (size_t)n_chunk + w_sets + 64),
+                             "coarse counts")))
                 return rc;
             uint32_t *part = reinterpret_cast<uint32_t *>(sl.coarse_info.p);
             int32_t *cq = reinterpret_cast<int32_t *>(part + 16 * (size_t)nq * n_chunk);
             int32_t *loc = cq + 4 * (size_t)nqb;
             int32_t *ctot = loc + (size_t)nq * n_chunk;
             int32_t *cbase = ctot + n_chunk;
-            RKR_HIP(c, launch_coarse_kth(sl.stream, cb, ldc, U, nq, cu, dv.n_guard, part, qi, cq));
+            GuardCols *gsets = reinterpret_cast<GuardCols *>(cbase + n_chunk);
+            int32_t *gset = reinterpret_cast<int32_t *>(gsets + 8);
+            int32_t *qmap = gset + nqb;
+            RKR_HIP(c, launch_coarse_kth(sl.stream, cb, ldc, U, nq, ccu, dv.n_guard, nullptr, part, qi, cq));
             hcq.assign((size_t)nq * 4, 0);
             RKR_HIP(c, hipMemcpyAsync(hcq.data(), cq, sizeof(int32_t) * 4 * nq, hipMemcpyDeviceToHost, sl.stream));
             RKR_HIP(c, hipStreamSynchronize(sl.stream));
+            // second pass for long lists: the queries whose k-th bound fell in
+            // guard bin beta are bucketed again against up to 7 columns inside
+            // that bin (the bin's last column included), spaced geometrically
+            // in rank
+            std::vector<GuardCols> sets{gc};
+            std::vector<int32_t> hgset((size_t)nq, 0), hqmap;
+            std::vector<std::pair<int32_t, int32_t>> groups;  // (set, first index in hqmap)
+            for (int32_t beta = 0; beta < gc.n; ++beta) {
+                const int32_t lo_col = beta == 0 ? 0 : gc.col[beta - 1] + 1, hi_col = gc.col[beta];
+                if (hi_col <= lo_col) continue;
+                const size_t first = hqmap.size();
+                for (int32_t i = 0; i < nq; ++i)
+                    if (hcq[(size_t)i * 4] == beta && hcq[(size_t)i * 4 + 1] > kCoarseRefine) hqmap.push_back(i);
+                if (hqmap.size() == first) continue;
+                GuardCols g2{};
+                const double ra = (double)hrk[lo_col] + 1.0, rb = (double)hrk[hi_col] + 1.0;
+                int32_t last = lo_col - 1;
+                for (int m = 1; m <= 6; ++m) {
+                    const double target = ra * std::pow(rb / ra, m / 7.0) - 1.0;
+                    int32_t j = last + 1;
+                    while (j + 1 < hi_col && (double)hrk[j + 1] <= target) ++j;
+                    if (j < hi_col && j > last && (double)hrk[j] <= target) {
+                        g2.col[g2.n++] = j;
+                        last = j;
+                    }
+                }
+                g2.col[g2.n++] = hi_col;
+                for (size_t t = first; t < hqmap.size(); ++t) hgset[hqmap[t]] = (int32_t)sets.size();
+                groups.push_back({(int32_t)sets.size(), (int32_t)first});
+                sets.push_back(g2);
+            }
+            if (!groups.empty()) {
+                // one launch for every refined query: set of each (local)
+                // query in qset, the sets themselves after set 0
+                const int32_t nr = (int32_t)hqmap.size();
+                std::vector<int32_t> hqset((size_t)nr);
+                for (int32_t t = 0; t < nr; ++t) hqset[t] = hgset[hqmap[t]] - 1;
+                int32_t *qset = qmap + nr;
+                RKR_HIP(c, hipMemcpyAsync(qmap, hqmap.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice, sl.stream));
+                RKR_HIP(c, hipMemcpyAsync(qset, hqset.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice, sl.stream));
+                RKR_HIP(c, hipMemcpyAsync(gsets, sets.data() + 1, sizeof(GuardCols) * (sets.size() - 1),
+                                          hipMemcpyHostToDevice, sl.stream));
+                CoarseLaunch L2 = L;
+                L2.qmap = qmap;
+                L2.nq = nr;
+                L2.sets = gsets;
+                L2.qset = qset;
+                L2.n_sets = (int32_t)sets.size() - 1;
+                RKR_HIP(c, launch_ip_coarse(sl.stream, L2));
+                for (size_t gi = 0; gi < groups.size(); ++gi) {
+                    const int32_t first = groups[gi].second;
+                    const int32_t end = gi + 1 < groups.size() ? groups[gi + 1].second : nr;
+                    RKR_HIP(c, launch_coarse_kth(sl.stream, cb, ldc, U, end - first, ccu, sets[groups[gi].first].n,
+                                                 qmap + first, part, qi, cq));
+                }
+                RKR_HIP(c, hipMemcpyAsync(hcq.data(), cq, sizeof(int32_t) * 4 * nq, hipMemcpyDeviceToHost, sl.stream));
+                RKR_HIP(c, hipStreamSynchronize(sl.stream));
+                tm.n_coarse_refined += (int64_t)hqmap.size();
+            }
             for (int32_t i = 0; i < nq; ++i) {
                 int32_t *e = &hcq[(size_t)i * 4];
                 if (e[0] < 0 || e[1] > coarse_cap) {
@@ -885,9 +961,12 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             bk_c = reinterpret_cast<uint32_t *>(cl_b + std::max<int64_t>(n_listed, 1));
             s_c = reinterpret_cast<float *>(bk_c + std::max<int64_t>(n_listed, 1));
             RKR_HIP(c, hipMemcpyAsync(cq, hcq.data(), sizeof(int32_t) * 4 * nq, hipMemcpyHostToDevice, sl.stream));
-            RKR_HIP(c, launch_coarse_compact(sl.stream, cb, ldc, U, nq, cu, part, cq, loc, ctot, cbase, cl_u, cl_b));
-            RKR_HIP(c, launch_coarse_exact(sl.stream, dv.users, q_in, ld, d, T, S, R, I, eps, gc, nq, cl_u, cl_b,
-                                           n_listed, bk_c, s_c, ghist, qi));
+            RKR_HIP(c, hipMemcpyAsync(gsets, sets.data(), sizeof(GuardCols) * sets.size(), hipMemcpyHostToDevice,
+                                      sl.stream));
+            RKR_HIP(c, hipMemcpyAsync(gset, hgset.data(), sizeof(int32_t) * nq, hipMemcpyHostToDevice, sl.stream));
+            RKR_HIP(c, launch_coarse_compact(sl.stream, cb, ldc, U, nq, ccu, part, cq, loc, ctot, cbase, cl_u, cl_b));
+            RKR_HIP(c, launch_coarse_exact(sl.stream, dv.users, q_in, ld, d, T, S, R, I, eps, gsets, gset, nq, cl_u,
+                                           cl_b, n_listed, bk_c, s_c, ghist, qi));
             RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
         } else {
             // <u, q> for every (query, user): qip[b * U + u]
@@ -931,7 +1010,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
         uint32_t *max_err_bits = reinterpret_cast<uint32_t *>(recheck_n + 1);
         const bool screened = c->form == RKR_FORM_SCREENED;
 
-        if (coarse)
+        if (coarse_b)
             RKR_HIP(c, launch_coarse_final(sl.stream, cl_u, cl_b, bk_c, s_c, n_listed, S, R, I, U, qi, cand_user,
                                            cand_rank, refine_total, refine_cand, refine_user, refine_query, refine_thr,
                                            qip));

@@ -38,7 +38,7 @@ struct Slot {
     bool own_stream = false;
     hipEvent_t ev[16] = {};
     DeviceBuf q_in, qip, bk, qinfo, cand, refine_idx, counts, out, recheck, small, hist;
-    DeviceBuf q_split, guards, coarse_info, coarse_list;  // coarse bounds
+    DeviceBuf q_split, coarse_info, coarse_list;  // coarse bounds
 };
 
 struct Ctx {
@@ -256,18 +256,31 @@ hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, 
                              uint16_t *hi, uint16_t *lo, float *norm);
 hipError_t launch_user_norms(hipStream_t s, const float *users, int64_t n_user, int32_t ld, int32_t K, float *norm);
 hipError_t launch_refine_screen(hipStream_t s, const ScreenLaunch &L);
-// Coarse buckets of a batch (screened <u,q> against <= 7 guard columns of
-// every user's table row): cb[b * n_user + u] = c_lo | c_hi << 4.
+// Guard columns of the coarse bounds: n (1..7) ascending table columns.
+struct GuardCols {
+    int32_t col[8];
+    int32_t n;
+};
+// Coarse buckets of a batch (screened <u,q> against the guard columns of
+// every user's table row): cb[row(b) * ldc + u] = c_lo | c_hi << 4, with
+// row(b) = qmap[b] (or b when qmap is null), which also picks the query rows.
 struct CoarseLaunch {
-    const float *users, *user_norm, *guards;  // guards: n_user x 8, descending, -inf padded
+    const float *users, *user_norm, *table;
     int64_t n_user;
+    int32_t S;
+    GuardCols gc;
     const uint16_t *q_hi, *q_lo;
     const float *q_norm;
+    const int32_t *qmap;
     int32_t nq, ld, K, Kp;
     float C, abs_floor;
     uint8_t *cb;            // nq rows of ldc bytes
     int64_t ldc;
     int64_t grid_cap;
+    // second pass: query b uses sets[qset[b]] (<= 7 sets) instead of gc
+    const GuardCols *sets = nullptr;
+    const int32_t *qset = nullptr;
+    int32_t n_sets = 0;
 };
 hipError_t launch_ip_coarse(hipStream_t s, const CoarseLaunch &L);
 hipError_t launch_recheck(hipStream_t s, const float *users, const float *items, int32_t ld, int32_t K,
@@ -351,23 +364,18 @@ hipError_t launch_hist_kth(hipStream_t s, const uint32_t *bk, int64_t n_user, in
 // user, the coarse k-th bound and list size per query (cq[b] = {beta_c,
 // count, offset, fill}), the listed users, their exact scores and bucket
 // pairs with the exact k-th bound (into qi), and the final candidates.
-struct GuardCols {
-    int32_t col[8];
-    int32_t n;
-};
-hipError_t launch_guard_rows(hipStream_t s, const float *table, int64_t n_user, int32_t S, const GuardCols &gc,
-                             float *guards);
 // cb rows are ldc bytes (ldc % 16 == 0); cu % 1024 == 0
 hipError_t launch_coarse_kth(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq, int64_t cu,
-                             int32_t n_guard, uint32_t *part, const QueryInfo *qi, int32_t *cq);
+                             int32_t n_guard, const int32_t *qmap, uint32_t *part, const QueryInfo *qi, int32_t *cq);
 // chunk-major list: loc (n_chunk x nq), ctot and cbase (n_chunk) are scratch
 hipError_t launch_coarse_compact(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq,
                                  int64_t cu, const uint32_t *part, const int32_t *cq, int32_t *loc, int32_t *ctot,
                                  int32_t *cbase, int32_t *cl_u, int32_t *cl_b);
 hipError_t launch_coarse_exact(hipStream_t s, const float *users, const float *queries, int32_t ld, int32_t dim,
                                const float *table, int32_t S, const int32_t *ranks, int64_t n_item, float eps,
-                               const GuardCols &gc, int32_t nq, const int32_t *cl_u, const int32_t *cl_b, int64_t n,
-                               uint32_t *bk_c, float *s_c, uint32_t *ghist, QueryInfo *qi);
+                               const GuardCols *gsets, const int32_t *gset, int32_t nq, const int32_t *cl_u,
+                               const int32_t *cl_b, int64_t n, uint32_t *bk_c, float *s_c, uint32_t *ghist,
+                               QueryInfo *qi);
 hipError_t launch_coarse_final(hipStream_t s, const int32_t *cl_u, const int32_t *cl_b, const uint32_t *bk_c,
                                const float *s_c, int64_t n, int32_t S, const int32_t *ranks, int64_t n_item,
                                int64_t n_user, QueryInfo *qi, int32_t *cand_user, int32_t *cand_rank,

@@ -69,7 +69,8 @@ class BatchTiming(C.Structure):
                 ("topk_ms", C.c_double),
                 ("h2d_ms", C.c_double), ("d2h_ms", C.c_double), ("n_batches", C.c_int64),
                 ("n_refined", C.c_int64), ("n_rechecked", C.c_int64), ("n_screened", C.c_int64),
-                ("max_screen_err", C.c_double), ("n_listed", C.c_int64), ("n_fallback", C.c_int64)]
+                ("max_screen_err", C.c_double), ("n_listed", C.c_int64), ("n_fallback", C.c_int64),
+                ("n_coarse_refined", C.c_int64)]
 
 
 class ServerConfig(C.Structure):

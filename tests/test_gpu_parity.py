@@ -280,7 +280,7 @@ def _same_answers(a, b, ks, stats=True):
 
 
 @pytest.mark.parametrize("d,dense,nq", [(150, 1, 70), (150, 0, 70), (37, 1, 70), (20, 0, 33), (150, 1, 1),
-                                        (64, 1, 5)])
+                                        (64, 1, 8), (64, 0, 9)])
 def test_coarse_bounds(orc, gpu_index_factory, d, dense, nq):
     """Coarse bounds (screened <u,q> on BF16 matrix cores against guard
     columns, exact scores and searches for the listed users only) keep the
@@ -304,6 +304,31 @@ def test_coarse_bounds(orc, gpu_index_factory, d, dense, nq):
     coarse = idx.query(queries, ks)
     _same_answers(full, coarse, ks)
     _same_answers(plain, coarse, ks)
+    _check_exact(orc, _normalized(orc, users), items, queries, coarse, ks)
+
+
+@pytest.mark.parametrize("dense", [1, 0])
+def test_coarse_bounds_second_pass(orc, gpu_index_factory, dense):
+    """Long coarse lists get a second pass against guard columns inside the
+    bin of their k-th bound (one launch, a guard set per query); answers and
+    prune counts stay those of the full path."""
+    _case(f"coarse bounds second pass dense={dense}")
+    rng = np.random.default_rng(31 + dense)
+    U, I, d, nq = 40000, 3000, 32, 40
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = (rng.standard_normal((I, d), dtype=np.float32) / np.float32(np.sqrt(d))
+             * rng.lognormal(0, 0.4, (I, 1)).astype(np.float32))
+    queries = np.concatenate([items[rng.choice(I, nq // 2, replace=False)],
+                              rng.standard_normal((nq // 2, d), dtype=np.float32) / np.float32(np.sqrt(d))])
+    ks = [int(k) for k in rng.integers(50, 400, nq)]
+    idx, _ = gpu_index_factory(users, items, 128, dense_top=dense)
+    idx.set_form("screened")
+    idx.set_option(9, 0)
+    full = idx.query(queries, ks)
+    idx.set_option(9, 1)
+    coarse = idx.query(queries, ks)
+    assert coarse.timing["n_coarse_refined"] > 0
+    _same_answers(full, coarse, ks)
     _check_exact(orc, _normalized(orc, users), items, queries, coarse, ks)
 
 
