@@ -343,8 +343,9 @@ struct IpCoarseParams {
     int32_t nq;
     int32_t ld, K, Kp;
     float C, abs_floor;
-    uint8_t *cb;                  // rows of ldc bytes
+    uint8_t *cb;                  // rows of ldc bytes (ldc % 128 == 0)
     int64_t ldc;
+    int64_t spare_row;            // row written for the stage's padding queries
     // per-query guard sets (second pass): query qq uses sets[qset[qq]]
     const GuardCols *sets;
     const int32_t *qset;
@@ -353,11 +354,13 @@ struct IpCoarseParams {
 
 constexpr int kMaxGuardSets = 7;
 
-// #{j < 7 : g[j] > x} (independent compares; -inf padded past G).
+// #{j < 7 : g[j] > x} (-inf padded past G).  g > x exactly when x - g is
+// negative or -0 (x - g is +0 for x == g and +inf for g = -inf), so the
+// count sums sign bits: no compare masks, no mask-read hazards.
 __device__ __forceinline__ uint32_t count_above8(const float (&g)[8], float x) {
     uint32_t c = 0;
 #pragma unroll
-    for (int j = 0; j < 7; ++j) c += g[j] > x ? 1u : 0u;
+    for (int j = 0; j < 7; ++j) c += __float_as_uint(x - g[j]) >> 31;
     return c;
 }
 
@@ -368,7 +371,7 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
     constexpr int KQ = KS * 4;
     __shared__ uint4 sA[2][QST * ROWQ];
     __shared__ float sQn[QST];
-    __shared__ int32_t sQr[QST];
+    __shared__ int64_t sQo[QST];  // byte offset of the query's cb row
     __shared__ int32_t sQs[QST];
     __shared__ float4 sG[SETS ? kMaxGuardSets * 128 * 2 : 1];      // [set][user][8 guards]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -427,7 +430,7 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
             if (threadIdx.x < QST) {
                 const int qq = q0 + (int)threadIdx.x;
                 const int32_t qr = qq < P.nq ? (P.qmap ? P.qmap[qq] : qq) : 0;
-                sQr[threadIdx.x] = qr;
+                sQo[threadIdx.x] = (qq < P.nq ? (int64_t)qr : P.spare_row) * P.ldc;
                 sQn[threadIdx.x] = qq < P.nq ? P.q_norm[qr] : 0.f;
                 if (SETS) sQs[threadIdx.x] = qq < P.nq ? P.qset[qq] : 0;
             }
@@ -451,10 +454,11 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
                 // D[4 lq + i][lr]: query q0 + g*16 + 4 lq + i, user row_base + b*16 + lr
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
+                    // rows are padded to whole row blocks and padding queries
+                    // write the spare row, so every lane stores
                     const int ql_ = g * 16 + 4 * lq + i;
-                    const bool qok = q0 + ql_ < P.nq;
                     const float qn = sQn[ql_];
-                    uint8_t *out = P.cb + (int64_t)sQr[ql_] * P.ldc + row_base + lr;
+                    uint8_t *out = P.cb + sQo[ql_] + row_base + lr;
 #pragma unroll
                     for (int b = 0; b < 2; ++b) {
                         if (SETS) {
@@ -466,7 +470,7 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
                         const float e = __fmaf_rn(ar[b], qn, P.abs_floor);
                         const uint32_t clo = count_above8(gd[b], acc[b][i] - e);
                         const uint32_t chi = count_above8(gd[b], acc[b][i] + e);
-                        if (qok && live[b]) out[b * 16] = (uint8_t)(clo | (chi << 4));
+                        out[b * 16] = (uint8_t)(clo | (chi << 4));
                     }
                 }
             }
@@ -561,8 +565,9 @@ hipError_t launch_ip_coarse(hipStream_t s, const CoarseLaunch &L) {
     const bool sets = L.sets != nullptr;
     if (sets ? (L.n_sets < 1 || L.n_sets > kMaxGuardSets || !L.qset) : (L.gc.n < 1 || L.gc.n > 7))
         return hipErrorInvalidValue;
+    if (L.ldc % 128 || L.ldc < L.n_user) return hipErrorInvalidValue;
     IpCoarseParams P{L.users, L.user_norm, L.table, L.n_user, L.S, L.gc, L.q_hi, L.q_lo, L.q_norm, L.qmap, L.nq,
-                     L.ld, L.K, L.Kp, L.C, L.abs_floor, L.cb, L.ldc, L.sets, L.qset, L.n_sets};
+                     L.ld, L.K, L.Kp, L.C, L.abs_floor, L.cb, L.ldc, L.spare_row, L.sets, L.qset, L.n_sets};
     const int64_t n_rb = (L.n_user + 127) / 128;
     const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(n_rb, L.grid_cap)));
 #define RKR_COARSE_CASE(ks)                                                                                   \

@@ -785,8 +785,9 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
     int rc;
     if ((rc = ensure(c, sl.q_in, sizeof(float) * (size_t)nqb * ld, "query vectors"))) return rc;
     if ((rc = ensure(c, sl.qip, sizeof(float) * (size_t)nqb * U, "query scores"))) return rc;
-    const int64_t ldc = (U + 15) / 16 * 16;  // coarse bucket rows
-    if ((rc = ensure(c, sl.bk, std::max<size_t>((pair_bytes - 4) * (size_t)U, (size_t)ldc) * (size_t)nqb,
+    // coarse bucket rows: whole row blocks of 128 users, plus a spare row
+    const int64_t ldc = (U + 127) / 128 * 128;
+    if ((rc = ensure(c, sl.bk, std::max((pair_bytes - 4) * (size_t)U * nqb, (size_t)ldc * (nqb + 1)),
                      "bucket pairs")))
         return rc;
     if ((rc = ensure(c, sl.qinfo, sizeof(QueryInfo) * (size_t)nqb + 64, "query info"))) return rc;
@@ -857,7 +858,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             L.users = dv.users; L.user_norm = dv.user_norm; L.table = T; L.n_user = U; L.S = S; L.gc = gc;
             L.q_hi = q_hi; L.q_lo = q_lo; L.q_norm = qn; L.qmap = nullptr; L.nq = nq; L.ld = ld; L.K = d;
             L.Kp = c->kp; L.C = c->screen_C; L.abs_floor = std::nextafter((float)absf, INFINITY); L.cb = cb;
-            L.ldc = ldc; L.grid_cap = (int64_t)c->n_cu * 3;
+            L.ldc = ldc; L.spare_row = nqb; L.grid_cap = (int64_t)c->n_cu * 3;
             RKR_HIP(c, launch_ip_coarse(sl.stream, L));
             RKR_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
             // coarse chunks: a fixed user count, so launches over few queries

@@ -274,8 +274,9 @@ struct CoarseLaunch {
     const int32_t *qmap;
     int32_t nq, ld, K, Kp;
     float C, abs_floor;
-    uint8_t *cb;            // nq rows of ldc bytes
+    uint8_t *cb;            // rows of ldc bytes, ldc % 128 == 0
     int64_t ldc;
+    int64_t spare_row;      // a row no query owns (written by padding lanes)
     int64_t grid_cap;
     // second pass: query b uses sets[qset[b]] (<= 7 sets) instead of gc
     const GuardCols *sets = nullptr;
@@ -364,7 +365,7 @@ hipError_t launch_hist_kth(hipStream_t s, const uint32_t *bk, int64_t n_user, in
 // user, the coarse k-th bound and list size per query (cq[b] = {beta_c,
 // count, offset, fill}), the listed users, their exact scores and bucket
 // pairs with the exact k-th bound (into qi), and the final candidates.
-// cb rows are ldc bytes (ldc % 16 == 0); cu % 1024 == 0
+// cb rows are ldc bytes (ldc % 16 == 0, bytes past n_user unread); cu % 1024 == 0
 hipError_t launch_coarse_kth(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq, int64_t cu,
                              int32_t n_guard, const int32_t *qmap, uint32_t *part, const QueryInfo *qi, int32_t *cq);
 // chunk-major list: loc (n_chunk x nq), ctot and cbase (n_chunk) are scratch
