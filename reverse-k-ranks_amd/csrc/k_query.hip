@@ -541,31 +541,32 @@ __device__ __forceinline__ float rule_dot(const float *__restrict__ a, const flo
     return s;
 }
 
-__device__ __forceinline__ void coarse_flush(uint64_t alo, uint64_t ahi, uint32_t *h) {
-#pragma unroll
-    for (int j = 0; j < 7; ++j) {
-        const uint32_t fl = (uint32_t)(alo >> (8 * j)) & 255u, fh = (uint32_t)(ahi >> (8 * j)) & 255u;
-        if (fl) atomicAdd(&h[j], fl);
-        if (fh) atomicAdd(&h[8 + j], fh);
-    }
+// Adds the byte counters of a 64-bit word (bins 0..7, 8 bits each) into
+// four words of 16-bit counters (bins 2w, 2w + 1 in word w).
+__device__ __forceinline__ void widen_add(uint64_t x, uint32_t (&w)[4]) {
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    w[0] += (lo & 0xFFu) | ((lo & 0xFF00u) << 8);
+    w[1] += ((lo >> 16) & 0xFFu) | ((lo >> 8) & 0xFF0000u);
+    w[2] += (hi & 0xFFu) | ((hi & 0xFF00u) << 8);
+    w[3] += ((hi >> 16) & 0xFFu) | ((hi >> 8) & 0xFF0000u);
 }
 
 // grid (chunks, queries): counts of c_lo and c_hi values 0..6 of the chunk's
 // users, part[(b * n_chunk + chunk) * 16 + {0..6 | 8..14}] (b = qmap[y] when
-// given).  16 users per
-// thread and load (rows of cb are ldc bytes, ldc % 16 == 0, chunk starts at
-// multiples of 1024); per-thread counters are bytes of a 64-bit word,
-// flushed before they can carry.
+// given).  16 users per thread and load (rows of cb are ldc bytes,
+// ldc % 16 == 0, chunks start at multiples of 1024).  Per-thread counters
+// are bytes of a 64-bit word, widened to 16 bits before they can carry; the
+// workgroup sums them by wave reductions (cu < 2^18 keeps the sums in 16
+// bits).
 __global__ void __launch_bounds__(CNT) k_coarse_hist(const uint8_t *__restrict__ cb, int64_t ldc, int64_t n_user,
                                                      int64_t cu, int32_t n_chunk, const int32_t *__restrict__ qmap,
                                                      uint32_t *__restrict__ part) {
-    __shared__ uint32_t h[16];
+    __shared__ uint32_t red[CNT / 64][8];
     const int64_t b = qmap ? qmap[blockIdx.y] : blockIdx.y;
-    if (threadIdx.x < 16) h[threadIdx.x] = 0;
-    __syncthreads();
     const uint8_t *v = cb + b * ldc;
     const int64_t u0 = (int64_t)blockIdx.x * cu, u1 = min(n_user, u0 + cu);
     uint64_t alo = 0, ahi = 0;
+    uint32_t wlo[4] = {0u, 0u, 0u, 0u}, whi[4] = {0u, 0u, 0u, 0u};
     int n = 0;
     for (int64_t u = u0 + 16 * (int64_t)threadIdx.x; u < u1; u += 16 * CNT) {
         const uint4 w = *reinterpret_cast<const uint4 *>(v + u);
@@ -581,14 +582,33 @@ __global__ void __launch_bounds__(CNT) k_coarse_hist(const uint8_t *__restrict__
         }
         n += 16;
         if (n >= 240) {
-            coarse_flush(alo, ahi, h);
+            widen_add(alo, wlo);
+            widen_add(ahi, whi);
             alo = ahi = 0;
             n = 0;
         }
     }
-    coarse_flush(alo, ahi, h);
+    widen_add(alo, wlo);
+    widen_add(ahi, whi);
+    uint32_t t[8] = {wlo[0], wlo[1], wlo[2], wlo[3], whi[0], whi[1], whi[2], whi[3]};
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) t[i] += __shfl_xor(t[i], o, 64);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) red[wave][i] = t[i];
     __syncthreads();
-    if (threadIdx.x < 16) part[((int64_t)b * n_chunk + blockIdx.x) * 16 + threadIdx.x] = h[threadIdx.x];
+    if (threadIdx.x < 16) {
+        // bin threadIdx.x: lo bins 0..7 then hi bins 0..7
+        const int i = threadIdx.x >> 1, sh = (threadIdx.x & 1) * 16;
+        uint32_t sum = 0;
+#pragma unroll
+        for (int w = 0; w < CNT / 64; ++w) sum += (red[w][i] >> sh) & 0xFFFFu;
+        const int bin = threadIdx.x & 7;
+        part[((int64_t)b * n_chunk + blockIdx.x) * 16 + threadIdx.x] = bin == 7 ? 0u : sum;
+    }
 }
 
 // One workgroup per query: beta_c = smallest c < n_guard with
@@ -881,7 +901,7 @@ hipError_t launch_topk(hipStream_t s, const QueryInfo *qi, int32_t nq, const int
 hipError_t launch_coarse_kth(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq, int64_t cu,
                              int32_t n_guard, const int32_t *qmap, uint32_t *part, const QueryInfo *qi, int32_t *cq) {
     if (nq <= 0 || n_user <= 0) return hipSuccess;
-    if (ldc % 16 || cu % 1024) return hipErrorInvalidValue;
+    if (ldc % 16 || cu % 1024 || cu >= (1 << 18)) return hipErrorInvalidValue;
     const int32_t n_chunk = (int32_t)((n_user + cu - 1) / cu);
     hipLaunchKernelGGL(k_coarse_hist, dim3((unsigned)n_chunk, (unsigned)nq), dim3(CNT), 0, s, cb, ldc, n_user, cu,
                        n_chunk, qmap, part);
