@@ -101,8 +101,12 @@ def serving(args, idx, wl, label):
     urng = np.random.default_rng(args.seed + 8)
 
     def updater():
+        # at most the reserved rounds: a run that falls behind its arrivals
+        # stops inserting rather than growing the side arrays without bound
         t_next = time.perf_counter() + 1.0
-        while not stop_upd.wait(max(0.0, t_next - time.perf_counter())):
+        for _ in range(rounds - 1):
+            if stop_upd.wait(max(0.0, t_next - time.perf_counter())):
+                break
             nu = urng.standard_normal((n_new_u, d), dtype=np.float32) / np.float32(np.sqrt(d))
             ni = urng.standard_normal((n_new_i, d), dtype=np.float32) / np.float32(np.sqrt(d))
             upd.append(srv.submit_update("insert_users", vectors=nu))

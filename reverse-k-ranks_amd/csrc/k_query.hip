@@ -616,7 +616,8 @@ __global__ void __launch_bounds__(CNT) k_coarse_hist(const uint8_t *__restrict__
 // and the number of users with c_hi <= beta_c.  cq[b] = {beta_c, count, -, 0}.
 __global__ void __launch_bounds__(CNT) k_coarse_kth(const uint32_t *__restrict__ part, int32_t n_chunk,
                                                     int32_t n_guard, const int32_t *__restrict__ qmap,
-                                                    const QueryInfo *__restrict__ qi, int32_t *__restrict__ cq) {
+                                                    const QueryInfo *__restrict__ qi, int32_t *__restrict__ cq,
+                                                    int32_t *__restrict__ hcnt) {
     __shared__ uint32_t sh[CNT];
     const int64_t b = qmap ? qmap[blockIdx.x] : blockIdx.x;
     uint32_t acc = 0;
@@ -630,6 +631,7 @@ __global__ void __launch_bounds__(CNT) k_coarse_kth(const uint32_t *__restrict__
         sh[threadIdx.x] = t;  // rows j >= 1 are no longer read
     }
     __syncthreads();
+    if (threadIdx.x < 8) hcnt[b * 8 + threadIdx.x] = threadIdx.x < 7 ? (int32_t)sh[8 + threadIdx.x] : 0;
     if (threadIdx.x != 0) return;
     const int64_t k = qi[b].topk;
     int32_t beta = -1;
@@ -788,6 +790,48 @@ __global__ void __launch_bounds__(CNT) k_coarse_exact(const float *__restrict__ 
     if (!same) atomicAdd(&g[S + 1 + hi], 1u);
 }
 
+// Update path, one thread per listed (query, user): the rank bound values of
+// the exact bounds with pending inserts and deletes (base bucket within the
+// guard range, side rows searched at the score rule's <u,q>), as
+// k_bounds_upd computes them, and <u,q> for the refinement and top-k.
+__global__ void __launch_bounds__(CNT) k_coarse_exact_upd(
+    const float *__restrict__ users, const float *__restrict__ queries, int32_t ld, int32_t dim,
+    const float *__restrict__ table, int32_t S, const int32_t *__restrict__ ranks, int64_t n_item_base, float eps,
+    const GuardCols *__restrict__ gsets, const int32_t *__restrict__ gset, const int32_t *__restrict__ cl_u,
+    const int32_t *__restrict__ cl_b, int64_t n, const float *__restrict__ ins, int32_t n_ins, int32_t ld_ins,
+    const float *__restrict__ del, int32_t n_del, int32_t ld_del, int64_t n_user, int32_t *__restrict__ lbs,
+    int32_t *__restrict__ ubs, float *__restrict__ qip) {
+    const int64_t i = (int64_t)blockIdx.x * CNT + threadIdx.x;
+    if (i >= n) return;
+    const int64_t u = cl_u[i];
+    const uint32_t packed = (uint32_t)cl_b[i];
+    const int64_t b = packed & 0xFFFFu;
+    const int32_t clo = (int32_t)((packed >> 16) & 7u), chi = (int32_t)((packed >> 20) & 7u);
+    const GuardCols gc = gsets[gset[b]];
+    const float sc = rule_dot(users + u * ld, queries + b * ld, dim);
+    int32_t r0 = 0, r1 = S;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (j == chi - 1) r0 = gc.col[j] + 1;
+        if (j == clo && j < gc.n) r1 = gc.col[j];
+    }
+    const float *row = table + u * (int64_t)S;
+    int32_t blo, bhi;
+    if (eps > 0.0f) {
+        blo = bucket_gt_in(row, r0, r1, __double2float_rd((double)sc - (double)eps));
+        bhi = bucket_gt_in(row, r0, r1, __double2float_ru((double)sc + (double)eps));
+    } else {
+        blo = bhi = bucket_gt_in(row, r0, r1, sc);
+    }
+    int32_t off = 0;
+    if (n_ins) off += bucket_gt(ins + u * (int64_t)ld_ins, n_ins, sc);
+    if (n_del) off -= bucket_gt(del + u * (int64_t)ld_del, n_del, sc);
+    const int64_t o = b * n_user + u;
+    lbs[o] = (blo == S ? (int32_t)n_item_base : ranks[blo]) + off;
+    ubs[o] = max((bhi == 0 ? 0 : ranks[bhi - 1] + 1) + off, 0);
+    qip[o] = sc;
+}
+
 // One thread per listed (query, user): the candidates (b_hi <= beta) with
 // their rank when the bounds fix it, else -1 and a refine row; <u,q> of each
 // candidate goes to qip for the top-k.
@@ -899,13 +943,15 @@ hipError_t launch_topk(hipStream_t s, const QueryInfo *qi, int32_t nq, const int
 }
 
 hipError_t launch_coarse_kth(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq, int64_t cu,
-                             int32_t n_guard, const int32_t *qmap, uint32_t *part, const QueryInfo *qi, int32_t *cq) {
+                             int32_t n_guard, const int32_t *qmap, uint32_t *part, const QueryInfo *qi, int32_t *cq,
+                             int32_t *hcnt) {
     if (nq <= 0 || n_user <= 0) return hipSuccess;
     if (ldc % 16 || cu % 1024 || cu >= (1 << 18)) return hipErrorInvalidValue;
     const int32_t n_chunk = (int32_t)((n_user + cu - 1) / cu);
     hipLaunchKernelGGL(k_coarse_hist, dim3((unsigned)n_chunk, (unsigned)nq), dim3(CNT), 0, s, cb, ldc, n_user, cu,
                        n_chunk, qmap, part);
-    hipLaunchKernelGGL(k_coarse_kth, dim3((unsigned)nq), dim3(CNT), 0, s, part, n_chunk, n_guard, qmap, qi, cq);
+    hipLaunchKernelGGL(k_coarse_kth, dim3((unsigned)nq), dim3(CNT), 0, s, part, n_chunk, n_guard, qmap, qi, cq,
+                       hcnt);
     return hipGetLastError();
 }
 
@@ -937,6 +983,19 @@ hipError_t launch_coarse_exact(hipStream_t s, const float *users, const float *q
                            dim, table, S, eps, same, gsets, gset, cl_u, cl_b, n, bk_c, s_c, ghist);
     hipLaunchKernelGGL(k_hist_kth, dim3((unsigned)nq), dim3(NT), 0, s, ghist, S, same, ranks, n_item,
                        (int)RKR_OUTPUT_EXACT, qi);
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_exact_upd(hipStream_t s, const float *users, const float *queries, int32_t ld, int32_t dim,
+                                   const float *table, int32_t S, const int32_t *ranks, int64_t n_item_base,
+                                   float eps, const GuardCols *gsets, const int32_t *gset, const int32_t *cl_u,
+                                   const int32_t *cl_b, int64_t n, const float *ins, int32_t n_ins, int32_t ld_ins,
+                                   const float *del, int32_t n_del, int32_t ld_del, int64_t n_user, int32_t *lbs,
+                                   int32_t *ubs, float *qip) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_coarse_exact_upd, dim3((unsigned)((n + CNT - 1) / CNT)), dim3(CNT), 0, s, users, queries, ld,
+                       dim, table, S, ranks, n_item_base, eps, gsets, gset, cl_u, cl_b, n, ins, n_ins, ld_ins, del,
+                       n_del, ld_del, n_user, lbs, ubs, qip);
     return hipGetLastError();
 }
 

@@ -173,6 +173,21 @@ __device__ __forceinline__ void bounds_values(const float *row, int32_t S, float
     ub = max((bhi == 0 ? 0 : ranks[bhi - 1] + 1) + off, 0);
 }
 
+// One thread per user: #{inserted scores > T[u][gc.col[j]]} for each guard,
+// maximised over the wave and then over the device (out zeroed before).
+__global__ void __launch_bounds__(256) k_ins_guard_max(const float *__restrict__ table, int64_t n_user, int32_t S,
+                                                       GuardCols gc, const float *__restrict__ ins, int32_t n_ins,
+                                                       int32_t ld_ins, int32_t *__restrict__ out) {
+    const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    for (int j = 0; j < gc.n; ++j) {
+        int32_t v = 0;
+        if (u < n_user) v = count_gt(ins + u * (int64_t)ld_ins, n_ins, table[u * (int64_t)S + gc.col[j]]);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+        if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(&out[j], v);
+    }
+}
+
 // grid (user blocks, queries); side rows at strides ld_ins / ld_del
 __global__ void __launch_bounds__(256) k_bounds_upd(const float *__restrict__ qip, const float *__restrict__ table,
                                                     int64_t n_user, int32_t S, const int32_t *__restrict__ ranks,
@@ -266,6 +281,8 @@ __global__ void __launch_bounds__(NT) k_vhist1(const int32_t *__restrict__ lbs, 
     __syncthreads();
     const int32_t *lv = lbs + b * n_user, *uv = ubs + b * n_user;
     for (int64_t u = u0 + threadIdx.x; u < u1; u += NT) {
+        // pairs without bounds (kNoBound) are never the k-th nor candidates
+        if (lv[u] == kNoBound) continue;
         atomicAdd(&hl[((uint32_t)lv[u] >> 11) & 2047u], 1u);
         atomicAdd(&hu[((uint32_t)uv[u] >> 11) & 2047u], 1u);
     }
@@ -310,6 +327,7 @@ __global__ void __launch_bounds__(NT) k_vhist2(const int32_t *__restrict__ lbs, 
     const int32_t *lv = lbs + b * n_user, *uv = ubs + b * n_user;
     for (int64_t u = u0 + threadIdx.x; u < u1; u += NT) {
         const uint32_t l = (uint32_t)lv[u], w = (uint32_t)uv[u];
+        if (l == (uint32_t)kNoBound) continue;
         if (((l >> 11) & 2047u) == hi) atomicAdd(&hl[l & 2047u], 1u);
         if (((w >> 11) & 2047u) == hi) atomicAdd(&hu[w & 2047u], 1u);
     }
@@ -367,7 +385,9 @@ __global__ void __launch_bounds__(NT) k_compact_values(const int32_t *__restrict
         const int64_t u = ub0 + threadIdx.x;
         bool cand = false;
         int32_t rank = -1;
-        if (u < u1 && uv[u] <= L) {
+        // a pair without bounds is never a candidate (a query whose bounds
+        // are all kNoBound selects L = kNoBound and counts no candidates)
+        if (u < u1 && uv[u] <= L && uv[u] != kNoBound) {
             cand = true;
             if (lv[u] == uv[u]) rank = lv[u];
         }
@@ -470,6 +490,15 @@ hipError_t launch_bounds_upd(hipStream_t s, const float *qip, const float *table
     dim3 grid((unsigned)((n_user + 255) / 256), (unsigned)nq);
     hipLaunchKernelGGL(k_bounds_upd, grid, dim3(256), 0, s, qip, table, n_user, S, ranks, n_item_base, eps, ins, n_ins,
                        ld_ins, del, n_del, ld_del, lbs, ubs);
+    return hipGetLastError();
+}
+
+hipError_t launch_ins_guard_max(hipStream_t s, const float *table, int64_t n_user, int32_t S, const GuardCols &gc,
+                                const float *ins, int32_t n_ins, int32_t ld_ins, int32_t *out) {
+    hipError_t e = hipMemsetAsync(out, 0, sizeof(int32_t) * 8, s);
+    if (e != hipSuccess || n_user <= 0 || n_ins <= 0) return e;
+    hipLaunchKernelGGL(k_ins_guard_max, dim3((unsigned)((n_user + 255) / 256)), dim3(256), 0, s, table, n_user, S, gc,
+                       ins, n_ins, ld_ins, out);
     return hipGetLastError();
 }
 

@@ -103,20 +103,25 @@ DataSnap snap_of(const Ctx *c) {
     d.qtable = c->qtable;
     d.qranks = c->qranks;
     d.n_qs = c->n_qs;
-    // guard columns of the coarse bounds: the table columns nearest below
-    // ranks 4^j - 1 (3, 15, ..., 16383), at most 7, ascending
+    guard_columns(c, d.gcol, d.n_guard);
+    for (int j = 0; j < 8; ++j) d.ins_gmax[j] = c->ins_gmax[j];
+    return d;
+}
+
+// Guard columns of the coarse bounds: the table columns nearest below ranks
+// 4^j - 1 (3, 15, ..., 16383), at most 7, ascending.
+void guard_columns(const Ctx *c, int32_t *col, int32_t &n) {
     const std::vector<int32_t> &hr = c->qtable ? c->h_qranks : c->h_ranks;
-    d.n_guard = 0;
+    n = 0;
     int32_t last = -1;
-    for (int64_t target = 3; target <= 16383 && d.n_guard < 7; target = target * 4 + 3) {
+    for (int64_t target = 3; target <= 16383 && n < 7; target = target * 4 + 3) {
         const auto it = std::upper_bound(hr.begin(), hr.end(), (int32_t)target);
         const int32_t j = (int32_t)(it - hr.begin()) - 1;
         if (j > last) {
-            d.gcol[d.n_guard++] = j;
+            col[n++] = j;
             last = j;
         }
     }
-    return d;
 }
 
 std::vector<int32_t> query_table_ranks(const std::vector<int32_t> &sample, int64_t n_item) {
@@ -769,7 +774,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
     const float eps = mode == RKR_OUTPUT_EXACT && !use_q ? c->table_eps : 0.0f;
     // pending item inserts/deletes: bounds become per-user values (k_update.hip)
     const bool upd = dv.n_ins > 0 || dv.n_del > 0;
-    const bool coarse = allow_coarse && c->coarse && c->form == RKR_FORM_SCREENED && mode == RKR_OUTPUT_EXACT && !upd &&
+    const bool coarse = allow_coarse && c->coarse && c->form == RKR_FORM_SCREENED && mode == RKR_OUTPUT_EXACT &&
                         c->kp > 0 && dv.user_norm && dv.n_guard > 0 && U > 0;
     const size_t pair_bytes = upd ? 12 : 8;  // <u,q> + buckets, or + two bound values
 
@@ -867,7 +872,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             const int64_t n_chunk = (U + ccu - 1) / ccu;
             // part | cq | loc | ctot | cbase | guard sets | gset | qmap | qset
             const size_t w_sets = sizeof(GuardCols) * 8 / 4;
-            if ((rc = ensure(c, sl.coarse_info, sizeof(uint32_t) * (17 * (size_t)nq * n_chunk + 7 * (size_t)nqb +
+            if ((rc = ensure(c, sl.coarse_info, sizeof(uint32_t) * (17 * (size_t)nq * n_chunk + 15 * (size_t)nqb +
                                                                    2 * (size_t)n_chunk + w_sets + 64),
                              "coarse counts")))
                 return rc;
@@ -878,11 +883,37 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             int32_t *cbase = ctot + n_chunk;
             GuardCols *gsets = reinterpret_cast<GuardCols *>(cbase + n_chunk);
             int32_t *gset = reinterpret_cast<int32_t *>(gsets + 8);
-            int32_t *qmap = gset + nqb;
-            RKR_HIP(c, launch_coarse_kth(sl.stream, cb, ldc, U, nq, ccu, dv.n_guard, nullptr, part, qi, cq));
+            int32_t *qmap = gset + nqb;  // then qset (nqb)
+            int32_t *hcnt = qmap + 2 * (size_t)nqb;
+            RKR_HIP(c, launch_coarse_kth(sl.stream, cb, ldc, U, nq, ccu, dv.n_guard, nullptr, part, qi, cq, hcnt));
             hcq.assign((size_t)nq * 4, 0);
+            std::vector<int32_t> hhc((size_t)nq * 8);
             RKR_HIP(c, hipMemcpyAsync(hcq.data(), cq, sizeof(int32_t) * 4 * nq, hipMemcpyDeviceToHost, sl.stream));
+            if (upd)
+                RKR_HIP(c, hipMemcpyAsync(hhc.data(), hcnt, sizeof(int32_t) * 8 * nq, hipMemcpyDeviceToHost, sl.stream));
             RKR_HIP(c, hipStreamSynchronize(sl.stream));
+            if (upd) {
+                // pending inserts move ranks: a user at or above guard m has
+                // rank <= R[j_m] + ins_gmax[m], so the k-th bound is at most
+                // V = R[j_beta] + ins_gmax[beta]; a user below guard c_hi - 1
+                // has rank >= R[j_{c_hi - 1}] + 1 - n_del, and is listed when
+                // that can be <= V
+                for (int32_t i = 0; i < nq; ++i) {
+                    int32_t *e = &hcq[(size_t)i * 4];
+                    if (e[0] < 0) continue;
+                    const int64_t V = (int64_t)hrk[gc.col[e[0]]] + dv.ins_gmax[e[0]];
+                    int32_t bl = 0;
+                    while (bl < gc.n && (int64_t)hrk[gc.col[bl]] + 1 - dv.n_del <= V) ++bl;
+                    if (bl >= gc.n) {
+                        e[0] = -1;  // every user could be a candidate: full path
+                        continue;
+                    }
+                    int64_t n = 0;
+                    for (int32_t cc = 0; cc <= bl; ++cc) n += hhc[(size_t)i * 8 + cc];
+                    e[0] = bl;
+                    e[1] = (int32_t)std::min<int64_t>(n, INT32_MAX);
+                }
+            }
             // second pass for long lists: the queries whose k-th bound fell in
             // guard bin beta are bucketed again against up to 7 columns inside
             // that bin (the bin's last column included), spaced geometrically
@@ -890,7 +921,9 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             std::vector<GuardCols> sets{gc};
             std::vector<int32_t> hgset((size_t)nq, 0), hqmap;
             std::vector<std::pair<int32_t, int32_t>> groups;  // (set, first index in hqmap)
-            for (int32_t beta = 0; beta < gc.n; ++beta) {
+            // (not with pending updates: the rank shift bounds are kept for
+            // the first pass's guards only)
+            for (int32_t beta = 0; beta < gc.n && !upd; ++beta) {
                 const int32_t lo_col = beta == 0 ? 0 : gc.col[beta - 1] + 1, hi_col = gc.col[beta];
                 if (hi_col <= lo_col) continue;
                 const size_t first = hqmap.size();
@@ -936,7 +969,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                     const int32_t first = groups[gi].second;
                     const int32_t end = gi + 1 < groups.size() ? groups[gi + 1].second : nr;
                     RKR_HIP(c, launch_coarse_kth(sl.stream, cb, ldc, U, end - first, ccu, sets[groups[gi].first].n,
-                                                 qmap + first, part, qi, cq));
+                                                 qmap + first, part, qi, cq, hcnt));
                 }
                 RKR_HIP(c, hipMemcpyAsync(hcq.data(), cq, sizeof(int32_t) * 4 * nq, hipMemcpyDeviceToHost, sl.stream));
                 RKR_HIP(c, hipStreamSynchronize(sl.stream));
@@ -966,8 +999,20 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                                       sl.stream));
             RKR_HIP(c, hipMemcpyAsync(gset, hgset.data(), sizeof(int32_t) * nq, hipMemcpyHostToDevice, sl.stream));
             RKR_HIP(c, launch_coarse_compact(sl.stream, cb, ldc, U, nq, ccu, part, cq, loc, ctot, cbase, cl_u, cl_b));
-            RKR_HIP(c, launch_coarse_exact(sl.stream, dv.users, q_in, ld, d, T, S, R, I, eps, gsets, gset, nq, cl_u,
-                                           cl_b, n_listed, bk_c, s_c, ghist, qi));
+            if (upd) {
+                // bound values of the listed pairs only (the rest are ruled
+                // out), then the value selection of the update path; the
+                // bound arrays reuse the coarse rows, read by now
+                RKR_HIP(c, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(lbv), kNoBound, 2 * (size_t)nq * U,
+                                             sl.stream));
+                RKR_HIP(c, launch_coarse_exact_upd(sl.stream, dv.users, q_in, ld, d, T, S, R, dv.n_item_base, eps, gsets,
+                                                   gset, cl_u, cl_b, n_listed, dv.side_ins, dv.n_ins, dv.ld_ins,
+                                                   dv.side_del, dv.n_del, dv.ld_del, U, lbv, ubv, qip));
+                RKR_HIP(c, launch_kth_values(sl.stream, lbv, ubv, U, nq, cu, ghist, qi));
+            } else {
+                RKR_HIP(c, launch_coarse_exact(sl.stream, dv.users, q_in, ld, d, T, S, R, I, eps, gsets, gset, nq, cl_u,
+                                               cl_b, n_listed, bk_c, s_c, ghist, qi));
+            }
             RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
         } else {
             // <u, q> for every (query, user): qip[b * U + u]
@@ -1011,7 +1056,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
         uint32_t *max_err_bits = reinterpret_cast<uint32_t *>(recheck_n + 1);
         const bool screened = c->form == RKR_FORM_SCREENED;
 
-        if (coarse_b)
+        if (coarse_b && !upd)
             RKR_HIP(c, launch_coarse_final(sl.stream, cl_u, cl_b, bk_c, s_c, n_listed, S, R, I, U, qi, cand_user,
                                            cand_rank, refine_total, refine_cand, refine_user, refine_query, refine_thr,
                                            qip));

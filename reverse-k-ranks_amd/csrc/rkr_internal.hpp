@@ -59,6 +59,7 @@ struct Ctx {
     int32_t screen_rows = 0;    // refine rows per wavefront of the screen (0 = 32)
     int dedup = 1;              // answer byte-equal queries of one call once
     int coarse = 1;             // coarse bounds on matrix cores (RKR_OPT_COARSE)
+    int32_t ins_gmax[8] = {};   // see DataSnap::ins_gmax (kept by the update path)
 
     // data
     int64_t n_user = 0, n_item = 0;
@@ -151,6 +152,12 @@ inline double max_abs(const float *x, size_t n) {
     return m;
 }
 
+// Update-path rank bound values stay below kMaxBoundValue (two-level 11-bit
+// histograms); kNoBound marks a (query, user) pair with no bound computed,
+// one the coarse bounds already ruled out.
+constexpr int64_t kMaxBoundValue = 1 << 22;
+constexpr int32_t kNoBound = (1 << 22) - 1;
+
 // Every partial sum of a score is bounded by dim * max|a| * max|b|; keep it
 // well inside FP32 range so no score overflows.
 constexpr double kScoreLimit = 1e37;
@@ -180,8 +187,15 @@ struct DataSnap {
     int32_t n_qs = 0;
     int32_t n_guard = 0;             // guard columns of the coarse bounds
     int32_t gcol[8] = {};
+    // pending inserts: the most inserted items any user has above its value
+    // at guard column j (a bound on the rank shift at that guard)
+    int32_t ins_gmax[8] = {};
 };
 DataSnap snap_of(const Ctx *c);
+// Guard columns of the coarse bounds for the context's searched table.
+void guard_columns(const Ctx *c, int32_t *col, int32_t &n);
+// Recomputes Ctx::ins_gmax after inserted items or users changed.
+int refresh_ins_guard_max(Ctx *c);
 // Ranks of the query table: the sample ranks and the dense top ranks below
 // n_item, ascending; empty when they would exceed kMaxSample.
 std::vector<int32_t> query_table_ranks(const std::vector<int32_t> &sample, int64_t n_item);
@@ -366,8 +380,21 @@ hipError_t launch_hist_kth(hipStream_t s, const uint32_t *bk, int64_t n_user, in
 // count, offset, fill}), the listed users, their exact scores and bucket
 // pairs with the exact k-th bound (into qi), and the final candidates.
 // cb rows are ldc bytes (ldc % 16 == 0, bytes past n_user unread); cu % 1024 == 0
+// hcnt[b * 8 + c] = #{users with c_hi == c}, c < 7
 hipError_t launch_coarse_kth(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq, int64_t cu,
-                             int32_t n_guard, const int32_t *qmap, uint32_t *part, const QueryInfo *qi, int32_t *cq);
+                             int32_t n_guard, const int32_t *qmap, uint32_t *part, const QueryInfo *qi, int32_t *cq,
+                             int32_t *hcnt);
+// Update path: exact rank bound values of the listed pairs (lbs / ubs /
+// qip at b * n_user + u; other pairs keep what the caller filled in).
+hipError_t launch_coarse_exact_upd(hipStream_t s, const float *users, const float *queries, int32_t ld, int32_t dim,
+                                   const float *table, int32_t S, const int32_t *ranks, int64_t n_item_base,
+                                   float eps, const GuardCols *gsets, const int32_t *gset, const int32_t *cl_u,
+                                   const int32_t *cl_b, int64_t n, const float *ins, int32_t n_ins, int32_t ld_ins,
+                                   const float *del, int32_t n_del, int32_t ld_del, int64_t n_user, int32_t *lbs,
+                                   int32_t *ubs, float *qip);
+// max over users of #{inserted scores > T[u][gc.col[j]]} into out[j] (zeroed here)
+hipError_t launch_ins_guard_max(hipStream_t s, const float *table, int64_t n_user, int32_t S, const GuardCols &gc,
+                                const float *ins, int32_t n_ins, int32_t ld_ins, int32_t *out);
 // chunk-major list: loc (n_chunk x nq), ctot and cbase (n_chunk) are scratch
 hipError_t launch_coarse_compact(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq,
                                  int64_t cu, const uint32_t *part, const int32_t *cq, int32_t *loc, int32_t *ctot,

@@ -41,7 +41,6 @@ namespace {
 
 constexpr int64_t kSortChunk = 1024;        // items scored per side-array merge
 constexpr int64_t kMaxSideItems = 16384;    // per-user side array length before a rebuild is required
-constexpr int64_t kMaxBoundValue = 1 << 22; // bound values must stay below this (k_vhist1/2)
 
 int check_updatable(Ctx *c) {
     if (!c->users) return set_err(c, RKR_ERR_STATE, "update: no data");
@@ -311,6 +310,23 @@ int keep_user_rows(Ctx *c, float **arr, int64_t w, int64_t ld, const int64_t *d_
 
 }  // namespace
 
+int rkr::refresh_ins_guard_max(Ctx *c) {
+    for (int j = 0; j < 8; ++j) c->ins_gmax[j] = 0;
+    const float *T = c->qtable ? c->qtable : c->table;
+    const int32_t S = c->qtable ? c->n_qs : c->n_sample;
+    if (!c->n_ins || !T || c->n_user <= 0) return RKR_OK;
+    GuardCols gc{};
+    guard_columns(c, gc.col, gc.n);
+    if (gc.n <= 0) return RKR_OK;
+    int rc;
+    if ((rc = ensure(c, c->upd_scratch, std::max<size_t>(c->upd_scratch.bytes, 64), "update scratch"))) return rc;
+    int32_t *d = reinterpret_cast<int32_t *>(c->upd_scratch.p);
+    RKR_HIP(c, launch_ins_guard_max(c->stream, T, c->n_user, S, gc, c->side_ins, c->n_ins, c->ld_ins, d));
+    RKR_HIP(c, hipMemcpyAsync(c->ins_gmax, d, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, c->stream));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    return RKR_OK;
+}
+
 extern "C" {
 
 int rkr_insert_items(rkr_ctx *c, const float *items, int64_t n) {
@@ -327,7 +343,8 @@ int rkr_insert_items(rkr_ctx *c, const float *items, int64_t n) {
         return set_err(c, RKR_ERR_INVALID, "insert_items: magnitudes would overflow FP32 scores");
     if ((int64_t)c->n_ins + n > kMaxSideItems)
         return set_err(c, RKR_ERR_NOMEM, "insert_items: too many pending inserts; rebuild the table");
-    if (c->n_item_base + c->n_ins + n >= kMaxBoundValue) return set_err(c, RKR_ERR_INVALID, "insert_items: too many items");
+    if (c->n_item_base + c->n_ins + n >= kMaxBoundValue - 1)
+        return set_err(c, RKR_ERR_INVALID, "insert_items: too many items");
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
     init_item_map(c);
@@ -348,7 +365,7 @@ int rkr_insert_items(rkr_ctx *c, const float *items, int64_t n) {
     if (c->kp) c->screen_abs = screen_abs_floor(c->kp, c->max_abs_user, c->max_abs_item);
     for (int64_t j = 0; j < n; ++j) c->cur_src.push_back(-(1 + (int64_t)ins0 + j));
     c->n_item = I0 + n;
-    return RKR_OK;
+    return rkr::refresh_ins_guard_max(c);
 }
 
 int rkr_delete_items(rkr_ctx *c, const int64_t *ids, int64_t n) {
@@ -487,7 +504,7 @@ int rkr_insert_users(rkr_ctx *c, const float *users, int64_t n) {
     c->max_abs_user = mu_new;
     if (c->kp) c->screen_abs = screen_abs_floor(c->kp, c->max_abs_user, c->max_abs_item);
     c->n_user = U1;
-    return RKR_OK;
+    return rkr::refresh_ins_guard_max(c);
 }
 
 int rkr_delete_users(rkr_ctx *c, const int64_t *ids, int64_t n) {

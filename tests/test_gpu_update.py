@@ -214,3 +214,45 @@ def test_reference_output_rejects_updates(gpu_index_factory):
     idx.set_output("reference")
     with pytest.raises(rkr.RkrError):
         idx.query(items[:2], 5)
+
+
+@pytest.mark.parametrize("deletes", [False, True])
+def test_coarse_bounds_with_updates(orc, gpu_index_factory, deletes):
+    """Screened form with pending inserts (and deletes): the coarse bounds
+    widen the list by the largest rank shift any user has at the k-th guard,
+    and the listed users get the update path's exact bound values; answers
+    and prune counts equal the full update path's and brute force."""
+    _case(f"coarse bounds with updates, deletes={deletes}")
+    rng, users, items = _data(77, U=9000, I=2500, d=32)
+    idx, _ = gpu_index_factory(users, items, 64)
+    m = Mirror(orc, users, items)
+    for r in range(3):
+        new = rng.standard_normal((60, items.shape[1]), dtype=np.float32) * np.float32(1.2)
+        idx.insert_items(new)
+        m.insert_items(new)
+        nu = rng.standard_normal((150, users.shape[1]), dtype=np.float32)
+        idx.insert_users(nu)
+        m.insert_users(nu)
+    if deletes:
+        gone = np.sort(rng.choice(len(m.items()), 40, replace=False)).astype(np.int64)
+        idx.delete_items(gone)
+        m.delete_items(gone)
+    cur = m.items()
+    nq = 48
+    queries = np.concatenate([cur[rng.choice(len(cur), nq // 2, replace=False)],
+                              rng.standard_normal((nq // 2, cur.shape[1]), dtype=np.float32)])
+    ks = [int(k) for k in rng.integers(1, 150, nq)]
+    idx.set_form("screened")
+    idx.set_option(9, 0)  # RKR_OPT_COARSE off: the full update path
+    full = idx.query(queries, ks)
+    idx.set_option(9, 1)
+    coarse = idx.query(queries, ks)
+    assert coarse.timing["n_listed"] > 0
+    bu, br, bq = orc.bruteforce(m.users, cur, queries, max(ks))
+    for i, k in enumerate(ks):
+        for res in (full, coarse):
+            assert np.array_equal(res.users[i, :k], bu[i, :k]), f"query {i}: users differ"
+            assert np.array_equal(res.ranks[i, :k], br[i, :k]), f"query {i}: ranks differ"
+        assert np.array_equal(full.qips[i, :k], coarse.qips[i, :k])
+        for f in ("n_candidate", "n_refine_user", "n_prune_user", "n_result_user"):
+            assert full.stats[i][f] == coarse.stats[i][f], f"query {i}: {f}"
