@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--screen-rows", type=int, default=0, choices=[0, 32, 64],
                     help="refine rows per wavefront of the screen (0 = library default)")
     ap.add_argument("--no-dedup", action="store_true", help="compute byte-equal queries of a call separately")
+    ap.add_argument("--dense-top", type=int, default=1, choices=[0, 1],
+                    help="0: no query table with dense top ranks (sampled table only)")
+    ap.add_argument("--coarse", type=int, default=1, choices=[0, 1], help="0: no coarse bounds (full bound search)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy (for scaling rocprofv3 byte counters) and exit")
     ap.add_argument("--update-mode", default="snapshot", choices=["snapshot", "barrier"],
@@ -216,7 +219,7 @@ def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_sample_q):
 # plain 1 GiB device copy gives, bench.py --copy-calibration).  Keyed by
 # (workload, form); other configurations report null.
 PROFILED_TRAFFIC = {
-    ("plain_gpu", "screened"): {"bytes": 2.886e9, "fetch_kib": 1331841.1, "write_kib": 21502.3,
+    ("plain_gpu", "screened"): {"bytes": 3.564e8, "fetch_kib": 168269.6, "write_kib": 11534.1,
                                 "profile": "profiles/k_refine_screen_100k_x_20k_d150_k50.md"},
 }
 
@@ -254,6 +257,8 @@ def main():
 
     idx = rkr.Index(0)
     idx.set_form(args.form)
+    idx.set_option(8, args.dense_top)  # RKR_OPT_DENSE_TOP
+    idx.set_option(9, args.coarse)     # RKR_OPT_COARSE
     idx.set_data(wl.users, wl.items, normalize_users=True)
     ranks = rkr.Index.uniform_sample_ranks(I, wl.n_sample)
     t0 = time.perf_counter()
@@ -312,7 +317,23 @@ def main():
     n_cand_per_step = sum(s["n_candidate"] for s in st)
     # dominant kernel and its algorithmic work
     dom = max(("refine", "ip", "bound"), key=lambda s: stages[s])
-    if dom == "refine":
+    coarse_used = tot.get("n_listed", 0) > 0 or tot.get("n_fallback", 0) > 0
+    n_distinct = int(len(np.unique(wl.queries, axis=0))) if not args.no_dedup else nq
+    if dom != "refine" and coarse_used:
+        # the coarse-bound kernel (first pass) is the largest single kernel of
+        # the ip + bound stages: 2*d flops per (query, user) pair, executed
+        # as 3 BF16 products over d padded to 32 on the matrix cores
+        kp = (d + 31) // 32 * 32
+        flops = 2.0 * U * d * n_distinct * args.steps
+        ach = flops / (tot["ip_ms"] * 1e-3) / 1e12
+        executed = ach * 3 * kp / d
+        roof = {"kernel": "k_ip_coarse", "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_BF16_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(ach / PEAK_BF16_TFLOPS, 4), "traffic": None,
+                "executed_mfma_tflops": round(executed, 2), "executed_frac": round(executed / PEAK_BF16_TFLOPS, 4),
+                "avg_launch_ms": round(tot["ip_ms"] / launches, 4),
+                "work_per_launch": f"2*U*d algorithmic flops x {n_distinct * args.steps / launches:.0f} distinct queries "
+                                   f"(executed: 3*2*U*{kp}); first pass only, the second pass is in the bound stage"}
+    elif dom == "refine":
         flops = 2.0 * I * d * tot["n_refined"]
         ach = flops / (tot["refine_ms"] * 1e-3) / 1e12
         if args.form == "screened":
@@ -381,7 +402,8 @@ def main():
         "data": "synthetic (seeded), inputs resident in HBM",
         "config": {"workload": label, "form": args.form, "queries_per_step": nq,
                    "distinct_queries_per_step": int(len(np.unique(wl.queries, axis=0))),
-                   "dedup": not args.no_dedup, "n_user": U, "n_item": I, "dim": d, "n_sample": wl.n_sample,
+                   "dedup": not args.no_dedup, "dense_top": bool(args.dense_top), "coarse": bool(args.coarse),
+                   "n_user": U, "n_item": I, "dim": d, "n_sample": wl.n_sample,
                    "autotuned": tuned},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
