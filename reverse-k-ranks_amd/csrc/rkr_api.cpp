@@ -154,6 +154,24 @@ int ensure(Ctx *c, DeviceBuf &b, size_t bytes, const char *what) {
     return RKR_OK;
 }
 
+int ensure_pinned(Ctx *c, PinnedBuf &b, size_t bytes) {
+    if (b.bytes >= bytes) return RKR_OK;
+    free_pinned(b);
+    const size_t want = std::max(bytes, b.bytes + b.bytes / 2);
+    if (hipHostMalloc(&b.p, want, hipHostMallocDefault) != hipSuccess) {
+        b.p = nullptr;
+        return set_err(c, RKR_ERR_NOMEM, "pinned host staging (" + std::to_string(want) + " bytes)");
+    }
+    b.bytes = want;
+    return RKR_OK;
+}
+
+void free_pinned(PinnedBuf &b) {
+    if (b.p) (void)hipHostFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
 namespace {
 
 void free_update_state(Ctx *c) {
@@ -242,6 +260,7 @@ void slot_free(Ctx *c, Slot &sl) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     free_bufs(c, {&sl.q_in, &sl.qip, &sl.bk, &sl.qinfo, &sl.cand, &sl.refine_idx, &sl.counts, &sl.out, &sl.recheck,
                   &sl.small, &sl.hist, &sl.q_split, &sl.coarse_info, &sl.coarse_list});
+    free_pinned(sl.stage);
     for (auto &e : sl.ev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (sl.own_stream && sl.stream) (void)hipStreamDestroy(sl.stream);
@@ -811,11 +830,31 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
     // list longer than coarse_cap takes the full path instead
     const int64_t coarse_cap = std::max<int64_t>(4096, U / 4);
 
-    std::vector<float> hq((size_t)nqb * ld, 0.0f);
-    std::vector<QueryInfo> hqi((size_t)nqb);
-    std::vector<int32_t> ho_user((size_t)nqb * max_k), ho_rank((size_t)nqb * max_k);
-    std::vector<float> ho_qip((size_t)nqb * max_k);
-    std::vector<int32_t> hcq;
+    // pinned staging: queries | query info | outputs (the device layout of
+    // sl.out) | coarse rows, bin counts, guard sets, maps | recheck counts,
+    // scalars; sections 256-byte aligned
+    const size_t out_words = 3 * (size_t)nqb * max_k;
+    size_t sz[10] = {sizeof(float) * (size_t)nqb * ld, sizeof(QueryInfo) * (size_t)nqb, 4 * out_words,
+                     sizeof(int32_t) * 4 * (size_t)nqb, sizeof(int32_t) * 8 * (size_t)nqb, sizeof(GuardCols) * 8,
+                     sizeof(int32_t) * 3 * (size_t)nqb, sizeof(int32_t) * (size_t)nqb, 64, 0};
+    size_t off[10], tot = 0;
+    for (int i = 0; i < 10; ++i) {
+        off[i] = tot;
+        tot += (sz[i] + 255) & ~(size_t)255;
+    }
+    if ((rc = ensure_pinned(c, sl.stage, tot))) return rc;
+    char *stg = reinterpret_cast<char *>(sl.stage.p);
+    float *hq = reinterpret_cast<float *>(stg + off[0]);
+    QueryInfo *hqi = reinterpret_cast<QueryInfo *>(stg + off[1]);
+    int32_t *ho_user = reinterpret_cast<int32_t *>(stg + off[2]);
+    int32_t *ho_rank = ho_user + (size_t)nqb * max_k;
+    float *ho_qip = reinterpret_cast<float *>(ho_rank + (size_t)nqb * max_k);
+    int32_t *hcq = reinterpret_cast<int32_t *>(stg + off[3]);
+    int32_t *hhc = reinterpret_cast<int32_t *>(stg + off[4]);
+    GuardCols *hsets = reinterpret_cast<GuardCols *>(stg + off[5]);
+    int32_t *hgset = reinterpret_cast<int32_t *>(stg + off[6]);  // then qmap, qset
+    int32_t *h_qrecheck = reinterpret_cast<int32_t *>(stg + off[7]);
+    int32_t *h_scalars = reinterpret_cast<int32_t *>(stg + off[8]);  // refine total, recheck count, error bits
 
     for (int64_t q0 = 0; q0 < n_query; q0 += nqb) {
         const int32_t nq = (int32_t)std::min<int64_t>(nqb, n_query - q0);
@@ -830,12 +869,13 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
 
         for (int32_t i = 0; i < nq; ++i) {
             std::memcpy(&hq[(size_t)i * ld], queries + (size_t)(q0 + i) * d, sizeof(float) * d);
+            for (int32_t k = d; k < ld; ++k) hq[(size_t)i * ld + k] = 0.0f;
             hqi[i] = QueryInfo{};
             hqi[i].topk = topk[q0 + i];
         }
         RKR_HIP(c, hipEventRecord(sl.ev[0], sl.stream));
-        RKR_HIP(c, hipMemcpyAsync(q_in, hq.data(), sizeof(float) * (size_t)nq * ld, hipMemcpyHostToDevice, sl.stream));
-        RKR_HIP(c, hipMemcpyAsync(qi, hqi.data(), sizeof(QueryInfo) * nq, hipMemcpyHostToDevice, sl.stream));
+        RKR_HIP(c, hipMemcpyAsync(q_in, hq, sizeof(float) * (size_t)nq * ld, hipMemcpyHostToDevice, sl.stream));
+        RKR_HIP(c, hipMemcpyAsync(qi, hqi, sizeof(QueryInfo) * nq, hipMemcpyHostToDevice, sl.stream));
         RKR_HIP(c, hipEventRecord(sl.ev[1], sl.stream));
         int32_t *lbv = reinterpret_cast<int32_t *>(bk);
         int32_t *ubv = lbv + (size_t)nq * U;
@@ -855,7 +895,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             uint16_t *q_lo = q_hi + (size_t)nqb * c->kp;
             float *qn = reinterpret_cast<float *>(q_lo + (size_t)nqb * c->kp);
             RKR_HIP(c, launch_prep_items(sl.stream, q_in, nq, ld, d, c->kp, q_hi, q_lo, qn));
-            const double mq = max_abs(hq.data(), (size_t)nq * ld);
+            const double mq = max_abs(hq, (size_t)nq * ld);
             const double absf = ((double)screen_abs_floor(c->kp, dv.max_abs_user, mq) + (double)eps) *
                                 (1.0 + std::ldexp(1.0, -18));
             uint8_t *cb = reinterpret_cast<uint8_t *>(bk);
@@ -886,11 +926,9 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             int32_t *qmap = gset + nqb;  // then qset (nqb)
             int32_t *hcnt = qmap + 2 * (size_t)nqb;
             RKR_HIP(c, launch_coarse_kth(sl.stream, cb, ldc, U, nq, ccu, dv.n_guard, nullptr, part, qi, cq, hcnt));
-            hcq.assign((size_t)nq * 4, 0);
-            std::vector<int32_t> hhc((size_t)nq * 8);
-            RKR_HIP(c, hipMemcpyAsync(hcq.data(), cq, sizeof(int32_t) * 4 * nq, hipMemcpyDeviceToHost, sl.stream));
+            RKR_HIP(c, hipMemcpyAsync(hcq, cq, sizeof(int32_t) * 4 * nq, hipMemcpyDeviceToHost, sl.stream));
             if (upd)
-                RKR_HIP(c, hipMemcpyAsync(hhc.data(), hcnt, sizeof(int32_t) * 8 * nq, hipMemcpyDeviceToHost, sl.stream));
+                RKR_HIP(c, hipMemcpyAsync(hhc, hcnt, sizeof(int32_t) * 8 * nq, hipMemcpyDeviceToHost, sl.stream));
             RKR_HIP(c, hipStreamSynchronize(sl.stream));
             if (upd) {
                 // pending inserts move ranks: a user at or above guard m has
@@ -919,7 +957,8 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             // that bin (the bin's last column included), spaced geometrically
             // in rank
             std::vector<GuardCols> sets{gc};
-            std::vector<int32_t> hgset((size_t)nq, 0), hqmap;
+            std::vector<int32_t> hqmap;
+            for (int32_t i = 0; i < nq; ++i) hgset[i] = 0;
             std::vector<std::pair<int32_t, int32_t>> groups;  // (set, first index in hqmap)
             // (not with pending updates: the rank shift bounds are kept for
             // the first pass's guards only)
@@ -951,13 +990,16 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                 // one launch for every refined query: set of each (local)
                 // query in qset, the sets themselves after set 0
                 const int32_t nr = (int32_t)hqmap.size();
-                std::vector<int32_t> hqset((size_t)nr);
-                for (int32_t t = 0; t < nr; ++t) hqset[t] = hgset[hqmap[t]] - 1;
+                int32_t *h_map = hgset + nqb, *h_set = h_map + nr;
+                for (int32_t t = 0; t < nr; ++t) {
+                    h_map[t] = hqmap[t];
+                    h_set[t] = hgset[hqmap[t]] - 1;
+                }
+                for (size_t t = 1; t < sets.size(); ++t) hsets[t - 1] = sets[t];
                 int32_t *qset = qmap + nr;
-                RKR_HIP(c, hipMemcpyAsync(qmap, hqmap.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice, sl.stream));
-                RKR_HIP(c, hipMemcpyAsync(qset, hqset.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice, sl.stream));
-                RKR_HIP(c, hipMemcpyAsync(gsets, sets.data() + 1, sizeof(GuardCols) * (sets.size() - 1),
-                                          hipMemcpyHostToDevice, sl.stream));
+                RKR_HIP(c, hipMemcpyAsync(qmap, h_map, sizeof(int32_t) * 2 * nr, hipMemcpyHostToDevice, sl.stream));
+                RKR_HIP(c, hipMemcpyAsync(gsets, hsets, sizeof(GuardCols) * (sets.size() - 1), hipMemcpyHostToDevice,
+                                          sl.stream));
                 CoarseLaunch L2 = L;
                 L2.qmap = qmap;
                 L2.nq = nr;
@@ -971,7 +1013,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                     RKR_HIP(c, launch_coarse_kth(sl.stream, cb, ldc, U, end - first, ccu, sets[groups[gi].first].n,
                                                  qmap + first, part, qi, cq, hcnt));
                 }
-                RKR_HIP(c, hipMemcpyAsync(hcq.data(), cq, sizeof(int32_t) * 4 * nq, hipMemcpyDeviceToHost, sl.stream));
+                RKR_HIP(c, hipMemcpyAsync(hcq, cq, sizeof(int32_t) * 4 * nq, hipMemcpyDeviceToHost, sl.stream));
                 RKR_HIP(c, hipStreamSynchronize(sl.stream));
                 tm.n_coarse_refined += (int64_t)hqmap.size();
             }
@@ -994,10 +1036,10 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             cl_b = cl_u + std::max<int64_t>(n_listed, 1);
             bk_c = reinterpret_cast<uint32_t *>(cl_b + std::max<int64_t>(n_listed, 1));
             s_c = reinterpret_cast<float *>(bk_c + std::max<int64_t>(n_listed, 1));
-            RKR_HIP(c, hipMemcpyAsync(cq, hcq.data(), sizeof(int32_t) * 4 * nq, hipMemcpyHostToDevice, sl.stream));
-            RKR_HIP(c, hipMemcpyAsync(gsets, sets.data(), sizeof(GuardCols) * sets.size(), hipMemcpyHostToDevice,
-                                      sl.stream));
-            RKR_HIP(c, hipMemcpyAsync(gset, hgset.data(), sizeof(int32_t) * nq, hipMemcpyHostToDevice, sl.stream));
+            for (size_t t = 0; t < sets.size(); ++t) hsets[t] = sets[t];
+            RKR_HIP(c, hipMemcpyAsync(cq, hcq, sizeof(int32_t) * 4 * nq, hipMemcpyHostToDevice, sl.stream));
+            RKR_HIP(c, hipMemcpyAsync(gsets, hsets, sizeof(GuardCols) * sets.size(), hipMemcpyHostToDevice, sl.stream));
+            RKR_HIP(c, hipMemcpyAsync(gset, hgset, sizeof(int32_t) * nq, hipMemcpyHostToDevice, sl.stream));
             RKR_HIP(c, launch_coarse_compact(sl.stream, cb, ldc, U, nq, ccu, part, cq, loc, ctot, cbase, cl_u, cl_b));
             if (upd) {
                 // bound values of the listed pairs only (the rest are ruled
@@ -1032,7 +1074,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                                            (mode == RKR_OUTPUT_EXACT && eps == 0.0f) ? 1 : 0, cu, ghist, qi));
         }
         RKR_HIP(c, launch_offsets(sl.stream, qi, nq, refine_total));
-        RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
+        RKR_HIP(c, hipMemcpyAsync(hqi, qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
         RKR_HIP(c, hipStreamSynchronize(sl.stream));
         const int64_t total = hqi[nq - 1].cand_off + hqi[nq - 1].n_settled + hqi[nq - 1].n_cand;
         int64_t total_cand = 0;
@@ -1066,9 +1108,11 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
         else
             RKR_HIP(c, launch_compact(sl.stream, bk, qip, U, S, R, I, nq, mode, cu, qi, cand_user, cand_rank,
                                       refine_total, refine_cand, refine_user, refine_query, refine_thr));
-        int32_t h_refine_total = 0, h_recheck_n = 0;
-        uint32_t h_err_bits = 0;
-        std::vector<int32_t> h_qrecheck((size_t)nq, 0);
+        int32_t &h_refine_total = h_scalars[0], &h_recheck_n = h_scalars[1];
+        uint32_t &h_err_bits = reinterpret_cast<uint32_t *>(h_scalars)[2];
+        h_refine_total = h_recheck_n = 0;
+        h_err_bits = 0;
+        for (int32_t i = 0; i < nq; ++i) h_qrecheck[i] = 0;
         // refinement + top-k; re-run when the recheck list overflowed
         for (int attempt = 0; attempt < 3; ++attempt) {
             int64_t rec_cap = 0;
@@ -1107,15 +1151,15 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             RKR_HIP(c, launch_topk(sl.stream, qi, nq, cand_user, cand_rank, qip, U, max_k, o_user, o_rank, o_qip,
                                    nullptr));
             RKR_HIP(c, hipEventRecord(sl.ev[6], sl.stream));
-            RKR_HIP(c, hipMemcpyAsync(ho_user.data(), o_user, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, sl.stream));
-            RKR_HIP(c, hipMemcpyAsync(ho_rank.data(), o_rank, sizeof(int32_t) * (size_t)nq * max_k, hipMemcpyDeviceToHost, sl.stream));
-            RKR_HIP(c, hipMemcpyAsync(ho_qip.data(), o_qip, sizeof(float) * (size_t)nq * max_k, hipMemcpyDeviceToHost, sl.stream));
-            RKR_HIP(c, hipMemcpyAsync(hqi.data(), qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
+            // outputs in one copy (the device layout), then the counters
+            RKR_HIP(c, hipMemcpyAsync(ho_user, o_user, 4 * out_words, hipMemcpyDeviceToHost, sl.stream));
+            RKR_HIP(c, hipMemcpyAsync(hqi, qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
             RKR_HIP(c, hipMemcpyAsync(&h_refine_total, refine_total, sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream));
             if (screened) {
-                RKR_HIP(c, hipMemcpyAsync(&h_recheck_n, recheck_n, sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream));
-                RKR_HIP(c, hipMemcpyAsync(&h_err_bits, max_err_bits, sizeof(uint32_t), hipMemcpyDeviceToHost, sl.stream));
-                RKR_HIP(c, hipMemcpyAsync(h_qrecheck.data(), q_recheck, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, sl.stream));
+                // q_recheck, recheck_n, max_err_bits are adjacent in sl.small
+                RKR_HIP(c, hipMemcpyAsync(h_qrecheck, q_recheck, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, sl.stream));
+                RKR_HIP(c, hipMemcpyAsync(&h_recheck_n, recheck_n, sizeof(int32_t) * 2, hipMemcpyDeviceToHost,
+                                          sl.stream));
             }
             RKR_HIP(c, hipEventRecord(sl.ev[7], sl.stream));
             RKR_HIP(c, hipStreamSynchronize(sl.stream));
@@ -1129,9 +1173,9 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             if (attempt == 2) return set_err(c, RKR_ERR_NOMEM, "query: recheck list kept overflowing");
         }
 
-        std::memcpy(out_user + (size_t)q0 * max_k, ho_user.data(), sizeof(int32_t) * (size_t)nq * max_k);
-        std::memcpy(out_rank + (size_t)q0 * max_k, ho_rank.data(), sizeof(int32_t) * (size_t)nq * max_k);
-        if (out_qip) std::memcpy(out_qip + (size_t)q0 * max_k, ho_qip.data(), sizeof(float) * (size_t)nq * max_k);
+        std::memcpy(out_user + (size_t)q0 * max_k, ho_user, sizeof(int32_t) * (size_t)nq * max_k);
+        std::memcpy(out_rank + (size_t)q0 * max_k, ho_rank, sizeof(int32_t) * (size_t)nq * max_k);
+        if (out_qip) std::memcpy(out_qip + (size_t)q0 * max_k, ho_qip, sizeof(float) * (size_t)nq * max_k);
         if (stats) {
             for (int32_t i = 0; i < nq; ++i) {
                 rkr_query_stats &st = stats[q0 + i];

@@ -33,12 +33,20 @@ struct DeviceBuf {
 // One in-flight query batch: its stream, timing events and scratch buffers.
 // rkr_query uses the context's main slot; the serving loop gives each worker
 // its own slot so batches overlap on the device.
+// Page-locked host memory for a slot's staging copies (asynchronous DMA, no
+// bounce buffer); grown between calls only.
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
 struct Slot {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     hipEvent_t ev[16] = {};
     DeviceBuf q_in, qip, bk, qinfo, cand, refine_idx, counts, out, recheck, small, hist;
     DeviceBuf q_split, coarse_info, coarse_list;  // coarse bounds
+    PinnedBuf stage;                               // host side of the batch copies
 };
 
 struct Ctx {
@@ -205,6 +213,8 @@ int set_err(Ctx *c, int code, const std::string &msg);
 int dev_alloc(Ctx *c, void **p, size_t bytes, const char *what);
 void dev_free(Ctx *c, void *p, size_t bytes);
 int ensure(Ctx *c, DeviceBuf &b, size_t bytes, const char *what);
+int ensure_pinned(Ctx *c, PinnedBuf &b, size_t bytes);
+void free_pinned(PinnedBuf &b);
 // cu_mask (optional): the CUs the slot's stream may use
 int slot_init(Ctx *c, Slot &sl, bool own_stream, const std::vector<uint32_t> *cu_mask = nullptr);
 void slot_free(Ctx *c, Slot &sl);

@@ -7,6 +7,8 @@ missing or no GPU is present, every call fails loudly.
 from __future__ import annotations
 
 import ctypes as C
+
+import numpy as np
 import os
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -61,6 +63,12 @@ class QueryStats(C.Structure):
     _fields_ = [("query_id", C.c_int32), ("n_prune_user", C.c_int32), ("n_result_user", C.c_int32),
                 ("n_refine_user", C.c_int32), ("n_candidate", C.c_int32), ("reserved", C.c_int32),
                 ("ip_cost", C.c_int64), ("n_rechecked", C.c_int64), ("n_screened", C.c_int64)]
+
+
+# numpy view of rkr_query_stats (same field order, sizes and offsets)
+QUERY_STATS_DTYPE = np.dtype([(f, np.int32 if t is C.c_int32 else np.int64) for f, t in QueryStats._fields_],
+                             align=True)
+assert QUERY_STATS_DTYPE.itemsize == C.sizeof(QueryStats)
 
 
 class BatchTiming(C.Structure):

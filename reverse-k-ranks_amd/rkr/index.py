@@ -190,13 +190,15 @@ class Index:
         users = np.empty((nq, mk), np.int32)
         ranks = np.empty((nq, mk), np.int32)
         qips = np.empty((nq, mk), np.float32)
-        stats = (L.QueryStats * max(nq, 1))()
+        # per-query stats land in a structured array with rkr_query_stats' layout
+        # (indexable as stats[i]["n_candidate"]; no per-query Python objects)
+        stats = np.empty(max(nq, 1), L.QUERY_STATS_DTYPE)
         tm = L.BatchTiming()
         _check(self.lib, self.ctx, self.lib.rkr_query(self.ctx, _ptr(queries), nq, _ptr(ks), mk, _ptr(users),
-                                                      _ptr(ranks), _ptr(qips), stats, C.byref(tm)))
-        st = [{f: getattr(s, f) for f, _ in L.QueryStats._fields_} for s in stats[:nq]]
+                                                      _ptr(ranks), _ptr(qips),
+                                                      stats.ctypes.data_as(C.POINTER(L.QueryStats)), C.byref(tm)))
         timing = {f: getattr(tm, f) for f, _ in L.BatchTiming._fields_}
-        return QueryResult(users, ranks, qips, st, timing)
+        return QueryResult(users, ranks, qips, stats[:nq], timing)
 
     def retrieval(self, query_item, topk: int, n_execute_query=None):
         """US::Index::Retrieval shape: list (per query) of topk UserRankElement, plus per-query stats."""
