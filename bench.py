@@ -67,8 +67,8 @@ def parse():
     ap.add_argument("--screen-rows", type=int, default=0, choices=[0, 32, 64],
                     help="refine rows per wavefront of the screen (0 = library default)")
     ap.add_argument("--no-dedup", action="store_true", help="compute byte-equal queries of a call separately")
-    ap.add_argument("--dense-top", type=int, default=1, choices=[0, 1],
-                    help="0: no query table with dense top ranks (sampled table only)")
+    ap.add_argument("--dense-top", type=int, default=2, choices=[0, 1, 2],
+                    help="query table: 0 none (sampled table only), 1 dense top ranks, 2 every rank to the cap")
     ap.add_argument("--coarse", type=int, default=1, choices=[0, 1], help="0: no coarse bounds (full bound search)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy (for scaling rocprofv3 byte counters) and exit")

@@ -60,11 +60,12 @@ typedef struct rkr_ctx rkr_ctx;
                                    call are answered once, with their largest k   */
 #define RKR_OPT_SCREEN_ROWS 7   /* refine rows per wavefront of the screen: 32 or 64
                                    (0 = 32)                                       */
-#define RKR_OPT_DENSE_TOP 8     /* 1 (default): rkr_build_table also keeps a query
-                                   table with dense top ranks (0..255, every 8th
-                                   to 1023, every 32nd to 4095) that exact output
-                                   searches; answers are unchanged, refinement
-                                   shrinks.  The sampled table is unchanged.     */
+#define RKR_OPT_DENSE_TOP 8     /* query table that exact output searches, kept
+                                   by rkr_build_table beside the unchanged sampled
+                                   table: 2 (default) every rank below the largest
+                                   D that fits 4096 columns; 1: 0..255, every 8th
+                                   to 1023, every 32nd to 4095; 0: none.  Answers
+                                   are unchanged; refinement shrinks.            */
 #define RKR_OPT_COARSE 9        /* 1 (default): screened form, exact output: the
                                    batch's <u,q> runs on BF16 matrix cores against
                                    7 guard columns per user, and only the users

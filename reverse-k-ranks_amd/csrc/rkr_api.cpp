@@ -124,11 +124,23 @@ void guard_columns(const Ctx *c, int32_t *col, int32_t &n) {
     }
 }
 
-std::vector<int32_t> query_table_ranks(const std::vector<int32_t> &sample, int64_t n_item) {
+std::vector<int32_t> query_table_ranks(const std::vector<int32_t> &sample, int64_t n_item, int mode) {
     std::vector<int32_t> r(sample);
-    for (int64_t x = 0; x < 256; ++x) r.push_back((int32_t)x);
-    for (int64_t x = 256; x < 1024; x += 8) r.push_back((int32_t)x);
-    for (int64_t x = 1024; x < 4096; x += 32) r.push_back((int32_t)x);
+    if (mode == 2) {
+        // every rank below D, with D as large as the column cap allows
+        int64_t lo = 0, hi = std::min<int64_t>(n_item, kMaxSample);
+        while (lo < hi) {
+            const int64_t D = (lo + hi + 1) / 2;
+            int64_t above = 0;
+            for (int32_t x : sample) above += x >= D ? 1 : 0;
+            if (D + above <= kMaxSample) lo = D; else hi = D - 1;
+        }
+        for (int64_t x = 0; x < lo; ++x) r.push_back((int32_t)x);
+    } else {
+        for (int64_t x = 0; x < 256; ++x) r.push_back((int32_t)x);
+        for (int64_t x = 256; x < 1024; x += 8) r.push_back((int32_t)x);
+        for (int64_t x = 1024; x < 4096; x += 32) r.push_back((int32_t)x);
+    }
     std::vector<int32_t> out;
     std::sort(r.begin(), r.end());
     for (int32_t x : r)
@@ -378,7 +390,7 @@ int rkr_set_option(rkr_ctx *c, int option, int64_t value) {
             c->screen_chunk = (int32_t)value;
             return RKR_OK;
         case RKR_OPT_DENSE_TOP:
-            if (value != 0 && value != 1) return set_err(c, RKR_ERR_INVALID, "dense top must be 0 or 1");
+            if (value < 0 || value > 2) return set_err(c, RKR_ERR_INVALID, "dense top must be 0, 1 or 2");
             c->dense_top = (int)value;
             return RKR_OK;
         case RKR_OPT_SCREEN_ROWS:
@@ -537,7 +549,7 @@ int rkr_build_table(rkr_ctx *c, const int32_t *ranks, int32_t S, double *build_m
     RKR_HIP(c, hipMemcpyAsync(c->ranks, ranks, sizeof(int32_t) * S, hipMemcpyHostToDevice, c->stream));
     if (c->dense_top) {
         // the query table: select at its ranks, then copy the sampled columns
-        c->h_qranks = query_table_ranks(c->h_ranks, c->n_item);
+        c->h_qranks = query_table_ranks(c->h_ranks, c->n_item, c->dense_top);
         const int32_t nqs = (int32_t)c->h_qranks.size();
         if (nqs > S) {
             std::vector<int32_t> col((size_t)S);

@@ -109,7 +109,8 @@ struct Ctx {
     float table_eps = 0.0f;
     std::vector<int32_t> h_ranks;
     // query table of exact output (built with the table): order statistics at
-    // the sample ranks and at dense ranks of every row's top (all of 0..255,
+    // the sample ranks and at dense ranks of every row's top (mode 2, the
+    // default: every rank up to the 4096-column cap; mode 1: all of 0..255,
     // every 8th to 1023, every 32nd to 4095).  Bounds from it settle most
     // high-ranking users without refinement; the sampled table is the subset
     // of its columns at qcol.  Absent (n_qs = 0) for installed tables.
@@ -118,7 +119,7 @@ struct Ctx {
     int32_t *qcol = nullptr;    // n_sample
     float *qtable = nullptr;    // [u_cap][n_qs]
     std::vector<int32_t> h_qranks;
-    int dense_top = 1;          // build the query table (RKR_OPT_DENSE_TOP)
+    int dense_top = 2;          // query table mode (RKR_OPT_DENSE_TOP)
 
     // screened form: hi/lo BF16 item rows (n_item x kp), norm bounds
     int32_t kp = 0;
@@ -206,7 +207,9 @@ void guard_columns(const Ctx *c, int32_t *col, int32_t &n);
 int refresh_ins_guard_max(Ctx *c);
 // Ranks of the query table: the sample ranks and the dense top ranks below
 // n_item, ascending; empty when they would exceed kMaxSample.
-std::vector<int32_t> query_table_ranks(const std::vector<int32_t> &sample, int64_t n_item);
+// mode 1: all of 0..255, every 8th to 1023, every 32nd to 4095; mode 2: every
+// rank below the largest D that keeps the table within kMaxSample columns.
+std::vector<int32_t> query_table_ranks(const std::vector<int32_t> &sample, int64_t n_item, int mode);
 
 // ---- error and memory helpers (rkr_api.cpp) --------------------------------
 int set_err(Ctx *c, int code, const std::string &msg);

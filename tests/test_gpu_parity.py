@@ -402,7 +402,7 @@ def test_dense_query_table(orc, form):
     ks = rng.integers(1, 120, 30).astype(np.int32)
     ranks = rkr.Index.uniform_sample_ranks(I, 40)
     out = {}
-    for dense in (0, 1):
+    for dense in (0, 1, 2):
         idx = rkr.Index(0)
         idx.set_option(8, dense)  # RKR_OPT_DENSE_TOP
         idx.set_form(form)
@@ -411,11 +411,15 @@ def test_dense_query_table(orc, form):
         res = idx.query(queries, ks, max_k=120)
         out[dense] = (idx.get_table(), res)
         idx.close()
-    assert np.array_equal(out[0][0], out[1][0])
-    assert np.array_equal(out[0][1].users, out[1][1].users) and np.array_equal(out[0][1].ranks, out[1][1].ranks)
-    refined = [sum(s["n_refine_user"] for s in out[x][1].stats) for x in (0, 1)]
-    assert refined[1] < refined[0]
-    _check_exact(orc, _normalized(orc, users), items, queries, out[1][1], ks)
+    refined = [sum(s["n_refine_user"] for s in out[x][1].stats) for x in (0, 1, 2)]
+    for dense in (1, 2):
+        assert np.array_equal(out[0][0], out[dense][0])
+        assert np.array_equal(out[0][1].users, out[dense][1].users)
+        assert np.array_equal(out[0][1].ranks, out[dense][1].ranks)
+        assert refined[dense] < refined[0]
+        _check_exact(orc, _normalized(orc, users), items, queries, out[dense][1], ks)
+    # every rank up to the cap (here all 3000 items) is a column: nothing is refined
+    assert refined[2] <= refined[1] and refined[2] == 0
 
 
 @pytest.mark.parametrize("dense", [0, 1])
