@@ -66,7 +66,7 @@ __global__ void k_prep_items(const float *__restrict__ items, int64_t n_item, in
         ss += (double)v * (double)v;
     }
     for (int o = 16; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 32);
-    if (threadIdx.x == 0) norm[p] = __double2float_ru(sqrt(ss));
+    if (threadIdx.x == 0 && norm) norm[p] = __double2float_ru(sqrt(ss));
 }
 
 // Upper bound of each user's L2 norm.
@@ -85,6 +85,7 @@ __global__ void k_user_norms(const float *__restrict__ users, int64_t n_user, in
 struct ScreenParams {
     const float *users;        // n_user x ld FP32
     const float *user_norm;    // n_user
+    const uint16_t *user_hi, *user_lo;  // n_user x Kp BF16 split
     const uint16_t *item_hi;   // n_item x Kp
     const uint16_t *item_lo;
     const float *item_norm;    // n_item
@@ -172,17 +173,13 @@ __global__ void __launch_bounds__(NT, RB == 2 ? 2 : 1) k_refine_screen(ScreenPar
 #pragma unroll
         for (int b = 0; b < RB; ++b) {
             const int64_t r = row_base + b * 16 + lr;
-            const float *urow = r < n_rows ? P.users + (int64_t)P.rows_user[r] * P.ld : nullptr;
+            // A fragments straight from the users' BF16 split (k = ks*32 + 8*lq .. +7)
+            const int64_t ur = r < n_rows ? (int64_t)P.rows_user[r] : -1;
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int k = ks * 32 + 8 * lq + i;
-                    const float v = (urow && k < P.K) ? urow[k] : 0.0f;
-                    const uint16_t h = bf16_bits(v);
-                    ah[b][ks].h[i] = h;
-                    al[b][ks].h[i] = bf16_bits(v - bf16_to_f32(h));
-                }
+                const uint4 z = make_uint4(0, 0, 0, 0);
+                ah[b][ks].q = ur >= 0 ? reinterpret_cast<const uint4 *>(P.user_hi + ur * P.Kp)[ks * 4 + lq] : z;
+                al[b][ks].q = ur >= 0 ? reinterpret_cast<const uint4 *>(P.user_lo + ur * P.Kp)[ks * 4 + lq] : z;
             }
         }
         // epilogue constants of the 4 rows this lane's accumulator covers:
@@ -333,6 +330,7 @@ __global__ void k_recheck(const float *__restrict__ users, const float *__restri
 struct IpCoarseParams {
     const float *users;           // n_user x ld
     const float *user_norm;       // n_user
+    const uint16_t *user_hi, *user_lo;  // n_user x Kp BF16 split
     const float *table;           // n_user x S
     int64_t n_user;
     int32_t S;
@@ -386,17 +384,12 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
         for (int b = 0; b < 2; ++b) {
             const int64_t r = row_base + b * 16 + lr;
             live[b] = r < P.n_user;
-            const float *urow = live[b] ? P.users + r * P.ld : nullptr;
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int k = ks * 32 + 8 * lq + i;
-                    const float v = (live[b] && k < P.K) ? urow[k] : 0.0f;
-                    const uint16_t h = bf16_bits(v);
-                    uh[b][ks].h[i] = h;
-                    ul[b][ks].h[i] = bf16_bits(v - bf16_to_f32(h));
-                }
+            for (int ks = 0; ks < KS; ++ks) {
+                const uint4 z = make_uint4(0, 0, 0, 0);
+                uh[b][ks].q = live[b] ? reinterpret_cast<const uint4 *>(P.user_hi + r * P.Kp)[ks * 4 + lq] : z;
+                ul[b][ks].q = live[b] ? reinterpret_cast<const uint4 *>(P.user_lo + r * P.Kp)[ks * 4 + lq] : z;
+            }
             if (!SETS) {
                 const float *trow = P.table + (live[b] ? r : 0) * (int64_t)P.S;
 #pragma unroll
@@ -501,8 +494,11 @@ int screen_max_kp() { return 256; }
 hipError_t launch_refine_screen(hipStream_t s, const ScreenLaunch &L) {
     if (L.max_rows <= 0 || L.n_item <= 0) return hipSuccess;
     ScreenParams P;
+    if (!L.user_hi || !L.user_lo) return hipErrorInvalidValue;
     P.users = L.users;
     P.user_norm = L.user_norm;
+    P.user_hi = L.user_hi;
+    P.user_lo = L.user_lo;
     P.item_hi = L.item_hi;
     P.item_lo = L.item_lo;
     P.item_norm = L.item_norm;
@@ -566,8 +562,10 @@ hipError_t launch_ip_coarse(hipStream_t s, const CoarseLaunch &L) {
     if (sets ? (L.n_sets < 1 || L.n_sets > kMaxGuardSets || !L.qset) : (L.gc.n < 1 || L.gc.n > 7))
         return hipErrorInvalidValue;
     if (L.ldc % 128 || L.ldc < L.n_user) return hipErrorInvalidValue;
-    IpCoarseParams P{L.users, L.user_norm, L.table, L.n_user, L.S, L.gc, L.q_hi, L.q_lo, L.q_norm, L.qmap, L.nq,
-                     L.ld, L.K, L.Kp, L.C, L.abs_floor, L.cb, L.ldc, L.spare_row, L.sets, L.qset, L.n_sets};
+    if (!L.user_hi || !L.user_lo) return hipErrorInvalidValue;
+    IpCoarseParams P{L.users, L.user_norm, L.user_hi, L.user_lo, L.table, L.n_user, L.S, L.gc, L.q_hi, L.q_lo,
+                     L.q_norm, L.qmap, L.nq, L.ld, L.K, L.Kp, L.C, L.abs_floor, L.cb, L.ldc, L.spare_row, L.sets,
+                     L.qset, L.n_sets};
     const int64_t n_rb = (L.n_user + 127) / 128;
     const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(n_rb, L.grid_cap)));
 #define RKR_COARSE_CASE(ks)                                                                                   \

@@ -88,6 +88,8 @@ DataSnap snap_of(const Ctx *c) {
     d.usersT = c->usersT;
     d.items = c->items;
     d.user_norm = c->user_norm;
+    d.user_hi = c->user_hi;
+    d.user_lo = c->user_lo;
     d.item_norm = c->item_norm;
     d.item_hi = c->item_hi;
     d.item_lo = c->item_lo;
@@ -230,6 +232,9 @@ void free_data(Ctx *c) {
     dev_free(c, c->item_lo, sizeof(uint16_t) * (size_t)c->n_item * c->kp);
     dev_free(c, c->item_norm, sizeof(float) * (size_t)c->n_item);
     dev_free(c, c->user_norm, sizeof(float) * (size_t)c->n_user);
+    dev_free(c, c->user_hi, 0);
+    dev_free(c, c->user_lo, 0);
+    c->user_hi = c->user_lo = nullptr;
     c->item_hi = c->item_lo = nullptr;
     c->item_norm = c->user_norm = nullptr;
     c->kp = 0;
@@ -517,12 +522,15 @@ int rkr_set_data(rkr_ctx *c, const float *users, int64_t n_user, const float *it
         if ((rc = dev_alloc_t(c, &c->item_hi, (size_t)n_item * kp, "item hi")) ||
             (rc = dev_alloc_t(c, &c->item_lo, (size_t)n_item * kp, "item lo")) ||
             (rc = dev_alloc_t(c, &c->item_norm, (size_t)n_item, "item norms")) ||
-            (rc = dev_alloc_t(c, &c->user_norm, (size_t)n_user, "user norms"))) {
+            (rc = dev_alloc_t(c, &c->user_norm, (size_t)n_user, "user norms")) ||
+            (rc = dev_alloc_t(c, &c->user_hi, (size_t)n_user * kp, "user hi")) ||
+            (rc = dev_alloc_t(c, &c->user_lo, (size_t)n_user * kp, "user lo"))) {
             free_data(c);
             return rc;
         }
         RKR_HIP(c, launch_prep_items(c->stream, c->items, n_item, ld, dim, kp, c->item_hi, c->item_lo, c->item_norm));
         RKR_HIP(c, launch_user_norms(c->stream, c->users, n_user, ld, dim, c->user_norm));
+        RKR_HIP(c, launch_prep_items(c->stream, c->users, n_user, ld, dim, kp, c->user_hi, c->user_lo, nullptr));
     }
     c->items_base = c->items;
     c->n_item_base = n_item;
@@ -913,6 +921,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             uint8_t *cb = reinterpret_cast<uint8_t *>(bk);
             CoarseLaunch L{};
             L.users = dv.users; L.user_norm = dv.user_norm; L.table = T; L.n_user = U; L.S = S; L.gc = gc;
+            L.user_hi = dv.user_hi; L.user_lo = dv.user_lo;
             L.q_hi = q_hi; L.q_lo = q_lo; L.q_norm = qn; L.qmap = nullptr; L.nq = nq; L.ld = ld; L.K = d;
             L.Kp = c->kp; L.C = c->screen_C; L.abs_floor = std::nextafter((float)absf, INFINITY); L.cb = cb;
             L.ldc = ldc; L.spare_row = nqb; L.grid_cap = (int64_t)c->n_cu * 3;
@@ -1140,6 +1149,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             if (screened) {
                 ScreenLaunch L{};
                 L.users = dv.users; L.user_norm = dv.user_norm;
+                L.user_hi = dv.user_hi; L.user_lo = dv.user_lo;
                 L.item_hi = dv.item_hi; L.item_lo = dv.item_lo; L.item_norm = dv.item_norm;
                 L.n_item = I; L.ld = ld; L.K = d; L.Kp = c->kp;
                 L.rows_user = refine_user; L.rows_thr = refine_thr; L.n_rows_dev = refine_total;

@@ -125,6 +125,7 @@ struct Ctx {
     int32_t kp = 0;
     uint16_t *item_hi = nullptr, *item_lo = nullptr;
     float *item_norm = nullptr, *user_norm = nullptr;
+    uint16_t *user_hi = nullptr, *user_lo = nullptr;  // BF16 hi/lo user rows ([u_cap] x kp)
     double max_abs_item = 0.0, max_abs_user_raw = 0.0;
     float screen_C = 0.0f, screen_abs = 0.0f;
     int n_cu = 256;
@@ -186,6 +187,7 @@ struct DataSnap {
     int64_t n_user = 0, n_item = 0, n_item_base = 0;
     const float *users = nullptr, *usersT = nullptr, *items = nullptr;
     const float *user_norm = nullptr, *item_norm = nullptr;
+    const uint16_t *user_hi = nullptr, *user_lo = nullptr;
     const uint16_t *item_hi = nullptr, *item_lo = nullptr;
     const float *table = nullptr, *side_ins = nullptr, *side_del = nullptr;
     int32_t n_ins = 0, n_del = 0, ld_ins = 0, ld_del = 0;
@@ -261,6 +263,7 @@ hipError_t launch_make_panels(hipStream_t s, const float *users, int64_t n_user,
 // ---- screened refinement (k_screen.hip) -------------------------------------
 struct ScreenLaunch {
     const float *users, *user_norm;
+    const uint16_t *user_hi, *user_lo;  // BF16 split of the user rows (n_user x Kp)
     const uint16_t *item_hi, *item_lo;
     const float *item_norm;
     int64_t n_item;
@@ -279,6 +282,8 @@ struct ScreenLaunch {
     int32_t chunk_items;    // most items per work-unit chunk (L2-resident slice)
     int32_t rows_per_wave;  // 32 or 64 refine rows per wavefront
 };
+// BF16 hi/lo rows (Kp columns, zero padded) and, when norm is non-null, row
+// norm upper bounds; used for items, queries and users
 hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, int32_t ld, int32_t K, int32_t Kp,
                              uint16_t *hi, uint16_t *lo, float *norm);
 hipError_t launch_user_norms(hipStream_t s, const float *users, int64_t n_user, int32_t ld, int32_t K, float *norm);
@@ -293,6 +298,7 @@ struct GuardCols {
 // row(b) = qmap[b] (or b when qmap is null), which also picks the query rows.
 struct CoarseLaunch {
     const float *users, *user_norm, *table;
+    const uint16_t *user_hi, *user_lo;  // BF16 split of the user rows (n_user x Kp)
     int64_t n_user;
     int32_t S;
     GuardCols gc;

@@ -89,6 +89,20 @@ int reserve_users(Ctx *c, int64_t need) {
         for (float *p : {users, panels, unorm, table, qtable, si, sd}) dev_free(c, p, 0);
         return rc;
     }
+    uint16_t *uhi = nullptr, *ulo = nullptr;
+    if (c->kp && ((rc = dev_alloc_t(c, &uhi, (size_t)cap * c->kp, "user hi")) ||
+                  (rc = dev_alloc_t(c, &ulo, (size_t)cap * c->kp, "user lo")))) {
+        for (float *p : {users, panels, unorm, table, qtable, si, sd}) dev_free(c, p, 0);
+        dev_free(c, uhi, 0);
+        dev_free(c, ulo, 0);
+        return rc;
+    }
+    if (c->kp && U > 0) {
+        RKR_HIP(c, hipMemcpyAsync(uhi, c->user_hi, sizeof(uint16_t) * (size_t)U * c->kp, hipMemcpyDeviceToDevice,
+                                  c->stream));
+        RKR_HIP(c, hipMemcpyAsync(ulo, c->user_lo, sizeof(uint16_t) * (size_t)U * c->kp, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    }
     if ((rc = copy_rows(c, users, ld, c->users, ld, U, ld)) ||
         (rc = copy_rows(c, panels, 64 * ld, c->usersT, 64 * ld, (U + 63) / 64, 64 * ld)) ||
         (c->kp && (rc = copy_rows(c, unorm, 1, c->user_norm, 1, U, 1))) ||
@@ -101,6 +115,10 @@ int reserve_users(Ctx *c, int64_t need) {
     for (float *p : {c->users, c->usersT, c->user_norm, c->table, c->qtable, c->side_ins, c->side_del,
                      c->side_ins_spare, c->side_del_spare})
         dev_free(c, p, 0);
+    dev_free(c, c->user_hi, 0);
+    dev_free(c, c->user_lo, 0);
+    c->user_hi = uhi;
+    c->user_lo = ulo;
     c->users = users;
     c->usersT = panels;
     c->user_norm = unorm;
@@ -469,7 +487,11 @@ int rkr_insert_users(rkr_ctx *c, const float *users, int64_t n) {
     if ((rc = upload_rows(c, h.data(), n, c->users + U0 * ld))) return rc;
     const int64_t p0 = U0 / 64;  // first panel that changes (lanes of earlier users are rewritten unchanged)
     RKR_HIP(c, launch_make_panels(c->stream, c->users + p0 * 64 * ld, U1 - p0 * 64, ld, c->usersT + p0 * 64 * ld));
-    if (c->kp) RKR_HIP(c, launch_user_norms(c->stream, c->users + U0 * ld, n, ld, c->dim, c->user_norm + U0));
+    if (c->kp) {
+        RKR_HIP(c, launch_user_norms(c->stream, c->users + U0 * ld, n, ld, c->dim, c->user_norm + U0));
+        RKR_HIP(c, launch_prep_items(c->stream, c->users + U0 * ld, n, ld, c->dim, c->kp, c->user_hi + U0 * c->kp,
+                                     c->user_lo + U0 * c->kp, nullptr));
+    }
     // table rows of the new users against the base items
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)((1ull << 31) / (sizeof(float) * c->n_item_base))));
     if ((rc = ensure(c, c->upd_scratch, sizeof(float) * (size_t)chunk * c->n_item_base, "score block"))) return rc;
@@ -544,20 +566,30 @@ int rkr_delete_users(rkr_ctx *c, const int64_t *ids, int64_t n) {
     c->h_users.swap(hu);
     c->n_user = nk;
     float *panels = nullptr, *unorm = nullptr;
+    uint16_t *uhi = nullptr, *ulo = nullptr;
     if ((rc = dev_alloc_t(c, &panels, (size_t)((c->u_cap + 63) / 64) * 64 * ld, "user panels"))) return rc;
     RKR_HIP(c, launch_make_panels(c->stream, c->users, c->n_user, ld, panels));
     if (c->kp) {
-        if ((rc = dev_alloc_t(c, &unorm, (size_t)c->u_cap, "user norms"))) {
+        if ((rc = dev_alloc_t(c, &unorm, (size_t)c->u_cap, "user norms")) ||
+            (rc = dev_alloc_t(c, &uhi, (size_t)c->u_cap * c->kp, "user hi")) ||
+            (rc = dev_alloc_t(c, &ulo, (size_t)c->u_cap * c->kp, "user lo"))) {
             dev_free(c, panels, 0);
+            dev_free(c, unorm, 0);
+            dev_free(c, uhi, 0);
             return rc;
         }
         RKR_HIP(c, launch_user_norms(c->stream, c->users, c->n_user, ld, c->dim, unorm));
+        RKR_HIP(c, launch_prep_items(c->stream, c->users, c->n_user, ld, c->dim, c->kp, uhi, ulo, nullptr));
     }
     RKR_HIP(c, hipStreamSynchronize(c->stream));
     dev_free(c, c->usersT, 0);
     dev_free(c, c->user_norm, 0);
+    dev_free(c, c->user_hi, 0);
+    dev_free(c, c->user_lo, 0);
     c->usersT = panels;
     c->user_norm = unorm;
+    c->user_hi = uhi;
+    c->user_lo = ulo;
     return RKR_OK;
 }
 
