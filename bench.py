@@ -75,8 +75,9 @@ def parse():
     ap.add_argument("--update-mode", default="snapshot", choices=["snapshot", "barrier"],
                     help="serving: updates prepared beside running batches, or exclusive barriers")
     ap.add_argument("--sat-seconds", type=float, default=3.0, help="serving: closed-loop saturation run length")
-    ap.add_argument("--max-batch", type=int, default=32, help="serving: queries per device batch")
+    ap.add_argument("--max-batch", type=int, default=64, help="serving: queries per device batch")
     ap.add_argument("--slots", type=int, default=2, help="serving: concurrent batches")
+    ap.add_argument("--no-inserts", action="store_true", help="serving: no insert stream (diagnostic)")
     ap.add_argument("--update-cus", type=int, default=0,
                     help="serving, snapshot mode: CUs for update kernels (0 = a quarter, -1 = all)")
     return ap.parse_args()
@@ -86,7 +87,8 @@ def serving(args, idx, wl, label):
     """Serving form: Poisson query arrivals (k uniform in [10, 200]) while an
     insert batch of 1% new users and 0.5% new items arrives every second.
     The saturating rate is measured first, closed-loop, under the same insert
-    stream; the open loop then runs at half of it (or --qps).  Latency is
+    stream once its first round has committed; the open loop then runs at half
+    of it (or --qps).  Latency is
     submit -> answer on the library's monotonic clock."""
     import threading
     import rkr
@@ -116,24 +118,66 @@ def serving(args, idx, wl, label):
             upd.append(srv.submit_update("insert_items", vectors=ni))
             t_next += 1.0
 
-    uth = threading.Thread(target=updater)
+    # lean client calls for the query stream: a ctypes call per request with
+    # reused buffers (the Python wrapper's per-call arrays and dicts would
+    # saturate the interpreter lock at tens of thousands of requests/s and
+    # submit in bursts)
+    import ctypes as C
+    from rkr import _lib as L
+    lib, h = srv.lib, srv.h
+    qp = np.ascontiguousarray(qpool, np.float32)
+    q_base, q_stride = qp.ctypes.data, qp.shape[1] * 4
+    t_box = C.c_uint64()
+    t_ref = C.byref(t_box)
+    obuf_u = np.empty(256, np.int32)
+    obuf_r = np.empty(256, np.int32)
+    obuf_q = np.empty(256, np.float32)
+    pu, pr, pq = obuf_u.ctypes.data, obuf_r.ctypes.data, obuf_q.ctypes.data
+    info = L.RequestInfo()
+    info_ref = C.byref(info)
+
+    def submit(row, k):
+        rc = lib.rkr_server_submit_query(h, q_base + int(row) * q_stride, int(k), t_ref)
+        if rc != L.RKR_OK:
+            raise L.RkrError(rc, "submit_query")
+        return t_box.value
+
+    def wait(ticket):
+        rc = lib.rkr_server_wait(h, ticket, pu, pr, pq, info_ref)
+        if rc != L.RKR_OK:
+            raise L.RkrError(rc, info.error.decode(errors="replace"))
+        return info.t_submit, info.t_start, info.t_done, info.batch_size
+
+    uth = threading.Thread(target=updater if not args.no_inserts else (lambda: None))
     uth.start()
+    if not args.no_inserts:
+        # measure in the steady state of the insert stream: wait until the
+        # first insert round has committed (queries see pending inserts)
+        while len(upd) < 2:
+            time.sleep(0.01)
+        done_flag = C.c_int(0)
+        while not done_flag.value:  # poll: the ticket is waited on (once) at the end
+            lib.rkr_server_poll(h, upd[1], C.byref(done_flag))
+            time.sleep(0.005)
     # saturating rate: closed loop with 2 x max_batch queries outstanding
     window = 512
-    pending = []
     t0 = time.perf_counter()
     n_sat = 0
+    sat_rows = rng.integers(0, I0, 1 << 20)
+    sat_ks = rng.integers(10, 201, 1 << 20)
+    j = 0
+    from collections import deque
+    pending = deque()
     while True:
         while len(pending) < window:
-            k = int(rng.integers(10, 201))
-            pending.append((srv.submit_query(qpool[rng.integers(0, I0)], k), k))
-        t, k = pending.pop(0)
-        srv.wait(t, k)
+            pending.append(submit(sat_rows[j % len(sat_rows)], sat_ks[j % len(sat_ks)]))
+            j += 1
+        wait(pending.popleft())
         n_sat += 1
         if time.perf_counter() - t0 >= args.sat_seconds:
             break
-    for t, k in pending:
-        srv.wait(t, k)
+    for t in pending:
+        wait(t)
         n_sat += 1
     sat = n_sat / (time.perf_counter() - t0)
     rate = args.qps if args.qps > 0 else sat / 2
@@ -145,29 +189,38 @@ def serving(args, idx, wl, label):
     tickets = []
     start = time.perf_counter()
 
+    # the submitter spins on the clock (sleep overshoots by about a
+    # millisecond and would submit in bursts); a short switch interval keeps
+    # the waiting thread responsive
+    lateness = []
+
     def submitter():
         for j in range(n_arr):
-            while True:
-                el = time.perf_counter() - start
-                if el >= arrivals[j]:
-                    break
-                time.sleep(min(arrivals[j] - el, 0.0005))
-            tickets.append((srv.submit_query(qpool[qi[j]], int(kq[j])), int(kq[j])))
+            while time.perf_counter() - start < arrivals[j]:
+                pass
+            lateness.append(time.perf_counter() - start - arrivals[j])
+            tickets.append(submit(qi[j], kq[j]))
+
+    switch = sys.getswitchinterval()
+    sys.setswitchinterval(1e-4)
 
     th = threading.Thread(target=submitter)
     th.start()
     lat, bsz, done = [], [], 0
+    t_sub, svc = [], []
     while done < n_arr:
         if done < len(tickets):
-            t, k = tickets[done]
-            _, _, _, info = srv.wait(t, k)
-            lat.append((info["t_done"] - info["t_submit"]) * 1e3)
-            bsz.append(info["batch_size"])
+            ts_, tst, tdn, bs = wait(tickets[done])
+            lat.append((tdn - ts_) * 1e3)
+            t_sub.append(ts_)
+            svc.append((tdn - tst) * 1e3)
+            bsz.append(bs)
             done += 1
         else:
             time.sleep(0.0002)
     th.join()
     wall = time.perf_counter() - start
+    sys.setswitchinterval(switch)
     stop_upd.set()
     uth.join()
     ulat, ustall = [], []
@@ -177,6 +230,10 @@ def serving(args, idx, wl, label):
     cnt = srv.counters()
     srv.stop()
     lat = np.array(lat)
+    # p99 per second of the open loop (warm-up and stalls show up here)
+    ts = np.array(t_sub) - t_sub[0] if t_sub else np.zeros(0)
+    per_s = [round(float(np.percentile(lat[(ts >= s0) & (ts < s0 + 1)], 99)), 2)
+             for s0 in range(int(np.ceil(ts.max())) if len(ts) else 0) if ((ts >= s0) & (ts < s0 + 1)).any()]
     out = {
         "metric": "serving queries/s with p50/p99 latency (exact ranks), inserts every 1 s",
         "value": round(len(lat) / wall, 2), "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
@@ -187,7 +244,12 @@ def serving(args, idx, wl, label):
                    "saturation": f"closed loop, {window} outstanding, {args.sat_seconds:g} s, same insert stream",
                    "duration_s": args.duration},
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-        "p999_ms": round(float(np.percentile(lat, 99.9)), 3),
+        "p999_ms": round(float(np.percentile(lat, 99.9)), 3), "p99_ms_per_second": per_s,
+        "submit_lateness_ms": {"p50": round(float(np.percentile(lateness, 50)) * 1e3, 3),
+                               "p99": round(float(np.percentile(lateness, 99)) * 1e3, 3)} if lateness else None,
+        "batch_service_ms": {"p50": round(float(np.percentile(svc, 50)), 3),
+                             "p99": round(float(np.percentile(svc, 99)), 3),
+                             "max": round(float(np.max(svc)), 3)} if svc else None,
         "mean_batch": round(float(np.mean(bsz)), 1), "queries": len(lat), "batches": cnt["n_batches"],
         "update_batches": len(ulat), "update_ms_p50": round(float(np.percentile(ulat, 50)), 2) if ulat else None,
         "update_ms_max": round(float(np.max(ulat)), 2) if ulat else None,

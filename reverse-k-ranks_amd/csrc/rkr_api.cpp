@@ -908,6 +908,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
         uint32_t *bk_c = nullptr;
         float *s_c = nullptr;
         int64_t n_listed = 0;
+        std::vector<int32_t> deep_check;  // update path: queries listed without the deepest bin
         if (coarse_b) {
             // coarse buckets on the matrix cores, the coarse k-th bound and
             // list sizes; exact scores and bucket pairs of the listed users
@@ -964,8 +965,11 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                     int32_t bl = 0;
                     while (bl < gc.n && (int64_t)hrk[gc.col[bl]] + 1 - dv.n_del <= V) ++bl;
                     if (bl >= gc.n) {
-                        e[0] = -1;  // every user could be a candidate: full path
-                        continue;
+                        // users below the deepest guard are left out; the
+                        // exact k-th value then has to stay below their
+                        // lowest possible rank (checked after the selection)
+                        bl = gc.n - 1;
+                        deep_check.push_back(i);
                     }
                     int64_t n = 0;
                     for (int32_t cc = 0; cc <= bl; ++cc) n += hhc[(size_t)i * 8 + cc];
@@ -1097,6 +1101,17 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
         RKR_HIP(c, launch_offsets(sl.stream, qi, nq, refine_total));
         RKR_HIP(c, hipMemcpyAsync(hqi, qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
         RKR_HIP(c, hipStreamSynchronize(sl.stream));
+        if (!deep_check.empty()) {
+            // a user below the deepest guard has rank >= r[j_last] + 1 - n_del:
+            // the selection over the listed users is exact when the k-th value
+            // L (QueryInfo.gamma of the value selection) is below that
+            const int64_t floor_rank = (int64_t)hrk[gc.col[gc.n - 1]] + 1 - dv.n_del;
+            for (int32_t i : deep_check)
+                if ((int64_t)hqi[i].gamma >= floor_rank) {
+                    if (fallback) fallback->push_back((int32_t)(q0 + i));
+                    tm.n_fallback += 1;
+                }
+        }
         const int64_t total = hqi[nq - 1].cand_off + hqi[nq - 1].n_settled + hqi[nq - 1].n_cand;
         int64_t total_cand = 0;
         for (int32_t i = 0; i < nq; ++i) total_cand += hqi[i].n_cand;

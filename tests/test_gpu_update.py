@@ -256,3 +256,17 @@ def test_coarse_bounds_with_updates(orc, gpu_index_factory, deletes):
         assert np.array_equal(full.qips[i, :k], coarse.qips[i, :k])
         for f in ("n_candidate", "n_refine_user", "n_prune_user", "n_result_user"):
             assert full.stats[i][f] == coarse.stats[i][f], f"query {i}: {f}"
+    # weak queries (negated items) with large k put the k-th bound at the
+    # deepest guard: users below it are left out of the list and the k-th
+    # value is checked against their lowest possible rank afterwards
+    weak = -cur[rng.choice(len(cur), 24, replace=False)]
+    kw = [int(k) for k in rng.integers(200, 1500, len(weak))]
+    idx.set_option(9, 0)
+    full = idx.query(weak, kw)
+    idx.set_option(9, 1)
+    coarse = idx.query(weak, kw)
+    bu, br, _ = orc.bruteforce(m.users, cur, weak, max(kw))
+    for i, k in enumerate(kw):
+        for res in (full, coarse):
+            assert np.array_equal(res.users[i, :k], bu[i, :k]), f"weak query {i}: users differ"
+            assert np.array_equal(res.ranks[i, :k], br[i, :k]), f"weak query {i}: ranks differ"
