@@ -189,6 +189,46 @@ __global__ void __launch_bounds__(NT) k_hist_kth(const uint32_t *__restrict__ gh
         h_hi[i] = same ? lo : g[S + 1 + i];
     }
     __syncthreads();
+    if (output_mode == RKR_OUTPUT_EXACT) {
+        // beta = first bin where the cumulative h_lo reaches k, and the
+        // candidate count sum_{j <= beta} h_hi[j]: block scans over runs of
+        // ceil((S + 1) / NT) bins per thread (counts fit in 32 bits)
+        __shared__ uint32_t red[NW + 1];
+        __shared__ int32_t s_beta;
+        const uint32_t k = (uint32_t)qi[b].topk;
+        const int32_t per = (S + 1 + NT - 1) / NT;
+        const int32_t j0 = threadIdx.x * per, j1 = min(S + 1, j0 + per);
+        uint32_t mine = 0;
+        for (int32_t j = j0; j < j1; ++j) mine += h_lo[j];
+        uint32_t tot;
+        uint32_t before = wg_excl_scan(mine, red, &tot);
+        if (threadIdx.x == 0) s_beta = S;
+        __syncthreads();
+        if (before < k && before + mine >= k) {
+            uint32_t cum = before;
+            for (int32_t j = j0; j < j1; ++j) {
+                cum += h_lo[j];
+                if (cum >= k) { s_beta = j; break; }
+            }
+        }
+        __syncthreads();
+        const int32_t beta = s_beta;
+        uint32_t nh = 0;
+        for (int32_t j = j0; j < min(j1, beta + 1); ++j) nh += h_hi[j];
+        (void)wg_excl_scan(nh, red, &tot);
+        if (threadIdx.x == 0) {
+            QueryInfo q = qi[b];
+            q.beta = beta;
+            q.n_refine = 0;
+            q.fill = 0;
+            q.mode = MODE_EXACT;
+            q.gamma = beta;
+            q.n_cand = (int32_t)tot;
+            q.n_settled = 0;
+            qi[b] = q;
+        }
+        return;
+    }
     if (threadIdx.x != 0) return;
     QueryInfo q = qi[b];
     const int64_t k = q.topk;
@@ -201,14 +241,7 @@ __global__ void __launch_bounds__(NT) k_hist_kth(const uint32_t *__restrict__ gh
     q.beta = beta;
     q.n_refine = 0;
     q.fill = 0;
-    if (output_mode == RKR_OUTPUT_EXACT) {
-        int64_t nc = 0;
-        for (int32_t j = 0; j <= beta; ++j) nc += h_hi[j];
-        q.mode = MODE_EXACT;
-        q.gamma = beta;
-        q.n_cand = (int32_t)nc;
-        q.n_settled = 0;
-    } else {
+    {
         const int32_t Lk = ref_lb(beta, S, ranks, n_item), Uk = ref_ub(beta, S, ranks, n_item);
         int64_t n_res = 0, n_ref = 0;
         for (int32_t j = 0; j <= S; ++j) {

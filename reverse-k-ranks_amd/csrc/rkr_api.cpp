@@ -887,9 +887,15 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
         int32_t *o_rank = o_user + (size_t)nqb * max_k;
         float *o_qip = reinterpret_cast<float *>(o_rank + (size_t)nqb * max_k);
 
+        float q_amax = 0.0f;  // largest |query value| of the batch (validated finite)
         for (int32_t i = 0; i < nq; ++i) {
-            std::memcpy(&hq[(size_t)i * ld], queries + (size_t)(q0 + i) * d, sizeof(float) * d);
-            for (int32_t k = d; k < ld; ++k) hq[(size_t)i * ld + k] = 0.0f;
+            const float *src = queries + (size_t)(q0 + i) * d;
+            float *dst = &hq[(size_t)i * ld];
+            for (int32_t k = 0; k < d; ++k) {
+                dst[k] = src[k];
+                q_amax = std::max(q_amax, std::fabs(src[k]));
+            }
+            for (int32_t k = d; k < ld; ++k) dst[k] = 0.0f;
             hqi[i] = QueryInfo{};
             hqi[i].topk = topk[q0 + i];
         }
@@ -916,7 +922,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             uint16_t *q_lo = q_hi + (size_t)nqb * c->kp;
             float *qn = reinterpret_cast<float *>(q_lo + (size_t)nqb * c->kp);
             RKR_HIP(c, launch_prep_items(sl.stream, q_in, nq, ld, d, c->kp, q_hi, q_lo, qn));
-            const double mq = max_abs(hq, (size_t)nq * ld);
+            const double mq = q_amax;
             const double absf = ((double)screen_abs_floor(c->kp, dv.max_abs_user, mq) + (double)eps) *
                                 (1.0 + std::ldexp(1.0, -18));
             uint8_t *cb = reinterpret_cast<uint8_t *>(bk);
