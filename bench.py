@@ -281,8 +281,8 @@ def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_sample_q):
 # plain 1 GiB device copy gives, bench.py --copy-calibration).  Keyed by
 # (workload, form); other configurations report null.
 PROFILED_TRAFFIC = {
-    ("plain_gpu", "screened"): {"bytes": 3.564e8, "fetch_kib": 168269.6, "write_kib": 11534.1,
-                                "profile": "profiles/k_refine_screen_100k_x_20k_d150_k50.md"},
+    ("plain_gpu", "screened"): {"kernel": "k_ip_coarse", "bytes": 3.449e8, "fetch_kib": 119473.2,
+                                "write_kib": 97848.5, "profile": "profiles/k_ip_coarse_100k_x_20k_d150_k50.md"},
 }
 
 
@@ -431,7 +431,7 @@ def main():
                 "work_per_launch": "U*(ceil(log2(S+1))*4 + 8) bytes per query"}
 
     prof = PROFILED_TRAFFIC.get((args.workload, args.form))
-    if prof and roof.get("kernel") == "k_refine_screen":
+    if prof and roof.get("kernel") == prof["kernel"]:
         roof["traffic"] = prof["bytes"]
         roof["traffic_source"] = prof["profile"]
 
