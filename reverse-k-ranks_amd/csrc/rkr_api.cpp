@@ -790,7 +790,10 @@ int validate_queries(Ctx *c, const DataSnap &d, const float *queries, int32_t n_
 
 namespace {
 
-constexpr int32_t kCoarseRefine = 4096;
+// a coarse list longer than max(kCoarseRefineMin, U / 64) gets a second pass:
+// the pass costs about a user row per query, an exact-scored entry about a
+// user row per entry, and the pass narrows a list several times over
+constexpr int64_t kCoarseRefineMin = 1024;
 constexpr int64_t kCoarseChunk = 16384;
 constexpr int32_t kCoarseMinBatch = 8;
 
@@ -846,9 +849,10 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
     }
     // ranks of the searched table's columns (guard placement)
     const std::vector<int32_t> &hrk = use_q ? c->h_qranks : c->h_ranks;
-    // coarse lists longer than kCoarseRefine get a second, finer pass; a
-    // list longer than coarse_cap takes the full path instead
+    // coarse lists longer than refine_min get a second, finer pass; a list
+    // longer than coarse_cap takes the full path instead
     const int64_t coarse_cap = std::max<int64_t>(4096, U / 4);
+    const int64_t refine_min = std::max<int64_t>(kCoarseRefineMin, U / 64);
 
     // pinned staging: queries | query info | outputs (the device layout of
     // sl.out) | coarse rows, bin counts, guard sets, maps | recheck counts,
@@ -998,7 +1002,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                 if (hi_col <= lo_col) continue;
                 const size_t first = hqmap.size();
                 for (int32_t i = 0; i < nq; ++i)
-                    if (hcq[(size_t)i * 4] == beta && hcq[(size_t)i * 4 + 1] > kCoarseRefine) hqmap.push_back(i);
+                    if (hcq[(size_t)i * 4] == beta && hcq[(size_t)i * 4 + 1] > refine_min) hqmap.push_back(i);
                 if (hqmap.size() == first) continue;
                 GuardCols g2{};
                 const double ra = (double)hrk[lo_col] + 1.0, rb = (double)hrk[hi_col] + 1.0;
