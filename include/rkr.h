@@ -167,6 +167,22 @@ int rkr_get_ranks(const rkr_ctx *ctx, int32_t *out_ranks, int32_t *out_n_sample)
 int rkr_save_table(rkr_ctx *ctx, const char *path);
 int rkr_load_table(rkr_ctx *ctx, const char *path, float eps);
 
+/* ---- full score table ------------------------------------------------------ */
+
+/* Scores of users [user_begin, user_begin + n_rows) against every current item,
+ * each row sorted in descending order, into out[n_rows][n_item] (host):
+ * ComputeScoreTableBatch::ComputeSortItemsBatch
+ * (src/include/score_computation/ComputeScoreTableBatch.hpp:79-92).  Scores
+ * follow the same rule as the table build. */
+int rkr_sorted_scores(rkr_ctx *ctx, int64_t user_begin, int64_t n_rows, float *out);
+
+/* Writes the sorted score rows of the first n_rows users (n_rows < 0: every
+ * user) to path in the layout of ReadAll's index file
+ * (src/include/alg/DiskIndex/ReadAll.hpp:80-87, as
+ * src/app/BuildScoreTableBatch.cpp:69-144 writes it): float[n_rows][n_item],
+ * row-major, no header.  elapsed_ms (optional) receives the wall time. */
+int rkr_save_score_table(rkr_ctx *ctx, const char *path, int64_t n_rows, double *elapsed_ms);
+
 /* ---- queries --------------------------------------------------------------- */
 
 /* Answers n_query reverse k-ranks queries (queries: n_query x dim, row-major

@@ -74,6 +74,17 @@ class Index {
     }
     void LoadTable(const std::string &path, float eps = 0.0f) { check(rkr_load_table(ctx_.get(), path.c_str(), eps)); }
     void SaveTable(const std::string &path) { check(rkr_save_table(ctx_.get(), path.c_str())); }
+    // ComputeScoreTableBatch::ComputeSortItemsBatch (ComputeScoreTableBatch.hpp:79-92): batch_n_user rows of
+    // n_item scores, each sorted in descending order, into distance_l.
+    void ComputeSortItemsBatch(int start_userID, int batch_n_user, float *distance_l) {
+        check(rkr_sorted_scores(ctx_.get(), start_userID, batch_n_user, distance_l));
+    }
+    // ReadAll's index file for the first n_user users (all when negative); returns the wall time in ms.
+    double SaveScoreTable(const std::string &path, int64_t n_user = -1) {
+        double ms = 0;
+        check(rkr_save_score_table(ctx_.get(), path.c_str(), n_user, &ms));
+        return ms;
+    }
 
     // BaseIndex::Retrieval (MethodBase.hpp:16-18) / US::Index::Retrieval (US.hpp:96-213).
     std::vector<std::vector<UserRankElement>> Retrieval(const VectorMatrix &query_item, const int &topk,

@@ -129,6 +129,14 @@ def score_rows(users, items, u0=0, nu=None) -> np.ndarray:
     return out
 
 
+def sorted_score_rows(users, items, u0=0, nu=None) -> np.ndarray:
+    """Each user's scores against every item, sorted in descending order: the
+    CPU branch of ComputeScoreTableBatch::ComputeSortItemsBatch
+    (src/include/score_computation/ComputeScoreTableBatch.hpp:79-92), the rows
+    ReadAll::BuildIndexLoop writes (src/include/alg/DiskIndex/ReadAll.hpp:80-87)."""
+    return -np.sort(-score_rows(users, items, u0, nu), axis=1)
+
+
 def build_table(users, items, ranks) -> np.ndarray:
     users, items = _c32(users), _c32(items)
     ranks = np.ascontiguousarray(ranks, np.int32)

@@ -7,6 +7,10 @@
 //       BuildSampleIndexByComputation (src/app/build-index/BuildSampleIndexByComputation.cpp:66-214):
 //       builds the sampled score table on the GPU and writes
 //       X/memory_index/US-N-n_sample_S.index.
+//   rkr_cli bst   --dataset_dir D --dataset_name N --index_dir X [--eval_size_gb G]
+//       BuildScoreTableBatch (src/app/BuildScoreTableBatch.cpp:56-144, 159-169): writes every user's
+//       sorted scores (ReadAll's layout, float[n_user][n_item]) to X/N.index; with eval_size_gb below
+//       the table's size, only the first eval_size_gb GiB of rows.
 //   rkr_cli rri   --dataset_dir D --dataset_name N --index_dir X --n_sample S --topk K
 //                 [--form plain|screened] [--output exact|reference] [--result_dir R]
 //       ReverseRankIndex with method US (src/app/ReverseRankIndex.cpp:83-219):
@@ -16,6 +20,7 @@
 // (src/include/util/VectorIO.hpp:40-68); users are normalised on load.
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -156,6 +161,31 @@ int cmd_bsibc(const Args &a) {
     return 0;
 }
 
+int cmd_bst(const Args &a) {
+    rkr::VectorMatrix user, item, query;
+    if (!read_data(a, user, item, query)) return 1;
+    // ComputeEvalUserSize (BuildScoreTableBatch.cpp:56-64): the rows that fit eval_size_gb GiB
+    const uint64_t eval_gb = std::stoull(get(a, "eval_size_gb", "500"));
+    const uint64_t row_bytes = (uint64_t)item.n_vector_ * sizeof(float);
+    const uint64_t total_gb = (uint64_t)user.n_vector_ * row_bytes / 1024 / 1024 / 1024;
+    const int64_t n_eval = eval_gb > total_gb ? user.n_vector_
+                                              : (int64_t)std::min<uint64_t>(eval_gb * 1024 * 1024 * 1024 / row_bytes,
+                                                                            (uint64_t)user.n_vector_);
+    try {
+        rkr::Index idx(std::stoi(get(a, "device", "0")));
+        idx.SetData(user, item, false);
+        mkdirs(get(a, "index_dir", "index"));
+        const std::string path = get(a, "index_dir", "index") + "/" + get(a, "dataset_name", "fake-normal") + ".index";
+        const double ms = idx.SaveScoreTable(path, n_eval);
+        std::printf("build index time: total %.3fs, %lld of %d users, wrote %s\n", ms * 1e-3, (long long)n_eval,
+                    user.n_vector_, path.c_str());
+    } catch (const rkr::Error &e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}
+
 int cmd_rri(const Args &a) {
     rkr::VectorMatrix user, item, query;
     if (!read_data(a, user, item, query)) return 1;
@@ -210,7 +240,7 @@ int cmd_rri(const Args &a) {
 
 int main(int argc, char **argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: rkr_cli fsr|bsibc|rri --dataset_dir D --dataset_name N --index_dir X "
+        std::fprintf(stderr, "usage: rkr_cli fsr|bsibc|bst|rri --dataset_dir D --dataset_name N --index_dir X "
                              "--n_sample S [--topk K] [--form plain|screened] [--output exact|reference]\n");
         return 2;
     }
@@ -222,6 +252,7 @@ int main(int argc, char **argv) {
     }
     if (cmd == "fsr") return cmd_fsr(a);
     if (cmd == "bsibc") return cmd_bsibc(a);
+    if (cmd == "bst") return cmd_bst(a);
     if (cmd == "rri") return cmd_rri(a);
     std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
     return 2;

@@ -650,6 +650,92 @@ int rkr_get_table(rkr_ctx *c, int64_t user_begin, int64_t n_rows, float *out) {
     return RKR_OK;
 }
 
+extern "C++" {
+namespace {
+// Device rows freed when the scope ends.
+struct ScopedRows {
+    rkr::Ctx *c;
+    float *p = nullptr;
+    size_t bytes = 0;
+    ~ScopedRows() { rkr::dev_free(c, p, bytes); }
+};
+}  // namespace
+
+// Calls sink(rows, user, m) for users [u0, u0 + n) in batches of m rows: rows
+// (device memory, stride n_item) holds each user's scores against the current
+// items, sorted in descending order.  The block size follows the memory budget
+// as the table build's does.
+template <class Sink>
+static int sorted_score_batches(rkr_ctx *c, int64_t u0, int64_t n, const char *what, Sink &&sink) {
+    const int64_t I = c->n_item;
+    const size_t row_bytes = sizeof(float) * (size_t)I;
+    const size_t avail = c->budget > c->held ? c->budget - c->held : 0;
+    int64_t nb = (int64_t)std::min<size_t>(avail / 3 / row_bytes, (size_t)(2ull << 30) / row_bytes);
+    if (c->build_batch > 0) nb = std::min<int64_t>(nb, c->build_batch);
+    nb = std::min<int64_t>(nb, n);
+    if (nb <= 0) return set_err(c, RKR_ERR_NOMEM, std::string(what) + ": no memory for a score block");
+    ScopedRows blk{c}, tmp{c};
+    blk.bytes = tmp.bytes = (size_t)nb * row_bytes;
+    int rc = dev_alloc_t(c, &blk.p, (size_t)nb * I, "score block");
+    if (!rc) rc = dev_alloc_t(c, &tmp.p, (size_t)nb * I, "sort scratch");
+    if (rc) return rc;
+    for (int64_t b0 = 0; b0 < n; b0 += nb) {
+        const int64_t m = std::min<int64_t>(nb, n - b0);
+        RKR_HIP(c, launch_scores_store(c->stream, c->users + (u0 + b0) * c->ld, nullptr, m, c->ld, c->items, I, c->ld,
+                                       c->dim, blk.p, I, true));
+        RKR_HIP(c, launch_sort_rows(c->stream, blk.p, I, m, I, blk.p, tmp.p, I));
+        if ((rc = sink(blk.p, u0 + b0, m))) return rc;
+    }
+    return RKR_OK;
+}
+}  // extern "C++"
+
+int rkr_sorted_scores(rkr_ctx *c, int64_t user_begin, int64_t n_rows, float *out) {
+    // ComputeScoreTableBatch::ComputeSortItemsBatch (ComputeScoreTableBatch.hpp:79-92)
+    if (!c || (n_rows > 0 && !out)) return c ? set_err(c, RKR_ERR_INVALID, "sorted_scores: bad arguments") : RKR_ERR_INVALID;
+    if (!c->users) return set_err(c, RKR_ERR_STATE, "sorted_scores: no data");
+    if (user_begin < 0 || n_rows < 0 || user_begin + n_rows > c->n_user)
+        return set_err(c, RKR_ERR_INVALID, "sorted_scores: row range out of bounds");
+    if (n_rows == 0) return RKR_OK;
+    RKR_HIP(c, hipSetDevice(c->device));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t I = c->n_item;
+    return sorted_score_batches(c, user_begin, n_rows, "sorted_scores", [&](const float *rows, int64_t u, int64_t m) {
+        RKR_HIP(c, hipMemcpyAsync(out + (u - user_begin) * I, rows, sizeof(float) * (size_t)(m * I),
+                                  hipMemcpyDeviceToHost, c->stream));
+        RKR_HIP(c, hipStreamSynchronize(c->stream));
+        return (int)RKR_OK;
+    });
+}
+
+int rkr_save_score_table(rkr_ctx *c, const char *path, int64_t n_rows, double *elapsed_ms) {
+    // ReadAll::BuildIndexLoop layout (ReadAll.hpp:80-87) as BuildScoreTableBatch.cpp:69-144 writes it
+    if (!c || !path) return c ? set_err(c, RKR_ERR_INVALID, "save_score_table: bad arguments") : RKR_ERR_INVALID;
+    if (!c->users) return set_err(c, RKR_ERR_STATE, "save_score_table: no data");
+    if (n_rows < 0) n_rows = c->n_user;
+    if (n_rows > c->n_user) return set_err(c, RKR_ERR_INVALID, "save_score_table: more rows than users");
+    RKR_HIP(c, hipSetDevice(c->device));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    const double t0 = rkr_now();
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return set_err(c, RKR_ERR_IO, std::string("save_score_table: cannot open ") + path);
+    const int64_t I = c->n_item;
+    std::vector<float> host;
+    int rc = n_rows == 0 ? (int)RKR_OK
+                         : sorted_score_batches(c, 0, n_rows, "save_score_table",
+                                                [&](const float *rows, int64_t, int64_t m) {
+        host.resize((size_t)(m * I));
+        RKR_HIP(c, hipMemcpyAsync(host.data(), rows, sizeof(float) * host.size(), hipMemcpyDeviceToHost, c->stream));
+        RKR_HIP(c, hipStreamSynchronize(c->stream));
+        if (std::fwrite(host.data(), sizeof(float), host.size(), f) != host.size())
+            return set_err(c, RKR_ERR_IO, std::string("save_score_table: write failed on ") + path);
+        return (int)RKR_OK;
+    });
+    if (std::fclose(f) != 0 && !rc) rc = set_err(c, RKR_ERR_IO, std::string("save_score_table: close failed on ") + path);
+    if (elapsed_ms) *elapsed_ms = (rkr_now() - t0) * 1e3;
+    return rc;
+}
+
 int rkr_get_ranks(const rkr_ctx *c, int32_t *out, int32_t *n_sample) {
     if (!c) return RKR_ERR_INVALID;
     if (n_sample) *n_sample = c->n_sample;

@@ -328,6 +328,12 @@ int screen_max_kp();
 hipError_t launch_select_rows(hipStream_t s, const float *scores, int64_t nb, int64_t n_item,
                               const int32_t *ranks, int32_t S, float *table_out);
 
+// ---- full sorted rows (k_sort.hip) ------------------------------------------
+// out[b * ld + i] = i-th largest of src[b * ld_src + 0..n_item), b < nb; tmp is
+// scratch of the same shape as out.  out may be src (ld == ld_src).
+hipError_t launch_sort_rows(hipStream_t s, const float *src, int64_t ld_src, int64_t nb, int64_t n_item, float *out,
+                            float *tmp, int64_t ld);
+
 // ---- query pipeline (k_query.hip) -------------------------------------------
 struct QueryInfo {          // one per query of a device batch
     int32_t topk;

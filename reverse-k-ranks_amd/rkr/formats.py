@@ -11,6 +11,10 @@
   (SampleSearch.hpp:158-179 with the name "US" from NameTranslation.hpp).
 * result CSVs: one line per query, comma-separated userID / rank / queryIP
   (WriteRankResult, src/include/util/FileIO.hpp:84-148).
+* full score table: float[n_user][n_item], each user's scores against every
+  item sorted in descending order, row-major with no header
+  (ReadAll::BuildIndexLoop / ReadDisk, src/include/alg/DiskIndex/ReadAll.hpp:80-87, 100-106),
+  stored as <index_dir>/<dataset>.index (src/app/BuildScoreTableBatch.cpp:168-169).
 """
 from __future__ import annotations
 
@@ -73,6 +77,40 @@ def read_table(path: str):
     if ranks.size != S or table.size != S * n_user:
         raise ValueError(f"{path}: truncated table file")
     return ranks, table.reshape(n_user, S), n_item
+
+
+def score_table_path(index_dir: str, dataset: str) -> str:
+    return os.path.join(index_dir, f"{dataset}.index")
+
+
+def write_score_table(path: str, rows: np.ndarray) -> None:
+    np.ascontiguousarray(rows, np.float32).tofile(path)
+
+
+def read_score_table(path: str, n_item: int, mmap: bool = True) -> np.ndarray:
+    """The (n_user, n_item) score table; memory-mapped read-only unless mmap is False.
+
+    The file has no header, so n_item comes from the dataset; a file whose size
+    is not a whole number of rows is rejected.
+    """
+    size = os.path.getsize(path)
+    row = 4 * int(n_item)
+    if n_item <= 0 or size % row != 0:
+        raise ValueError(f"{path}: {size} bytes is not a whole number of {n_item}-item rows")
+    n_user = size // row
+    if mmap:
+        return np.memmap(path, dtype=np.float32, mode="r", shape=(n_user, n_item))
+    return np.fromfile(path, dtype=np.float32).reshape(n_user, n_item)
+
+
+def read_score_rows(path: str, n_item: int, user: int, start: int, count: int) -> np.ndarray:
+    """Items [start, start + count) of one user's sorted row (ReadAll::ReadDisk, ReadAll.hpp:100-106)."""
+    with open(path, "rb") as f:
+        f.seek((int(user) * int(n_item) + int(start)) * 4)
+        out = np.fromfile(f, dtype=np.float32, count=count)
+    if out.size != count:
+        raise ValueError(f"{path}: truncated score table")
+    return out
 
 
 def _csv_rows(a) -> str:

@@ -141,6 +141,20 @@ class Index:
     def load_table(self, path: str, eps: float = 0.0):
         _check(self.lib, self.ctx, self.lib.rkr_load_table(self.ctx, path.encode(), eps))
 
+    # ---- full score table (ComputeScoreTableBatch.hpp:79-92, ReadAll.hpp:80-87) ----
+    def sorted_scores(self, user_begin: int = 0, n_rows=None) -> np.ndarray:
+        """Every item's score for users [user_begin, user_begin + n_rows), each row descending."""
+        n_rows = self.n_user - user_begin if n_rows is None else n_rows
+        out = np.empty((n_rows, self.n_item), np.float32)
+        _check(self.lib, self.ctx, self.lib.rkr_sorted_scores(self.ctx, user_begin, n_rows, _ptr(out)))
+        return out
+
+    def save_score_table(self, path: str, n_rows: int = -1) -> float:
+        """Writes ReadAll's index file (n_rows x n_item FP32, no header); returns the wall time in ms."""
+        ms = C.c_double()
+        _check(self.lib, self.ctx, self.lib.rkr_save_score_table(self.ctx, path.encode(), n_rows, C.byref(ms)))
+        return ms.value
+
     # ---- updates (BaseUpdateIndex, src/include/struct/MethodBaseUpdate.hpp:15-40) ----
     def insert_items(self, items):
         items = _f32(items)

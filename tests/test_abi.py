@@ -103,6 +103,17 @@ def test_formats_roundtrip(tmp_path):
                                       [[0.5, 0.25], [1.0, 2.0]])
     assert open(paths[0]).read() == "1,2\n3,4\n"
     assert open(paths[1]).read() == "0,5\n1,1\n"
+    # full score table: float[n_user][n_item], no header
+    S = -np.sort(-rng.standard_normal((6, 11), dtype=np.float32), axis=1)
+    sp = formats.score_table_path(str(tmp_path), "ds")
+    assert sp.endswith("ds.index")
+    formats.write_score_table(sp, S)
+    assert os.path.getsize(sp) == 6 * 11 * 4
+    assert np.array_equal(formats.read_score_table(sp, 11), S)
+    assert np.array_equal(formats.read_score_table(sp, 11, mmap=False), S)
+    assert np.array_equal(formats.read_score_rows(sp, 11, 4, 3, 5), S[4, 3:8])
+    with pytest.raises(ValueError):
+        formats.read_score_table(sp, 7)
 
 
 def test_reference_dataset_fixture_layout(fake30d):

@@ -226,6 +226,84 @@ def test_save_load_roundtrip(tmp_path, gpu_index_factory):
     assert np.array_equal(idx.get_table(), T)
 
 
+# ---- full score table (ComputeSortItemsBatch rows, ReadAll's index file) --------
+
+@pytest.mark.parametrize("U,I,d,seed", [
+    (300, 1000, 30, 21),      # one LDS tile
+    (64, 8192, 16, 22),       # exactly one tile
+    (70, 8193, 9, 23),        # one tile and one value: a single merge pass
+    (40, 20000, 150, 24),     # plain-GPU row width: two merge passes, a partial run
+    (9, 50001, 33, 25),       # precision row width: three merge passes
+    (3, 1, 4, 26),            # one item
+])
+def test_sorted_scores_match_oracle(orc, U, I, d, seed):
+    _case(f"sorted scores U={U} I={I} d={d}")
+    import rkr
+    rng = np.random.default_rng(seed)
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = rng.standard_normal((I, d), dtype=np.float32) * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32)
+    idx = rkr.Index(0)
+    idx.set_data(users, items)
+    un = _normalized(orc, users)
+    got = idx.sorted_scores()
+    assert got.shape == (U, I)
+    assert np.array_equal(got, orc.sorted_score_rows(un, items))
+    u0, n = U // 3, max(1, U // 4)
+    assert np.array_equal(idx.sorted_scores(u0, n), got[u0:u0 + n])
+
+
+def test_sorted_scores_ties_batches_and_table(orc, tmp_path):
+    """Tied and zero scores, a zero user, rows split over several device batches;
+    the sampled table is the sorted rows at the sample ranks; the file holds the rows."""
+    _case("sorted scores ties + file")
+    import rkr
+    from rkr import formats
+    rng = np.random.default_rng(27)
+    d, I, U = 12, 9000, 37
+    base = rng.standard_normal((50, d), dtype=np.float32)
+    items = base[rng.integers(0, 50, I)]
+    items[:700] = 0.0
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    users[0] = 0.0
+    idx = rkr.Index(0)
+    idx.set_option(4, 8)  # RKR_OPT_BUILD_BATCH: 8 users per device batch
+    idx.set_data(users, items)
+    got = idx.sorted_scores()
+    assert np.array_equal(got, orc.sorted_score_rows(_normalized(orc, users), items))
+    ranks = rkr.Index.uniform_sample_ranks(I, 300)
+    idx.build_table(ranks)
+    assert np.array_equal(idx.get_table(), got[:, ranks])
+    p = formats.score_table_path(str(tmp_path), "ties")
+    idx.save_score_table(p)
+    assert os.path.getsize(p) == U * I * 4
+    assert np.array_equal(formats.read_score_table(p, I), got)
+    assert np.array_equal(formats.read_score_rows(p, I, 5, 100, 30), got[5, 100:130])
+    idx.save_score_table(p, 10)
+    assert np.array_equal(formats.read_score_table(p, I, mmap=False), got[:10])
+
+
+def test_sorted_scores_follow_updates(orc):
+    """After item inserts/deletes and user deletes the rows cover the current sets."""
+    _case("sorted scores after updates")
+    import rkr
+    rng = np.random.default_rng(28)
+    users = rng.standard_normal((200, 16), dtype=np.float32)
+    items = rng.standard_normal((3000, 16), dtype=np.float32)
+    new = rng.standard_normal((120, 16), dtype=np.float32)
+    idx = rkr.Index(0)
+    idx.set_data(users, items)
+    idx.build_table(rkr.Index.uniform_sample_ranks(3000, 64))
+    idx.insert_items(new)
+    idx.delete_items(np.array([0, 5, 2999, 3001], np.int64))
+    idx.delete_users(np.array([3, 150], np.int64))
+    cur_items = np.concatenate([items, new])
+    cur_items = np.delete(cur_items, [0, 5, 2999, 3001], axis=0)
+    cur_users = np.delete(_normalized(orc, users), [3, 150], axis=0)
+    got = idx.sorted_scores()
+    assert got.shape == (198, 3116)
+    assert np.array_equal(got, orc.sorted_score_rows(cur_users, cur_items))
+
+
 # ---- screened form: BF16 MFMA screen + exact recheck -------------------------
 
 def _screened(idx):
@@ -487,3 +565,21 @@ def test_cli_pipeline_fake30d(orc, fake30d, tmp_path):
     u, _, q = fake30d
     bu, br, _ = orc.bruteforce(orc.normalize_users(u), fake30d[1], q, 10)
     assert np.array_equal(users, bu) and np.array_equal(ranks, br)
+
+
+def test_cli_bst_fake30d(orc, fake30d, tmp_path):
+    """bst writes ReadAll's index file: every user's sorted scores, or the rows
+    that fit eval_size_gb."""
+    import subprocess
+    from conftest import GOLDEN, ROOT
+    from rkr import formats
+    _case("cli bst")
+    cli = os.path.join(ROOT, "reverse-k-ranks_amd", "rkr_cli")
+    common = ["--dataset_dir", GOLDEN, "--dataset_name", "fake-normal-30d", "--index_dir", str(tmp_path / "index")]
+    r = subprocess.run([cli, "bst"] + common, capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stderr
+    u, items, _ = fake30d
+    rows = formats.read_score_table(formats.score_table_path(str(tmp_path / "index"), "fake-normal-30d"),
+                                    items.shape[0])
+    assert rows.shape == (u.shape[0], items.shape[0])
+    assert np.array_equal(rows, orc.sorted_score_rows(orc.normalize_users(u), items))

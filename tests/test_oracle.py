@@ -124,6 +124,21 @@ def test_build_table_is_sorted_gather(orc):
     assert np.all(np.diff(T, axis=1) <= 0)
 
 
+def test_sorted_score_rows(orc):
+    """Full sorted rows: the same multiset as the scores, non-increasing, and equal to the
+    C oracle's own qsort-based table at every rank 0..I-1."""
+    rng = np.random.default_rng(12)
+    users = rng.standard_normal((25, 9), dtype=np.float32)
+    items = rng.standard_normal((513, 9), dtype=np.float32)
+    items[:40] = items[40:80]                      # ties
+    s = orc.score_rows(users, items)
+    r = orc.sorted_score_rows(users, items)
+    assert np.array_equal(np.sort(r, axis=1), np.sort(s, axis=1))
+    assert np.all(np.diff(r, axis=1) <= 0)
+    assert np.array_equal(r, orc.build_table(users, items, np.arange(513, dtype=np.int32)))
+    assert np.array_equal(orc.sorted_score_rows(users, items, 5, 7), r[5:12])
+
+
 # ---- a6 coarse binary search (SampleSearch.hpp:91-130) ------------------------
 
 def _bsearch_brute(row, ranks, I, q):
