@@ -37,6 +37,8 @@ WORKLOADS = {
     "plain_cpu": ("plain_cpu", 10, "plain_cpu: 1k users x 5k items x d=30, S=128, 100 queries, k=10"),
     "precision": ("precision", 50, "precision: 1M users x 50k items x d=150, S=512, 1000 queries, k=50"),
     "batching": ("batching", None, "batching: 1M users x 50k items x d=150, S=512, 10000 Zipf(1.1) queries, k in [10,200]"),
+    "score_table": ("plain_gpu", None, "score_table: full sorted score rows (ReadAll's index file) of 100k users "
+                                       "x 20k items x d=150"),
     "serving": ("serving", None, "serving: 2M users x 100k items x d=128, S=512, Poisson arrivals, k in [10,200], "
                                  "+1% users and +0.5% items every 1 s"),
 }
@@ -258,6 +260,66 @@ def serving(args, idx, wl, label):
     print(json.dumps(out), flush=True)
 
 
+def score_table(args, idx, wl, label):
+    """Full score table (ReadAll's index file, every user's scores sorted in
+    descending order) of every user, one step = one rkr_save_score_table call
+    over all users.  The file goes to /dev/null so the file system stays out
+    of the figure; the rows still cross PCIe into host memory."""
+    U, I, d = wl.users.shape[0], wl.items.shape[0], wl.users.shape[1]
+    for _ in range(args.warmup):
+        idx.save_score_table(os.devnull, min(U, 8192))
+    idx.lib.rkr_sync(idx.ctx)
+    tot = {"gemm_ms": 0.0, "sort_ms": 0.0, "compute_sort_ms": 0.0, "save_ms": 0.0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for key, v in idx.save_score_table(os.devnull).items():
+            tot[key] += v
+    elapsed = time.perf_counter() - t0
+    rows = args.steps * U
+    # k_sort_rows reads and writes each score once: 8*I bytes per row
+    sort_gbs = 8.0 * I * rows / (tot["sort_ms"] * 1e-3) / 1e9
+    gemm_tflops = 2.0 * U * I * d * args.steps / (tot["gemm_ms"] * 1e-3) / 1e12
+    roof = {"kernel": "k_sort_rows", "bound": "hbm", "achieved": round(sort_gbs, 1), "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": round(sort_gbs / PEAK_HBM_GBS, 4), "traffic": None,
+            "work_per_launch": "8*n_item bytes per row (each score read and written once) x rows per launch",
+            "score_gemm": {"kernel": "k_scores_store", "achieved_tflops": round(gemm_tflops, 2),
+                           "peak_fp32_tflops": PEAK_FP32_TFLOPS, "note": "exact sequential FP32 rule, no FMA"}}
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        oracle.build()
+        cores = min(os.cpu_count() or 1, 16)
+        oracle.set_num_threads(cores)
+        n_s = args.cpu_sample or 2000
+        un = np.ascontiguousarray(wl.users[:n_s], np.float32).copy()
+        idx.lib.rkr_normalize_users(un.ctypes.data, n_s, d)
+        c0 = time.perf_counter()
+        ref = oracle.sorted_score_rows(un, wl.items)
+        dt = time.perf_counter() - c0
+        agree = bool(np.array_equal(ref[:64], idx.sorted_scores(0, 64)))
+        cpu = {"value": round(n_s / dt, 2), "unit": "rows/s", "cores": cores, "kind": "port",
+               "sample": f"first {n_s} users' rows (OpenMP scores, numpy sort)", "seconds": round(dt, 3),
+               "first_64_rows_identical": agree}
+    out = {
+        "metric": "sorted score rows/s (full score table, ReadAll layout)",
+        "value": round(rows / elapsed, 1),
+        "unit": "rows/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded), inputs resident in HBM; rows copied to host memory",
+        "config": {"workload": label, "n_user": U, "n_item": I, "dim": d},
+        "device_ms_per_step": {"score_gemm": round(tot["gemm_ms"] / args.steps, 3),
+                               "row_sort": round(tot["sort_ms"] / args.steps, 3)},
+        "compute_sort_ms_per_step": round(tot["compute_sort_ms"] / args.steps, 3),
+        "device_rows_per_s": round(rows / ((tot["gemm_ms"] + tot["sort_ms"]) * 1e-3), 1),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_sample_q):
     """The oracle's US query path (reference algorithm, OpenMP over users as the
     reference runs it) on the first n_sample_q queries, same table."""
@@ -322,6 +384,10 @@ def main():
     idx.set_option(8, args.dense_top)  # RKR_OPT_DENSE_TOP
     idx.set_option(9, args.coarse)     # RKR_OPT_COARSE
     idx.set_data(wl.users, wl.items, normalize_users=True)
+    if args.workload == "score_table":
+        score_table(args, idx, wl, label)
+        idx.close()
+        return
     ranks = rkr.Index.uniform_sample_ranks(I, wl.n_sample)
     t0 = time.perf_counter()
     build_ms = idx.build_table(ranks)

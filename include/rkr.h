@@ -180,8 +180,12 @@ int rkr_sorted_scores(rkr_ctx *ctx, int64_t user_begin, int64_t n_rows, float *o
  * user) to path in the layout of ReadAll's index file
  * (src/include/alg/DiskIndex/ReadAll.hpp:80-87, as
  * src/app/BuildScoreTableBatch.cpp:69-144 writes it): float[n_rows][n_item],
- * row-major, no header.  elapsed_ms (optional) receives the wall time. */
-int rkr_save_score_table(rkr_ctx *ctx, const char *path, int64_t n_rows, double *elapsed_ms);
+ * row-major, no header.  timing_ms (optional, 4 doubles) receives the
+ * reference's split of the build time (BuildScoreTableBatch.cpp:113-127):
+ * [0] device time of the score GEMMs, [1] device time of the row sorts,
+ * [2] wall time of computing and copying the rows to the host ("compute sort
+ * time"), [3] wall time of writing the file ("save index time"). */
+int rkr_save_score_table(rkr_ctx *ctx, const char *path, int64_t n_rows, double *timing_ms);
 
 /* ---- queries --------------------------------------------------------------- */
 

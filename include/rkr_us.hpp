@@ -16,6 +16,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "rkr.h"
@@ -79,11 +80,12 @@ class Index {
     void ComputeSortItemsBatch(int start_userID, int batch_n_user, float *distance_l) {
         check(rkr_sorted_scores(ctx_.get(), start_userID, batch_n_user, distance_l));
     }
-    // ReadAll's index file for the first n_user users (all when negative); returns the wall time in ms.
-    double SaveScoreTable(const std::string &path, int64_t n_user = -1) {
-        double ms = 0;
-        check(rkr_save_score_table(ctx_.get(), path.c_str(), n_user, &ms));
-        return ms;
+    // ReadAll's index file for the first n_user users (all when negative); returns the wall time in ms
+    // of computing the rows and of writing them.
+    std::pair<double, double> SaveScoreTable(const std::string &path, int64_t n_user = -1) {
+        double t[4] = {0, 0, 0, 0};
+        check(rkr_save_score_table(ctx_.get(), path.c_str(), n_user, t));
+        return {t[2], t[3]};
     }
 
     // BaseIndex::Retrieval (MethodBase.hpp:16-18) / US::Index::Retrieval (US.hpp:96-213).

@@ -176,9 +176,10 @@ int cmd_bst(const Args &a) {
         idx.SetData(user, item, false);
         mkdirs(get(a, "index_dir", "index"));
         const std::string path = get(a, "index_dir", "index") + "/" + get(a, "dataset_name", "fake-normal") + ".index";
-        const double ms = idx.SaveScoreTable(path, n_eval);
-        std::printf("build index time: total %.3fs, %lld of %d users, wrote %s\n", ms * 1e-3, (long long)n_eval,
-                    user.n_vector_, path.c_str());
+        const auto t = idx.SaveScoreTable(path, n_eval);
+        std::printf("build index time: total %.3fs, compute sort time %.3f s, save index time %.3f s, "
+                    "%lld of %d users, wrote %s\n", (t.first + t.second) * 1e-3, t.first * 1e-3, t.second * 1e-3,
+                    (long long)n_eval, user.n_vector_, path.c_str());
     } catch (const rkr::Error &e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

@@ -666,7 +666,8 @@ struct ScopedRows {
 // items, sorted in descending order.  The block size follows the memory budget
 // as the table build's does.
 template <class Sink>
-static int sorted_score_batches(rkr_ctx *c, int64_t u0, int64_t n, const char *what, Sink &&sink) {
+static int sorted_score_batches(rkr_ctx *c, int64_t u0, int64_t n, const char *what, Sink &&sink,
+                                double *gemm_ms = nullptr, double *sort_ms = nullptr) {
     const int64_t I = c->n_item;
     const size_t row_bytes = sizeof(float) * (size_t)I;
     const size_t avail = c->budget > c->held ? c->budget - c->held : 0;
@@ -681,10 +682,15 @@ static int sorted_score_batches(rkr_ctx *c, int64_t u0, int64_t n, const char *w
     if (rc) return rc;
     for (int64_t b0 = 0; b0 < n; b0 += nb) {
         const int64_t m = std::min<int64_t>(nb, n - b0);
+        RKR_HIP(c, hipEventRecord(c->ev[0], c->stream));
         RKR_HIP(c, launch_scores_store(c->stream, c->users + (u0 + b0) * c->ld, nullptr, m, c->ld, c->items, I, c->ld,
                                        c->dim, blk.p, I, true));
+        RKR_HIP(c, hipEventRecord(c->ev[1], c->stream));
         RKR_HIP(c, launch_sort_rows(c->stream, blk.p, I, m, I, blk.p, tmp.p, I));
-        if ((rc = sink(blk.p, u0 + b0, m))) return rc;
+        RKR_HIP(c, hipEventRecord(c->ev[2], c->stream));
+        if ((rc = sink(blk.p, u0 + b0, m))) return rc;  // the sink synchronises the stream
+        if (gemm_ms) *gemm_ms += ev_ms(c->ev, 0, 1);
+        if (sort_ms) *sort_ms += ev_ms(c->ev, 1, 2);
     }
     return RKR_OK;
 }
@@ -708,7 +714,7 @@ int rkr_sorted_scores(rkr_ctx *c, int64_t user_begin, int64_t n_rows, float *out
     });
 }
 
-int rkr_save_score_table(rkr_ctx *c, const char *path, int64_t n_rows, double *elapsed_ms) {
+int rkr_save_score_table(rkr_ctx *c, const char *path, int64_t n_rows, double *timing_ms) {
     // ReadAll::BuildIndexLoop layout (ReadAll.hpp:80-87) as BuildScoreTableBatch.cpp:69-144 writes it
     if (!c || !path) return c ? set_err(c, RKR_ERR_INVALID, "save_score_table: bad arguments") : RKR_ERR_INVALID;
     if (!c->users) return set_err(c, RKR_ERR_STATE, "save_score_table: no data");
@@ -716,10 +722,11 @@ int rkr_save_score_table(rkr_ctx *c, const char *path, int64_t n_rows, double *e
     if (n_rows > c->n_user) return set_err(c, RKR_ERR_INVALID, "save_score_table: more rows than users");
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
-    const double t0 = rkr_now();
     FILE *f = std::fopen(path, "wb");
     if (!f) return set_err(c, RKR_ERR_IO, std::string("save_score_table: cannot open ") + path);
     const int64_t I = c->n_item;
+    double gemm_ms = 0, sort_ms = 0, write_s = 0;
+    const double t0 = rkr_now();
     std::vector<float> host;
     int rc = n_rows == 0 ? (int)RKR_OK
                          : sorted_score_batches(c, 0, n_rows, "save_score_table",
@@ -727,12 +734,21 @@ int rkr_save_score_table(rkr_ctx *c, const char *path, int64_t n_rows, double *e
         host.resize((size_t)(m * I));
         RKR_HIP(c, hipMemcpyAsync(host.data(), rows, sizeof(float) * host.size(), hipMemcpyDeviceToHost, c->stream));
         RKR_HIP(c, hipStreamSynchronize(c->stream));
-        if (std::fwrite(host.data(), sizeof(float), host.size(), f) != host.size())
-            return set_err(c, RKR_ERR_IO, std::string("save_score_table: write failed on ") + path);
+        const double w0 = rkr_now();
+        const bool ok = std::fwrite(host.data(), sizeof(float), host.size(), f) == host.size();
+        write_s += rkr_now() - w0;
+        if (!ok) return set_err(c, RKR_ERR_IO, std::string("save_score_table: write failed on ") + path);
         return (int)RKR_OK;
-    });
+    }, &gemm_ms, &sort_ms);
+    const double w0 = rkr_now();
     if (std::fclose(f) != 0 && !rc) rc = set_err(c, RKR_ERR_IO, std::string("save_score_table: close failed on ") + path);
-    if (elapsed_ms) *elapsed_ms = (rkr_now() - t0) * 1e3;
+    write_s += rkr_now() - w0;
+    if (timing_ms) {
+        timing_ms[0] = gemm_ms;
+        timing_ms[1] = sort_ms;
+        timing_ms[2] = (rkr_now() - t0 - write_s) * 1e3;
+        timing_ms[3] = write_s * 1e3;
+    }
     return rc;
 }
 

@@ -134,7 +134,7 @@ def load():
     L.rkr_save_table.argtypes = [vp, C.c_char_p]
     L.rkr_load_table.argtypes = [vp, C.c_char_p, C.c_float]
     L.rkr_sorted_scores.argtypes = [vp, i64, i64, f32p]
-    L.rkr_save_score_table.argtypes = [vp, C.c_char_p, i64, C.POINTER(C.c_double)]
+    L.rkr_save_score_table.argtypes = [vp, C.c_char_p, i64, C.c_void_p]
     L.rkr_query.argtypes = [vp, f32p, i32, i32p, i32, i32p, i32p, f32p, C.POINTER(QueryStats),
                             C.POINTER(BatchTiming)]
     L.rkr_insert_items.argtypes = [vp, f32p, i64]

@@ -149,11 +149,12 @@ class Index:
         _check(self.lib, self.ctx, self.lib.rkr_sorted_scores(self.ctx, user_begin, n_rows, _ptr(out)))
         return out
 
-    def save_score_table(self, path: str, n_rows: int = -1) -> float:
-        """Writes ReadAll's index file (n_rows x n_item FP32, no header); returns the wall time in ms."""
-        ms = C.c_double()
-        _check(self.lib, self.ctx, self.lib.rkr_save_score_table(self.ctx, path.encode(), n_rows, C.byref(ms)))
-        return ms.value
+    def save_score_table(self, path: str, n_rows: int = -1) -> dict:
+        """Writes ReadAll's index file (n_rows x n_item FP32, no header); returns the time split in ms."""
+        t = np.zeros(4, np.float64)
+        _check(self.lib, self.ctx, self.lib.rkr_save_score_table(self.ctx, path.encode(), n_rows,
+                                                                   t.ctypes.data_as(C.c_void_p)))
+        return dict(zip(("gemm_ms", "sort_ms", "compute_sort_ms", "save_ms"), t.tolist()))
 
     # ---- updates (BaseUpdateIndex, src/include/struct/MethodBaseUpdate.hpp:15-40) ----
     def insert_items(self, items):
