@@ -187,6 +187,48 @@ int rkr_sorted_scores(rkr_ctx *ctx, int64_t user_begin, int64_t n_rows, float *o
  * time"), [3] wall time of writing the file ("save index time"). */
 int rkr_save_score_table(rkr_ctx *ctx, const char *path, int64_t n_rows, double *timing_ms);
 
+/* ---- QS sample-rank training ---------------------------------------------- */
+
+/* Sample-rank methods of rkr_find_optimal_sample (the sample names of
+ * SampleMethodName, src/include/util/NameTranslation.hpp:12-34). */
+#define RKR_SAMPLE_OPTIMAL_PART 0     /* QS: candidates are every k-th rank and the rank before it */
+#define RKR_SAMPLE_OPTIMAL_UNIFORM 1  /* candidates floor(j * (n_item-1) / n_sample_query)          */
+
+/* SampleItem (src/app/build-index/QueryRankBySample.hpp:20-37): the first
+ * n_sample_item item IDs of 0..n_item-1 after std::shuffle with
+ * std::mt19937 seeded 0.  Host only. */
+int rkr_sample_items(int64_t n_item, int32_t n_sample_item, int32_t *out_item_ids);
+
+/* ComputeQueryRankByComputation
+ * (src/app/build-index/QueryRankByComputation.hpp:122-242): for each sample
+ * item s (an ID among the current items) and every user u,
+ * rank(u, s) = #{items whose score beats <u, s>} (QueryRankGPU.hpp:63-92);
+ * accu[s][r] = #{u : rank(u, s) <= r}, r = 0..n_item, so accu is
+ * n_sample_item x (n_item + 1) int32 (host).  device_ms (optional) receives
+ * the device time. */
+int rkr_query_rank_distribution(rkr_ctx *ctx, const int32_t *item_ids, int32_t n_sample_item, int32_t *accu,
+                                double *device_ms);
+
+/* ComputeSortKthRank (src/app/build-index/QueryRankBySample.hpp:103-134):
+ * kth[s] = first r with accu[s][r] >= sample_topk; the samples sorted by kth
+ * ascending (std::sort) give out_sort_sample_ids and out_sort_kth_rank.
+ * Host only. */
+int rkr_sort_kth_rank(const int32_t *accu, int64_t n_item, int32_t n_sample_item, int32_t sample_topk,
+                      int32_t *out_sort_kth_rank, int32_t *out_sort_sample_ids);
+
+/* FindOptimalSampleFast
+ * (src/include/alg/RankBoundRefinement/FindSampleRank/OptimalSampleRankFast.hpp:102-305)
+ * with the costs of QueryDistribution (OptimalSampleRank.hpp:17-330): picks
+ * n_sample strictly increasing ranks minimising the summed number of users
+ * whose rank lies in the sample bucket holding each query's k-th rank.
+ * sort_kth_rank (ascending) and accu (n_sample_query x (n_item + 1), rows in
+ * the same order) are what qdibc writes.  The DP runs on the device.
+ * train_io_cost (optional) receives the minimum cost (0 when every candidate
+ * fits). */
+int rkr_find_optimal_sample(rkr_ctx *ctx, int64_t n_user, int64_t n_item, int32_t n_sample_query,
+                            const int32_t *sort_kth_rank, const int32_t *accu, int32_t method, int32_t n_sample,
+                            int32_t *out_ranks, int64_t *train_io_cost);
+
 /* ---- queries --------------------------------------------------------------- */
 
 /* Answers n_query reverse k-ranks queries (queries: n_query x dim, row-major

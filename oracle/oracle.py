@@ -246,3 +246,148 @@ def set_num_threads(n: int) -> None:
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+# ---- QS sample-rank training ---------------------------------------------------
+
+def query_rank_distribution(users, items, sample_ids) -> np.ndarray:
+    """accu[s][r] = #{u : rank(u, s) <= r} with rank(u, s) = #{p : <u,p> > <u, item_s>}:
+    ComputeQueryRankByComputation (src/app/build-index/QueryRankByComputation.hpp:122-242)
+    with QueryRankCPU's lower_bound over the descending row (QueryRankCPU.hpp:101-108):
+    #{p : s_p > x} = n_item - #{p : s_p <= x}, searched in the ascending row."""
+    S = score_rows(users, items)
+    ids = np.asarray(sample_ids, np.int64)
+    n_item = items.shape[0]
+    ranks = np.empty((S.shape[0], len(ids)), np.int64)
+    for u in range(S.shape[0]):
+        asc = np.sort(S[u])
+        ranks[u] = n_item - np.searchsorted(asc, S[u, ids], side="right")
+    accu = np.zeros((len(ids), n_item + 1), np.int64)
+    for s in range(len(ids)):
+        accu[s] = np.cumsum(np.bincount(ranks[:, s], minlength=n_item + 1))
+    return accu.astype(np.int32)
+
+
+def sort_kth_rank(accu, topk):
+    """ComputeSortKthRank (src/app/build-index/QueryRankBySample.hpp:103-134): the first r with
+    accu[s][r] >= topk, samples ordered by it (ties in index order here)."""
+    kth = np.array([int(np.searchsorted(row, topk, side="left")) for row in np.asarray(accu)], np.int64)
+    order = np.argsort(kth, kind="stable")
+    return kth[order].astype(np.int32), order.astype(np.int32)
+
+
+def _fill_ranks(rank_set, n_sample, last_sample, n_item):
+    """The fill of FindOptimalSampleFast (OptimalSampleRankFast.hpp:121-141, 245-266)."""
+    n_remain = n_sample - len(rank_set)
+    if n_remain <= 0:
+        return rank_set
+    delta = last_sample * 1.0 / n_remain
+    x = 0.0
+    while len(rank_set) < n_sample and int(x) < last_sample:
+        if int(x) not in rank_set:
+            rank_set.add(int(x))
+            x += delta
+        else:
+            x += 1
+    r = 0
+    while len(rank_set) < n_sample and r < n_item:
+        rank_set.add(r)
+        r += 1
+    return rank_set
+
+
+def candidate_ranks(n_item, sort_kth, method):
+    """QueryDistribution's candidate ranks (OptimalSampleRank.hpp:48-91): method 0 is OptimalPart,
+    1 OptimalUniform."""
+    sort_kth = [int(v) for v in sort_kth]
+    if method == 0:
+        rs = set()
+        for k in sort_kth:
+            rs.add(k)
+            if k != 0:
+                rs.add(k - 1)
+        return sorted(rs)
+    Q = len(sort_kth)
+    delta = (n_item - 1) * 1.0 / Q
+    return sorted(int(np.floor(s * delta)) for s in range(Q))
+
+
+def find_optimal_sample_fast(n_user, n_item, sort_kth, accu, method, n_sample):
+    """FindOptimalSampleFast (OptimalSampleRankFast.hpp:102-305) with QueryDistribution's costs
+    computed by its own loops over the queries (OptimalSampleRank.hpp:206-318).  Returns
+    (ranks, train_io_cost); pure Python, small inputs only."""
+    sort_kth = [int(v) for v in sort_kth]
+    accu = np.asarray(accu, np.int64)
+    Q = len(sort_kth)
+    rank = candidate_ranks(n_item, sort_kth, method)
+    R = len(rank)
+    if R <= n_sample:
+        rs = _fill_ranks(set(rank), n_sample, n_item if Q < 500 else rank[-1], n_item)
+        return sorted(rs), 0
+    import bisect
+    last_pos = [bisect.bisect_right(sort_kth, rank[j]) for j in range(R)]      # first q with kth > rank
+    next_first = [bisect.bisect_left(sort_kth, rank[j] + 1) for j in range(R)]  # first q with kth >= rank+1
+
+    def begin(j):
+        return sum(int(accu[q][rank[j]]) for q in range(last_pos[j]))
+
+    def end(j):
+        r1 = rank[j] + 1
+        return sum(int(n_user - accu[q][r1 - 1]) for q in range(next_first[j], Q))
+
+    def between(ub, lb):
+        r_ub, r_lb = rank[ub] + 1, rank[lb]
+        qb, qe = next_first[ub], last_pos[lb]
+        if qb >= Q or qb > qe:
+            return 0
+        return sum(int(accu[q][r_lb] - accu[q][r_ub - 1]) for q in range(qb, qe))
+
+    E = [end(j) for j in range(R)]
+    dp = [[0] * R for _ in range(n_sample)]
+    pos = [[0] * R for _ in range(n_sample)]
+    for r in range(R):
+        dp[0][r] = begin(r) + E[r]
+        pos[0][r] = r
+    T = {}
+    for lb in range(1, R):
+        for ub in range(lb):
+            T[(ub, lb)] = -E[ub] + between(ub, lb) + E[lb]
+    for s in range(1, n_sample):
+        for r in range(R):
+            dp[s][r] = dp[s - 1][r]
+            pos[s][r] = pos[s - 1][r]
+            for p in range(r):
+                c = dp[s - 1][p] + T[(p, r)]
+                if dp[s][r] > c:
+                    dp[s][r] = c
+                    pos[s][r] = p
+    best, idx = None, -1
+    for r in range(R):
+        if best is None or dp[n_sample - 1][r] < best:
+            best, idx = dp[n_sample - 1][r], r
+    out = [0] * n_sample
+    for s in range(n_sample - 1, -1, -1):
+        out[s] = rank[idx]
+        idx = pos[s][idx]
+    if len(set(out)) != n_sample:
+        out = sorted(_fill_ranks(set(out), n_sample, rank[-1], n_item))
+    return out, int(best)
+
+
+def sample_set_cost(n_user, sort_kth, accu, ranks):
+    """The quantity the DP minimises, from its meaning: for every query, the number of users whose
+    rank lies in the bucket of the sorted sample ranks that holds the query's k-th rank
+    ([0, r_0], [r_0 + 1, r_1], ..., [r_last + 1, n_item])."""
+    accu = np.asarray(accu, np.int64)
+    rs = sorted(set(int(r) for r in ranks))
+    import bisect
+    tot = 0
+    for q, k in enumerate(sort_kth):
+        i = bisect.bisect_left(rs, int(k))  # first sample rank >= k
+        if i == 0:
+            tot += int(accu[q][rs[0]])
+        elif i == len(rs):
+            tot += int(n_user - accu[q][rs[-1]])
+        else:
+            tot += int(accu[q][rs[i]] - accu[q][rs[i - 1]])
+    return tot

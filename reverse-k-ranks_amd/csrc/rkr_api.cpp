@@ -752,6 +752,65 @@ int rkr_save_score_table(rkr_ctx *c, const char *path, int64_t n_rows, double *t
     return rc;
 }
 
+int rkr_query_rank_distribution(rkr_ctx *c, const int32_t *item_ids, int32_t n_sample_item, int32_t *accu,
+                                double *device_ms) {
+    // ComputeQueryRankByComputation (src/app/build-index/QueryRankByComputation.hpp:122-242): for every sample
+    // item s and user u, rank(u, s) = #{items scoring above <u, s>}, the lower_bound with greater<float> in
+    // u's sorted row (QueryRankGPU.hpp:63-92); accu[s][r] = #{u : rank(u, s) <= r}, r = 0..n_item.
+    if (!c || !item_ids || !accu || n_sample_item <= 0)
+        return c ? set_err(c, RKR_ERR_INVALID, "query_rank_distribution: bad arguments") : RKR_ERR_INVALID;
+    if (!c->users) return set_err(c, RKR_ERR_STATE, "query_rank_distribution: no data");
+    const int64_t I = c->n_item, W = I + 1, ns = n_sample_item;
+    std::vector<int64_t> ids((size_t)ns);
+    for (int64_t s = 0; s < ns; ++s) {
+        if (item_ids[s] < 0 || item_ids[s] >= I) return set_err(c, RKR_ERR_INVALID, "query_rank_distribution: bad item id");
+        ids[s] = item_ids[s];
+    }
+    RKR_HIP(c, hipSetDevice(c->device));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    // device buffers of other element types are held as raw bytes
+    ScopedRows smat{c}, qbuf{c}, dids{c}, hist{c};
+    smat.bytes = sizeof(float) * (size_t)(ns * c->ld);
+    dids.bytes = sizeof(int64_t) * (size_t)ns;
+    hist.bytes = sizeof(int32_t) * (size_t)(ns * W);
+    int rc = dev_alloc(c, reinterpret_cast<void **>(&smat.p), smat.bytes, "sample items");
+    if (!rc) rc = dev_alloc(c, reinterpret_cast<void **>(&dids.p), dids.bytes, "sample ids");
+    if (!rc) rc = dev_alloc(c, reinterpret_cast<void **>(&hist.p), hist.bytes, "rank histogram");
+    if (rc) return rc;
+    int32_t *h = reinterpret_cast<int32_t *>(hist.p);
+    const int64_t *d_ids = reinterpret_cast<const int64_t *>(dids.p);
+    RKR_HIP(c, hipMemcpyAsync(dids.p, ids.data(), dids.bytes, hipMemcpyHostToDevice, c->stream));
+    RKR_HIP(c, hipMemsetAsync(hist.p, 0, hist.bytes, c->stream));
+    RKR_HIP(c, launch_gather_rows(c->stream, c->items, c->ld, d_ids, ns, c->ld, smat.p, c->ld));
+    double ms = 0;
+    rc = sorted_score_batches(c, 0, c->n_user, "query_rank_distribution", [&](const float *rows, int64_t u, int64_t m) {
+        if (qbuf.bytes < sizeof(float) * (size_t)(m * ns)) {  // the first batch is the largest
+            dev_free(c, qbuf.p, qbuf.bytes);
+            qbuf.p = nullptr;
+            qbuf.bytes = sizeof(float) * (size_t)(m * ns);
+            const int r2 = dev_alloc(c, reinterpret_cast<void **>(&qbuf.p), qbuf.bytes, "sample scores");
+            if (r2) return r2;
+        }
+        RKR_HIP(c, hipEventRecord(c->ev[3], c->stream));
+        RKR_HIP(c, launch_scores_store(c->stream, c->users + u * c->ld, nullptr, m, c->ld, smat.p, ns, c->ld, c->dim,
+                                       qbuf.p, ns, true));
+        RKR_HIP(c, launch_rank_hist(c->stream, rows, I, m, I, qbuf.p, ns, (int32_t)ns, h));
+        RKR_HIP(c, hipEventRecord(c->ev[4], c->stream));
+        RKR_HIP(c, hipEventSynchronize(c->ev[4]));
+        ms += ev_ms(c->ev, 0, 2) + ev_ms(c->ev, 3, 4);
+        return (int)RKR_OK;
+    });
+    if (rc) return rc;
+    RKR_HIP(c, hipMemcpyAsync(accu, hist.p, hist.bytes, hipMemcpyDeviceToHost, c->stream));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    for (int64_t s = 0; s < ns; ++s) {  // counts per rank -> users with rank <= r
+        int32_t *row = accu + s * W;
+        for (int64_t r = 1; r < W; ++r) row[r] += row[r - 1];
+    }
+    if (device_ms) *device_ms = ms;
+    return RKR_OK;
+}
+
 int rkr_get_ranks(const rkr_ctx *c, int32_t *out, int32_t *n_sample) {
     if (!c) return RKR_ERR_INVALID;
     if (n_sample) *n_sample = c->n_sample;

@@ -11,6 +11,14 @@
   (SampleSearch.hpp:158-179 with the name "US" from NameTranslation.hpp).
 * result CSVs: one line per query, comma-separated userID / rank / queryIP
   (WriteRankResult, src/include/util/FileIO.hpp:84-148).
+* query distribution of QS training: <index_dir>/query_distribution/
+  <dataset>-n_sample_item_<Q>-sample_topk_<K>/ holding sample_itemID_l.txt
+  (one item ID per line, samples in k-th-rank order), sort_kth_rank_l.index
+  (int32[Q]) and accu_n_user_rank_l.index (int32[Q][n_item + 1], same order)
+  (WriteDistributionBelowTopk, src/app/build-index/FileIO.hpp:56-137); the
+  trained ranks go to <index_dir>/qrs_to_sample_index/
+  QS-<dataset>-n_sample_<S>-n_sample_query_<Q>-sample_topk_<K>.index in the
+  table-file header layout without scores (SampleSearch.hpp:158-194).
 * full score table: float[n_user][n_item], each user's scores against every
   item sorted in descending order, row-major with no header
   (ReadAll::BuildIndexLoop / ReadDisk, src/include/alg/DiskIndex/ReadAll.hpp:80-87, 100-106),
@@ -77,6 +85,52 @@ def read_table(path: str):
     if ranks.size != S or table.size != S * n_user:
         raise ValueError(f"{path}: truncated table file")
     return ranks, table.reshape(n_user, S), n_item
+
+
+def query_distribution_dir(index_dir: str, dataset: str, n_sample_item: int, sample_topk: int) -> str:
+    return os.path.join(index_dir, "query_distribution",
+                        f"{dataset}-n_sample_item_{n_sample_item}-sample_topk_{sample_topk}")
+
+
+def write_query_distribution(path: str, sample_item_ids, sort_kth_rank, accu_sorted) -> None:
+    os.makedirs(path, exist_ok=True)
+    with open(os.path.join(path, "sample_itemID_l.txt"), "w") as f:
+        f.write("".join(f"{int(i)}\n" for i in sample_item_ids))
+    np.ascontiguousarray(sort_kth_rank, np.int32).tofile(os.path.join(path, "sort_kth_rank_l.index"))
+    np.ascontiguousarray(accu_sorted, np.int32).tofile(os.path.join(path, "accu_n_user_rank_l.index"))
+
+
+def read_query_distribution(path: str, n_item: int):
+    """Returns (sample_item_ids, sort_kth_rank, accu_sorted)."""
+    ids = np.loadtxt(os.path.join(path, "sample_itemID_l.txt"), dtype=np.int64, ndmin=1).astype(np.int32)
+    kth = np.fromfile(os.path.join(path, "sort_kth_rank_l.index"), dtype=np.int32)
+    accu = np.fromfile(os.path.join(path, "accu_n_user_rank_l.index"), dtype=np.int32)
+    if kth.size != ids.size or accu.size != kth.size * (n_item + 1):
+        raise ValueError(f"{path}: query distribution files disagree in size")
+    return ids, kth, accu.reshape(kth.size, n_item + 1)
+
+
+def qs_ranks_path(index_dir: str, dataset: str, n_sample: int, n_sample_query: int, sample_topk: int) -> str:
+    return os.path.join(index_dir, "qrs_to_sample_index",
+                        f"QS-{dataset}-n_sample_{n_sample}-n_sample_query_{n_sample_query}-sample_topk_{sample_topk}.index")
+
+
+def write_ranks(path: str, ranks, n_item: int, n_user: int) -> None:
+    """The rank file fsr writes: size_t n_sample, n_item, n_user, then int32 ranks."""
+    ranks = np.ascontiguousarray(ranks, np.int32)
+    with open(path, "wb") as f:
+        np.array([ranks.size, n_item, n_user], dtype=np.uint64).tofile(f)
+        ranks.tofile(f)
+
+
+def read_ranks(path: str):
+    """Returns (ranks, n_item, n_user) of a rank file."""
+    with open(path, "rb") as f:
+        S, n_item, n_user = (int(v) for v in np.fromfile(f, dtype=np.uint64, count=3))
+        ranks = np.fromfile(f, dtype=np.int32, count=S)
+    if ranks.size != S:
+        raise ValueError(f"{path}: truncated rank file")
+    return ranks, n_item, n_user
 
 
 def score_table_path(index_dir: str, dataset: str) -> str:

@@ -134,3 +134,29 @@ def test_synth_shapes_and_seeds():
     # Zipf over norm-sorted items: the largest-norm item is the most frequent query
     top = np.bincount(z.query_items, minlength=2000).argmax()
     assert top == int(np.argmax(norms))
+
+
+def test_sample_items_host():
+    """SampleItem: a prefix of one fixed shuffle of 0..I-1 (std::mt19937 seeded 0)."""
+    from rkr import train
+    full = train.sample_items(1000, 1000)
+    assert sorted(full.tolist()) == list(range(1000))
+    assert np.array_equal(train.sample_items(1000, 37), full[:37])
+    assert np.array_equal(train.sample_items(1000, 37), train.sample_items(1000, 37))
+    assert not np.array_equal(full[:50], np.arange(50))
+    for bad in ((0, 1), (10, 0), (10, 11)):
+        with pytest.raises(Exception):
+            train.sample_items(*bad)
+
+
+def test_sort_kth_rank_host_matches_oracle(orc):
+    from rkr import train
+    rng = np.random.default_rng(5)
+    counts = rng.integers(0, 4, size=(300, 51))
+    accu = np.cumsum(counts, axis=1).astype(np.int32)
+    for topk in (1, 10, 40):
+        kth, ids = train.sort_kth_rank(accu, topk)
+        okth, _ = orc.sort_kth_rank(accu, topk)
+        assert np.array_equal(kth, okth)
+        assert sorted(ids.tolist()) == list(range(300))
+        assert np.array_equal(np.array([np.searchsorted(accu[i], topk) for i in ids]), kth)

@@ -1,0 +1,102 @@
+"""GPU parity of QS sample-rank training (qdibc + fsr --method_name QS) against the oracle.
+
+The rank distribution is integer output and must match bit for bit; the DP's
+ranks and minimum cost must equal the oracle's restatement of
+FindOptimalSampleFast, whose minimum the oracle tests pin to a brute-force
+search over sample sets.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def _data(seed, U, I, d, dup=0):
+    rng = np.random.default_rng(seed)
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = rng.standard_normal((I, d), dtype=np.float32) * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32)
+    if dup:
+        items[:dup] = items[dup:2 * dup]  # tied scores
+    return rng, users, items
+
+
+@pytest.mark.parametrize("U,I,d,ns,batch,seed", [
+    (200, 1000, 30, 40, 0, 1),       # one sort tile
+    (64, 9000, 12, 25, 16, 2),       # rows over one tile, several device batches
+    (33, 20001, 20, 7, 0, 3),        # two merge passes, odd width
+    (5, 40, 3, 40, 0, 4),            # every item sampled
+])
+def test_rank_distribution_matches_oracle(orc, U, I, d, ns, batch, seed):
+    _case(f"rank distribution U={U} I={I} ns={ns} batch={batch}")
+    import rkr
+    from rkr import train
+    rng, users, items = _data(seed, U, I, d, dup=5)
+    idx = rkr.Index(0)
+    if batch:
+        idx.set_option(4, batch)  # RKR_OPT_BUILD_BATCH
+    idx.set_data(users, items)
+    ids = rng.choice(I, ns, replace=False).astype(np.int32)
+    accu, ms = train.query_rank_distribution(idx, ids)
+    ref = orc.query_rank_distribution(orc.normalize_users(users), items, ids)
+    assert np.array_equal(accu, ref)
+    assert ms > 0
+
+
+def _synthetic_distribution(rng, U, I, Q, spread):
+    """Monotone per-query user counts whose k-th ranks spread over [0, spread)."""
+    accu = np.zeros((Q, I + 1), np.int64)
+    for q in range(Q):
+        ranks = np.minimum(rng.integers(0, spread, U) + rng.integers(0, I - spread, U) * (rng.random(U) < 0.3), I - 1)
+        accu[q] = np.cumsum(np.bincount(ranks, minlength=I + 1))
+    return accu.astype(np.int32)
+
+
+@pytest.mark.parametrize("method", ["OptimalPart", "OptimalUniform"])
+@pytest.mark.parametrize("U,I,Q,topk,n_sample,seed", [
+    (300, 400, 40, 20, 12, 5),     # DP path: more candidates than samples
+    (300, 400, 40, 20, 3, 6),      # DP with few samples (ranks repeat, then the fill)
+    (50, 300, 25, 5, 200, 7),      # every candidate fits: the fill path, n_sample_query < 500
+    (20, 900, 600, 3, 700, 8),     # the fill path with n_sample_query >= 500
+])
+def test_find_optimal_sample_matches_oracle(orc, method, U, I, Q, topk, n_sample, seed):
+    _case(f"optimal sample {method} U={U} I={I} Q={Q} S={n_sample}")
+    import rkr
+    from rkr import train
+    rng = np.random.default_rng(seed)
+    accu = _synthetic_distribution(rng, U, I, Q, max(2, I // 8))
+    kth, order = train.sort_kth_rank(accu, topk)
+    acc_s = accu[order]
+    idx = rkr.Index(0)
+    ranks, cost = train.find_optimal_sample(idx, U, I, kth, acc_s, n_sample, method)
+    m = 0 if method == "OptimalPart" else 1
+    oranks, ocost = orc.find_optimal_sample_fast(U, I, kth, acc_s, m, n_sample)
+    assert ranks.tolist() == list(oranks) and cost == ocost
+    assert np.all(np.diff(ranks) > 0) and ranks[0] >= 0 and ranks[-1] < I
+
+
+def test_trained_ranks_answer_exactly(orc):
+    """qdibc + fsr on the device, then a table on the trained ranks: answers equal brute force,
+    and the training cost is the cost of the trained ranks."""
+    _case("train QS ranks end to end")
+    import rkr
+    from rkr import train
+    rng, users, items = _data(9, 2000, 3000, 16)
+    idx = rkr.Index(0)
+    idx.set_data(users, items)
+    out = train.train_sample_ranks(idx, n_sample=64, n_sample_query=150, sample_topk=20)
+    ranks = out["ranks"]
+    assert ranks.shape == (64,) and np.all(np.diff(ranks) > 0)
+    un = orc.normalize_users(users)
+    ref = orc.query_rank_distribution(un, items, out["sample_item_ids"])
+    assert np.array_equal(ref, out["accu_sorted"])
+    # extra ranks from the fill can only split buckets, so the final set costs at most the DP minimum
+    assert orc.sample_set_cost(2000, out["sort_kth_rank"], out["accu_sorted"], ranks) <= out["train_io_cost"]
+    idx.build_table(ranks)
+    queries = items[rng.choice(3000, 20, replace=False)]
+    res = idx.query(queries, 20)
+    bu, br, _ = orc.bruteforce(un, items, queries, 20)
+    assert np.array_equal(res.users, bu) and np.array_equal(res.ranks, br)

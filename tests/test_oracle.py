@@ -322,3 +322,63 @@ def test_safe_bounds_fix_rank_zero_and_sampled_positions(orc):
         q = srt[ranks[j]]  # q ties the sampled entry j: r[j] items beat it
         lbs, ubs = orc.safe_bounds(row, ranks, I, q)
         assert lbs == ranks[j] and ubs == ranks[j - 1] + 1
+
+
+# ---- QS sample-rank training (QueryRankByComputation.hpp, OptimalSampleRankFast.hpp) ----
+
+def _train_inputs(orc, seed, U, I, ns, d=4):
+    rng = np.random.default_rng(seed)
+    users = orc.normalize_users(rng.standard_normal((U, d), dtype=np.float32))
+    items = rng.standard_normal((I, d), dtype=np.float32)
+    ids = rng.choice(I, ns, replace=False)
+    return rng, users, items, ids
+
+
+def test_query_rank_distribution_invariants(orc):
+    """Counts of users by rank: non-decreasing rows that reach n_user by rank I-1 (an item never
+    beats itself), and the rank of a sample item's own score row is the strict count."""
+    _, users, items, ids = _train_inputs(orc, 3, 40, 60, 12)
+    items[10] = items[11]                                   # a tie
+    accu = orc.query_rank_distribution(users, items, ids)
+    assert accu.shape == (12, 61)
+    assert np.all(np.diff(accu, axis=1) >= 0) and np.all(accu[:, 59] == 40) and np.all(accu[:, 60] == 40)
+    S = orc.score_rows(users, items)
+    ranks = (S[:, None, :] > S[:, ids][:, :, None]).sum(axis=2)  # brute-force strict counts
+    for s in range(12):
+        assert np.array_equal(accu[s], np.cumsum(np.bincount(ranks[:, s], minlength=61)))
+
+
+def test_sort_kth_rank_is_lower_bound(orc):
+    accu = np.array([[0, 1, 3, 5, 5], [2, 2, 4, 5, 5], [0, 0, 0, 1, 5]], np.int32)
+    kth, order = orc.sort_kth_rank(accu, 3)
+    assert list(kth) == [2, 2, 4] and list(order) == [0, 1, 2]
+    kth, order = orc.sort_kth_rank(accu, 2)
+    assert list(kth) == [0, 2, 4] and list(order) == [1, 0, 2]
+
+
+def test_optimal_sample_dp_equals_brute_force(orc):
+    """The DP's minimum equals the best cost over every subset of candidate ranks (at most n_sample
+    of them), with the cost taken from its meaning: users whose rank shares the query's k-th-rank
+    bucket."""
+    import itertools
+    rng = np.random.default_rng(0)
+    checked = 0
+    for trial in range(25):
+        U, I, ns = int(rng.integers(3, 20)), int(rng.integers(8, 15)), int(rng.integers(3, 8))
+        _, users, items, ids = _train_inputs(orc, 100 + trial, U, I, ns)
+        accu = orc.query_rank_distribution(users, items, ids)
+        kth, order = orc.sort_kth_rank(accu, int(rng.integers(1, U + 1)))
+        acc_s = accu[order]
+        for method in (0, 1):
+            cand = sorted(set(orc.candidate_ranks(I, kth, method)))
+            for n_sample in (1, 2, 3):
+                ranks, cost = orc.find_optimal_sample_fast(U, I, kth, acc_s, method, n_sample)
+                assert len(ranks) == n_sample and ranks == sorted(set(ranks)) and all(0 <= r < I for r in ranks)
+                if len(orc.candidate_ranks(I, kth, method)) <= n_sample:
+                    assert set(cand) <= set(ranks)
+                    continue
+                bf = min(orc.sample_set_cost(U, kth, acc_s, c)
+                         for m in range(1, n_sample + 1) for c in itertools.combinations(cand, m))
+                assert cost == bf
+                checked += 1
+    assert checked > 50
