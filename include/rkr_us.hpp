@@ -170,4 +170,21 @@ inline std::unique_ptr<Index> BuildIndex(VectorMatrix &data_item, VectorMatrix &
     return idx;
 }
 
+// QS::BuildIndex (src/impl/QS.hpp, called at ReverseRankIndex.cpp:113-123): loads
+// <index_dir>/memory_index/QS-<dataset_name>-n_sample_<n_sample>-n_sample_query_<q>-sample_topk_<k>.index,
+// the table bsibc --method_name QS builds on the trained ranks.  Answers come from the same exact query
+// path as US.
+inline std::unique_ptr<Index> BuildIndexQS(VectorMatrix &data_item, VectorMatrix &user, const char *dataset_name,
+                                           const int &n_sample, const int &n_sample_query, const int &sample_topk,
+                                           const int &n_thread, const char *index_dir, int device = 0) {
+    (void)n_thread;
+    auto idx = std::make_unique<Index>(device);
+    idx->SetData(user, data_item, /*normalize_users=*/false);
+    char path[1024];
+    std::snprintf(path, sizeof path, "%s/memory_index/QS-%s-n_sample_%d-n_sample_query_%d-sample_topk_%d.index",
+                  index_dir, dataset_name, n_sample, n_sample_query, sample_topk);
+    idx->LoadTable(path);
+    return idx;
+}
+
 }  // namespace rkr

@@ -7,6 +7,14 @@
 //       BuildSampleIndexByComputation (src/app/build-index/BuildSampleIndexByComputation.cpp:66-214):
 //       builds the sampled score table on the GPU and writes
 //       X/memory_index/US-N-n_sample_S.index.
+//   rkr_cli qdibc --dataset_dir D --dataset_name N --index_dir X [--n_sample_item Q] [--sample_topk K]
+//       QueryDistributionIndexByComputation (src/app/build-index/QueryDistributionIndexByComputation.cpp,
+//       QueryRankByComputation.hpp:244-297): samples Q items, ranks each for every user on the GPU and
+//       writes X/query_distribution/N-n_sample_item_Q-sample_topk_K/ (FileIO.hpp:56-137).
+//   --method_name QS (fsr, bsibc, rri; with --n_sample_query Q --sample_topk K): fsr trains the ranks
+//       from qdibc's files with FindOptimalSampleFast into
+//       X/qrs_to_sample_index/QS-N-n_sample_S-n_sample_query_Q-sample_topk_K.index; bsibc builds the
+//       table on them into X/memory_index/ under the same name; rri answers from that table.
 //   rkr_cli bst   --dataset_dir D --dataset_name N --index_dir X [--eval_size_gb G]
 //       BuildScoreTableBatch (src/app/BuildScoreTableBatch.cpp:56-144, 159-169): writes every user's
 //       sorted scores (ReadAll's layout, float[n_user][n_item]) to X/N.index; with eval_size_gb below
@@ -100,9 +108,107 @@ bool read_data(const Args &a, rkr::VectorMatrix &user, rkr::VectorMatrix &item, 
     return true;
 }
 
+bool is_qs(const Args &a) { return get(a, "method_name", "US") == "QS"; }
+
+// SampleSearch::SaveIndex / LoadIndex names (SampleSearch.hpp:158-179): QS adds the training parameters
 std::string index_file(const Args &a, const char *sub, int S) {
-    return get(a, "index_dir", "index") + "/" + sub + "/US-" + get(a, "dataset_name", "fake-normal") + "-n_sample_" +
-           std::to_string(S) + ".index";
+    std::string name = get(a, "index_dir", "index") + "/" + sub + "/" + (is_qs(a) ? "QS-" : "US-") +
+                       get(a, "dataset_name", "fake-normal") + "-n_sample_" + std::to_string(S);
+    if (is_qs(a))
+        name += "-n_sample_query_" + get(a, "n_sample_query", "150") + "-sample_topk_" + get(a, "sample_topk", "50");
+    return name + ".index";
+}
+
+std::string query_distribution_dir(const Args &a, const std::string &n_sample_item, const std::string &topk) {
+    return get(a, "index_dir", "index") + "/query_distribution/" + get(a, "dataset_name", "fake-normal") +
+           "-n_sample_item_" + n_sample_item + "-sample_topk_" + topk;
+}
+
+bool write_file(const std::string &path, const void *p, size_t bytes) {
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(p, 1, bytes, f) == bytes;
+    return std::fclose(f) == 0 && ok;
+}
+
+bool read_file(const std::string &path, void *p, size_t bytes) {
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    const bool ok = std::fread(p, 1, bytes, f) == bytes && std::fgetc(f) == EOF;
+    std::fclose(f);
+    return ok;
+}
+
+int cmd_qdibc(const Args &a) {
+    rkr::VectorMatrix user, item, query;
+    if (!read_data(a, user, item, query)) return 1;
+    const std::string qs = get(a, "n_sample_item", "150"), ks = get(a, "sample_topk", "50");
+    const int Q = std::stoi(qs), K = std::stoi(ks);
+    const int64_t I = item.n_vector_, W = I + 1;
+    std::vector<int32_t> ids((size_t)std::max(Q, 1)), kth((size_t)std::max(Q, 1)), order((size_t)std::max(Q, 1));
+    if (rkr_sample_items(I, Q, ids.data()) != RKR_OK) {
+        std::fprintf(stderr, "n_sample_item must be in [1, n_item]\n");
+        return 1;
+    }
+    try {
+        rkr::Index idx(std::stoi(get(a, "device", "0")));
+        idx.SetData(user, item, false);
+        std::vector<int32_t> accu((size_t)(Q * W)), sorted((size_t)(Q * W));
+        double ms = 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        if (int rc = rkr_query_rank_distribution(idx.ctx(), ids.data(), Q, accu.data(), &ms))
+            throw rkr::Error(rc, rkr_last_error(idx.ctx()));
+        if (rkr_sort_kth_rank(accu.data(), I, Q, K, kth.data(), order.data()) != RKR_OK)
+            throw rkr::Error(RKR_ERR_INVALID, "bad sample_topk");
+        const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (int s = 0; s < Q; ++s)
+            std::copy(accu.begin() + (int64_t)order[s] * W, accu.begin() + (int64_t)(order[s] + 1) * W,
+                      sorted.begin() + (int64_t)s * W);
+        // WriteDistributionBelowTopk (FileIO.hpp:56-137): samples in k-th-rank order
+        const std::string dir = query_distribution_dir(a, qs, ks);
+        mkdirs(dir);
+        std::ofstream fi(dir + "/sample_itemID_l.txt");
+        for (int s = 0; s < Q; ++s) fi << ids[order[s]] << "\n";
+        fi.close();
+        if (!fi || !write_file(dir + "/sort_kth_rank_l.index", kth.data(), sizeof(int32_t) * Q) ||
+            !write_file(dir + "/accu_n_user_rank_l.index", sorted.data(), sizeof(int32_t) * sorted.size())) {
+            std::fprintf(stderr, "cannot write %s\n", dir.c_str());
+            return 1;
+        }
+        std::printf("report build_index_time %.3fs (device rank distribution %.3fs), wrote %s\n", total, ms * 1e-3,
+                    dir.c_str());
+    } catch (const rkr::Error &e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}
+
+// FindSampleRank with method QS (FindSampleRank.cpp:71-167, FindOptimalSampleFast) from qdibc's files
+int train_qs_ranks(const Args &a, int64_t n_user, int64_t n_item, int S, std::vector<int32_t> &r) {
+    const std::string qs = get(a, "n_sample_query", "150"), ks = get(a, "sample_topk", "50");
+    const int Q = std::stoi(qs);
+    const int64_t W = n_item + 1;
+    const std::string dir = query_distribution_dir(a, qs, ks);
+    std::vector<int32_t> kth((size_t)std::max(Q, 1)), accu((size_t)(std::max(Q, 1) * W));
+    if (Q <= 0 || !read_file(dir + "/sort_kth_rank_l.index", kth.data(), sizeof(int32_t) * Q) ||
+        !read_file(dir + "/accu_n_user_rank_l.index", accu.data(), sizeof(int32_t) * accu.size())) {
+        std::fprintf(stderr, "cannot read the query distribution in %s (run qdibc)\n", dir.c_str());
+        return 1;
+    }
+    try {
+        rkr::Index idx(std::stoi(get(a, "device", "0")));
+        int64_t cost = 0;
+        r.assign((size_t)S, 0);
+        if (int rc = rkr_find_optimal_sample(idx.ctx(), n_user, n_item, Q, kth.data(), accu.data(),
+                                             RKR_SAMPLE_OPTIMAL_PART, S, r.data(), &cost))
+            throw rkr::Error(rc, rkr_last_error(idx.ctx()));
+        std::printf("train io cost: %lld\n", (long long)cost);
+    } catch (const rkr::Error &e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
 }
 
 int cmd_fsr(const Args &a) {
@@ -110,7 +216,13 @@ int cmd_fsr(const Args &a) {
     if (!read_data(a, user, item, query)) return 1;
     const int S = std::stoi(get(a, "n_sample", "-1"));
     std::vector<int32_t> r((size_t)std::max(S, 1));
-    if (rkr_uniform_sample_ranks(item.n_vector_, S, r.data()) != RKR_OK) {
+    if (S <= 0 || S >= item.n_vector_) {
+        std::fprintf(stderr, "n_sample too small or too large\n");
+        return 1;
+    }
+    if (is_qs(a)) {
+        if (train_qs_ranks(a, user.n_vector_, item.n_vector_, S, r)) return 1;
+    } else if (rkr_uniform_sample_ranks(item.n_vector_, S, r.data()) != RKR_OK) {
         std::fprintf(stderr, "n_sample too small or too large\n");
         return 1;
     }
@@ -138,6 +250,11 @@ int cmd_bsibc(const Args &a) {
         have = std::fread(hdr, sizeof(size_t), 3, f) == 3 && hdr[0] == (size_t)S &&
                std::fread(r.data(), sizeof(int32_t), (size_t)S, f) == (size_t)S;
         std::fclose(f);
+    }
+    if (!have && is_qs(a)) {
+        std::fprintf(stderr, "no trained ranks at %s (run fsr --method_name QS)\n",
+                     index_file(a, "qrs_to_sample_index", S).c_str());
+        return 1;
     }
     if (!have && rkr_uniform_sample_ranks(item.n_vector_, S, r.data()) != RKR_OK) {
         std::fprintf(stderr, "n_sample too small or too large\n");
@@ -197,7 +314,10 @@ int cmd_rri(const Args &a) {
     const std::string ds = get(a, "dataset_name", "fake-normal");
     try {
         auto t0 = std::chrono::steady_clock::now();
-        auto idx = rkr::BuildIndex(item, user, ds.c_str(), get(a, "index_dir", "index").c_str(), S, -1,
+        auto idx = is_qs(a) ? rkr::BuildIndexQS(item, user, ds.c_str(), S, std::stoi(get(a, "n_sample_query", "150")),
+                                                std::stoi(get(a, "sample_topk", "50")), -1,
+                                                get(a, "index_dir", "index").c_str(), std::stoi(get(a, "device", "0")))
+                            : rkr::BuildIndex(item, user, ds.c_str(), get(a, "index_dir", "index").c_str(), S, -1,
                                    std::stoi(get(a, "device", "0")));
         idx->SetForm(form == "screened" ? RKR_FORM_SCREENED : RKR_FORM_PLAIN);
         idx->SetOutput(output == "reference" ? RKR_OUTPUT_REFERENCE : RKR_OUTPUT_EXACT);
@@ -207,10 +327,15 @@ int cmd_rri(const Args &a) {
         std::printf("%s\nbuild index time: total %.3fs\n", idx->PerformanceStatistics(topk).c_str(), build_s);
         // WriteRankResult (FileIO.hpp:84-148) and WriteQueryPerformance (FileIO.hpp:175-206)
         char param[256];
-        std::snprintf(param, sizeof param, "top%d-n_sample_%d-n_thread_%d", topk, S, -1);
+        if (is_qs(a))  // ReverseRankIndex.cpp:122-123
+            std::snprintf(param, sizeof param, "top%d-n_sample_%d-n_sample_query_%s-sample_topk_%s-n_thread_%d", topk,
+                          S, get(a, "n_sample_query", "150").c_str(), get(a, "sample_topk", "50").c_str(), -1);
+        else
+            std::snprintf(param, sizeof param, "top%d-n_sample_%d-n_thread_%d", topk, S, -1);
+        const std::string method = is_qs(a) ? "-QS-" : "-US-";
         mkdirs(res_dir + "/rank");
         mkdirs(res_dir + "/single_query_performance");
-        const std::string stem = res_dir + "/rank/" + ds + "-US-" + param;
+        const std::string stem = res_dir + "/rank/" + ds + method + param;
         std::ofstream fu(stem + "-userID.csv"), fr(stem + "-rank.csv"), fi(stem + "-IP.csv");
         for (const auto &row : res) {
             for (size_t j = 0; j < row.size(); ++j) {
@@ -220,7 +345,7 @@ int cmd_rri(const Args &a) {
                 fi << row[j].queryIP_ << sep;
             }
         }
-        std::ofstream fp(res_dir + "/single_query_performance/" + ds + "-US-" + param + "-single-query-performance.csv");
+        std::ofstream fp(res_dir + "/single_query_performance/" + ds + method + param + "-single-query-performance.csv");
         fp << "queryID,n_prune_user,n_result_user,n_refine_user,ip_cost,io_cost,total_time,memory_index_time,io_time\n";
         char buf[512];
         for (const auto &p : perf) {
@@ -241,14 +366,15 @@ int cmd_rri(const Args &a) {
 
 int main(int argc, char **argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: rkr_cli fsr|bsibc|bst|rri --dataset_dir D --dataset_name N --index_dir X "
+        std::fprintf(stderr, "usage: rkr_cli qdibc|fsr|bsibc|bst|rri --dataset_dir D --dataset_name N --index_dir X "
                              "--n_sample S [--topk K] [--form plain|screened] [--output exact|reference]\n");
         return 2;
     }
     const std::string cmd = argv[1];
     const Args a = parse(argc, argv, 2);
-    if (get(a, "method_name", "US") != "US") {
-        std::fprintf(stderr, "only method_name US is implemented\n");
+    if (cmd == "qdibc") return cmd_qdibc(a);
+    if (get(a, "method_name", "US") != "US" && !is_qs(a)) {
+        std::fprintf(stderr, "method_name US and QS are implemented\n");
         return 2;
     }
     if (cmd == "fsr") return cmd_fsr(a);

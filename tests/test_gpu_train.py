@@ -100,3 +100,38 @@ def test_trained_ranks_answer_exactly(orc):
     res = idx.query(queries, 20)
     bu, br, _ = orc.bruteforce(un, items, queries, 20)
     assert np.array_equal(res.users, bu) and np.array_equal(res.ranks, br)
+
+
+def test_cli_qs_pipeline_fake30d(orc, fake30d, tmp_path):
+    """qdibc -> fsr -> bsibc -> rri with --method_name QS on the reference fixture: the query
+    distribution files hold the oracle's counts, fsr's ranks are the oracle DP's, and the answers
+    are exact."""
+    import os
+    import subprocess
+    from conftest import GOLDEN, ROOT
+    from rkr import formats
+    _case("cli qdibc/fsr/bsibc/rri QS")
+    cli = os.path.join(ROOT, "reverse-k-ranks_amd", "rkr_cli")
+    idir = str(tmp_path / "index")
+    common = ["--dataset_dir", GOLDEN, "--dataset_name", "fake-normal-30d", "--index_dir", idir]
+    qs = ["--method_name", "QS", "--n_sample", "128", "--n_sample_query", "150", "--sample_topk", "10"]
+    steps = (["qdibc", "--n_sample_item", "150", "--sample_topk", "10"], ["fsr"] + qs, ["bsibc"] + qs,
+             ["rri"] + qs + ["--topk", "10", "--result_dir", str(tmp_path / "result")])
+    for cmd in steps:
+        r = subprocess.run([cli] + cmd + common, capture_output=True, text=True, timeout=100)
+        assert r.returncode == 0, (cmd[0], r.stderr)
+    users, items, queries = fake30d
+    un = orc.normalize_users(users)
+    ids, kth, accu = formats.read_query_distribution(
+        formats.query_distribution_dir(idir, "fake-normal-30d", 150, 10), items.shape[0])
+    assert np.array_equal(accu, orc.query_rank_distribution(un, items, ids))
+    okth, _ = orc.sort_kth_rank(accu, 10)
+    assert np.array_equal(kth, okth)
+    ranks, n_item, n_user = formats.read_ranks(formats.qs_ranks_path(idir, "fake-normal-30d", 128, 150, 10))
+    oranks, _ = orc.find_optimal_sample_fast(1000, 5000, kth, accu, 0, 128)
+    assert (n_item, n_user) == (5000, 1000) and ranks.tolist() == list(oranks)
+    stem = tmp_path / "result" / "rank" / "fake-normal-30d-QS-top10-n_sample_128-n_sample_query_150-sample_topk_10-n_thread_-1"
+    got_u = np.loadtxt(f"{stem}-userID.csv", delimiter=",", dtype=np.int64)
+    got_r = np.loadtxt(f"{stem}-rank.csv", delimiter=",", dtype=np.int64)
+    bu, br, _ = orc.bruteforce(un, items, queries, 10)
+    assert np.array_equal(got_u, bu) and np.array_equal(got_r, br)
