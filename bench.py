@@ -39,6 +39,8 @@ WORKLOADS = {
     "batching": ("batching", None, "batching: 1M users x 50k items x d=150, S=512, 10000 Zipf(1.1) queries, k in [10,200]"),
     "score_table": ("plain_gpu", None, "score_table: full sorted score rows (ReadAll's index file) of 100k users "
                                        "x 20k items x d=150"),
+    "qs_train": ("plain_gpu", None, "qs_train: QS sample-rank training (qdibc + fsr) of 100k users x 20k items x d=150, "
+                                    "9000 sample queries, sample_topk 600, 512 ranks"),
     "serving": ("serving", None, "serving: 2M users x 100k items x d=128, S=512, Poisson arrivals, k in [10,200], "
                                  "+1% users and +0.5% items every 1 s"),
 }
@@ -59,6 +61,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--duration", type=float, default=10.0, help="serving: open-loop seconds")
     ap.add_argument("--qps", type=float, default=0.0, help="serving: arrival rate (0 = half the saturating rate)")
+    ap.add_argument("--qs-queries", type=int, default=9000, help="qs_train: sample queries (n_sample_query)")
+    ap.add_argument("--qs-topk", type=int, default=600, help="qs_train: sample_topk")
     ap.add_argument("--n-user", type=int, default=None)
     ap.add_argument("--n-item", type=int, default=None)
     ap.add_argument("--pre-updates", type=int, default=0,
@@ -320,6 +324,81 @@ def score_table(args, idx, wl, label):
     print(json.dumps(out), flush=True)
 
 
+def qs_train(args, idx, wl, label):
+    """QS sample-rank training: sample items, each sample's rank distribution over
+    every user (score GEMM, row sort and rank histogram on the device), the
+    k-th-rank sort and the FindOptimalSampleFast DP (transit costs and layers on
+    the device).  One step = one full training."""
+    from rkr import train
+    U, I, d = wl.users.shape[0], wl.items.shape[0], wl.users.shape[1]
+    Q, K, S = args.qs_queries, args.qs_topk, wl.n_sample
+    for _ in range(args.warmup):
+        train.query_rank_distribution(idx, train.sample_items(I, 64))
+    idx.lib.rkr_sync(idx.ctx)
+    stage = {"sample_items": 0.0, "rank_distribution": 0.0, "sort_kth_rank": 0.0, "optimal_sample": 0.0}
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        ids = train.sample_items(I, Q)
+        b = time.perf_counter()
+        accu, ms = train.query_rank_distribution(idx, ids)
+        c = time.perf_counter()
+        kth, order = train.sort_kth_rank(accu, K)
+        acc_s = accu[order]
+        e = time.perf_counter()
+        ranks, cost = train.find_optimal_sample(idx, U, I, kth, acc_s, S, "OptimalPart")
+        f = time.perf_counter()
+        dev_ms += ms
+        for key, v in zip(stage, (b - a, c - b, e - c, f - e)):
+            stage[key] += v
+    elapsed = time.perf_counter() - t0
+    n_cand = len(set(int(v) for v in kth) | set(int(v) - 1 for v in kth if v > 0))
+    # the score GEMM of the rank distribution: 2*U*I*d flops (plus the 2*U*Q*d sample scores)
+    gemm_tflops = 2.0 * U * (I + Q) * d * args.steps / (dev_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        oracle.build()
+        cores = min(os.cpu_count() or 1, 16)
+        oracle.set_num_threads(cores)
+        n_s = args.cpu_sample or 500
+        un = np.ascontiguousarray(wl.users[:n_s], np.float32).copy()
+        idx.lib.rkr_normalize_users(un.ctypes.data, n_s, d)
+        c0 = time.perf_counter()
+        part = oracle.query_rank_distribution(un, wl.items, ids)
+        dt = time.perf_counter() - c0
+        cpu = {"value": round(n_s / dt, 2), "unit": "users/s (rank distribution)", "cores": cores, "kind": "port",
+               "sample": f"first {n_s} users' ranks of the same {Q} sample items (OpenMP scores, numpy "
+                         f"sort and search)", "seconds": round(dt, 3),
+               "gpu_users_per_s": round(U * args.steps / stage["rank_distribution"], 1),
+               "note": "the oracle's DP is pure Python for small inputs and is not timed here"}
+        del part
+    out = {
+        "metric": "QS training time (s): qdibc + fsr --method_name QS",
+        "value": round(elapsed / args.steps, 3),
+        "unit": "s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 scores, int32/int64 counts",
+        "data": "synthetic (seeded), inputs resident in HBM",
+        "config": {"workload": label, "n_user": U, "n_item": I, "dim": d, "n_sample_query": Q, "sample_topk": K,
+                   "n_sample": S},
+        "stage_s_per_step": {k_: round(v / args.steps, 4) for k_, v in stage.items()},
+        "rank_distribution_device_ms": round(dev_ms / args.steps, 2),
+        "candidate_ranks": n_cand,
+        "train_io_cost": int(cost),
+        "roofline": {"kernel": "rank distribution (k_scores_store + k_sort_rows + k_rank_hist)", "bound": "mfma",
+                     "achieved": round(gemm_tflops, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(gemm_tflops / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "work_per_launch": "2*U*(I+Q)*d score flops per training (exact FP32 rule, VALU)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_sample_q):
     """The oracle's US query path (reference algorithm, OpenMP over users as the
     reference runs it) on the first n_sample_q queries, same table."""
@@ -384,8 +463,8 @@ def main():
     idx.set_option(8, args.dense_top)  # RKR_OPT_DENSE_TOP
     idx.set_option(9, args.coarse)     # RKR_OPT_COARSE
     idx.set_data(wl.users, wl.items, normalize_users=True)
-    if args.workload == "score_table":
-        score_table(args, idx, wl, label)
+    if args.workload in ("score_table", "qs_train"):
+        (score_table if args.workload == "score_table" else qs_train)(args, idx, wl, label)
         idx.close()
         return
     ranks = rkr.Index.uniform_sample_ranks(I, wl.n_sample)
