@@ -954,7 +954,7 @@ namespace {
 // a coarse list longer than max(kCoarseRefineMin, U / 64) gets a second pass:
 // the pass costs about a user row per query, an exact-scored entry about a
 // user row per entry, and the pass narrows a list several times over
-constexpr int64_t kCoarseRefineMin = 1024;
+constexpr int64_t kCoarseRefineMin = 4096;
 constexpr int64_t kCoarseChunk = 16384;
 constexpr int32_t kCoarseMinBatch = 8;
 
