@@ -364,7 +364,10 @@ __device__ __forceinline__ uint32_t count_above8(const float (&g)[8], float x) {
 
 template <int KS, bool SETS>
 __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
-    constexpr int QST = 32;                                          // queries per LDS stage
+    // queries per LDS stage: 64 halves the stage barriers and query refills per
+    // user block; the second pass keeps 32 so its guard sets also fit 3
+    // workgroups per CU in LDS
+    constexpr int QST = SETS ? 32 : 64;
     constexpr int ROWQ = (KS * 32 + KPAD_LDS) / 8;
     constexpr int KQ = KS * 4;
     __shared__ uint4 sA[2][QST * ROWQ];
