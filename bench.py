@@ -422,8 +422,8 @@ def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_sample_q):
 # plain 1 GiB device copy gives, bench.py --copy-calibration).  Keyed by
 # (workload, form); other configurations report null.
 PROFILED_TRAFFIC = {
-    ("plain_gpu", "screened"): {"kernel": "k_ip_coarse", "bytes": 3.449e8, "fetch_kib": 119473.2,
-                                "write_kib": 97848.5, "profile": "profiles/k_ip_coarse_100k_x_20k_d150_k50.md"},
+    ("plain_gpu", "screened"): {"kernel": "k_ip_coarse", "bytes": 2.529e8, "fetch_kib": 74559.9,
+                                "write_kib": 97847.8, "profile": "profiles/k_ip_coarse_100k_x_20k_d150_k50.md"},
 }
 
 
