@@ -160,3 +160,30 @@ def test_sort_kth_rank_host_matches_oracle(orc):
         assert np.array_equal(kth, okth)
         assert sorted(ids.tolist()) == list(range(300))
         assert np.array_equal(np.array([np.searchsorted(accu[i], topk) for i in ids]), kth)
+
+
+def test_query_distribution_and_rank_files_roundtrip(tmp_path):
+    """qdibc's three files (FileIO.hpp:56-137) and fsr's rank file (SampleSearch.hpp:158-194)."""
+    from rkr import formats
+    rng = np.random.default_rng(9)
+    Q, I = 7, 30
+    ids = rng.choice(I, Q, replace=False).astype(np.int32)
+    accu = np.cumsum(rng.integers(0, 3, size=(Q, I + 1)), axis=1).astype(np.int32)
+    kth = np.sort(rng.integers(0, I, Q)).astype(np.int32)
+    d = formats.query_distribution_dir(str(tmp_path), "ds", Q, 5)
+    assert d.endswith(os.path.join("query_distribution", "ds-n_sample_item_7-sample_topk_5"))
+    formats.write_query_distribution(d, ids, kth, accu)
+    assert os.path.getsize(os.path.join(d, "sort_kth_rank_l.index")) == 4 * Q
+    assert os.path.getsize(os.path.join(d, "accu_n_user_rank_l.index")) == 4 * Q * (I + 1)
+    assert open(os.path.join(d, "sample_itemID_l.txt")).read().split() == [str(i) for i in ids]
+    i2, k2, a2 = formats.read_query_distribution(d, I)
+    assert np.array_equal(i2, ids) and np.array_equal(k2, kth) and np.array_equal(a2, accu)
+    with pytest.raises(ValueError):
+        formats.read_query_distribution(d, I + 1)
+    rp = formats.qs_ranks_path(str(tmp_path), "ds", 4, Q, 5)
+    assert os.path.basename(rp) == "QS-ds-n_sample_4-n_sample_query_7-sample_topk_5.index"
+    os.makedirs(os.path.dirname(rp), exist_ok=True)
+    formats.write_ranks(rp, [0, 3, 9, 20], I, 100)
+    assert os.path.getsize(rp) == 3 * 8 + 4 * 4
+    r, n_item, n_user = formats.read_ranks(rp)
+    assert r.tolist() == [0, 3, 9, 20] and (n_item, n_user) == (I, 100)
