@@ -66,6 +66,12 @@ typedef struct rkr_ctx rkr_ctx;
                                    D that fits 4096 columns; 1: 0..255, every 8th
                                    to 1023, every 32nd to 4095; 0: none.  Answers
                                    are unchanged; refinement shrinks.            */
+#define RKR_OPT_QUERY_SLOTS 10  /* 2 (default): rkr_query calls of >= 512 queries
+                                   that span two or more device batches run in two
+                                   parts on two streams, each driven by its own host
+                                   thread, so one part's host work between kernels
+                                   overlaps the other's kernels; 1: one stream.
+                                   Answers are unchanged.                        */
 #define RKR_OPT_COARSE 9        /* 1 (default): screened form, exact output: the
                                    batch's <u,q> runs on BF16 matrix cores against
                                    7 guard columns per user, and only the users

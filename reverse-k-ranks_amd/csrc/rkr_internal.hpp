@@ -67,6 +67,7 @@ struct Ctx {
     int32_t screen_rows = 0;    // refine rows per wavefront of the screen (0 = 32)
     int dedup = 1;              // answer byte-equal queries of one call once
     int coarse = 1;             // coarse bounds on matrix cores (RKR_OPT_COARSE)
+    int query_slots = 2;        // rkr_query halves on main + aux (RKR_OPT_QUERY_SLOTS)
     int32_t ins_gmax[8] = {};   // see DataSnap::ins_gmax (kept by the update path)
 
     // data
@@ -132,6 +133,8 @@ struct Ctx {
 
     // query scratch of rkr_query (grown on demand) and build scratch
     Slot main;
+    Slot aux;                   // second stream of split rkr_query calls, made on first use
+    bool aux_ready = false;
     DeviceBuf misc;
 
     // buffers an update thread freed while batches may still read them

@@ -35,6 +35,7 @@ RKR_OPT_DEDUP = 6
 RKR_OPT_SCREEN_ROWS = 7
 RKR_OPT_DENSE_TOP = 8
 RKR_OPT_COARSE = 9
+RKR_OPT_QUERY_SLOTS = 10
 
 STATUS_NAMES = {
     RKR_OK: "RKR_OK", RKR_ERR_INVALID: "RKR_ERR_INVALID", RKR_ERR_STATE: "RKR_ERR_STATE",

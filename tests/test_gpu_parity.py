@@ -226,6 +226,33 @@ def test_save_load_roundtrip(tmp_path, gpu_index_factory):
     assert np.array_equal(idx.get_table(), T)
 
 
+@pytest.mark.parametrize("form,batch", [("screened", 300), ("plain", 300), ("screened", 150)])
+def test_two_query_slots_agree(orc, gpu_index_factory, form, batch):
+    """Calls of >= 512 queries run in two halves on two streams and host threads
+    (RKR_OPT_QUERY_SLOTS = 2, the default): answers, per-query counters and query
+    IDs equal the one-stream run and brute force, with duplicates, mixed k and
+    device batches of 300 or 150 (the split falls on a batch boundary)."""
+    _case(f"two query slots form={form} batch={batch}")
+    rng = np.random.default_rng(31)
+    U, I, d = 4000, 2500, 24
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = rng.standard_normal((I, d), dtype=np.float32) * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32)
+    queries = items[rng.integers(0, I, 700)]          # some byte-equal repeats
+    ks = rng.integers(1, 120, 700).astype(np.int32)
+    idx, _ = gpu_index_factory(users, items, 64)
+    idx.set_form(form)
+    idx.set_option(3, batch)  # RKR_OPT_QUERY_BATCH
+    two = idx.query(queries, ks, max_k=120)
+    idx.set_option(10, 1)  # RKR_OPT_QUERY_SLOTS: one stream
+    one = idx.query(queries, ks, max_k=120)
+    assert np.array_equal(two.users, one.users) and np.array_equal(two.ranks, one.ranks)
+    assert np.array_equal(two.qips, one.qips)
+    assert [s["query_id"] for s in two.stats] == list(range(700))
+    for key in ("n_candidate", "n_refine_user", "n_result_user", "n_prune_user"):
+        assert [s[key] for s in two.stats] == [s[key] for s in one.stats], key
+    _check_exact(orc, _normalized(orc, users), items, queries, two, ks)
+
+
 # ---- full score table (ComputeSortItemsBatch rows, ReadAll's index file) --------
 
 @pytest.mark.parametrize("U,I,d,seed", [
