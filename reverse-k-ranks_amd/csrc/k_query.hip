@@ -562,6 +562,25 @@ constexpr int CNT = 256;
 __device__ __forceinline__ float rule_dot(const float *__restrict__ a, const float *__restrict__ b, int32_t d) {
     float s = 0.0f;
     int32_t k = 0;
+    // 32 terms are loaded before they are summed: lanes of a wavefront read
+    // different rows, and taking a whole 128-byte stretch per lane at once
+    // uses each cache line fully before it can be evicted (the sum's order is
+    // unchanged)
+    for (; k + 32 <= d; k += 32) {
+        float4 x[8], y[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            x[t] = *reinterpret_cast<const float4 *>(a + k + 4 * t);
+            y[t] = *reinterpret_cast<const float4 *>(b + k + 4 * t);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            s = s + x[t].x * y[t].x;
+            s = s + x[t].y * y[t].y;
+            s = s + x[t].z * y[t].z;
+            s = s + x[t].w * y[t].w;
+        }
+    }
     for (; k + 4 <= d; k += 4) {
         const float4 x = *reinterpret_cast<const float4 *>(a + k);
         const float4 y = *reinterpret_cast<const float4 *>(b + k);
