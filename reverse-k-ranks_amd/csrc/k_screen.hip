@@ -295,7 +295,28 @@ __global__ void k_recheck(const float *__restrict__ users, const float *__restri
         const float *p = items + (int64_t)rc.y * ld;
         float s = 0.0f;
         double s64 = 0.0;
-        for (int k = 0; k < K; k += 4) {
+        int k = 0;
+        // 32 terms per load burst: each lane reads its own rows, so a whole
+        // 128-byte stretch is taken before its cache lines can be evicted
+        for (; k + 32 <= K; k += 32) {
+            float4 a[8], b[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                a[t] = *reinterpret_cast<const float4 *>(u + k + 4 * t);
+                b[t] = *reinterpret_cast<const float4 *>(p + k + 4 * t);
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const float av[4] = {a[t].x, a[t].y, a[t].z, a[t].w}, bv[4] = {b[t].x, b[t].y, b[t].z, b[t].w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float prod = av[j] * bv[j];
+                    s = s + prod;
+                    s64 += (double)av[j] * (double)bv[j];
+                }
+            }
+        }
+        for (; k < K; k += 4) {
             const float4 a = *reinterpret_cast<const float4 *>(u + k);
             const float4 b = *reinterpret_cast<const float4 *>(p + k);
             const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
@@ -323,8 +344,8 @@ __global__ void k_recheck(const float *__restrict__ users, const float *__restri
 // values of u's table row, T[u][col_0] > T[u][col_1] > ... (col ascending):
 //   c_lo = #{g : guard_g > s^ - e},  c_hi = #{g : guard_g > s^ + e}
 // and cb[row(b) * ldc + u] = c_lo | c_hi << 4.  abs_floor also carries the
-// table's own eps.  Queries are the A operand (staged through LDS, 32 per
-// stage), users the B operand held in VGPRs: lane (lr, lq) of a 16x16 tile
+// table's own eps.  Queries are the A operand (staged through LDS, 64 per
+// stage in the first pass, 32 with guard sets), users the B operand held in VGPRs: lane (lr, lq) of a 16x16 tile
 // owns user lr and queries 4 lq .. 4 lq + 3, so its guards stay in registers.  qmap (optional) maps the
 // launch's queries to rows of q_hi / q_lo / q_norm and cb.
 struct IpCoarseParams {
