@@ -1338,8 +1338,16 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                                            (mode == RKR_OUTPUT_EXACT && eps == 0.0f) ? 1 : 0, cu, ghist, qi));
         }
         RKR_HIP(c, launch_offsets(sl.stream, qi, nq, refine_total));
-        RKR_HIP(c, hipMemcpyAsync(hqi, qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
-        RKR_HIP(c, hipStreamSynchronize(sl.stream));
+        // Coarse bounds without pending updates: every candidate is a listed
+        // pair, so the list length bounds the candidate and refine arrays and
+        // the batch goes on without waiting for the counts (the refinement and
+        // top-k read them on the device).  Otherwise the counts size them.
+        const bool sized_by_list = coarse_b && !upd;
+        int64_t total = n_listed, total_cand = n_listed;
+        if (!sized_by_list) {
+            RKR_HIP(c, hipMemcpyAsync(hqi, qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
+            RKR_HIP(c, hipStreamSynchronize(sl.stream));
+        }
         if (!deep_check.empty()) {
             // a user below the deepest guard has rank >= r[j_last] + 1 - n_del:
             // the selection over the listed users is exact when the k-th value
@@ -1351,9 +1359,11 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                     tm.n_fallback += 1;
                 }
         }
-        const int64_t total = hqi[nq - 1].cand_off + hqi[nq - 1].n_settled + hqi[nq - 1].n_cand;
-        int64_t total_cand = 0;
-        for (int32_t i = 0; i < nq; ++i) total_cand += hqi[i].n_cand;
+        if (!sized_by_list) {
+            total = hqi[nq - 1].cand_off + hqi[nq - 1].n_settled + hqi[nq - 1].n_cand;
+            total_cand = 0;
+            for (int32_t i = 0; i < nq; ++i) total_cand += hqi[i].n_cand;
+        }
         if (total >= INT32_MAX) return set_err(c, RKR_ERR_NOMEM, "query: candidate count exceeds 2^31; lower the query batch");
         const size_t cap = (size_t)std::max<int64_t>(total, 1);
         const size_t rcap = (size_t)std::max<int64_t>(total_cand, 1);
