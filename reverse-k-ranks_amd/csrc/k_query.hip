@@ -727,7 +727,7 @@ __global__ void __launch_bounds__(NT) k_coarse_chunk_offsets(const uint32_t *__r
 
 // One workgroup: cbase[ch] = exclusive scan of ctot (chunk-major list).
 __global__ void __launch_bounds__(NT) k_coarse_chunk_base(const int32_t *__restrict__ ctot, int32_t n_chunk,
-                                                          int32_t *__restrict__ cbase) {
+                                                          int32_t *__restrict__ cbase, int32_t *__restrict__ total) {
     __shared__ uint32_t red[NW + 1];
     uint32_t base = 0;
     for (int32_t c0 = 0; c0 < n_chunk; c0 += NT) {
@@ -737,6 +737,21 @@ __global__ void __launch_bounds__(NT) k_coarse_chunk_base(const int32_t *__restr
         const uint32_t e = wg_excl_scan(v, red, &tot);
         if (i < n_chunk) cbase[i] = (int32_t)(base + e);
         base += tot;
+    }
+    if (total && threadIdx.x == 0) *total = (int32_t)base;  // the list's length
+}
+
+// One workgroup: a query whose coarse record has no k-th bound, a list over
+// cap, or a list longer than the bound the host put in field 2 (its first
+// pass's length) is sent to the full path (beta = -1, nothing listed).
+__global__ void __launch_bounds__(NT) k_coarse_mark(int32_t *__restrict__ cq, int32_t nq, int64_t cap) {
+    for (int32_t b = threadIdx.x; b < nq; b += NT) {
+        int32_t *e = cq + (int64_t)b * 4;
+        if (e[0] < 0 || e[1] > cap || e[1] > e[2]) {
+            e[0] = -1;
+            e[1] = 0;
+        }
+        e[3] = 0;
     }
 }
 
@@ -809,10 +824,11 @@ __global__ void __launch_bounds__(CNT) k_coarse_exact(const float *__restrict__ 
                                                       const GuardCols *__restrict__ gsets,
                                                       const int32_t *__restrict__ gset, const int32_t *__restrict__ cl_u,
                                                       const int32_t *__restrict__ cl_b, int64_t n,
+                                                      const int32_t *__restrict__ n_dev,
                                                       uint32_t *__restrict__ bk_c, float *__restrict__ s_c,
                                                       uint32_t *__restrict__ ghist) {
     const int64_t i = (int64_t)blockIdx.x * CNT + threadIdx.x;
-    if (i >= n) return;
+    if (i >= (n_dev ? min(n, (int64_t)*n_dev) : n)) return;
     const int64_t u = cl_u[i];
     const uint32_t packed = (uint32_t)cl_b[i];
     const int64_t b = packed & 0xFFFFu;
@@ -889,7 +905,8 @@ __global__ void __launch_bounds__(CNT) k_coarse_exact_upd(
 // candidate goes to qip for the top-k.
 __global__ void __launch_bounds__(CNT) k_coarse_final(const int32_t *__restrict__ cl_u, const int32_t *__restrict__ cl_b,
                                                       const uint32_t *__restrict__ bk_c, const float *__restrict__ s_c,
-                                                      int64_t n, int32_t S, const int32_t *__restrict__ ranks,
+                                                      int64_t n, const int32_t *__restrict__ n_dev, int32_t S,
+                                                      const int32_t *__restrict__ ranks,
                                                       int64_t n_item, int64_t n_user, QueryInfo *__restrict__ qi,
                                                       int32_t *__restrict__ cand_user, int32_t *__restrict__ cand_rank,
                                                       int32_t *__restrict__ refine_total,
@@ -898,7 +915,7 @@ __global__ void __launch_bounds__(CNT) k_coarse_final(const int32_t *__restrict_
                                                       int32_t *__restrict__ refine_query,
                                                       float *__restrict__ refine_thr, float *__restrict__ qip) {
     const int64_t i = (int64_t)blockIdx.x * CNT + threadIdx.x;
-    if (i >= n) return;
+    if (i >= (n_dev ? min(n, (int64_t)*n_dev) : n)) return;
     const int32_t b = cl_b[i] & 0xFFFF;
     const uint32_t x = bk_c[i];
     const int32_t blo = (int32_t)(x & 0xFFFFu), bhi = (int32_t)(x >> 16);
@@ -1009,13 +1026,13 @@ hipError_t launch_coarse_kth(hipStream_t s, const uint8_t *cb, int64_t ldc, int6
 
 hipError_t launch_coarse_compact(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq,
                                  int64_t cu, const uint32_t *part, const int32_t *cq, int32_t *loc, int32_t *ctot,
-                                 int32_t *cbase, int32_t *cl_u, int32_t *cl_b) {
+                                 int32_t *cbase, int32_t *cl_u, int32_t *cl_b, int32_t *total) {
     if (nq <= 0 || n_user <= 0) return hipSuccess;
     if (ldc % 16 || cu % 1024 || nq > 65535) return hipErrorInvalidValue;
     const int32_t n_chunk = (int32_t)((n_user + cu - 1) / cu);
     hipLaunchKernelGGL(k_coarse_chunk_offsets, dim3((unsigned)n_chunk), dim3(NT), 0, s, part, n_chunk, nq, cq, loc,
                        ctot);
-    hipLaunchKernelGGL(k_coarse_chunk_base, dim3(1), dim3(NT), 0, s, ctot, n_chunk, cbase);
+    hipLaunchKernelGGL(k_coarse_chunk_base, dim3(1), dim3(NT), 0, s, ctot, n_chunk, cbase, total);
     hipLaunchKernelGGL(k_coarse_compact, dim3((unsigned)n_chunk, (unsigned)nq), dim3(CNT), 0, s, cb, ldc, n_user, cu,
                        nq, cq, loc, cbase, cl_u, cl_b);
     return hipGetLastError();
@@ -1024,15 +1041,15 @@ hipError_t launch_coarse_compact(hipStream_t s, const uint8_t *cb, int64_t ldc, 
 hipError_t launch_coarse_exact(hipStream_t s, const float *users, const float *queries, int32_t ld, int32_t dim,
                                const float *table, int32_t S, const int32_t *ranks, int64_t n_item, float eps,
                                const GuardCols *gsets, const int32_t *gset, int32_t nq, const int32_t *cl_u,
-                               const int32_t *cl_b, int64_t n, uint32_t *bk_c, float *s_c, uint32_t *ghist,
-                               QueryInfo *qi) {
+                               const int32_t *cl_b, int64_t n, const int32_t *n_dev, uint32_t *bk_c, float *s_c,
+                               uint32_t *ghist, QueryInfo *qi) {
     if (nq <= 0) return hipSuccess;
     const int same = eps == 0.0f ? 1 : 0;
     hipError_t e = hipMemsetAsync(ghist, 0, sizeof(uint32_t) * 2 * (size_t)(S + 1) * nq, s);
     if (e != hipSuccess) return e;
     if (n > 0)
         hipLaunchKernelGGL(k_coarse_exact, dim3((unsigned)((n + CNT - 1) / CNT)), dim3(CNT), 0, s, users, queries, ld,
-                           dim, table, S, eps, same, gsets, gset, cl_u, cl_b, n, bk_c, s_c, ghist);
+                           dim, table, S, eps, same, gsets, gset, cl_u, cl_b, n, n_dev, bk_c, s_c, ghist);
     hipLaunchKernelGGL(k_hist_kth, dim3((unsigned)nq), dim3(NT), 0, s, ghist, S, same, ranks, n_item,
                        (int)RKR_OUTPUT_EXACT, qi);
     return hipGetLastError();
@@ -1051,14 +1068,21 @@ hipError_t launch_coarse_exact_upd(hipStream_t s, const float *users, const floa
     return hipGetLastError();
 }
 
+hipError_t launch_coarse_mark(hipStream_t s, int32_t *cq, int32_t nq, int64_t cap) {
+    if (nq <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_coarse_mark, dim3(1), dim3(NT), 0, s, cq, nq, cap);
+    return hipGetLastError();
+}
+
 hipError_t launch_coarse_final(hipStream_t s, const int32_t *cl_u, const int32_t *cl_b, const uint32_t *bk_c,
-                               const float *s_c, int64_t n, int32_t S, const int32_t *ranks, int64_t n_item,
+                               const float *s_c, int64_t n, const int32_t *n_dev, int32_t S, const int32_t *ranks,
+                               int64_t n_item,
                                int64_t n_user, QueryInfo *qi, int32_t *cand_user, int32_t *cand_rank,
                                int32_t *refine_total, int32_t *refine_cand, int32_t *refine_user,
                                int32_t *refine_query, float *refine_thr, float *qip) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_coarse_final, dim3((unsigned)((n + CNT - 1) / CNT)), dim3(CNT), 0, s, cl_u, cl_b, bk_c, s_c, n,
-                       S, ranks, n_item, n_user, qi, cand_user, cand_rank, refine_total, refine_cand, refine_user,
+                       n_dev, S, ranks, n_item, n_user, qi, cand_user, cand_rank, refine_total, refine_cand, refine_user,
                        refine_query, refine_thr, qip);
     return hipGetLastError();
 }

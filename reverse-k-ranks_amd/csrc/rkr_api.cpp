@@ -1146,7 +1146,10 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
         int32_t *cl_u = nullptr, *cl_b = nullptr;
         uint32_t *bk_c = nullptr;
         float *s_c = nullptr;
-        int64_t n_listed = 0;
+        int64_t n_listed = 0;             // the list length, or a bound on it when dev_lists
+        bool dev_lists = false;           // the lists' true length and fallbacks are on the device
+        int32_t *d_list_n = nullptr;      // the list length (device), when dev_lists
+        int32_t *cq_dev = nullptr;        // the coarse per-query records (device)
         std::vector<int32_t> deep_check;  // update path: queries listed without the deepest bin
         if (coarse_b) {
             // coarse buckets on the matrix cores, the coarse k-th bound and
@@ -1179,6 +1182,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                 return rc;
             uint32_t *part = reinterpret_cast<uint32_t *>(sl.coarse_info.p);
             int32_t *cq = reinterpret_cast<int32_t *>(part + 16 * (size_t)nq * n_chunk);
+            cq_dev = cq;
             int32_t *loc = cq + 4 * (size_t)nqb;
             int32_t *ctot = loc + (size_t)nq * n_chunk;
             int32_t *cbase = ctot + n_chunk;
@@ -1250,24 +1254,40 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                 groups.push_back({(int32_t)sets.size(), (int32_t)first});
                 sets.push_back(g2);
             }
-            if (!groups.empty()) {
-                // one launch for every refined query: set of each (local)
-                // query in qset, the sets themselves after set 0
+            // With a second pass the lists stay on the device: the batch goes
+            // on without reading the new counts.  A second-pass list is a
+            // subset of the query's first-pass list (its guards lie inside the
+            // bin that held the k-th bound), so the first-pass lengths bound
+            // the list; k_coarse_mark sends a query whose list exceeds its
+            // bound or the cap to the full path, and the final counts and
+            // those queries are read with the outputs.
+            dev_lists = !groups.empty();
+            if (dev_lists) {
                 const int32_t nr = (int32_t)hqmap.size();
+                for (int32_t i = 0; i < nq; ++i) {
+                    int32_t *e = &hcq[(size_t)i * 4];
+                    const int32_t b1 = e[0] >= 0 ? (int32_t)std::min<int64_t>(e[1], coarse_cap) : 0;
+                    e[2] = b1;  // field 2 carries the bound to k_coarse_mark
+                    n_listed += b1;
+                }
+                // one launch for every refined query: set of each (local)
+                // query in qset; every set is uploaded once, set 0 (the first
+                // pass's guards) first, and the second pass reads from set 1
                 int32_t *h_map = hgset + nqb, *h_set = h_map + nr;
                 for (int32_t t = 0; t < nr; ++t) {
                     h_map[t] = hqmap[t];
                     h_set[t] = hgset[hqmap[t]] - 1;
                 }
-                for (size_t t = 1; t < sets.size(); ++t) hsets[t - 1] = sets[t];
+                for (size_t t = 0; t < sets.size(); ++t) hsets[t] = sets[t];
                 int32_t *qset = qmap + nr;
+                RKR_HIP(c, hipMemcpyAsync(cq, hcq, sizeof(int32_t) * 4 * nq, hipMemcpyHostToDevice, sl.stream));
                 RKR_HIP(c, hipMemcpyAsync(qmap, h_map, sizeof(int32_t) * 2 * nr, hipMemcpyHostToDevice, sl.stream));
-                RKR_HIP(c, hipMemcpyAsync(gsets, hsets, sizeof(GuardCols) * (sets.size() - 1), hipMemcpyHostToDevice,
+                RKR_HIP(c, hipMemcpyAsync(gsets, hsets, sizeof(GuardCols) * sets.size(), hipMemcpyHostToDevice,
                                           sl.stream));
                 CoarseLaunch L2 = L;
                 L2.qmap = qmap;
                 L2.nq = nr;
-                L2.sets = gsets;
+                L2.sets = gsets + 1;
                 L2.qset = qset;
                 L2.n_sets = (int32_t)sets.size() - 1;
                 RKR_HIP(c, launch_ip_coarse(sl.stream, L2));
@@ -1277,34 +1297,39 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                     RKR_HIP(c, launch_coarse_kth(sl.stream, cb, ldc, U, end - first, ccu, sets[groups[gi].first].n,
                                                  qmap + first, part, qi, cq, hcnt));
                 }
-                RKR_HIP(c, hipMemcpyAsync(hcq, cq, sizeof(int32_t) * 4 * nq, hipMemcpyDeviceToHost, sl.stream));
-                RKR_HIP(c, hipStreamSynchronize(sl.stream));
+                RKR_HIP(c, launch_coarse_mark(sl.stream, cq, nq, coarse_cap));
                 tm.n_coarse_refined += (int64_t)hqmap.size();
-            }
-            for (int32_t i = 0; i < nq; ++i) {
-                int32_t *e = &hcq[(size_t)i * 4];
-                if (e[0] < 0 || e[1] > coarse_cap) {
-                    if (fallback) fallback->push_back((int32_t)(q0 + i));
-                    tm.n_fallback += 1;
-                    e[0] = -1;
-                    e[1] = 0;
+            } else {
+                for (int32_t i = 0; i < nq; ++i) {
+                    int32_t *e = &hcq[(size_t)i * 4];
+                    if (e[0] < 0 || e[1] > coarse_cap) {
+                        if (fallback) fallback->push_back((int32_t)(q0 + i));
+                        tm.n_fallback += 1;
+                        e[0] = -1;
+                        e[1] = 0;
+                    }
+                    e[2] = (int32_t)n_listed;
+                    e[3] = 0;
+                    n_listed += e[1];
                 }
-                e[2] = (int32_t)n_listed;
-                e[3] = 0;
-                n_listed += e[1];
+                tm.n_listed += n_listed;
             }
             if (n_listed >= INT32_MAX) return set_err(c, RKR_ERR_NOMEM, "query: coarse list exceeds 2^31; lower the query batch");
-            tm.n_listed += n_listed;
             if ((rc = ensure(c, sl.coarse_list, 16 * (size_t)std::max<int64_t>(n_listed, 1), "coarse list"))) return rc;
             cl_u = reinterpret_cast<int32_t *>(sl.coarse_list.p);
             cl_b = cl_u + std::max<int64_t>(n_listed, 1);
             bk_c = reinterpret_cast<uint32_t *>(cl_b + std::max<int64_t>(n_listed, 1));
             s_c = reinterpret_cast<float *>(bk_c + std::max<int64_t>(n_listed, 1));
-            for (size_t t = 0; t < sets.size(); ++t) hsets[t] = sets[t];
-            RKR_HIP(c, hipMemcpyAsync(cq, hcq, sizeof(int32_t) * 4 * nq, hipMemcpyHostToDevice, sl.stream));
-            RKR_HIP(c, hipMemcpyAsync(gsets, hsets, sizeof(GuardCols) * sets.size(), hipMemcpyHostToDevice, sl.stream));
+            if (!dev_lists) {
+                for (size_t t = 0; t < sets.size(); ++t) hsets[t] = sets[t];
+                RKR_HIP(c, hipMemcpyAsync(cq, hcq, sizeof(int32_t) * 4 * nq, hipMemcpyHostToDevice, sl.stream));
+                RKR_HIP(c, hipMemcpyAsync(gsets, hsets, sizeof(GuardCols) * sets.size(), hipMemcpyHostToDevice,
+                                          sl.stream));
+            }
             RKR_HIP(c, hipMemcpyAsync(gset, hgset, sizeof(int32_t) * nq, hipMemcpyHostToDevice, sl.stream));
-            RKR_HIP(c, launch_coarse_compact(sl.stream, cb, ldc, U, nq, ccu, part, cq, loc, ctot, cbase, cl_u, cl_b));
+            d_list_n = dev_lists ? hcnt + 8 * (size_t)nqb : nullptr;  // in the spare words of coarse_info
+            RKR_HIP(c, launch_coarse_compact(sl.stream, cb, ldc, U, nq, ccu, part, cq, loc, ctot, cbase, cl_u, cl_b,
+                                             d_list_n));
             if (upd) {
                 // bound values of the listed pairs only (the rest are ruled
                 // out), then the value selection of the update path; the
@@ -1317,7 +1342,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                 RKR_HIP(c, launch_kth_values(sl.stream, lbv, ubv, U, nq, cu, ghist, qi));
             } else {
                 RKR_HIP(c, launch_coarse_exact(sl.stream, dv.users, q_in, ld, d, T, S, R, I, eps, gsets, gset, nq, cl_u,
-                                               cl_b, n_listed, bk_c, s_c, ghist, qi));
+                                               cl_b, n_listed, d_list_n, bk_c, s_c, ghist, qi));
             }
             RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
         } else {
@@ -1384,7 +1409,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
         const bool screened = c->form == RKR_FORM_SCREENED;
 
         if (coarse_b && !upd)
-            RKR_HIP(c, launch_coarse_final(sl.stream, cl_u, cl_b, bk_c, s_c, n_listed, S, R, I, U, qi, cand_user,
+            RKR_HIP(c, launch_coarse_final(sl.stream, cl_u, cl_b, bk_c, s_c, n_listed, d_list_n, S, R, I, U, qi, cand_user,
                                            cand_rank, refine_total, refine_cand, refine_user, refine_query, refine_thr,
                                            qip));
         else if (upd)
@@ -1441,6 +1466,11 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             RKR_HIP(c, hipMemcpyAsync(ho_user, o_user, 4 * out_words, hipMemcpyDeviceToHost, sl.stream));
             RKR_HIP(c, hipMemcpyAsync(hqi, qi, sizeof(QueryInfo) * nq, hipMemcpyDeviceToHost, sl.stream));
             RKR_HIP(c, hipMemcpyAsync(&h_refine_total, refine_total, sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream));
+            if (dev_lists) {
+                // the lists' length and the queries k_coarse_mark sent to the full path
+                RKR_HIP(c, hipMemcpyAsync(hcq, cq_dev, sizeof(int32_t) * 4 * nq, hipMemcpyDeviceToHost, sl.stream));
+                RKR_HIP(c, hipMemcpyAsync(&h_scalars[3], d_list_n, sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream));
+            }
             if (screened) {
                 // q_recheck, recheck_n, max_err_bits are adjacent in sl.small
                 RKR_HIP(c, hipMemcpyAsync(h_qrecheck, q_recheck, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, sl.stream));
@@ -1457,6 +1487,14 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             sl.recheck.bytes = 0;
             if ((rc = ensure(c, sl.recheck, need, "recheck list"))) return rc;
             if (attempt == 2) return set_err(c, RKR_ERR_NOMEM, "query: recheck list kept overflowing");
+        }
+        if (dev_lists) {
+            for (int32_t i = 0; i < nq; ++i)
+                if (hcq[(size_t)i * 4] < 0) {
+                    if (fallback) fallback->push_back((int32_t)(q0 + i));
+                    tm.n_fallback += 1;
+                }
+            tm.n_listed += h_scalars[3];
         }
 
         std::memcpy(out_user + (size_t)q0 * max_k, ho_user, sizeof(int32_t) * (size_t)nq * max_k);

@@ -436,17 +436,23 @@ hipError_t launch_coarse_exact_upd(hipStream_t s, const float *users, const floa
 // max over users of #{inserted scores > T[u][gc.col[j]]} into out[j] (zeroed here)
 hipError_t launch_ins_guard_max(hipStream_t s, const float *table, int64_t n_user, int32_t S, const GuardCols &gc,
                                 const float *ins, int32_t n_ins, int32_t ld_ins, int32_t *out);
-// chunk-major list: loc (n_chunk x nq), ctot and cbase (n_chunk) are scratch
+// chunk-major list: loc (n_chunk x nq), ctot and cbase (n_chunk) are scratch;
+// total (optional) receives the list's length
 hipError_t launch_coarse_compact(hipStream_t s, const uint8_t *cb, int64_t ldc, int64_t n_user, int32_t nq,
                                  int64_t cu, const uint32_t *part, const int32_t *cq, int32_t *loc, int32_t *ctot,
-                                 int32_t *cbase, int32_t *cl_u, int32_t *cl_b);
+                                 int32_t *cbase, int32_t *cl_u, int32_t *cl_b, int32_t *total);
+// cq[b] = {-1, 0, -, 0} for a query with no k-th bound, a count over cap or a
+// count over the bound in field 2 (the full path answers it)
+hipError_t launch_coarse_mark(hipStream_t s, int32_t *cq, int32_t nq, int64_t cap);
 hipError_t launch_coarse_exact(hipStream_t s, const float *users, const float *queries, int32_t ld, int32_t dim,
                                const float *table, int32_t S, const int32_t *ranks, int64_t n_item, float eps,
                                const GuardCols *gsets, const int32_t *gset, int32_t nq, const int32_t *cl_u,
-                               const int32_t *cl_b, int64_t n, uint32_t *bk_c, float *s_c, uint32_t *ghist,
-                               QueryInfo *qi);
+                               const int32_t *cl_b, int64_t n, const int32_t *n_dev, uint32_t *bk_c, float *s_c,
+                               uint32_t *ghist, QueryInfo *qi);
+// n_dev (optional): the list's length on the device (at most n; n sizes the grid)
 hipError_t launch_coarse_final(hipStream_t s, const int32_t *cl_u, const int32_t *cl_b, const uint32_t *bk_c,
-                               const float *s_c, int64_t n, int32_t S, const int32_t *ranks, int64_t n_item,
+                               const float *s_c, int64_t n, const int32_t *n_dev, int32_t S, const int32_t *ranks,
+                               int64_t n_item,
                                int64_t n_user, QueryInfo *qi, int32_t *cand_user, int32_t *cand_rank,
                                int32_t *refine_total, int32_t *refine_cand, int32_t *refine_user,
                                int32_t *refine_query, float *refine_thr, float *qip);
