@@ -659,12 +659,20 @@ int rkr_get_table(rkr_ctx *c, int64_t user_begin, int64_t n_rows, float *out) {
 
 extern "C++" {
 namespace {
-// Device rows freed when the scope ends.
-struct ScopedRows {
+// A device allocation freed when the scope ends.
+struct ScopedDev {
     rkr::Ctx *c;
-    float *p = nullptr;
+    void *p = nullptr;
     size_t bytes = 0;
-    ~ScopedRows() { rkr::dev_free(c, p, bytes); }
+    int alloc(size_t n, const char *what) {
+        rkr::dev_free(c, p, bytes);
+        p = nullptr;
+        bytes = n;
+        return rkr::dev_alloc(c, &p, n, what);
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+    ~ScopedDev() { rkr::dev_free(c, p, bytes); }
 };
 }  // namespace
 
@@ -682,20 +690,20 @@ static int sorted_score_batches(rkr_ctx *c, int64_t u0, int64_t n, const char *w
     if (c->build_batch > 0) nb = std::min<int64_t>(nb, c->build_batch);
     nb = std::min<int64_t>(nb, n);
     if (nb <= 0) return set_err(c, RKR_ERR_NOMEM, std::string(what) + ": no memory for a score block");
-    ScopedRows blk{c}, tmp{c};
-    blk.bytes = tmp.bytes = (size_t)nb * row_bytes;
-    int rc = dev_alloc_t(c, &blk.p, (size_t)nb * I, "score block");
-    if (!rc) rc = dev_alloc_t(c, &tmp.p, (size_t)nb * I, "sort scratch");
+    ScopedDev blk_buf{c}, tmp_buf{c};
+    int rc = blk_buf.alloc((size_t)nb * row_bytes, "score block");
+    if (!rc) rc = tmp_buf.alloc((size_t)nb * row_bytes, "sort scratch");
     if (rc) return rc;
+    float *blk = blk_buf.as<float>(), *tmp = tmp_buf.as<float>();
     for (int64_t b0 = 0; b0 < n; b0 += nb) {
         const int64_t m = std::min<int64_t>(nb, n - b0);
         RKR_HIP(c, hipEventRecord(c->ev[0], c->stream));
         RKR_HIP(c, launch_scores_store(c->stream, c->users + (u0 + b0) * c->ld, nullptr, m, c->ld, c->items, I, c->ld,
-                                       c->dim, blk.p, I, true));
+                                       c->dim, blk, I, true));
         RKR_HIP(c, hipEventRecord(c->ev[1], c->stream));
-        RKR_HIP(c, launch_sort_rows(c->stream, blk.p, I, m, I, blk.p, tmp.p, I));
+        RKR_HIP(c, launch_sort_rows(c->stream, blk, I, m, I, blk, tmp, I));
         RKR_HIP(c, hipEventRecord(c->ev[2], c->stream));
-        if ((rc = sink(blk.p, u0 + b0, m))) return rc;  // the sink synchronises the stream
+        if ((rc = sink(blk, u0 + b0, m))) return rc;  // the sink synchronises the stream
         if (gemm_ms) *gemm_ms += ev_ms(c->ev, 0, 1);
         if (sort_ms) *sort_ms += ev_ms(c->ev, 1, 2);
     }
@@ -775,33 +783,26 @@ int rkr_query_rank_distribution(rkr_ctx *c, const int32_t *item_ids, int32_t n_s
     }
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
-    // device buffers of other element types are held as raw bytes
-    ScopedRows smat{c}, qbuf{c}, dids{c}, hist{c};
-    smat.bytes = sizeof(float) * (size_t)(ns * c->ld);
-    dids.bytes = sizeof(int64_t) * (size_t)ns;
-    hist.bytes = sizeof(int32_t) * (size_t)(ns * W);
-    int rc = dev_alloc(c, reinterpret_cast<void **>(&smat.p), smat.bytes, "sample items");
-    if (!rc) rc = dev_alloc(c, reinterpret_cast<void **>(&dids.p), dids.bytes, "sample ids");
-    if (!rc) rc = dev_alloc(c, reinterpret_cast<void **>(&hist.p), hist.bytes, "rank histogram");
+    ScopedDev smat{c}, qbuf{c}, dids{c}, hist{c};
+    int rc = smat.alloc(sizeof(float) * (size_t)(ns * c->ld), "sample items");
+    if (!rc) rc = dids.alloc(sizeof(int64_t) * (size_t)ns, "sample ids");
+    if (!rc) rc = hist.alloc(sizeof(int32_t) * (size_t)(ns * W), "rank histogram");
     if (rc) return rc;
-    int32_t *h = reinterpret_cast<int32_t *>(hist.p);
-    const int64_t *d_ids = reinterpret_cast<const int64_t *>(dids.p);
+    int32_t *h = hist.as<int32_t>();
     RKR_HIP(c, hipMemcpyAsync(dids.p, ids.data(), dids.bytes, hipMemcpyHostToDevice, c->stream));
     RKR_HIP(c, hipMemsetAsync(hist.p, 0, hist.bytes, c->stream));
-    RKR_HIP(c, launch_gather_rows(c->stream, c->items, c->ld, d_ids, ns, c->ld, smat.p, c->ld));
+    RKR_HIP(c, launch_gather_rows(c->stream, c->items, c->ld, dids.as<const int64_t>(), ns, c->ld, smat.as<float>(),
+                                  c->ld));
     double ms = 0;
     rc = sorted_score_batches(c, 0, c->n_user, "query_rank_distribution", [&](const float *rows, int64_t u, int64_t m) {
         if (qbuf.bytes < sizeof(float) * (size_t)(m * ns)) {  // the first batch is the largest
-            dev_free(c, qbuf.p, qbuf.bytes);
-            qbuf.p = nullptr;
-            qbuf.bytes = sizeof(float) * (size_t)(m * ns);
-            const int r2 = dev_alloc(c, reinterpret_cast<void **>(&qbuf.p), qbuf.bytes, "sample scores");
+            const int r2 = qbuf.alloc(sizeof(float) * (size_t)(m * ns), "sample scores");
             if (r2) return r2;
         }
         RKR_HIP(c, hipEventRecord(c->ev[3], c->stream));
-        RKR_HIP(c, launch_scores_store(c->stream, c->users + u * c->ld, nullptr, m, c->ld, smat.p, ns, c->ld, c->dim,
-                                       qbuf.p, ns, true));
-        RKR_HIP(c, launch_rank_hist(c->stream, rows, I, m, I, qbuf.p, ns, (int32_t)ns, h));
+        RKR_HIP(c, launch_scores_store(c->stream, c->users + u * c->ld, nullptr, m, c->ld, smat.as<float>(), ns, c->ld,
+                                       c->dim, qbuf.as<float>(), ns, true));
+        RKR_HIP(c, launch_rank_hist(c->stream, rows, I, m, I, qbuf.as<float>(), ns, (int32_t)ns, h));
         RKR_HIP(c, hipEventRecord(c->ev[4], c->stream));
         RKR_HIP(c, hipEventSynchronize(c->ev[4]));
         ms += ev_ms(c->ev, 0, 2) + ev_ms(c->ev, 3, 4);
