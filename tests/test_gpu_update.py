@@ -270,3 +270,32 @@ def test_coarse_bounds_with_updates(orc, gpu_index_factory, deletes):
         for res in (full, coarse):
             assert np.array_equal(res.users[i, :k], bu[i, :k]), f"weak query {i}: users differ"
             assert np.array_equal(res.ranks[i, :k], br[i, :k]), f"weak query {i}: ranks differ"
+
+
+@pytest.mark.parametrize("form", ["plain", "screened"])
+def test_two_query_slots_with_pending_updates(orc, gpu_index_factory, form):
+    """A call split over two streams (device batches of 200, 650 queries) with pending item
+    inserts and deletes and inserted users: both parts read the same snapshot of the side
+    arrays, so the answers equal the one-stream run and brute force."""
+    _case(f"two query slots with pending updates ({form})")
+    rng, users, items = _data(35)
+    idx, _ = gpu_index_factory(users, items, 40)
+    m = Mirror(orc, users, items)
+    new = rng.standard_normal((120, items.shape[1]), dtype=np.float32)
+    idx.insert_items(new)
+    m.insert_items(new)
+    idx.delete_items(np.array([1, 4, 9, 2100], np.int64))
+    m.delete_items([1, 4, 9, 2100])
+    nu = rng.standard_normal((300, items.shape[1]), dtype=np.float32)
+    idx.insert_users(nu)
+    m.insert_users(nu)
+    cur = m.items()
+    queries = cur[rng.integers(0, len(cur), 650)]
+    idx.set_option(3, 200)  # RKR_OPT_QUERY_BATCH: four device batches
+    res = m.check(idx, queries, 20, form)
+    idx.set_option(10, 1)   # RKR_OPT_QUERY_SLOTS: one stream
+    one = idx.query(queries, 20)
+    assert np.array_equal(res.users, one.users) and np.array_equal(res.ranks, one.ranks)
+    for bad in (0, 3):
+        with pytest.raises(Exception):
+            idx.set_option(10, bad)
