@@ -135,3 +135,44 @@ def test_cli_qs_pipeline_fake30d(orc, fake30d, tmp_path):
     got_r = np.loadtxt(f"{stem}-rank.csv", delimiter=",", dtype=np.int64)
     bu, br, _ = orc.bruteforce(un, items, queries, 10)
     assert np.array_equal(got_u, bu) and np.array_equal(got_r, br)
+
+
+def test_training_and_score_table_errors(orc, tmp_path):
+    """The new entry points fail with status codes, never exit or fault: bad row ranges, bad item
+    IDs, an unwritable path, a sample_topk above the user count, a bad method."""
+    _case("training and score table errors")
+    import rkr
+    from rkr import _lib as L
+    from rkr import train
+    rng, users, items = _data(41, 50, 300, 8)
+    idx = rkr.Index(0)
+    with pytest.raises(L.RkrError) as e:
+        idx.sorted_scores()  # no data yet
+    assert e.value.status == L.RKR_ERR_STATE
+    idx.set_data(users, items)
+    for u0, n in ((-1, 2), (49, 2), (0, 51)):
+        with pytest.raises(L.RkrError) as e:
+            idx.sorted_scores(u0, n)
+        assert e.value.status == L.RKR_ERR_INVALID
+    with pytest.raises(L.RkrError) as e:
+        idx.save_score_table(str(tmp_path / "missing_dir" / "x.index"))
+    assert e.value.status == L.RKR_ERR_IO
+    with pytest.raises(L.RkrError) as e:
+        train.query_rank_distribution(idx, np.array([0, 300], np.int32))
+    assert e.value.status == L.RKR_ERR_INVALID
+    accu, _ = train.query_rank_distribution(idx, np.arange(20, dtype=np.int32))
+    kth, order = train.sort_kth_rank(accu, 51)  # one more than the users: no k-th rank below n_item
+    assert np.all(kth == 301)
+    with pytest.raises(L.RkrError) as e:
+        train.find_optimal_sample(idx, 50, 300, kth, accu[order], 16)
+    assert e.value.status == L.RKR_ERR_INVALID
+    kth, order = train.sort_kth_rank(accu, 5)
+    with pytest.raises(KeyError):
+        train.find_optimal_sample(idx, 50, 300, kth, accu[order], 16, "OptimalAll")
+    out = np.empty(4, np.int32)
+    acc_s = np.ascontiguousarray(accu[order])
+    rc = idx.lib.rkr_find_optimal_sample(idx.ctx, 50, 300, 20, kth.ctypes.data, acc_s.ctypes.data, 7, 4,
+                                         out.ctypes.data, None)
+    assert rc == L.RKR_ERR_INVALID
+    # the context still answers after the errors
+    assert np.array_equal(idx.sorted_scores(0, 3), orc.sorted_score_rows(orc.normalize_users(users[:3]), items))
