@@ -413,7 +413,7 @@ def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_sample_q):
     dt = time.perf_counter() - t0
     agree = float(np.mean([set(ou[i]) == set(gpu_users[i, :k]) for i in range(len(q))]))
     return {"value": round(len(q) / dt, 4), "unit": "queries/s", "cores": cores, "kind": "port",
-            "sample": f"first {len(q)} of the same queries, same table (US path, OpenMP, -O2 no fast-math)",
+            "sample": f"first {len(q)} of the same queries, same table (US path, OpenMP, -O3 no fast-math)",
             "seconds": round(dt, 3), "user_set_agreement": agree}
 
 
