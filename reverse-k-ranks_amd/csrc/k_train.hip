@@ -40,23 +40,26 @@ __device__ inline int64_t tri_off(int64_t lb) { return lb * (lb - 1) / 2; }
 
 // T[tri_off(lb) + ub] = -End(ub) + Between(ub, lb) + End(lb) for ub < lb, where
 // Between(ub, lb) = sum over queries q in [pos[ub], pos[lb]) of
-// accu[q][rank_lb] - accu[q][rank_ub], read as differences of the column
-// prefix sums C[j][q] = sum_{q' < q} accu[q'][rank_j] (Q + 1 entries per j).
+// accu[q][rank_lb] - accu[q][rank_ub], read as differences of the prefix sums
+// S_j(x) = sum_{q' < x} accu[q'][rank_j], which C keeps at the breakpoints:
+// C[col[j'] * R + j] = S_j(pos[j']).  For one lb the threads read C's rows
+// col[lb] and col[ub] at consecutive ub.
 __global__ void __launch_bounds__(256) k_transit(const int64_t *__restrict__ C, int32_t Q,
-                                                 const int32_t *__restrict__ pos, const int64_t *__restrict__ end_cost,
-                                                 int32_t R, int64_t *__restrict__ T) {
+                                                 const int32_t *__restrict__ pos, const int32_t *__restrict__ col,
+                                                 const int64_t *__restrict__ end_cost, int32_t R,
+                                                 int64_t *__restrict__ T) {
     const int64_t lb = blockIdx.x;
     if (lb < 1 || lb >= R) return;
     const int32_t pe = pos[lb];
-    const int64_t *Cl = C + lb * (int64_t)(Q + 1);
+    const int64_t *Ce = C + (int64_t)col[lb] * R;
     const int64_t el = end_cost[lb];
     int64_t *Tl = T + tri_off(lb);
     for (int64_t ub = threadIdx.x; ub < lb; ub += blockDim.x) {
         const int32_t pb = pos[ub];
         int64_t between = 0;
         if (pb < Q && pb < pe) {
-            const int64_t *Cu = C + ub * (int64_t)(Q + 1);
-            between = (Cl[pe] - Cl[pb]) - (Cu[pe] - Cu[pb]);
+            const int64_t *Cb = C + (int64_t)col[ub] * R;
+            between = (Ce[lb] - Cb[lb]) - (Ce[ub] - Cb[ub]);
         }
         Tl[ub] = -end_cost[ub] + between + el;
     }
@@ -109,10 +112,10 @@ hipError_t launch_rank_hist(hipStream_t s, const float *sorted, int64_t ld, int6
     return hipGetLastError();
 }
 
-hipError_t launch_transit(hipStream_t s, const int64_t *C, int32_t Q, const int32_t *pos, const int64_t *end_cost,
-                          int32_t R, int64_t *T) {
+hipError_t launch_transit(hipStream_t s, const int64_t *C, int32_t Q, const int32_t *pos, const int32_t *col,
+                          const int64_t *end_cost, int32_t R, int64_t *T) {
     if (R < 2) return hipSuccess;
-    hipLaunchKernelGGL(k_transit, dim3((unsigned)R), dim3(256), 0, s, C, Q, pos, end_cost, R, T);
+    hipLaunchKernelGGL(k_transit, dim3((unsigned)R), dim3(256), 0, s, C, Q, pos, col, end_cost, R, T);
     return hipGetLastError();
 }
 

@@ -343,9 +343,10 @@ hipError_t launch_sort_rows(hipStream_t s, const float *src, int64_t ld_src, int
 hipError_t launch_rank_hist(hipStream_t s, const float *sorted, int64_t ld, int64_t nb, int64_t n_item, const float *q,
                             int64_t ldq, int32_t ns, int32_t *hist);
 // Transit-cost triangle T[lb * (lb - 1) / 2 + ub], 0 <= ub < lb < R, from the
-// column prefix sums C[j * (Q + 1) + q], the query positions pos[j] and End(j).
-hipError_t launch_transit(hipStream_t s, const int64_t *C, int32_t Q, const int32_t *pos, const int64_t *end_cost,
-                          int32_t R, int64_t *T);
+// query positions pos[j], End(j) and the prefix sums kept at the breakpoints:
+// C[col[j'] * R + j] = sum over queries q < pos[j'] of accu[q][rank_j].
+hipError_t launch_transit(hipStream_t s, const int64_t *C, int32_t Q, const int32_t *pos, const int32_t *col,
+                          const int64_t *end_cost, int32_t R, int64_t *T);
 // One DP layer: dp_cur[r] = min(dp_prev[r], min_{p<r} dp_prev[p] + T[r][p]).
 hipError_t launch_dp_layer(hipStream_t s, const int64_t *T, const int64_t *dp_prev, const int32_t *pos_prev, int32_t R,
                            int64_t *dp_cur, int32_t *pos_cur);

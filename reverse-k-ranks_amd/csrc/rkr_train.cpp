@@ -20,6 +20,7 @@
 #include <random>
 #include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "rkr_internal.hpp"
@@ -150,24 +151,53 @@ int rkr_find_optimal_sample(rkr_ctx *c, int64_t n_user, int64_t n_item, int32_t 
 
     // pos[j]: first query whose k-th rank exceeds rank[j] (BuildRankIDMap,
     // OptimalSampleRank.hpp:206-231: "this last" and "next first" coincide for
-    // integer ranks).  C[j][q]: prefix sums of accu[.][rank[j]] over queries.
-    std::vector<int32_t> pos((size_t)R);
+    // integer ranks).  The costs read the prefix sums
+    // S_j(x) = sum_{q < x} accu[q][rank[j]] only at x = 0, x = Q and x = pos[j'],
+    // so they are kept at those P breakpoints cols[0..P) only, one row of R per
+    // breakpoint: C[c * R + j] = S_j(cols[c]), and col[j] is pos[j]'s index in cols.
+    std::vector<int32_t> pos((size_t)R), col((size_t)R), cols;
     for (int64_t j = 0; j < R; ++j)
         pos[j] = (int32_t)(std::upper_bound(sort_kth_rank, sort_kth_rank + Q, rank[j]) - sort_kth_rank);
-    std::vector<int64_t> C((size_t)(R * (Q + 1)));
-    for (int64_t j = 0; j < R; ++j) C[(size_t)(j * (Q + 1))] = 0;
-    for (int64_t q = 0; q < Q; ++q) {
-        const int32_t *row = accu + q * W;
-        for (int64_t j = 0; j < R; ++j) C[(size_t)(j * (Q + 1) + q + 1)] = C[(size_t)(j * (Q + 1) + q)] + row[rank[j]];
+    cols.reserve((size_t)R + 2);
+    cols.push_back(0);
+    for (int64_t j = 0; j < R; ++j)  // pos ascends with the ranks
+        if (pos[j] != cols.back()) cols.push_back(pos[j]);
+    if (cols.back() != (int32_t)Q) cols.push_back((int32_t)Q);
+    const int64_t P = (int64_t)cols.size();
+    for (int64_t j = 0; j < R; ++j) col[j] = (int32_t)(std::lower_bound(cols.begin(), cols.end(), pos[j]) - cols.begin());
+    std::vector<int64_t> C((size_t)(P * R)), run((size_t)R, 0);
+    // one pass over the queries in order per range of ranks; ranges are disjoint
+    // columns of C, so host threads take them without sharing anything they write
+    auto sums = [&](int64_t j0, int64_t j1) {
+        if (j0 >= j1) return;
+        int64_t ci = 0;
+        for (int64_t q = 0;; ++q) {
+            for (; ci < P && cols[ci] == q; ++ci) std::copy(&run[j0], &run[j0] + (j1 - j0), &C[(size_t)(ci * R + j0)]);
+            if (q == Q) break;
+            const int32_t *row = accu + q * W;
+            for (int64_t j = j0; j < j1; ++j) run[j] += row[rank[j]];
+        }
+    };
+    const int64_t nt = std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16,
+                                          std::max<int64_t>(1, Q * R >> 14), R});
+    std::vector<std::thread> th;
+    int64_t launched = 1;
+    try {
+        th.reserve((size_t)nt);
+        for (; launched < nt; ++launched) th.emplace_back(sums, R * launched / nt, R * (launched + 1) / nt);
+    } catch (...) {  // a range without a thread runs on this one
     }
+    sums(0, R / nt);
+    for (int64_t t = launched; t < nt; ++t) sums(R * t / nt, R * (t + 1) / nt);
+    for (auto &x : th) x.join();
     std::vector<int64_t> end_cost((size_t)R), dp0((size_t)R);
     std::vector<int32_t> pos0((size_t)R);
+    const int64_t *CQ = &C[(size_t)((P - 1) * R)];  // S_j(Q)
     for (int64_t j = 0; j < R; ++j) {
-        const int64_t *Cj = &C[(size_t)(j * (Q + 1))];
-        const int64_t p = pos[j];
+        const int64_t p = pos[j], Sp = C[(size_t)((int64_t)col[j] * R + j)];
         // End: Upsilon([rank+1, n_item]); Begin: Upsilon([0, rank]) (OptimalSampleRank.hpp:270-308)
-        end_cost[j] = (Q - p) * n_user - (Cj[Q] - Cj[p]);
-        dp0[j] = Cj[p] + end_cost[j];
+        end_cost[j] = (Q - p) * n_user - (CQ[j] - Sp);
+        dp0[j] = Sp + end_cost[j];
         pos0[j] = (int32_t)j;
     }
 
@@ -178,16 +208,17 @@ int rkr_find_optimal_sample(rkr_ctx *c, int64_t n_user, int64_t n_item, int32_t 
     int rc;
     if ((rc = dC.alloc(C.size(), "training prefix sums")) || (rc = dEnd.alloc((size_t)R, "training costs")) ||
         (rc = dT.alloc((size_t)(R * (R - 1) / 2), "transit costs")) || (rc = dDp.alloc((size_t)(2 * R), "DP row")) ||
-        (rc = dPos.alloc((size_t)R, "query positions")) ||
+        (rc = dPos.alloc((size_t)(2 * R), "query positions")) ||
         (rc = dPath.alloc((size_t)n_sample * R, "DP positions")))
         return rc;
     hipStream_t s = c->stream;
     RKR_HIP(c, hipMemcpyAsync(dC.p, C.data(), sizeof(int64_t) * C.size(), hipMemcpyHostToDevice, s));
     RKR_HIP(c, hipMemcpyAsync(dEnd.p, end_cost.data(), sizeof(int64_t) * R, hipMemcpyHostToDevice, s));
     RKR_HIP(c, hipMemcpyAsync(dPos.p, pos.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
+    RKR_HIP(c, hipMemcpyAsync(dPos.p + R, col.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
     RKR_HIP(c, hipMemcpyAsync(dDp.p, dp0.data(), sizeof(int64_t) * R, hipMemcpyHostToDevice, s));
     RKR_HIP(c, hipMemcpyAsync(dPath.p, pos0.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
-    RKR_HIP(c, launch_transit(s, dC.p, (int32_t)Q, dPos.p, dEnd.p, (int32_t)R, dT.p));
+    RKR_HIP(c, launch_transit(s, dC.p, (int32_t)Q, dPos.p, dPos.p + R, dEnd.p, (int32_t)R, dT.p));
     for (int32_t l = 1; l < n_sample; ++l)
         RKR_HIP(c, launch_dp_layer(s, dT.p, dDp.p + ((l - 1) & 1) * R, dPath.p + (int64_t)(l - 1) * R, (int32_t)R,
                                    dDp.p + (l & 1) * R, dPath.p + (int64_t)l * R));
