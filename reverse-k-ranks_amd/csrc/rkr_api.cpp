@@ -811,10 +811,25 @@ int rkr_query_rank_distribution(rkr_ctx *c, const int32_t *item_ids, int32_t n_s
     if (rc) return rc;
     RKR_HIP(c, hipMemcpyAsync(accu, hist.p, hist.bytes, hipMemcpyDeviceToHost, c->stream));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
-    for (int64_t s = 0; s < ns; ++s) {  // counts per rank -> users with rank <= r
-        int32_t *row = accu + s * W;
-        for (int64_t r = 1; r < W; ++r) row[r] += row[r - 1];
+    // counts per rank -> users with rank <= r; host threads take disjoint rows
+    auto scan = [accu, W](int64_t s0, int64_t s1) {
+        for (int64_t s = s0; s < s1; ++s) {
+            int32_t *row = accu + s * W;
+            for (int64_t r = 1; r < W; ++r) row[r] += row[r - 1];
+        }
+    };
+    const int64_t nt = std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16,
+                                          std::max<int64_t>(1, ns * W >> 16), ns});
+    std::vector<std::thread> th;
+    int64_t launched = 1;
+    try {
+        th.reserve((size_t)nt);
+        for (; launched < nt; ++launched) th.emplace_back(scan, ns * launched / nt, ns * (launched + 1) / nt);
+    } catch (...) {  // rows without a thread are scanned on this one
     }
+    scan(0, ns / nt);
+    for (int64_t t = launched; t < nt; ++t) scan(ns * t / nt, ns * (t + 1) / nt);
+    for (auto &x : th) x.join();
     if (device_ms) *device_ms = ms;
     return RKR_OK;
 }
