@@ -74,6 +74,8 @@ def train_sample_ranks(idx: Index, n_sample: int, n_sample_query: int, sample_to
     ids = sample_items(idx.n_item, n_sample_query)
     accu, ms = query_rank_distribution(idx, ids)
     kth, order = sort_kth_rank(accu, sample_topk)
-    ranks, cost = find_optimal_sample(idx, idx.n_user, idx.n_item, kth, accu[order], n_sample, method)
+    accu_sorted = accu[order]
+    del accu
+    ranks, cost = find_optimal_sample(idx, idx.n_user, idx.n_item, kth, accu_sorted, n_sample, method)
     return {"ranks": ranks, "train_io_cost": cost, "sample_item_ids": ids[order], "sort_kth_rank": kth,
-            "accu_sorted": accu[order], "rank_distribution_device_ms": ms}
+            "accu_sorted": accu_sorted, "rank_distribution_device_ms": ms}
