@@ -809,27 +809,61 @@ int rkr_query_rank_distribution(rkr_ctx *c, const int32_t *item_ids, int32_t n_s
         return (int)RKR_OK;
     });
     if (rc) return rc;
-    RKR_HIP(c, hipMemcpyAsync(accu, hist.p, hist.bytes, hipMemcpyDeviceToHost, c->stream));
-    RKR_HIP(c, hipStreamSynchronize(c->stream));
-    // counts per rank -> users with rank <= r; host threads take disjoint rows
-    auto scan = [accu, W](int64_t s0, int64_t s1) {
-        for (int64_t s = s0; s < s1; ++s) {
-            int32_t *row = accu + s * W;
-            for (int64_t r = 1; r < W; ++r) row[r] += row[r - 1];
+    // The counts leave through two pinned staging buffers of about 32 MB: while
+    // one chunk of rows copies, host threads turn the previous chunk's counts per
+    // rank into users with rank <= r, writing the caller's rows.
+    const int64_t chunk_rows = std::min<int64_t>(ns, std::max<int64_t>(1, ((int64_t)8 << 20) / W));
+    const int64_t n_chunk = (ns + chunk_rows - 1) / chunk_rows;
+    struct Staging {
+        hipStream_t s;
+        void *p[2] = {nullptr, nullptr};
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Staging() {
+            (void)hipStreamSynchronize(s);  // no copy into a buffer outlives it
+            for (int k = 0; k < 2; ++k) {
+                if (p[k]) (void)hipHostFree(p[k]);
+                if (e[k]) (void)hipEventDestroy(e[k]);
+            }
         }
-    };
-    const int64_t nt = std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16,
-                                          std::max<int64_t>(1, ns * W >> 16), ns});
-    std::vector<std::thread> th;
-    int64_t launched = 1;
-    try {
-        th.reserve((size_t)nt);
-        for (; launched < nt; ++launched) th.emplace_back(scan, ns * launched / nt, ns * (launched + 1) / nt);
-    } catch (...) {  // rows without a thread are scanned on this one
+    } stg{c->stream};
+    for (int k = 0; k < 2 && k < n_chunk; ++k) {
+        RKR_HIP(c, hipHostMalloc(&stg.p[k], sizeof(int32_t) * (size_t)(chunk_rows * W), hipHostMallocDefault));
+        RKR_HIP(c, hipEventCreateWithFlags(&stg.e[k], hipEventDisableTiming));
     }
-    scan(0, ns / nt);
-    for (int64_t t = launched; t < nt; ++t) scan(ns * t / nt, ns * (t + 1) / nt);
-    for (auto &x : th) x.join();
+    auto issue = [&](int64_t i) {
+        const int64_t r0 = i * chunk_rows, nr = std::min(chunk_rows, ns - r0);
+        hipError_t e = hipMemcpyAsync(stg.p[i & 1], h + r0 * W, sizeof(int32_t) * (size_t)(nr * W),
+                                      hipMemcpyDeviceToHost, c->stream);
+        return e == hipSuccess ? hipEventRecord(stg.e[i & 1], c->stream) : e;
+    };
+    RKR_HIP(c, issue(0));
+    if (n_chunk > 1) RKR_HIP(c, issue(1));
+    for (int64_t i = 0; i < n_chunk; ++i) {
+        RKR_HIP(c, hipEventSynchronize(stg.e[i & 1]));
+        const int64_t r0 = i * chunk_rows, nr = std::min(chunk_rows, ns - r0);
+        const int32_t *src = static_cast<const int32_t *>(stg.p[i & 1]);
+        auto scan = [src, accu, r0, W](int64_t s0, int64_t s1) {
+            for (int64_t s = s0; s < s1; ++s) {
+                const int32_t *x = src + s * W;
+                int32_t *row = accu + (r0 + s) * W;
+                int32_t run = 0;
+                for (int64_t r = 0; r < W; ++r) row[r] = run += x[r];
+            }
+        };
+        const int64_t nt = std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16,
+                                              std::max<int64_t>(1, nr * W >> 16), nr});
+        std::vector<std::thread> th;
+        int64_t launched = 1;
+        try {
+            th.reserve((size_t)nt);
+            for (; launched < nt; ++launched) th.emplace_back(scan, nr * launched / nt, nr * (launched + 1) / nt);
+        } catch (...) {  // rows without a thread are scanned on this one
+        }
+        scan(0, nr / nt);
+        for (int64_t t = launched; t < nt; ++t) scan(nr * t / nt, nr * (t + 1) / nt);
+        for (auto &x : th) x.join();
+        if (i + 2 < n_chunk) RKR_HIP(c, issue(i + 2));  // the buffer just read takes the chunk after next
+    }
     if (device_ms) *device_ms = ms;
     return RKR_OK;
 }
