@@ -674,6 +674,90 @@ struct ScopedDev {
     T *as() const { return static_cast<T *>(p); }
     ~ScopedDev() { rkr::dev_free(c, p, bytes); }
 };
+
+// Runs fn(i0, i1) over [0, n) in up to 16 ranges, one host thread per range and
+// no range under min_per units; a range whose thread cannot start runs here.
+template <class F>
+void host_ranges(int64_t n, int64_t min_per, F &&fn) {
+    if (n <= 0) return;
+    const int64_t nt = std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16,
+                                          std::max<int64_t>(1, n / std::max<int64_t>(1, min_per))});
+    std::vector<std::thread> th;
+    int64_t launched = 1;
+    try {
+        th.reserve((size_t)nt);
+        for (; launched < nt; ++launched) th.emplace_back(fn, n * launched / nt, n * (launched + 1) / nt);
+    } catch (...) {
+    }
+    fn(0, n / nt);
+    for (int64_t t = launched; t < nt; ++t) fn(n * t / nt, n * (t + 1) / nt);
+    for (auto &x : th) x.join();
+}
+
+// Device-to-host copies through two pinned staging buffers: while one chunk
+// crosses the link, the host consumes the previous one.  A plain copy into
+// pageable memory runs at a fraction of the link's rate.
+constexpr size_t kStageBytes = (size_t)32 << 20;
+struct Staging {
+    hipStream_t s;
+    size_t chunk = 0;
+    void *p[2] = {nullptr, nullptr};
+    hipEvent_t e[2] = {nullptr, nullptr};
+    explicit Staging(hipStream_t st) : s(st) {}
+    Staging(const Staging &) = delete;
+    Staging &operator=(const Staging &) = delete;
+    ~Staging() {
+        (void)hipStreamSynchronize(s);  // no copy into a buffer outlives it
+        for (int k = 0; k < 2; ++k) {
+            if (p[k]) (void)hipHostFree(p[k]);
+            if (e[k]) (void)hipEventDestroy(e[k]);
+        }
+    }
+    hipError_t init(size_t bytes) {
+        chunk = std::max<size_t>(bytes, 1);
+        for (int k = 0; k < 2; ++k) {
+            hipError_t r = hipHostMalloc(&p[k], chunk, hipHostMallocDefault);
+            if (r == hipSuccess) r = hipEventCreateWithFlags(&e[k], hipEventDisableTiming);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    }
+    // Chunk k of n_chunk: issue(k, buf) enqueues its copy into buf on the stream
+    // (at most `chunk` bytes); take(k, buf) consumes it once it has arrived and
+    // returns false to stop.
+    template <class Issue, class Take>
+    hipError_t run(int64_t n_chunk, Issue &&issue, Take &&take) {
+        auto put = [&](int64_t k) {
+            const hipError_t r = issue(k, p[k & 1]);
+            return r == hipSuccess ? hipEventRecord(e[k & 1], s) : r;
+        };
+        hipError_t r = n_chunk > 0 ? put(0) : hipSuccess;
+        if (r == hipSuccess && n_chunk > 1) r = put(1);
+        for (int64_t k = 0; r == hipSuccess && k < n_chunk; ++k) {
+            r = hipEventSynchronize(e[k & 1]);
+            if (r != hipSuccess || !take(k, static_cast<const char *>(p[k & 1]))) break;
+            if (k + 2 < n_chunk) r = put(k + 2);  // the buffer just consumed takes the chunk after next
+        }
+        return r;
+    }
+    // dst[0, bytes) = src[0, bytes), src in device memory; host threads copy out.
+    hipError_t to_host(void *dst, const void *src, size_t bytes) {
+        const int64_t n_chunk = (int64_t)((bytes + chunk - 1) / chunk);
+        return run(
+            n_chunk,
+            [&](int64_t k, void *buf) {
+                const size_t off = (size_t)k * chunk, len = std::min(chunk, bytes - off);
+                return hipMemcpyAsync(buf, static_cast<const char *>(src) + off, len, hipMemcpyDeviceToHost, s);
+            },
+            [&](int64_t k, const char *buf) {
+                const size_t off = (size_t)k * chunk, len = std::min(chunk, bytes - off);
+                char *d = static_cast<char *>(dst) + off;
+                host_ranges((int64_t)len, (int64_t)1 << 20,
+                            [d, buf](int64_t a, int64_t b) { std::memcpy(d + a, buf + a, (size_t)(b - a)); });
+                return true;
+            });
+    }
+};
 }  // namespace
 
 // Calls sink(rows, user, m) for users [u0, u0 + n) in batches of m rows: rows
@@ -721,10 +805,10 @@ int rkr_sorted_scores(rkr_ctx *c, int64_t user_begin, int64_t n_rows, float *out
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
     const int64_t I = c->n_item;
+    Staging stg{c->stream};
+    RKR_HIP(c, stg.init(std::min(kStageBytes, sizeof(float) * (size_t)(n_rows * I))));
     return sorted_score_batches(c, user_begin, n_rows, "sorted_scores", [&](const float *rows, int64_t u, int64_t m) {
-        RKR_HIP(c, hipMemcpyAsync(out + (u - user_begin) * I, rows, sizeof(float) * (size_t)(m * I),
-                                  hipMemcpyDeviceToHost, c->stream));
-        RKR_HIP(c, hipStreamSynchronize(c->stream));
+        RKR_HIP(c, stg.to_host(out + (u - user_begin) * I, rows, sizeof(float) * (size_t)(m * I)));
         return (int)RKR_OK;
     });
 }
@@ -737,21 +821,34 @@ int rkr_save_score_table(rkr_ctx *c, const char *path, int64_t n_rows, double *t
     if (n_rows > c->n_user) return set_err(c, RKR_ERR_INVALID, "save_score_table: more rows than users");
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t I = c->n_item;
+    // the file is written straight from the pinned staging buffers, one chunk
+    // while the next crosses the link
+    Staging stg{c->stream};
+    RKR_HIP(c, stg.init(std::min(kStageBytes, sizeof(float) * (size_t)(n_rows * I))));
     FILE *f = std::fopen(path, "wb");
     if (!f) return set_err(c, RKR_ERR_IO, std::string("save_score_table: cannot open ") + path);
-    const int64_t I = c->n_item;
     double gemm_ms = 0, sort_ms = 0, write_s = 0;
     const double t0 = rkr_now();
-    std::vector<float> host;
     int rc = n_rows == 0 ? (int)RKR_OK
                          : sorted_score_batches(c, 0, n_rows, "save_score_table",
                                                 [&](const float *rows, int64_t, int64_t m) {
-        host.resize((size_t)(m * I));
-        RKR_HIP(c, hipMemcpyAsync(host.data(), rows, sizeof(float) * host.size(), hipMemcpyDeviceToHost, c->stream));
-        RKR_HIP(c, hipStreamSynchronize(c->stream));
-        const double w0 = rkr_now();
-        const bool ok = std::fwrite(host.data(), sizeof(float), host.size(), f) == host.size();
-        write_s += rkr_now() - w0;
+        const size_t bytes = sizeof(float) * (size_t)(m * I);
+        bool ok = true;
+        RKR_HIP(c, stg.run(
+                       (int64_t)((bytes + stg.chunk - 1) / stg.chunk),
+                       [&](int64_t k, void *buf) {
+                           const size_t off = (size_t)k * stg.chunk, len = std::min(stg.chunk, bytes - off);
+                           return hipMemcpyAsync(buf, reinterpret_cast<const char *>(rows) + off, len,
+                                                 hipMemcpyDeviceToHost, c->stream);
+                       },
+                       [&](int64_t k, const char *buf) {
+                           const size_t off = (size_t)k * stg.chunk, len = std::min(stg.chunk, bytes - off);
+                           const double w0 = rkr_now();
+                           ok = std::fwrite(buf, 1, len, f) == len;
+                           write_s += rkr_now() - w0;
+                           return ok;
+                       }));
         if (!ok) return set_err(c, RKR_ERR_IO, std::string("save_score_table: write failed on ") + path);
         return (int)RKR_OK;
     }, &gemm_ms, &sort_ms);
@@ -809,61 +906,32 @@ int rkr_query_rank_distribution(rkr_ctx *c, const int32_t *item_ids, int32_t n_s
         return (int)RKR_OK;
     });
     if (rc) return rc;
-    // The counts leave through two pinned staging buffers of about 32 MB: while
-    // one chunk of rows copies, host threads turn the previous chunk's counts per
-    // rank into users with rank <= r, writing the caller's rows.
-    const int64_t chunk_rows = std::min<int64_t>(ns, std::max<int64_t>(1, ((int64_t)8 << 20) / W));
-    const int64_t n_chunk = (ns + chunk_rows - 1) / chunk_rows;
-    struct Staging {
-        hipStream_t s;
-        void *p[2] = {nullptr, nullptr};
-        hipEvent_t e[2] = {nullptr, nullptr};
-        ~Staging() {
-            (void)hipStreamSynchronize(s);  // no copy into a buffer outlives it
-            for (int k = 0; k < 2; ++k) {
-                if (p[k]) (void)hipHostFree(p[k]);
-                if (e[k]) (void)hipEventDestroy(e[k]);
-            }
-        }
-    } stg{c->stream};
-    for (int k = 0; k < 2 && k < n_chunk; ++k) {
-        RKR_HIP(c, hipHostMalloc(&stg.p[k], sizeof(int32_t) * (size_t)(chunk_rows * W), hipHostMallocDefault));
-        RKR_HIP(c, hipEventCreateWithFlags(&stg.e[k], hipEventDisableTiming));
-    }
-    auto issue = [&](int64_t i) {
-        const int64_t r0 = i * chunk_rows, nr = std::min(chunk_rows, ns - r0);
-        hipError_t e = hipMemcpyAsync(stg.p[i & 1], h + r0 * W, sizeof(int32_t) * (size_t)(nr * W),
-                                      hipMemcpyDeviceToHost, c->stream);
-        return e == hipSuccess ? hipEventRecord(stg.e[i & 1], c->stream) : e;
-    };
-    RKR_HIP(c, issue(0));
-    if (n_chunk > 1) RKR_HIP(c, issue(1));
-    for (int64_t i = 0; i < n_chunk; ++i) {
-        RKR_HIP(c, hipEventSynchronize(stg.e[i & 1]));
-        const int64_t r0 = i * chunk_rows, nr = std::min(chunk_rows, ns - r0);
-        const int32_t *src = static_cast<const int32_t *>(stg.p[i & 1]);
-        auto scan = [src, accu, r0, W](int64_t s0, int64_t s1) {
-            for (int64_t s = s0; s < s1; ++s) {
-                const int32_t *x = src + s * W;
-                int32_t *row = accu + (r0 + s) * W;
-                int32_t run = 0;
-                for (int64_t r = 0; r < W; ++r) row[r] = run += x[r];
-            }
-        };
-        const int64_t nt = std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16,
-                                              std::max<int64_t>(1, nr * W >> 16), nr});
-        std::vector<std::thread> th;
-        int64_t launched = 1;
-        try {
-            th.reserve((size_t)nt);
-            for (; launched < nt; ++launched) th.emplace_back(scan, nr * launched / nt, nr * (launched + 1) / nt);
-        } catch (...) {  // rows without a thread are scanned on this one
-        }
-        scan(0, nr / nt);
-        for (int64_t t = launched; t < nt; ++t) scan(nr * t / nt, nr * (t + 1) / nt);
-        for (auto &x : th) x.join();
-        if (i + 2 < n_chunk) RKR_HIP(c, issue(i + 2));  // the buffer just read takes the chunk after next
-    }
+    // The counts leave through the pinned staging buffers in chunks of whole
+    // rows; host threads turn each chunk's counts per rank into users with
+    // rank <= r, written into the caller's rows, while the next chunk copies.
+    const int64_t chunk_rows = std::min<int64_t>(ns, std::max<int64_t>(1, (int64_t)(kStageBytes / sizeof(int32_t)) / W));
+    Staging stg{c->stream};
+    RKR_HIP(c, stg.init(sizeof(int32_t) * (size_t)(chunk_rows * W)));
+    RKR_HIP(c, stg.run(
+                   (ns + chunk_rows - 1) / chunk_rows,
+                   [&](int64_t k, void *buf) {
+                       const int64_t r0 = k * chunk_rows, nr = std::min(chunk_rows, ns - r0);
+                       return hipMemcpyAsync(buf, h + r0 * W, sizeof(int32_t) * (size_t)(nr * W),
+                                             hipMemcpyDeviceToHost, c->stream);
+                   },
+                   [&](int64_t k, const char *buf) {
+                       const int64_t r0 = k * chunk_rows, nr = std::min(chunk_rows, ns - r0);
+                       const int32_t *src = reinterpret_cast<const int32_t *>(buf);
+                       host_ranges(nr, std::max<int64_t>(1, ((int64_t)1 << 16) / W), [src, accu, r0, W](int64_t s0, int64_t s1) {
+                           for (int64_t s = s0; s < s1; ++s) {
+                               const int32_t *x = src + s * W;
+                               int32_t *row = accu + (r0 + s) * W;
+                               int32_t run = 0;
+                               for (int64_t r = 0; r < W; ++r) row[r] = run += x[r];
+                           }
+                       });
+                       return true;
+                   }));
     if (device_ms) *device_ms = ms;
     return RKR_OK;
 }

@@ -262,10 +262,12 @@ def test_two_query_slots_agree(orc, gpu_index_factory, form, batch):
     (40, 20000, 150, 24),     # plain-GPU row width: two merge passes, a partial run
     (9, 50001, 33, 25),       # precision row width: three merge passes
     (3, 1, 4, 26),            # one item
+    (1000, 20000, 8, 28),     # 80 MB of rows: three 32 MB host staging chunks, the last one partial
 ])
-def test_sorted_scores_match_oracle(orc, U, I, d, seed):
+def test_sorted_scores_match_oracle(orc, U, I, d, seed, tmp_path):
     _case(f"sorted scores U={U} I={I} d={d}")
     import rkr
+    from rkr import formats
     rng = np.random.default_rng(seed)
     users = rng.standard_normal((U, d), dtype=np.float32)
     items = rng.standard_normal((I, d), dtype=np.float32) * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32)
@@ -277,6 +279,9 @@ def test_sorted_scores_match_oracle(orc, U, I, d, seed):
     assert np.array_equal(got, orc.sorted_score_rows(un, items))
     u0, n = U // 3, max(1, U // 4)
     assert np.array_equal(idx.sorted_scores(u0, n), got[u0:u0 + n])
+    p = formats.score_table_path(str(tmp_path), "t")
+    idx.save_score_table(p)
+    assert np.array_equal(formats.read_score_table(p, I, mmap=False), got)
 
 
 def test_sorted_scores_ties_batches_and_table(orc, tmp_path):
