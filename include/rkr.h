@@ -179,7 +179,9 @@ int rkr_load_table(rkr_ctx *ctx, const char *path, float eps);
  * each row sorted in descending order, into out[n_rows][n_item] (host):
  * ComputeScoreTableBatch::ComputeSortItemsBatch
  * (src/include/score_computation/ComputeScoreTableBatch.hpp:79-92).  Scores
- * follow the same rule as the table build. */
+ * follow the same rule as the table build.  The rows reach out through two
+ * pinned host staging buffers of up to 32 MB each, allocated for the call,
+ * with up to 16 host threads moving data out of them. */
 int rkr_sorted_scores(rkr_ctx *ctx, int64_t user_begin, int64_t n_rows, float *out);
 
 /* Writes the sorted score rows of the first n_rows users (n_rows < 0: every
@@ -190,7 +192,9 @@ int rkr_sorted_scores(rkr_ctx *ctx, int64_t user_begin, int64_t n_rows, float *o
  * reference's split of the build time (BuildScoreTableBatch.cpp:113-127):
  * [0] device time of the score GEMMs, [1] device time of the row sorts,
  * [2] wall time of computing and copying the rows to the host ("compute sort
- * time"), [3] wall time of writing the file ("save index time"). */
+ * time"), [3] wall time of writing the file ("save index time").  The file
+ * is written from two pinned staging buffers of up to 32 MB each, one while
+ * the next copies, so [2] and [3] overlap. */
 int rkr_save_score_table(rkr_ctx *ctx, const char *path, int64_t n_rows, double *timing_ms);
 
 /* ---- QS sample-rank training ---------------------------------------------- */
@@ -211,7 +215,8 @@ int rkr_sample_items(int64_t n_item, int32_t n_sample_item, int32_t *out_item_id
  * rank(u, s) = #{items whose score beats <u, s>} (QueryRankGPU.hpp:63-92);
  * accu[s][r] = #{u : rank(u, s) <= r}, r = 0..n_item, so accu is
  * n_sample_item x (n_item + 1) int32 (host).  device_ms (optional) receives
- * the device time. */
+ * the device time.  The counts leave through two pinned 32 MB staging
+ * buffers; up to 16 host threads take the prefix sums. */
 int rkr_query_rank_distribution(rkr_ctx *ctx, const int32_t *item_ids, int32_t n_sample_item, int32_t *accu,
                                 double *device_ms);
 
