@@ -645,18 +645,6 @@ int rkr_set_table(rkr_ctx *c, const int32_t *ranks, int32_t S, const float *tabl
     return RKR_OK;
 }
 
-int rkr_get_table(rkr_ctx *c, int64_t user_begin, int64_t n_rows, float *out) {
-    if (!c || !out) return RKR_ERR_INVALID;
-    if (!c->table) return set_err(c, RKR_ERR_STATE, "get_table: no table");
-    if (user_begin < 0 || n_rows < 0 || user_begin + n_rows > c->n_user)
-        return set_err(c, RKR_ERR_INVALID, "get_table: row range out of bounds");
-    RKR_HIP(c, hipSetDevice(c->device));
-    RKR_HIP(c, hipStreamSynchronize(c->stream));
-    RKR_HIP(c, hipMemcpy(out, c->table + user_begin * c->n_sample, sizeof(float) * (size_t)n_rows * c->n_sample,
-                         hipMemcpyDeviceToHost));
-    return RKR_OK;
-}
-
 extern "C++" {
 namespace {
 // A device allocation freed when the scope ends.
@@ -794,6 +782,21 @@ static int sorted_score_batches(rkr_ctx *c, int64_t u0, int64_t n, const char *w
     return RKR_OK;
 }
 }  // extern "C++"
+
+int rkr_get_table(rkr_ctx *c, int64_t user_begin, int64_t n_rows, float *out) {
+    if (!c || !out) return RKR_ERR_INVALID;
+    if (!c->table) return set_err(c, RKR_ERR_STATE, "get_table: no table");
+    if (user_begin < 0 || n_rows < 0 || user_begin + n_rows > c->n_user)
+        return set_err(c, RKR_ERR_INVALID, "get_table: row range out of bounds");
+    if (n_rows == 0) return RKR_OK;
+    RKR_HIP(c, hipSetDevice(c->device));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    const size_t bytes = sizeof(float) * (size_t)n_rows * c->n_sample;
+    Staging stg{c->stream};
+    RKR_HIP(c, stg.init(std::min(kStageBytes, bytes)));
+    RKR_HIP(c, stg.to_host(out, c->table + user_begin * c->n_sample, bytes));
+    return RKR_OK;
+}
 
 int rkr_sorted_scores(rkr_ctx *c, int64_t user_begin, int64_t n_rows, float *out) {
     // ComputeScoreTableBatch::ComputeSortItemsBatch (ComputeScoreTableBatch.hpp:79-92)
@@ -947,16 +950,31 @@ int rkr_save_table(rkr_ctx *c, const char *path) {
     // SampleSearch::SaveIndex layout (SampleSearch.hpp:186-194)
     if (!c || !path) return RKR_ERR_INVALID;
     if (!c->table) return set_err(c, RKR_ERR_STATE, "save_table: no table");
-    std::vector<float> host((size_t)c->n_user * c->n_sample);
-    int rc = rkr_get_table(c, 0, c->n_user, host.data());
-    if (rc) return rc;
+    RKR_HIP(c, hipSetDevice(c->device));
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
+    // the table is written straight from the pinned staging buffers, one chunk
+    // while the next crosses the link
+    const size_t bytes = sizeof(float) * (size_t)c->n_user * c->n_sample;
+    Staging stg{c->stream};
+    RKR_HIP(c, stg.init(std::min(kStageBytes, std::max<size_t>(bytes, 1))));
     FILE *f = std::fopen(path, "wb");
     if (!f) return set_err(c, RKR_ERR_IO, std::string("save_table: cannot open ") + path);
     const size_t hdr[3] = {(size_t)c->n_sample, (size_t)c->n_item, (size_t)c->n_user};
     bool ok = std::fwrite(hdr, sizeof(size_t), 3, f) == 3;
     ok = ok && std::fwrite(c->h_ranks.data(), sizeof(int32_t), c->n_sample, f) == (size_t)c->n_sample;
-    ok = ok && std::fwrite(host.data(), sizeof(float), host.size(), f) == host.size();
+    const hipError_t e = !ok ? hipSuccess : stg.run(
+        (int64_t)((bytes + stg.chunk - 1) / stg.chunk),
+        [&](int64_t k, void *buf) {
+            const size_t off = (size_t)k * stg.chunk, len = std::min(stg.chunk, bytes - off);
+            return hipMemcpyAsync(buf, reinterpret_cast<const char *>(c->table) + off, len, hipMemcpyDeviceToHost,
+                                  c->stream);
+        },
+        [&](int64_t k, const char *buf) {
+            const size_t off = (size_t)k * stg.chunk, len = std::min(stg.chunk, bytes - off);
+            return ok = std::fwrite(buf, 1, len, f) == len;
+        });
     ok = (std::fclose(f) == 0) && ok;
+    if (e != hipSuccess) return set_err(c, RKR_ERR_HIP, std::string("save_table: ") + hipGetErrorString(e));
     if (!ok) return set_err(c, RKR_ERR_IO, std::string("save_table: short write to ") + path);
     return RKR_OK;
 }

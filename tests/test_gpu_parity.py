@@ -36,9 +36,11 @@ def test_table_fake30d_matches_oracle(orc, fake30d, gpu_index_factory):
     (130, 20000, 150, 512, 3),   # plain-GPU row width
     (64, 50001, 33, 512, 4),     # row larger than one LDS gather pass
     (5, 3, 4, 2, 5),             # tiny
+    (20000, 2000, 8, 512, 6),    # 41 MB table: two 32 MB host staging chunks, the second partial
 ])
-def test_table_random_matches_oracle(orc, gpu_index_factory, U, I, d, S, seed):
+def test_table_random_matches_oracle(orc, gpu_index_factory, U, I, d, S, seed, tmp_path):
     _case(f"table U={U} I={I} d={d} S={S}")
+    from rkr import formats
     rng = np.random.default_rng(seed)
     users = rng.standard_normal((U, d), dtype=np.float32)
     items = rng.standard_normal((I, d), dtype=np.float32) * rng.lognormal(0, 0.5, (I, 1)).astype(np.float32)
@@ -46,6 +48,10 @@ def test_table_random_matches_oracle(orc, gpu_index_factory, U, I, d, S, seed):
     T = idx.get_table()
     T_ref = orc.build_table(_normalized(orc, users), items, ranks)
     assert np.array_equal(T, T_ref)
+    p = str(tmp_path / "t.index")
+    idx.save_table(p)
+    f_ranks, f_table, f_items = formats.read_table(p)
+    assert f_items == I and np.array_equal(f_ranks, ranks) and np.array_equal(f_table, T)
 
 
 def test_table_ties_and_degenerate_rows(orc, gpu_index_factory):
