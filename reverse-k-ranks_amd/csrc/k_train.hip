@@ -5,7 +5,8 @@
 //    QueryRankGPU::ComputeRankOfQuery, src/app/build-index/QueryRankGPU.hpp:63-92,225-240):
 //    for every user u and sample item s, rank(u, s) = #{items whose score
 //    beats <u, s>} -- the lower_bound with greater<float> in u's descending
-//    row.  k_rank_hist counts hist[s][rank] over the users of a batch.
+//    row.  k_rank_hist_lds (row staged in LDS) or k_rank_hist counts
+//    hist[s][rank] over the users of a batch.
 // 2. The sample-rank DP of FindOptimalSampleFast
 //    (src/include/alg/RankBoundRefinement/FindSampleRank/OptimalSampleRankFast.hpp:160-205):
 //    k_transit fills the transit-cost triangle of QueryDistribution
@@ -30,6 +31,29 @@ __global__ void __launch_bounds__(256) k_rank_hist(const float *__restrict__ sor
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (row[mid] > x) lo = mid + 1;
+            else hi = mid;
+        }
+        atomicAdd(hist + (int64_t)s * (n_item + 1) + lo, 1);
+    }
+}
+
+// The same count with the row staged in LDS first (n_item floats of dynamic
+// LDS): each binary search then reads LDS instead of issuing its ~15 dependent
+// requests to the caches.  Used when the row fits the workgroup's LDS.
+__global__ void __launch_bounds__(512) k_rank_hist_lds(const float *__restrict__ sorted, int64_t ld, int32_t n_item,
+                                                       const float *__restrict__ q, int64_t ldq, int32_t ns,
+                                                       int32_t *__restrict__ hist) {
+    extern __shared__ float srow[];
+    const int64_t u = blockIdx.x;
+    const float *row = sorted + u * ld;
+    for (int i = threadIdx.x; i < n_item; i += blockDim.x) srow[i] = row[i];
+    __syncthreads();
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+        const float x = q[u * ldq + s];
+        int lo = 0, hi = n_item;  // first position whose value is not > x
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (srow[mid] > x) lo = mid + 1;
             else hi = mid;
         }
         atomicAdd(hist + (int64_t)s * (n_item + 1) + lo, 1);
@@ -108,7 +132,23 @@ hipError_t launch_rank_hist(hipStream_t s, const float *sorted, int64_t ld, int6
                             int64_t ldq, int32_t ns, int32_t *hist) {
     if (nb <= 0 || ns <= 0) return hipSuccess;
     if (n_item <= 0 || n_item >= INT_MAX || nb > INT_MAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_rank_hist, dim3((unsigned)nb), dim3(256), 0, s, sorted, ld, (int32_t)n_item, q, ldq, ns, hist);
+    int dev = 0, lds_max = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    const size_t lds = sizeof(float) * (size_t)n_item;
+    bool use_lds = lds <= (size_t)lds_max;
+    if (use_lds && hipFuncSetAttribute((const void *)k_rank_hist_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds) != hipSuccess) {
+        (void)hipGetLastError();  // a refused attribute: the row stays in global memory
+        use_lds = false;
+    }
+    if (use_lds)
+        hipLaunchKernelGGL(k_rank_hist_lds, dim3((unsigned)nb), dim3(512), lds, s, sorted, ld, (int32_t)n_item, q, ldq,
+                           ns, hist);
+    else
+        hipLaunchKernelGGL(k_rank_hist, dim3((unsigned)nb), dim3(256), 0, s, sorted, ld, (int32_t)n_item, q, ldq, ns,
+                           hist);
     return hipGetLastError();
 }
 

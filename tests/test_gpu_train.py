@@ -30,6 +30,7 @@ def _data(seed, U, I, d, dup=0):
     (33, 20001, 20, 7, 0, 3),        # two merge passes, odd width
     (5, 40, 3, 40, 0, 4),            # every item sampled
     (33, 20001, 20, 900, 0, 10),     # counts leave in three staging chunks of 419, 419 and 62 rows
+    (9, 50001, 12, 30, 0, 11),       # a 200 KB row: more than a workgroup's LDS, searched in global memory
 ])
 def test_rank_distribution_matches_oracle(orc, U, I, d, ns, batch, seed):
     _case(f"rank distribution U={U} I={I} ns={ns} batch={batch}")
