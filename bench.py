@@ -1,18 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the reverse k-ranks US path on one MI355X.
 
-Workload (default): 100k users x 20k items x d=150 (users N(0,1)/sqrt(d),
-items N(0,1)/sqrt(d) x lognormal(0,0.5) norms, seeded), uniform sample
-table of 512 ranks built on the device, 1,000 queries drawn from the items,
-k=50, exact ranks.  One step = one rkr_query call over the 1,000-query
-batch with every input resident in HBM (only the query vectors and the
-answers cross PCIe).
+Workload (default, the shape the headline metric is quoted at): 1M users x
+50k items x d=150 (users N(0,1)/sqrt(d), items N(0,1)/sqrt(d) x
+lognormal(0,0.5) norms, seeded), uniform sample table of 512 ranks built on
+the device, 1,000 queries drawn from the items, k=50, exact ranks.  One step
+= one rkr_query call over the 1,000-query batch with every input resident in
+HBM (only the query vectors and the answers cross PCIe).
 
 Prints one JSON line: queries/s (value), single-query p50/p99 latency, table
 build time, the roofline of the dominant kernel (HIP-event time of that
-kernel inside the timed region) and the CPU baseline (the oracle's
-restatement of the reference's OpenMP US query path, timed on a bounded
-sample of the same queries against the same table).
+kernel inside the timed region), the same queries through the other forms
+(sampled table only, plain FP32) so every ratio is like for like, and the
+CPU baseline (the oracle's restatement of the reference's OpenMP US query
+path on every core of this process's CPU share, timed on a bounded sample of
+the same queries against the same table).
 """
 from __future__ import annotations
 
@@ -51,12 +53,16 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="plain_gpu", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="precision", choices=sorted(WORKLOADS))
     ap.add_argument("--form", default="screened", choices=["plain", "screened"],
                     help="screened: BF16 matrix-core screen + exact recheck (same answers as plain)")
     ap.add_argument("--n-query", type=int, default=None)
     ap.add_argument("--latency-queries", type=int, default=200)
-    ap.add_argument("--cpu-sample", type=int, default=None, help="queries in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="queries in the CPU baseline sample (default: as many as fit --cpu-seconds)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline: time bound of the sample")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the same queries through the sampled-table-only and plain FP32 forms")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--duration", type=float, default=10.0, help="serving: open-loop seconds")
@@ -293,7 +299,7 @@ def score_table(args, idx, wl, label):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         oracle.build()
-        cores = min(os.cpu_count() or 1, 16)
+        cores = host_cpu()["threads"]
         oracle.set_num_threads(cores)
         n_s = args.cpu_sample or 2000
         un = np.ascontiguousarray(wl.users[:n_s], np.float32).copy()
@@ -361,7 +367,7 @@ def qs_train(args, idx, wl, label):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         oracle.build()
-        cores = min(os.cpu_count() or 1, 16)
+        cores = host_cpu()["threads"]
         oracle.set_num_threads(cores)
         n_s = args.cpu_sample or 500
         un = np.ascontiguousarray(wl.users[:n_s], np.float32).copy()
@@ -399,22 +405,55 @@ def qs_train(args, idx, wl, label):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_sample_q):
+def host_cpu():
+    """Threads the CPU baseline uses and what the host reports about its CPUs.
+    The threads are this process's CPU share: OMP_NUM_THREADS when the
+    environment sets it (one GPU's share of a shared host), else every CPU the
+    process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else allowed
+    return {"threads": min(threads, allowed), "nproc": allowed, "host_cpus": os.cpu_count(), "model": model,
+            "omp_num_threads_env": env or None}
+
+
+def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_max, seconds):
     """The oracle's US query path (reference algorithm, OpenMP over users as the
-    reference runs it) on the first n_sample_q queries, same table."""
+    reference runs it) on the first queries of the workload, same table, in
+    chunks of 4 until `seconds` have passed or n_max queries have run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     oracle.build()
-    cores = min(os.cpu_count() or 1, 16)
-    oracle.set_num_threads(cores)
-    q = wl.queries[:n_sample_q]
+    hc = host_cpu()
+    oracle.set_num_threads(hc["threads"])
+    n_max = min(n_max, len(wl.queries))
+    ou_all = []
+    n = 0
     t0 = time.perf_counter()
-    ou, orank, oq = oracle.us_retrieval(users_norm, wl.items, table, ranks, q, k)
+    while n < n_max and (time.perf_counter() - t0 < seconds or n == 0):
+        m = min(4, n_max - n)
+        ou, _, _ = oracle.us_retrieval(users_norm, wl.items, table, ranks, wl.queries[n:n + m], k)
+        ou_all.extend(ou)
+        n += m
     dt = time.perf_counter() - t0
-    agree = float(np.mean([set(ou[i]) == set(gpu_users[i, :k]) for i in range(len(q))]))
-    return {"value": round(len(q) / dt, 4), "unit": "queries/s", "cores": cores, "kind": "port",
-            "sample": f"first {len(q)} of the same queries, same table (US path, OpenMP, -O3 no fast-math)",
-            "seconds": round(dt, 3), "user_set_agreement": agree}
+    agree = float(np.mean([set(ou_all[i]) == set(gpu_users[i, :k]) for i in range(n)]))
+    return {"value": round(n / dt, 4), "unit": "queries/s", "cores": hc["threads"], "kind": "port",
+            "sample": f"first {n} of the same queries (k={k}), same sampled table: US path, OpenMP over "
+                      f"{hc['threads']} threads, oracle -O3 without fast-math",
+            "seconds": round(dt, 3), "user_set_agreement": agree, "cpu_model": hc["model"],
+            "nproc": hc["nproc"], "host_cpus": hc["host_cpus"], "omp_num_threads_env": hc["omp_num_threads_env"]}
 
 
 # HBM traffic of the dominant kernel's launch, from the rocprofv3 --pmc runs
@@ -424,6 +463,8 @@ def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_sample_q):
 PROFILED_TRAFFIC = {
     ("plain_gpu", "screened"): {"kernel": "k_ip_coarse", "bytes": 2.529e8, "fetch_kib": 74559.9,
                                 "write_kib": 97847.8, "profile": "profiles/k_ip_coarse_100k_x_20k_d150_k50.md"},
+    ("precision", "screened"): {"kernel": "k_ip_coarse", "bytes": 2.505e9, "fetch_kib": 733558.7,
+                                "write_kib": 979018.9, "profile": "profiles/k_ip_coarse_1M_x_50k_d150_k50.md"},
 }
 
 
@@ -580,16 +621,53 @@ def main():
         roof["traffic"] = prof["bytes"]
         roof["traffic_source"] = prof["profile"]
 
+    q_cols = idx.query_table_cols()
     cpu = None
     if not args.no_cpu:
         T = idx.get_table()
         users_norm = np.ascontiguousarray(wl.users, np.float32).copy()
         rkr.load().rkr_normalize_users(users_norm.ctypes.data, U, d)
-        n_s = args.cpu_sample or 20
+        n_s = args.cpu_sample or 256
         k_cpu = int(ks[0])
         # CPU sample runs one k (the reference API takes a single topk)
         ref_users = idx.query(wl.queries[:n_s], np.full(n_s, k_cpu, np.int32)).users
-        cpu = cpu_baseline(wl, T, ranks, users_norm, ref_users, k_cpu, n_s)
+        cpu = cpu_baseline(wl, T, ranks, users_norm, ref_users, k_cpu, n_s, args.cpu_seconds)
+        del T, users_norm
+
+    # The same queries through the other forms, so that every ratio compares
+    # like with like: the query table (every rank to 4096 columns) is a part of
+    # this build that the reference's sampled table lacks.
+    compare = None
+    if not args.no_compare and not args.pre_updates:
+        compare = {}
+        ref = res
+        built = {"dense": args.dense_top}
+
+        def timed(tag, form, dense, coarse, steps, warm=1):
+            idx.set_form(form)
+            idx.set_option(9, coarse)
+            if dense != built["dense"]:
+                idx.set_option(8, dense)
+                idx.build_table(ranks)
+                built["dense"] = dense
+            for _ in range(warm):
+                idx.query(wl.queries, ks, max_k=max_k)
+            idx.lib.rkr_sync(idx.ctx)
+            a = time.perf_counter()
+            for _ in range(steps):
+                r = idx.query(wl.queries, ks, max_k=max_k)
+            idx.lib.rkr_sync(idx.ctx)
+            q_s = steps * nq / (time.perf_counter() - a)
+            same = bool(np.array_equal(r.users, ref.users) and np.array_equal(r.ranks, ref.ranks))
+            compare[tag] = {"queries_per_s": round(q_s, 2), "form": form, "query_table_cols": idx.query_table_cols(),
+                            "coarse": bool(coarse), "answers_equal_headline": same,
+                            "headline_over_this": round(qps / q_s, 2)}
+
+        cmp_steps = max(1, min(args.steps, 3))
+        if args.form == "screened":
+            timed("screened_sampled_table_only", "screened", 0, args.coarse, cmp_steps)
+        timed("plain_fp32_sampled_table_only", "plain", 0, 0, 1)
+        timed("plain_fp32_same_query_table", "plain", args.dense_top, 0, cmp_steps)
 
     out = {
         "metric": "queries/s at k=50, exact ranks (plus p50/p99 latency and score-table build time)",
@@ -609,9 +687,10 @@ def main():
         "data": "synthetic (seeded), inputs resident in HBM",
         "config": {"workload": label, "form": args.form, "queries_per_step": nq,
                    "distinct_queries_per_step": int(len(np.unique(wl.queries, axis=0))),
-                   "dedup": not args.no_dedup, "dense_top": bool(args.dense_top), "coarse": bool(args.coarse),
-                   "n_user": U, "n_item": I, "dim": d, "n_sample": wl.n_sample,
+                   "dedup": not args.no_dedup, "dense_top": args.dense_top, "query_table_cols": q_cols,
+                   "coarse": bool(args.coarse), "n_user": U, "n_item": I, "dim": d, "n_sample": wl.n_sample,
                    "autotuned": tuned},
+        "distinct_queries_per_s": round(qps * n_distinct / nq, 2),
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "latency_note": "single-query calls (batch of 1), host wall time incl. PCIe copies",
@@ -629,6 +708,7 @@ def main():
                            "second_pass_queries_per_step": round(tot["n_coarse_refined"] / max(1, args.steps), 2)}
                           if tot.get("n_listed", 0) or tot.get("n_fallback", 0) else None),
         "roofline": roof,
+        "same_queries_other_forms": compare,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -169,7 +169,9 @@ int rkr_get_ranks(const rkr_ctx *ctx, int32_t *out_ranks, int32_t *out_n_sample)
 
 /* Table file in the layout of SampleSearch::SaveIndex / LoadIndex
  * (SampleSearch.hpp:186-194, 229-241): size_t n_sample, size_t n_item,
- * size_t n_user, int32 ranks[n_sample], float table[n_user][n_sample]. */
+ * size_t n_user, int32 ranks[n_sample], float table[n_user][n_sample].
+ * rkr_save_table returns RKR_ERR_STATE while item inserts or deletes are
+ * pending (the rows describe the base item set): rebuild first. */
 int rkr_save_table(rkr_ctx *ctx, const char *path);
 int rkr_load_table(rkr_ctx *ctx, const char *path, float eps);
 
@@ -247,7 +249,9 @@ int rkr_find_optimal_sample(rkr_ctx *ctx, int64_t n_user, int64_t n_item, int32_
  * Outputs are n_query x max_k, row-major; entries past topk[i] are -1.
  * out_qip receives <u,q> of each returned user (may be NULL); stats
  * (n_query entries) and timing may be NULL.  With RKR_OPT_DEDUP on, byte-equal
- * query vectors are computed once and share that run's stats.  Replaces US::Index::Retrieval
+ * query vectors are computed once, with the largest k asked for them, and share
+ * that run's stats (counters of the largest k); under RKR_OUTPUT_REFERENCE only
+ * queries equal in vector and k share a run.  Replaces US::Index::Retrieval
  * (src/impl/US.hpp:96-213), which BaseIndex::Retrieval declares
  * (src/include/struct/MethodBase.hpp:14-30). */
 int rkr_query(rkr_ctx *ctx, const float *queries, int32_t n_query, const int32_t *topk, int32_t max_k,
@@ -257,6 +261,11 @@ int rkr_query(rkr_ctx *ctx, const float *queries, int32_t n_query, const int32_t
 /* Bytes of device memory the index holds (table + ranks), as
  * SampleSearch::IndexSizeByte (SampleSearch.hpp:246-250). */
 int64_t rkr_index_size_bytes(const rkr_ctx *ctx);
+
+/* Columns of the query table exact output searches (RKR_OPT_DENSE_TOP), or 0
+ * when the context has none and searches the sampled table.  No counterpart
+ * in the reference, whose only table is the sampled one. */
+int32_t rkr_query_table_cols(const rkr_ctx *ctx);
 
 /* ---- updates (exact output only) -------------------------------------------
  * The table keeps describing the item set it was built from; each user keeps

@@ -61,6 +61,8 @@ def lib():
                                 C.POINTER(C.c_int32), _i8p, _i8p]
         L.orc_refine_rank.argtypes = [_f32p, _f32p, _i64, _i32, C.c_float]
         L.orc_refine_rank.restype = C.c_int32
+        L.orc_refine_ranks.argtypes = [_f32p, _f32p, _f32p, _i64, _i32, C.c_void_p, C.c_void_p, _i64, _i32p,
+                                       C.c_void_p]
         L.orc_us_retrieval.argtypes = [_f32p, _i64, _f32p, _i64, _i32, _f32p, _i32p, _i32, _f32p, _i32, _i32,
                                        _i32p, _i32p, _f32p, C.c_void_p]
         L.orc_us_retrieval.restype = C.c_int
@@ -176,6 +178,21 @@ def prune(lb, ub, topk, prune_l=None, result_l=None, n_result=0, n_prune=0, refi
 def refine_rank(user, items, qip) -> int:
     user, items = _c32(user), _c32(items)
     return int(lib().orc_refine_rank(user, items, items.shape[0], items.shape[1], float(np.float32(qip))))
+
+
+def refine_ranks(users, queries, items, user_id, q_id):
+    """refine_rank of many (user, query) pairs on every oracle thread: returns
+    (ranks, <u,q>) with the query's InnerProduct computed as the reference's
+    GEMV loop does (US.hpp:124-136)."""
+    users, queries, items = _c32(users), _c32(queries), _c32(items)
+    uid = np.ascontiguousarray(user_id, np.int64)
+    qid = np.ascontiguousarray(q_id, np.int64)
+    n = uid.shape[0]
+    out = np.empty(n, np.int32)
+    qip = np.empty(n, np.float32)
+    lib().orc_refine_ranks(users, queries, items, items.shape[0], users.shape[1], uid.ctypes.data,
+                           qid.ctypes.data, n, out, qip.ctypes.data)
+    return out, qip
 
 
 def _query_out(nq, k):

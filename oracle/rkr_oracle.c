@@ -260,6 +260,20 @@ int32_t orc_refine_rank(const float *user, const float *items, int64_t n_item, i
     return rank;
 }
 
+/* orc_refine_rank for n (user, query) pairs, OpenMP over the pairs:       */
+/* user_id[i] indexes users, q_id[i] indexes queries; out[i] receives the  */
+/* strict count and qip_out[i] (may be NULL) the pair's InnerProduct.      */
+void orc_refine_ranks(const float *users, const float *queries, const float *items, int64_t n_item, int32_t d,
+                      const int64_t *user_id, const int64_t *q_id, int64_t n, int32_t *out, float *qip_out) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t i = 0; i < n; i++) {
+        const float *u = users + user_id[i] * (int64_t)d;
+        const float qip = orc_ip_f32(queries + q_id[i] * (int64_t)d, u, d);
+        out[i] = orc_refine_rank(u, items, n_item, d, qip);
+        if (qip_out) qip_out[i] = qip;
+    }
+}
+
 typedef struct { int32_t user, rank; float qip; } orc_elem;
 
 /* UserRankElement::operator<, src/include/struct/UserRankElement.hpp:59-64 */

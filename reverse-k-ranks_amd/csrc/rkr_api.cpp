@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -697,15 +698,28 @@ struct Staging {
     ~Staging() {
         (void)hipStreamSynchronize(s);  // no copy into a buffer outlives it
         for (int k = 0; k < 2; ++k) {
-            if (p[k]) (void)hipHostFree(p[k]);
+            if (p[k]) {
+                if (pinned) (void)hipHostFree(p[k]);
+                else std::free(p[k]);
+            }
             if (e[k]) (void)hipEventDestroy(e[k]);
         }
     }
+    bool pinned = true;
+    // Pinned buffers when the runtime grants them; otherwise ordinary pageable
+    // buffers, which the same asynchronous copies fill at the pageable rate.
     hipError_t init(size_t bytes) {
         chunk = std::max<size_t>(bytes, 1);
+        for (int k = 0; k < 2 && pinned; ++k)
+            if (hipHostMalloc(&p[k], chunk, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                for (int j = 0; j < k; ++j) (void)hipHostFree(p[j]);
+                p[0] = p[1] = nullptr;
+                pinned = false;
+            }
         for (int k = 0; k < 2; ++k) {
-            hipError_t r = hipHostMalloc(&p[k], chunk, hipHostMallocDefault);
-            if (r == hipSuccess) r = hipEventCreateWithFlags(&e[k], hipEventDisableTiming);
+            if (!pinned && !(p[k] = std::malloc(chunk))) return hipErrorOutOfMemory;
+            const hipError_t r = hipEventCreateWithFlags(&e[k], hipEventDisableTiming);
             if (r != hipSuccess) return r;
         }
         return hipSuccess;
@@ -950,6 +964,12 @@ int rkr_save_table(rkr_ctx *c, const char *path) {
     // SampleSearch::SaveIndex layout (SampleSearch.hpp:186-194)
     if (!c || !path) return RKR_ERR_INVALID;
     if (!c->table) return set_err(c, RKR_ERR_STATE, "save_table: no table");
+    // The table describes the item set it was built from; with pending inserts
+    // or deletes a saved file would pair those rows with the current n_item and
+    // lose the updates' rank offsets when it is loaded back.
+    if (c->n_ins > 0 || c->n_del > 0)
+        return set_err(c, RKR_ERR_STATE,
+                       "save_table: item updates are pending; rebuild the table (rkr_build_table) before saving");
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
     // the table is written straight from the pinned staging buffers, one chunk
@@ -1007,6 +1027,8 @@ int64_t rkr_index_size_bytes(const rkr_ctx *c) {
     if (!c) return 0;
     return (int64_t)sizeof(int32_t) * c->n_sample + (int64_t)sizeof(float) * c->n_user * c->n_sample;
 }
+
+int32_t rkr_query_table_cols(const rkr_ctx *c) { return c && c->qtable ? c->n_qs : 0; }
 
 }  // extern "C"
 
@@ -1087,14 +1109,22 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
         return query_split(c, snap, queries, n_query, topk, max_k, out_user, out_rank, out_qip, stats, timing);
 
     // Identical query vectors (byte-equal) are answered once, with the largest
-    // k asked for them: the answer for k is the first k entries of the answer
-    // for any larger k, since (rank, userID) is a total order.
+    // k asked for them: under exact output the answer for k is the first k
+    // entries of the answer for any larger k, since (rank, userID) is a total
+    // order.  Reference output lists the users its bounds settle first, and that
+    // set depends on k (PruneCandidateByBound.hpp:35-77 keeps k-entry heaps), so
+    // there only queries equal in vector and k share a run.
     const size_t row = sizeof(float) * (size_t)c->dim;
+    const bool by_k = c->output == RKR_OUTPUT_REFERENCE;
+    std::vector<std::string> keys((size_t)n_query);
     std::unordered_map<std::string_view, int32_t> seen;
     seen.reserve((size_t)n_query * 2);
     std::vector<int32_t> rep_of((size_t)n_query), reps;
     for (int32_t i = 0; i < n_query; ++i) {
-        const std::string_view key(reinterpret_cast<const char *>(queries + (size_t)i * c->dim), row);
+        std::string &kb = keys[i];
+        kb.assign(reinterpret_cast<const char *>(queries + (size_t)i * c->dim), row);
+        if (by_k) kb.append(reinterpret_cast<const char *>(&topk[i]), sizeof(int32_t));
+        const std::string_view key(kb);
         auto it = seen.emplace(key, (int32_t)reps.size());
         if (it.second) reps.push_back(i);
         rep_of[i] = it.first->second;

@@ -50,7 +50,7 @@ EXPORTS = (
     "rkr_set_table", "rkr_get_table", "rkr_get_ranks", "rkr_save_table", "rkr_load_table", "rkr_sorted_scores",
     "rkr_save_score_table", "rkr_sample_items", "rkr_query_rank_distribution", "rkr_sort_kth_rank",
     "rkr_find_optimal_sample", "rkr_query",
-    "rkr_index_size_bytes", "rkr_insert_items", "rkr_delete_items", "rkr_insert_users", "rkr_delete_users",
+    "rkr_index_size_bytes", "rkr_query_table_cols", "rkr_insert_items", "rkr_delete_items", "rkr_insert_users", "rkr_delete_users",
     "rkr_get_counts", "rkr_reserve", "rkr_now", "rkr_server_start", "rkr_server_submit_query", "rkr_server_submit_update",
     "rkr_server_wait", "rkr_server_poll", "rkr_server_counters", "rkr_server_stop",
 )
@@ -162,8 +162,11 @@ def load():
     L.rkr_server_stop.argtypes = [vp]
     L.rkr_index_size_bytes.argtypes = [vp]
     L.rkr_index_size_bytes.restype = C.c_int64
+    L.rkr_query_table_cols.argtypes = [vp]
+    L.rkr_query_table_cols.restype = C.c_int32
     for name in EXPORTS:
-        if name not in ("rkr_destroy", "rkr_last_error", "rkr_version", "rkr_index_size_bytes", "rkr_now"):
+        if name not in ("rkr_destroy", "rkr_last_error", "rkr_version", "rkr_index_size_bytes", "rkr_now",
+                        "rkr_query_table_cols"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

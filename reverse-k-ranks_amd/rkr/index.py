@@ -194,6 +194,10 @@ class Index:
     def index_size_bytes(self) -> int:
         return int(self.lib.rkr_index_size_bytes(self.ctx))
 
+    def query_table_cols(self) -> int:
+        """Columns of the query table exact output searches (0: the sampled table)."""
+        return int(self.lib.rkr_query_table_cols(self.ctx))
+
     # ---- queries ----------------------------------------------------------------
     def query(self, queries, topk, max_k=None) -> QueryResult:
         queries = _f32(queries)

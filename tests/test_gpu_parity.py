@@ -104,6 +104,27 @@ def test_query_reference_output_fake30d(orc, fake30d, gpu_index_factory):
     assert np.array_equal(res.qips, oq)
 
 
+def test_query_reference_output_dedup_mixed_k(orc, fake30d, gpu_index_factory):
+    """Reference output with byte-equal queries of different k: the users the
+    bounds settle depend on k, so each (vector, k) must be answered as the
+    reference answers it alone (US.hpp:173-186), with deduplication on."""
+    users, items, queries = fake30d
+    _case("query reference-output duplicates with mixed k")
+    idx, ranks = gpu_index_factory(users, items, 128)
+    idx.set_output("reference")
+    pick = np.array([0, 1, 0, 2, 1, 0, 2, 0, 1, 1])
+    ks = np.array([3, 10, 25, 7, 40, 3, 60, 11, 10, 2], np.int32)
+    res = idx.query(queries[pick], ks, max_k=60)
+    un = _normalized(orc, users)
+    T = orc.build_table(un, items, ranks)
+    for i, (qi, k) in enumerate(zip(pick, ks)):
+        ou, orank, oq = orc.us_retrieval(un, items, T, ranks, queries[qi:qi + 1], int(k))
+        assert np.array_equal(res.users[i, :k], ou[0]), (i, k)
+        assert np.array_equal(res.ranks[i, :k], orank[0]), (i, k)
+        assert np.array_equal(res.qips[i, :k], oq[0]), (i, k)
+        assert np.all(res.users[i, k:] == -1)
+
+
 @pytest.mark.parametrize("nq", [1, 3, 16, 17, 100])
 def test_query_batch_shapes_agree(orc, fake30d, gpu_index_factory, nq):
     """The GEMV (<= 16 queries) and GEMM paths give identical answers."""

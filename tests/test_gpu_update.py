@@ -201,6 +201,38 @@ def test_rebuild_after_updates_matches_oracle_table(orc, gpu_index_factory):
     m.check(idx, cur[:8], 20)
 
 
+def test_save_table_refuses_pending_updates(orc, gpu_index_factory, tmp_path):
+    """A saved table must describe the item count in its header: with pending
+    inserts or deletes the save is refused, and after a rebuild the file
+    round-trips and answers stay exact."""
+    _case("save_table with pending updates")
+    import rkr
+    from rkr import formats
+    rng, users, items = _data(35, U=500, I=700)
+    idx, _ = gpu_index_factory(users, items, 20)
+    m = Mirror(orc, users, items)
+    new = rng.standard_normal((40, items.shape[1]), dtype=np.float32)
+    idx.insert_items(new)
+    m.insert_items(new)
+    p = str(tmp_path / "t.index")
+    with pytest.raises(rkr.RkrError):
+        idx.save_table(p)
+    idx.delete_items(np.array([5], np.int64))
+    m.delete_items([5])
+    with pytest.raises(rkr.RkrError):
+        idx.save_table(p)
+    m.check(idx, m.items()[:6], 15)  # the refusals left the context usable
+    cur = m.items()
+    ranks = idx.uniform_sample_ranks(len(cur), 20)
+    idx.build_table(ranks)
+    idx.save_table(p)
+    f_ranks, f_table, f_items = formats.read_table(p)
+    assert f_items == len(cur) and np.array_equal(f_ranks, ranks)
+    assert np.array_equal(f_table, orc.build_table(m.users, cur, ranks))
+    idx.load_table(p)
+    m.check(idx, cur[:6], 15)
+
+
 def test_reference_output_rejects_updates(gpu_index_factory):
     _case("reference output + updates")
     import rkr

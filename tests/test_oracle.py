@@ -233,6 +233,16 @@ def test_refine_rank_matches_reference_code(orc, fake30d):
                                                                          30, qip, u)
 
 
+def test_refine_ranks_batch_matches_golden(orc, fake30d):
+    """The batched, threaded form used by the full-size GPU tests gives the
+    golden (rank, <u,q>) of every pair."""
+    g = np.load(os.path.join(GOLDEN, "refine_vectors.npz"))
+    users, items, queries = fake30d
+    un = orc.normalize_users(users)
+    r, q = orc.refine_ranks(un, queries, items, g["user"], g["query"])
+    assert np.array_equal(r, g["rank"]) and np.array_equal(q, g["qip"])
+
+
 def test_refine_rank_strict_count(orc):
     items = np.array([[1.0], [2.0], [2.0], [3.0]], np.float32)
     u = np.array([1.0], np.float32)
