@@ -203,6 +203,8 @@ def us_retrieval(users, items, table, ranks, queries, topk, stats=False):
     """The reference US query path (US.hpp:96-213) over every query."""
     users, items, table, queries = _c32(users), _c32(items), _c32(table), _c32(queries)
     ranks = np.ascontiguousarray(ranks, np.int32)
+    if table.shape != (users.shape[0], ranks.shape[0]):
+        raise ValueError(f"table {table.shape} does not match {users.shape[0]} users x {ranks.shape[0]} ranks")
     nq = queries.shape[0]
     ou, orank, oq = _query_out(nq, topk)
     st = (QStat * max(nq, 1))()
@@ -219,6 +221,8 @@ def exact_query(users, items, table, ranks, queries, topk, eps=0.0, stats=False)
     """Exact reverse k-ranks through the sampled table, every query."""
     users, items, table, queries = _c32(users), _c32(items), _c32(table), _c32(queries)
     ranks = np.ascontiguousarray(ranks, np.int32)
+    if table.shape != (users.shape[0], ranks.shape[0]):
+        raise ValueError(f"table {table.shape} does not match {users.shape[0]} users x {ranks.shape[0]} ranks")
     nq = queries.shape[0]
     ks = np.broadcast_to(np.asarray(topk, np.int32), (nq,))
     kmax = int(ks.max()) if nq else 0

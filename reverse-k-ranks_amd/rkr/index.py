@@ -123,7 +123,9 @@ class Index:
     def get_table(self, user_begin: int = 0, n_rows=None) -> np.ndarray:
         S = C.c_int32()
         _check(self.lib, self.ctx, self.lib.rkr_get_ranks(self.ctx, None, C.byref(S)))
-        n_rows = self.n_user - user_begin if n_rows is None else n_rows
+        if n_rows is None:  # the library's count: updates may have come through a server
+            self._refresh_counts()
+            n_rows = self.n_user - user_begin
         out = np.empty((n_rows, S.value), np.float32)
         _check(self.lib, self.ctx, self.lib.rkr_get_table(self.ctx, user_begin, n_rows, _ptr(out)))
         return out
@@ -144,6 +146,7 @@ class Index:
     # ---- full score table (ComputeScoreTableBatch.hpp:79-92, ReadAll.hpp:80-87) ----
     def sorted_scores(self, user_begin: int = 0, n_rows=None) -> np.ndarray:
         """Every item's score for users [user_begin, user_begin + n_rows), each row descending."""
+        self._refresh_counts()
         n_rows = self.n_user - user_begin if n_rows is None else n_rows
         out = np.empty((n_rows, self.n_item), np.float32)
         _check(self.lib, self.ctx, self.lib.rkr_sorted_scores(self.ctx, user_begin, n_rows, _ptr(out)))
