@@ -186,6 +186,19 @@ int rkr_load_table(rkr_ctx *ctx, const char *path, float eps);
  * with up to 16 host threads moving data out of them. */
 int rkr_sorted_scores(rkr_ctx *ctx, int64_t user_begin, int64_t n_rows, float *out);
 
+/* rkr_sorted_scores without any allocation in the call.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  out_dev is
+ * caller-owned DEVICE memory on the context's GPU, n_rows x n_item floats,
+ * row-major; the rows are identical to rkr_sorted_scores'.  The call only
+ * enqueues work on the context's stream (rkr_sync waits for it).  The row
+ * sort's scratch is a fixed 32 MiB device array per GPU, held until the
+ * process ends: n_item may be at most 8,388,608 (RKR_ERR_INVALID above), and
+ * rows run in chunks of 8,388,608 / n_item.  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (two contexts' streams at once)
+ * race; calls on one context's stream follow each other and are safe. */
+int rkr_sorted_scores_noalloc(rkr_ctx *ctx, int64_t user_begin, int64_t n_rows, float *out_dev);
+
 /* Writes the sorted score rows of the first n_rows users (n_rows < 0: every
  * user) to path in the layout of ReadAll's index file
  * (src/include/alg/DiskIndex/ReadAll.hpp:80-87, as

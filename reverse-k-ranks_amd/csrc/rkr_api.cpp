@@ -352,6 +352,9 @@ int rkr_create(int device, size_t mem_budget_bytes, rkr_ctx **out) {
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) { rkr_destroy(c); return RKR_ERR_HIP; }
     if (slot_init(c, c->main, false) != RKR_OK) { rkr_destroy(c); return RKR_ERR_HIP; }
+    // the no-allocation entry point's scratch: its address on this device,
+    // looked up once per device (a failure is reported by that entry point)
+    (void)noalloc_init(device);
     *out = c;
     return RKR_OK;
 }

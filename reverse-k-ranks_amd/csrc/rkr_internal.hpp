@@ -469,6 +469,10 @@ hipError_t launch_topk(hipStream_t s, const QueryInfo *qi, int32_t nq, const int
                        int32_t *out_user, int32_t *out_rank, float *out_qip, int64_t *scratch_keys);
 
 // ---- update path (k_update.hip) ---------------------------------------------
+// The sort scratch of rkr_sorted_scores_noalloc: looks up its device address
+// on `device` (current) once per device.
+hipError_t noalloc_init(int device);
+
 hipError_t launch_sort_rows_desc(hipStream_t s, float *x, int64_t n_rows, int32_t n, int64_t ld);
 hipError_t launch_merge_rows(hipStream_t s, const float *a, int64_t lda, int32_t na, const float *b, int64_t ldb,
                              int32_t nb, float *out, int64_t ldo, int64_t n_rows);

@@ -48,7 +48,7 @@ EXPORTS = (
     "rkr_create", "rkr_destroy", "rkr_last_error", "rkr_set_option", "rkr_get_option", "rkr_sync",
     "rkr_version", "rkr_normalize_users", "rkr_set_data", "rkr_uniform_sample_ranks", "rkr_build_table",
     "rkr_set_table", "rkr_get_table", "rkr_get_ranks", "rkr_save_table", "rkr_load_table", "rkr_sorted_scores",
-    "rkr_save_score_table", "rkr_sample_items", "rkr_query_rank_distribution", "rkr_sort_kth_rank",
+    "rkr_sorted_scores_noalloc", "rkr_save_score_table", "rkr_sample_items", "rkr_query_rank_distribution", "rkr_sort_kth_rank",
     "rkr_find_optimal_sample", "rkr_query",
     "rkr_index_size_bytes", "rkr_query_table_cols", "rkr_insert_items", "rkr_delete_items", "rkr_insert_users", "rkr_delete_users",
     "rkr_get_counts", "rkr_reserve", "rkr_now", "rkr_server_start", "rkr_server_submit_query", "rkr_server_submit_update",
@@ -139,6 +139,7 @@ def load():
     L.rkr_save_table.argtypes = [vp, C.c_char_p]
     L.rkr_load_table.argtypes = [vp, C.c_char_p, C.c_float]
     L.rkr_sorted_scores.argtypes = [vp, i64, i64, f32p]
+    L.rkr_sorted_scores_noalloc.argtypes = [vp, i64, i64, C.c_void_p]
     L.rkr_save_score_table.argtypes = [vp, C.c_char_p, i64, C.c_void_p]
     L.rkr_sample_items.argtypes = [i64, i32, i32p]
     L.rkr_query_rank_distribution.argtypes = [vp, i32p, i32, i32p, C.POINTER(C.c_double)]
