@@ -77,6 +77,13 @@ typedef struct rkr_ctx rkr_ctx;
                                    7 guard columns per user, and only the users
                                    this cannot rule out are scored exactly and
                                    searched; answers are unchanged.             */
+#define RKR_OPT_QSRP 11         /* 0 (default); 1: exact queries without pending
+                                   updates screen users with the QSRP bounds that
+                                   rkr_qsrp_build made (partial-dimension inner
+                                   product bound + per-user regression rank
+                                   bound) before exact scores and the table
+                                   search; it replaces the coarse bounds.
+                                   Answers are unchanged.                        */
 
 /* Per-query counters; mirrors SingleQueryPerformance
  * (src/include/util/FileIO.hpp:150-173) plus the screening figures. */
@@ -279,6 +286,42 @@ int64_t rkr_index_size_bytes(const rkr_ctx *ctx);
  * when the context has none and searches the sampled table.  No counterpart
  * in the reference, whose only table is the sampled one. */
 int32_t rkr_query_table_cols(const rkr_ctx *ctx);
+
+/* ---- QSRP screen ------------------------------------------------------------
+ * The reference's QSRPNormalLP method (src/impl/QSRPNormalLP.hpp:137-253):
+ * users rotated into the SVD basis of the user matrix (SVD::PerformSVD,
+ * src/include/alg/SVD.hpp:62-123), an inner-product interval from the first
+ * check_dim dimensions plus a Cauchy-Schwarz bound of the rest
+ * (PartDimPartNorm::IPBound, src/include/alg/QueryIPBound/PartDimPartNorm.hpp:
+ * 52-74), and a per-user line from the normal CDF of a score to its sample
+ * position, fitted on the host as the minimax (LP) fit of
+ * NormalLinearRegressionLP (src/include/alg/RegressionPruning/
+ * NormalLinearRegressionLP.hpp:149-283) and evaluated as ComputeRankBound
+ * (:326-398).  Here every bound carries its rounding margins, so the screen
+ * only ever removes users that cannot be in the answer; answers stay exact.
+ *
+ * rkr_qsrp_build fits the screen to the current users and sampled table
+ * (SIGMA: the singular-value share that sets check_dim, the reference's
+ * default 0.7); the screen is dropped when the table or the users change.
+ * build_ms (may be NULL) receives the host time of the build. */
+int rkr_qsrp_build(rkr_ctx *ctx, float sigma, double *build_ms);
+/* check_dim of the built screen, and the users whose minimax line was not
+ * decreasing and took the reference's fallback line.  RKR_ERR_STATE when
+ * none is built. */
+int rkr_qsrp_info(const rkr_ctx *ctx, int32_t *check_dim, int64_t *n_fallback_fit);
+/* The screen's bucket bounds of every (query, user), before any exact score:
+ * out[q * n_user + u] = b_lo | b_hi << 16 with b_hi <= #{T[u][j] > <u,q>} <=
+ * b_lo over the searched table T (the query table when there is one).  For
+ * checking the screen; rkr_query runs it inside the batch. */
+int rkr_qsrp_screen(rkr_ctx *ctx, const float *queries, int32_t n_query, uint32_t *out);
+/* The device's CDF (A&S 7.1.26, CDFPhi of NormalLinearRegressionLP.hpp:125-
+ * 147, in FP32) at n points, for checking the error bound the screen assumes. */
+int rkr_qsrp_phi(rkr_ctx *ctx, const float *z, int64_t n, float *out);
+/* Host helper (no device): the minimax line through n points (x_i, y_i) that
+ * rkr_qsrp_build fits per user -- the optimum of NormalLinearRegressionLP's LP
+ * (minimise e subject to |a x_i + b - y_i| <= e).  RKR_ERR_INVALID when the
+ * points have fewer than two distinct x. */
+int rkr_qsrp_fit_line(const double *x, const double *y, int32_t n, double *a, double *b, double *e);
 
 /* ---- updates (exact output only) -------------------------------------------
  * The table keeps describing the item set it was built from; each user keeps

@@ -74,6 +74,14 @@ class Index {
         return ms;
     }
     void LoadTable(const std::string &path, float eps = 0.0f) { check(rkr_load_table(ctx_.get(), path.c_str(), eps)); }
+    // QSRPNormalLP's screen (SVD.hpp:62-123 + NormalLinearRegressionLP.hpp:149-283) fitted to the
+    // loaded table and users, then used by Retrieval; SIGMA as BuildSVDIndex's --SIGMA (default 0.7).
+    double BuildQSRP(float sigma = 0.7f) {
+        double ms = 0;
+        check(rkr_qsrp_build(ctx_.get(), sigma, &ms));
+        check(rkr_set_option(ctx_.get(), RKR_OPT_QSRP, 1));
+        return ms;
+    }
     void SaveTable(const std::string &path) { check(rkr_save_table(ctx_.get(), path.c_str())); }
     // ComputeScoreTableBatch::ComputeSortItemsBatch (ComputeScoreTableBatch.hpp:79-92): batch_n_user rows of
     // n_item scores, each sorted in descending order, into distance_l.
@@ -184,6 +192,19 @@ inline std::unique_ptr<Index> BuildIndexQS(VectorMatrix &data_item, VectorMatrix
     std::snprintf(path, sizeof path, "%s/memory_index/QS-%s-n_sample_%d-n_sample_query_%d-sample_topk_%d.index",
                   index_dir, dataset_name, n_sample, n_sample_query, sample_topk);
     idx->LoadTable(path);
+    return idx;
+}
+
+// QSRPNormalLP::BuildIndex (src/impl/QSRPNormalLP.hpp:258-322, called at ReverseRankIndex.cpp:125-135):
+// the QS table, then the SVD rotation and the per-user regression, which the reference loads from the
+// files bsvdi and bri write and which are fitted here when the index is built.
+inline std::unique_ptr<Index> BuildIndexQSRP(VectorMatrix &data_item, VectorMatrix &user, const char *dataset_name,
+                                             const int &n_sample, const int &n_sample_query, const int &sample_topk,
+                                             const int &n_thread, const char *index_dir, float sigma = 0.7f,
+                                             int device = 0) {
+    auto idx = BuildIndexQS(data_item, user, dataset_name, n_sample, n_sample_query, sample_topk, n_thread, index_dir,
+                            device);
+    idx->BuildQSRP(sigma);
     return idx;
 }
 

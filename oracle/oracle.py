@@ -412,3 +412,53 @@ def sample_set_cost(n_user, sort_kth, accu, ranks):
         else:
             tot += int(accu[q][rs[i]] - accu[q][rs[i - 1]])
     return tot
+
+
+# ---- QSRP screen (QSRPNormalLP: SVD + PartDimPartNorm + NormalLinearRegressionLP) ----
+
+def cdf_phi(x: float) -> float:
+    """CDFPhi, src/include/alg/RegressionPruning/NormalLinearRegressionLP.hpp:125-147:
+    Abramowitz & Stegun 7.1.26 for erf (absolute error <= 1.5e-7), in double."""
+    import math
+    a1, a2, a3, a4, a5, p = 0.254829592, -0.284496736, 1.421413741, -1.453152027, 1.061405429, 0.3275911
+    sign = -1.0 if x < 0 else 1.0
+    z = abs(x) / math.sqrt(2.0)
+    t = 1.0 / (1.0 + p * z)
+    y = 1.0 - (((((a5 * t + a4) * t) + a3) * t + a2) * t + a1) * t * math.exp(-z * z)
+    return 0.5 * (1.0 + sign * y)
+
+
+def regression_points(T, mu, inv_sigma):
+    """The constraint points of NormalLinearRegressionLP::AssignCache
+    (NormalLinearRegressionLP.hpp:149-181): for sample interval j, sample position j + 1
+    against the CDF at both ends of the interval."""
+    xs, ys = [], []
+    for j in range(len(T) - 1):
+        for v in (T[j], T[j + 1]):
+            xs.append(cdf_phi(float(np.float32(np.float32(v) - np.float32(mu)) * np.float32(inv_sigma))))
+            ys.append(float(j + 1))
+    return np.array(xs), np.array(ys)
+
+
+def lp_min_error_bruteforce(x, y):
+    """The LP of NormalLinearRegressionLP::LinearProgram (:183-213): minimise e over (a, b, e)
+    subject to |a x_i + b - y_i| <= e, by enumerating every vertex (three tight constraints,
+    a 3x3 solve) and keeping the feasible one with the smallest e.  Brute force for small n."""
+    import itertools
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    # a constraint is tight when a x_i + b - y_i = s e for s = +1 or -1:
+    # the row (x_i, 1, -s) . (a, b, e) = y_i
+    rows = [(xi, 1.0, -sgn, yi) for xi, yi in zip(x, y) for sgn in (1.0, -1.0)]
+    best = np.inf
+    for r1, r2, r3 in itertools.combinations(range(len(rows)), 3):
+        A = np.array([rows[r][:3] for r in (r1, r2, r3)])
+        rhs = np.array([rows[r][3] for r in (r1, r2, r3)])
+        if abs(np.linalg.det(A)) < 1e-12:
+            continue
+        a, b, e = np.linalg.solve(A, rhs)
+        if e < -1e-12:
+            continue
+        if np.all(np.abs(a * x + b - y) <= e + 1e-9 * (1 + abs(e))):
+            best = min(best, e)
+    return best

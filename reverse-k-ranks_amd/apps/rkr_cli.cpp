@@ -108,7 +108,9 @@ bool read_data(const Args &a, rkr::VectorMatrix &user, rkr::VectorMatrix &item, 
     return true;
 }
 
-bool is_qs(const Args &a) { return get(a, "method_name", "US") == "QS"; }
+// QSRPNormalLP runs on the QS table (SampleSearchIndexName, NameTranslation.hpp:36-48)
+bool is_qsrp(const Args &a) { return get(a, "method_name", "US") == "QSRPNormalLP"; }
+bool is_qs(const Args &a) { return get(a, "method_name", "US") == "QS" || is_qsrp(a); }
 
 // SampleSearch::SaveIndex / LoadIndex names (SampleSearch.hpp:158-179): QS adds the training parameters
 std::string index_file(const Args &a, const char *sub, int S) {
@@ -321,6 +323,7 @@ int cmd_rri(const Args &a) {
                                    std::stoi(get(a, "device", "0")));
         idx->SetForm(form == "screened" ? RKR_FORM_SCREENED : RKR_FORM_PLAIN);
         idx->SetOutput(output == "reference" ? RKR_OUTPUT_REFERENCE : RKR_OUTPUT_EXACT);
+        if (is_qsrp(a)) idx->BuildQSRP(std::stof(get(a, "SIGMA", "0.7")));  // BuildSVDIndex's --SIGMA
         const double build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         std::vector<rkr::SingleQueryPerformance> perf;
         auto res = idx->Retrieval(query, topk, query.n_vector_, perf);
@@ -332,7 +335,7 @@ int cmd_rri(const Args &a) {
                           S, get(a, "n_sample_query", "150").c_str(), get(a, "sample_topk", "50").c_str(), -1);
         else
             std::snprintf(param, sizeof param, "top%d-n_sample_%d-n_thread_%d", topk, S, -1);
-        const std::string method = is_qs(a) ? "-QS-" : "-US-";
+        const std::string method = is_qsrp(a) ? "-QSRPNormalLP-" : is_qs(a) ? "-QS-" : "-US-";
         mkdirs(res_dir + "/rank");
         mkdirs(res_dir + "/single_query_performance");
         const std::string stem = res_dir + "/rank/" + ds + method + param;
@@ -374,7 +377,7 @@ int main(int argc, char **argv) {
     const Args a = parse(argc, argv, 2);
     if (cmd == "qdibc") return cmd_qdibc(a);
     if (get(a, "method_name", "US") != "US" && !is_qs(a)) {
-        std::fprintf(stderr, "method_name US and QS are implemented\n");
+        std::fprintf(stderr, "method_name US, QS and QSRPNormalLP are implemented\n");
         return 2;
     }
     if (cmd == "fsr") return cmd_fsr(a);

@@ -11,15 +11,16 @@
 // One 1024-thread workgroup per row, all state in LDS (up to 160 KB on
 // gfx950):
 //   1. min/max of the row;
-//   2. histogram over NB linear value bins; bin(x) is monotone in x, so a
-//      reverse prefix sum tells, for every r_j, its bin and its residual rank
-//      inside the bin;
-//   3. the bins holding targets are gathered into LDS (as many per pass as
-//      fit), and each bin is ranked by one wavefront: a counting select for
-//      bins of <= 32 elements, a register bitonic sort up to 256;
-//   4. bins too large for that are narrowed by repeated histograms over
-//      their value range (wg_select) -- this also makes ties and skewed
-//      rows exact.
+//   2. histogram over NB linear value bins (FP32 arithmetic: fl(x - lo) and
+//      fl(t * scale) are monotone in x, so the bin is too); a reverse prefix
+//      sum tells, for every r_j, its bin and the elements above that bin;
+//   3. one more pass over the row gathers the elements of the bins holding
+//      targets into LDS (the histogram's storage), with a bitmap of target
+//      bins deciding membership -- several passes only when they exceed it;
+//   4. each bin is ranked by one thread (<= 16 elements), one wavefront
+//      (register bitonic sort up to 256, counting above) or the whole
+//      workgroup (wg_select, which narrows the bin by repeated histograms --
+//      this also makes ties and skewed rows exact).
 #include "rkr_internal.hpp"
 
 namespace rkr {
@@ -27,34 +28,41 @@ namespace {
 
 constexpr int NT = 1024;          // threads per row
 constexpr int NW = NT / 64;       // wavefronts per row
-constexpr int NB = 4096;          // level-0 bins
+constexpr int NB = 16384;         // level-0 bins
 constexpr int NB2 = 1024;         // bins of the narrowing loop
 constexpr int SMAX = 4096;        // largest n_sample handled
-constexpr int GCAP = 22528;       // floats gathered per pass (88 KB)
+constexpr int GCAP = NB;          // floats gathered per pass (the histogram's 64 KB)
+constexpr int THREAD_SEL_MAX = 16;  // bins ranked by one thread
 constexpr int WAVE_SEL_MAX = 2048;  // bins larger than this use wg_select
-constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint16_t DEEP = 0x8000;
 
 struct SelSmem {
-    uint32_t incl[NB + 1];        // incl[b] = #elements in bins >= b
-    uint32_t cur[NB];             // gather cursor of member bins, NONE otherwise
-    uint32_t hist2[NB2 + 1];
+    union {
+        uint32_t incl[NB + 1];    // incl[b] = #elements in bins >= b (steps 2-3)
+        float gbuf[GCAP];         // gathered elements of the target bins (step 4)
+    } u;
+    uint32_t tbits[NB / 32];      // target bins of the current pass
+    uint16_t tpre[NB / 32];       // target bins of the pass below each bitmap word
     uint16_t tb[SMAX];            // bin of target j
-    uint16_t fb[SMAX];            // first target index of each distinct bin
+    uint16_t fb[SMAX];            // first target index of each distinct bin (bins descending)
+    uint32_t fh[SMAX];            // elements in distinct bin f
+    uint32_t fabove[SMAX];        // elements in the bins above distinct bin f
+    uint32_t fcur[SMAX];          // gather cursor of distinct bin f (ends at its segment's end)
     uint16_t grp[SMAX + 1];       // group start (index into fb) | DEEP
-    uint16_t coop[SMAX];          // fb indices needing wg_select in this group
+    uint16_t coop[SMAX];          // distinct bins of a group needing wg_select
+    uint32_t hist2[NB2 + 1];
     float red_f[2 * NW];
     uint32_t red_u[NW + 1];
-    int32_t nfb, ngrp, ncoop, sel_b, one_pass;
-    float gbuf[GCAP];
+    int32_t nfb, ngrp, ncoop, sel_b;
 };
 
-// Monotone in x for x in [lo, hi]; anything that is not >= 0 (NaN) maps to
-// bin 0 so an LDS index is always in range.
-__device__ __forceinline__ int bin_of(float x, float lo, double scale, int nb) {
-    const double t = ((double)x - (double)lo) * scale;
-    if (!(t >= 0.0)) return 0;
-    if (!(t < (double)nb)) return nb - 1;
+// Monotone in x for x >= lo with FP32 arithmetic (every rounding step is
+// monotone); NaN (only from 0 * inf at x == lo) maps to bin 0, overflow to the
+// last bin, so an LDS index is always in range.
+__device__ __forceinline__ int bin_f32(float x, float lo, float scale, int nb) {
+    const float t = (x - lo) * scale;
+    if (!(t >= 0.0f)) return 0;
+    if (!(t < (float)nb)) return nb - 1;
     return (int)t;
 }
 
@@ -187,9 +195,9 @@ __device__ uint32_t wg_excl_scan(SelSmem &sm, uint32_t v, uint32_t *total) {
 // Turns counts h[0..nb) into incl[b] = sum_{b' >= b} h[b'] in place
 // (incl[nb] = 0).  nb is a multiple of NT.
 __device__ void wg_reverse_incl(SelSmem &sm, uint32_t *h, int nb) {
-    const int per = nb / NT;
+    const int per = nb / NT;  // <= 16
     const int base = threadIdx.x * per;
-    uint32_t loc[4];
+    uint32_t loc[16];
     uint32_t s = 0;
     for (int i = 0; i < per; ++i) { loc[i] = h[base + i]; s += loc[i]; }
     uint32_t total;
@@ -218,12 +226,12 @@ __device__ float wg_select(SelSmem &sm, const float *src, int64_t n, float lo, f
     // loop ends for finite inputs; the round cap only guards non-finite data
     // (rejected by the host before any kernel runs).
     for (int round = 0; round < 256 && lo < hi; ++round) {
-        const double scale = (double)NB2 / ((double)hi - (double)lo);
+        const float scale = (float)NB2 / (hi - lo);
         for (int i = threadIdx.x; i < NB2; i += NT) sm.hist2[i] = 0;
         __syncthreads();
         for (int64_t i = threadIdx.x; i < n; i += NT) {
             const float x = src[i];
-            if (x >= lo && x <= hi) atomicAdd(&sm.hist2[bin_of(x, lo, scale, NB2)], 1u);
+            if (x >= lo && x <= hi) atomicAdd(&sm.hist2[bin_f32(x, lo, scale, NB2)], 1u);
         }
         __syncthreads();
         wg_reverse_incl(sm, sm.hist2, NB2);
@@ -234,13 +242,33 @@ __device__ float wg_select(SelSmem &sm, const float *src, int64_t n, float lo, f
         float mn = INFINITY, mx = -INFINITY;
         for (int64_t i = threadIdx.x; i < n; i += NT) {
             const float x = src[i];
-            if (x >= lo && x <= hi && bin_of(x, lo, scale, NB2) == b) { mn = fminf(mn, x); mx = fmaxf(mx, x); }
+            if (x >= lo && x <= hi && bin_f32(x, lo, scale, NB2) == b) { mn = fminf(mn, x); mx = fmaxf(mx, x); }
         }
         wg_minmax(sm, mn, mx);
         lo = mn;
         hi = mx;
     }
     return lo;
+}
+
+// Ranks one bin of h <= THREAD_SEL_MAX elements (gbuf[seg..seg+h)) in one
+// thread and writes T[j] for its targets j0 <= j < j1.
+__device__ __forceinline__ void thread_select(const float *gbuf, uint32_t seg, uint32_t h, uint32_t above,
+                                              const int32_t *__restrict__ ranks, int j0, int j1, float *T) {
+    float v[THREAD_SEL_MAX];
+#pragma unroll
+    for (int i = 0; i < THREAD_SEL_MAX; ++i) v[i] = (uint32_t)i < h ? gbuf[seg + i] : -INFINITY;
+    // position of element i in descending order, ties broken by index; the
+    // -inf padding ranks after every element of the bin (scores are finite)
+#pragma unroll
+    for (int i = 0; i < THREAD_SEL_MAX; ++i) {
+        uint32_t pos = 0;
+#pragma unroll
+        for (int k = 0; k < THREAD_SEL_MAX; ++k) pos += (v[k] > v[i] || (v[k] == v[i] && k < i)) ? 1u : 0u;
+        if ((uint32_t)i < h)
+            for (int j = j0; j < j1; ++j)
+                if ((uint32_t)ranks[j] - above == pos) T[j] = v[i];
+    }
 }
 
 __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ scores, int64_t n_item,
@@ -250,7 +278,7 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
     const int64_t row = blockIdx.x;
     const float *x = scores + row * n_item;
     float *T = table_out + row * (int64_t)S;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
 
     // 1. range
     float mn = INFINITY, mx = -INFINITY;
@@ -261,84 +289,91 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
         for (int j = tid; j < S; j += NT) T[j] = lo0;
         return;
     }
-    const double scale0 = (double)NB / ((double)hi0 - (double)lo0);
+    const float scale0 = (float)NB / (hi0 - lo0);
 
     // 2. histogram + reverse prefix sums
-    for (int i = tid; i < NB; i += NT) { sm.incl[i] = 0; sm.cur[i] = NONE; }
+    for (int i = tid; i < NB; i += NT) sm.u.incl[i] = 0;
+    for (int i = tid; i < NB / 32; i += NT) sm.tbits[i] = 0;
     __syncthreads();
-    row_for_each(x, n_item, [&](float v) { atomicAdd(&sm.incl[bin_of(v, lo0, scale0, NB)], 1u); });
+    row_for_each(x, n_item, [&](float v) { atomicAdd(&sm.u.incl[bin_f32(v, lo0, scale0, NB)], 1u); });
     __syncthreads();
-    wg_reverse_incl(sm, sm.incl, NB);
+    wg_reverse_incl(sm, sm.u.incl, NB);
 
     // bin of every target
-    for (int j = tid; j < S; j += NT) sm.tb[j] = (uint16_t)find_bin(sm.incl, NB, (uint32_t)ranks[j]);
+    for (int j = tid; j < S; j += NT) sm.tb[j] = (uint16_t)find_bin(sm.u.incl, NB, (uint32_t)ranks[j]);
     __syncthreads();
 
-    // 3. distinct target bins (parallel scans).  fb[] lists the first target of
-    // each distinct bin; seg[] its gather offset when all bins fit one pass.
+    // 3. distinct target bins (parallel scans): fb[f] is the first target of
+    // distinct bin f, fh[f] its element count, fabove[f] the elements above it
     {
         constexpr int PER = SMAX / NT;  // targets per thread (S <= SMAX)
-        uint32_t nf = 0, wsum = 0;
-        uint32_t fl[PER], wt[PER];
+        uint32_t nf = 0;
+        bool fl[PER];
         for (int i = 0; i < PER; ++i) {
             const int j = tid * PER + i;
-            fl[i] = wt[i] = 0;
-            if (j < S && (j == 0 || sm.tb[j] != sm.tb[j - 1])) {
-                const int b = sm.tb[j];
-                fl[i] = 1;
-                wt[i] = sm.incl[b] - sm.incl[b + 1];
-            }
-            nf += fl[i];
-            wsum += wt[i];
+            fl[i] = j < S && (j == 0 || sm.tb[j] != sm.tb[j - 1]);
+            nf += fl[i] ? 1u : 0u;
         }
-        uint32_t nf_tot, w_tot;
+        uint32_t nf_tot;
         uint32_t fpos = wg_excl_scan(sm, nf, &nf_tot);
-        uint32_t wpos = wg_excl_scan(sm, wsum, &w_tot);
         for (int i = 0; i < PER; ++i) {
             if (!fl[i]) continue;
             const int j = tid * PER + i;
+            const int b = sm.tb[j];
             sm.fb[fpos] = (uint16_t)j;
-            sm.cur[sm.tb[j]] = wpos;  // used only by the one-pass case below
+            sm.fh[fpos] = sm.u.incl[b] - sm.u.incl[b + 1];
+            sm.fabove[fpos] = sm.u.incl[b + 1];
             ++fpos;
-            wpos += wt[i];
         }
-        if (tid == 0) {
-            sm.nfb = (int)nf_tot;
-            sm.one_pass = w_tot <= GCAP;
-        }
+        if (tid == 0) sm.nfb = (int)nf_tot;
     }
     __syncthreads();
-    if (!sm.one_pass) {
-        // rows too wide for one pass: clear the offsets and pack bins greedily
-        // into passes of <= GCAP floats (a bin larger than GCAP is narrowed
-        // from global memory on its own)
-        for (int f = tid; f < sm.nfb; f += NT) sm.cur[sm.tb[sm.fb[f]]] = NONE;
-        __syncthreads();
+    // the histogram is no longer needed: its storage becomes the gather buffer.
+    // fcur[f] = elements of the distinct bins before f (a parallel scan); when
+    // they all fit, one pass gathers them at those offsets.  Otherwise bins are
+    // packed greedily into passes of <= GCAP floats, and a bin larger than
+    // GCAP is narrowed from global memory on its own.
+    const int nfb = sm.nfb;
+    {
+        constexpr int PER = SMAX / NT;
+        uint32_t h[PER], sum = 0;
+        for (int i = 0; i < PER; ++i) {
+            const int f = tid * PER + i;
+            h[i] = f < nfb ? sm.fh[f] : 0u;
+            sum += h[i];
+        }
+        uint32_t total;
+        uint32_t off = wg_excl_scan(sm, sum, &total);
+        for (int i = 0; i < PER; ++i) {
+            const int f = tid * PER + i;
+            if (f < nfb) sm.fcur[f] = off;
+            off += h[i];
+        }
         if (tid == 0) {
-            const int nfb = sm.nfb;
-            int ngrp = 0;
-            uint32_t fill = 0;
-            for (int f = 0; f < nfb; ++f) {
-                const int b = sm.tb[sm.fb[f]];
-                const uint32_t h = sm.incl[b] - sm.incl[b + 1];
-                if (h > GCAP) {
-                    sm.grp[ngrp++] = (uint16_t)(f | DEEP);
-                    fill = GCAP;  // next bin starts a new group
-                } else {
-                    if (ngrp == 0 || fill + h > GCAP) { sm.grp[ngrp++] = (uint16_t)f; fill = 0; }
-                    fill += h;
+            if (total <= GCAP) {
+                sm.grp[0] = 0;
+                sm.grp[1] = (uint16_t)nfb;
+                sm.ngrp = 1;
+            } else {
+                int ngrp = 0;
+                uint32_t fill = GCAP;
+                for (int f = 0; f < nfb; ++f) {
+                    const uint32_t hf = sm.fh[f];
+                    if (hf > GCAP) {
+                        sm.grp[ngrp++] = (uint16_t)(f | DEEP);
+                        fill = GCAP;  // next bin starts a new group
+                    } else {
+                        if (fill + hf > GCAP) { sm.grp[ngrp++] = (uint16_t)f; fill = 0; }
+                        fill += hf;
+                    }
                 }
+                sm.grp[ngrp] = (uint16_t)nfb;
+                sm.ngrp = ngrp;
             }
-            sm.grp[ngrp] = (uint16_t)nfb;
-            sm.ngrp = ngrp;
         }
-    } else if (tid == 0) {
-        sm.grp[0] = 0;
-        sm.grp[1] = (uint16_t)sm.nfb;
-        sm.ngrp = 1;
     }
     __syncthreads();
-    const int nfb = sm.nfb, ngrp = sm.ngrp;
+    const int ngrp = sm.ngrp;
 
     for (int g = 0; g < ngrp; ++g) {
         const bool deep = (sm.grp[g] & DEEP) != 0;
@@ -351,57 +386,76 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
             const int b = sm.tb[j0];
             float bmn = INFINITY, bmx = -INFINITY;
             row_for_each(x, n_item, [&](float v) {
-                if (bin_of(v, lo0, scale0, NB) == b) { bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
+                if (bin_f32(v, lo0, scale0, NB) == b) { bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
             });
             wg_minmax(sm, bmn, bmx);
             for (int j = j0; j < j1; ++j) {
-                const uint32_t t = (uint32_t)ranks[j] - sm.incl[b + 1];
+                const uint32_t t = (uint32_t)ranks[j] - sm.fabove[f0];
                 const float v = wg_select(sm, x, n_item, bmn, bmx, t);
                 if (tid == 0) T[j] = v;
             }
             continue;
         }
-        // segment offsets of the member bins (already set in the one-pass case)
-        if (tid == 0) {
-            if (!sm.one_pass) {
-                uint32_t off = 0;
-                for (int f = f0; f < f1; ++f) {
-                    const int b = sm.tb[sm.fb[f]];
-                    sm.cur[b] = off;
-                    off += sm.incl[b] - sm.incl[b + 1];
-                }
-            }
-            sm.ncoop = 0;
+        // member bins: bitmap bits, then segment starts (bins descending in f)
+        for (int f = f0 + tid; f < f1; f += NT) {
+            const int b = sm.tb[sm.fb[f]];
+            atomicOr(&sm.tbits[b >> 5], 1u << (b & 31));
+        }
+        if (ngrp > 1) {
+            // several passes: offsets restart at each group's first bin
+            __syncthreads();
+            const uint32_t base = sm.fcur[f0];
+            __syncthreads();
+            for (int f = f0 + tid; f < f1; f += NT) sm.fcur[f] -= base;
+        }
+        if (tid == 0) sm.ncoop = 0;
+        __syncthreads();
+        {
+            // member bins below each bitmap word (ascending bins)
+            const uint32_t cnt = tid < NB / 32 ? (uint32_t)__popc(sm.tbits[tid]) : 0u;
+            uint32_t tot;
+            const uint32_t pre = wg_excl_scan(sm, cnt, &tot);
+            if (tid < NB / 32) sm.tpre[tid] = (uint16_t)pre;
         }
         __syncthreads();
+        // gather: the member bin of rank m in ascending bin order is f1 - 1 - m
         row_for_each(x, n_item, [&](float v) {
-            const int b = bin_of(v, lo0, scale0, NB);
-            if (sm.cur[b] != NONE) sm.gbuf[atomicAdd(&sm.cur[b], 1u)] = v;
+            const int b = bin_f32(v, lo0, scale0, NB);
+            const uint32_t word = sm.tbits[b >> 5], bit = 1u << (b & 31);
+            if (word & bit) {
+                const int m = (int)sm.tpre[b >> 5] + __popc(word & (bit - 1u));
+                sm.u.gbuf[atomicAdd(&sm.fcur[f1 - 1 - m], 1u)] = v;
+            }
         });
         __syncthreads();
-        // one wavefront per bin: position of each element in descending order
-        const int w = tid >> 6, lane = tid & 63;
-        for (int f = f0 + w; f < f1; f += NW) {
+        // small bins: one thread each
+        for (int f = f0 + tid; f < f1; f += NT) {
+            const uint32_t h = sm.fh[f];
+            if (h > THREAD_SEL_MAX) continue;
             const int j0 = sm.fb[f];
             const int j1 = (f + 1 < nfb) ? sm.fb[f + 1] : S;
-            const int b = sm.tb[j0];
-            const uint32_t h = sm.incl[b] - sm.incl[b + 1];
-            const uint32_t seg = sm.cur[b] - h;
+            thread_select(sm.u.gbuf, sm.fcur[f] - h, h, sm.fabove[f], ranks, j0, j1, T);
+        }
+        // mid-sized bins: one wavefront each
+        for (int f = f0 + w; f < f1; f += NW) {
+            const uint32_t h = sm.fh[f];
+            if (h <= THREAD_SEL_MAX) continue;
+            const int j0 = sm.fb[f];
+            const int j1 = (f + 1 < nfb) ? sm.fb[f + 1] : S;
+            const uint32_t seg = sm.fcur[f] - h, above = sm.fabove[f];
             if (h > WAVE_SEL_MAX) {
                 if (lane == 0) sm.coop[atomicAdd(&sm.ncoop, 1)] = (uint16_t)f;
                 continue;
             }
-            const uint32_t above = sm.incl[b + 1];
-            // mid-sized bins: sort in registers, O(h log^2 h)
-            if (h > 32 && h <= 64) { wave_sort_select<1>(sm.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
-            if (h > 64 && h <= 128) { wave_sort_select<2>(sm.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
-            if (h > 128 && h <= 256) { wave_sort_select<4>(sm.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
-            // small (and rare very large) bins: rank each element by counting
+            if (h <= 64) { wave_sort_select<1>(sm.u.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
+            if (h <= 128) { wave_sort_select<2>(sm.u.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
+            if (h <= 256) { wave_sort_select<4>(sm.u.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
+            // larger bins (rare): rank each element by counting
             for (uint32_t i = lane; i < h; i += 64) {
-                const float e = sm.gbuf[seg + i];
+                const float e = sm.u.gbuf[seg + i];
                 uint32_t pos = 0;
                 for (uint32_t k = 0; k < h; ++k) {
-                    const float o = sm.gbuf[seg + k];
+                    const float o = sm.u.gbuf[seg + k];
                     pos += (o > e || (o == e && k < i)) ? 1u : 0u;
                 }
                 const uint32_t r = above + pos;
@@ -410,26 +464,28 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
             }
         }
         __syncthreads();
+        // bins too large for a wavefront: the whole workgroup
         const int ncoop = sm.ncoop;
         for (int c = 0; c < ncoop; ++c) {
             const int f = sm.coop[c];
             const int j0 = sm.fb[f];
             const int j1 = (f + 1 < nfb) ? sm.fb[f + 1] : S;
-            const int b = sm.tb[j0];
-            const uint32_t h = sm.incl[b] - sm.incl[b + 1];
-            const uint32_t seg = sm.cur[b] - h;
+            const uint32_t h = sm.fh[f];
+            const uint32_t seg = sm.fcur[f] - h;
             float bmn = INFINITY, bmx = -INFINITY;
-            for (uint32_t i = tid; i < h; i += NT) { const float v = sm.gbuf[seg + i]; bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
+            for (uint32_t i = tid; i < h; i += NT) { const float v = sm.u.gbuf[seg + i]; bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
             wg_minmax(sm, bmn, bmx);
             for (int j = j0; j < j1; ++j) {
-                const uint32_t t = (uint32_t)ranks[j] - sm.incl[b + 1];
-                const float v = wg_select(sm, sm.gbuf + seg, h, bmn, bmx, t);
+                const uint32_t t = (uint32_t)ranks[j] - sm.fabove[f];
+                const float v = wg_select(sm, sm.u.gbuf + seg, h, bmn, bmx, t);
                 if (tid == 0) T[j] = v;
             }
         }
         // release the member bins
-        __syncthreads();
-        for (int f = f0 + tid; f < f1; f += NT) sm.cur[sm.tb[sm.fb[f]]] = NONE;
+        for (int f = f0 + tid; f < f1; f += NT) {
+            const int b = sm.tb[sm.fb[f]];
+            sm.tbits[b >> 5] = 0;
+        }
         __syncthreads();
     }
 }

@@ -109,6 +109,15 @@ DataSnap snap_of(const Ctx *c) {
     d.n_qs = c->n_qs;
     guard_columns(c, d.gcol, d.n_guard);
     for (int j = 0; j < 8; ++j) d.ins_gmax[j] = c->ins_gmax[j];
+    if (c->qsrp_users && c->qsrp_n_user == c->n_user) {
+        d.qsrp_users = c->qsrp_users;
+        d.qsrp_rot = c->qsrp_rot;
+        d.qsrp_par = c->qsrp_par;
+        d.qsrp_cd = c->qsrp_cd;
+        d.qsrp_ldr = c->qsrp_ldr;
+        d.qsrp_gcd = c->qsrp_gcd;
+        d.qsrp_crot = c->qsrp_crot;
+    }
     return d;
 }
 
@@ -188,6 +197,13 @@ void free_pinned(PinnedBuf &b) {
     b.bytes = 0;
 }
 
+void free_qsrp(Ctx *c) {
+    for (float *p : {c->qsrp_users, c->qsrp_rot, c->qsrp_par}) dev_free(c, p, 0);
+    c->qsrp_users = c->qsrp_rot = c->qsrp_par = nullptr;
+    c->qsrp_cd = c->qsrp_ldr = 0;
+    c->qsrp_n_user = c->qsrp_fallback = 0;
+}
+
 namespace {
 
 void free_update_state(Ctx *c) {
@@ -207,6 +223,7 @@ void free_update_state(Ctx *c) {
 }
 
 void free_table(Ctx *c) {
+    free_qsrp(c);  // fitted to this table
     dev_free(c, c->ranks, 0);
     dev_free(c, c->table, 0);
     dev_free(c, c->qranks, 0);
@@ -420,6 +437,10 @@ int rkr_set_option(rkr_ctx *c, int option, int64_t value) {
             if (value != 1 && value != 2) return set_err(c, RKR_ERR_INVALID, "query slots must be 1 or 2");
             c->query_slots = (int)value;
             return RKR_OK;
+        case RKR_OPT_QSRP:
+            if (value != 0 && value != 1) return set_err(c, RKR_ERR_INVALID, "qsrp must be 0 or 1");
+            c->qsrp = (int)value;
+            return RKR_OK;
         default:
             return set_err(c, RKR_ERR_INVALID, "unknown option");
     }
@@ -438,6 +459,7 @@ int rkr_get_option(const rkr_ctx *c, int option, int64_t *value) {
         case RKR_OPT_DENSE_TOP: *value = c->dense_top; return RKR_OK;
         case RKR_OPT_COARSE: *value = c->coarse; return RKR_OK;
         case RKR_OPT_QUERY_SLOTS: *value = c->query_slots; return RKR_OK;
+        case RKR_OPT_QSRP: *value = c->qsrp; return RKR_OK;
         default: return RKR_ERR_INVALID;
     }
 }
@@ -1214,8 +1236,11 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
     const float eps = mode == RKR_OUTPUT_EXACT && !use_q ? c->table_eps : 0.0f;
     // pending item inserts/deletes: bounds become per-user values (k_update.hip)
     const bool upd = dv.n_ins > 0 || dv.n_del > 0;
-    const bool coarse = allow_coarse && c->coarse && c->form == RKR_FORM_SCREENED && mode == RKR_OUTPUT_EXACT &&
-                        c->kp > 0 && dv.user_norm && dv.n_guard > 0 && U > 0;
+    // the QSRP screen (RKR_OPT_QSRP, built by rkr_qsrp_build) replaces the bound
+    // stage of exact batches without pending updates
+    const bool qsrp = c->qsrp && dv.qsrp_users && mode == RKR_OUTPUT_EXACT && !upd && U > 0;
+    const bool coarse = !qsrp && allow_coarse && c->coarse && c->form == RKR_FORM_SCREENED &&
+                        mode == RKR_OUTPUT_EXACT && c->kp > 0 && dv.user_norm && dv.n_guard > 0 && U > 0;
     const size_t pair_bytes = upd ? 12 : 8;  // <u,q> + buckets, or + two bound values
 
     // queries per device batch
@@ -1238,6 +1263,8 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
     if ((rc = ensure(c, sl.qinfo, sizeof(QueryInfo) * (size_t)nqb + 64, "query info"))) return rc;
     if ((rc = ensure(c, sl.out, (sizeof(int32_t) * 2 + sizeof(float)) * (size_t)nqb * max_k, "outputs"))) return rc;
     if ((rc = ensure(c, sl.hist, sizeof(uint32_t) * hist_words((int32_t)nqb, S), "bound histograms"))) return rc;
+    if (qsrp && (rc = ensure(c, sl.q_split, sizeof(float) * (size_t)nqb * (dv.qsrp_ldr + 3), "QSRP query rows")))
+        return rc;
     GuardCols gc{};
     if (coarse) {
         if ((rc = ensure(c, sl.q_split, (sizeof(uint16_t) * 2 * c->kp + sizeof(float)) * (size_t)nqb, "query BF16 rows")))
@@ -1514,6 +1541,29 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                                                cl_b, n_listed, d_list_n, bk_c, s_c, ghist, qi));
             }
             RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
+        } else if (qsrp) {
+            // QSRP screen: rotated queries and their part norms, the left-part
+            // scores over the first check_dim dimensions, bounds from the
+            // partial-dimension interval and the regression, the k-th bound
+            // over those, then the exact score and table search for the users
+            // they keep, and the k-th bound again (the exact path's from here)
+            float *qrot = reinterpret_cast<float *>(sl.q_split.p);
+            float *qn = qrot + (size_t)nqb * dv.qsrp_ldr;
+            RKR_HIP(c, launch_scores_store(sl.stream, q_in, nullptr, nq, ld, dv.qsrp_rot, d, ld, d, qrot,
+                                           dv.qsrp_ldr, true));
+            RKR_HIP(c, launch_qsrp_qnorms(sl.stream, qrot, dv.qsrp_ldr, q_in, ld, d, dv.qsrp_cd, nq, qn));
+            RKR_HIP(c, launch_scores_store(sl.stream, dv.qsrp_users, nullptr, U, dv.qsrp_ldr, qrot, nq, dv.qsrp_ldr,
+                                           dv.qsrp_cd, qip, U, false));
+            RKR_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
+            // eps widens the interval as the exact path widens its search; the
+            // floor covers gradual underflow in the d products of a score
+            const float absf = std::nextafter((float)((double)eps + (double)d * std::ldexp(1.0, -140)), INFINITY);
+            RKR_HIP(c, launch_qsrp_bounds(sl.stream, qip, U, nq, dv.qsrp_par, qn, dv.qsrp_gcd, dv.qsrp_crot, absf,
+                                          c->n_sample, use_q ? c->qcol : nullptr, S, bk));
+            RKR_HIP(c, launch_hist_kth(sl.stream, bk, U, S, R, I, nq, mode, 0, cu, ghist, qi));
+            RKR_HIP(c, launch_qsrp_exact(sl.stream, dv.users, q_in, ld, d, T, S, eps, U, nq, qi, qip, bk));
+            RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
+            RKR_HIP(c, launch_hist_kth(sl.stream, bk, U, S, R, I, nq, mode, 0, cu, ghist, qi));
         } else {
             // <u, q> for every (query, user): qip[b * U + u]
             if (nq <= 16 && (size_t)nq * d * sizeof(float) <= 65536)
@@ -1665,6 +1715,8 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                 }
             tm.n_listed += h_scalars[3];
         }
+        if (qsrp)
+            for (int32_t i = 0; i < nq; ++i) tm.n_listed += hqi[i].n_screen;
 
         std::memcpy(out_user + (size_t)q0 * max_k, ho_user, sizeof(int32_t) * (size_t)nq * max_k);
         std::memcpy(out_rank + (size_t)q0 * max_k, ho_rank, sizeof(int32_t) * (size_t)nq * max_k);
@@ -1683,7 +1735,8 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                     st.n_result_user = hqi[i].n_settled;
                     st.n_prune_user = (int32_t)(U - hqi[i].n_settled - hqi[i].n_cand);
                 }
-                st.ip_cost = U + (int64_t)hqi[i].n_refine * I;
+                // inner products: every user's (QSRP: the screen's survivors), then the refinement's
+                st.ip_cost = (qsrp ? (int64_t)hqi[i].n_screen : U) + (int64_t)hqi[i].n_refine * I;
                 if (screened) {
                     st.n_screened = (int64_t)hqi[i].n_refine * I;
                     st.n_rechecked = h_qrecheck[i];

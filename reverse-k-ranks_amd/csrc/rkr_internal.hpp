@@ -131,6 +131,18 @@ struct Ctx {
     float screen_C = 0.0f, screen_abs = 0.0f;
     int n_cu = 256;
 
+    // QSRP screen (rkr_qsrp_build): users rotated into the SVD basis, the query
+    // rotation, and per-user norms and regression parameters (kQsrp* rows of
+    // n_user floats); built for the current table and users, dropped when
+    // either changes
+    int qsrp = 0;                   // use the screen in exact queries (RKR_OPT_QSRP)
+    float *qsrp_users = nullptr;    // n_user x qsrp_ldr
+    float *qsrp_rot = nullptr;      // dim rows of ld: row r = s_r w_r
+    float *qsrp_par = nullptr;      // [kQsrpPar][n_user]
+    int32_t qsrp_cd = 0, qsrp_ldr = 0;
+    int64_t qsrp_n_user = 0, qsrp_fallback = 0;
+    float qsrp_sigma = 0.0f, qsrp_gcd = 0.0f, qsrp_crot = 0.0f;
+
     // query scratch of rkr_query (grown on demand) and build scratch
     Slot main;
     Slot aux;                   // second stream of split rkr_query calls, made on first use
@@ -147,6 +159,20 @@ struct Ctx {
     std::vector<Retired> retired;
     std::unordered_map<void *, size_t> alloc_bytes;  // size of every live allocation (mem_mu)
 };
+
+// Rows of Ctx::qsrp_par: regression line a, b, the score mean and 1/sigma of
+// its normal CDF argument, the bound's error in sample positions, the row's
+// largest and smallest sampled scores, and the norms of the rotated user's
+// left and right parts and of the user.
+enum {
+    kQsrpA = 0, kQsrpB, kQsrpMu, kQsrpInvSigma, kQsrpErr, kQsrpTmax, kQsrpTmin, kQsrpNrmL, kQsrpNrmR, kQsrpNrmU,
+    kQsrpPar
+};
+// Bound on |Phi_device(z) - Phi(z)| for the FP32 evaluation of the A&S CDF,
+// argument rounding included (checked against FP64 erf by the GPU tests).
+constexpr double kQsrpPhiDevErr = 2e-6;
+// Frees the QSRP screen's buffers (the table or the users changed).
+void free_qsrp(Ctx *c);
 
 // Frees made by this thread go to Ctx::retired instead of hipFree.
 void set_defer_frees(bool on);
@@ -204,6 +230,10 @@ struct DataSnap {
     // pending inserts: the most inserted items any user has above its value
     // at guard column j (a bound on the rank shift at that guard)
     int32_t ins_gmax[8] = {};
+    // QSRP screen (null when not built)
+    const float *qsrp_users = nullptr, *qsrp_rot = nullptr, *qsrp_par = nullptr;
+    int32_t qsrp_cd = 0, qsrp_ldr = 0;
+    float qsrp_gcd = 0.0f, qsrp_crot = 0.0f;
 };
 DataSnap snap_of(const Ctx *c);
 // Guard columns of the coarse bounds for the context's searched table.
@@ -362,7 +392,7 @@ struct QueryInfo {          // one per query of a device batch
     int32_t mode;           // classification case (reference output)
     int32_t pad;
     int32_t fill;           // candidates written so far (chunked compaction)
-    int32_t pad2;
+    int32_t n_screen;       // users a screen kept and scored exactly (QSRP)
     int64_t cand_off;       // offset of this query's candidates
 };
 
@@ -469,6 +499,18 @@ hipError_t launch_topk(hipStream_t s, const QueryInfo *qi, int32_t nq, const int
                        int32_t *out_user, int32_t *out_rank, float *out_qip, int64_t *scratch_keys);
 
 // ---- update path (k_update.hip) ---------------------------------------------
+// QSRP screen kernels (k_qsrp.hip).  qn holds 3 floats per query: upper bounds
+// of the norms of the rotated query's left and right parts and of the query.
+hipError_t launch_phi_eval(hipStream_t s, const float *z, int64_t n, float *out);
+hipError_t launch_qsrp_qnorms(hipStream_t s, const float *qrot, int32_t ldr, const float *q, int32_t ld, int32_t d,
+                              int32_t cd, int32_t nq, float *qn);
+hipError_t launch_qsrp_bounds(hipStream_t s, const float *left, int64_t n_user, int32_t nq, const float *par,
+                              const float *qn, float gcd, float crot, float absf, int32_t S, const int32_t *qcol,
+                              int32_t S_T, uint32_t *bk);
+hipError_t launch_qsrp_exact(hipStream_t s, const float *users, const float *q_in, int32_t ld, int32_t d,
+                             const float *T, int32_t S, float eps, int64_t n_user, int32_t nq, QueryInfo *qi,
+                             float *qip, uint32_t *bk);
+
 // The sort scratch of rkr_sorted_scores_noalloc: looks up its device address
 // on `device` (current) once per device.
 hipError_t noalloc_init(int device);

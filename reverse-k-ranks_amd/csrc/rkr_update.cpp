@@ -45,6 +45,7 @@ constexpr int64_t kMaxSideItems = 16384;    // per-user side array length before
 int check_updatable(Ctx *c) {
     if (!c->users) return set_err(c, RKR_ERR_STATE, "update: no data");
     if (!c->table) return set_err(c, RKR_ERR_STATE, "update: no table (build or load one first)");
+    free_qsrp(c);  // the QSRP screen describes the users and items it was built for
     return RKR_OK;
 }
 

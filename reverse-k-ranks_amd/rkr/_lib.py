@@ -53,7 +53,10 @@ EXPORTS = (
     "rkr_index_size_bytes", "rkr_query_table_cols", "rkr_insert_items", "rkr_delete_items", "rkr_insert_users", "rkr_delete_users",
     "rkr_get_counts", "rkr_reserve", "rkr_now", "rkr_server_start", "rkr_server_submit_query", "rkr_server_submit_update",
     "rkr_server_wait", "rkr_server_poll", "rkr_server_counters", "rkr_server_stop",
+    "rkr_qsrp_build", "rkr_qsrp_info", "rkr_qsrp_screen", "rkr_qsrp_phi", "rkr_qsrp_fit_line",
 )
+
+RKR_OPT_QSRP = 11
 
 RKR_SAMPLE_OPTIMAL_PART = 0
 RKR_SAMPLE_OPTIMAL_UNIFORM = 1
@@ -161,6 +164,11 @@ def load():
     L.rkr_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(C.c_int)]
     L.rkr_server_counters.argtypes = [vp] + [C.POINTER(C.c_int64)] * 3
     L.rkr_server_stop.argtypes = [vp]
+    L.rkr_qsrp_build.argtypes = [vp, C.c_float, C.POINTER(C.c_double)]
+    L.rkr_qsrp_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+    L.rkr_qsrp_screen.argtypes = [vp, f32p, i32, C.c_void_p]
+    L.rkr_qsrp_phi.argtypes = [vp, f32p, i64, f32p]
+    L.rkr_qsrp_fit_line.argtypes = [C.c_void_p, C.c_void_p, i32] + [C.POINTER(C.c_double)] * 3
     L.rkr_index_size_bytes.argtypes = [vp]
     L.rkr_index_size_bytes.restype = C.c_int64
     L.rkr_query_table_cols.argtypes = [vp]

@@ -197,6 +197,34 @@ class Index:
     def index_size_bytes(self) -> int:
         return int(self.lib.rkr_index_size_bytes(self.ctx))
 
+    # ---- QSRP screen (QSRPNormalLP, src/impl/QSRPNormalLP.hpp:137-253) --------
+    def qsrp_build(self, sigma: float = 0.7) -> float:
+        """SVD rotation and per-user regression of the current users and table; returns host ms."""
+        ms = C.c_double(0.0)
+        _check(self.lib, self.ctx, self.lib.rkr_qsrp_build(self.ctx, float(sigma), C.byref(ms)))
+        return ms.value
+
+    def qsrp_info(self) -> dict:
+        cd, nf = C.c_int32(), C.c_int64()
+        _check(self.lib, self.ctx, self.lib.rkr_qsrp_info(self.ctx, C.byref(cd), C.byref(nf)))
+        return {"check_dim": cd.value, "n_fallback_fit": nf.value}
+
+    def qsrp_screen(self, queries) -> np.ndarray:
+        """(n_query, n_user) uint32 bucket pairs b_lo | b_hi << 16 of the screen alone."""
+        q = _f32(queries)
+        if q.ndim == 1:
+            q = q[None, :]
+        self._refresh_counts()
+        out = np.empty((q.shape[0], self.n_user), np.uint32)
+        _check(self.lib, self.ctx, self.lib.rkr_qsrp_screen(self.ctx, _ptr(q), q.shape[0], out.ctypes.data))
+        return out
+
+    def qsrp_phi(self, z) -> np.ndarray:
+        z = _f32(z).ravel()
+        out = np.empty_like(z)
+        _check(self.lib, self.ctx, self.lib.rkr_qsrp_phi(self.ctx, _ptr(z), z.shape[0], _ptr(out)))
+        return out
+
     def query_table_cols(self) -> int:
         """Columns of the query table exact output searches (0: the sampled table)."""
         return int(self.lib.rkr_query_table_cols(self.ctx))
