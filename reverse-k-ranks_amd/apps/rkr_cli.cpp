@@ -24,6 +24,13 @@
 //       ReverseRankIndex with method US (src/app/ReverseRankIndex.cpp:83-219):
 //       answers every query item and writes R/rank/N-US-top<K>-n_sample_<S>-{userID,rank,IP}.csv
 //       and R/single_query_performance/...csv (FileIO.hpp:84-206 layouts).
+//   rkr_cli rriu  --dataset_dir D --dataset_name N --index_dir X --n_sample S --topk K
+//                 --update_type user|data_item --update_operator insert|delete --updateID 0..4
+//       ReverseRankIndexUpdate (src/app/ReverseRankIndexUpdate.cpp:81-227): loads the table bsibc
+//       built on the base sets, inserts the first updateID quarters of N_{user,data_item}_update.fvecs
+//       or deletes as many users/items from the front, answers every query over the updated sets
+//       and writes the result CSVs with -updateID_<i>-<type>-<operator> in their names.  Methods
+//       QSUpdate and QSRPNormalLPUpdate read the QS table, US the US table.
 // Datasets are <D>/<N>/<N>_{data_item,user,query_item}.fvecs
 // (src/include/util/VectorIO.hpp:40-68); users are normalised on load.
 #include <sys/stat.h>
@@ -108,9 +115,13 @@ bool read_data(const Args &a, rkr::VectorMatrix &user, rkr::VectorMatrix &item, 
     return true;
 }
 
-// QSRPNormalLP runs on the QS table (SampleSearchIndexName, NameTranslation.hpp:36-48)
+// QSRPNormalLP and the update methods run on the QS table (SampleSearchIndexName,
+// NameTranslation.hpp:36-48)
 bool is_qsrp(const Args &a) { return get(a, "method_name", "US") == "QSRPNormalLP"; }
-bool is_qs(const Args &a) { return get(a, "method_name", "US") == "QS" || is_qsrp(a); }
+bool is_qs(const Args &a) {
+    const std::string m = get(a, "method_name", "US");
+    return m == "QS" || m == "QSUpdate" || m == "QSRPNormalLPUpdate" || is_qsrp(a);
+}
 
 // SampleSearch::SaveIndex / LoadIndex names (SampleSearch.hpp:158-179): QS adds the training parameters
 std::string index_file(const Args &a, const char *sub, int S) {
@@ -365,17 +376,112 @@ int cmd_rri(const Args &a) {
     return 0;
 }
 
+// The update run of ReverseRankIndexUpdate.cpp:81-227 on the table of the base sets.
+int cmd_rriu(const Args &a) {
+    rkr::VectorMatrix user, item, query, upd;
+    if (!read_data(a, user, item, query)) return 1;
+    const std::string type = get(a, "update_type", "data_item"), op = get(a, "update_operator", "insert");
+    const int update_id = std::stoi(get(a, "updateID", "0"));
+    if (update_id < 0 || update_id > 4) {  // :153-156
+        std::fprintf(stderr, "updateID should be in [0, 4]\n");
+        return 2;
+    }
+    if (type != "user" && type != "data_item") {
+        std::fprintf(stderr, "no such update type %s\n", type.c_str());
+        return 2;
+    }
+    if (op != "insert" && op != "delete") {
+        std::fprintf(stderr, "no such update operator %s\n", op.c_str());
+        return 2;
+    }
+    // readDataUpdate (VectorIOUpdate.hpp:43-98): N_user_update.fvecs (normalised) or
+    // N_data_item_update.fvecs
+    const std::string base = get(a, "dataset_dir", ".") + "/" + get(a, "dataset_name", "fake-normal") + "/" +
+                             get(a, "dataset_name", "fake-normal");
+    if (!load_fvecs(base + (type == "user" ? "_user_update.fvecs" : "_data_item_update.fvecs"), upd)) {
+        std::fprintf(stderr, "cannot read the %s update vectors\n", type.c_str());
+        return 1;
+    }
+    if (type == "user") rkr_normalize_users(upd.data.data(), upd.n_vector_, upd.vec_dim_);
+    // the first updateID quarters of the update set (:158-171)
+    const int n_vecs = upd.n_vector_ / 4 * update_id;
+    const int S = std::stoi(get(a, "n_sample", "20"));
+    const int topk = std::stoi(get(a, "topk", "10"));
+    const std::string ds = get(a, "dataset_name", "fake-normal");
+    const std::string res_dir = get(a, "result_dir", "result");
+    try {
+        auto idx = is_qs(a) ? rkr::BuildIndexQS(item, user, ds.c_str(), S, std::stoi(get(a, "n_sample_query", "150")),
+                                                std::stoi(get(a, "sample_topk", "50")), -1,
+                                                get(a, "index_dir", "index").c_str(), std::stoi(get(a, "device", "0")))
+                            : rkr::BuildIndex(item, user, ds.c_str(), get(a, "index_dir", "index").c_str(), S, -1,
+                                              std::stoi(get(a, "device", "0")));
+        rkr_ctx *c = idx->ctx();
+        auto check = [&](int rc) {
+            if (rc != RKR_OK) throw rkr::Error(rc, rkr_last_error(c));
+        };
+        int n_user = user.n_vector_;
+        if (update_id != 0) {  // :174-200
+            if (op == "insert") {
+                if (type == "user") {
+                    check(rkr_insert_users(c, upd.data.data(), n_vecs));
+                    n_user += n_vecs;
+                } else {
+                    check(rkr_insert_items(c, upd.data.data(), n_vecs));
+                }
+            } else {
+                std::vector<int64_t> ids((size_t)n_vecs);
+                for (int i = 0; i < n_vecs; ++i) ids[i] = i;
+                if (type == "user") {
+                    check(rkr_delete_users(c, ids.data(), n_vecs));
+                    n_user -= n_vecs;
+                } else {
+                    check(rkr_delete_items(c, ids.data(), n_vecs));
+                }
+            }
+        }
+        // Retrieval over the updated sets (:202-205)
+        const int nq = query.n_vector_;
+        std::vector<int32_t> ks((size_t)nq, topk), uo((size_t)nq * topk), ro((size_t)nq * topk);
+        std::vector<float> qo((size_t)nq * topk);
+        if (topk > n_user) throw rkr::Error(RKR_ERR_INVALID, "top-k is too large");
+        check(rkr_query(c, query.data.data(), nq, ks.data(), topk, uo.data(), ro.data(), qo.data(), nullptr, nullptr));
+        int64_t cu = 0, ci = 0, ni = 0, nd = 0;
+        check(rkr_get_counts(c, &cu, &ci, &ni, &nd));
+        std::printf("UpdateID: %d, n_data_item %lld, n_user %lld\n", update_id, (long long)ci, (long long)cu);
+        char param[512];
+        std::snprintf(param, sizeof param, "top%d-n_sample_%d-n_thread_%d-updateID_%d-%s-%s", topk, S, -1, update_id,
+                      type.c_str(), op.c_str());
+        mkdirs(res_dir + "/rank");
+        const std::string stem = res_dir + "/rank/" + ds + "-" + get(a, "method_name", "US") + "-" + param;
+        std::ofstream fu(stem + "-userID.csv"), fr(stem + "-rank.csv"), fi(stem + "-IP.csv");
+        for (int q = 0; q < nq; ++q)
+            for (int j = 0; j < topk; ++j) {
+                const char *sep = j + 1 < topk ? "," : "\n";
+                const size_t o = (size_t)q * topk + j;
+                fu << uo[o] << sep;
+                fr << ro[o] << sep;
+                fi << qo[o] << sep;
+            }
+        std::printf("wrote %s-{userID,rank,IP}.csv\n", stem.c_str());
+    } catch (const rkr::Error &e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: rkr_cli qdibc|fsr|bsibc|bst|rri --dataset_dir D --dataset_name N --index_dir X "
+        std::fprintf(stderr, "usage: rkr_cli qdibc|fsr|bsibc|bst|rri|rriu --dataset_dir D --dataset_name N --index_dir X "
                              "--n_sample S [--topk K] [--form plain|screened] [--output exact|reference]\n");
         return 2;
     }
     const std::string cmd = argv[1];
     const Args a = parse(argc, argv, 2);
     if (cmd == "qdibc") return cmd_qdibc(a);
+    if (cmd == "rriu") return cmd_rriu(a);
     if (get(a, "method_name", "US") != "US" && !is_qs(a)) {
         std::fprintf(stderr, "method_name US, QS and QSRPNormalLP are implemented\n");
         return 2;

@@ -331,3 +331,44 @@ def test_two_query_slots_with_pending_updates(orc, gpu_index_factory, form):
     for bad in (0, 3):
         with pytest.raises(Exception):
             idx.set_option(10, bad)
+
+
+@pytest.mark.parametrize("dataset,utype", [("fake-normal-30d-update-item", "data_item"),
+                                           ("fake-normal-30d-update-user", "user")])
+def test_cli_rriu_reference_update_fixtures(orc, tmp_path, dataset, utype):
+    """The reference's update fixtures (dataset/fake-normal-30d-update-{item,user}: 400 items or 100
+    users to insert) through bsibc and rriu (ReverseRankIndexUpdate.cpp:81-227): for updateID 0, 2 and
+    4, inserts of the first updateID quarters and deletes of as many vectors from the front, the
+    answers written to the result CSVs equal brute force over the updated sets."""
+    import os
+    import subprocess
+    from conftest import GOLDEN, ROOT
+    from rkr import formats
+    _case(f"cli rriu {dataset}")
+    cli = os.path.join(ROOT, "reverse-k-ranks_amd", "rkr_cli")
+    idir, rdir = str(tmp_path / "index"), str(tmp_path / "result")
+    common = ["--dataset_dir", GOLDEN, "--dataset_name", dataset, "--index_dir", idir, "--n_sample", "128"]
+    r = subprocess.run([cli, "bsibc"] + common, capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stderr
+    users, items, queries = formats.read_dataset(GOLDEN, dataset)
+    upd = formats.read_fvecs(os.path.join(GOLDEN, dataset, f"{dataset}_{utype}_update.fvecs"))
+    un = orc.normalize_users(users)
+    for op in ("insert", "delete"):
+        for uid in (0, 2, 4):
+            if op == "delete" and uid == 0:
+                continue
+            r = subprocess.run([cli, "rriu"] + common + ["--topk", "10", "--result_dir", rdir, "--update_type", utype,
+                                                          "--update_operator", op, "--updateID", str(uid)],
+                               capture_output=True, text=True, timeout=100)
+            assert r.returncode == 0, (op, uid, r.stderr)
+            n = len(upd) // 4 * uid
+            cur_u, cur_i = un, items
+            if utype == "user":
+                cur_u = np.concatenate([un, orc.normalize_users(upd[:n])]) if op == "insert" else un[n:]
+            else:
+                cur_i = np.concatenate([items, upd[:n]]) if op == "insert" else items[n:]
+            stem = os.path.join(rdir, "rank", f"{dataset}-US-top10-n_sample_128-n_thread_-1-updateID_{uid}-{utype}-{op}")
+            got_u = np.loadtxt(f"{stem}-userID.csv", delimiter=",", dtype=np.int64)
+            got_r = np.loadtxt(f"{stem}-rank.csv", delimiter=",", dtype=np.int64)
+            bu, br, _ = orc.bruteforce(cur_u, cur_i, queries, 10)
+            assert np.array_equal(got_u, bu) and np.array_equal(got_r, br), (op, uid)
