@@ -92,6 +92,12 @@ def parse():
     ap.add_argument("--no-inserts", action="store_true", help="serving: no insert stream (diagnostic)")
     ap.add_argument("--update-cus", type=int, default=0,
                     help="serving, snapshot mode: CUs for update kernels (0 = a quarter, -1 = all)")
+    ap.add_argument("--rebuild-pending", type=int, default=0,
+                    help="serving: pending item updates at which the loop rebuilds the table "
+                         "(0 = the library's 12288, -1 = never)")
+    ap.add_argument("--side-reserve", type=int, default=0,
+                    help="serving: inserted items the side arrays are reserved for "
+                         "(0 = every insert round of the run, at most 16384)")
     return ap.parse_args()
 
 
@@ -111,9 +117,10 @@ def serving(args, idx, wl, label):
     n_new_i = max(1, I0 // 200)
     # capacity for every insert round of the run, so updates never allocate
     rounds = int(np.ceil(args.sat_seconds + args.duration)) + 3
-    idx.reserve(U0 + rounds * n_new_u, rounds * n_new_i)
+    side = args.side_reserve if args.side_reserve > 0 else min(16384, rounds * n_new_i)
+    idx.reserve(U0 + rounds * n_new_u, side)
     srv = rkr.Server(idx, max_batch=args.max_batch, n_slots=args.slots, update_mode=args.update_mode,
-                     update_cus=args.update_cus)
+                     update_cus=args.update_cus, rebuild_pending=args.rebuild_pending)
     upd, stop_upd = [], threading.Event()
     urng = np.random.default_rng(args.seed + 8)
 
@@ -235,11 +242,14 @@ def serving(args, idx, wl, label):
     sys.setswitchinterval(switch)
     stop_upd.set()
     uth.join()
-    ulat, ustall = [], []
+    ulat, ufail = [], []
     for t in upd:
         _, _, _, info = srv.wait(t)
         ulat.append((info["t_done"] - info["t_start"]) * 1e3)
+        if info["status"] != 0:
+            ufail.append(info["error"])
     cnt = srv.counters()
+    n_rebuilds = srv.rebuilds()
     srv.stop()
     lat = np.array(lat)
     # p99 per second of the open loop (warm-up and stalls show up here)
@@ -254,7 +264,8 @@ def serving(args, idx, wl, label):
                    "max_batch": args.max_batch, "slots": args.slots, "update_cus": args.update_cus,
                    "arrival_rate": round(rate, 1), "saturating_rate": round(sat, 1),
                    "saturation": f"closed loop, {window} outstanding, {args.sat_seconds:g} s, same insert stream",
-                   "duration_s": args.duration},
+                   "duration_s": args.duration, "dense_top": args.dense_top,
+                   "rebuild_pending": args.rebuild_pending, "side_reserve": side},
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "p999_ms": round(float(np.percentile(lat, 99.9)), 3), "p99_ms_per_second": per_s,
         "submit_lateness_ms": {"p50": round(float(np.percentile(lateness, 50)) * 1e3, 3),
@@ -265,6 +276,7 @@ def serving(args, idx, wl, label):
         "mean_batch": round(float(np.mean(bsz)), 1), "queries": len(lat), "batches": cnt["n_batches"],
         "update_batches": len(ulat), "update_ms_p50": round(float(np.percentile(ulat, 50)), 2) if ulat else None,
         "update_ms_max": round(float(np.max(ulat)), 2) if ulat else None,
+        "updates_failed": len(ufail), "update_errors": sorted(set(ufail))[:3], "table_rebuilds": n_rebuilds,
         "final_counts": idx.counts(),
     }
     print(json.dumps(out), flush=True)
@@ -463,8 +475,8 @@ def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_max, seconds):
 PROFILED_TRAFFIC = {
     ("plain_gpu", "screened"): {"kernel": "k_ip_coarse", "bytes": 2.529e8, "fetch_kib": 74559.9,
                                 "write_kib": 97847.8, "profile": "profiles/k_ip_coarse_100k_x_20k_d150_k50.md"},
-    ("precision", "screened"): {"kernel": "k_ip_coarse", "bytes": 2.505e9, "fetch_kib": 733558.7,
-                                "write_kib": 979018.9, "profile": "profiles/k_ip_coarse_1M_x_50k_d150_k50.md"},
+    ("precision", "screened"): {"kernel": "k_ip_coarse", "bytes": 1.769e9, "fetch_kib": 374879.0,
+                                "write_kib": 977798.4, "profile": "profiles/k_ip_coarse_1M_x_50k_d150_k50.md"},
 }
 
 

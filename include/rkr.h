@@ -370,6 +370,11 @@ typedef struct rkr_server_config {
     int32_t update_mode; /* RKR_SERVE_BARRIER (0) or RKR_SERVE_SNAPSHOT              */
     int32_t update_cus;  /* snapshot mode: CUs the update kernels may run on, so
                             batches keep the rest (0 = a quarter, < 0 = all)       */
+    int32_t rebuild_pending; /* pending inserted + deleted items at which the update
+                                side rebuilds the table over the current items (new
+                                uniform sample ranks, same n_sample) so inserts keep
+                                flowing past the side arrays' cap: 0 = 12288 (three
+                                quarters of the 16384 cap), < 0 = never            */
 } rkr_server_config;
 
 /* Barrier: an update waits for the batches in flight and blocks new ones, so
@@ -411,6 +416,8 @@ int rkr_server_wait(rkr_server *srv, uint64_t ticket, int32_t *out_user, int32_t
                     rkr_request_info *info);
 int rkr_server_poll(rkr_server *srv, uint64_t ticket, int *done);
 int rkr_server_counters(rkr_server *srv, int64_t *n_batches, int64_t *n_queries, int64_t *n_updates);
+/* Table rebuilds the serving loop has made (rkr_server_config.rebuild_pending). */
+int rkr_server_rebuilds(rkr_server *srv, int64_t *n_rebuilds);
 /* Answers everything queued, joins the workers and frees the server. */
 int rkr_server_stop(rkr_server *srv);
 

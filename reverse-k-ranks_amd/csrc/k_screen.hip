@@ -353,6 +353,7 @@ struct IpCoarseParams {
     const float *user_norm;       // n_user
     const uint16_t *user_hi, *user_lo;  // n_user x Kp BF16 split
     const float *table;           // n_user x S
+    const float *gvals;           // n_user x 8 guard values (first pass), or null
     int64_t n_user;
     int32_t S;
     GuardCols gc;
@@ -381,6 +382,24 @@ __device__ __forceinline__ uint32_t count_above8(const float (&g)[8], float x) {
 #pragma unroll
     for (int j = 0; j < 7; ++j) c += __float_as_uint(x - g[j]) >> 31;
     return c;
+}
+
+typedef unsigned short ushort2_t __attribute__((ext_vector_type(2)));
+
+// The coarse byte c_lo | c_hi << 4 with c_lo = #{g > xlo} and c_hi = #{g > xhi},
+// by the same sign bits: one byte permute spreads the signs of xlo - g and
+// xhi - g over the two 16-bit halves of a word (selector 9: the sign of the
+// second source's top byte, 11: of the first's), and one packed 16-bit
+// subtraction counts both.
+__device__ __forceinline__ uint32_t coarse_byte(const float (&g)[8], float xlo, float xhi) {
+    ushort2_t acc = {0, 0};
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const uint32_t m = __builtin_amdgcn_perm(__float_as_uint(xhi - g[j]), __float_as_uint(xlo - g[j]), 0x0b0b0909u);
+        acc -= __builtin_bit_cast(ushort2_t, m);  // each half is 0xffff where its sign was set
+    }
+    const uint32_t c = __builtin_bit_cast(uint32_t, acc);
+    return (c | (c >> 12)) & 0xffu;
 }
 
 template <int KS, bool SETS>
@@ -415,9 +434,20 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
                 ul[b][ks].q = live[b] ? reinterpret_cast<const uint4 *>(P.user_lo + r * P.Kp)[ks * 4 + lq] : z;
             }
             if (!SETS) {
-                const float *trow = P.table + (live[b] ? r : 0) * (int64_t)P.S;
+                if (P.gvals) {
+                    // the user's guard values, one 32-byte row
+                    const float4 *g4 = reinterpret_cast<const float4 *>(P.gvals + (live[b] ? r : 0) * 8);
+                    const float4 x0 = g4[0], x1 = g4[1];
+                    const float ninf = -INFINITY;
+                    gd[b][0] = live[b] ? x0.x : ninf; gd[b][1] = live[b] ? x0.y : ninf;
+                    gd[b][2] = live[b] ? x0.z : ninf; gd[b][3] = live[b] ? x0.w : ninf;
+                    gd[b][4] = live[b] ? x1.x : ninf; gd[b][5] = live[b] ? x1.y : ninf;
+                    gd[b][6] = live[b] ? x1.z : ninf; gd[b][7] = ninf;
+                } else {
+                    const float *trow = P.table + (live[b] ? r : 0) * (int64_t)P.S;
 #pragma unroll
-                for (int j = 0; j < 8; ++j) gd[b][j] = (live[b] && j < P.gc.n) ? trow[P.gc.col[j]] : -INFINITY;
+                    for (int j = 0; j < 8; ++j) gd[b][j] = (live[b] && j < P.gc.n) ? trow[P.gc.col[j]] : -INFINITY;
+                }
             }
             ar[b] = live[b] ? P.C * P.user_norm[r] : 0.0f;
         }
@@ -485,9 +515,7 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
                             gd[b][4] = x1.x; gd[b][5] = x1.y; gd[b][6] = x1.z; gd[b][7] = x1.w;
                         }
                         const float e = __fmaf_rn(ar[b], qn, P.abs_floor);
-                        const uint32_t clo = count_above8(gd[b], acc[b][i] - e);
-                        const uint32_t chi = count_above8(gd[b], acc[b][i] + e);
-                        out[b * 16] = (uint8_t)(clo | (chi << 4));
+                        out[b * 16] = (uint8_t)coarse_byte(gd[b], acc[b][i] - e, acc[b][i] + e);
                     }
                 }
             }
@@ -495,7 +523,28 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
     }
 }
 
+// gv[u * 8 + j] = T[u][col[j]] for the n guard columns, -inf past them: the
+// coarse pass's guard values of every user in one 32-byte row (the columns sit
+// in different cache lines of a table row).
+__global__ void __launch_bounds__(256) k_gather_guards(const float *__restrict__ T, int64_t S, GuardCols gc,
+                                                       int64_t u0, int64_t n_rows, float *__restrict__ gv) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_rows * 8) return;
+    const int64_t u = u0 + i / 8;
+    const int j = (int)(i % 8);
+    gv[u * 8 + j] = j < gc.n ? T[u * S + gc.col[j]] : -INFINITY;
+}
+
 }  // namespace
+
+hipError_t launch_gather_guards(hipStream_t s, const float *T, int64_t S, const GuardCols &gc, int64_t u0,
+                                int64_t n_rows, float *gv) {
+    if (n_rows <= 0) return hipSuccess;
+    if (gc.n < 0 || gc.n > 7) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gather_guards, dim3((unsigned)((n_rows * 8 + 255) / 256)), dim3(256), 0, s, T, S, gc, u0,
+                       n_rows, gv);
+    return hipGetLastError();
+}
 
 hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, int32_t ld, int32_t K, int32_t Kp,
                              uint16_t *hi, uint16_t *lo, float *norm) {
@@ -587,7 +636,7 @@ hipError_t launch_ip_coarse(hipStream_t s, const CoarseLaunch &L) {
         return hipErrorInvalidValue;
     if (L.ldc % 128 || L.ldc < L.n_user) return hipErrorInvalidValue;
     if (!L.user_hi || !L.user_lo) return hipErrorInvalidValue;
-    IpCoarseParams P{L.users, L.user_norm, L.user_hi, L.user_lo, L.table, L.n_user, L.S, L.gc, L.q_hi, L.q_lo,
+    IpCoarseParams P{L.users, L.user_norm, L.user_hi, L.user_lo, L.table, sets ? nullptr : L.gvals, L.n_user, L.S, L.gc, L.q_hi, L.q_lo,
                      L.q_norm, L.qmap, L.nq, L.ld, L.K, L.Kp, L.C, L.abs_floor, L.cb, L.ldc, L.spare_row, L.sets,
                      L.qset, L.n_sets};
     const int64_t n_rb = (L.n_user + 127) / 128;

@@ -108,6 +108,7 @@ DataSnap snap_of(const Ctx *c) {
     d.qranks = c->qranks;
     d.n_qs = c->n_qs;
     guard_columns(c, d.gcol, d.n_guard);
+    d.gvals = c->gvals;
     for (int j = 0; j < 8; ++j) d.ins_gmax[j] = c->ins_gmax[j];
     if (c->qsrp_users && c->qsrp_n_user == c->n_user) {
         d.qsrp_users = c->qsrp_users;
@@ -135,6 +136,16 @@ void guard_columns(const Ctx *c, int32_t *col, int32_t &n) {
             last = j;
         }
     }
+}
+
+int refresh_guard_values(Ctx *c, int64_t u0, int64_t n) {
+    if (!c->gvals || n <= 0) return RKR_OK;
+    GuardCols gc{};
+    guard_columns(c, gc.col, gc.n);
+    const float *T = c->qtable ? c->qtable : c->table;
+    const int64_t S = c->qtable ? c->n_qs : c->n_sample;
+    RKR_HIP(c, launch_gather_guards(c->stream, T, S, gc, u0, n, c->gvals));
+    return RKR_OK;
 }
 
 std::vector<int32_t> query_table_ranks(const std::vector<int32_t> &sample, int64_t n_item, int mode) {
@@ -206,14 +217,21 @@ void free_qsrp(Ctx *c) {
 
 namespace {
 
-void free_update_state(Ctx *c) {
-    for (float *p : {c->items_ins, c->items_del, c->side_ins, c->side_del, c->side_ins_spare, c->side_del_spare})
-        dev_free(c, p, 0);
-    c->items_ins = c->items_del = c->side_ins = c->side_del = nullptr;
-    c->side_ins_spare = c->side_del_spare = nullptr;
+// Forgets the pending item updates.  keep_buffers leaves the side arrays and
+// item lists allocated at their row lengths and capacities for the inserts
+// that follow a rebuild (a serving loop's memory stays flat); with two buffers
+// per side the next update writes the one no batch of the new state reads.
+void free_update_state(Ctx *c, bool keep_buffers = false) {
+    if (!keep_buffers) {
+        for (float *p : {c->items_ins, c->items_del, c->side_ins, c->side_del, c->side_ins_spare, c->side_del_spare})
+            dev_free(c, p, 0);
+        c->items_ins = c->items_del = c->side_ins = c->side_del = nullptr;
+        c->side_ins_spare = c->side_del_spare = nullptr;
+        c->ld_ins = c->ld_del = 0;
+        c->ins_cap = c->del_cap = 0;
+    }
     c->n_ins = c->n_del = 0;
-    c->ld_ins = c->ld_del = 0;
-    c->ins_cap = c->del_cap = 0;
+    for (int j = 0; j < 8; ++j) c->ins_gmax[j] = 0;
     if (c->items_base && c->items_base != c->items)
         dev_free(c, c->items_base, sizeof(float) * (size_t)c->n_item_base * c->ld);
     c->items_base = c->items;
@@ -229,8 +247,9 @@ void free_table(Ctx *c) {
     dev_free(c, c->qranks, 0);
     dev_free(c, c->qcol, 0);
     dev_free(c, c->qtable, 0);
+    dev_free(c, c->gvals, 0);
     c->ranks = c->qranks = c->qcol = nullptr;
-    c->table = c->qtable = nullptr;
+    c->table = c->qtable = c->gvals = nullptr;
     c->n_sample = c->n_qs = 0;
     c->h_ranks.clear();
     c->h_qranks.clear();
@@ -573,6 +592,22 @@ int rkr_set_data(rkr_ctx *c, const float *users, int64_t n_user, const float *it
 
 int rkr_build_table(rkr_ctx *c, const int32_t *ranks, int32_t S, double *build_ms) {
     if (!c) return RKR_ERR_INVALID;
+    return rkr::build_table(c, ranks, S, build_ms, false);
+}
+
+}  // extern "C"
+
+namespace rkr {
+
+int rebuild_table(Ctx *c) {
+    if (!c->table) return set_err(c, RKR_ERR_STATE, "rebuild: no table");
+    std::vector<int32_t> ranks((size_t)c->n_sample);
+    const int rc = rkr_uniform_sample_ranks(c->n_item, c->n_sample, ranks.data());
+    if (rc) return set_err(c, rc, "rebuild: n_sample does not fit the current items");
+    return build_table(c, ranks.data(), c->n_sample, nullptr, true);
+}
+
+int build_table(Ctx *c, const int32_t *ranks, int32_t S, double *build_ms, bool keep_update_buffers) {
     if (!c->users) return set_err(c, RKR_ERR_STATE, "build_table: no data");
     if (!ranks || S > kMaxSample || !ranks_valid(ranks, S, c->n_item))
         return set_err(c, RKR_ERR_INVALID, "build_table: ranks must be strictly increasing in [0, n_item), "
@@ -580,10 +615,11 @@ int rkr_build_table(rkr_ctx *c, const int32_t *ranks, int32_t S, double *build_m
     RKR_HIP(c, hipSetDevice(c->device));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
     free_table(c);
-    free_update_state(c);  // the new table describes the current item set
+    free_update_state(c, keep_update_buffers);  // the new table describes the current item set
     int rc = dev_alloc_t(c, &c->ranks, (size_t)S, "ranks");
     if (rc) return rc;
     rc = dev_alloc_t(c, &c->table, (size_t)std::max(c->u_cap, c->n_user) * S, "table");
+    if (!rc) rc = dev_alloc_t(c, &c->gvals, (size_t)std::max(c->u_cap, c->n_user) * 8, "guard values");
     if (rc) { free_table(c); return rc; }
     c->n_sample = S;
     c->h_ranks.assign(ranks, ranks + S);
@@ -643,11 +679,16 @@ int rkr_build_table(rkr_ctx *c, const int32_t *ranks, int32_t S, double *build_m
             RKR_HIP(c, launch_select_rows(c->stream, scores, n, c->n_item, c->ranks, S, c->table + u0 * S));
         }
     }
+    if ((rc = refresh_guard_values(c, 0, c->n_user))) { free_table(c); return rc; }
     RKR_HIP(c, hipEventRecord(c->ev[1], c->stream));
     RKR_HIP(c, hipStreamSynchronize(c->stream));
     if (build_ms) *build_ms = ev_ms(c->ev, 0, 1);
     return RKR_OK;
 }
+
+}  // namespace rkr
+
+extern "C" {
 
 int rkr_set_table(rkr_ctx *c, const int32_t *ranks, int32_t S, const float *table, float eps) {
     if (!c) return RKR_ERR_INVALID;
@@ -662,12 +703,15 @@ int rkr_set_table(rkr_ctx *c, const int32_t *ranks, int32_t S, const float *tabl
     int rc = dev_alloc_t(c, &c->ranks, (size_t)S, "ranks");
     if (rc) return rc;
     rc = dev_alloc_t(c, &c->table, (size_t)std::max(c->u_cap, c->n_user) * S, "table");
+    if (!rc) rc = dev_alloc_t(c, &c->gvals, (size_t)std::max(c->u_cap, c->n_user) * 8, "guard values");
     if (rc) { free_table(c); return rc; }
     c->n_sample = S;
     c->h_ranks.assign(ranks, ranks + S);
     c->table_eps = eps;
     RKR_HIP(c, hipMemcpy(c->ranks, ranks, sizeof(int32_t) * S, hipMemcpyHostToDevice));
     RKR_HIP(c, hipMemcpy(c->table, table, sizeof(float) * (size_t)c->n_user * S, hipMemcpyHostToDevice));
+    if ((rc = refresh_guard_values(c, 0, c->n_user))) { free_table(c); return rc; }
+    RKR_HIP(c, hipStreamSynchronize(c->stream));
     return RKR_OK;
 }
 
@@ -1359,7 +1403,8 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                                 (1.0 + std::ldexp(1.0, -18));
             uint8_t *cb = reinterpret_cast<uint8_t *>(bk);
             CoarseLaunch L{};
-            L.users = dv.users; L.user_norm = dv.user_norm; L.table = T; L.n_user = U; L.S = S; L.gc = gc;
+            L.users = dv.users; L.user_norm = dv.user_norm; L.table = T; L.gvals = dv.gvals; L.n_user = U; L.S = S;
+            L.gc = gc;
             L.user_hi = dv.user_hi; L.user_lo = dv.user_lo;
             L.q_hi = q_hi; L.q_lo = q_lo; L.q_norm = qn; L.qmap = nullptr; L.nq = nq; L.ld = ld; L.K = d;
             L.Kp = c->kp; L.C = c->screen_C; L.abs_floor = std::nextafter((float)absf, INFINITY); L.cb = cb;

@@ -120,6 +120,9 @@ struct Ctx {
     int32_t *qcol = nullptr;    // n_sample
     float *qtable = nullptr;    // [u_cap][n_qs]
     std::vector<int32_t> h_qranks;
+    // guard values of the coarse bounds' first pass: [u_cap][8], the values of
+    // the searched table at guard_columns(), -inf past them; kept with the table
+    float *gvals = nullptr;
     int dense_top = 2;          // query table mode (RKR_OPT_DENSE_TOP)
 
     // screened form: hi/lo BF16 item rows (n_item x kp), norm bounds
@@ -227,6 +230,7 @@ struct DataSnap {
     int32_t n_qs = 0;
     int32_t n_guard = 0;             // guard columns of the coarse bounds
     int32_t gcol[8] = {};
+    const float *gvals = nullptr;    // their values per user (8 floats), or null
     // pending inserts: the most inserted items any user has above its value
     // at guard column j (a bound on the rank shift at that guard)
     int32_t ins_gmax[8] = {};
@@ -238,6 +242,14 @@ struct DataSnap {
 DataSnap snap_of(const Ctx *c);
 // Guard columns of the coarse bounds for the context's searched table.
 void guard_columns(const Ctx *c, int32_t *col, int32_t &n);
+// Fills Ctx::gvals rows [u0, u0 + n) from the searched table.
+// The table over the current items at sampled ranks S (build_table), and the
+// same at uniform ranks of the current n_sample (rebuild_table): the pending
+// item updates fold into the new table; keep_update_buffers leaves their
+// buffers allocated for the updates that follow.
+int build_table(Ctx *c, const int32_t *ranks, int32_t S, double *build_ms, bool keep_update_buffers);
+int rebuild_table(Ctx *c);
+int refresh_guard_values(Ctx *c, int64_t u0, int64_t n);
 // Recomputes Ctx::ins_gmax after inserted items or users changed.
 int refresh_ins_guard_max(Ctx *c);
 // Ranks of the query table: the sample ranks and the dense top ranks below
@@ -331,6 +343,7 @@ struct GuardCols {
 // row(b) = qmap[b] (or b when qmap is null), which also picks the query rows.
 struct CoarseLaunch {
     const float *users, *user_norm, *table;
+    const float *gvals = nullptr;       // n_user x 8 guard values of gc (first pass), or null: read the table
     const uint16_t *user_hi, *user_lo;  // BF16 split of the user rows (n_user x Kp)
     int64_t n_user;
     int32_t S;
@@ -510,6 +523,10 @@ hipError_t launch_qsrp_bounds(hipStream_t s, const float *left, int64_t n_user, 
 hipError_t launch_qsrp_exact(hipStream_t s, const float *users, const float *q_in, int32_t ld, int32_t d,
                              const float *T, int32_t S, float eps, int64_t n_user, int32_t nq, QueryInfo *qi,
                              float *qip, uint32_t *bk);
+
+// gv[u * 8 + j] = T[u][gc.col[j]] (-inf for j >= gc.n), rows [u0, u0 + n_rows).
+hipError_t launch_gather_guards(hipStream_t s, const float *T, int64_t S, const GuardCols &gc, int64_t u0,
+                                int64_t n_rows, float *gv);
 
 // The sort scratch of rkr_sorted_scores_noalloc: looks up its device address
 // on `device` (current) once per device.

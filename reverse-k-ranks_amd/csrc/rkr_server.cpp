@@ -72,6 +72,7 @@ struct Request {
 struct rkr_server {
     Ctx *c = nullptr;
     rkr_server_config cfg{};
+    int64_t rebuild_at = 12288;  // pending item updates that trigger a table rebuild (< 0: never)
     std::vector<std::unique_ptr<Slot>> slots;
     std::vector<std::thread> workers;
     std::mutex mu;
@@ -84,7 +85,7 @@ struct rkr_server {
     int busy = 0;            // query batches in flight
     bool exclusive = false;  // an update is running (barrier mode)
     bool stopping = false;
-    int64_t n_batches = 0, n_queries = 0, n_updates = 0;
+    int64_t n_batches = 0, n_queries = 0, n_updates = 0, n_rebuilds = 0;
     // snapshot mode: the committed state, its version, and the versions the
     // batches in flight started from (version -> batches)
     DataSnap published;
@@ -106,14 +107,28 @@ struct rkr_server {
     // no batch in flight started before version v
     bool drained_before(uint64_t v) const { return inflight.empty() || inflight.begin()->first >= v; }
 
+    // Applies one update; when the pending inserted and deleted items then reach
+    // the rebuild threshold, rebuilds the table over the current items (uniform
+    // ranks, same n_sample), which folds the side arrays into a fresh base, so an
+    // endless insert stream never meets the side arrays' cap.  In snapshot mode
+    // this runs on the update thread and its CUs: batches keep reading the
+    // previous state until the commit, and the replaced buffers are retired.
     int apply_update(Request *r) {
+        int rc;
         switch (r->kind) {
-            case Q_INSERT_USERS: return rkr_insert_users(static_cast<rkr_ctx *>(c), r->data.data(), r->n);
-            case Q_INSERT_ITEMS: return rkr_insert_items(static_cast<rkr_ctx *>(c), r->data.data(), r->n);
-            case Q_DELETE_USERS: return rkr_delete_users(static_cast<rkr_ctx *>(c), r->ids.data(), r->n);
-            case Q_DELETE_ITEMS: return rkr_delete_items(static_cast<rkr_ctx *>(c), r->ids.data(), r->n);
+            case Q_INSERT_USERS: rc = rkr_insert_users(static_cast<rkr_ctx *>(c), r->data.data(), r->n); break;
+            case Q_INSERT_ITEMS: rc = rkr_insert_items(static_cast<rkr_ctx *>(c), r->data.data(), r->n); break;
+            case Q_DELETE_USERS: rc = rkr_delete_users(static_cast<rkr_ctx *>(c), r->ids.data(), r->n); break;
+            case Q_DELETE_ITEMS: rc = rkr_delete_items(static_cast<rkr_ctx *>(c), r->ids.data(), r->n); break;
             default: return RKR_ERR_INVALID;
         }
+        if (rc || rebuild_at < 0 || (int64_t)c->n_ins + c->n_del < rebuild_at) return rc;
+        rc = rebuild_table(c);
+        if (!rc) {
+            std::lock_guard<std::mutex> g(mu);
+            ++n_rebuilds;
+        }
+        return rc;
     }
 
     void run_batch(Slot &sl, const DataSnap &snap, std::vector<Request *> &all) {
@@ -319,6 +334,8 @@ int rkr_server_start(rkr_ctx *c, const rkr_server_config *cfg, rkr_server **out)
     s->cfg.max_batch = cfg && cfg->max_batch > 0 ? cfg->max_batch : 256;
     s->cfg.n_slots = cfg && cfg->n_slots > 0 ? std::min(cfg->n_slots, 4) : 2;
     s->cfg.update_mode = cfg ? cfg->update_mode : RKR_SERVE_BARRIER;
+    s->cfg.rebuild_pending = cfg ? cfg->rebuild_pending : 0;
+    s->rebuild_at = s->cfg.rebuild_pending == 0 ? 12288 : s->cfg.rebuild_pending < 0 ? -1 : s->cfg.rebuild_pending;
     if (s->cfg.update_mode != RKR_SERVE_BARRIER && s->cfg.update_mode != RKR_SERVE_SNAPSHOT) {
         delete s;
         return set_err(c, RKR_ERR_INVALID, "server: unknown update mode");
@@ -441,6 +458,13 @@ int rkr_server_counters(rkr_server *s, int64_t *n_batches, int64_t *n_queries, i
     if (n_batches) *n_batches = s->n_batches;
     if (n_queries) *n_queries = s->n_queries;
     if (n_updates) *n_updates = s->n_updates;
+    return RKR_OK;
+}
+
+int rkr_server_rebuilds(rkr_server *s, int64_t *n_rebuilds) {
+    if (!s || !n_rebuilds) return RKR_ERR_INVALID;
+    std::lock_guard<std::mutex> g(s->mu);
+    *n_rebuilds = s->n_rebuilds;
     return RKR_OK;
 }
 

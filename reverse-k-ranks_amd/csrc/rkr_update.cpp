@@ -78,22 +78,23 @@ int reserve_users(Ctx *c, int64_t need) {
     const int32_t ld = c->ld, S = c->n_sample;
     const int32_t QS = c->n_qs;
     float *users = nullptr, *panels = nullptr, *unorm = nullptr, *table = nullptr, *qtable = nullptr, *si = nullptr,
-          *sd = nullptr;
+          *sd = nullptr, *gv = nullptr;
     int rc = RKR_OK;
-    if ((rc = dev_alloc_t(c, &users, (size_t)cap * ld, "users")) ||
+    if ((c->gvals && (rc = dev_alloc_t(c, &gv, (size_t)cap * 8, "guard values"))) ||
+        (rc = dev_alloc_t(c, &users, (size_t)cap * ld, "users")) ||
         (rc = dev_alloc_t(c, &panels, (size_t)((cap + 63) / 64) * 64 * ld, "user panels")) ||
         (c->kp && (rc = dev_alloc_t(c, &unorm, (size_t)cap, "user norms"))) ||
         (rc = dev_alloc_t(c, &table, (size_t)cap * S, "table")) ||
         (QS && (rc = dev_alloc_t(c, &qtable, (size_t)cap * QS, "query table"))) ||
         (c->ld_ins && (rc = dev_alloc_t(c, &si, (size_t)cap * c->ld_ins, "side array"))) ||
         (c->ld_del && (rc = dev_alloc_t(c, &sd, (size_t)cap * c->ld_del, "side array")))) {
-        for (float *p : {users, panels, unorm, table, qtable, si, sd}) dev_free(c, p, 0);
+        for (float *p : {users, panels, unorm, table, qtable, si, sd, gv}) dev_free(c, p, 0);
         return rc;
     }
     uint16_t *uhi = nullptr, *ulo = nullptr;
     if (c->kp && ((rc = dev_alloc_t(c, &uhi, (size_t)cap * c->kp, "user hi")) ||
                   (rc = dev_alloc_t(c, &ulo, (size_t)cap * c->kp, "user lo")))) {
-        for (float *p : {users, panels, unorm, table, qtable, si, sd}) dev_free(c, p, 0);
+        for (float *p : {users, panels, unorm, table, qtable, si, sd, gv}) dev_free(c, p, 0);
         dev_free(c, uhi, 0);
         dev_free(c, ulo, 0);
         return rc;
@@ -109,13 +110,15 @@ int reserve_users(Ctx *c, int64_t need) {
         (c->kp && (rc = copy_rows(c, unorm, 1, c->user_norm, 1, U, 1))) ||
         (rc = copy_rows(c, table, S, c->table, S, U, S)) ||
         (QS && (rc = copy_rows(c, qtable, QS, c->qtable, QS, U, QS))) ||
+        (gv && (rc = copy_rows(c, gv, 8, c->gvals, 8, U, 8))) ||
         (si && (rc = copy_rows(c, si, c->ld_ins, c->side_ins, c->ld_ins, U, c->n_ins))) ||
         (sd && (rc = copy_rows(c, sd, c->ld_del, c->side_del, c->ld_del, U, c->n_del))))
         return rc;
     RKR_HIP(c, hipStreamSynchronize(c->stream));
     for (float *p : {c->users, c->usersT, c->user_norm, c->table, c->qtable, c->side_ins, c->side_del,
-                     c->side_ins_spare, c->side_del_spare})
+                     c->side_ins_spare, c->side_del_spare, c->gvals})
         dev_free(c, p, 0);
+    c->gvals = gv;
     dev_free(c, c->user_hi, 0);
     dev_free(c, c->user_lo, 0);
     c->user_hi = uhi;
@@ -511,6 +514,7 @@ int rkr_insert_users(rkr_ctx *c, const float *users, int64_t n) {
             RKR_HIP(c, launch_select_rows(c->stream, block, m, c->n_item_base, c->ranks, S, c->table + (U0 + u0) * S));
         }
     }
+    if ((rc = refresh_guard_values(c, U0, n))) return rc;
     // side rows of the new users, sorted in place at the side arrays' stride
     for (int side = 0; side < 2; ++side) {
         float *arr = side == 0 ? c->side_ins : c->side_del;
@@ -557,6 +561,7 @@ int rkr_delete_users(rkr_ctx *c, const int64_t *ids, int64_t n) {
     rc = keep_user_rows(c, &c->users, ld, ld, d_keep, nk, "users");
     if (!rc) rc = keep_user_rows(c, &c->table, c->n_sample, c->n_sample, d_keep, nk, "table");
     if (!rc) rc = keep_user_rows(c, &c->qtable, c->n_qs, c->n_qs, d_keep, nk, "query table");
+    if (!rc) rc = keep_user_rows(c, &c->gvals, 8, 8, d_keep, nk, "guard values");
     if (!rc) rc = keep_user_rows(c, &c->side_ins, c->n_ins, c->ld_ins, d_keep, nk, "side array");
     if (!rc) rc = keep_user_rows(c, &c->side_del, c->n_del, c->ld_del, d_keep, nk, "side array");
     dev_free(c, d_keep, 0);

@@ -52,7 +52,7 @@ EXPORTS = (
     "rkr_find_optimal_sample", "rkr_query",
     "rkr_index_size_bytes", "rkr_query_table_cols", "rkr_insert_items", "rkr_delete_items", "rkr_insert_users", "rkr_delete_users",
     "rkr_get_counts", "rkr_reserve", "rkr_now", "rkr_server_start", "rkr_server_submit_query", "rkr_server_submit_update",
-    "rkr_server_wait", "rkr_server_poll", "rkr_server_counters", "rkr_server_stop",
+    "rkr_server_wait", "rkr_server_poll", "rkr_server_counters", "rkr_server_rebuilds", "rkr_server_stop",
     "rkr_qsrp_build", "rkr_qsrp_info", "rkr_qsrp_screen", "rkr_qsrp_phi", "rkr_qsrp_fit_line",
 )
 
@@ -92,7 +92,7 @@ class BatchTiming(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_batch", C.c_int32), ("n_slots", C.c_int32), ("update_mode", C.c_int32),
-                ("update_cus", C.c_int32)]
+                ("update_cus", C.c_int32), ("rebuild_pending", C.c_int32)]
 
 
 RKR_SERVE_BARRIER = 0
@@ -164,6 +164,7 @@ def load():
     L.rkr_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(C.c_int)]
     L.rkr_server_counters.argtypes = [vp] + [C.POINTER(C.c_int64)] * 3
     L.rkr_server_stop.argtypes = [vp]
+    L.rkr_server_rebuilds.argtypes = [vp, C.POINTER(C.c_int64)]
     L.rkr_qsrp_build.argtypes = [vp, C.c_float, C.POINTER(C.c_double)]
     L.rkr_qsrp_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
     L.rkr_qsrp_screen.argtypes = [vp, f32p, i32, C.c_void_p]

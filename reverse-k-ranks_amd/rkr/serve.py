@@ -12,11 +12,11 @@ class Server:
     """Continuous-batching request loop on an Index's device context."""
 
     def __init__(self, index, max_batch: int = 256, n_slots: int = 2, update_mode: str = "barrier",
-                 update_cus: int = 0):
+                 update_cus: int = 0, rebuild_pending: int = 0):
         self.index = index
         self.lib = index.lib
         mode = {"barrier": L.RKR_SERVE_BARRIER, "snapshot": L.RKR_SERVE_SNAPSHOT}[update_mode]
-        cfg = L.ServerConfig(max_batch, n_slots, mode, update_cus)
+        cfg = L.ServerConfig(max_batch, n_slots, mode, update_cus, rebuild_pending)
         h = C.c_void_p()
         rc = self.lib.rkr_server_start(index.ctx, C.byref(cfg), C.byref(h))
         if rc != L.RKR_OK:
@@ -64,6 +64,12 @@ class Server:
         v = [C.c_int64() for _ in range(3)]
         self.lib.rkr_server_counters(self.h, *[C.byref(x) for x in v])
         return dict(zip(("n_batches", "n_queries", "n_updates"), (x.value for x in v)))
+
+    def rebuilds(self) -> int:
+        """Table rebuilds the loop made when pending item updates reached rebuild_pending."""
+        v = C.c_int64()
+        self.lib.rkr_server_rebuilds(self.h, C.byref(v))
+        return v.value
 
     def now(self) -> float:
         return float(self.lib.rkr_now())

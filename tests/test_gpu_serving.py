@@ -144,3 +144,48 @@ def test_snapshot_updates(orc):
         cnt = srv.counters()
         assert cnt["n_updates"] == 2
     assert idx.counts()["n_user"] == 5800 and idx.counts()["n_item"] == 2500
+
+
+@pytest.mark.parametrize("mode", ["barrier", "snapshot"])
+def test_inserts_past_the_side_cap_rebuild_the_table(orc, mode):
+    """A steady stream of item inserts with a small rebuild threshold: the loop
+    rebuilds the table over the current items mid-stream (folding the side
+    arrays into a fresh base) and keeps accepting inserts; every answer equals
+    brute force over a state the stream passed through (barrier mode: exactly
+    the updates submitted before the query)."""
+    _case(f"serving rebuilds under inserts ({mode})")
+    import rkr
+    rng, users, items, idx = _setup(seed=45, U=3000, I=1500, d=24, S=32)
+    k = 25
+    qs = items[rng.choice(len(items), 6, replace=False)]
+    rounds = [rng.standard_normal((40, items.shape[1]), dtype=np.float32) * np.float32(1.1) for _ in range(7)]
+    un = orc.normalize_users(users)
+    states = [items]
+    for r in rounds:
+        states.append(np.concatenate([states[-1], r]))
+    truth = [orc.bruteforce(un, st, qs, k)[:2] for st in states]
+    with rkr.Server(idx, max_batch=8, n_slots=2, update_mode=mode, rebuild_pending=100) as srv:
+        tickets = []
+        for r in rounds:
+            tickets.append(("q", [srv.submit_query(q, k) for q in qs]))
+            tickets.append(("u", srv.submit_update("insert_items", vectors=r)))
+        tickets.append(("q", [srv.submit_query(q, k) for q in qs]))
+        n_upd = 0
+        for kind, t in tickets:
+            if kind == "u":
+                assert srv.wait(t)[3]["status"] == 0
+                n_upd += 1
+                continue
+            for j, tq in enumerate(t):
+                u, r, _, info = srv.wait(tq, k)
+                assert info["status"] == 0
+                ok = [s for s, (bu, br) in enumerate(truth) if np.array_equal(u, bu[j]) and np.array_equal(r, br[j])]
+                assert ok, f"query {j} after {n_upd} insert rounds matches no state"
+                if mode == "barrier":
+                    assert n_upd in ok
+        assert srv.rebuilds() >= 2  # 280 inserted items, a rebuild at every 100 pending
+    c = idx.counts()
+    assert c["n_item"] == 1500 + 280 and c["n_inserted"] < 100
+    bu, br = truth[-1]
+    res = idx.query(qs, k)
+    assert np.array_equal(res.users, bu) and np.array_equal(res.ranks, br)
