@@ -475,8 +475,8 @@ def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_max, seconds):
 PROFILED_TRAFFIC = {
     ("plain_gpu", "screened"): {"kernel": "k_ip_coarse", "bytes": 2.529e8, "fetch_kib": 74559.9,
                                 "write_kib": 97847.8, "profile": "profiles/k_ip_coarse_100k_x_20k_d150_k50.md"},
-    ("precision", "screened"): {"kernel": "k_ip_coarse", "bytes": 1.769e9, "fetch_kib": 374879.0,
-                                "write_kib": 977798.4, "profile": "profiles/k_ip_coarse_1M_x_50k_d150_k50.md"},
+    ("precision", "screened"): {"kernel": "k_ip_coarse", "bytes": 1.401e9, "fetch_kib": 195471.2,
+                                "write_kib": 977638.4, "profile": "profiles/k_ip_coarse_1M_x_50k_d150_k50.md"},
 }
 
 
@@ -582,17 +582,18 @@ def main():
     if dom != "refine" and coarse_used:
         # the coarse-bound kernel (first pass) is the largest single kernel of
         # the ip + bound stages: 2*d flops per (query, user) pair, executed
-        # as 3 BF16 products over d padded to 32 on the matrix cores
+        # as one BF16 product over d padded to 32 on the matrix cores
         kp = (d + 31) // 32 * 32
         flops = 2.0 * U * d * n_distinct * args.steps
         ach = flops / (tot["ip_ms"] * 1e-3) / 1e12
-        executed = ach * 3 * kp / d
+        executed = ach * kp / d
         roof = {"kernel": "k_ip_coarse", "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_BF16_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(ach / PEAK_BF16_TFLOPS, 4), "traffic": None,
                 "executed_mfma_tflops": round(executed, 2), "executed_frac": round(executed / PEAK_BF16_TFLOPS, 4),
                 "avg_launch_ms": round(tot["ip_ms"] / launches, 4),
                 "work_per_launch": f"2*U*d algorithmic flops x {n_distinct * args.steps / launches:.0f} distinct queries "
-                                   f"(executed: 3*2*U*{kp}); first pass only, the second pass is in the bound stage"}
+                                   f"(executed: 2*U*{kp}, one BF16 product); first pass only, the second pass is in "
+                                   "the bound stage"}
     elif dom == "refine":
         flops = 2.0 * I * d * tot["n_refined"]
         ach = flops / (tot["refine_ms"] * 1e-3) / 1e12
@@ -693,8 +694,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16" if args.form == "screened" else "f32",
-        "dtype_note": ("refinement screen on BF16 matrix cores (3-product hi/lo split, FP32 accumulation), "
-                       "undecided items rechecked with the exact FP32 rule; answers identical to the f32 form"
+        "dtype_note": ("coarse bounds on BF16 matrix cores (one product per term in the first pass, the "
+                       "3-product hi/lo split in the second), refinement screen with the 3-product split, FP32 "
+                       "accumulation throughout; undecided items rechecked with the exact FP32 rule; answers "
+                       "identical to the f32 form"
                        if args.form == "screened" else "exact sequential FP32 score rule"),
         "data": "synthetic (seeded), inputs resident in HBM",
         "config": {"workload": label, "form": args.form, "queries_per_step": nq,

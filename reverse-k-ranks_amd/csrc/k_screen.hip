@@ -338,16 +338,22 @@ __global__ void k_recheck(const float *__restrict__ users, const float *__restri
 }
 
 
-// Coarse buckets of a query batch.  For every (query b, user u) the screened
-// estimate s^ = <u,q_b> (the same BF16 hi/lo three-product form as above,
-// |s^ - s| <= e = C ||u|| ||q|| + abs_floor) is compared with G <= 7 guard
-// values of u's table row, T[u][col_0] > T[u][col_1] > ... (col ascending):
+// Coarse buckets of a query batch.  For every (query b, user u) a screened
+// estimate s^ of <u,q_b> with |s^ - s| <= e = C ||u|| ||q|| + abs_floor is
+// compared with G <= 7 guard values of u's table row, T[u][col_0] >=
+// T[u][col_1] >= ... (col ascending):
 //   c_lo = #{g : guard_g > s^ - e},  c_hi = #{g : guard_g > s^ + e}
 // and cb[row(b) * ldc + u] = c_lo | c_hi << 4.  abs_floor also carries the
-// table's own eps.  Queries are the A operand (staged through LDS, 64 per
-// stage in the first pass, 32 with guard sets), users the B operand held in VGPRs: lane (lr, lq) of a 16x16 tile
-// owns user lr and queries 4 lq .. 4 lq + 3, so its guards stay in registers.  qmap (optional) maps the
-// launch's queries to rows of q_hi / q_lo / q_norm and cb.
+// table's own eps.  The first pass (one guard set) takes the single product
+// s^ = sum_k u_hi q_hi, so C there covers the BF16 rounding of both factors
+// (2^-7 + 2^-16 of sum|u_k q_k|): a wider interval only lists a few more
+// users, and the pass does a third of the matrix work.  The second pass
+// (guard sets inside one bin) keeps the three-product form above.  Queries
+// are the A operand (staged through LDS, 64 per stage in the first pass, 32
+// with guard sets), users the B operand held in VGPRs: lane (lr, lq) of a
+// 16x16 tile owns user lr and queries 4 lq .. 4 lq + 3, so its guards stay in
+// registers.  qmap (optional) maps the launch's queries to rows of q_hi /
+// q_lo / q_norm and cb.
 struct IpCoarseParams {
     const float *users;           // n_user x ld
     const float *user_norm;       // n_user
@@ -374,43 +380,34 @@ struct IpCoarseParams {
 
 constexpr int kMaxGuardSets = 7;
 
-// #{j < 7 : g[j] > x} (-inf padded past G).  g > x exactly when x - g is
-// negative or -0 (x - g is +0 for x == g and +inf for g = -inf), so the
-// count sums sign bits: no compare masks, no mask-read hazards.
-__device__ __forceinline__ uint32_t count_above8(const float (&g)[8], float x) {
-    uint32_t c = 0;
-#pragma unroll
-    for (int j = 0; j < 7; ++j) c += __float_as_uint(x - g[j]) >> 31;
-    return c;
-}
-
-typedef unsigned short ushort2_t __attribute__((ext_vector_type(2)));
-
-// The coarse byte c_lo | c_hi << 4 with c_lo = #{g > xlo} and c_hi = #{g > xhi},
-// by the same sign bits: one byte permute spreads the signs of xlo - g and
-// xhi - g over the two 16-bit halves of a word (selector 9: the sign of the
-// second source's top byte, 11: of the first's), and one packed 16-bit
-// subtraction counts both.
-__device__ __forceinline__ uint32_t coarse_byte(const float (&g)[8], float xlo, float xhi) {
-    ushort2_t acc = {0, 0};
-#pragma unroll
-    for (int j = 0; j < 7; ++j) {
-        const uint32_t m = __builtin_amdgcn_perm(__float_as_uint(xhi - g[j]), __float_as_uint(xlo - g[j]), 0x0b0b0909u);
-        acc -= __builtin_bit_cast(ushort2_t, m);  // each half is 0xffff where its sign was set
-    }
-    const uint32_t c = __builtin_bit_cast(uint32_t, acc);
-    return (c | (c >> 12)) & 0xffu;
+// #{j < 7 : g[j] > x} for g[0] >= g[1] >= ... >= g[6] (-inf padded past G):
+// the predicate g[j] > x is monotone in j, so three tests of a branchless
+// binary search decide the count -- g[3] splits {0..3} from {4..7}, then
+// g[5] or g[1], then one of g[6], g[4], g[2], g[0].
+__device__ __forceinline__ uint32_t guards_above(const float (&g)[8], float x) {
+    const bool b2 = g[3] > x;
+    const float p1 = __builtin_unpredictable(b2) ? g[5] : g[1];
+    const bool b1 = p1 > x;
+    const float pa = __builtin_unpredictable(b1) ? g[6] : g[4];
+    const float pb = __builtin_unpredictable(b1) ? g[2] : g[0];
+    const float p0 = __builtin_unpredictable(b2) ? pa : pb;
+    const bool b0 = p0 > x;
+    return (b2 ? 4u : 0u) | (b1 ? 2u : 0u) | (b0 ? 1u : 0u);
 }
 
 template <int KS, bool SETS>
-__global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
+__global__ void __launch_bounds__(NT, SETS ? 2 : (KS >= 7 ? 3 : 4)) k_ip_coarse(IpCoarseParams P) {
+    // first pass: one product, so only the queries' hi halves are staged and
+    // only the users' hi halves are held (4 workgroups per CU fit)
+    constexpr bool ONE = !SETS;
+    constexpr int NH = ONE ? 1 : 2;
     // queries per LDS stage: 64 halves the stage barriers and query refills per
     // user block; the second pass keeps 32 so its guard sets also fit 3
     // workgroups per CU in LDS
     constexpr int QST = SETS ? 32 : 64;
     constexpr int ROWQ = (KS * 32 + KPAD_LDS) / 8;
     constexpr int KQ = KS * 4;
-    __shared__ uint4 sA[2][QST * ROWQ];
+    __shared__ uint4 sA[NH][QST * ROWQ];
     __shared__ float sQn[QST];
     __shared__ int64_t sQo[QST];  // byte offset of the query's cb row
     __shared__ int32_t sQs[QST];
@@ -420,18 +417,26 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
     const int64_t n_rb = (P.n_user + 127) / 128;
     for (int64_t rb = blockIdx.x; rb < n_rb; rb += gridDim.x) {
         const int64_t row_base = rb * 128 + wave * 32;
-        Frag uh[2][KS], ul[2][KS];
+        Frag uh[2][KS], ul[2][ONE ? 1 : KS];
         float gd[2][8], ar[2];
         bool live[2];
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             const int64_t r = row_base + b * 16 + lr;
             live[b] = r < P.n_user;
+            // rows past n_user load row 0 and are zeroed after the load (a
+            // select between a load and a constant can become a load through
+            // a selected pointer, flat and from scratch)
+            const int64_t rr = live[b] ? r : 0;
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const uint4 z = make_uint4(0, 0, 0, 0);
-                uh[b][ks].q = live[b] ? reinterpret_cast<const uint4 *>(P.user_hi + r * P.Kp)[ks * 4 + lq] : z;
-                ul[b][ks].q = live[b] ? reinterpret_cast<const uint4 *>(P.user_lo + r * P.Kp)[ks * 4 + lq] : z;
+                const uint4 h = reinterpret_cast<const uint4 *>(P.user_hi + rr * P.Kp)[ks * 4 + lq];
+                uh[b][ks].q = live[b] ? h : z;
+                if (!ONE) {
+                    const uint4 l = reinterpret_cast<const uint4 *>(P.user_lo + rr * P.Kp)[ks * 4 + lq];
+                    ul[b][ONE ? 0 : ks].q = live[b] ? l : z;
+                }
             }
             if (!SETS) {
                 if (P.gvals) {
@@ -463,7 +468,7 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
         }
         for (int32_t q0 = 0; q0 < P.nq; q0 += QST) {
             __syncthreads();  // every wave is done with the previous stage
-            for (int c = threadIdx.x; c < 2 * QST * KQ; c += NT) {
+            for (int c = threadIdx.x; c < NH * QST * KQ; c += NT) {
                 const int half = c / (QST * KQ);
                 const int rem = c % (QST * KQ);
                 const int qq = q0 + rem / KQ;
@@ -490,12 +495,15 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
                 for (int ks = 0; ks < KS; ++ks) {
                     Frag qh, ql;
                     qh.q = sA[0][it * ROWQ + ks * 4 + lq];
-                    ql.q = sA[1][it * ROWQ + ks * 4 + lq];
+                    if (!ONE) ql.q = sA[NH - 1][it * ROWQ + ks * 4 + lq];
 #pragma unroll
                     for (int b = 0; b < 2; ++b) {
                         acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qh.v, uh[b][ks].v, acc[b], 0, 0, 0);
-                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qh.v, ul[b][ks].v, acc[b], 0, 0, 0);
-                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ql.v, uh[b][ks].v, acc[b], 0, 0, 0);
+                        if (!ONE) {
+                            acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qh.v, ul[b][ONE ? 0 : ks].v, acc[b], 0,
+                                                                              0, 0);
+                            acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ql.v, uh[b][ks].v, acc[b], 0, 0, 0);
+                        }
                     }
                 }
                 // D[4 lq + i][lr]: query q0 + g*16 + 4 lq + i, user row_base + b*16 + lr
@@ -515,7 +523,8 @@ __global__ void __launch_bounds__(NT, 2) k_ip_coarse(IpCoarseParams P) {
                             gd[b][4] = x1.x; gd[b][5] = x1.y; gd[b][6] = x1.z; gd[b][7] = x1.w;
                         }
                         const float e = __fmaf_rn(ar[b], qn, P.abs_floor);
-                        out[b * 16] = (uint8_t)coarse_byte(gd[b], acc[b][i] - e, acc[b][i] + e);
+                        out[b * 16] = (uint8_t)(guards_above(gd[b], acc[b][i] - e) |
+                                                (guards_above(gd[b], acc[b][i] + e) << 4));
                     }
                 }
             }

@@ -570,6 +570,13 @@ int rkr_set_data(rkr_ctx *c, const float *users, int64_t n_user, const float *it
         const double g_mfma = n3 * e23 / (1.0 - n3 * e23), g_rule = dim * e24 / (1.0 - dim * e24);
         const double C = (3.2 * std::ldexp(1.0, -16) + 1.03 * g_mfma + g_rule) * 1.01;
         c->screen_C = std::nextafter((float)C, INFINITY);
+        // the coarse first pass's single product u_hi q_hi: each BF16 factor
+        // is within 2^-8 of its FP32 value (round to nearest), so a product
+        // is within 2^-7 + 2^-16 of the exact one, then kp FP32 additions
+        // (x1.01 for the factors' magnitudes) and the rule's own rounding
+        const double n1 = (double)kp, g_mfma1 = n1 * e23 / (1.0 - n1 * e23);
+        const double C1 = (std::ldexp(1.0, -7) + std::ldexp(1.0, -16) + 1.01 * g_mfma1 + g_rule) * 1.01;
+        c->coarse_C1 = std::nextafter((float)C1, INFINITY);
         c->screen_abs = screen_abs_floor(kp, c->max_abs_user, c->max_abs_item);
         if ((rc = dev_alloc_t(c, &c->item_hi, (size_t)n_item * kp, "item hi")) ||
             (rc = dev_alloc_t(c, &c->item_lo, (size_t)n_item * kp, "item lo")) ||
@@ -1407,8 +1414,9 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             L.gc = gc;
             L.user_hi = dv.user_hi; L.user_lo = dv.user_lo;
             L.q_hi = q_hi; L.q_lo = q_lo; L.q_norm = qn; L.qmap = nullptr; L.nq = nq; L.ld = ld; L.K = d;
-            L.Kp = c->kp; L.C = c->screen_C; L.abs_floor = std::nextafter((float)absf, INFINITY); L.cb = cb;
-            L.ldc = ldc; L.spare_row = nqb; L.grid_cap = (int64_t)c->n_cu * 3;
+            L.Kp = c->kp; L.C = c->coarse_C1; L.abs_floor = std::nextafter((float)absf, INFINITY); L.cb = cb;
+            L.ldc = ldc; L.spare_row = nqb;
+            L.grid_cap = (int64_t)c->n_cu * (c->kp / 32 >= 7 ? 3 : 4);  // the first pass's workgroups per CU
             RKR_HIP(c, launch_ip_coarse(sl.stream, L));
             RKR_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
             // coarse chunks: a fixed user count, so launches over few queries
@@ -1526,6 +1534,8 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                 RKR_HIP(c, hipMemcpyAsync(gsets, hsets, sizeof(GuardCols) * sets.size(), hipMemcpyHostToDevice,
                                           sl.stream));
                 CoarseLaunch L2 = L;
+                L2.C = c->screen_C;  // the second pass's three-product estimate
+                L2.grid_cap = (int64_t)c->n_cu * 3;
                 L2.qmap = qmap;
                 L2.nq = nr;
                 L2.sets = gsets + 1;

@@ -132,6 +132,7 @@ struct Ctx {
     uint16_t *user_hi = nullptr, *user_lo = nullptr;  // BF16 hi/lo user rows ([u_cap] x kp)
     double max_abs_item = 0.0, max_abs_user_raw = 0.0;
     float screen_C = 0.0f, screen_abs = 0.0f;
+    float coarse_C1 = 0.0f;  // error constant of the coarse first pass (one BF16 product)
     int n_cu = 256;
 
     // QSRP screen (rkr_qsrp_build): users rotated into the SVD basis, the query
