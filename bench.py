@@ -227,7 +227,12 @@ def serving(args, idx, wl, label):
     th.start()
     lat, bsz, done = [], [], 0
     t_sub, svc = [], []
+    t_note = time.perf_counter()
     while done < n_arr:
+        if time.perf_counter() - t_note >= 10.0:  # a progress line every 10 s of a long run
+            t_note = time.perf_counter()
+            print(f"[serving] {t_note - start:.0f} s: {done} of {n_arr} answered, {len(upd)} updates submitted, "
+                  f"{srv.rebuilds()} rebuilds", file=sys.stderr, flush=True)
         if done < len(tickets):
             ts_, tst, tdn, bs = wait(tickets[done])
             lat.append((tdn - ts_) * 1e3)
@@ -473,7 +478,7 @@ def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_max, seconds):
 # plain 1 GiB device copy gives, bench.py --copy-calibration).  Keyed by
 # (workload, form); other configurations report null.
 PROFILED_TRAFFIC = {
-    ("plain_gpu", "screened"): {"kernel": "k_ip_coarse", "bytes": 2.529e8, "fetch_kib": 74559.9,
+    ("plain_gpu", "screened"): {"kernel": "k_ip_coarse", "bytes": 1.488e8, "fetch_kib": 23712.0,
                                 "write_kib": 97847.8, "profile": "profiles/k_ip_coarse_100k_x_20k_d150_k50.md"},
     ("precision", "screened"): {"kernel": "k_ip_coarse", "bytes": 1.401e9, "fetch_kib": 195471.2,
                                 "write_kib": 977638.4, "profile": "profiles/k_ip_coarse_1M_x_50k_d150_k50.md"},

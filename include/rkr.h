@@ -374,7 +374,11 @@ typedef struct rkr_server_config {
                                 side rebuilds the table over the current items (new
                                 uniform sample ranks, same n_sample) so inserts keep
                                 flowing past the side arrays' cap: 0 = 12288 (three
-                                quarters of the 16384 cap), < 0 = never            */
+                                quarters of the 16384 cap), < 0 = never.  With
+                                rebuilds on, an item update that would outgrow the
+                                side arrays' reserved length (rkr_reserve) also
+                                rebuilds first, so the arrays never grow while
+                                serving                                            */
 } rkr_server_config;
 
 /* Barrier: an update waits for the batches in flight and blocks new ones, so

@@ -359,7 +359,7 @@ struct IpCoarseParams {
     const float *user_norm;       // n_user
     const uint16_t *user_hi, *user_lo;  // n_user x Kp BF16 split
     const float *table;           // n_user x S
-    const float *gvals;           // n_user x 8 guard values (first pass), or null
+    const float *gvals;           // n_user x kGuardRow guard values (all sets), or null
     int64_t n_user;
     int32_t S;
     GuardCols gc;
@@ -376,6 +376,7 @@ struct IpCoarseParams {
     const GuardCols *sets;
     const int32_t *qset;
     int32_t n_sets;
+    int32_t set_id[7];            // guard-row set of each launch set (with gvals)
 };
 
 constexpr int kMaxGuardSets = 7;
@@ -441,7 +442,7 @@ __global__ void __launch_bounds__(NT, SETS ? 2 : (KS >= 7 ? 3 : 4)) k_ip_coarse(
             if (!SETS) {
                 if (P.gvals) {
                     // the user's guard values, one 32-byte row
-                    const float4 *g4 = reinterpret_cast<const float4 *>(P.gvals + (live[b] ? r : 0) * 8);
+                    const float4 *g4 = reinterpret_cast<const float4 *>(P.gvals + (live[b] ? r : 0) * kGuardRow);
                     const float4 x0 = g4[0], x1 = g4[1];
                     const float ninf = -INFINITY;
                     gd[b][0] = live[b] ? x0.x : ninf; gd[b][1] = live[b] ? x0.y : ninf;
@@ -462,8 +463,17 @@ __global__ void __launch_bounds__(NT, SETS ? 2 : (KS >= 7 ? 3 : 4)) k_ip_coarse(
             for (int t = threadIdx.x; t < P.n_sets * 128 * 8; t += NT) {
                 const int set = t / (128 * 8), ul = (t / 8) % 128, j = t % 8;
                 const int64_t r = rb * 128 + ul;
-                const GuardCols &gc = P.sets[set];
-                g[t] = (r < P.n_user && j < gc.n) ? P.table[r * P.S + gc.col[j]] : -INFINITY;
+                float v = -INFINITY;
+                if (r < P.n_user) {
+                    if (P.gvals) {
+                        // 32 contiguous bytes of the user's guard row
+                        v = P.gvals[r * kGuardRow + P.set_id[set] * 8 + j];
+                    } else {
+                        const GuardCols &gc = P.sets[set];
+                        if (j < gc.n) v = P.table[r * P.S + gc.col[j]];
+                    }
+                }
+                g[t] = v;
             }
         }
         for (int32_t q0 = 0; q0 < P.nq; q0 += QST) {
@@ -532,26 +542,32 @@ __global__ void __launch_bounds__(NT, SETS ? 2 : (KS >= 7 ? 3 : 4)) k_ip_coarse(
     }
 }
 
-// gv[u * 8 + j] = T[u][col[j]] for the n guard columns, -inf past them: the
-// coarse pass's guard values of every user in one 32-byte row (the columns sit
-// in different cache lines of a table row).
-__global__ void __launch_bounds__(256) k_gather_guards(const float *__restrict__ T, int64_t S, GuardCols gc,
+// gv[u * kGuardRow + 8 s + j] = T[u][set s column j], -inf past the set's n:
+// the coarse passes' guard values of every user in one 256-byte row (the
+// columns sit in different cache lines of a table row), gathered once per
+// table.
+__global__ void __launch_bounds__(256) k_gather_guards(const float *__restrict__ T, int64_t S, GuardSets gs,
                                                        int64_t u0, int64_t n_rows, float *__restrict__ gv) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n_rows * 8) return;
-    const int64_t u = u0 + i / 8;
-    const int j = (int)(i % 8);
-    gv[u * 8 + j] = j < gc.n ? T[u * S + gc.col[j]] : -INFINITY;
+    if (i >= n_rows * kGuardRow) return;
+    const int64_t u = u0 + i / kGuardRow;
+    const int sj = (int)(i % kGuardRow), j = sj & 7;
+    const GuardCols &g = gs.set[sj >> 3];
+    gv[u * kGuardRow + sj] = j < g.n ? T[u * S + g.col[j]] : -INFINITY;
 }
 
 }  // namespace
 
-hipError_t launch_gather_guards(hipStream_t s, const float *T, int64_t S, const GuardCols &gc, int64_t u0,
+hipError_t launch_gather_guards(hipStream_t s, const float *T, int64_t S, const GuardSets &gs, int64_t u0,
                                 int64_t n_rows, float *gv) {
     if (n_rows <= 0) return hipSuccess;
-    if (gc.n < 0 || gc.n > 7) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_gather_guards, dim3((unsigned)((n_rows * 8 + 255) / 256)), dim3(256), 0, s, T, S, gc, u0,
-                       n_rows, gv);
+    for (const GuardCols &g : gs.set) {
+        if (g.n < 0 || g.n > 7) return hipErrorInvalidValue;
+        for (int j = 0; j < g.n; ++j)
+            if (g.col[j] < 0 || g.col[j] >= S) return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k_gather_guards, dim3((unsigned)((n_rows * kGuardRow + 255) / 256)), dim3(256), 0, s, T, S, gs,
+                       u0, n_rows, gv);
     return hipGetLastError();
 }
 
@@ -645,9 +661,14 @@ hipError_t launch_ip_coarse(hipStream_t s, const CoarseLaunch &L) {
         return hipErrorInvalidValue;
     if (L.ldc % 128 || L.ldc < L.n_user) return hipErrorInvalidValue;
     if (!L.user_hi || !L.user_lo) return hipErrorInvalidValue;
-    IpCoarseParams P{L.users, L.user_norm, L.user_hi, L.user_lo, L.table, sets ? nullptr : L.gvals, L.n_user, L.S, L.gc, L.q_hi, L.q_lo,
-                     L.q_norm, L.qmap, L.nq, L.ld, L.K, L.Kp, L.C, L.abs_floor, L.cb, L.ldc, L.spare_row, L.sets,
-                     L.qset, L.n_sets};
+    IpCoarseParams P{L.users, L.user_norm, L.user_hi, L.user_lo, L.table, L.gvals, L.n_user, L.S, L.gc, L.q_hi,
+                     L.q_lo, L.q_norm, L.qmap, L.nq, L.ld, L.K, L.Kp, L.C, L.abs_floor, L.cb, L.ldc, L.spare_row,
+                     L.sets, L.qset, L.n_sets, {}};
+    if (sets && L.gvals)
+        for (int t = 0; t < L.n_sets; ++t) {
+            if (L.set_id[t] < 1 || L.set_id[t] >= kGuardSets) return hipErrorInvalidValue;
+            P.set_id[t] = L.set_id[t];
+        }
     const int64_t n_rb = (L.n_user + 127) / 128;
     const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(n_rb, L.grid_cap)));
 #define RKR_COARSE_CASE(ks)                                                                                   \

@@ -109,6 +109,12 @@ DataSnap snap_of(const Ctx *c) {
     d.n_qs = c->n_qs;
     guard_columns(c, d.gcol, d.n_guard);
     d.gvals = c->gvals;
+    d.ranks = c->ranks;
+    d.n_sample = c->n_sample;
+    d.table_eps = c->table_eps;
+    d.qcol = c->qcol;
+    d.hrk = std::make_shared<const std::vector<int32_t>>(c->qtable ? c->h_qranks : c->h_ranks);
+    if (d.n_guard > 0) d.gsets = guard_sets(c);
     for (int j = 0; j < 8; ++j) d.ins_gmax[j] = c->ins_gmax[j];
     if (c->qsrp_users && c->qsrp_n_user == c->n_user) {
         d.qsrp_users = c->qsrp_users;
@@ -120,6 +126,32 @@ DataSnap snap_of(const Ctx *c) {
         d.qsrp_crot = c->qsrp_crot;
     }
     return d;
+}
+
+// 64-bit hash of n bytes salted with `salt`: four independent multiply-add
+// lanes over 8-byte words (zero-padded tail), then an xorshift-multiply fold.
+// The dedup key of a query row; equal hashes are confirmed by a byte compare.
+uint64_t row_hash(const void *p, size_t n, uint32_t salt) {
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    const uint64_t m = 0xBF58476D1CE4E5B9ull;
+    uint64_t h[4] = {0x9E3779B97F4A7C15ull ^ n, 0xC2B2AE3D27D4EB4Full ^ salt, 0x165667B19E3779F9ull,
+                     0xD6E8FEB86659FD93ull};
+    size_t i = 0;
+    for (; i + 32 <= n; i += 32) {
+        uint64_t w[4];
+        std::memcpy(w, b + i, 32);
+        for (int l = 0; l < 4; ++l) h[l] = (h[l] + w[l]) * m;
+    }
+    if (i < n) {
+        uint64_t w[4] = {0, 0, 0, 0};
+        std::memcpy(w, b + i, n - i);
+        for (int l = 0; l < 4; ++l) h[l] = (h[l] + w[l]) * m;
+    }
+    uint64_t x = (h[0] ^ (h[0] >> 29)) + 3 * (h[1] ^ (h[1] >> 31)) + 5 * (h[2] ^ (h[2] >> 27)) +
+                 7 * (h[3] ^ (h[3] >> 33));
+    x ^= x >> 29;
+    x *= 0x94D049BB133111EBull;
+    return x ^ (x >> 32);
 }
 
 // Guard columns of the coarse bounds: the table columns nearest below ranks
@@ -138,13 +170,41 @@ void guard_columns(const Ctx *c, int32_t *col, int32_t &n) {
     }
 }
 
+// Set 0: the guard columns.  Set 1 + beta: up to 7 columns inside guard bin
+// beta (the columns after guard beta - 1 up to guard beta, the bin's last
+// column included), spaced geometrically in rank, for the second coarse pass
+// of queries whose k-th bound fell in that bin.
+GuardSets guard_sets(const Ctx *c) {
+    GuardSets gs{};
+    GuardCols &gc = gs.set[0];
+    guard_columns(c, gc.col, gc.n);
+    const std::vector<int32_t> &hrk = c->qtable ? c->h_qranks : c->h_ranks;
+    for (int32_t beta = 0; beta < gc.n; ++beta) {
+        GuardCols &g2 = gs.set[1 + beta];
+        const int32_t lo_col = beta == 0 ? 0 : gc.col[beta - 1] + 1, hi_col = gc.col[beta];
+        if (hi_col <= lo_col) continue;
+        const double ra = (double)hrk[lo_col] + 1.0, rb = (double)hrk[hi_col] + 1.0;
+        int32_t last = lo_col - 1;
+        for (int m = 1; m <= 6; ++m) {
+            const double target = ra * std::pow(rb / ra, m / 7.0) - 1.0;
+            int32_t j = last + 1;
+            while (j + 1 < hi_col && (double)hrk[j + 1] <= target) ++j;
+            if (j < hi_col && j > last && (double)hrk[j] <= target) {
+                g2.col[g2.n++] = j;
+                last = j;
+            }
+        }
+        g2.col[g2.n++] = hi_col;
+    }
+    return gs;
+}
+
 int refresh_guard_values(Ctx *c, int64_t u0, int64_t n) {
     if (!c->gvals || n <= 0) return RKR_OK;
-    GuardCols gc{};
-    guard_columns(c, gc.col, gc.n);
+    const GuardSets gs = guard_sets(c);
     const float *T = c->qtable ? c->qtable : c->table;
     const int64_t S = c->qtable ? c->n_qs : c->n_sample;
-    RKR_HIP(c, launch_gather_guards(c->stream, T, S, gc, u0, n, c->gvals));
+    RKR_HIP(c, launch_gather_guards(c->stream, T, S, gs, u0, n, c->gvals));
     return RKR_OK;
 }
 
@@ -626,7 +686,7 @@ int build_table(Ctx *c, const int32_t *ranks, int32_t S, double *build_ms, bool 
     int rc = dev_alloc_t(c, &c->ranks, (size_t)S, "ranks");
     if (rc) return rc;
     rc = dev_alloc_t(c, &c->table, (size_t)std::max(c->u_cap, c->n_user) * S, "table");
-    if (!rc) rc = dev_alloc_t(c, &c->gvals, (size_t)std::max(c->u_cap, c->n_user) * 8, "guard values");
+    if (!rc) rc = dev_alloc_t(c, &c->gvals, (size_t)std::max(c->u_cap, c->n_user) * kGuardRow, "guard values");
     if (rc) { free_table(c); return rc; }
     c->n_sample = S;
     c->h_ranks.assign(ranks, ranks + S);
@@ -710,7 +770,7 @@ int rkr_set_table(rkr_ctx *c, const int32_t *ranks, int32_t S, const float *tabl
     int rc = dev_alloc_t(c, &c->ranks, (size_t)S, "ranks");
     if (rc) return rc;
     rc = dev_alloc_t(c, &c->table, (size_t)std::max(c->u_cap, c->n_user) * S, "table");
-    if (!rc) rc = dev_alloc_t(c, &c->gvals, (size_t)std::max(c->u_cap, c->n_user) * 8, "guard values");
+    if (!rc) rc = dev_alloc_t(c, &c->gvals, (size_t)std::max(c->u_cap, c->n_user) * kGuardRow, "guard values");
     if (rc) { free_table(c); return rc; }
     c->n_sample = S;
     c->h_ranks.assign(ranks, ranks + S);
@@ -1192,18 +1252,33 @@ int rkr_query(rkr_ctx *c, const float *queries, int32_t n_query, const int32_t *
     // there only queries equal in vector and k share a run.
     const size_t row = sizeof(float) * (size_t)c->dim;
     const bool by_k = c->output == RKR_OUTPUT_REFERENCE;
-    std::vector<std::string> keys((size_t)n_query);
-    std::unordered_map<std::string_view, int32_t> seen;
-    seen.reserve((size_t)n_query * 2);
+    // open addressing over a hash of the row's bytes (and k), byte compare on
+    // a hash match: no per-query key copies
+    size_t cap = 16;
+    while (cap < 2 * (size_t)n_query) cap <<= 1;
+    std::vector<int32_t> slot(cap, -1);  // index into reps
     std::vector<int32_t> rep_of((size_t)n_query), reps;
+    std::vector<uint64_t> rep_hash;
     for (int32_t i = 0; i < n_query; ++i) {
-        std::string &kb = keys[i];
-        kb.assign(reinterpret_cast<const char *>(queries + (size_t)i * c->dim), row);
-        if (by_k) kb.append(reinterpret_cast<const char *>(&topk[i]), sizeof(int32_t));
-        const std::string_view key(kb);
-        auto it = seen.emplace(key, (int32_t)reps.size());
-        if (it.second) reps.push_back(i);
-        rep_of[i] = it.first->second;
+        const float *qi = queries + (size_t)i * c->dim;
+        const uint64_t h = row_hash(qi, row, by_k ? (uint32_t)topk[i] : 0u);
+        size_t pos = (size_t)h & (cap - 1);
+        int32_t found = -1;
+        for (; slot[pos] >= 0; pos = (pos + 1) & (cap - 1)) {
+            const int32_t r = slot[pos];
+            if (rep_hash[r] == h && std::memcmp(queries + (size_t)reps[r] * c->dim, qi, row) == 0 &&
+                (!by_k || topk[reps[r]] == topk[i])) {
+                found = r;
+                break;
+            }
+        }
+        if (found < 0) {
+            found = (int32_t)reps.size();
+            slot[pos] = found;
+            reps.push_back(i);
+            rep_hash.push_back(h);
+        }
+        rep_of[i] = found;
     }
     const int32_t n_rep = (int32_t)reps.size();
     if (n_rep == n_query)
@@ -1282,9 +1357,9 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
     // the score rule, eps 0); reference output keeps the sampled table
     const bool use_q = mode == RKR_OUTPUT_EXACT && dv.qtable;
     const float *T = use_q ? dv.qtable : dv.table;
-    const int32_t *R = use_q ? dv.qranks : c->ranks;
-    const int32_t S = use_q ? dv.n_qs : c->n_sample;
-    const float eps = mode == RKR_OUTPUT_EXACT && !use_q ? c->table_eps : 0.0f;
+    const int32_t *R = use_q ? dv.qranks : dv.ranks;
+    const int32_t S = use_q ? dv.n_qs : dv.n_sample;
+    const float eps = mode == RKR_OUTPUT_EXACT && !use_q ? dv.table_eps : 0.0f;
     // pending item inserts/deletes: bounds become per-user values (k_update.hip)
     const bool upd = dv.n_ins > 0 || dv.n_del > 0;
     // the QSRP screen (RKR_OPT_QSRP, built by rkr_qsrp_build) replaces the bound
@@ -1323,8 +1398,10 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
         for (int j = 0; j < dv.n_guard; ++j) gc.col[j] = dv.gcol[j];
         gc.n = dv.n_guard;
     }
-    // ranks of the searched table's columns (guard placement)
-    const std::vector<int32_t> &hrk = use_q ? c->h_qranks : c->h_ranks;
+    // ranks of the searched table's columns (guard placement; the coarse
+    // bounds and the update path run in exact output only, which searches the
+    // query table when there is one)
+    const std::vector<int32_t> &hrk = *dv.hrk;
     // coarse lists longer than refine_min get a second, finer pass; a list
     // longer than coarse_cap takes the full path instead
     const int64_t coarse_cap = std::max<int64_t>(4096, U / 4);
@@ -1479,29 +1556,18 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             std::vector<std::pair<int32_t, int32_t>> groups;  // (set, first index in hqmap)
             // (not with pending updates: the rank shift bounds are kept for
             // the first pass's guards only)
+            std::vector<int32_t> set_ids;  // guard-row set of each second-pass set
             for (int32_t beta = 0; beta < gc.n && !upd; ++beta) {
-                const int32_t lo_col = beta == 0 ? 0 : gc.col[beta - 1] + 1, hi_col = gc.col[beta];
-                if (hi_col <= lo_col) continue;
+                const GuardCols &g2 = dv.gsets.set[1 + beta];
+                if (g2.n == 0) continue;
                 const size_t first = hqmap.size();
                 for (int32_t i = 0; i < nq; ++i)
                     if (hcq[(size_t)i * 4] == beta && hcq[(size_t)i * 4 + 1] > refine_min) hqmap.push_back(i);
                 if (hqmap.size() == first) continue;
-                GuardCols g2{};
-                const double ra = (double)hrk[lo_col] + 1.0, rb = (double)hrk[hi_col] + 1.0;
-                int32_t last = lo_col - 1;
-                for (int m = 1; m <= 6; ++m) {
-                    const double target = ra * std::pow(rb / ra, m / 7.0) - 1.0;
-                    int32_t j = last + 1;
-                    while (j + 1 < hi_col && (double)hrk[j + 1] <= target) ++j;
-                    if (j < hi_col && j > last && (double)hrk[j] <= target) {
-                        g2.col[g2.n++] = j;
-                        last = j;
-                    }
-                }
-                g2.col[g2.n++] = hi_col;
                 for (size_t t = first; t < hqmap.size(); ++t) hgset[hqmap[t]] = (int32_t)sets.size();
                 groups.push_back({(int32_t)sets.size(), (int32_t)first});
                 sets.push_back(g2);
+                set_ids.push_back(1 + beta);
             }
             // With a second pass the lists stay on the device: the batch goes
             // on without reading the new counts.  A second-pass list is a
@@ -1541,6 +1607,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                 L2.sets = gsets + 1;
                 L2.qset = qset;
                 L2.n_sets = (int32_t)sets.size() - 1;
+                for (int32_t t = 0; t < L2.n_sets; ++t) L2.set_id[t] = set_ids[t];
                 RKR_HIP(c, launch_ip_coarse(sl.stream, L2));
                 for (size_t gi = 0; gi < groups.size(); ++gi) {
                     const int32_t first = groups[gi].second;
@@ -1614,7 +1681,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             // floor covers gradual underflow in the d products of a score
             const float absf = std::nextafter((float)((double)eps + (double)d * std::ldexp(1.0, -140)), INFINITY);
             RKR_HIP(c, launch_qsrp_bounds(sl.stream, qip, U, nq, dv.qsrp_par, qn, dv.qsrp_gcd, dv.qsrp_crot, absf,
-                                          c->n_sample, use_q ? c->qcol : nullptr, S, bk));
+                                          dv.n_sample, use_q ? dv.qcol : nullptr, S, bk));
             RKR_HIP(c, launch_hist_kth(sl.stream, bk, U, S, R, I, nq, mode, 0, cu, ghist, qi));
             RKR_HIP(c, launch_qsrp_exact(sl.stream, dv.users, q_in, ld, d, T, S, eps, U, nq, qi, qip, bk));
             RKR_HIP(c, hipEventRecord(sl.ev[3], sl.stream));

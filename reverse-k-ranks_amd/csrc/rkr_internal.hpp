@@ -9,6 +9,7 @@
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <memory>
 #include <vector>
 
 #include "../../include/rkr.h"
@@ -212,6 +213,21 @@ inline float screen_abs_floor(int32_t kp, double max_abs_user, double max_abs_it
     return std::nextafter((float)absf, INFINITY);
 }
 
+struct GuardCols {
+    int32_t col[8];
+    int32_t n;
+};
+// Guard sets of the coarse bounds: set 0 holds the first pass's columns, set
+// 1 + beta the second pass's columns inside guard bin beta (empty when the bin
+// has no column of its own).  Each user keeps the values of all of them in one
+// row of kGuardRow floats, set s at floats 8 s .. 8 s + 7 (-inf past its n).
+constexpr int kGuardSets = 8;
+constexpr int kGuardRow = 8 * kGuardSets;
+struct GuardSets {
+    GuardCols set[kGuardSets];
+};
+
+
 // Everything a query batch reads from the data and table state, captured
 // when the batch starts.  Outside a snapshot-mode server it is the context's
 // current state; in one, the last committed update's state, so updates can
@@ -231,7 +247,17 @@ struct DataSnap {
     int32_t n_qs = 0;
     int32_t n_guard = 0;             // guard columns of the coarse bounds
     int32_t gcol[8] = {};
-    const float *gvals = nullptr;    // their values per user (8 floats), or null
+    const float *gvals = nullptr;    // guard values per user (kGuardRow floats, all sets), or null
+    // the sampled table's ranks (device), count and eps, the query table's
+    // column of each sampled column, and the searched table's ranks on the
+    // host (the guard placement): captured with the table, so a batch never
+    // mixes a table with the ranks of one rebuilt after it started
+    const int32_t *ranks = nullptr;
+    int32_t n_sample = 0;
+    float table_eps = 0.0f;
+    const int32_t *qcol = nullptr;
+    std::shared_ptr<const std::vector<int32_t>> hrk;
+    GuardSets gsets{};               // the coarse guard sets of that table
     // pending inserts: the most inserted items any user has above its value
     // at guard column j (a bound on the rank shift at that guard)
     int32_t ins_gmax[8] = {};
@@ -241,15 +267,16 @@ struct DataSnap {
     float qsrp_gcd = 0.0f, qsrp_crot = 0.0f;
 };
 DataSnap snap_of(const Ctx *c);
+GuardSets guard_sets(const Ctx *c);
 // Guard columns of the coarse bounds for the context's searched table.
 void guard_columns(const Ctx *c, int32_t *col, int32_t &n);
-// Fills Ctx::gvals rows [u0, u0 + n) from the searched table.
 // The table over the current items at sampled ranks S (build_table), and the
 // same at uniform ranks of the current n_sample (rebuild_table): the pending
 // item updates fold into the new table; keep_update_buffers leaves their
 // buffers allocated for the updates that follow.
 int build_table(Ctx *c, const int32_t *ranks, int32_t S, double *build_ms, bool keep_update_buffers);
 int rebuild_table(Ctx *c);
+// Fills Ctx::gvals rows [u0, u0 + n) from the searched table.
 int refresh_guard_values(Ctx *c, int64_t u0, int64_t n);
 // Recomputes Ctx::ins_gmax after inserted items or users changed.
 int refresh_ins_guard_max(Ctx *c);
@@ -335,16 +362,13 @@ hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, 
 hipError_t launch_user_norms(hipStream_t s, const float *users, int64_t n_user, int32_t ld, int32_t K, float *norm);
 hipError_t launch_refine_screen(hipStream_t s, const ScreenLaunch &L);
 // Guard columns of the coarse bounds: n (1..7) ascending table columns.
-struct GuardCols {
-    int32_t col[8];
-    int32_t n;
-};
+
 // Coarse buckets of a batch (screened <u,q> against the guard columns of
 // every user's table row): cb[row(b) * ldc + u] = c_lo | c_hi << 4, with
 // row(b) = qmap[b] (or b when qmap is null), which also picks the query rows.
 struct CoarseLaunch {
     const float *users, *user_norm, *table;
-    const float *gvals = nullptr;       // n_user x 8 guard values of gc (first pass), or null: read the table
+    const float *gvals = nullptr;       // n_user x kGuardRow guard values, or null: read the table
     const uint16_t *user_hi, *user_lo;  // BF16 split of the user rows (n_user x Kp)
     int64_t n_user;
     int32_t S;
@@ -358,10 +382,12 @@ struct CoarseLaunch {
     int64_t ldc;
     int64_t spare_row;      // a row no query owns (written by padding lanes)
     int64_t grid_cap;
-    // second pass: query b uses sets[qset[b]] (<= 7 sets) instead of gc
+    // second pass: query b uses sets[qset[b]] (<= 7 sets) instead of gc;
+    // set_id[i] is the guard-row set (1 + beta) of launch set i
     const GuardCols *sets = nullptr;
     const int32_t *qset = nullptr;
     int32_t n_sets = 0;
+    int32_t set_id[7] = {};
 };
 hipError_t launch_ip_coarse(hipStream_t s, const CoarseLaunch &L);
 hipError_t launch_recheck(hipStream_t s, const float *users, const float *items, int32_t ld, int32_t K,
@@ -525,8 +551,9 @@ hipError_t launch_qsrp_exact(hipStream_t s, const float *users, const float *q_i
                              const float *T, int32_t S, float eps, int64_t n_user, int32_t nq, QueryInfo *qi,
                              float *qip, uint32_t *bk);
 
-// gv[u * 8 + j] = T[u][gc.col[j]] (-inf for j >= gc.n), rows [u0, u0 + n_rows).
-hipError_t launch_gather_guards(hipStream_t s, const float *T, int64_t S, const GuardCols &gc, int64_t u0,
+// gv[u * kGuardRow + 8 s + j] = T[u][gs.set[s].col[j]] (-inf for j >= its n),
+// rows [u0, u0 + n_rows).
+hipError_t launch_gather_guards(hipStream_t s, const float *T, int64_t S, const GuardSets &gs, int64_t u0,
                                 int64_t n_rows, float *gv);
 
 // The sort scratch of rkr_sorted_scores_noalloc: looks up its device address

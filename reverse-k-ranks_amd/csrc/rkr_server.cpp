@@ -110,11 +110,28 @@ struct rkr_server {
     // Applies one update; when the pending inserted and deleted items then reach
     // the rebuild threshold, rebuilds the table over the current items (uniform
     // ranks, same n_sample), which folds the side arrays into a fresh base, so an
-    // endless insert stream never meets the side arrays' cap.  In snapshot mode
-    // this runs on the update thread and its CUs: batches keep reading the
-    // previous state until the commit, and the replaced buffers are retired.
+    // endless insert stream never meets the side arrays' cap.  An item update
+    // that would outgrow the side arrays' reserved length rebuilds first, so
+    // the arrays never grow while serving (inserts that queued up behind a long
+    // rebuild then fit).  In snapshot mode this runs on the update thread and
+    // its CUs: batches keep reading the previous state until the commit, and
+    // the replaced buffers are retired.
+    int rebuild() {
+        const int rc = rebuild_table(c);
+        if (!rc) {
+            std::lock_guard<std::mutex> g(mu);
+            ++n_rebuilds;
+        }
+        return rc;
+    }
+
     int apply_update(Request *r) {
         int rc;
+        if (rebuild_at >= 0 && c->table &&
+            ((r->kind == Q_INSERT_ITEMS && c->n_ins > 0 && (int64_t)c->n_ins + r->n > c->ld_ins) ||
+             (r->kind == Q_DELETE_ITEMS && c->n_del > 0 && (int64_t)c->n_del + r->n > c->ld_del)) &&
+            (rc = rebuild()))
+            return rc;
         switch (r->kind) {
             case Q_INSERT_USERS: rc = rkr_insert_users(static_cast<rkr_ctx *>(c), r->data.data(), r->n); break;
             case Q_INSERT_ITEMS: rc = rkr_insert_items(static_cast<rkr_ctx *>(c), r->data.data(), r->n); break;
@@ -123,12 +140,7 @@ struct rkr_server {
             default: return RKR_ERR_INVALID;
         }
         if (rc || rebuild_at < 0 || (int64_t)c->n_ins + c->n_del < rebuild_at) return rc;
-        rc = rebuild_table(c);
-        if (!rc) {
-            std::lock_guard<std::mutex> g(mu);
-            ++n_rebuilds;
-        }
-        return rc;
+        return rebuild();
     }
 
     void run_batch(Slot &sl, const DataSnap &snap, std::vector<Request *> &all) {

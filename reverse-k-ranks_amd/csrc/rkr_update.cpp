@@ -80,7 +80,7 @@ int reserve_users(Ctx *c, int64_t need) {
     float *users = nullptr, *panels = nullptr, *unorm = nullptr, *table = nullptr, *qtable = nullptr, *si = nullptr,
           *sd = nullptr, *gv = nullptr;
     int rc = RKR_OK;
-    if ((c->gvals && (rc = dev_alloc_t(c, &gv, (size_t)cap * 8, "guard values"))) ||
+    if ((c->gvals && (rc = dev_alloc_t(c, &gv, (size_t)cap * kGuardRow, "guard values"))) ||
         (rc = dev_alloc_t(c, &users, (size_t)cap * ld, "users")) ||
         (rc = dev_alloc_t(c, &panels, (size_t)((cap + 63) / 64) * 64 * ld, "user panels")) ||
         (c->kp && (rc = dev_alloc_t(c, &unorm, (size_t)cap, "user norms"))) ||
@@ -110,7 +110,7 @@ int reserve_users(Ctx *c, int64_t need) {
         (c->kp && (rc = copy_rows(c, unorm, 1, c->user_norm, 1, U, 1))) ||
         (rc = copy_rows(c, table, S, c->table, S, U, S)) ||
         (QS && (rc = copy_rows(c, qtable, QS, c->qtable, QS, U, QS))) ||
-        (gv && (rc = copy_rows(c, gv, 8, c->gvals, 8, U, 8))) ||
+        (gv && (rc = copy_rows(c, gv, kGuardRow, c->gvals, kGuardRow, U, kGuardRow))) ||
         (si && (rc = copy_rows(c, si, c->ld_ins, c->side_ins, c->ld_ins, U, c->n_ins))) ||
         (sd && (rc = copy_rows(c, sd, c->ld_del, c->side_del, c->ld_del, U, c->n_del))))
         return rc;
@@ -561,7 +561,7 @@ int rkr_delete_users(rkr_ctx *c, const int64_t *ids, int64_t n) {
     rc = keep_user_rows(c, &c->users, ld, ld, d_keep, nk, "users");
     if (!rc) rc = keep_user_rows(c, &c->table, c->n_sample, c->n_sample, d_keep, nk, "table");
     if (!rc) rc = keep_user_rows(c, &c->qtable, c->n_qs, c->n_qs, d_keep, nk, "query table");
-    if (!rc) rc = keep_user_rows(c, &c->gvals, 8, 8, d_keep, nk, "guard values");
+    if (!rc) rc = keep_user_rows(c, &c->gvals, kGuardRow, kGuardRow, d_keep, nk, "guard values");
     if (!rc) rc = keep_user_rows(c, &c->side_ins, c->n_ins, c->ld_ins, d_keep, nk, "side array");
     if (!rc) rc = keep_user_rows(c, &c->side_del, c->n_del, c->ld_del, d_keep, nk, "side array");
     dev_free(c, d_keep, 0);
