@@ -110,11 +110,20 @@ struct ScreenParams {
 template <int KS, int RB>
 __global__ void __launch_bounds__(NT, RB == 2 ? 2 : 1) k_refine_screen(ScreenParams P) {
     constexpr int RW = 4 * 16 * RB;  // rows per workgroup
-    // items per LDS stage: 32 for long rows keeps the prefetch registers
-    // within the 256-VGPR budget of two waves per SIMD
-    constexpr int IST = KS >= 5 ? 32 : ITEMS_STAGE;
-    __shared__ uint4 sB[2][IST * (KS * 32 + KPAD_LDS) / 8];  // [hi|lo][item][k] as 16-byte chunks
-    __shared__ float sN[IST];                                // item norm bounds of the stage
+    // items per LDS stage: 32 keeps the two prefetch register sets at 8 KS
+    // VGPRs
+    constexpr int IST = 32;
+    // two stage buffers: stage m of a unit is computed from buffer m & 1
+    // while stage m + 1 is written to the other and stage m + 3 is loaded
+    // into registers, with one barrier per stage
+    __shared__ uint4 sB[2][2][IST * (KS * 32 + KPAD_LDS) / 8];  // [buf][hi|lo][item][k] as 16-byte chunks
+    __shared__ float sN[2][IST];                                // item norm bounds of the stage
+    // undecided (row, item) pairs of the unit, appended with LDS atomics and
+    // flushed to the recheck list when the unit ends (a global atomic that
+    // returns a slot would make the stage loop wait for its prefetch loads)
+    constexpr int QCAP = 256;
+    __shared__ int4 sQ[QCAP];
+    __shared__ int32_t sQn, sQbase;
     constexpr int ROWQ = (KS * 32 + KPAD_LDS) / 8;                   // uint4 per LDS item row
     constexpr int KQ = KS * 4;                                       // uint4 per global item row (Kp = 32 KS)
     constexpr int PER_T = 2 * IST * KQ / NT;                 // uint4 each thread stages per stage
@@ -132,37 +141,46 @@ __global__ void __launch_bounds__(NT, RB == 2 ? 2 : 1) k_refine_screen(ScreenPar
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
-    // Stage st+1 is loaded into registers while stage st is computed from
-    // LDS, then written to LDS between two barriers.
-    uint4 pre[PER_T];
-    float pre_n = -INFINITY;
-    auto load_stage = [&](int64_t st) {
-        const int64_t p0 = st * IST;
-#pragma unroll
-        for (int j = 0; j < PER_T; ++j) {
-            const int c = threadIdx.x + j * NT;
-            const int half = c / (IST * KQ);
-            const int rem = c % (IST * KQ);
-            const int64_t p = p0 + rem / KQ;
-            pre[j] = p < P.n_item ? reinterpret_cast<const uint4 *>(half ? P.item_lo : P.item_hi)[p * KQ + rem % KQ]
-                                  : make_uint4(0, 0, 0, 0);
-        }
-        if (threadIdx.x < IST) {
-            const int64_t p = p0 + threadIdx.x;
-            pre_n = p < P.n_item ? P.item_norm[p] : -INFINITY;
-        }
-    };
-    auto store_stage = [&]() {
-#pragma unroll
-        for (int j = 0; j < PER_T; ++j) {
-            const int c = threadIdx.x + j * NT;
-            const int half = c / (IST * KQ);
-            const int rem = c % (IST * KQ);
-            sB[half][(rem / KQ) * ROWQ + rem % KQ] = pre[j];
-        }
-        if (threadIdx.x < IST) sN[threadIdx.x] = pre_n;
-    };
+    // Register sets for the stages loaded ahead: with two (where the A
+    // fragments leave room), stage m of a unit lives in set A (m even) or B
+    // (m odd) from its load, three stages ahead, until it is written to LDS;
+    // with one, two stages ahead.  Every lane issues the same
+    // loads in every call, with no branch around them and no use of their
+    // values until the set is written to LDS, so the memory counter then
+    // waits for exactly the older set, never for the loads just issued.  Rows
+    // past n_item load the last item; compute() gives them a norm bound of
+    // -inf, so they never count.
+    constexpr int NSET = (KS <= 4 || (RB == 4 && KS <= 5)) ? 2 : 1;
+    // native vectors, not HIP's uint4 struct: a struct copy is a memcpy that
+    // keeps the sets in scratch memory
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 preA[PER_T], preB[NSET == 2 ? PER_T : 1];
+    float preA_n = -INFINITY, preB_n = -INFINITY;
+#define RKR_LOAD_STAGE(st, dst, dst_n)                                                                        \
+    do {                                                                                                    \
+        const int64_t p0_ = (st) * IST;                                                                     \
+        _Pragma("unroll") for (int j = 0; j < PER_T; ++j) {                                                 \
+            const int c = threadIdx.x + j * NT;                                                             \
+            const int half = c / (IST * KQ);                                                                \
+            const int rem = c % (IST * KQ);                                                                 \
+            const int64_t p = min(p0_ + rem / KQ, P.n_item - 1);                                            \
+            dst[j] = reinterpret_cast<const u32x4 *>(half ? P.item_lo : P.item_hi)[p * KQ + rem % KQ];     \
+        }                                                                                                   \
+        dst_n = P.item_norm[min(p0_ + (int64_t)(threadIdx.x % IST), P.n_item - 1)];                         \
+    } while (0)
+#define RKR_STORE_STAGE(src, src_n, buf)                                                                      \
+    do {                                                                                                    \
+        _Pragma("unroll") for (int j = 0; j < PER_T; ++j) {                                                 \
+            const int c = threadIdx.x + j * NT;                                                             \
+            const int half = c / (IST * KQ);                                                                \
+            const int rem = c % (IST * KQ);                                                                 \
+            *reinterpret_cast<u32x4 *>(&sB[buf][half][(rem / KQ) * ROWQ + rem % KQ]) = src[j];              \
+        }                                                                                                   \
+        if (threadIdx.x < IST) sN[buf][threadIdx.x] = src_n;                                                \
+    } while (0)
 
+    if (threadIdx.x == 0) sQn = 0;
+    __syncthreads();
     for (int64_t unit = blockIdx.x; unit < n_units; unit += gridDim.x) {
         const int64_t chunk = unit / n_rb;
         const int64_t rb = unit % n_rb;
@@ -207,15 +225,20 @@ __global__ void __launch_bounds__(NT, RB == 2 ? 2 : 1) k_refine_screen(ScreenPar
 #pragma unroll
             for (int i = 0; i < 4; ++i) cnt[b][i] = 0;
 
+        // Wait here for the unit's A fragments and epilogue constants: the
+        // stage loop's prefetch loads are issued after them, and a use of an
+        // older load inside the loop would otherwise wait for every load in
+        // flight (the memory counter is in order), the prefetch included.
+        __builtin_amdgcn_s_waitcnt(0);
         const int64_t st0 = chunk * st_per_chunk;
-        const int64_t st1 = min(n_stage_total, st0 + st_per_chunk);
-        if (st0 < st1) load_stage(st0);
-        for (int64_t st = st0; st < st1; ++st) {
+        const int64_t n_st = max((int64_t)0, min(n_stage_total, st0 + st_per_chunk) - st0);
+        // Stage m (of this unit) is computed from LDS buffer m & 1.  Before it,
+        // stage m + 1 goes from register set (m + 1) & 1 to the other buffer
+        // (its last readers, stage m - 1, passed the barrier that ended it) and
+        // stage m + 3 starts loading into that register set; one barrier per
+        // stage then orders the buffers.
+        auto compute = [&](int buf, int64_t st) {
             const int64_t p0 = st * IST;
-            __syncthreads();  // every wave is done with the previous stage
-            store_stage();
-            __syncthreads();
-            if (st + 1 < st1) load_stage(st + 1);
 #pragma unroll 1
             for (int g = 0; g < IST / 16; ++g) {
                 const int it = g * 16 + lr;
@@ -229,8 +252,8 @@ __global__ void __launch_bounds__(NT, RB == 2 ? 2 : 1) k_refine_screen(ScreenPar
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks) {
                     Frag bh, bl;
-                    bh.q = sB[0][it * ROWQ + ks * 4 + lq];
-                    bl.q = sB[1][it * ROWQ + ks * 4 + lq];
+                    bh.q = sB[buf][0][it * ROWQ + ks * 4 + lq];
+                    bl.q = sB[buf][1][it * ROWQ + ks * 4 + lq];
 #pragma unroll
                     for (int b = 0; b < RB; ++b) {
                         accs[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[b][ks].v, bh.v, accs[b], 0, 0, 0);
@@ -240,7 +263,7 @@ __global__ void __launch_bounds__(NT, RB == 2 ? 2 : 1) k_refine_screen(ScreenPar
                 }
                 // decide the 8 outputs without branches; only undecided ones
                 // (rare) take the recheck path
-                const float np = sN[it];
+                const float np = p < P.n_item ? sN[buf][it] : -INFINITY;
                 uint32_t undecided = 0;
 #pragma unroll
                 for (int b = 0; b < RB; ++b)
@@ -259,11 +282,72 @@ __global__ void __launch_bounds__(NT, RB == 2 ? 2 : 1) k_refine_screen(ScreenPar
                         for (int i = 0; i < 4; ++i)
                             if (undecided & (1u << (b * 4 + i))) {
                                 const int64_t r = row_base + b * 16 + 4 * lq + i;
-                                const int32_t slot = atomicAdd(P.recheck_n, 1);
-                                if (slot < P.recheck_cap)
-                                    P.recheck[slot] = make_int4((int32_t)r, (int32_t)p, __float_as_int(accs[b][i]), 0);
+                                const int4 e = make_int4((int32_t)r, (int32_t)p, __float_as_int(accs[b][i]), 0);
+                                const int32_t q = atomicAdd(&sQn, 1);
+                                if (q < QCAP) {
+                                    sQ[q] = e;
+                                } else {
+                                    // the queue is full: straight to the list
+                                    const int32_t slot = atomicAdd(P.recheck_n, 1);
+                                    if (slot < P.recheck_cap) P.recheck[slot] = e;
+                                }
                             }
                 }
+            }
+        };
+        // stages past the unit's last load its last again (never computed),
+        // so the loads stay unconditional
+        const int64_t st_last = st0 + n_st - 1;
+        auto ahead = [&](int64_t m) { return min(st0 + m, st_last); };
+        if constexpr (NSET == 2) {
+            if (n_st > 0) {
+                RKR_LOAD_STAGE(ahead(0), preA, preA_n);
+                RKR_LOAD_STAGE(ahead(1), preB, preB_n);
+                RKR_STORE_STAGE(preA, preA_n, 0);  // the previous unit ended with a barrier
+                RKR_LOAD_STAGE(ahead(2), preA, preA_n);
+                __syncthreads();
+            }
+            for (int64_t m = 0; m < n_st; m += 2) {
+                // even stage: buffer 0; stage m + 1 is in set B
+                RKR_STORE_STAGE(preB, preB_n, 1);
+                RKR_LOAD_STAGE(ahead(m + 3), preB, preB_n);
+                compute(0, st0 + m);
+                __syncthreads();
+                if (m + 1 >= n_st) break;
+                // odd stage: buffer 1; stage m + 2 is in set A
+                RKR_STORE_STAGE(preA, preA_n, 0);
+                RKR_LOAD_STAGE(ahead(m + 4), preA, preA_n);
+                compute(1, st0 + m + 1);
+                __syncthreads();
+            }
+        } else {
+            if (n_st > 0) {
+                RKR_LOAD_STAGE(ahead(0), preA, preA_n);
+                RKR_STORE_STAGE(preA, preA_n, 0);  // the previous unit ended with a barrier
+                RKR_LOAD_STAGE(ahead(1), preA, preA_n);
+                __syncthreads();
+            }
+            for (int64_t m = 0; m < n_st; ++m) {
+                const int buf = (int)(m & 1);
+                RKR_STORE_STAGE(preA, preA_n, buf ^ 1);
+                RKR_LOAD_STAGE(ahead(m + 2), preA, preA_n);
+                compute(buf, st0 + m);
+                __syncthreads();
+            }
+        }
+        // the unit's queued rechecks go to the list (the stage loop ended
+        // with a barrier, so the queue is complete)
+        {
+            const int32_t nqd = min(sQn, QCAP);
+            if (nqd > 0) {
+                if (threadIdx.x == 0) sQbase = atomicAdd(P.recheck_n, nqd);
+                __syncthreads();
+                const int32_t base = sQbase;
+                for (int t = threadIdx.x; t < nqd; t += NT)
+                    if ((int64_t)base + t < P.recheck_cap) P.recheck[base + t] = sQ[t];
+                __syncthreads();  // every thread has read sQbase and the queue
+                if (threadIdx.x == 0) sQn = 0;
+                __syncthreads();
             }
         }
         // reduce each row's count over the 16 lanes (columns) that share it
@@ -279,6 +363,9 @@ __global__ void __launch_bounds__(NT, RB == 2 ? 2 : 1) k_refine_screen(ScreenPar
             }
     }
 }
+
+#undef RKR_LOAD_STAGE
+#undef RKR_STORE_STAGE
 
 // Exact decision of the rechecked (row, item) pairs with the sequential FP32
 // rule; also records the largest |s^ - s_fp64| seen (as float bits).

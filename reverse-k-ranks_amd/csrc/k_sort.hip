@@ -5,7 +5,16 @@
 // (src/include/score_computation/ComputeScoreTableBatch.hpp:83-89).  Only the values are kept, so the
 // order of equal values is irrelevant and the sort need not be stable.
 //
-// One 1024-thread workgroup per row:
+// One 1024-thread workgroup per row.  Rows of up to NMAX values are sorted in LDS (bucket path):
+//   1. the row is loaded into registers (coalesced), with its min and max;
+//   2. a histogram over NBK linear value bins, descending (FP32 arithmetic: fl(max - x) and
+//      fl(t * scale) are monotone in x, so bin order is value order), and its exclusive scan;
+//   3. every value is scattered into its bin's segment of an LDS buffer (any order inside a bin);
+//   4. each value's final position is its bin's start plus the values of its bin above it (and the
+//      equal ones before it): threads walk the buffer in order, so the lanes of a wavefront read the
+//      same few bins and write within a few hundred bytes of the output row.
+// A row longer than NMAX, or one with a bin over BIN_MAX values (heavy ties, extreme skew), takes
+// the merge path instead:
 //   1. tiles of TILE values are sorted in LDS by a bitonic network (a partial last tile is padded with
 //      -inf, which sorts to its end and is not written back);
 //   2. sorted runs are merged pairwise -- run length TILE, 2*TILE, ... -- between the output row and
@@ -18,8 +27,32 @@ namespace rkr {
 namespace {
 
 constexpr int NT = 1024;                 // threads per row
+constexpr int NW = NT / 64;              // wavefronts per row
 constexpr int TILE = 8192;               // values sorted in LDS at a time (32 KB)
 constexpr int PER = TILE / 2 / NT;       // compare-exchange pairs per thread and stage
+constexpr int NBK = 8192;                // value bins of the bucket path
+constexpr int NMAX = 28672;              // longest row of the bucket path (112 KB of LDS)
+constexpr int EPT = NMAX / NT;           // values per thread
+constexpr uint32_t BIN_MAX = 192;        // largest bin the bucket path ranks by counting
+
+struct SortSmem {
+    union {
+        float tile[TILE];                // merge path
+        struct {
+            float buf[NMAX];             // the row, bin by bin, then in order
+            uint32_t cnt[NBK];           // bin counts, then bin ends (32 KB)
+            float red_f[2 * NW];
+            uint32_t red_u[NW + 1];
+        } b;
+    } u;
+};
+
+__device__ __forceinline__ int sort_bin(float x, float hi, float scale) {
+    const float t = (hi - x) * scale;
+    if (!(t >= 0.0f)) return 0;          // NaN from 0 * inf at x == hi
+    if (!(t < (float)NBK)) return NBK - 1;
+    return (int)t;
+}
 
 // Sorts s[0..TILE) in descending order.  Stage (k, j) compares i with i + j for
 // the i whose bit j is clear; blocks of k elements alternate direction so that
@@ -55,17 +88,131 @@ __device__ inline int merge_split(const float *A, int na, const float *B, int nb
     return lo;
 }
 
-// src and out may be the same rows: every value of src is read before the
-// output row is written where it lies.
-__global__ void __launch_bounds__(NT) k_sort_rows(const float *src, int64_t ld_src, int32_t n, float *out,
-                                                  float *tmp, int64_t ld) {
-    __shared__ float s[TILE];
-    const int64_t row = blockIdx.x;
-    const float *x = src + row * ld_src;
+// Bucket path for one row of n <= NMAX values; returns false (having written
+// nothing) when a bin holds more than BIN_MAX values.  x and y may be the same
+// row: the row is in registers before anything is written.
+__device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, int32_t n, float *y) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // register slots e < ne hold values (the test is uniform, so the slots
+    // past a short row are skipped, not run with no lanes active)
+    const int ne = (n + NT - 1) / NT;
+    float v[EPT];
+    float mn = INFINITY, mx = -INFINITY;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        v[e] = 0.0f;
+        if (e < ne) {
+            const int i = (int)threadIdx.x + e * NT;
+            if (i < n) {
+                v[e] = x[i];
+                mn = fminf(mn, v[e]);
+                mx = fmaxf(mx, v[e]);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = fminf(mn, __shfl_xor(mn, o, 64));
+        mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    }
+    if (lane == 0) {
+        sm.u.b.red_f[w] = mn;
+        sm.u.b.red_f[NW + w] = mx;
+    }
+    for (int b = threadIdx.x; b < NBK; b += NT) sm.u.b.cnt[b] = 0;
+    __syncthreads();
+    mn = sm.u.b.red_f[0];
+    mx = sm.u.b.red_f[NW];
+    for (int i = 1; i < NW; ++i) {
+        mn = fminf(mn, sm.u.b.red_f[i]);
+        mx = fmaxf(mx, sm.u.b.red_f[NW + i]);
+    }
+    if (!(mx > mn)) {
+        // one distinct value: the row is its own sorted order
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int i = (int)threadIdx.x + e * NT;
+            if (e < ne && i < n) y[i] = v[e];
+        }
+        return true;
+    }
+    const float scale = (float)NBK / (mx - mn);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e)
+        if (e < ne && (int)threadIdx.x + e * NT < n) atomicAdd(&sm.u.b.cnt[sort_bin(v[e], mx, scale)], 1u);
+    __syncthreads();
+    // exclusive scan of the bin counts (4 bins per thread) and the largest bin
+    constexpr int BPT = NBK / NT;
+    uint32_t c[BPT], run = 0, big = 0;
+#pragma unroll
+    for (int k = 0; k < BPT; ++k) {
+        c[k] = sm.u.b.cnt[threadIdx.x * BPT + k];
+        run += c[k];
+        big = max(big, c[k]);
+    }
+    uint32_t incl = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) big = max(big, __shfl_xor(big, o, 64));
+    __syncthreads();  // every thread has read its counts
+    if (lane == 63) sm.u.b.red_u[w] = incl;
+    if (lane == 0) sm.u.b.red_f[w] = __uint_as_float(big);
+    __syncthreads();
+    uint32_t base = 0, bmax = 0;
+    for (int i = 0; i < NW; ++i) {
+        if (i < w) base += sm.u.b.red_u[i];
+        bmax = max(bmax, __float_as_uint(sm.u.b.red_f[i]));
+    }
+    if (bmax > BIN_MAX) return false;  // uniform across the workgroup
+    uint32_t start = base + incl - run;
+#pragma unroll
+    for (int k = 0; k < BPT; ++k) {
+        sm.u.b.cnt[threadIdx.x * BPT + k] = start;  // scatter cursor, ends as the bin's end
+        start += c[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EPT; ++e)
+        if (e < ne && (int)threadIdx.x + e * NT < n) {
+            const uint32_t p = atomicAdd(&sm.u.b.cnt[sort_bin(v[e], mx, scale)], 1u);
+            sm.u.b.buf[p] = v[e];
+        }
+    __syncthreads();
+    // final positions, in buffer order (the lanes of a wavefront read the
+    // same few bins): the bin's start plus the values of the bin above this
+    // one, and the equal ones before it.  The value goes straight to its
+    // place in the output row; the row's input is no longer read.
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const float x0 = sm.u.b.buf[i];
+        const int b = sort_bin(x0, mx, scale);
+        const uint32_t b0 = b ? sm.u.b.cnt[b - 1] : 0u, b1 = sm.u.b.cnt[b];
+        // four LDS reads in flight per step (a bin holds 2-3 values on
+        // average); reads past the bin are clamped to the row and masked out
+        uint32_t r = b0;
+        for (uint32_t j = b0; j < b1; j += 4) {
+            float z[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) z[k] = sm.u.b.buf[min(j + k, (uint32_t)n - 1u)];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                r += (j + k < b1 && (z[k] > x0 || (z[k] == x0 && j + k < (uint32_t)i))) ? 1u : 0u;
+        }
+        y[r] = x0;
+    }
+    return true;
+}
+
+// Merge path for one row; x and y may be the same row: every value of x is
+// read before y is written where it lies.
+__device__ void merge_sort_row(float *s, const float *x, int32_t n, float *out_row, float *tmp_row) {
     int passes = 0;
     for (int64_t w = TILE; w < n; w <<= 1) ++passes;
-    float *a = (passes & 1) ? tmp + row * ld : out + row * ld;  // sorted tiles
-    float *b = (passes & 1) ? out + row * ld : tmp + row * ld;
+    float *a = (passes & 1) ? tmp_row : out_row;  // sorted tiles
+    float *b = (passes & 1) ? out_row : tmp_row;
 
     for (int t0 = 0; t0 < n; t0 += TILE) {
         const int m = min(TILE, n - t0);
@@ -104,6 +251,18 @@ __global__ void __launch_bounds__(NT) k_sort_rows(const float *src, int64_t ld_s
         a = b;
         b = t;
     }
+}
+
+__global__ void __launch_bounds__(NT) k_sort_rows(const float *src, int64_t ld_src, int32_t n, float *out,
+                                                  float *tmp, int64_t ld) {
+    __shared__ SortSmem sm;
+    const int64_t row = blockIdx.x;
+    const float *x = src + row * ld_src;
+    if (n <= NMAX) {
+        if (bucket_sort_row(sm, x, n, out + row * ld)) return;
+        __syncthreads();  // the bucket path's LDS is done with before the tiles reuse it
+    }
+    merge_sort_row(sm.u.tile, x, n, out + row * ld, tmp + row * ld);
 }
 
 }  // namespace

@@ -230,19 +230,27 @@ class Index:
         return int(self.lib.rkr_query_table_cols(self.ctx))
 
     # ---- queries ----------------------------------------------------------------
-    def query(self, queries, topk, max_k=None) -> QueryResult:
+    def query(self, queries, topk, max_k=None, out=None) -> QueryResult:
+        """Answers a batch.  ``out`` (a QueryResult of an earlier call with the
+        same query count and max_k) is written over instead of allocating new
+        arrays, which spares a serving loop the first-touch page faults of fresh
+        output buffers on every call."""
         queries = _f32(queries)
         if queries.ndim == 1:
             queries = queries[None, :]
         nq = queries.shape[0]
         ks = np.ascontiguousarray(np.broadcast_to(np.asarray(topk, np.int32), (nq,)), np.int32)
         mk = int(max_k if max_k is not None else (ks.max() if nq else 1))
-        users = np.empty((nq, mk), np.int32)
-        ranks = np.empty((nq, mk), np.int32)
-        qips = np.empty((nq, mk), np.float32)
-        # per-query stats land in a structured array with rkr_query_stats' layout
-        # (indexable as stats[i]["n_candidate"]; no per-query Python objects)
-        stats = np.empty(max(nq, 1), L.QUERY_STATS_DTYPE)
+        if out is not None and out.users.shape == (nq, mk) and len(out.stats) == nq and nq > 0:
+            users, ranks, qips, stats = out.users, out.ranks, out.qips, out.stats
+        else:
+            users = np.empty((nq, mk), np.int32)
+            ranks = np.empty((nq, mk), np.int32)
+            qips = np.empty((nq, mk), np.float32)
+            # per-query stats land in a structured array with rkr_query_stats'
+            # layout (indexable as stats[i]["n_candidate"]; no per-query Python
+            # objects)
+            stats = np.empty(max(nq, 1), L.QUERY_STATS_DTYPE)
         tm = L.BatchTiming()
         _check(self.lib, self.ctx, self.lib.rkr_query(self.ctx, _ptr(queries), nq, _ptr(ks), mk, _ptr(users),
                                                       _ptr(ranks), _ptr(qips),

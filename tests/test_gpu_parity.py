@@ -341,6 +341,33 @@ def test_sorted_scores_ties_batches_and_table(orc, tmp_path):
     assert np.array_equal(formats.read_score_table(p, I, mmap=False), got[:10])
 
 
+@pytest.mark.parametrize("case", ["moderate_ties", "outlier", "lds_max", "lds_max_plus_one"])
+def test_sorted_scores_row_shapes(orc, case):
+    """Rows the LDS bucket sort takes (ties of a few dozen copies, up to 28672
+    values) and rows it hands to the merge path (an outlier that crowds every
+    other value into one bin, 28673 values) sort to the oracle's rows."""
+    _case(f"sorted scores {case}")
+    import rkr
+    rng = np.random.default_rng(29)
+    d = 10
+    if case == "moderate_ties":
+        base = rng.standard_normal((400, d), dtype=np.float32)
+        items = base[rng.integers(0, 400, 8000)]
+    elif case == "outlier":
+        items = rng.standard_normal((6000, d), dtype=np.float32)
+        items[17] *= np.float32(1e5)
+    else:
+        items = rng.standard_normal((28672 if case == "lds_max" else 28673, d), dtype=np.float32)
+    users = rng.standard_normal((12, d), dtype=np.float32)
+    idx = rkr.Index(0)
+    try:
+        idx.set_data(users, items)
+        got = idx.sorted_scores()
+        assert np.array_equal(got, orc.sorted_score_rows(_normalized(orc, users), items))
+    finally:
+        idx.close()
+
+
 def test_sorted_scores_follow_updates(orc):
     """After item inserts/deletes and user deletes the rows cover the current sets."""
     _case("sorted scores after updates")
@@ -642,3 +669,28 @@ def test_cli_bst_fake30d(orc, fake30d, tmp_path):
                                     items.shape[0])
     assert rows.shape == (u.shape[0], items.shape[0])
     assert np.array_equal(rows, orc.sorted_score_rows(orc.normalize_users(u), items))
+
+
+def test_query_into_reused_outputs(orc, gpu_index_factory):
+    """Index.query(..., out=previous) writes the same answers and stats into the
+    previous call's arrays; a different batch shape allocates new ones."""
+    _case("query outputs reused")
+    rng = np.random.default_rng(77)
+    U, I, d = 3000, 1500, 24
+    users = rng.standard_normal((U, d), dtype=np.float32)
+    items = rng.standard_normal((I, d), dtype=np.float32)
+    idx, _ = gpu_index_factory(users, items, 64)
+    q1, q2 = items[:40], items[40:80]
+    ks = rng.integers(1, 60, 40).astype(np.int32)
+    a = idx.query(q1, ks, max_k=60)
+    first = a.users.copy(), a.ranks.copy()
+    b = idx.query(q2, ks, max_k=60, out=a)
+    assert b.users is a.users  # the same arrays, written over
+    fresh = idx.query(q2, ks, max_k=60)
+    assert np.array_equal(b.users, fresh.users) and np.array_equal(b.ranks, fresh.ranks)
+    assert np.array_equal(b.stats["n_candidate"], fresh.stats["n_candidate"])
+    c = idx.query(q1, ks, max_k=60, out=b)
+    assert np.array_equal(c.users, first[0]) and np.array_equal(c.ranks, first[1])
+    d = idx.query(q1[:5], ks[:5], max_k=60, out=c)
+    assert d.users is not c.users and d.users.shape == (5, 60)
+    assert np.array_equal(d.users, first[0][:5])
