@@ -43,6 +43,8 @@ WORKLOADS = {
                                        "x 20k items x d=150"),
     "qs_train": ("plain_gpu", None, "qs_train: QS sample-rank training (qdibc + fsr) of 100k users x 20k items x d=150, "
                                     "9000 sample queries, sample_topk 600, 512 ranks"),
+    "serving_precision": ("precision", 50, "serving at 1M users x 50k items x d=150, S=512, k=50: Poisson arrivals, "
+                                           "no inserts"),
     "serving": ("serving", None, "serving: 2M users x 100k items x d=128, S=512, Poisson arrivals, k in [10,200], "
                                  "+1% users and +0.5% items every 1 s"),
 }
@@ -101,7 +103,7 @@ def parse():
     return ap.parse_args()
 
 
-def serving(args, idx, wl, label):
+def serving(args, idx, wl, label, k_fixed=None):
     """Serving form: Poisson query arrivals (k uniform in [10, 200]) while an
     insert batch of 1% new users and 0.5% new items arrives every second.
     The saturating rate is measured first, closed-loop, under the same insert
@@ -118,7 +120,8 @@ def serving(args, idx, wl, label):
     # capacity for every insert round of the run, so updates never allocate
     rounds = int(np.ceil(args.sat_seconds + args.duration)) + 3
     side = args.side_reserve if args.side_reserve > 0 else min(16384, rounds * n_new_i)
-    idx.reserve(U0 + rounds * n_new_u, side)
+    if not args.no_inserts:
+        idx.reserve(U0 + rounds * n_new_u, side)
     srv = rkr.Server(idx, max_batch=args.max_batch, n_slots=args.slots, update_mode=args.update_mode,
                      update_cus=args.update_cus, rebuild_pending=args.rebuild_pending)
     upd, stop_upd = [], threading.Event()
@@ -183,7 +186,7 @@ def serving(args, idx, wl, label):
     t0 = time.perf_counter()
     n_sat = 0
     sat_rows = rng.integers(0, I0, 1 << 20)
-    sat_ks = rng.integers(10, 201, 1 << 20)
+    sat_ks = rng.integers(10, 201, 1 << 20) if k_fixed is None else np.full(1 << 20, k_fixed)
     j = 0
     from collections import deque
     pending = deque()
@@ -203,7 +206,7 @@ def serving(args, idx, wl, label):
     # open loop
     n_arr = int(rate * args.duration)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_arr))
-    kq = rng.integers(10, 201, n_arr)
+    kq = rng.integers(10, 201, n_arr) if k_fixed is None else np.full(n_arr, k_fixed)
     qi = rng.integers(0, I0, n_arr)
     tickets = []
     start = time.perf_counter()
@@ -262,7 +265,8 @@ def serving(args, idx, wl, label):
     per_s = [round(float(np.percentile(lat[(ts >= s0) & (ts < s0 + 1)], 99)), 2)
              for s0 in range(int(np.ceil(ts.max())) if len(ts) else 0) if ((ts >= s0) & (ts < s0 + 1)).any()]
     out = {
-        "metric": "serving queries/s with p50/p99 latency (exact ranks), inserts every 1 s",
+        "metric": "serving queries/s with p50/p99 latency (exact ranks)" + ("" if args.no_inserts else
+                                                                           ", inserts every 1 s"),
         "value": round(len(lat) / wall, 2), "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "bf16" if args.form == "screened" else "f32", "data": "synthetic (seeded), open-loop Poisson arrivals",
         "config": {"workload": label, "form": args.form, "update_mode": args.update_mode,
@@ -540,9 +544,13 @@ def main():
             idx.insert_users(rng_u.standard_normal((max(1, U // 100), d), dtype=np.float32) / np.float32(np.sqrt(d)))
             idx.insert_items(rng_u.standard_normal((max(1, I // 200), d), dtype=np.float32) / np.float32(np.sqrt(d)))
         U, I = idx.n_user, idx.n_item
-    if args.workload == "serving":
+    if args.workload in ("serving", "serving_precision"):
         print(json.dumps({"table_build_ms": round(build_ms, 2), "n_user": U, "n_item": I}), file=sys.stderr, flush=True)
-        serving(args, idx, wl, label)
+        if args.workload == "serving_precision":
+            # no update stream: no CU split for one either
+            args.no_inserts = True
+            args.update_mode = "barrier"
+        serving(args, idx, wl, label, k_fixed)
         idx.close()
         return
 

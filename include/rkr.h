@@ -59,7 +59,7 @@ typedef struct rkr_ctx rkr_ctx;
 #define RKR_OPT_DEDUP 6         /* 1 (default): byte-equal queries of one rkr_query
                                    call are answered once, with their largest k   */
 #define RKR_OPT_SCREEN_ROWS 7   /* refine rows per wavefront of the screen: 32 or 64
-                                   (0 = 32)                                       */
+                                   (0 = 32 for d <= 160, 64 above)                */
 #define RKR_OPT_DENSE_TOP 8     /* query table that exact output searches, kept
                                    by rkr_build_table beside the unchanged sampled
                                    table: 2 (default) every rank below the largest

@@ -1788,7 +1788,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                 L.target_units = (int64_t)c->n_cu * 8;
                 L.grid_cap = (int64_t)c->n_cu * 4;
                 L.chunk_items = c->screen_chunk > 0 ? c->screen_chunk : (1 << 30);
-                L.rows_per_wave = c->screen_rows > 0 ? c->screen_rows : 32;
+                L.rows_per_wave = c->screen_rows > 0 ? c->screen_rows : (c->kp / 32 >= 6 ? 64 : 32);  // d > 160: 32 rows spill
                 RKR_HIP(c, launch_refine_screen(sl.stream, L));
                 RKR_HIP(c, hipEventRecord(sl.ev[8], sl.stream));
                 RKR_HIP(c, launch_recheck(sl.stream, dv.users, dv.items, ld, d, refine_user, refine_thr, refine_query,
