@@ -251,24 +251,43 @@ __device__ float wg_select(SelSmem &sm, const float *src, int64_t n, float lo, f
     return lo;
 }
 
-// Ranks one bin of h <= THREAD_SEL_MAX elements (gbuf[seg..seg+h)) in one
-// thread and writes T[j] for its targets j0 <= j < j1.
+// Ranks one bin of h <= H elements (gbuf[seg..seg+h)) in one thread and
+// writes T[j] for its targets j0 <= j < j1.  The h values go to registers
+// (padding -inf, which ranks after every element: scores are finite); an
+// element's position in descending order, ties broken by index, is the count
+// of elements before it (H * H register compares); each target then picks the
+// element at its position.
+template <int H>
+__device__ __forceinline__ void thread_select_n(const float *gbuf, uint32_t seg, uint32_t h, uint32_t above,
+                                                const int32_t *__restrict__ ranks, int j0, int j1, float *T) {
+    float v[H];
+#pragma unroll
+    for (int i = 0; i < H; ++i) v[i] = (uint32_t)i < h ? gbuf[seg + i] : -INFINITY;
+    uint32_t pos[H];
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        uint32_t p = 0;
+#pragma unroll
+        for (int k = 0; k < H; ++k) p += (v[k] > v[i] || (v[k] == v[i] && k < i)) ? 1u : 0u;
+        pos[i] = p;
+    }
+    for (int j = j0; j < j1; ++j) {
+        const uint32_t t = (uint32_t)ranks[j] - above;  // < h
+        float x = 0.0f;
+#pragma unroll
+        for (int i = 0; i < H; ++i) x = pos[i] == t ? v[i] : x;
+        T[j] = x;
+    }
+}
+
+// The smallest register network that holds the bin (most bins of a row hold
+// one to a few elements).
 __device__ __forceinline__ void thread_select(const float *gbuf, uint32_t seg, uint32_t h, uint32_t above,
                                               const int32_t *__restrict__ ranks, int j0, int j1, float *T) {
-    float v[THREAD_SEL_MAX];
-#pragma unroll
-    for (int i = 0; i < THREAD_SEL_MAX; ++i) v[i] = (uint32_t)i < h ? gbuf[seg + i] : -INFINITY;
-    // position of element i in descending order, ties broken by index; the
-    // -inf padding ranks after every element of the bin (scores are finite)
-#pragma unroll
-    for (int i = 0; i < THREAD_SEL_MAX; ++i) {
-        uint32_t pos = 0;
-#pragma unroll
-        for (int k = 0; k < THREAD_SEL_MAX; ++k) pos += (v[k] > v[i] || (v[k] == v[i] && k < i)) ? 1u : 0u;
-        if ((uint32_t)i < h)
-            for (int j = j0; j < j1; ++j)
-                if ((uint32_t)ranks[j] - above == pos) T[j] = v[i];
-    }
+    if (h <= 2) thread_select_n<2>(gbuf, seg, h, above, ranks, j0, j1, T);
+    else if (h <= 4) thread_select_n<4>(gbuf, seg, h, above, ranks, j0, j1, T);
+    else if (h <= 8) thread_select_n<8>(gbuf, seg, h, above, ranks, j0, j1, T);
+    else thread_select_n<THREAD_SEL_MAX>(gbuf, seg, h, above, ranks, j0, j1, T);
 }
 
 __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ scores, int64_t n_item,
