@@ -30,6 +30,7 @@ constexpr int NT = 1024;          // threads per row
 constexpr int NW = NT / 64;       // wavefronts per row
 constexpr int NB = 16384;         // level-0 bins
 constexpr int NB2 = 1024;         // bins of the narrowing loop
+constexpr int NB1 = 1024;         // coarse bins that apportion the level-0 bins
 constexpr int SMAX = 4096;        // largest n_sample handled
 constexpr int GCAP = NB;          // floats gathered per pass (the histogram's 64 KB)
 constexpr int THREAD_SEL_MAX = 16;  // bins ranked by one thread
@@ -51,6 +52,7 @@ struct SelSmem {
     uint16_t grp[SMAX + 1];       // group start (index into fb) | DEEP
     uint16_t coop[SMAX];          // distinct bins of a group needing wg_select
     uint32_t hist2[NB2 + 1];
+    uint16_t fstart[NB1 + 1];     // first level-0 bin of each coarse bin (NB at the end)
     float red_f[2 * NW];
     uint32_t red_u[NW + 1];
     int32_t nfb, ngrp, ncoop, sel_b;
@@ -64,6 +66,21 @@ __device__ __forceinline__ int bin_f32(float x, float lo, float scale, int nb) {
     if (!(t >= 0.0f)) return 0;
     if (!(t < (float)nb)) return nb - 1;
     return (int)t;
+}
+
+// Level-0 bin of x: its coarse bin c (NB1 linear bins over [lo, hi]) owns
+// level-0 bins fstart[c] .. fstart[c + 1] - 1, in proportion to the row's
+// elements in c, and x takes the one its place inside c falls in.  Monotone
+// in x: t is (see bin_f32), so are c and, within one c, t - c.
+__device__ __forceinline__ int fine_bin(const uint16_t *fstart, float x, float lo, float scale1) {
+    const float t = (x - lo) * scale1;
+    int c;
+    float frac;
+    if (!(t >= 0.0f)) { c = 0; frac = 0.0f; }
+    else if (!(t < (float)NB1)) { c = NB1 - 1; frac = 1.0f; }
+    else { c = (int)t; frac = t - (float)c; }
+    const int s0 = fstart[c], n = (int)fstart[c + 1] - s0;  // n >= 1
+    return s0 + min(n - 1, (int)(frac * (float)n));
 }
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -308,13 +325,39 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
         for (int j = tid; j < S; j += NT) T[j] = lo0;
         return;
     }
-    const float scale0 = (float)NB / (hi0 - lo0);
+    // 2a. coarse histogram: the level-0 bins are apportioned to NB1 linear
+    // coarse bins by their element counts (each gets at least one), so
+    // level-0 bins are narrow where the row is dense (a row with a heavy tail
+    // otherwise puts hundreds of elements in each central linear bin)
+    const float scale1 = (float)NB1 / (hi0 - lo0);
+    for (int i = tid; i < NB1; i += NT) sm.hist2[i] = 0;
+    __syncthreads();
+    row_for_each(x, n_item, [&](float v) { atomicAdd(&sm.hist2[bin_f32(v, lo0, scale1, NB1)], 1u); });
+    __syncthreads();
+    {
+        constexpr int PER1 = NB1 / NT > 0 ? NB1 / NT : 1;  // coarse bins per thread
+        uint32_t c_loc[PER1], sum = 0;
+        for (int i = 0; i < PER1; ++i) {
+            const int c = tid * PER1 + i;
+            c_loc[i] = c < NB1 ? sm.hist2[c] : 0u;
+            sum += c_loc[i];
+        }
+        uint32_t total;
+        uint32_t cum = wg_excl_scan(sm, sum, &total);  // elements below the thread's first coarse bin
+        for (int i = 0; i < PER1; ++i) {
+            const int c = tid * PER1 + i;
+            if (c < NB1)
+                sm.fstart[c] = (uint16_t)(c + (int)(((uint64_t)(NB - NB1) * cum) / (uint64_t)max(total, 1u)));
+            cum += c_loc[i];
+        }
+        if (tid == 0) sm.fstart[NB1] = (uint16_t)NB;
+    }
 
-    // 2. histogram + reverse prefix sums
+    // 2b. histogram + reverse prefix sums
     for (int i = tid; i < NB; i += NT) sm.u.incl[i] = 0;
     for (int i = tid; i < NB / 32; i += NT) sm.tbits[i] = 0;
     __syncthreads();
-    row_for_each(x, n_item, [&](float v) { atomicAdd(&sm.u.incl[bin_f32(v, lo0, scale0, NB)], 1u); });
+    row_for_each(x, n_item, [&](float v) { atomicAdd(&sm.u.incl[fine_bin(sm.fstart, v, lo0, scale1)], 1u); });
     __syncthreads();
     wg_reverse_incl(sm, sm.u.incl, NB);
 
@@ -405,7 +448,7 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
             const int b = sm.tb[j0];
             float bmn = INFINITY, bmx = -INFINITY;
             row_for_each(x, n_item, [&](float v) {
-                if (bin_f32(v, lo0, scale0, NB) == b) { bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
+                if (fine_bin(sm.fstart, v, lo0, scale1) == b) { bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
             });
             wg_minmax(sm, bmn, bmx);
             for (int j = j0; j < j1; ++j) {
@@ -439,7 +482,7 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
         __syncthreads();
         // gather: the member bin of rank m in ascending bin order is f1 - 1 - m
         row_for_each(x, n_item, [&](float v) {
-            const int b = bin_f32(v, lo0, scale0, NB);
+            const int b = fine_bin(sm.fstart, v, lo0, scale1);
             const uint32_t word = sm.tbits[b >> 5], bit = 1u << (b & 31);
             if (word & bit) {
                 const int m = (int)sm.tpre[b >> 5] + __popc(word & (bit - 1u));
