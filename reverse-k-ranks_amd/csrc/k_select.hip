@@ -33,8 +33,10 @@ constexpr int NB2 = 1024;         // bins of the narrowing loop
 constexpr int NB1 = 1024;         // coarse bins that apportion the level-0 bins
 constexpr int SMAX = 4096;        // largest n_sample handled
 constexpr int GCAP = NB;          // floats gathered per pass (the histogram's 64 KB)
-constexpr int THREAD_SEL_MAX = 16;  // bins ranked by one thread
+constexpr int EMAX = 64;            // bins up to this size are ranked element-parallel
 constexpr int WAVE_SEL_MAX = 2048;  // bins larger than this use wg_select
+constexpr int MIDCAP = GCAP / (EMAX + 1) + 1;          // wavefront bins of one pass, at most
+constexpr int COOPCAP = GCAP / (WAVE_SEL_MAX + 1) + 1; // workgroup bins of one pass, at most
 constexpr uint16_t DEEP = 0x8000;
 
 struct SelSmem {
@@ -50,38 +52,65 @@ struct SelSmem {
     uint32_t fabove[SMAX];        // elements in the bins above distinct bin f
     uint32_t fcur[SMAX];          // gather cursor of distinct bin f (ends at its segment's end)
     uint16_t grp[SMAX + 1];       // group start (index into fb) | DEEP
-    uint16_t coop[SMAX];          // distinct bins of a group needing wg_select
+    uint16_t mid[MIDCAP];         // distinct bins of a pass ranked by one wavefront
+    uint16_t coop[COOPCAP];       // distinct bins of a pass needing wg_select
+    uint32_t sbits[GCAP / 32];    // first gathered element of each bin of the pass
+    uint16_t spre[GCAP / 32];     // bins starting below each sbits word
     uint32_t hist2[NB2 + 1];
     uint16_t fstart[NB1 + 1];     // first level-0 bin of each coarse bin (NB at the end)
     float red_f[2 * NW];
     uint32_t red_u[NW + 1];
-    int32_t nfb, ngrp, ncoop, sel_b;
+    int32_t nfb, ngrp, ncoop, nmid, sel_b;
 };
 
 // Monotone in x for x >= lo with FP32 arithmetic (every rounding step is
-// monotone); NaN (only from 0 * inf at x == lo) maps to bin 0, overflow to the
-// last bin, so an LDS index is always in range.
+// monotone); NaN (only from 0 * inf at x == lo) maps to bin 0 (fmaxf returns
+// the number), overflow to the last bin, so an LDS index is always in range.
+// Clamped with min/max rather than branches: this runs once per element.
 __device__ __forceinline__ int bin_f32(float x, float lo, float scale, int nb) {
     const float t = (x - lo) * scale;
-    if (!(t >= 0.0f)) return 0;
-    if (!(t < (float)nb)) return nb - 1;
-    return (int)t;
+    return (int)fminf(fmaxf(t, 0.0f), (float)(nb - 1));
 }
 
 // Level-0 bin of x: its coarse bin c (NB1 linear bins over [lo, hi]) owns
 // level-0 bins fstart[c] .. fstart[c + 1] - 1, in proportion to the row's
 // elements in c, and x takes the one its place inside c falls in.  Monotone
 // in x: t is (see bin_f32), so are c and, within one c, t - c.
+// t is clamped to [0, NB1] (NaN to 0): below the range c = 0 with frac 0,
+// at or past its end c = NB1 - 1 with frac 1.
 __device__ __forceinline__ int fine_bin(const uint16_t *fstart, float x, float lo, float scale1) {
-    const float t = (x - lo) * scale1;
-    int c;
-    float frac;
-    if (!(t >= 0.0f)) { c = 0; frac = 0.0f; }
-    else if (!(t < (float)NB1)) { c = NB1 - 1; frac = 1.0f; }
-    else { c = (int)t; frac = t - (float)c; }
+    const float t = fminf(fmaxf((x - lo) * scale1, 0.0f), (float)NB1);
+    const int c = min((int)t, NB1 - 1);
+    const float frac = t - (float)c;
     const int s0 = fstart[c], n = (int)fstart[c + 1] - s0;  // n >= 1
     return s0 + min(n - 1, (int)(frac * (float)n));
 }
+
+// A row held in registers: thread t owns float4 t + k*NT for k < C4 (rows of
+// up to 16*C4*NT values whose length is a multiple of 4 and whose start is
+// 16-byte aligned).  The row is then read from memory once; every later pass
+// over it runs from registers.
+template <int C4>
+struct RowRegs {
+    float4 r[C4];
+    int nv;  // float4s this thread owns
+    __device__ __forceinline__ void load(const float *__restrict__ x, int64_t n) {
+        const float4 *x4 = reinterpret_cast<const float4 *>(x);
+        const int n4 = (int)(n >> 2), t = threadIdx.x;
+        nv = n4 > t ? (n4 - t + NT - 1) / NT : 0;
+        // unconditional loads of a clamped index (every thread owns at least
+        // one float4 whenever the row has NT of them; others read x4[0]), so
+        // all C4 loads are in flight together
+#pragma unroll
+        for (int k = 0; k < C4; ++k) r[k] = x4[min(t + k * NT, max(n4 - 1, 0))];
+    }
+    template <class F>
+    __device__ __forceinline__ void for_each(F &&f) const {
+#pragma unroll
+        for (int k = 0; k < C4; ++k)
+            if (k < nv) { f(r[k].x); f(r[k].y); f(r[k].z); f(r[k].w); }
+    }
+};
 
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
@@ -173,7 +202,7 @@ __device__ __forceinline__ void wave_sort_select(const float *gbuf, uint32_t seg
 }
 
 // Workgroup min/max; every thread gets the result.
-__device__ void wg_minmax(SelSmem &sm, float &mn, float &mx) {
+__device__ __forceinline__ void wg_minmax(SelSmem &sm, float &mn, float &mx) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     mn = wave_min(mn);
     mx = wave_max(mx);
@@ -187,7 +216,7 @@ __device__ void wg_minmax(SelSmem &sm, float &mn, float &mx) {
 
 // Exclusive scan of one value per thread; returns the thread's prefix and
 // writes the total to *total.
-__device__ uint32_t wg_excl_scan(SelSmem &sm, uint32_t v, uint32_t *total) {
+__device__ __forceinline__ uint32_t wg_excl_scan(SelSmem &sm, uint32_t v, uint32_t *total) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t x = v;
 #pragma unroll
@@ -209,17 +238,63 @@ __device__ uint32_t wg_excl_scan(SelSmem &sm, uint32_t v, uint32_t *total) {
     return r;
 }
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 // Turns counts h[0..nb) into incl[b] = sum_{b' >= b} h[b'] in place
-// (incl[nb] = 0).  nb is a multiple of NT.
-__device__ void wg_reverse_incl(SelSmem &sm, uint32_t *h, int nb) {
-    const int per = nb / NT;  // <= 16
+// (incl[nb] = 0).  nb is a multiple of NT; thread t owns h[t*per, t*per+per).
+// With per = 16 a thread's bins are four 16-byte granules: lane t reads (and
+// writes) granule (i + t/2) mod 4 in step i, so the lanes of one step cover
+// all 32 LDS banks instead of the 8 that a stride of 16 words reaches.
+template <int nb>
+__device__ __forceinline__ void wg_reverse_incl(SelSmem &sm, uint32_t *h) {
+    constexpr int per = nb / NT;
+    static_assert(per >= 1 && per * NT == nb, "bins must be a multiple of the workgroup");
     const int base = threadIdx.x * per;
-    uint32_t loc[16];
-    uint32_t s = 0;
-    for (int i = 0; i < per; ++i) { loc[i] = h[base + i]; s += loc[i]; }
-    uint32_t total;
-    uint32_t e = wg_excl_scan(sm, s, &total);
-    for (int i = 0; i < per; ++i) { h[base + i] = total - e; e += loc[i]; }
+    if constexpr (per == 16) {
+        const int rot = (threadIdx.x >> 1) & 3;
+        u32x4 q[4], nat[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q[i] = *reinterpret_cast<const u32x4 *>(&h[base + 4 * ((i + rot) & 3)]);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {  // granule g was read in step (g - rot) mod 4
+            const int i = (g - rot) & 3;
+            u32x4 v = q[0];
+#pragma unroll
+            for (int k = 1; k < 4; ++k) v = __builtin_unpredictable(i == k) ? q[k] : v;
+            nat[g] = v;
+        }
+        uint32_t s = 0;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) s += nat[g].x + nat[g].y + nat[g].z + nat[g].w;
+        uint32_t total;
+        uint32_t e = wg_excl_scan(sm, s, &total);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            u32x4 o;
+            o.x = total - e; e += nat[g].x;
+            o.y = total - e; e += nat[g].y;
+            o.z = total - e; e += nat[g].z;
+            o.w = total - e; e += nat[g].w;
+            nat[g] = o;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {  // step i writes granule (i + rot) mod 4
+            const int g = (i + rot) & 3;
+            u32x4 v = nat[0];
+#pragma unroll
+            for (int k = 1; k < 4; ++k) v = __builtin_unpredictable(g == k) ? nat[k] : v;
+            *reinterpret_cast<u32x4 *>(&h[base + 4 * g]) = v;
+        }
+    } else {
+        uint32_t loc[per];
+        uint32_t s = 0;
+#pragma unroll
+        for (int i = 0; i < per; ++i) { loc[i] = h[base + i]; s += loc[i]; }
+        uint32_t total;
+        uint32_t e = wg_excl_scan(sm, s, &total);
+#pragma unroll
+        for (int i = 0; i < per; ++i) { h[base + i] = total - e; e += loc[i]; }
+    }
     if (threadIdx.x == 0) h[nb] = 0;
     __syncthreads();
 }
@@ -251,7 +326,7 @@ __device__ float wg_select(SelSmem &sm, const float *src, int64_t n, float lo, f
             if (x >= lo && x <= hi) atomicAdd(&sm.hist2[bin_f32(x, lo, scale, NB2)], 1u);
         }
         __syncthreads();
-        wg_reverse_incl(sm, sm.hist2, NB2);
+        wg_reverse_incl<NB2>(sm, sm.hist2);
         if (threadIdx.x == 0) sm.sel_b = find_bin(sm.hist2, NB2, t);
         __syncthreads();
         const int b = sm.sel_b;
@@ -268,45 +343,169 @@ __device__ float wg_select(SelSmem &sm, const float *src, int64_t n, float lo, f
     return lo;
 }
 
-// Ranks one bin of h <= H elements (gbuf[seg..seg+h)) in one thread and
-// writes T[j] for its targets j0 <= j < j1.  The h values go to registers
-// (padding -inf, which ranks after every element: scores are finite); an
-// element's position in descending order, ties broken by index, is the count
-// of elements before it (H * H register compares); each target then picks the
-// element at its position.
-template <int H>
-__device__ __forceinline__ void thread_select_n(const float *gbuf, uint32_t seg, uint32_t h, uint32_t above,
-                                                const int32_t *__restrict__ ranks, int j0, int j1, float *T) {
-    float v[H];
-#pragma unroll
-    for (int i = 0; i < H; ++i) v[i] = (uint32_t)i < h ? gbuf[seg + i] : -INFINITY;
-    uint32_t pos[H];
-#pragma unroll
-    for (int i = 0; i < H; ++i) {
-        uint32_t p = 0;
-#pragma unroll
-        for (int k = 0; k < H; ++k) p += (v[k] > v[i] || (v[k] == v[i] && k < i)) ? 1u : 0u;
-        pos[i] = p;
+// One group of target bins of the row (a pass of the gather over the row, or
+// one bin larger than LDS narrowed on its own): `each` visits the row's values.
+template <class Each>
+__device__ __forceinline__ void select_group(SelSmem &sm, int g, int ngrp, int nfb, const float *__restrict__ x,
+                                             int64_t n_item, const int32_t *__restrict__ ranks, int32_t S, float *T,
+                                             float lo0, float scale1, Each &&each) {
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const bool deep = (sm.grp[g] & DEEP) != 0;
+    const int f0 = sm.grp[g] & ~DEEP;
+    const int f1 = sm.grp[g + 1] & ~DEEP;
+    if (deep) {
+        // one bin larger than LDS: narrow it from the row in global memory
+        const int j0 = sm.fb[f0];
+        const int j1 = (f0 + 1 < nfb) ? sm.fb[f0 + 1] : S;
+        const int b = sm.tb[j0];
+        float bmn = INFINITY, bmx = -INFINITY;
+        each([&](float v) {
+            if (fine_bin(sm.fstart, v, lo0, scale1) == b) { bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
+        });
+        wg_minmax(sm, bmn, bmx);
+        for (int j = j0; j < j1; ++j) {
+            const uint32_t t = (uint32_t)ranks[j] - sm.fabove[f0];
+            const float v = wg_select(sm, x, n_item, bmn, bmx, t);
+            if (tid == 0) T[j] = v;
+        }
+        return;
     }
-    for (int j = j0; j < j1; ++j) {
-        const uint32_t t = (uint32_t)ranks[j] - above;  // < h
-        float x = 0.0f;
-#pragma unroll
-        for (int i = 0; i < H; ++i) x = pos[i] == t ? v[i] : x;
-        T[j] = x;
+    // member bins: bitmap bits, then segment starts (bins descending in f)
+    for (int f = f0 + tid; f < f1; f += NT) {
+        const int b = sm.tb[sm.fb[f]];
+        atomicOr(&sm.tbits[b >> 5], 1u << (b & 31));
     }
+    if (ngrp > 1) {
+        // several passes: offsets restart at each group's first bin
+        __syncthreads();
+        const uint32_t base = sm.fcur[f0];
+        __syncthreads();
+        for (int f = f0 + tid; f < f1; f += NT) sm.fcur[f] -= base;
+    }
+    for (int i = tid; i < GCAP / 32; i += NT) sm.sbits[i] = 0;
+    if (tid == 0) { sm.ncoop = 0; sm.nmid = 0; }
+    __syncthreads();
+    {
+        // member bins below each bitmap word (ascending bins)
+        const uint32_t cnt = tid < NB / 32 ? (uint32_t)__popc(sm.tbits[tid]) : 0u;
+        uint32_t tot;
+        const uint32_t pre = wg_excl_scan(sm, cnt, &tot);
+        if (tid < NB / 32) sm.tpre[tid] = (uint16_t)pre;
+    }
+    __syncthreads();
+    // gather: the member bin of rank m in ascending bin order is f1 - 1 - m
+    each([&](float v) {
+        const int b = fine_bin(sm.fstart, v, lo0, scale1);
+        const uint32_t word = sm.tbits[b >> 5], bit = 1u << (b & 31);
+        if (word & bit) {
+            const int m = (int)sm.tpre[b >> 5] + __popc(word & (bit - 1u));
+            sm.u.gbuf[atomicAdd(&sm.fcur[f1 - 1 - m], 1u)] = v;
+        }
+    });
+    __syncthreads();
+    // Bins of up to EMAX elements are ranked element-parallel: each gathered
+    // element finds its bin from the segment starts (a bitmap over the gather
+    // buffer and its popcount prefix), counts the elements of its segment
+    // that order before it (larger, or equal at a lower index) and, if that
+    // rank is a target, writes it.  Larger bins are listed for a wavefront or
+    // the whole workgroup.
+    for (int f = f0 + tid; f < f1; f += NT) {
+        const uint32_t h = sm.fh[f];
+        const uint32_t st = sm.fcur[f] - h;
+        atomicOr(&sm.sbits[st >> 5], 1u << (st & 31));
+        if (h > WAVE_SEL_MAX) sm.coop[atomicAdd(&sm.ncoop, 1)] = (uint16_t)f;
+        else if (h > EMAX) sm.mid[atomicAdd(&sm.nmid, 1)] = (uint16_t)f;
+    }
+    __syncthreads();
+    {
+        const uint32_t cnt = tid < GCAP / 32 ? (uint32_t)__popc(sm.sbits[tid]) : 0u;
+        uint32_t tot;
+        const uint32_t pre = wg_excl_scan(sm, cnt, &tot);
+        if (tid < GCAP / 32) sm.spre[tid] = (uint16_t)pre;
+    }
+    __syncthreads();
+    const uint32_t gtot = sm.fcur[f1 - 1];  // elements gathered in this pass
+    for (uint32_t i = tid; i < gtot; i += NT) {
+        const uint32_t word = sm.sbits[i >> 5];
+        const int f = f0 + (int)sm.spre[i >> 5] + __popc(word & (0xffffffffu >> (31 - (i & 31)))) - 1;
+        const uint32_t h = sm.fh[f];
+        if (h > EMAX) continue;
+        const uint32_t seg = sm.fcur[f] - h;
+        const float v = sm.u.gbuf[i];
+        uint32_t pos = 0;
+        for (uint32_t q = seg; q < seg + h; ++q) {
+            const float o = sm.u.gbuf[q];
+            pos += (o > v || (o == v && q < i)) ? 1u : 0u;
+        }
+        const uint32_t r = sm.fabove[f] + pos;
+        const int j0 = sm.fb[f], j1 = (f + 1 < nfb) ? sm.fb[f + 1] : S;
+        const uint32_t r0 = (uint32_t)ranks[j0];
+        if (r < r0) continue;
+        // ranks increase strictly, so rank r can only sit at index <= j0 + (r - r0)
+        int hi = min(j0 + (int)(r - r0), j1 - 1), lo = j0;
+        if ((uint32_t)ranks[hi] <= r) {
+            lo = hi;
+        } else {
+            --hi;
+            while (lo < hi) {
+                const int m = (lo + hi + 1) >> 1;
+                if ((uint32_t)ranks[m] <= r) lo = m; else hi = m - 1;
+            }
+        }
+        if ((uint32_t)ranks[lo] == r) T[lo] = v;
+    }
+    // mid-sized bins: one wavefront each
+    const int nmid = sm.nmid;
+    for (int m = w; m < nmid; m += NW) {
+        const int f = sm.mid[m];
+        const uint32_t h = sm.fh[f];
+        const int j0 = sm.fb[f];
+        const int j1 = (f + 1 < nfb) ? sm.fb[f + 1] : S;
+        const uint32_t seg = sm.fcur[f] - h, above = sm.fabove[f];
+        if (h <= 128) { wave_sort_select<2>(sm.u.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
+        if (h <= 256) { wave_sort_select<4>(sm.u.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
+        // larger bins (rare): rank each element by counting
+        for (uint32_t i = lane; i < h; i += 64) {
+            const float e = sm.u.gbuf[seg + i];
+            uint32_t pos = 0;
+            for (uint32_t k = 0; k < h; ++k) {
+                const float o = sm.u.gbuf[seg + k];
+                pos += (o > e || (o == e && k < i)) ? 1u : 0u;
+            }
+            const uint32_t r = above + pos;
+            for (int j = j0; j < j1; ++j)
+                if ((uint32_t)ranks[j] == r) T[j] = e;
+        }
+    }
+    __syncthreads();
+    // bins too large for a wavefront: the whole workgroup
+    const int ncoop = sm.ncoop;
+    for (int c = 0; c < ncoop; ++c) {
+        const int f = sm.coop[c];
+        const int j0 = sm.fb[f];
+        const int j1 = (f + 1 < nfb) ? sm.fb[f + 1] : S;
+        const uint32_t h = sm.fh[f];
+        const uint32_t seg = sm.fcur[f] - h;
+        float bmn = INFINITY, bmx = -INFINITY;
+        for (uint32_t i = tid; i < h; i += NT) { const float v = sm.u.gbuf[seg + i]; bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
+        wg_minmax(sm, bmn, bmx);
+        for (int j = j0; j < j1; ++j) {
+            const uint32_t t = (uint32_t)ranks[j] - sm.fabove[f];
+            const float v = wg_select(sm, sm.u.gbuf + seg, h, bmn, bmx, t);
+            if (tid == 0) T[j] = v;
+        }
+    }
+    // release the member bins
+    for (int f = f0 + tid; f < f1; f += NT) {
+        const int b = sm.tb[sm.fb[f]];
+        sm.tbits[b >> 5] = 0;
+    }
+    __syncthreads();
 }
 
-// The smallest register network that holds the bin (most bins of a row hold
-// one to a few elements).
-__device__ __forceinline__ void thread_select(const float *gbuf, uint32_t seg, uint32_t h, uint32_t above,
-                                              const int32_t *__restrict__ ranks, int j0, int j1, float *T) {
-    if (h <= 2) thread_select_n<2>(gbuf, seg, h, above, ranks, j0, j1, T);
-    else if (h <= 4) thread_select_n<4>(gbuf, seg, h, above, ranks, j0, j1, T);
-    else if (h <= 8) thread_select_n<8>(gbuf, seg, h, above, ranks, j0, j1, T);
-    else thread_select_n<THREAD_SEL_MAX>(gbuf, seg, h, above, ranks, j0, j1, T);
-}
-
+// C4 > 0: the row is held in registers (RowRegs<C4>); C4 == 0: every pass
+// reads it from memory.
+template <int C4>
 __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ scores, int64_t n_item,
                                                    const int32_t *__restrict__ ranks, int32_t S,
                                                    float *__restrict__ table_out) {
@@ -315,10 +514,16 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
     const float *x = scores + row * n_item;
     float *T = table_out + row * (int64_t)S;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    RowRegs<(C4 > 0 ? C4 : 1)> rr;
+    if constexpr (C4 > 0) rr.load(x, n_item);
+    auto each = [&](auto &&f) {
+        if constexpr (C4 > 0) rr.for_each(f);
+        else row_for_each(x, n_item, f);
+    };
 
     // 1. range
     float mn = INFINITY, mx = -INFINITY;
-    row_for_each(x, n_item, [&](float v) { mn = fminf(mn, v); mx = fmaxf(mx, v); });
+    each([&](float v) { mn = fminf(mn, v); mx = fmaxf(mx, v); });
     wg_minmax(sm, mn, mx);
     const float lo0 = mn, hi0 = mx;
     if (!(lo0 < hi0) || isinf(lo0) || isinf(hi0)) {
@@ -332,7 +537,7 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
     const float scale1 = (float)NB1 / (hi0 - lo0);
     for (int i = tid; i < NB1; i += NT) sm.hist2[i] = 0;
     __syncthreads();
-    row_for_each(x, n_item, [&](float v) { atomicAdd(&sm.hist2[bin_f32(v, lo0, scale1, NB1)], 1u); });
+    each([&](float v) { atomicAdd(&sm.hist2[bin_f32(v, lo0, scale1, NB1)], 1u); });
     __syncthreads();
     {
         constexpr int PER1 = NB1 / NT > 0 ? NB1 / NT : 1;  // coarse bins per thread
@@ -357,12 +562,33 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
     for (int i = tid; i < NB; i += NT) sm.u.incl[i] = 0;
     for (int i = tid; i < NB / 32; i += NT) sm.tbits[i] = 0;
     __syncthreads();
-    row_for_each(x, n_item, [&](float v) { atomicAdd(&sm.u.incl[fine_bin(sm.fstart, v, lo0, scale1)], 1u); });
+    each([&](float v) { atomicAdd(&sm.u.incl[fine_bin(sm.fstart, v, lo0, scale1)], 1u); });
     __syncthreads();
-    wg_reverse_incl(sm, sm.u.incl, NB);
+    wg_reverse_incl<NB>(sm, sm.u.incl);
 
-    // bin of every target
-    for (int j = tid; j < S; j += NT) sm.tb[j] = (uint16_t)find_bin(sm.u.incl, NB, (uint32_t)ranks[j]);
+    // bin of every target: the largest b with incl[b] > r, by a fixed-length
+    // search (incl[0] > r for every target, incl[NB] = 0), the thread's
+    // targets searched side by side so their LDS reads overlap
+    {
+        constexpr int PER = SMAX / NT;
+        uint32_t r[PER];
+        int b[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int j = tid + i * NT;
+            r[i] = j < S ? (uint32_t)ranks[j] : 0u;
+            b[i] = 0;
+        }
+#pragma unroll
+        for (int step = NB / 2; step >= 1; step >>= 1)
+#pragma unroll
+            for (int i = 0; i < PER; ++i) b[i] += sm.u.incl[b[i] + step] > r[i] ? step : 0;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int j = tid + i * NT;
+            if (j < S) sm.tb[j] = (uint16_t)b[i];
+        }
+    }
     __syncthreads();
 
     // 3. distinct target bins (parallel scans): fb[f] is the first target of
@@ -437,119 +663,13 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
     __syncthreads();
     const int ngrp = sm.ngrp;
 
-    for (int g = 0; g < ngrp; ++g) {
-        const bool deep = (sm.grp[g] & DEEP) != 0;
-        const int f0 = sm.grp[g] & ~DEEP;
-        const int f1 = sm.grp[g + 1] & ~DEEP;
-        if (deep) {
-            // one bin larger than LDS: narrow it from the row in global memory
-            const int j0 = sm.fb[f0];
-            const int j1 = (f0 + 1 < nfb) ? sm.fb[f0 + 1] : S;
-            const int b = sm.tb[j0];
-            float bmn = INFINITY, bmx = -INFINITY;
-            row_for_each(x, n_item, [&](float v) {
-                if (fine_bin(sm.fstart, v, lo0, scale1) == b) { bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
-            });
-            wg_minmax(sm, bmn, bmx);
-            for (int j = j0; j < j1; ++j) {
-                const uint32_t t = (uint32_t)ranks[j] - sm.fabove[f0];
-                const float v = wg_select(sm, x, n_item, bmn, bmx, t);
-                if (tid == 0) T[j] = v;
-            }
-            continue;
-        }
-        // member bins: bitmap bits, then segment starts (bins descending in f)
-        for (int f = f0 + tid; f < f1; f += NT) {
-            const int b = sm.tb[sm.fb[f]];
-            atomicOr(&sm.tbits[b >> 5], 1u << (b & 31));
-        }
-        if (ngrp > 1) {
-            // several passes: offsets restart at each group's first bin
-            __syncthreads();
-            const uint32_t base = sm.fcur[f0];
-            __syncthreads();
-            for (int f = f0 + tid; f < f1; f += NT) sm.fcur[f] -= base;
-        }
-        if (tid == 0) sm.ncoop = 0;
-        __syncthreads();
-        {
-            // member bins below each bitmap word (ascending bins)
-            const uint32_t cnt = tid < NB / 32 ? (uint32_t)__popc(sm.tbits[tid]) : 0u;
-            uint32_t tot;
-            const uint32_t pre = wg_excl_scan(sm, cnt, &tot);
-            if (tid < NB / 32) sm.tpre[tid] = (uint16_t)pre;
-        }
-        __syncthreads();
-        // gather: the member bin of rank m in ascending bin order is f1 - 1 - m
-        row_for_each(x, n_item, [&](float v) {
-            const int b = fine_bin(sm.fstart, v, lo0, scale1);
-            const uint32_t word = sm.tbits[b >> 5], bit = 1u << (b & 31);
-            if (word & bit) {
-                const int m = (int)sm.tpre[b >> 5] + __popc(word & (bit - 1u));
-                sm.u.gbuf[atomicAdd(&sm.fcur[f1 - 1 - m], 1u)] = v;
-            }
-        });
-        __syncthreads();
-        // small bins: one thread each
-        for (int f = f0 + tid; f < f1; f += NT) {
-            const uint32_t h = sm.fh[f];
-            if (h > THREAD_SEL_MAX) continue;
-            const int j0 = sm.fb[f];
-            const int j1 = (f + 1 < nfb) ? sm.fb[f + 1] : S;
-            thread_select(sm.u.gbuf, sm.fcur[f] - h, h, sm.fabove[f], ranks, j0, j1, T);
-        }
-        // mid-sized bins: one wavefront each
-        for (int f = f0 + w; f < f1; f += NW) {
-            const uint32_t h = sm.fh[f];
-            if (h <= THREAD_SEL_MAX) continue;
-            const int j0 = sm.fb[f];
-            const int j1 = (f + 1 < nfb) ? sm.fb[f + 1] : S;
-            const uint32_t seg = sm.fcur[f] - h, above = sm.fabove[f];
-            if (h > WAVE_SEL_MAX) {
-                if (lane == 0) sm.coop[atomicAdd(&sm.ncoop, 1)] = (uint16_t)f;
-                continue;
-            }
-            if (h <= 64) { wave_sort_select<1>(sm.u.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
-            if (h <= 128) { wave_sort_select<2>(sm.u.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
-            if (h <= 256) { wave_sort_select<4>(sm.u.gbuf, seg, h, above, ranks, j0, j1, T); continue; }
-            // larger bins (rare): rank each element by counting
-            for (uint32_t i = lane; i < h; i += 64) {
-                const float e = sm.u.gbuf[seg + i];
-                uint32_t pos = 0;
-                for (uint32_t k = 0; k < h; ++k) {
-                    const float o = sm.u.gbuf[seg + k];
-                    pos += (o > e || (o == e && k < i)) ? 1u : 0u;
-                }
-                const uint32_t r = above + pos;
-                for (int j = j0; j < j1; ++j)
-                    if ((uint32_t)ranks[j] == r) T[j] = e;
-            }
-        }
-        __syncthreads();
-        // bins too large for a wavefront: the whole workgroup
-        const int ncoop = sm.ncoop;
-        for (int c = 0; c < ncoop; ++c) {
-            const int f = sm.coop[c];
-            const int j0 = sm.fb[f];
-            const int j1 = (f + 1 < nfb) ? sm.fb[f + 1] : S;
-            const uint32_t h = sm.fh[f];
-            const uint32_t seg = sm.fcur[f] - h;
-            float bmn = INFINITY, bmx = -INFINITY;
-            for (uint32_t i = tid; i < h; i += NT) { const float v = sm.u.gbuf[seg + i]; bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
-            wg_minmax(sm, bmn, bmx);
-            for (int j = j0; j < j1; ++j) {
-                const uint32_t t = (uint32_t)ranks[j] - sm.fabove[f];
-                const float v = wg_select(sm, sm.u.gbuf + seg, h, bmn, bmx, t);
-                if (tid == 0) T[j] = v;
-            }
-        }
-        // release the member bins
-        for (int f = f0 + tid; f < f1; f += NT) {
-            const int b = sm.tb[sm.fb[f]];
-            sm.tbits[b >> 5] = 0;
-        }
-        __syncthreads();
-    }
+    // the first group reads the row from wherever the kernel holds it; any
+    // later one (rows whose target bins exceed LDS) reads it from memory, so
+    // the registers of a register-held row are free after the first gather
+    if (ngrp > 0) select_group(sm, 0, ngrp, nfb, x, n_item, ranks, S, T, lo0, scale1, each);
+    for (int g = 1; g < ngrp; ++g)
+        select_group(sm, g, ngrp, nfb, x, n_item, ranks, S, T, lo0, scale1,
+                     [&](auto &&f) { row_for_each(x, n_item, f); });
 }
 
 }  // namespace
@@ -558,7 +678,15 @@ hipError_t launch_select_rows(hipStream_t s, const float *scores, int64_t nb, in
                               int32_t S, float *table_out) {
     if (nb <= 0) return hipSuccess;
     if (S > SMAX || S <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_select_rows, dim3((unsigned)nb), dim3(NT), 0, s, scores, n_item, ranks, S, table_out);
+    // rows of a multiple of 4 values (every row then starts 16-byte aligned)
+    // up to 16 * C4 * NT values go through registers
+    const bool regs = (n_item & 3) == 0 && (reinterpret_cast<uintptr_t>(scores) & 15) == 0;
+    const int64_t per = (n_item / 4 + NT - 1) / NT;  // float4s per thread
+    const dim3 grid((unsigned)nb), block(NT);
+    if (regs && per <= 4) hipLaunchKernelGGL(k_select_rows<4>, grid, block, 0, s, scores, n_item, ranks, S, table_out);
+    else if (regs && per <= 8) hipLaunchKernelGGL(k_select_rows<8>, grid, block, 0, s, scores, n_item, ranks, S, table_out);
+    else if (regs && per <= 13) hipLaunchKernelGGL(k_select_rows<13>, grid, block, 0, s, scores, n_item, ranks, S, table_out);
+    else hipLaunchKernelGGL(k_select_rows<0>, grid, block, 0, s, scores, n_item, ranks, S, table_out);
     return hipGetLastError();
 }
 

@@ -121,8 +121,9 @@ struct Ctx {
     int32_t *qcol = nullptr;    // n_sample
     float *qtable = nullptr;    // [u_cap][n_qs]
     std::vector<int32_t> h_qranks;
-    // guard values of the coarse bounds' first pass: [u_cap][8], the values of
-    // the searched table at guard_columns(), -inf past them; kept with the table
+    // guard rows of the coarse bounds: [u_cap][kGuardRow], the values of the
+    // searched table at the columns of every guard set, -inf past each set's
+    // length; kept with the table
     float *gvals = nullptr;
     int dense_top = 2;          // query table mode (RKR_OPT_DENSE_TOP)
 

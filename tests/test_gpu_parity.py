@@ -35,6 +35,8 @@ def test_table_fake30d_matches_oracle(orc, fake30d, gpu_index_factory):
     (257, 4099, 7, 100, 2),      # ragged: I, d not multiples of 4/128
     (130, 20000, 150, 512, 3),   # plain-GPU row width
     (64, 50001, 33, 512, 4),     # row larger than one LDS gather pass
+    (64, 50000, 33, 512, 8),     # 50k-value rows held in registers by the selection
+    (48, 60000, 20, 512, 9),     # aligned rows too long for registers: read from memory
     (5, 3, 4, 2, 5),             # tiny
     (20000, 2000, 8, 512, 6),    # 41 MB table: two 32 MB host staging chunks, the second partial
 ])
