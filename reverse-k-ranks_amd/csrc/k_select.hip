@@ -9,18 +9,23 @@
 // r-th order statistic is unique as a value, so no full sort is needed.
 //
 // One 1024-thread workgroup per row, all state in LDS (up to 160 KB on
-// gfx950):
+// gfx950).  Rows of up to 53,248 values (a multiple of 4) are read from
+// memory once, into registers; every later pass runs from there.
 //   1. min/max of the row;
-//   2. histogram over NB linear value bins (FP32 arithmetic: fl(x - lo) and
-//      fl(t * scale) are monotone in x, so the bin is too); a reverse prefix
-//      sum tells, for every r_j, its bin and the elements above that bin;
+//   2. a coarse histogram of 1024 linear bins apportions NB level-0 bins to
+//      the coarse bins by their element counts (narrow bins where the row is
+//      dense), then a histogram over the level-0 bins (FP32 arithmetic:
+//      fl(x - lo) and fl(t * scale) are monotone in x, so the bin is too); a
+//      reverse prefix sum tells, for every r_j, its bin and the elements
+//      above that bin;
 //   3. one more pass over the row gathers the elements of the bins holding
 //      targets into LDS (the histogram's storage), with a bitmap of target
 //      bins deciding membership -- several passes only when they exceed it;
-//   4. each bin is ranked by one thread (<= 16 elements), one wavefront
-//      (register bitonic sort up to 256, counting above) or the whole
-//      workgroup (wg_select, which narrows the bin by repeated histograms --
-//      this also makes ties and skewed rows exact).
+//   4. bins of up to 64 elements are ranked element-parallel (each element
+//      counts the elements of its bin that order before it), larger ones by
+//      one wavefront (register bitonic sort up to 256, counting above) or
+//      the whole workgroup (wg_select, which narrows the bin by repeated
+//      histograms -- this also makes ties and skewed rows exact).
 #include "rkr_internal.hpp"
 
 namespace rkr {
@@ -72,18 +77,34 @@ __device__ __forceinline__ int bin_f32(float x, float lo, float scale, int nb) {
     return (int)fminf(fmaxf(t, 0.0f), (float)(nb - 1));
 }
 
-// Level-0 bin of x: its coarse bin c (NB1 linear bins over [lo, hi]) owns
-// level-0 bins fstart[c] .. fstart[c + 1] - 1, in proportion to the row's
-// elements in c, and x takes the one its place inside c falls in.  Monotone
-// in x: t is (see bin_f32), so are c and, within one c, t - c.
-// t is clamped to [0, NB1] (NaN to 0): below the range c = 0 with frac 0,
-// at or past its end c = NB1 - 1 with frac 1.
-__device__ __forceinline__ int fine_bin(const uint16_t *fstart, float x, float lo, float scale1) {
-    const float t = fminf(fmaxf((x - lo) * scale1, 0.0f), (float)NB1);
-    const int c = min((int)t, NB1 - 1);
-    const float frac = t - (float)c;
-    const int s0 = fstart[c], n = (int)fstart[c + 1] - s0;  // n >= 1
-    return s0 + min(n - 1, (int)(frac * (float)n));
+// Level-0 bins of N values.  The coarse bin c of x (NB1 linear bins over
+// [lo, hi]) owns level-0 bins fstart[c] .. fstart[c + 1] - 1, in proportion
+// to the row's elements in c, and x takes the one its place inside c falls
+// in.  Monotone in x: t is (see bin_f32), so are c and, within one c, t - c.
+// t is clamped to [0, NB1] (NaN to 0): below the range c = 0 with frac 0, at
+// or past its end c = NB1 - 1 with frac 1.  The LDS reads of all N values are
+// issued together.  (The two 16-bit reads of a value fall in one or two
+// adjacent banks; one 8-byte entry per coarse bin was slower for its bank
+// conflicts.)
+template <int N>
+__device__ __forceinline__ void fine_bins(const uint16_t *fstart, const float (&v)[N], float lo, float scale1,
+                                          int (&b)[N]) {
+    int c[N];
+    float frac[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const float t = fminf(fmaxf((v[q] - lo) * scale1, 0.0f), (float)NB1);
+        c[q] = min((int)t, NB1 - 1);
+        frac[q] = t - (float)c[q];
+    }
+    int s0[N], s1[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) { s0[q] = fstart[c[q]]; s1[q] = fstart[c[q] + 1]; }
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const int n = s1[q] - s0[q];  // >= 1
+        b[q] = s0[q] + min(n - 1, (int)(frac[q] * (float)n));
+    }
 }
 
 // A row held in registers: thread t owns float4 t + k*NT for k < C4 (rows of
@@ -104,11 +125,15 @@ struct RowRegs {
 #pragma unroll
         for (int k = 0; k < C4; ++k) r[k] = x4[min(t + k * NT, max(n4 - 1, 0))];
     }
+    // f(const float (&v)[4]) for every float4 the thread owns
     template <class F>
     __device__ __forceinline__ void for_each(F &&f) const {
 #pragma unroll
         for (int k = 0; k < C4; ++k)
-            if (k < nv) { f(r[k].x); f(r[k].y); f(r[k].z); f(r[k].w); }
+            if (k < nv) {
+                const float v[4] = {r[k].x, r[k].y, r[k].z, r[k].w};
+                f(v);
+            }
     }
 };
 
@@ -123,9 +148,10 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// Calls f(v) for every element of row x[0..n), in no particular order.
-// Aligned rows use 16-byte loads with four in flight per thread, so a pass
-// over a row is not a chain of dependent load latencies.
+// Visits every element of row x[0..n), in no particular order, in small
+// batches: f(const float (&v)[N]) with N = 4 (aligned rows, 16-byte loads,
+// four in flight per thread) or N = 1.  A batch lets f issue the LDS reads
+// of all its values before it waits for any of them.
 template <class F>
 __device__ __forceinline__ void row_for_each(const float *__restrict__ x, int64_t n, F &&f) {
     if ((n & 3) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0) {
@@ -134,17 +160,20 @@ __device__ __forceinline__ void row_for_each(const float *__restrict__ x, int64_
         int64_t i = threadIdx.x;
         for (; i + 3 * NT < n4; i += 4 * NT) {
             const float4 a = x4[i], b = x4[i + NT], c = x4[i + 2 * NT], d = x4[i + 3 * NT];
-            f(a.x); f(a.y); f(a.z); f(a.w);
-            f(b.x); f(b.y); f(b.z); f(b.w);
-            f(c.x); f(c.y); f(c.z); f(c.w);
-            f(d.x); f(d.y); f(d.z); f(d.w);
+            const float va[4] = {a.x, a.y, a.z, a.w}, vb[4] = {b.x, b.y, b.z, b.w};
+            const float vc[4] = {c.x, c.y, c.z, c.w}, vd[4] = {d.x, d.y, d.z, d.w};
+            f(va); f(vb); f(vc); f(vd);
         }
         for (; i < n4; i += NT) {
             const float4 a = x4[i];
-            f(a.x); f(a.y); f(a.z); f(a.w);
+            const float va[4] = {a.x, a.y, a.z, a.w};
+            f(va);
         }
     } else {
-        for (int64_t i = threadIdx.x; i < n; i += NT) f(x[i]);
+        for (int64_t i = threadIdx.x; i < n; i += NT) {
+            const float v[1] = {x[i]};
+            f(v);
+        }
     }
 }
 
@@ -359,8 +388,13 @@ __device__ __forceinline__ void select_group(SelSmem &sm, int g, int ngrp, int n
         const int j1 = (f0 + 1 < nfb) ? sm.fb[f0 + 1] : S;
         const int b = sm.tb[j0];
         float bmn = INFINITY, bmx = -INFINITY;
-        each([&](float v) {
-            if (fine_bin(sm.fstart, v, lo0, scale1) == b) { bmn = fminf(bmn, v); bmx = fmaxf(bmx, v); }
+        each([&](const auto &v) {
+            constexpr int N = sizeof(v) / sizeof(float);
+            int bv[N];
+            fine_bins<N>(sm.fstart, v, lo0, scale1, bv);
+#pragma unroll
+            for (int q = 0; q < N; ++q)
+                if (bv[q] == b) { bmn = fminf(bmn, v[q]); bmx = fmaxf(bmx, v[q]); }
         });
         wg_minmax(sm, bmn, bmx);
         for (int j = j0; j < j1; ++j) {
@@ -394,13 +428,27 @@ __device__ __forceinline__ void select_group(SelSmem &sm, int g, int ngrp, int n
     }
     __syncthreads();
     // gather: the member bin of rank m in ascending bin order is f1 - 1 - m
-    each([&](float v) {
-        const int b = fine_bin(sm.fstart, v, lo0, scale1);
-        const uint32_t word = sm.tbits[b >> 5], bit = 1u << (b & 31);
-        if (word & bit) {
-            const int m = (int)sm.tpre[b >> 5] + __popc(word & (bit - 1u));
-            sm.u.gbuf[atomicAdd(&sm.fcur[f1 - 1 - m], 1u)] = v;
+    // A batch's bins, bitmap words and prefixes are read together, and its
+    // members' slots are all claimed before the first is written.
+    each([&](const auto &v) {
+        constexpr int N = sizeof(v) / sizeof(float);
+        int b[N];
+        fine_bins<N>(sm.fstart, v, lo0, scale1, b);
+        uint32_t word[N], pre[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) { word[q] = sm.tbits[b[q] >> 5]; pre[q] = sm.tpre[b[q] >> 5]; }
+        uint32_t slot[N];
+        bool mem[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            const uint32_t bit = 1u << (b[q] & 31);
+            mem[q] = (word[q] & bit) != 0;
+            slot[q] = 0;
+            if (mem[q]) slot[q] = atomicAdd(&sm.fcur[f1 - 1 - ((int)pre[q] + __popc(word[q] & (bit - 1u)))], 1u);
         }
+#pragma unroll
+        for (int q = 0; q < N; ++q)
+            if (mem[q]) sm.u.gbuf[slot[q]] = v[q];
     });
     __syncthreads();
     // Bins of up to EMAX elements are ranked element-parallel: each gathered
@@ -523,7 +571,9 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
 
     // 1. range
     float mn = INFINITY, mx = -INFINITY;
-    each([&](float v) { mn = fminf(mn, v); mx = fmaxf(mx, v); });
+    each([&](const auto &v) {
+        for (float e : v) { mn = fminf(mn, e); mx = fmaxf(mx, e); }
+    });
     wg_minmax(sm, mn, mx);
     const float lo0 = mn, hi0 = mx;
     if (!(lo0 < hi0) || isinf(lo0) || isinf(hi0)) {
@@ -537,7 +587,9 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
     const float scale1 = (float)NB1 / (hi0 - lo0);
     for (int i = tid; i < NB1; i += NT) sm.hist2[i] = 0;
     __syncthreads();
-    each([&](float v) { atomicAdd(&sm.hist2[bin_f32(v, lo0, scale1, NB1)], 1u); });
+    each([&](const auto &v) {
+        for (float e : v) atomicAdd(&sm.hist2[bin_f32(e, lo0, scale1, NB1)], 1u);
+    });
     __syncthreads();
     {
         constexpr int PER1 = NB1 / NT > 0 ? NB1 / NT : 1;  // coarse bins per thread
@@ -562,7 +614,13 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
     for (int i = tid; i < NB; i += NT) sm.u.incl[i] = 0;
     for (int i = tid; i < NB / 32; i += NT) sm.tbits[i] = 0;
     __syncthreads();
-    each([&](float v) { atomicAdd(&sm.u.incl[fine_bin(sm.fstart, v, lo0, scale1)], 1u); });
+    each([&](const auto &v) {
+        constexpr int N = sizeof(v) / sizeof(float);
+        int b[N];
+        fine_bins<N>(sm.fstart, v, lo0, scale1, b);
+#pragma unroll
+        for (int q = 0; q < N; ++q) atomicAdd(&sm.u.incl[b[q]], 1u);
+    });
     __syncthreads();
     wg_reverse_incl<NB>(sm, sm.u.incl);
 
