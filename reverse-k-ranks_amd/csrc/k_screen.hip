@@ -468,23 +468,42 @@ struct IpCoarseParams {
 
 constexpr int kMaxGuardSets = 7;
 
-// #{j < 7 : g[j] > x} for g[0] >= g[1] >= ... >= g[6] (-inf padded past G):
-// the predicate g[j] > x is monotone in j, so three tests of a branchless
-// binary search decide the count -- g[3] splits {0..3} from {4..7}, then
-// g[5] or g[1], then one of g[6], g[4], g[2], g[0].
-__device__ __forceinline__ uint32_t guards_above(const float (&g)[8], float x) {
-    const bool b2 = g[3] > x;
-    const float p1 = __builtin_unpredictable(b2) ? g[5] : g[1];
-    const bool b1 = p1 > x;
-    const float pa = __builtin_unpredictable(b1) ? g[6] : g[4];
-    const float pb = __builtin_unpredictable(b1) ? g[2] : g[0];
-    const float p0 = __builtin_unpredictable(b2) ? pa : pb;
-    const bool b0 = p0 > x;
-    return (b2 ? 4u : 0u) | (b1 ? 2u : 0u) | (b0 ? 1u : 0u);
+// t + t + b for a lane predicate b: one v_addc_co_u32 with b's lane mask as
+// the carry-in (the ballot of a compare is that compare's mask, so no
+// 0/1 value is materialised first).
+__device__ __forceinline__ uint32_t twice_plus(uint32_t t, bool b) {
+    uint32_t r;
+    uint64_t carry_out;
+    asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(t), "s"(__builtin_amdgcn_ballot_w64(b)));
+    return r;
+}
+
+// c_lo | c_hi << 4 with c = #{j < 7 : g[j] > x} for x = xl and x = xh, over
+// g[0] >= g[1] >= ... >= g[6] (-inf padded past G).  The predicate g[j] > x
+// is monotone in j, so three tests of a branchless binary search decide each
+// count, c = 4 b2 + 2 b1 + b0: g[3] splits {0..3} from {4..7}, then g[5] or
+// g[1], then one of g[6], g[4], g[2], g[0].  The byte is assembled from the
+// six test bits, high bit first, by doubling with carry-in (bit 3 stays 0).
+__device__ __forceinline__ uint32_t guard_byte(const float (&g)[8], float xl, float xh) {
+    const bool l2 = g[3] > xl, h2 = g[3] > xh;
+    const float lp1 = __builtin_unpredictable(l2) ? g[5] : g[1];
+    const float hp1 = __builtin_unpredictable(h2) ? g[5] : g[1];
+    const bool l1 = lp1 > xl, h1 = hp1 > xh;
+    const float lpa = __builtin_unpredictable(l1) ? g[6] : g[4], lpb = __builtin_unpredictable(l1) ? g[2] : g[0];
+    const float hpa = __builtin_unpredictable(h1) ? g[6] : g[4], hpb = __builtin_unpredictable(h1) ? g[2] : g[0];
+    const bool l0 = (__builtin_unpredictable(l2) ? lpa : lpb) > xl;
+    const bool h0 = (__builtin_unpredictable(h2) ? hpa : hpb) > xh;
+    uint32_t t = h2 ? 1u : 0u;
+    t = twice_plus(t, h1);
+    t = twice_plus(t, h0);
+    t = t + t;
+    t = twice_plus(t, l2);
+    t = twice_plus(t, l1);
+    return twice_plus(t, l0);
 }
 
 template <int KS, bool SETS>
-__global__ void __launch_bounds__(NT, SETS ? 2 : (KS >= 7 ? 3 : 4)) k_ip_coarse(IpCoarseParams P) {
+__global__ void __launch_bounds__(NT, SETS ? 2 : coarse_first_pass_wgs(KS)) k_ip_coarse(IpCoarseParams P) {
     // first pass: one product, so only the queries' hi halves are staged and
     // only the users' hi halves are held (4 workgroups per CU fit)
     constexpr bool ONE = !SETS;
@@ -565,16 +584,28 @@ __global__ void __launch_bounds__(NT, SETS ? 2 : (KS >= 7 ? 3 : 4)) k_ip_coarse(
         }
         for (int32_t q0 = 0; q0 < P.nq; q0 += QST) {
             __syncthreads();  // every wave is done with the previous stage
-            for (int c = threadIdx.x; c < NH * QST * KQ; c += NT) {
-                const int half = c / (QST * KQ);
-                const int rem = c % (QST * KQ);
-                const int qq = q0 + rem / KQ;
-                uint4 v = make_uint4(0, 0, 0, 0);
-                if (qq < P.nq) {
-                    const int64_t qr = P.qmap ? P.qmap[qq] : qq;
-                    v = reinterpret_cast<const uint4 *>(half ? P.q_lo : P.q_hi)[qr * KQ + rem % KQ];
+            {
+                // the stage's query rows: KS 16-byte pieces per thread, all
+                // loads in flight before the first LDS write (rows past nq
+                // load the last query and are zeroed after the load)
+                static_assert(NH * QST * KQ == KS * NT, "one piece per thread and k-step");
+                uint4 v[KS];
+                int dst[KS];
+                bool ok[KS];
+#pragma unroll
+                for (int k = 0; k < KS; ++k) {
+                    const int c = (int)threadIdx.x + k * NT;
+                    const int half = c / (QST * KQ);
+                    const int rem = c % (QST * KQ);
+                    const int qq = q0 + rem / KQ;
+                    ok[k] = qq < P.nq;
+                    const int qc = min(qq, P.nq - 1);
+                    const int64_t qr = P.qmap ? P.qmap[qc] : qc;
+                    v[k] = reinterpret_cast<const uint4 *>(half ? P.q_lo : P.q_hi)[qr * KQ + rem % KQ];
+                    dst[k] = half * (QST * ROWQ) + (rem / KQ) * ROWQ + rem % KQ;
                 }
-                sA[half][(rem / KQ) * ROWQ + rem % KQ] = v;
+#pragma unroll
+                for (int k = 0; k < KS; ++k) (&sA[0][0])[dst[k]] = ok[k] ? v[k] : make_uint4(0, 0, 0, 0);
             }
             if (threadIdx.x < QST) {
                 const int qq = q0 + (int)threadIdx.x;
@@ -620,8 +651,7 @@ __global__ void __launch_bounds__(NT, SETS ? 2 : (KS >= 7 ? 3 : 4)) k_ip_coarse(
                             gd[b][4] = x1.x; gd[b][5] = x1.y; gd[b][6] = x1.z; gd[b][7] = x1.w;
                         }
                         const float e = __fmaf_rn(ar[b], qn, P.abs_floor);
-                        out[b * 16] = (uint8_t)(guards_above(gd[b], acc[b][i] - e) |
-                                                (guards_above(gd[b], acc[b][i] + e) << 4));
+                        out[b * 16] = (uint8_t)guard_byte(gd[b], acc[b][i] - e, acc[b][i] + e);
                     }
                 }
             }

@@ -1493,7 +1493,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             L.q_hi = q_hi; L.q_lo = q_lo; L.q_norm = qn; L.qmap = nullptr; L.nq = nq; L.ld = ld; L.K = d;
             L.Kp = c->kp; L.C = c->coarse_C1; L.abs_floor = std::nextafter((float)absf, INFINITY); L.cb = cb;
             L.ldc = ldc; L.spare_row = nqb;
-            L.grid_cap = (int64_t)c->n_cu * (c->kp / 32 >= 7 ? 3 : 4);  // the first pass's workgroups per CU
+            L.grid_cap = (int64_t)c->n_cu * coarse_first_pass_wgs(c->kp / 32);
             RKR_HIP(c, launch_ip_coarse(sl.stream, L));
             RKR_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
             // coarse chunks: a fixed user count, so launches over few queries

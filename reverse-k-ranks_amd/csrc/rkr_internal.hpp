@@ -223,6 +223,10 @@ struct GuardCols {
 // has no column of its own).  Each user keeps the values of all of them in one
 // row of kGuardRow floats, set s at floats 8 s .. 8 s + 7 (-inf past its n).
 constexpr int kGuardSets = 8;
+// Workgroups per CU of the coarse first pass (256 threads, users' BF16 hi
+// halves in VGPRs) for Kp = 32 ks: its launch bound and its persistent grid.
+// (Five fit at ks <= 5 only by spilling the staged query pieces: slower.)
+constexpr int coarse_first_pass_wgs(int ks) { return ks >= 7 ? 3 : 4; }
 constexpr int kGuardRow = 8 * kGuardSets;
 struct GuardSets {
     GuardCols set[kGuardSets];
