@@ -7,8 +7,11 @@
 //
 // One 1024-thread workgroup per row.  Rows of up to NMAX values are sorted in LDS (bucket path):
 //   1. the row is loaded into registers (coalesced), with its min and max;
-//   2. a histogram over NBK linear value bins, descending (FP32 arithmetic: fl(max - x) and
-//      fl(t * scale) are monotone in x, so bin order is value order), and its exclusive scan;
+//   2. a histogram over NBK value bins, descending, and its exclusive scan.  The bins are
+//      apportioned to NB1 linear coarse bins by the coarse bins' counts (a first histogram), so
+//      bins are narrow where the row is dense and hold about n / NBK values each even when a
+//      heavy tail crowds most of the row into a few linear bins.  FP32 arithmetic: fl(max - x)
+//      and fl(t * scale) are monotone in x, so bin order is value order;
 //   3. every value is scattered into its bin's segment of an LDS buffer (any order inside a bin);
 //   4. each value's final position is its bin's start plus the values of its bin above it (and the
 //      equal ones before it): threads walk the buffer in order, so the lanes of a wavefront read the
@@ -34,24 +37,31 @@ constexpr int NBK = 8192;                // value bins of the bucket path
 constexpr int NMAX = 28672;              // longest row of the bucket path (112 KB of LDS)
 constexpr int EPT = NMAX / NT;           // values per thread
 constexpr uint32_t BIN_MAX = 192;        // largest bin the bucket path ranks by counting
+constexpr int NB1 = 1024;                // linear coarse bins that apportion the NBK bins
 
 struct SortSmem {
     union {
         float tile[TILE];                // merge path
         struct {
             float buf[NMAX];             // the row, bin by bin, then in order
-            uint32_t cnt[NBK];           // bin counts, then bin ends (32 KB)
+            uint32_t cnt[NBK];           // coarse counts, then bin counts, then bin ends (32 KB)
+            uint16_t fstart[NB1 + 1];    // first bin of each coarse bin (NBK at the end)
             float red_f[2 * NW];
             uint32_t red_u[NW + 1];
         } b;
     } u;
 };
 
-__device__ __forceinline__ int sort_bin(float x, float hi, float scale) {
-    const float t = (hi - x) * scale;
-    if (!(t >= 0.0f)) return 0;          // NaN from 0 * inf at x == hi
-    if (!(t < (float)NBK)) return NBK - 1;
-    return (int)t;
+// Bin of x, descending: its coarse bin c (NB1 linear bins from hi down)
+// owns bins fstart[c] .. fstart[c + 1] - 1, and x takes the one its place
+// inside c falls in.  t is clamped to [0, NB1] (NaN from 0 * inf at x == hi
+// to 0), so c = NB1 - 1 with frac 1 at and past the range's end.
+__device__ __forceinline__ int sort_bin(const uint16_t *fstart, float x, float hi, float scale1) {
+    const float t = fminf(fmaxf((hi - x) * scale1, 0.0f), (float)NB1);
+    const int c = min((int)t, NB1 - 1);
+    const float frac = t - (float)c;
+    const int s0 = fstart[c], nc = (int)fstart[c + 1] - s0;  // nc >= 1
+    return s0 + min(nc - 1, (int)(frac * (float)nc));
 }
 
 // Sorts s[0..TILE) in descending order.  Stage (k, j) compares i with i + j for
@@ -136,10 +146,43 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
         }
         return true;
     }
-    const float scale = (float)NBK / (mx - mn);
+    const float scale = (float)NB1 / (mx - mn);
+    // coarse histogram (cnt was zeroed above), then each coarse bin c gets
+    // bins fstart[c] .. fstart[c + 1] - 1: one plus its share of the other
+    // NBK - NB1 by its count
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
-        if (e < ne && (int)threadIdx.x + e * NT < n) atomicAdd(&sm.u.b.cnt[sort_bin(v[e], mx, scale)], 1u);
+        if (e < ne && (int)threadIdx.x + e * NT < n)
+            atomicAdd(&sm.u.b.cnt[(int)fminf(fmaxf((mx - v[e]) * scale, 0.0f), (float)(NB1 - 1))], 1u);
+    __syncthreads();
+    {
+        static_assert(NB1 == NT, "one coarse bin per thread");
+        const uint32_t cc = sm.u.b.cnt[threadIdx.x];
+        uint32_t incl = cc;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) sm.u.b.red_u[w] = incl;
+        __syncthreads();
+        uint32_t below = incl - cc;
+        for (int i = 0; i < w; ++i) below += sm.u.b.red_u[i];
+        sm.u.b.fstart[threadIdx.x] =
+            (uint16_t)(threadIdx.x + (uint32_t)(((uint64_t)(NBK - NB1) * below) / (uint64_t)n));
+        if (threadIdx.x == 0) sm.u.b.fstart[NB1] = (uint16_t)NBK;
+        __syncthreads();  // every thread has read its coarse count
+    }
+    for (int b = threadIdx.x; b < NBK; b += NT) sm.u.b.cnt[b] = 0;
+    __syncthreads();
+    // every value's bin, computed once: its LDS reads are issued for all the
+    // thread's values before any is used
+    int bin[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) bin[e] = e < ne ? sort_bin(sm.u.b.fstart, v[e], mx, scale) : 0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e)
+        if (e < ne && (int)threadIdx.x + e * NT < n) atomicAdd(&sm.u.b.cnt[bin[e]], 1u);
     __syncthreads();
     // exclusive scan of the bin counts (4 bins per thread) and the largest bin
     constexpr int BPT = NBK / NT;
@@ -175,33 +218,62 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
         start += c[k];
     }
     __syncthreads();
+    // every slot is claimed before the first value is written
+    uint32_t slot[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        slot[e] = 0;
+        if (e < ne && (int)threadIdx.x + e * NT < n) slot[e] = atomicAdd(&sm.u.b.cnt[bin[e]], 1u);
+    }
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
-        if (e < ne && (int)threadIdx.x + e * NT < n) {
-            const uint32_t p = atomicAdd(&sm.u.b.cnt[sort_bin(v[e], mx, scale)], 1u);
-            sm.u.b.buf[p] = v[e];
-        }
+        if (e < ne && (int)threadIdx.x + e * NT < n) sm.u.b.buf[slot[e]] = v[e];
     __syncthreads();
     // final positions, in buffer order (the lanes of a wavefront read the
     // same few bins): the bin's start plus the values of the bin above this
     // one, and the equal ones before it.  The value goes straight to its
-    // place in the output row; the row's input is no longer read.
-    for (int i = threadIdx.x; i < n; i += NT) {
-        const float x0 = sm.u.b.buf[i];
-        const int b = sort_bin(x0, mx, scale);
-        const uint32_t b0 = b ? sm.u.b.cnt[b - 1] : 0u, b1 = sm.u.b.cnt[b];
-        // four LDS reads in flight per step (a bin holds 2-3 values on
-        // average); reads past the bin are clamped to the row and masked out
-        uint32_t r = b0;
-        for (uint32_t j = b0; j < b1; j += 4) {
-            float z[4];
+    // place in the output row; the row's input is no longer read.  A thread
+    // takes RB values at a time, so their bin lookups and the counting reads
+    // (four per value and step; a bin holds about n / NBK values) are in
+    // flight together; reads past a bin are clamped to the row and masked out.
+    constexpr int RB = 4;
+    for (int i0 = threadIdx.x; i0 < n; i0 += RB * NT) {
+        float x0[RB];
+        int b[RB];
+        uint32_t b0[RB], b1[RB], r[RB], len = 0;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) z[k] = sm.u.b.buf[min(j + k, (uint32_t)n - 1u)];
+        for (int q = 0; q < RB; ++q) x0[q] = sm.u.b.buf[min(i0 + q * NT, n - 1)];
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                r += (j + k < b1 && (z[k] > x0 || (z[k] == x0 && j + k < (uint32_t)i))) ? 1u : 0u;
+        for (int q = 0; q < RB; ++q) b[q] = sort_bin(sm.u.b.fstart, x0[q], mx, scale);
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            b0[q] = b[q] ? sm.u.b.cnt[b[q] - 1] : 0u;
+            b1[q] = sm.u.b.cnt[b[q]];
         }
-        y[r] = x0;
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            r[q] = b0[q];
+            len = max(len, b1[q] - b0[q]);
+        }
+        for (uint32_t j = 0; j < len; j += 4) {
+            float z[RB][4];
+#pragma unroll
+            for (int q = 0; q < RB; ++q)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) z[q][k] = sm.u.b.buf[min(b0[q] + j + k, (uint32_t)n - 1u)];
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const uint32_t i = (uint32_t)(i0 + q * NT);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t pos = b0[q] + j + k;
+                    r[q] += (pos < b1[q] && (z[q][k] > x0[q] || (z[q][k] == x0[q] && pos < i))) ? 1u : 0u;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < RB; ++q)
+            if (i0 + q * NT < n) y[r[q]] = x0[q];
     }
     return true;
 }
