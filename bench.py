@@ -310,11 +310,17 @@ def score_table(args, idx, wl, label):
     # k_sort_rows reads and writes each score once: 8*I bytes per row
     sort_gbs = 8.0 * I * rows / (tot["sort_ms"] * 1e-3) / 1e9
     gemm_tflops = 2.0 * U * I * d * args.steps / (tot["gemm_ms"] * 1e-3) / 1e12
+    # HBM bytes of one full-batch launch (26,843 rows of 20,000 scores) from the PMC counters
+    traffic = SORT_TRAFFIC_BYTES if (U, I) == (100_000, 20_000) else None
     roof = {"kernel": "k_sort_rows", "bound": "hbm", "achieved": round(sort_gbs, 1), "peak": PEAK_HBM_GBS,
-            "unit": "GB/s", "frac": round(sort_gbs / PEAK_HBM_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(sort_gbs / PEAK_HBM_GBS, 4), "traffic": traffic,
+            "traffic_source": "profiles/k_sort_rows_100k_x_20k_score_table.md (per full-batch launch)",
             "work_per_launch": "8*n_item bytes per row (each score read and written once) x rows per launch",
             "score_gemm": {"kernel": "k_scores_store", "achieved_tflops": round(gemm_tflops, 2),
-                           "peak_fp32_tflops": PEAK_FP32_TFLOPS, "note": "exact sequential FP32 rule, no FMA"}}
+                           "peak_fp32_tflops": PEAK_FP32_TFLOPS,
+                           "rule_ceiling_tflops": PEAK_FP32_TFLOPS / 2,
+                           "note": "exact sequential FP32 rule: v_mul + v_add, no FMA, so two VALU issues per "
+                                   "product and a ceiling of half the FMA-counted FP32 peak"}}
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -482,11 +488,16 @@ def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_max, seconds):
 # plain 1 GiB device copy gives, bench.py --copy-calibration).  Keyed by
 # (workload, form); other configurations report null.
 PROFILED_TRAFFIC = {
-    ("plain_gpu", "screened"): {"kernel": "k_ip_coarse", "bytes": 1.488e8, "fetch_kib": 23712.0,
+    ("plain_gpu", "screened"): {"kernel": "k_ip_coarse", "bytes": 1.486e8, "fetch_kib": 23608.4,
                                 "write_kib": 97847.8, "profile": "profiles/k_ip_coarse_100k_x_20k_d150_k50.md"},
-    ("precision", "screened"): {"kernel": "k_ip_coarse", "bytes": 1.401e9, "fetch_kib": 195471.2,
-                                "write_kib": 977638.4, "profile": "profiles/k_ip_coarse_1M_x_50k_d150_k50.md"},
+    ("precision", "screened"): {"kernel": "k_ip_coarse", "bytes": 1.492e9, "fetch_kib": 239864.4,
+                                "write_kib": 977654.9, "profile": "profiles/k_ip_coarse_1M_x_50k_d150_k50.md"},
 }
+
+
+# k_sort_rows, 100k x 20k: FETCH_SIZE 1,048,727.3 KiB (x2.0) + WRITE_SIZE 2,098,289.4 KiB (x1.0) per
+# full-batch launch (26,843 rows)
+SORT_TRAFFIC_BYTES = 4.297e9
 
 
 def copy_calibration():

@@ -13,7 +13,10 @@
 // an 8 x 8 block of a 128 x 128 tile, split as rows {4ty..4ty+3} and
 // {64+4ty..64+4ty+3} (same for columns) so that every ds_read_b128 of a
 // wave covers 256 contiguous bytes.  K advances in chunks of 16 staged
-// through LDS in k-major layout, double-buffered.
+// through LDS in k-major layout, double-buffered.  k_scores_store is held to
+// 168 VGPRs so that 3 workgroups share a CU: gfx950 issues independent FP32
+// VALU work fastest with several waves per SIMD ready (9% over 2 per SIMD).
+// A fourth does not fit without spilling the tile.
 #include "rkr_internal.hpp"
 
 namespace rkr {
@@ -111,7 +114,7 @@ __device__ __forceinline__ void tile_scores(TileSmem &sm, const float *__restric
 __device__ __forceinline__ int tile_row(int i) { return (i < 4) ? ((threadIdx.x >> 4) * 4 + i) : (64 + (threadIdx.x >> 4) * 4 + i - 4); }
 __device__ __forceinline__ int tile_col(int j) { return (j < 4) ? ((threadIdx.x & 15) * 4 + j) : (64 + (threadIdx.x & 15) * 4 + j - 4); }
 
-__global__ void __launch_bounds__(NT) k_scores_store(const float *__restrict__ A, const int32_t *__restrict__ a_rows,
+__global__ void __launch_bounds__(NT, 3) k_scores_store(const float *__restrict__ A, const int32_t *__restrict__ a_rows,
                                                      int64_t M, int32_t lda, const float *__restrict__ B, int64_t N,
                                                      int32_t ldb, int32_t K, float *__restrict__ C, int64_t ldc,
                                                      int c_row_major_m, int64_t n_tiles_n) {
