@@ -558,6 +558,23 @@ __global__ void __launch_bounds__(NT) k_topk(const QueryInfo *__restrict__ qi, c
 
 constexpr int CNT = 256;
 
+// Workgroups are dispatched to the 8 XCDs in turn (block p on XCD p % 8), and
+// each XCD has its own L2.  For a list of n_live entries in CNT-entry blocks,
+// this gives block p the entries of logical block (p % 8) * per + p / 8, so
+// each XCD takes one contiguous eighth of the list: entries close in the
+// list (the same users, for the coarse list's chunk-major order) share one
+// L2.  Returns -1 for a block past the list.  The grid must hold a multiple
+// of 8 blocks covering the largest list (xcd_grid), so that every block p <
+// 8 * per exists and the mapping is a bijection onto them.
+__device__ __forceinline__ int64_t xcd_block(int64_t n_live) {
+    const int64_t blocks = (n_live + CNT - 1) / CNT, per = (blocks + 7) / 8;
+    const int64_t p = blockIdx.x;
+    if (p >= per * 8) return -1;
+    const int64_t lb = (p % 8) * per + p / 8;
+    return lb < blocks ? lb : -1;
+}
+inline unsigned xcd_grid(int64_t n_max) { return (unsigned)(((n_max + CNT - 1) / CNT + 7) / 8 * 8); }
+
 // The score rule: FP32 products summed in ascending dimension order.
 __device__ __forceinline__ float rule_dot(const float *__restrict__ a, const float *__restrict__ b, int32_t d) {
     float s = 0.0f;
@@ -827,8 +844,10 @@ __global__ void __launch_bounds__(CNT) k_coarse_exact(const float *__restrict__ 
                                                       const int32_t *__restrict__ n_dev,
                                                       uint32_t *__restrict__ bk_c, float *__restrict__ s_c,
                                                       uint32_t *__restrict__ ghist) {
-    const int64_t i = (int64_t)blockIdx.x * CNT + threadIdx.x;
-    if (i >= (n_dev ? min(n, (int64_t)*n_dev) : n)) return;
+    const int64_t n_live = n_dev ? min(n, (int64_t)*n_dev) : n;
+    const int64_t lb = xcd_block(n_live);
+    const int64_t i = lb * CNT + threadIdx.x;
+    if (lb < 0 || i >= n_live) return;
     const int64_t u = cl_u[i];
     const uint32_t packed = (uint32_t)cl_b[i];
     const int64_t b = packed & 0xFFFFu;
@@ -869,8 +888,9 @@ __global__ void __launch_bounds__(CNT) k_coarse_exact_upd(
     const int32_t *__restrict__ cl_b, int64_t n, const float *__restrict__ ins, int32_t n_ins, int32_t ld_ins,
     const float *__restrict__ del, int32_t n_del, int32_t ld_del, int64_t n_user, int32_t *__restrict__ lbs,
     int32_t *__restrict__ ubs, float *__restrict__ qip) {
-    const int64_t i = (int64_t)blockIdx.x * CNT + threadIdx.x;
-    if (i >= n) return;
+    const int64_t lb = xcd_block(n);
+    const int64_t i = lb * CNT + threadIdx.x;
+    if (lb < 0 || i >= n) return;
     const int64_t u = cl_u[i];
     const uint32_t packed = (uint32_t)cl_b[i];
     const int64_t b = packed & 0xFFFFu;
@@ -1048,7 +1068,7 @@ hipError_t launch_coarse_exact(hipStream_t s, const float *users, const float *q
     hipError_t e = hipMemsetAsync(ghist, 0, sizeof(uint32_t) * 2 * (size_t)(S + 1) * nq, s);
     if (e != hipSuccess) return e;
     if (n > 0)
-        hipLaunchKernelGGL(k_coarse_exact, dim3((unsigned)((n + CNT - 1) / CNT)), dim3(CNT), 0, s, users, queries, ld,
+        hipLaunchKernelGGL(k_coarse_exact, dim3(xcd_grid(n)), dim3(CNT), 0, s, users, queries, ld,
                            dim, table, S, eps, same, gsets, gset, cl_u, cl_b, n, n_dev, bk_c, s_c, ghist);
     hipLaunchKernelGGL(k_hist_kth, dim3((unsigned)nq), dim3(NT), 0, s, ghist, S, same, ranks, n_item,
                        (int)RKR_OUTPUT_EXACT, qi);
@@ -1062,7 +1082,7 @@ hipError_t launch_coarse_exact_upd(hipStream_t s, const float *users, const floa
                                    const float *del, int32_t n_del, int32_t ld_del, int64_t n_user, int32_t *lbs,
                                    int32_t *ubs, float *qip) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_coarse_exact_upd, dim3((unsigned)((n + CNT - 1) / CNT)), dim3(CNT), 0, s, users, queries, ld,
+    hipLaunchKernelGGL(k_coarse_exact_upd, dim3(xcd_grid(n)), dim3(CNT), 0, s, users, queries, ld,
                        dim, table, S, ranks, n_item_base, eps, gsets, gset, cl_u, cl_b, n, ins, n_ins, ld_ins, del,
                        n_del, ld_del, n_user, lbs, ubs, qip);
     return hipGetLastError();
