@@ -230,7 +230,8 @@ __device__ __forceinline__ void wave_sort_select(const float *gbuf, uint32_t seg
     }
 }
 
-// Workgroup min/max; every thread gets the result.
+// Workgroup min/max; every thread gets the result.  One barrier: every call
+// site reaches another barrier before red_f is written again.
 __device__ __forceinline__ void wg_minmax(SelSmem &sm, float &mn, float &mx) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     mn = wave_min(mn);
@@ -239,12 +240,14 @@ __device__ __forceinline__ void wg_minmax(SelSmem &sm, float &mn, float &mx) {
     __syncthreads();
     mn = sm.red_f[0];
     mx = sm.red_f[NW];
+#pragma unroll
     for (int i = 1; i < NW; ++i) { mn = fminf(mn, sm.red_f[i]); mx = fmaxf(mx, sm.red_f[NW + i]); }
-    __syncthreads();
 }
 
 // Exclusive scan of one value per thread; returns the thread's prefix and
-// writes the total to *total.
+// writes the total to *total.  One barrier: each thread adds the wave totals
+// itself, and every call site reaches another barrier before red_u is written
+// again.
 __device__ __forceinline__ uint32_t wg_excl_scan(SelSmem &sm, uint32_t v, uint32_t *total) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t x = v;
@@ -255,16 +258,15 @@ __device__ __forceinline__ uint32_t wg_excl_scan(SelSmem &sm, uint32_t v, uint32
     }
     if (lane == 63) sm.red_u[w] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (int i = 0; i < NW; ++i) { const uint32_t t = sm.red_u[i]; sm.red_u[i] = s; s += t; }
-        sm.red_u[NW] = s;
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        const uint32_t t = sm.red_u[i];
+        before += i < w ? t : 0u;
+        tot += t;
     }
-    __syncthreads();
-    const uint32_t r = sm.red_u[w] + x - v;
-    *total = sm.red_u[NW];
-    __syncthreads();
-    return r;
+    *total = tot;
+    return before + x - v;
 }
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
