@@ -44,6 +44,15 @@ constexpr int MIDCAP = GCAP / (EMAX + 1) + 1;          // wavefront bins of one 
 constexpr int COOPCAP = GCAP / (WAVE_SEL_MAX + 1) + 1; // workgroup bins of one pass, at most
 constexpr uint16_t DEEP = 0x8000;
 
+// The target ranks: from LDS (16-bit copies, rows of up to 65,536 items) or
+// from memory.  The choice is uniform over the workgroup.
+struct Ranks {
+    const int32_t *g;
+    const uint16_t *l;  // the LDS copy
+    bool lds;
+    __device__ __forceinline__ uint32_t operator[](int j) const { return lds ? (uint32_t)l[j] : (uint32_t)g[j]; }
+};
+
 struct SelSmem {
     union {
         uint32_t incl[NB + 1];    // incl[b] = #elements in bins >= b (steps 2-3)
@@ -63,6 +72,7 @@ struct SelSmem {
     uint16_t spre[GCAP / 32];     // bins starting below each sbits word
     uint32_t hist2[NB2 + 1];
     uint16_t fstart[NB1 + 1];     // first level-0 bin of each coarse bin (NB at the end)
+    uint16_t srk[SMAX];           // the target ranks, when n_item <= 65536
     float red_f[2 * NW];
     uint32_t red_u[NW + 1];
     int32_t nfb, ngrp, ncoop, nmid, sel_b;
@@ -209,7 +219,7 @@ __device__ __forceinline__ void wave_bitonic_desc(float (&v)[E]) {
 // element at position ranks[j] - above.
 template <int E>
 __device__ __forceinline__ void wave_sort_select(const float *gbuf, uint32_t seg, uint32_t h, uint32_t above,
-                                                 const int32_t *__restrict__ ranks, int j0, int j1, float *T) {
+                                                 const Ranks &ranks, int j0, int j1, float *T) {
     const int lane = threadIdx.x & 63;
     float v[E];
 #pragma unroll
@@ -378,7 +388,7 @@ __device__ float wg_select(SelSmem &sm, const float *src, int64_t n, float lo, f
 // one bin larger than LDS narrowed on its own): `each` visits the row's values.
 template <class Each>
 __device__ __forceinline__ void select_group(SelSmem &sm, int g, int ngrp, int nfb, const float *__restrict__ x,
-                                             int64_t n_item, const int32_t *__restrict__ ranks, int32_t S, float *T,
+                                             int64_t n_item, const Ranks &ranks, int32_t S, float *T,
                                              float lo0, float scale1, Each &&each) {
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const bool deep = (sm.grp[g] & DEEP) != 0;
@@ -557,13 +567,19 @@ __device__ __forceinline__ void select_group(SelSmem &sm, int g, int ngrp, int n
 // reads it from memory.
 template <int C4>
 __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ scores, int64_t n_item,
-                                                   const int32_t *__restrict__ ranks, int32_t S,
+                                                   const int32_t *__restrict__ granks, int32_t S,
                                                    float *__restrict__ table_out) {
     __shared__ SelSmem sm;
     const int64_t row = blockIdx.x;
     const float *x = scores + row * n_item;
     float *T = table_out + row * (int64_t)S;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    // the target ranks in LDS when they fit 16 bits (first read after the
+    // barriers of the passes below)
+    const bool lds_ranks = n_item <= 65536;
+    if (lds_ranks)
+        for (int j = tid; j < S; j += NT) sm.srk[j] = (uint16_t)granks[j];
+    const Ranks ranks{granks, sm.srk, lds_ranks};
     RowRegs<(C4 > 0 ? C4 : 1)> rr;
     if constexpr (C4 > 0) rr.load(x, n_item);
     auto each = [&](auto &&f) {

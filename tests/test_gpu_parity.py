@@ -37,6 +37,8 @@ def test_table_fake30d_matches_oracle(orc, fake30d, gpu_index_factory):
     (64, 50001, 33, 512, 4),     # row larger than one LDS gather pass
     (64, 50000, 33, 512, 8),     # 50k-value rows held in registers by the selection
     (48, 60000, 20, 512, 9),     # aligned rows too long for registers: read from memory
+    (16, 65536, 12, 256, 10),    # the largest rows whose target ranks the selection keeps in LDS
+    (16, 70000, 12, 256, 11),    # ranks read from memory
     (5, 3, 4, 2, 5),             # tiny
     (20000, 2000, 8, 512, 6),    # 41 MB table: two 32 MB host staging chunks, the second partial
 ])
