@@ -71,7 +71,7 @@ struct SelSmem {
     uint32_t sbits[GCAP / 32];    // first gathered element of each bin of the pass
     uint16_t spre[GCAP / 32];     // bins starting below each sbits word
     uint32_t hist2[NB2 + 1];
-    uint16_t fstart[NB1 + 1];     // first level-0 bin of each coarse bin (NB at the end)
+    uint32_t fseg[NB1];           // level-0 bins of coarse bin c: s0 | s1 << 16, bins s0 .. s1 - 1
     uint16_t srk[SMAX];           // the target ranks, when n_item <= 65536
     float red_f[2 * NW];
     uint32_t red_u[NW + 1];
@@ -88,7 +88,8 @@ __device__ __forceinline__ int bin_f32(float x, float lo, float scale, int nb) {
 }
 
 // Level-0 bins of N values.  The coarse bin c of x (NB1 linear bins over
-// [lo, hi]) owns level-0 bins fstart[c] .. fstart[c + 1] - 1, in proportion
+// [lo, hi]) owns level-0 bins s0 .. s1 - 1 (fseg[c] = s0 | s1 << 16: one LDS
+// read), in proportion
 // to the row's elements in c, and x takes the one its place inside c falls
 // in.  Monotone in x: t is (see bin_f32), so are c and, within one c, t - c.
 // t is clamped to [0, NB1] (NaN to 0): below the range c = 0 with frac 0, at
@@ -97,7 +98,7 @@ __device__ __forceinline__ int bin_f32(float x, float lo, float scale, int nb) {
 // adjacent banks; one 8-byte entry per coarse bin was slower for its bank
 // conflicts.)
 template <int N>
-__device__ __forceinline__ void fine_bins(const uint16_t *fstart, const float (&v)[N], float lo, float scale1,
+__device__ __forceinline__ void fine_bins(const uint32_t *fseg, const float (&v)[N], float lo, float scale1,
                                           int (&b)[N]) {
     int c[N];
     float frac[N];
@@ -109,7 +110,11 @@ __device__ __forceinline__ void fine_bins(const uint16_t *fstart, const float (&
     }
     int s0[N], s1[N];
 #pragma unroll
-    for (int q = 0; q < N; ++q) { s0[q] = fstart[c[q]]; s1[q] = fstart[c[q] + 1]; }
+    for (int q = 0; q < N; ++q) {
+        const uint32_t w = fseg[c[q]];
+        s0[q] = (int)(w & 0xFFFFu);
+        s1[q] = (int)(w >> 16);
+    }
 #pragma unroll
     for (int q = 0; q < N; ++q) {
         const int n = s1[q] - s0[q];  // >= 1
@@ -403,7 +408,7 @@ __device__ __forceinline__ void select_group(SelSmem &sm, int g, int ngrp, int n
         each([&](const auto &v) {
             constexpr int N = sizeof(v) / sizeof(float);
             int bv[N];
-            fine_bins<N>(sm.fstart, v, lo0, scale1, bv);
+            fine_bins<N>(sm.fseg, v, lo0, scale1, bv);
 #pragma unroll
             for (int q = 0; q < N; ++q)
                 if (bv[q] == b) { bmn = fminf(bmn, v[q]); bmx = fmaxf(bmx, v[q]); }
@@ -445,7 +450,7 @@ __device__ __forceinline__ void select_group(SelSmem &sm, int g, int ngrp, int n
     each([&](const auto &v) {
         constexpr int N = sizeof(v) / sizeof(float);
         int b[N];
-        fine_bins<N>(sm.fstart, v, lo0, scale1, b);
+        fine_bins<N>(sm.fseg, v, lo0, scale1, b);
         uint32_t word[N], pre[N];
 #pragma unroll
         for (int q = 0; q < N; ++q) { word[q] = sm.tbits[b[q] >> 5]; pre[q] = sm.tpre[b[q] >> 5]; }
@@ -619,13 +624,15 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
         }
         uint32_t total;
         uint32_t cum = wg_excl_scan(sm, sum, &total);  // elements below the thread's first coarse bin
+        // coarse bin c starts at level-0 bin c + floor((NB - NB1) * below(c) / total); the last ends at NB
+        const uint64_t den = max(total, 1u);
         for (int i = 0; i < PER1; ++i) {
             const int c = tid * PER1 + i;
-            if (c < NB1)
-                sm.fstart[c] = (uint16_t)(c + (int)(((uint64_t)(NB - NB1) * cum) / (uint64_t)max(total, 1u)));
+            const uint32_t s0 = (uint32_t)c + (uint32_t)(((uint64_t)(NB - NB1) * cum) / den);
             cum += c_loc[i];
+            const uint32_t s1 = (uint32_t)c + 1u + (uint32_t)(((uint64_t)(NB - NB1) * cum) / den);
+            if (c < NB1) sm.fseg[c] = s0 | (s1 << 16);
         }
-        if (tid == 0) sm.fstart[NB1] = (uint16_t)NB;
     }
 
     // 2b. histogram + reverse prefix sums
@@ -635,7 +642,7 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
     each([&](const auto &v) {
         constexpr int N = sizeof(v) / sizeof(float);
         int b[N];
-        fine_bins<N>(sm.fstart, v, lo0, scale1, b);
+        fine_bins<N>(sm.fseg, v, lo0, scale1, b);
 #pragma unroll
         for (int q = 0; q < N; ++q) atomicAdd(&sm.u.incl[b[q]], 1u);
     });
