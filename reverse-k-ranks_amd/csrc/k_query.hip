@@ -610,23 +610,42 @@ __device__ __forceinline__ float rule_dot(const float *__restrict__ a, const flo
     return s;
 }
 
-// Adds the byte counters of a 64-bit word (bins 0..7, 8 bits each) into
-// four words of 16-bit counters (bins 2w, 2w + 1 in word w).
-__device__ __forceinline__ void widen_add(uint64_t x, uint32_t (&w)[4]) {
-    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    w[0] += (lo & 0xFFu) | ((lo & 0xFF00u) << 8);
-    w[1] += ((lo >> 16) & 0xFFu) | ((lo >> 8) & 0xFF0000u);
-    w[2] += (hi & 0xFFu) | ((hi & 0xFF00u) << 8);
-    w[3] += ((hi >> 16) & 0xFFu) | ((hi >> 8) & 0xFF0000u);
+// Nibble counters of 8 bytes' c_lo (low 3 bits) or c_hi (bits 4..6): bin c
+// at bits 4c..4c+3 (at most 8 per nibble).
+template <int SH>
+__device__ __forceinline__ uint32_t nibble_count8(uint32_t x0, uint32_t x1) {
+    uint32_t n = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        n += 1u << (__builtin_amdgcn_ubfe(x0, 8 * j + SH, 3) << 2);
+        n += 1u << (__builtin_amdgcn_ubfe(x1, 8 * j + SH, 3) << 2);
+    }
+    return n;
+}
+
+// Adds two nibble-counter words into byte counters: even bins (0, 2, 4, 6)
+// in the bytes of e, odd bins in the bytes of o.
+__device__ __forceinline__ void nibbles_to_bytes(uint32_t a, uint32_t b, uint32_t &e, uint32_t &o) {
+    e += (a & 0x0F0F0F0Fu) + (b & 0x0F0F0F0Fu);
+    o += ((a >> 4) & 0x0F0F0F0Fu) + ((b >> 4) & 0x0F0F0F0Fu);
+}
+
+// Adds byte counters (even bins in e, odd bins in o) into four words of
+// 16-bit counters (bins 2w, 2w + 1 in word w).
+__device__ __forceinline__ void widen_add(uint32_t e, uint32_t o, uint32_t (&w)[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] += ((e >> (8 * k)) & 0xFFu) | (((o >> (8 * k)) & 0xFFu) << 16);
 }
 
 // grid (chunks, queries): counts of c_lo and c_hi values 0..6 of the chunk's
 // users, part[(b * n_chunk + chunk) * 16 + {0..6 | 8..14}] (b = qmap[y] when
-// given).  16 users per thread and load (rows of cb are ldc bytes,
-// ldc % 16 == 0, chunks start at multiples of 1024).  Per-thread counters
-// are bytes of a 64-bit word, widened to 16 bits before they can carry; the
-// workgroup sums them by wave reductions (cu < 2^18 keeps the sums in 16
-// bits).
+// given).  16 users per thread and load, four loads in flight (rows of cb
+// are ldc bytes, ldc % 16 == 0, chunks start at multiples of 1024).  Each
+// 16-byte load is counted into nibble counters (8 bytes per word, so no
+// nibble passes 8), added into byte counters (at most 16 per load), and those
+// are widened to 16 bits every 15 loads, before they can carry; the workgroup
+// sums them by wave reductions (cu < 2^18 keeps the sums in 16 bits).  Only
+// a chunk's last load tests its bytes against the chunk's end.
 __global__ void __launch_bounds__(CNT) k_coarse_hist(const uint8_t *__restrict__ cb, int64_t ldc, int64_t n_user,
                                                      int64_t cu, int32_t n_chunk, const int32_t *__restrict__ qmap,
                                                      uint32_t *__restrict__ part) {
@@ -634,31 +653,42 @@ __global__ void __launch_bounds__(CNT) k_coarse_hist(const uint8_t *__restrict__
     const int64_t b = qmap ? qmap[blockIdx.y] : blockIdx.y;
     const uint8_t *v = cb + b * ldc;
     const int64_t u0 = (int64_t)blockIdx.x * cu, u1 = min(n_user, u0 + cu);
-    uint64_t alo = 0, ahi = 0;
+    uint32_t elo = 0, olo = 0, ehi = 0, ohi = 0;
     uint32_t wlo[4] = {0u, 0u, 0u, 0u}, whi[4] = {0u, 0u, 0u, 0u};
     int n = 0;
-    for (int64_t u = u0 + 16 * (int64_t)threadIdx.x; u < u1; u += 16 * CNT) {
-        const uint4 w = *reinterpret_cast<const uint4 *>(v + u);
-        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-        const int64_t rem = u1 - u;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            if (j < rem) {
-                const uint32_t x = ws[j >> 2] >> (8 * (j & 3));
-                alo += 1ull << ((x & 7u) * 8u);
-                ahi += 1ull << (((x >> 4) & 7u) * 8u);
-            }
-        }
-        n += 16;
-        if (n >= 240) {
-            widen_add(alo, wlo);
-            widen_add(ahi, whi);
-            alo = ahi = 0;
+    auto count16 = [&](const uint4 w) {
+        nibbles_to_bytes(nibble_count8<0>(w.x, w.y), nibble_count8<0>(w.z, w.w), elo, olo);
+        nibbles_to_bytes(nibble_count8<4>(w.x, w.y), nibble_count8<4>(w.z, w.w), ehi, ohi);
+        if (++n == 15) {
+            widen_add(elo, olo, wlo);
+            widen_add(ehi, ohi, whi);
+            elo = olo = ehi = ohi = 0;
             n = 0;
         }
+    };
+    constexpr int64_t STEP = 16 * CNT;
+    int64_t u = u0 + 16 * (int64_t)threadIdx.x;
+    for (; u + 3 * STEP + 16 <= u1; u += 4 * STEP) {
+        uint4 w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[k] = *reinterpret_cast<const uint4 *>(v + u + k * STEP);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) count16(w[k]);
     }
-    widen_add(alo, wlo);
-    widen_add(ahi, whi);
+    for (; u + 16 <= u1; u += STEP) count16(*reinterpret_cast<const uint4 *>(v + u));
+    if (u < u1) {
+        // the chunk's last, partial load: bytes past the end count as bin 7
+        // (not reported)
+        uint4 w = *reinterpret_cast<const uint4 *>(v + u);
+        const int rem = (int)(u1 - u);
+        uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (j >= rem) ws[j >> 2] |= 0x77u << (8 * (j & 3));
+        count16(make_uint4(ws[0], ws[1], ws[2], ws[3]));
+    }
+    widen_add(elo, olo, wlo);
+    widen_add(ehi, ohi, whi);
     uint32_t t[8] = {wlo[0], wlo[1], wlo[2], wlo[3], whi[0], whi[1], whi[2], whi[3]};
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
