@@ -45,7 +45,7 @@ struct SortSmem {
         struct {
             float buf[NMAX];             // the row, bin by bin, then in order
             uint32_t cnt[NBK];           // coarse counts, then bin counts, then bin ends (32 KB)
-            uint16_t fstart[NB1 + 1];    // first bin of each coarse bin (NBK at the end)
+            uint32_t fseg[NB1];          // bins of coarse bin c: s0 | s1 << 16, bins s0 .. s1 - 1
             float red_f[2 * NW];
             uint32_t red_u[NW + 1];
         } b;
@@ -53,14 +53,15 @@ struct SortSmem {
 };
 
 // Bin of x, descending: its coarse bin c (NB1 linear bins from hi down)
-// owns bins fstart[c] .. fstart[c + 1] - 1, and x takes the one its place
+// owns bins s0 .. s1 - 1 (fseg[c] = s0 | s1 << 16: one LDS read), and x takes the one its place
 // inside c falls in.  t is clamped to [0, NB1] (NaN from 0 * inf at x == hi
 // to 0), so c = NB1 - 1 with frac 1 at and past the range's end.
-__device__ __forceinline__ int sort_bin(const uint16_t *fstart, float x, float hi, float scale1) {
+__device__ __forceinline__ int sort_bin(const uint32_t *fseg, float x, float hi, float scale1) {
     const float t = fminf(fmaxf((hi - x) * scale1, 0.0f), (float)NB1);
     const int c = min((int)t, NB1 - 1);
     const float frac = t - (float)c;
-    const int s0 = fstart[c], nc = (int)fstart[c + 1] - s0;  // nc >= 1
+    const uint32_t g = fseg[c];
+    const int s0 = (int)(g & 0xFFFFu), nc = (int)(g >> 16) - s0;  // nc >= 1
     return s0 + min(nc - 1, (int)(frac * (float)nc));
 }
 
@@ -104,22 +105,42 @@ __device__ inline int merge_split(const float *A, int na, const float *B, int nb
 __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, int32_t n, float *y) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     // register slots e < ne hold values (the test is uniform, so the slots
-    // past a short row are skipped, not run with no lanes active)
-    const int ne = (n + NT - 1) / NT;
+    // past a short row are skipped, not run with no lanes active); ok[e]
+    // marks the slots that hold one of the row's values.  Rows of a multiple
+    // of 4 values starting 16-byte aligned load 16 bytes per slot group (slot
+    // 4k + c holds value 4 (t + k NT) + c), others 4 bytes per slot (value
+    // t + e NT); either way every load of a thread is issued before the
+    // first is used, from clamped addresses.
+    const bool vec = (n & 3) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+    const int ne = vec ? 4 * ((n / 4 + NT - 1) / NT) : (n + NT - 1) / NT;
     float v[EPT];
-    float mn = INFINITY, mx = -INFINITY;
+    bool ok[EPT];
+    if (vec) {
+        const float4 *x4 = reinterpret_cast<const float4 *>(x);
+        const int n4 = n / 4;
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-        v[e] = 0.0f;
-        if (e < ne) {
+        for (int k = 0; k < EPT / 4; ++k) {
+            const int i = (int)threadIdx.x + k * NT;
+            const float4 q = x4[min(i, n4 - 1)];
+            v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) ok[4 * k + c] = i < n4;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
             const int i = (int)threadIdx.x + e * NT;
-            if (i < n) {
-                v[e] = x[i];
-                mn = fminf(mn, v[e]);
-                mx = fmaxf(mx, v[e]);
-            }
+            v[e] = x[min(i, n - 1)];
+            ok[e] = i < n;
         }
     }
+    float mn = INFINITY, mx = -INFINITY;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e)
+        if (ok[e]) {
+            mn = fminf(mn, v[e]);
+            mx = fmaxf(mx, v[e]);
+        }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         mn = fminf(mn, __shfl_xor(mn, o, 64));
@@ -139,20 +160,16 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
     }
     if (!(mx > mn)) {
         // one distinct value: the row is its own sorted order
-#pragma unroll
-        for (int e = 0; e < EPT; ++e) {
-            const int i = (int)threadIdx.x + e * NT;
-            if (e < ne && i < n) y[i] = v[e];
-        }
+        for (int i = threadIdx.x; i < n; i += NT) y[i] = x[i];  // each element in place (x may be y)
         return true;
     }
     const float scale = (float)NB1 / (mx - mn);
     // coarse histogram (cnt was zeroed above), then each coarse bin c gets
-    // bins fstart[c] .. fstart[c + 1] - 1: one plus its share of the other
+    // bins s0 .. s1 - 1: one plus its share of the other
     // NBK - NB1 by its count
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
-        if (e < ne && (int)threadIdx.x + e * NT < n)
+        if (e < ne && ok[e])
             atomicAdd(&sm.u.b.cnt[(int)fminf(fmaxf((mx - v[e]) * scale, 0.0f), (float)(NB1 - 1))], 1u);
     __syncthreads();
     {
@@ -168,9 +185,10 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
         __syncthreads();
         uint32_t below = incl - cc;
         for (int i = 0; i < w; ++i) below += sm.u.b.red_u[i];
-        sm.u.b.fstart[threadIdx.x] =
-            (uint16_t)(threadIdx.x + (uint32_t)(((uint64_t)(NBK - NB1) * below) / (uint64_t)n));
-        if (threadIdx.x == 0) sm.u.b.fstart[NB1] = (uint16_t)NBK;
+        // the last coarse bin ends at bin NBK (below + cc = n there)
+        const uint32_t s0 = threadIdx.x + (uint32_t)(((uint64_t)(NBK - NB1) * below) / (uint64_t)n);
+        const uint32_t s1 = threadIdx.x + 1u + (uint32_t)(((uint64_t)(NBK - NB1) * (below + cc)) / (uint64_t)n);
+        sm.u.b.fseg[threadIdx.x] = s0 | (s1 << 16);
         __syncthreads();  // every thread has read its coarse count
     }
     for (int b = threadIdx.x; b < NBK; b += NT) sm.u.b.cnt[b] = 0;
@@ -179,10 +197,10 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
     // thread's values before any is used
     int bin[EPT];
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) bin[e] = e < ne ? sort_bin(sm.u.b.fstart, v[e], mx, scale) : 0;
+    for (int e = 0; e < EPT; ++e) bin[e] = e < ne ? sort_bin(sm.u.b.fseg, v[e], mx, scale) : 0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
-        if (e < ne && (int)threadIdx.x + e * NT < n) atomicAdd(&sm.u.b.cnt[bin[e]], 1u);
+        if (e < ne && ok[e]) atomicAdd(&sm.u.b.cnt[bin[e]], 1u);
     __syncthreads();
     // exclusive scan of the bin counts (4 bins per thread) and the largest bin
     constexpr int BPT = NBK / NT;
@@ -223,11 +241,11 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
         slot[e] = 0;
-        if (e < ne && (int)threadIdx.x + e * NT < n) slot[e] = atomicAdd(&sm.u.b.cnt[bin[e]], 1u);
+        if (e < ne && ok[e]) slot[e] = atomicAdd(&sm.u.b.cnt[bin[e]], 1u);
     }
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
-        if (e < ne && (int)threadIdx.x + e * NT < n) sm.u.b.buf[slot[e]] = v[e];
+        if (e < ne && ok[e]) sm.u.b.buf[slot[e]] = v[e];
     __syncthreads();
     // final positions, in buffer order (the lanes of a wavefront read the
     // same few bins): the bin's start plus the values of the bin above this
@@ -244,7 +262,7 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
 #pragma unroll
         for (int q = 0; q < RB; ++q) x0[q] = sm.u.b.buf[min(i0 + q * NT, n - 1)];
 #pragma unroll
-        for (int q = 0; q < RB; ++q) b[q] = sort_bin(sm.u.b.fstart, x0[q], mx, scale);
+        for (int q = 0; q < RB; ++q) b[q] = sort_bin(sm.u.b.fseg, x0[q], mx, scale);
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
             b0[q] = b[q] ? sm.u.b.cnt[b[q] - 1] : 0u;
