@@ -578,7 +578,7 @@ __global__ void __launch_bounds__(NT) k_select_rows(const float *__restrict__ sc
     const int64_t row = blockIdx.x;
     const float *x = scores + row * n_item;
     float *T = table_out + row * (int64_t)S;
-    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x;
     // the target ranks in LDS when they fit 16 bits (first read after the
     // barriers of the passes below)
     const bool lds_ranks = n_item <= 65536;
