@@ -82,7 +82,7 @@ extern "C" int rkr_sorted_scores_noalloc(rkr_ctx *c, int64_t user_begin, int64_t
         float *rows = out_dev + b0 * I;
         RKR_HIP(c, launch_scores_store(c->stream, c->users + (user_begin + b0) * c->ld, nullptr, m, c->ld, c->items,
                                        I, c->ld, c->dim, rows, I, true));
-        RKR_HIP(c, launch_sort_rows(c->stream, rows, I, m, I, rows, tmp, I));
+        RKR_HIP(c, launch_sort_rows(c->stream, rows, I, m, I, rows, tmp, I, c->n_cu));
     }
     return RKR_OK;
 }

@@ -5,15 +5,18 @@
 // (src/include/score_computation/ComputeScoreTableBatch.hpp:83-89).  Only the values are kept, so the
 // order of equal values is irrelevant and the sort need not be stable.
 //
-// One 1024-thread workgroup per row.  Rows of up to NMAX values are sorted in LDS (bucket path):
-//   1. the row is loaded into registers (coalesced), with its min and max;
+// One 1024-thread workgroup per CU, taking rows in turn.  Rows of up to NMAX values are sorted in LDS
+// (bucket path):
+//   1. the row is loaded into registers (coalesced; its loads were issued while the workgroup's
+//      previous row was ranked), with its min and max;
 //   2. a histogram over NBK value bins, descending, and its exclusive scan.  The bins are
 //      apportioned to NB1 linear coarse bins by the coarse bins' counts (a first histogram), so
 //      bins are narrow where the row is dense and hold about n / NBK values each even when a
 //      heavy tail crowds most of the row into a few linear bins.  FP32 arithmetic: fl(max - x)
 //      and fl(t * scale) are monotone in x, so bin order is value order;
-//   3. every value is scattered into its bin's segment of an LDS buffer (any order inside a bin);
-//   4. each value's final position is its bin's start plus the values of its bin above it (and the
+//   3. every value's order-preserving key is scattered into its bin's segment of an LDS buffer (any
+//      order inside a bin);
+//   4. each value's final position is its bin's start plus the keys of its bin above it (and the
 //      equal ones before it): threads walk the buffer in order, so the lanes of a wavefront read the
 //      same few bins and write within a few hundred bytes of the output row.
 // A row longer than NMAX, or one with a bin over BIN_MAX values (heavy ties, extreme skew), takes
@@ -24,6 +27,8 @@
 //      a scratch row.  Each thread finds where its share of a merged run starts by a binary search
 //      along the merge path, then merges that share sequentially.  The tiles go to whichever of the
 //      two rows makes the last pass land in the output row.
+#include <algorithm>
+
 #include "rkr_internal.hpp"
 
 namespace rkr {
@@ -38,12 +43,14 @@ constexpr int NMAX = 28672;              // longest row of the bucket path (112 
 constexpr int EPT = NMAX / NT;           // values per thread
 constexpr uint32_t BIN_MAX = 192;        // largest bin the bucket path ranks by counting
 constexpr int NB1 = 1024;                // linear coarse bins that apportion the NBK bins
+constexpr int KPAD = 256;                // zero keys past the row's end, read by the rank step
+static_assert(KPAD >= (int)BIN_MAX + 3, "the rank step reads up to BIN_MAX + 2 keys past a bin's start");
 
 struct SortSmem {
     union {
         float tile[TILE];                // merge path
         struct {
-            float buf[NMAX];             // the row, bin by bin, then in order
+            uint32_t key[NMAX + KPAD];   // the row's keys, bin by bin, then KPAD zero keys
             uint32_t cnt[NBK];           // coarse counts, then bin counts, then bin ends (32 KB)
             uint32_t fseg[NB1];          // bins of coarse bin c: s0 | s1 << 16, bins s0 .. s1 - 1
             float red_f[2 * NW];
@@ -63,6 +70,19 @@ __device__ __forceinline__ int sort_bin(const uint32_t *fseg, float x, float hi,
     const uint32_t g = fseg[c];
     const int s0 = (int)(g & 0xFFFFu), nc = (int)(g >> 16) - s0;  // nc >= 1
     return s0 + min(nc - 1, (int)(frac * (float)nc));
+}
+
+// Order-preserving key of a float: key(a) < key(b) whenever a < b.  Every NaN
+// maps to the largest key (the bins put NaN first); -0 keys just below +0.
+__device__ __forceinline__ uint32_t sort_key(float f) {
+    const uint32_t u = __float_as_uint(f);
+    const uint32_t k = u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
+    return f == f ? k : 0xFFFFFFFFu;
+}
+
+// The float of a key (NaN keys give one quiet NaN).
+__device__ __forceinline__ float key_value(uint32_t k) {
+    return __uint_as_float(k ^ ((int32_t)k < 0 ? 0x80000000u : 0xFFFFFFFFu));
 }
 
 // Sorts s[0..TILE) in descending order.  Stage (k, j) compares i with i + j for
@@ -99,45 +119,73 @@ __device__ inline int merge_split(const float *A, int na, const float *B, int nb
     return lo;
 }
 
-// Bucket path for one row of n <= NMAX values; returns false (having written
-// nothing) when a bin holds more than BIN_MAX values.  x and y may be the same
-// row: the row is in registers before anything is written.
-__device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, int32_t n, float *y) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // register slots e < ne hold values (the test is uniform, so the slots
-    // past a short row are skipped, not run with no lanes active); ok[e]
-    // marks the slots that hold one of the row's values.  Rows of a multiple
-    // of 4 values starting 16-byte aligned load 16 bytes per slot group (slot
-    // 4k + c holds value 4 (t + k NT) + c), others 4 bytes per slot (value
-    // t + e NT); either way every load of a thread is issued before the
-    // first is used, from clamped addresses.
-    const bool vec = (n & 3) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
-    const int ne = vec ? 4 * ((n / 4 + NT - 1) / NT) : (n + NT - 1) / NT;
-    float v[EPT];
-    bool ok[EPT];
-    if (vec) {
+// A row of n <= NMAX values in registers, EPT slots per thread.  Rows of a
+// multiple of 4 values whose rows start 16-byte aligned (vec) take 16 bytes
+// per slot group (slot 4k + c holds value 4 (t + k NT) + c), others 4 bytes
+// per slot (value t + e NT).  Slots e < row_slots<vec>(n) can hold values (the
+// test is uniform, so the slots past a short row are skipped, not run with no
+// lanes active); in_row says whether this thread's slot e holds one.
+template <bool vec>
+__device__ __forceinline__ int row_slots(int n) {
+    return vec ? 4 * ((n / 4 + NT - 1) / NT) : (n + NT - 1) / NT;
+}
+
+template <bool vec>
+__device__ __forceinline__ bool in_row(int t, int e, int n) {
+    return vec ? t + (e >> 2) * NT < (n >> 2) : t + e * NT < n;
+}
+
+// Issues every load of the thread's slots before any is used, from clamped
+// addresses; the caller's first use of v waits for them.
+template <bool vec>
+__device__ __forceinline__ void load_row(int t, float (&v)[EPT], const float *x, int n) {
+    if constexpr (vec) {
         const float4 *x4 = reinterpret_cast<const float4 *>(x);
         const int n4 = n / 4;
 #pragma unroll
         for (int k = 0; k < EPT / 4; ++k) {
-            const int i = (int)threadIdx.x + k * NT;
-            const float4 q = x4[min(i, n4 - 1)];
+            const float4 q = x4[min(t + k * NT, n4 - 1)];
             v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+        }
+    } else {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) ok[4 * k + c] = i < n4;
+        for (int e = 0; e < EPT; ++e) v[e] = x[min(t + e * NT, n - 1)];
+    }
+}
+
+// The inverse of load_row: each slot's value back to its place in y.
+template <bool vec>
+__device__ __forceinline__ void store_row(int t, const float (&v)[EPT], float *y, int n) {
+    if constexpr (vec) {
+        float4 *y4 = reinterpret_cast<float4 *>(y);
+#pragma unroll
+        for (int k = 0; k < EPT / 4; ++k) {
+            const int i = t + k * NT;
+            if (i < n / 4) y4[i] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
         }
     } else {
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
-            const int i = (int)threadIdx.x + e * NT;
-            v[e] = x[min(i, n - 1)];
-            ok[e] = i < n;
+            const int i = t + e * NT;
+            if (i < n) y[i] = v[e];
         }
     }
+}
+
+// Bucket path for the row held in v (load_row's layout); returns false, having
+// written nothing and with v unchanged, when a bin holds more than BIN_MAX
+// values.  Otherwise it writes the row's sorted order to y and issues the
+// loads of row xn into v once v's values are in LDS, so they are in flight
+// while this row is ranked.  y may be the row v came from.
+template <bool vec>
+__device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], int32_t n, float *y,
+                                                const float *xn, int t) {
+    const int lane = t & 63, w = t >> 6;
+    const int ne = row_slots<vec>(n);
     float mn = INFINITY, mx = -INFINITY;
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
-        if (ok[e]) {
+        if (in_row<vec>(t, e, n)) {
             mn = fminf(mn, v[e]);
             mx = fmaxf(mx, v[e]);
         }
@@ -150,7 +198,7 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
         sm.u.b.red_f[w] = mn;
         sm.u.b.red_f[NW + w] = mx;
     }
-    for (int b = threadIdx.x; b < NBK; b += NT) sm.u.b.cnt[b] = 0;
+    for (int b = t; b < NBK; b += NT) sm.u.b.cnt[b] = 0;
     __syncthreads();
     mn = sm.u.b.red_f[0];
     mx = sm.u.b.red_f[NW];
@@ -159,8 +207,9 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
         mx = fmaxf(mx, sm.u.b.red_f[NW + i]);
     }
     if (!(mx > mn)) {
-        // one distinct value: the row is its own sorted order
-        for (int i = threadIdx.x; i < n; i += NT) y[i] = x[i];  // each element in place (x may be y)
+        // one distinct value (NaNs aside): the row is its own sorted order
+        store_row<vec>(t, v, y, n);
+        load_row<vec>(t, v, xn, n);
         return true;
     }
     const float scale = (float)NB1 / (mx - mn);
@@ -169,12 +218,12 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
     // NBK - NB1 by its count
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
-        if (e < ne && ok[e])
+        if (e < ne && in_row<vec>(t, e, n))
             atomicAdd(&sm.u.b.cnt[(int)fminf(fmaxf((mx - v[e]) * scale, 0.0f), (float)(NB1 - 1))], 1u);
     __syncthreads();
     {
         static_assert(NB1 == NT, "one coarse bin per thread");
-        const uint32_t cc = sm.u.b.cnt[threadIdx.x];
+        const uint32_t cc = sm.u.b.cnt[t];
         uint32_t incl = cc;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -186,12 +235,12 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
         uint32_t below = incl - cc;
         for (int i = 0; i < w; ++i) below += sm.u.b.red_u[i];
         // the last coarse bin ends at bin NBK (below + cc = n there)
-        const uint32_t s0 = threadIdx.x + (uint32_t)(((uint64_t)(NBK - NB1) * below) / (uint64_t)n);
-        const uint32_t s1 = threadIdx.x + 1u + (uint32_t)(((uint64_t)(NBK - NB1) * (below + cc)) / (uint64_t)n);
-        sm.u.b.fseg[threadIdx.x] = s0 | (s1 << 16);
+        const uint32_t s0 = t + (uint32_t)(((uint64_t)(NBK - NB1) * below) / (uint64_t)n);
+        const uint32_t s1 = t + 1u + (uint32_t)(((uint64_t)(NBK - NB1) * (below + cc)) / (uint64_t)n);
+        sm.u.b.fseg[t] = s0 | (s1 << 16);
         __syncthreads();  // every thread has read its coarse count
     }
-    for (int b = threadIdx.x; b < NBK; b += NT) sm.u.b.cnt[b] = 0;
+    for (int b = t; b < NBK; b += NT) sm.u.b.cnt[b] = 0;
     __syncthreads();
     // every value's bin, computed once: its LDS reads are issued for all the
     // thread's values before any is used
@@ -200,14 +249,14 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
     for (int e = 0; e < EPT; ++e) bin[e] = e < ne ? sort_bin(sm.u.b.fseg, v[e], mx, scale) : 0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
-        if (e < ne && ok[e]) atomicAdd(&sm.u.b.cnt[bin[e]], 1u);
+        if (e < ne && in_row<vec>(t, e, n)) atomicAdd(&sm.u.b.cnt[bin[e]], 1u);
     __syncthreads();
     // exclusive scan of the bin counts (4 bins per thread) and the largest bin
     constexpr int BPT = NBK / NT;
     uint32_t c[BPT], run = 0, big = 0;
 #pragma unroll
     for (int k = 0; k < BPT; ++k) {
-        c[k] = sm.u.b.cnt[threadIdx.x * BPT + k];
+        c[k] = sm.u.b.cnt[t * BPT + k];
         run += c[k];
         big = max(big, c[k]);
     }
@@ -232,7 +281,7 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
     uint32_t start = base + incl - run;
 #pragma unroll
     for (int k = 0; k < BPT; ++k) {
-        sm.u.b.cnt[threadIdx.x * BPT + k] = start;  // scatter cursor, ends as the bin's end
+        sm.u.b.cnt[t * BPT + k] = start;  // scatter cursor, ends as the bin's end
         start += c[k];
     }
     __syncthreads();
@@ -241,57 +290,60 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, const float *x, in
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
         slot[e] = 0;
-        if (e < ne && ok[e]) slot[e] = atomicAdd(&sm.u.b.cnt[bin[e]], 1u);
+        if (e < ne && in_row<vec>(t, e, n)) slot[e] = atomicAdd(&sm.u.b.cnt[bin[e]], 1u);
     }
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
-        if (e < ne && ok[e]) sm.u.b.buf[slot[e]] = v[e];
+        if (e < ne && in_row<vec>(t, e, n)) sm.u.b.key[slot[e]] = sort_key(v[e]);
+    if (t < KPAD) sm.u.b.key[n + t] = 0u;  // below every value's key
     __syncthreads();
+    load_row<vec>(t, v, xn, n);  // v's values are in LDS
     // final positions, in buffer order (the lanes of a wavefront read the
-    // same few bins): the bin's start plus the values of the bin above this
-    // one, and the equal ones before it.  The value goes straight to its
-    // place in the output row; the row's input is no longer read.  A thread
-    // takes RB values at a time, so their bin lookups and the counting reads
-    // (four per value and step; a bin holds about n / NBK values) are in
-    // flight together; reads past a bin are clamped to the row and masked out.
+    // same few bins): the bin's start plus the number of the bin's keys that
+    // order before this one.  (key, position) pairs order by key, larger
+    // first, then by position, lower first, so equal values take consecutive
+    // places; one 64-bit compare of (key, ~position) does both.  Each value
+    // reads a window from its bin's start, as long as the longest bin of the
+    // thread's RB values rounded up to 4: the keys past its bin belong to later
+    // bins (smaller values, smaller keys) or to the zero padding, and never
+    // order before it, so the window needs no bounds.  The value goes straight
+    // to its place in the output row.  A thread takes RB values at a time, so
+    // their bin lookups and the window reads are in flight together.
     constexpr int RB = 4;
-    for (int i0 = threadIdx.x; i0 < n; i0 += RB * NT) {
+    for (int i0 = t; i0 < n; i0 += RB * NT) {
+        uint32_t kx[RB], b0[RB], b1[RB], r[RB], len = 0;
         float x0[RB];
-        int b[RB];
-        uint32_t b0[RB], b1[RB], r[RB], len = 0;
+        uint64_t me[RB];
 #pragma unroll
-        for (int q = 0; q < RB; ++q) x0[q] = sm.u.b.buf[min(i0 + q * NT, n - 1)];
-#pragma unroll
-        for (int q = 0; q < RB; ++q) b[q] = sort_bin(sm.u.b.fseg, x0[q], mx, scale);
+        for (int q = 0; q < RB; ++q) kx[q] = sm.u.b.key[min(i0 + q * NT, n - 1)];
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
-            b0[q] = b[q] ? sm.u.b.cnt[b[q] - 1] : 0u;
-            b1[q] = sm.u.b.cnt[b[q]];
+            x0[q] = key_value(kx[q]);
+            const int b = sort_bin(sm.u.b.fseg, x0[q], mx, scale);
+            b0[q] = b ? sm.u.b.cnt[b - 1] : 0u;
+            b1[q] = sm.u.b.cnt[b];
         }
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
-            r[q] = b0[q];
+            r[q] = 0;
             len = max(len, b1[q] - b0[q]);
+            me[q] = ((uint64_t)kx[q] << 32) | (uint32_t)~(uint32_t)(i0 + q * NT);
         }
         for (uint32_t j = 0; j < len; j += 4) {
-            float z[RB][4];
+            uint32_t z[RB][4];
 #pragma unroll
             for (int q = 0; q < RB; ++q)
 #pragma unroll
-                for (int k = 0; k < 4; ++k) z[q][k] = sm.u.b.buf[min(b0[q] + j + k, (uint32_t)n - 1u)];
+                for (int k = 0; k < 4; ++k) z[q][k] = sm.u.b.key[b0[q] + j + k];
 #pragma unroll
-            for (int q = 0; q < RB; ++q) {
-                const uint32_t i = (uint32_t)(i0 + q * NT);
+            for (int q = 0; q < RB; ++q)
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const uint32_t pos = b0[q] + j + k;
-                    r[q] += (pos < b1[q] && (z[q][k] > x0[q] || (z[q][k] == x0[q] && pos < i))) ? 1u : 0u;
-                }
-            }
+                for (int k = 0; k < 4; ++k)
+                    r[q] += (((uint64_t)z[q][k] << 32) | (uint32_t)~(b0[q] + j + k)) > me[q] ? 1u : 0u;
         }
 #pragma unroll
         for (int q = 0; q < RB; ++q)
-            if (i0 + q * NT < n) y[r[q]] = x0[q];
+            if (i0 + q * NT < n) y[b0[q] + r[q]] = x0[q];
     }
     return true;
 }
@@ -343,25 +395,67 @@ __device__ void merge_sort_row(float *s, const float *x, int32_t n, float *out_r
     }
 }
 
-__global__ void __launch_bounds__(NT) k_sort_rows(const float *src, int64_t ld_src, int32_t n, float *out,
-                                                  float *tmp, int64_t ld) {
+// vec (every row starts 16-byte aligned and n is a multiple of 4): persistent,
+// workgroup p sorts rows p, p + G, p + 2G, ... (G = gridDim.x, one workgroup
+// per CU), and on the bucket path the next row's loads are in flight while the
+// current row is ranked.  Otherwise one workgroup per row (G = n_rows).
+template <bool vec>
+__global__ void __launch_bounds__(NT) k_sort_rows(const float *src, int64_t ld_src, int64_t n_rows, int32_t n,
+                                                  float *out, float *tmp, int64_t ld) {
     __shared__ SortSmem sm;
-    const int64_t row = blockIdx.x;
-    const float *x = src + row * ld_src;
-    if (n <= NMAX) {
-        if (bucket_sort_row(sm, x, n, out + row * ld)) return;
-        __syncthreads();  // the bucket path's LDS is done with before the tiles reuse it
+    int64_t row = blockIdx.x;
+    if (n > NMAX) {
+        for (; row < n_rows; row += gridDim.x) {
+            merge_sort_row(sm.u.tile, src + row * ld_src, n, out + row * ld, tmp + row * ld);
+            __syncthreads();  // the tiles are free before the next row's first tile
+        }
+        return;
     }
-    merge_sort_row(sm.u.tile, x, n, out + row * ld, tmp + row * ld);
+    float v[EPT];
+    if constexpr (!vec) {
+        // the row's own address as the next row: those loads' values are unused
+        const float *x = src + row * ld_src;
+        load_row<false>(threadIdx.x, v, x, n);
+        if (bucket_sort_row<false>(sm, v, n, out + row * ld, x, threadIdx.x)) return;
+        __syncthreads();  // the bucket path's LDS is done with before the tiles reuse it
+        merge_sort_row(sm.u.tile, x, n, out + row * ld, tmp + row * ld);
+        return;
+    }
+    if (row < n_rows) load_row<vec>(threadIdx.x, v, src + row * ld_src, n);
+    for (; row < n_rows; row += gridDim.x) {
+        // the next row's loads are issued unconditionally (so v is dead across
+        // the merge path); a workgroup's last row loads its own row again, which
+        // it has just read, and the values are not used.  n and the thread index
+        // pass through an empty asm statement each iteration, so the per-slot
+        // indexes and LDS addresses derived from them are computed where they
+        // are used instead of being hoisted out of the loop and held in
+        // registers (which spilled)
+        int32_t nn = n;
+        int t = threadIdx.x;
+        asm volatile("" : "+s"(nn), "+v"(t));
+        const int64_t next = row + gridDim.x;
+        const float *xn = src + (next < n_rows ? next : row) * ld_src;
+        if (!bucket_sort_row<vec>(sm, v, nn, out + row * ld, xn, t)) {
+            __syncthreads();  // the bucket path's LDS is done with before the tiles reuse it
+            merge_sort_row(sm.u.tile, src + row * ld_src, n, out + row * ld, tmp + row * ld);
+            load_row<vec>(t, v, xn, nn);
+        }
+        __syncthreads();  // this row's LDS is done with before the next row's first write
+    }
 }
 
 }  // namespace
 
 hipError_t launch_sort_rows(hipStream_t s, const float *src, int64_t ld_src, int64_t nb, int64_t n_item, float *out,
-                            float *tmp, int64_t ld) {
+                            float *tmp, int64_t ld, int grid_cap) {
     if (nb <= 0 || n_item <= 0) return hipSuccess;
-    if (n_item > (int64_t)1 << 30 || ld < n_item || ld_src < n_item) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_sort_rows, dim3((unsigned)nb), dim3(NT), 0, s, src, ld_src, (int32_t)n_item, out, tmp, ld);
+    if (n_item > (int64_t)1 << 30 || ld < n_item || ld_src < n_item || grid_cap <= 0) return hipErrorInvalidValue;
+    const bool vec = (n_item & 3) == 0 && (ld_src & 3) == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0;
+    const unsigned grid = (unsigned)(vec ? std::min<int64_t>(nb, grid_cap) : nb);
+    if (vec)
+        hipLaunchKernelGGL(k_sort_rows<true>, dim3(grid), dim3(NT), 0, s, src, ld_src, nb, (int32_t)n_item, out, tmp, ld);
+    else
+        hipLaunchKernelGGL(k_sort_rows<false>, dim3(grid), dim3(NT), 0, s, src, ld_src, nb, (int32_t)n_item, out, tmp, ld);
     return hipGetLastError();
 }
 

@@ -923,7 +923,7 @@ static int sorted_score_batches(rkr_ctx *c, int64_t u0, int64_t n, const char *w
         RKR_HIP(c, launch_scores_store(c->stream, c->users + (u0 + b0) * c->ld, nullptr, m, c->ld, c->items, I, c->ld,
                                        c->dim, blk, I, true));
         RKR_HIP(c, hipEventRecord(c->ev[1], c->stream));
-        RKR_HIP(c, launch_sort_rows(c->stream, blk, I, m, I, blk, tmp, I));
+        RKR_HIP(c, launch_sort_rows(c->stream, blk, I, m, I, blk, tmp, I, c->n_cu));
         RKR_HIP(c, hipEventRecord(c->ev[2], c->stream));
         if ((rc = sink(blk, u0 + b0, m))) return rc;  // the sink synchronises the stream
         if (gemm_ms) *gemm_ms += ev_ms(c->ev, 0, 1);

@@ -408,9 +408,10 @@ hipError_t launch_select_rows(hipStream_t s, const float *scores, int64_t nb, in
 
 // ---- full sorted rows (k_sort.hip) ------------------------------------------
 // out[b * ld + i] = i-th largest of src[b * ld_src + 0..n_item), b < nb; tmp is
-// scratch of the same shape as out.  out may be src (ld == ld_src).
+// scratch of the same shape as out.  out may be src (ld == ld_src).  At most
+// grid_cap workgroups (one fits per CU), each taking rows in turn.
 hipError_t launch_sort_rows(hipStream_t s, const float *src, int64_t ld_src, int64_t nb, int64_t n_item, float *out,
-                            float *tmp, int64_t ld);
+                            float *tmp, int64_t ld, int grid_cap);
 
 // ---- QS sample-rank training (k_train.hip) ----------------------------------
 // hist[s * (n_item + 1) + #{i : sorted[u * ld + i] > q[u * ldq + s]}] += 1 for

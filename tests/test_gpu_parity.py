@@ -293,6 +293,7 @@ def test_two_query_slots_agree(orc, gpu_index_factory, form, batch):
     (40, 20000, 150, 24),     # plain-GPU row width: two merge passes, a partial run
     (9, 50001, 33, 25),       # precision row width: three merge passes
     (3, 1, 4, 26),            # one item
+    (50, 9999, 20, 31),       # an odd row width: the bucket path's 4-byte loads, one row per workgroup
     (1000, 20000, 8, 28),     # 80 MB of rows: three 32 MB host staging chunks, the last one partial
 ])
 def test_sorted_scores_match_oracle(orc, U, I, d, seed, tmp_path):
