@@ -495,9 +495,9 @@ PROFILED_TRAFFIC = {
 }
 
 
-# k_sort_rows, 100k x 20k: FETCH_SIZE 1,048,727.3 KiB (x2.0) + WRITE_SIZE 2,098,289.4 KiB (x1.0) per
+# k_sort_rows, 100k x 20k: FETCH_SIZE 1,054,886.2 KiB (x2.0) + WRITE_SIZE 2,115,848.5 KiB (x1.0) per
 # full-batch launch (26,843 rows)
-SORT_TRAFFIC_BYTES = 4.297e9
+SORT_TRAFFIC_BYTES = 4.327e9
 
 
 def copy_calibration():

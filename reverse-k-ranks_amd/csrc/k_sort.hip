@@ -251,7 +251,7 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], i
     for (int e = 0; e < EPT; ++e)
         if (e < ne && in_row<vec>(t, e, n)) atomicAdd(&sm.u.b.cnt[bin[e]], 1u);
     __syncthreads();
-    // exclusive scan of the bin counts (4 bins per thread) and the largest bin
+    // exclusive scan of the bin counts (BPT bins per thread) and the largest bin
     constexpr int BPT = NBK / NT;
     uint32_t c[BPT], run = 0, big = 0;
 #pragma unroll
