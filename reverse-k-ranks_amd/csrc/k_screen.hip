@@ -587,21 +587,31 @@ __global__ void __launch_bounds__(NT, SETS ? 2 : coarse_first_pass_wgs(KS)) k_ip
             {
                 // the stage's query rows: KS 16-byte pieces per thread, all
                 // loads in flight before the first LDS write (rows past nq
-                // load the last query and are zeroed after the load)
+                // load the last query and are zeroed after the load).  The
+                // query map's entries are all read before the first row load,
+                // so the row loads do not wait for one another.
                 static_assert(NH * QST * KQ == KS * NT, "one piece per thread and k-step");
                 uint4 v[KS];
-                int dst[KS];
+                int dst[KS], qr[KS];
                 bool ok[KS];
+#pragma unroll
+                for (int k = 0; k < KS; ++k) {
+                    const int c = (int)threadIdx.x + k * NT;
+                    const int rem = c % (QST * KQ);
+                    const int qq = q0 + rem / KQ;
+                    ok[k] = qq < P.nq;
+                    qr[k] = min(qq, P.nq - 1);
+                }
+                if (P.qmap) {
+#pragma unroll
+                    for (int k = 0; k < KS; ++k) qr[k] = P.qmap[qr[k]];
+                }
 #pragma unroll
                 for (int k = 0; k < KS; ++k) {
                     const int c = (int)threadIdx.x + k * NT;
                     const int half = c / (QST * KQ);
                     const int rem = c % (QST * KQ);
-                    const int qq = q0 + rem / KQ;
-                    ok[k] = qq < P.nq;
-                    const int qc = min(qq, P.nq - 1);
-                    const int64_t qr = P.qmap ? P.qmap[qc] : qc;
-                    v[k] = reinterpret_cast<const uint4 *>(half ? P.q_lo : P.q_hi)[qr * KQ + rem % KQ];
+                    v[k] = reinterpret_cast<const uint4 *>(half ? P.q_lo : P.q_hi)[(int64_t)qr[k] * KQ + rem % KQ];
                     dst[k] = half * (QST * ROWQ) + (rem / KQ) * ROWQ + rem % KQ;
                 }
 #pragma unroll
