@@ -10,7 +10,8 @@
 //   1. the row is loaded into registers (coalesced; its loads were issued while the workgroup's
 //      previous row was ranked), with its min and max;
 //   2. a histogram over NBK value bins, descending, and its exclusive scan.  The bins are
-//      apportioned to NB1 linear coarse bins by the coarse bins' counts (a first histogram), so
+//      apportioned to NB1 linear coarse bins by the coarse bins' counts (a first histogram, of
+//      half the row), so
 //      bins are narrow where the row is dense and hold about n / NBK values each even when a
 //      heavy tail crowds most of the row into a few linear bins.  FP32 arithmetic: fl(max - x)
 //      and fl(t * scale) are monotone in x, so bin order is value order;
@@ -213,12 +214,15 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], i
         return true;
     }
     const float scale = (float)NB1 / (mx - mn);
-    // coarse histogram (cnt was zeroed above), then each coarse bin c gets
-    // bins s0 .. s1 - 1: one plus its share of the other
-    // NBK - NB1 by its count
+    // coarse histogram (cnt was zeroed above) of a sample, the values of the
+    // slots e < nes (the row's first ns values), then each coarse bin c gets
+    // bins s0 .. s1 - 1: one plus its share of the other NBK - NB1 by its
+    // sampled count.  The sample only shapes the bins; every value is binned.
+    const int nes = vec ? 4 * ((ne / 4 + 1) / 2) : (ne + 1) / 2;
+    const uint32_t ns = (uint32_t)min(n, NT * nes);
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
-        if (e < ne && in_row<vec>(t, e, n))
+        if (e < nes && in_row<vec>(t, e, n))
             atomicAdd(&sm.u.b.cnt[(int)fminf(fmaxf((mx - v[e]) * scale, 0.0f), (float)(NB1 - 1))], 1u);
     __syncthreads();
     {
@@ -234,9 +238,9 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], i
         __syncthreads();
         uint32_t below = incl - cc;
         for (int i = 0; i < w; ++i) below += sm.u.b.red_u[i];
-        // the last coarse bin ends at bin NBK (below + cc = n there)
-        const uint32_t s0 = t + (uint32_t)(((uint64_t)(NBK - NB1) * below) / (uint64_t)n);
-        const uint32_t s1 = t + 1u + (uint32_t)(((uint64_t)(NBK - NB1) * (below + cc)) / (uint64_t)n);
+        // the last coarse bin ends at bin NBK (below + cc = ns there)
+        const uint32_t s0 = t + (uint32_t)(((uint64_t)(NBK - NB1) * below) / (uint64_t)ns);
+        const uint32_t s1 = t + 1u + (uint32_t)(((uint64_t)(NBK - NB1) * (below + cc)) / (uint64_t)ns);
         sm.u.b.fseg[t] = s0 | (s1 << 16);
         __syncthreads();  // every thread has read its coarse count
     }
