@@ -54,7 +54,8 @@ struct SortSmem {
             uint32_t key[NMAX + KPAD];   // the row's keys, bin by bin, then KPAD zero keys
             uint32_t cnt[NBK];           // coarse counts, then bin counts, then bin ends (32 KB)
             uint32_t fseg[NB1];          // bins of coarse bin c: s0 | s1 << 16, bins s0 .. s1 - 1
-            float red_f[2 * NW];
+            uint32_t coarse[NB1];        // coarse counts (zero between rows)
+            float red_f[2 * NW];         // the row's min and max per wavefront, then the largest bin
             uint32_t red_u[NW + 1];
         } b;
     } u;
@@ -173,16 +174,11 @@ __device__ __forceinline__ void store_row(int t, const float (&v)[EPT], float *y
     }
 }
 
-// Bucket path for the row held in v (load_row's layout); returns false, having
-// written nothing and with v unchanged, when a bin holds more than BIN_MAX
-// values.  Otherwise it writes the row's sorted order to y and issues the
-// loads of row xn into v once v's values are in LDS, so they are in flight
-// while this row is ranked.  y may be the row v came from.
+// Each wavefront's min and max of the row held in v, into red_f (read by the
+// row's bucket path after the next barrier).
 template <bool vec>
-__device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], int32_t n, float *y,
-                                                const float *xn, int t) {
+__device__ __forceinline__ void row_partials(SortSmem &sm, const float (&v)[EPT], int32_t n, int t) {
     const int lane = t & 63, w = t >> 6;
-    const int ne = row_slots<vec>(n);
     float mn = INFINITY, mx = -INFINITY;
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
@@ -199,10 +195,21 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], i
         sm.u.b.red_f[w] = mn;
         sm.u.b.red_f[NW + w] = mx;
     }
-    for (int b = t; b < NBK; b += NT) sm.u.b.cnt[b] = 0;
-    __syncthreads();
-    mn = sm.u.b.red_f[0];
-    mx = sm.u.b.red_f[NW];
+}
+
+// Bucket path for the row held in v (load_row's layout); returns false, having
+// written nothing and with v unchanged, when a bin holds more than BIN_MAX
+// values.  Otherwise it writes the row's sorted order to y and (vec: the
+// persistent kernel) issues the loads of row xn into v once v's values are in
+// LDS, so they are in flight while this row is ranked, and leaves row xn's
+// min/max partials in red_f.  On entry red_f holds this row's partials and
+// coarse is zero.  y may be the row v came from.
+template <bool vec>
+__device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], int32_t n, float *y,
+                                                const float *xn, int t) {
+    const int lane = t & 63, w = t >> 6;
+    const int ne = row_slots<vec>(n);
+    float mn = sm.u.b.red_f[0], mx = sm.u.b.red_f[NW];
     for (int i = 1; i < NW; ++i) {
         mn = fminf(mn, sm.u.b.red_f[i]);
         mx = fmaxf(mx, sm.u.b.red_f[NW + i]);
@@ -210,11 +217,16 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], i
     if (!(mx > mn)) {
         // one distinct value (NaNs aside): the row is its own sorted order
         store_row<vec>(t, v, y, n);
-        load_row<vec>(t, v, xn, n);
+        if constexpr (vec) {
+            load_row<vec>(t, v, xn, n);
+            __syncthreads();  // every thread has read this row's partials
+            row_partials<vec>(sm, v, n, t);
+        }
         return true;
     }
+    for (int b = t; b < NBK; b += NT) sm.u.b.cnt[b] = 0;  // the bin counts, after the coarse pass
     const float scale = (float)NB1 / (mx - mn);
-    // coarse histogram (cnt was zeroed above) of a sample, the values of the
+    // coarse histogram (coarse is zero on entry) of a sample, the values of the
     // slots e < nes (the row's first ns values), then each coarse bin c gets
     // bins s0 .. s1 - 1: one plus its share of the other NBK - NB1 by its
     // sampled count.  The sample only shapes the bins; every value is binned.
@@ -223,11 +235,12 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], i
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
         if (e < nes && in_row<vec>(t, e, n))
-            atomicAdd(&sm.u.b.cnt[(int)fminf(fmaxf((mx - v[e]) * scale, 0.0f), (float)(NB1 - 1))], 1u);
+            atomicAdd(&sm.u.b.coarse[(int)fminf(fmaxf((mx - v[e]) * scale, 0.0f), (float)(NB1 - 1))], 1u);
     __syncthreads();
     {
         static_assert(NB1 == NT, "one coarse bin per thread");
-        const uint32_t cc = sm.u.b.cnt[t];
+        const uint32_t cc = sm.u.b.coarse[t];
+        sm.u.b.coarse[t] = 0u;  // zero again for the next row
         uint32_t incl = cc;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -242,10 +255,8 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], i
         const uint32_t s0 = t + (uint32_t)(((uint64_t)(NBK - NB1) * below) / (uint64_t)ns);
         const uint32_t s1 = t + 1u + (uint32_t)(((uint64_t)(NBK - NB1) * (below + cc)) / (uint64_t)ns);
         sm.u.b.fseg[t] = s0 | (s1 << 16);
-        __syncthreads();  // every thread has read its coarse count
+        __syncthreads();  // the bins' segments and the zeroed counts are visible
     }
-    for (int b = t; b < NBK; b += NT) sm.u.b.cnt[b] = 0;
-    __syncthreads();
     // every value's bin, computed once: its LDS reads are issued for all the
     // thread's values before any is used
     int bin[EPT];
@@ -301,7 +312,7 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], i
         if (e < ne && in_row<vec>(t, e, n)) sm.u.b.key[slot[e]] = sort_key(v[e]);
     if (t < KPAD) sm.u.b.key[n + t] = 0u;  // below every value's key
     __syncthreads();
-    load_row<vec>(t, v, xn, n);  // v's values are in LDS
+    if constexpr (vec) load_row<vec>(t, v, xn, n);  // v's values are in LDS
     // final positions, in buffer order (the lanes of a wavefront read the
     // same few bins): the bin's start plus the number of the bin's keys that
     // order before this one.  (key, position) pairs order by key, larger
@@ -349,6 +360,8 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], i
         for (int q = 0; q < RB; ++q)
             if (i0 + q * NT < n) y[b0[q] + r[q]] = x0[q];
     }
+    // the largest-bin reads of red_f ended before the scatter's barrier
+    if constexpr (vec) row_partials<vec>(sm, v, n, t);
     return true;
 }
 
@@ -416,16 +429,21 @@ __global__ void __launch_bounds__(NT) k_sort_rows(const float *src, int64_t ld_s
         return;
     }
     float v[EPT];
+    if (row >= n_rows) return;
+    for (int i = threadIdx.x; i < NB1; i += NT) sm.u.b.coarse[i] = 0u;
     if constexpr (!vec) {
-        // the row's own address as the next row: those loads' values are unused
         const float *x = src + row * ld_src;
         load_row<false>(threadIdx.x, v, x, n);
-        if (bucket_sort_row<false>(sm, v, n, out + row * ld, x, threadIdx.x)) return;
+        row_partials<false>(sm, v, n, threadIdx.x);
+        __syncthreads();
+        if (bucket_sort_row<false>(sm, v, n, out + row * ld, nullptr, threadIdx.x)) return;
         __syncthreads();  // the bucket path's LDS is done with before the tiles reuse it
         merge_sort_row(sm.u.tile, x, n, out + row * ld, tmp + row * ld);
         return;
     }
-    if (row < n_rows) load_row<vec>(threadIdx.x, v, src + row * ld_src, n);
+    load_row<vec>(threadIdx.x, v, src + row * ld_src, n);
+    row_partials<vec>(sm, v, n, threadIdx.x);
+    __syncthreads();
     for (; row < n_rows; row += gridDim.x) {
         // the next row's loads are issued unconditionally (so v is dead across
         // the merge path); a workgroup's last row loads its own row again, which
@@ -443,8 +461,9 @@ __global__ void __launch_bounds__(NT) k_sort_rows(const float *src, int64_t ld_s
             __syncthreads();  // the bucket path's LDS is done with before the tiles reuse it
             merge_sort_row(sm.u.tile, src + row * ld_src, n, out + row * ld, tmp + row * ld);
             load_row<vec>(t, v, xn, nn);
+            row_partials<vec>(sm, v, nn, t);
         }
-        __syncthreads();  // this row's LDS is done with before the next row's first write
+        __syncthreads();  // this row's LDS is done with, the next row's partials are visible
     }
 }
 
