@@ -283,7 +283,8 @@ __device__ __forceinline__ bool bucket_sort_row(SortSmem &sm, float (&v)[EPT], i
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) big = max(big, __shfl_xor(big, o, 64));
-    __syncthreads();  // every thread has read its counts
+    // red_u and red_f were last read before the barriers of the coarse pass and
+    // the bin histogram, and each thread overwrites only its own counts below
     if (lane == 63) sm.u.b.red_u[w] = incl;
     if (lane == 0) sm.u.b.red_f[w] = __uint_as_float(big);
     __syncthreads();
