@@ -51,35 +51,63 @@ union Frag {
 
 // Splits every item row into hi/lo BF16 rows (Kp columns, zero padded) and
 // stores an upper bound of each item's L2 norm (FP64 sum, rounded up).
+// With cs (queries of the coarse first pass) it also stores each row q's
+// error scale of the one-product estimate u_hi . q_hi,
+//   cs = (1 + rho) (g1 ||q_hi|| + ||q_r||) + rho (||q_hi|| + ||q_r||) + g_rule ||q||,
+// rounded up, where q_r = q - q_hi (exact in FP32) and rho >= ||u_r|| / ||u||
+// over the users: |u_hi . q_hi (FP32 sums) - s(u, q)| <= ||u|| cs + the
+// denormal floor.  (u_hi . q_r + u_r . q_hi + u_r . q_r by Cauchy-Schwarz with
+// ||u_hi|| <= (1 + rho) ||u||; g1 the FP32 accumulation of the Kp exact
+// products, g_rule the sequential rule's own rounding.)
 __global__ void k_prep_items(const float *__restrict__ items, int64_t n_item, int32_t ld, int32_t K, int32_t Kp,
-                             uint16_t *__restrict__ hi, uint16_t *__restrict__ lo, float *__restrict__ norm) {
+                             uint16_t *__restrict__ hi, uint16_t *__restrict__ lo, float *__restrict__ norm,
+                             float *__restrict__ cs, const float *__restrict__ rho, double g1, double g_rule) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.y + threadIdx.y;
     if (p >= n_item) return;
     const float *x = items + p * ld;
-    double ss = 0.0;
+    double ss = 0.0, sh = 0.0, sr = 0.0;
     for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
         const float v = k < K ? x[k] : 0.0f;
         const uint16_t h = bf16_bits(v);
-        const uint16_t l = bf16_bits(v - bf16_to_f32(h));
+        const float r = v - bf16_to_f32(h);  // exact: bf16(v) is within a factor 2 of v
+        const uint16_t l = bf16_bits(r);
         hi[p * Kp + k] = h;
         lo[p * Kp + k] = l;
         ss += (double)v * (double)v;
+        sh += (double)bf16_to_f32(h) * (double)bf16_to_f32(h);
+        sr += (double)r * (double)r;
     }
-    for (int o = 16; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 32);
+    for (int o = 16; o > 0; o >>= 1) {
+        ss += __shfl_xor(ss, o, 32);
+        sh += __shfl_xor(sh, o, 32);
+        sr += __shfl_xor(sr, o, 32);
+    }
     if (threadIdx.x == 0 && norm) norm[p] = __double2float_ru(sqrt(ss));
+    if (threadIdx.x == 0 && cs) {
+        // the FP64 sums' own rounding (< 2^-40 relative) is covered by the
+        // 2^-30 margin
+        const double rh = (double)*rho, nh = sqrt(sh), nr = sqrt(sr);
+        const double c = (1.0 + rh) * (g1 * nh + nr) + rh * (nh + nr) + g_rule * sqrt(ss);
+        cs[p] = __double2float_ru(c * (1.0 + 0x1p-30));
+    }
 }
 
-// Upper bound of each user's L2 norm.
+// Upper bound of each user's L2 norm; with rho, also raises *rho to at least
+// ||u - bf16(u)|| / ||u|| of every user (rounded up; non-negative floats
+// order as their bits).
 __global__ void k_user_norms(const float *__restrict__ users, int64_t n_user, int32_t ld, int32_t K,
-                             float *__restrict__ norm) {
+                             float *__restrict__ norm, float *__restrict__ rho) {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n_user) return;
-    double ss = 0.0;
+    double ss = 0.0, sr = 0.0;
     for (int k = 0; k < K; ++k) {
-        const double v = users[u * ld + k];
+        const float f = users[u * ld + k];
+        const double v = f, r = f - bf16_to_f32(bf16_bits(f));
         ss += v * v;
+        sr += r * r;
     }
     norm[u] = __double2float_ru(sqrt(ss));
+    if (rho && ss > 0.0) atomicMax(reinterpret_cast<int *>(rho), __float_as_int(__double2float_ru(sqrt(sr / ss) * (1.0 + 0x1p-30))));
 }
 
 struct ScreenParams {
@@ -699,18 +727,21 @@ hipError_t launch_gather_guards(hipStream_t s, const float *T, int64_t S, const 
 }
 
 hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, int32_t ld, int32_t K, int32_t Kp,
-                             uint16_t *hi, uint16_t *lo, float *norm) {
+                             uint16_t *hi, uint16_t *lo, float *norm, float *cs, const float *rho, double g1,
+                             double g_rule) {
     if (n_item <= 0) return hipSuccess;
+    if (cs && !rho) return hipErrorInvalidValue;
     dim3 block(32, 8);
     hipLaunchKernelGGL(k_prep_items, dim3((unsigned)((n_item + 7) / 8)), block, 0, s, items, n_item, ld, K, Kp, hi, lo,
-                       norm);
+                       norm, cs, rho, g1, g_rule);
     return hipGetLastError();
 }
 
-hipError_t launch_user_norms(hipStream_t s, const float *users, int64_t n_user, int32_t ld, int32_t K, float *norm) {
+hipError_t launch_user_norms(hipStream_t s, const float *users, int64_t n_user, int32_t ld, int32_t K, float *norm,
+                             float *rho) {
     if (n_user <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_user_norms, dim3((unsigned)((n_user + 255) / 256)), dim3(256), 0, s, users, n_user, ld, K,
-                       norm);
+                       norm, rho);
     return hipGetLastError();
 }
 

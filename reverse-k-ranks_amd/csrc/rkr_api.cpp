@@ -90,6 +90,7 @@ DataSnap snap_of(const Ctx *c) {
     d.usersT = c->usersT;
     d.items = c->items;
     d.user_norm = c->user_norm;
+    d.rho = c->rho;
     d.user_hi = c->user_hi;
     d.user_lo = c->user_lo;
     d.item_norm = c->item_norm;
@@ -448,6 +449,7 @@ int rkr_create(int device, size_t mem_budget_bytes, rkr_ctx **out) {
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) { rkr_destroy(c); return RKR_ERR_HIP; }
     if (slot_init(c, c->main, false) != RKR_OK) { rkr_destroy(c); return RKR_ERR_HIP; }
+    if (hipMalloc(&c->rho, 256) != hipSuccess) { c->rho = nullptr; rkr_destroy(c); return RKR_ERR_NOMEM; }
     // the no-allocation entry point's scratch: its address on this device,
     // looked up once per device (a failure is reported by that entry point)
     (void)noalloc_init(device);
@@ -465,6 +467,7 @@ void rkr_destroy(rkr_ctx *c) {
     free_data(c);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->rho) (void)hipFree(c->rho);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -630,13 +633,13 @@ int rkr_set_data(rkr_ctx *c, const float *users, int64_t n_user, const float *it
         const double g_mfma = n3 * e23 / (1.0 - n3 * e23), g_rule = dim * e24 / (1.0 - dim * e24);
         const double C = (3.2 * std::ldexp(1.0, -16) + 1.03 * g_mfma + g_rule) * 1.01;
         c->screen_C = std::nextafter((float)C, INFINITY);
-        // the coarse first pass's single product u_hi q_hi: each BF16 factor
-        // is within 2^-8 of its FP32 value (round to nearest), so a product
-        // is within 2^-7 + 2^-16 of the exact one, then kp FP32 additions
-        // (x1.01 for the factors' magnitudes) and the rule's own rounding
-        const double n1 = (double)kp, g_mfma1 = n1 * e23 / (1.0 - n1 * e23);
-        const double C1 = (std::ldexp(1.0, -7) + std::ldexp(1.0, -16) + 1.01 * g_mfma1 + g_rule) * 1.01;
-        c->coarse_C1 = std::nextafter((float)C1, INFINITY);
+        // the coarse first pass's single product u_hi q_hi: its error scale
+        // per query (k_prep_items) takes the users' largest BF16 residual
+        // ratio rho (k_user_norms), the FP32 accumulation of kp exact
+        // products and the rule's own rounding
+        const double n1 = (double)kp;
+        c->coarse_g1 = n1 * e23 / (1.0 - n1 * e23);
+        c->coarse_grule = g_rule;
         c->screen_abs = screen_abs_floor(kp, c->max_abs_user, c->max_abs_item);
         if ((rc = dev_alloc_t(c, &c->item_hi, (size_t)n_item * kp, "item hi")) ||
             (rc = dev_alloc_t(c, &c->item_lo, (size_t)n_item * kp, "item lo")) ||
@@ -648,7 +651,8 @@ int rkr_set_data(rkr_ctx *c, const float *users, int64_t n_user, const float *it
             return rc;
         }
         RKR_HIP(c, launch_prep_items(c->stream, c->items, n_item, ld, dim, kp, c->item_hi, c->item_lo, c->item_norm));
-        RKR_HIP(c, launch_user_norms(c->stream, c->users, n_user, ld, dim, c->user_norm));
+        RKR_HIP(c, hipMemsetAsync(c->rho, 0, sizeof(float), c->stream));
+        RKR_HIP(c, launch_user_norms(c->stream, c->users, n_user, ld, dim, c->user_norm, c->rho));
         RKR_HIP(c, launch_prep_items(c->stream, c->users, n_user, ld, dim, kp, c->user_hi, c->user_lo, nullptr));
     }
     c->items_base = c->items;
@@ -1397,7 +1401,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
         return rc;
     GuardCols gc{};
     if (coarse) {
-        if ((rc = ensure(c, sl.q_split, (sizeof(uint16_t) * 2 * c->kp + sizeof(float)) * (size_t)nqb, "query BF16 rows")))
+        if ((rc = ensure(c, sl.q_split, (sizeof(uint16_t) * 2 * c->kp + 2 * sizeof(float)) * (size_t)nqb, "query BF16 rows")))
             return rc;
         for (int j = 0; j < dv.n_guard; ++j) gc.col[j] = dv.gcol[j];
         gc.n = dv.n_guard;
@@ -1485,7 +1489,9 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             uint16_t *q_hi = reinterpret_cast<uint16_t *>(sl.q_split.p);
             uint16_t *q_lo = q_hi + (size_t)nqb * c->kp;
             float *qn = reinterpret_cast<float *>(q_lo + (size_t)nqb * c->kp);
-            RKR_HIP(c, launch_prep_items(sl.stream, q_in, nq, ld, d, c->kp, q_hi, q_lo, qn));
+            float *qcs = qn + nqb;  // the first pass's error scales
+            RKR_HIP(c, launch_prep_items(sl.stream, q_in, nq, ld, d, c->kp, q_hi, q_lo, qn, qcs, dv.rho, c->coarse_g1,
+                                         c->coarse_grule));
             const double mq = q_amax;
             const double absf = ((double)screen_abs_floor(c->kp, dv.max_abs_user, mq) + (double)eps) *
                                 (1.0 + std::ldexp(1.0, -18));
@@ -1494,8 +1500,10 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
             L.users = dv.users; L.user_norm = dv.user_norm; L.table = T; L.gvals = dv.gvals; L.n_user = U; L.S = S;
             L.gc = gc;
             L.user_hi = dv.user_hi; L.user_lo = dv.user_lo;
-            L.q_hi = q_hi; L.q_lo = q_lo; L.q_norm = qn; L.qmap = nullptr; L.nq = nq; L.ld = ld; L.K = d;
-            L.Kp = c->kp; L.C = c->coarse_C1; L.abs_floor = std::nextafter((float)absf, INFINITY); L.cb = cb;
+            // first pass: e = C ||u|| cs + floor, C = 1 + 2^-20 covering the
+            // FP32 evaluation of that product and sum
+            L.q_hi = q_hi; L.q_lo = q_lo; L.q_norm = qcs; L.qmap = nullptr; L.nq = nq; L.ld = ld; L.K = d;
+            L.Kp = c->kp; L.C = 1.0f + 0x1p-20f; L.abs_floor = std::nextafter((float)absf, INFINITY); L.cb = cb;
             L.ldc = ldc; L.spare_row = nqb;
             L.grid_cap = (int64_t)c->n_cu * coarse_first_pass_wgs(c->kp / 32);
             RKR_HIP(c, launch_ip_coarse(sl.stream, L));
@@ -1605,6 +1613,7 @@ int run_batches(Ctx *c, Slot &sl, const DataSnap &dv, const float *queries, int3
                                           sl.stream));
                 CoarseLaunch L2 = L;
                 L2.C = c->screen_C;  // the second pass's three-product estimate
+                L2.q_norm = qn;
                 L2.grid_cap = (int64_t)c->n_cu * 3;
                 L2.qmap = qmap;
                 L2.nq = nr;

@@ -134,7 +134,11 @@ struct Ctx {
     uint16_t *user_hi = nullptr, *user_lo = nullptr;  // BF16 hi/lo user rows ([u_cap] x kp)
     double max_abs_item = 0.0, max_abs_user_raw = 0.0;
     float screen_C = 0.0f, screen_abs = 0.0f;
-    float coarse_C1 = 0.0f;  // error constant of the coarse first pass (one BF16 product)
+    // the coarse first pass (one BF16 product): the FP32 accumulation and
+    // rule rounding factors of its error scale, and (device, 4 bytes, for the
+    // context's life) rho >= ||u - bf16(u)|| / ||u|| over every user loaded
+    double coarse_g1 = 0.0, coarse_grule = 0.0;
+    float *rho = nullptr;
     int n_cu = 256;
 
     // QSRP screen (rkr_qsrp_build): users rotated into the SVD basis, the query
@@ -241,6 +245,7 @@ struct DataSnap {
     int64_t n_user = 0, n_item = 0, n_item_base = 0;
     const float *users = nullptr, *usersT = nullptr, *items = nullptr;
     const float *user_norm = nullptr, *item_norm = nullptr;
+    const float *rho = nullptr;      // >= ||u - bf16(u)|| / ||u|| over the users (device)
     const uint16_t *user_hi = nullptr, *user_lo = nullptr;
     const uint16_t *item_hi = nullptr, *item_lo = nullptr;
     const float *table = nullptr, *side_ins = nullptr, *side_del = nullptr;
@@ -361,10 +366,17 @@ struct ScreenLaunch {
     int32_t rows_per_wave;  // 32 or 64 refine rows per wavefront
 };
 // BF16 hi/lo rows (Kp columns, zero padded) and, when norm is non-null, row
-// norm upper bounds; used for items, queries and users
+// norm upper bounds; used for items, queries and users.  With cs (queries of
+// the coarse first pass): each row's error scale of the one-product
+// estimate, given rho (device) >= ||u - bf16(u)|| / ||u|| over the users and
+// the accumulation and rule rounding factors g1, g_rule.
 hipError_t launch_prep_items(hipStream_t s, const float *items, int64_t n_item, int32_t ld, int32_t K, int32_t Kp,
-                             uint16_t *hi, uint16_t *lo, float *norm);
-hipError_t launch_user_norms(hipStream_t s, const float *users, int64_t n_user, int32_t ld, int32_t K, float *norm);
+                             uint16_t *hi, uint16_t *lo, float *norm, float *cs = nullptr, const float *rho = nullptr,
+                             double g1 = 0.0, double g_rule = 0.0);
+// Users' norm upper bounds; with rho (device), raises *rho to at least every
+// user's ||u - bf16(u)|| / ||u||.
+hipError_t launch_user_norms(hipStream_t s, const float *users, int64_t n_user, int32_t ld, int32_t K, float *norm,
+                             float *rho);
 hipError_t launch_refine_screen(hipStream_t s, const ScreenLaunch &L);
 // Guard columns of the coarse bounds: n (1..7) ascending table columns.
 

@@ -492,7 +492,7 @@ int rkr_insert_users(rkr_ctx *c, const float *users, int64_t n) {
     const int64_t p0 = U0 / 64;  // first panel that changes (lanes of earlier users are rewritten unchanged)
     RKR_HIP(c, launch_make_panels(c->stream, c->users + p0 * 64 * ld, U1 - p0 * 64, ld, c->usersT + p0 * 64 * ld));
     if (c->kp) {
-        RKR_HIP(c, launch_user_norms(c->stream, c->users + U0 * ld, n, ld, c->dim, c->user_norm + U0));
+        RKR_HIP(c, launch_user_norms(c->stream, c->users + U0 * ld, n, ld, c->dim, c->user_norm + U0, c->rho));
         RKR_HIP(c, launch_prep_items(c->stream, c->users + U0 * ld, n, ld, c->dim, c->kp, c->user_hi + U0 * c->kp,
                                      c->user_lo + U0 * c->kp, nullptr));
     }
@@ -584,7 +584,7 @@ int rkr_delete_users(rkr_ctx *c, const int64_t *ids, int64_t n) {
             dev_free(c, uhi, 0);
             return rc;
         }
-        RKR_HIP(c, launch_user_norms(c->stream, c->users, c->n_user, ld, c->dim, unorm));
+        RKR_HIP(c, launch_user_norms(c->stream, c->users, c->n_user, ld, c->dim, unorm, c->rho));
         RKR_HIP(c, launch_prep_items(c->stream, c->users, c->n_user, ld, c->dim, c->kp, uhi, ulo, nullptr));
     }
     RKR_HIP(c, hipStreamSynchronize(c->stream));
