@@ -5,16 +5,15 @@
 // (src/include/score_computation/ComputeScoreTableBatch.hpp:83-89).  Only the values are kept, so the
 // order of equal values is irrelevant and the sort need not be stable.
 //
-// One 1024-thread workgroup per CU, taking rows in turn.  Rows of up to NMAX values are sorted in LDS
-// (bucket path):
-//   1. the row is loaded into registers (coalesced; its loads were issued while the workgroup's
-//      previous row was ranked), with its min and max;
+// One 1024-thread workgroup per CU, taking rows in turn (rows that load 16 bytes at a time; others
+// take one workgroup each).  Rows of up to NMAX values are sorted in LDS (bucket path):
+//   1. the row is in registers (coalesced loads, issued while the workgroup's previous row was
+//      ranked), and each wavefront's min and max of it were taken before that row's last barrier;
 //   2. a histogram over NBK value bins, descending, and its exclusive scan.  The bins are
 //      apportioned to NB1 linear coarse bins by the coarse bins' counts (a first histogram, of
-//      half the row), so
-//      bins are narrow where the row is dense and hold about n / NBK values each even when a
-//      heavy tail crowds most of the row into a few linear bins.  FP32 arithmetic: fl(max - x)
-//      and fl(t * scale) are monotone in x, so bin order is value order;
+//      half the row), so bins are narrow where the row is dense and hold about n / NBK values
+//      each even when a heavy tail crowds most of the row into a few linear bins.  FP32
+//      arithmetic: fl(max - x) and fl(t * scale) are monotone in x, so bin order is value order;
 //   3. every value's order-preserving key is scattered into its bin's segment of an LDS buffer (any
 //      order inside a bin);
 //   4. each value's final position is its bin's start plus the keys of its bin above it (and the
@@ -52,7 +51,7 @@ struct SortSmem {
         float tile[TILE];                // merge path
         struct {
             uint32_t key[NMAX + KPAD];   // the row's keys, bin by bin, then KPAD zero keys
-            uint32_t cnt[NBK];           // coarse counts, then bin counts, then bin ends (32 KB)
+            uint32_t cnt[NBK];           // bin counts, then scatter cursors ending as bin ends
             uint32_t fseg[NB1];          // bins of coarse bin c: s0 | s1 << 16, bins s0 .. s1 - 1
             uint32_t coarse[NB1];        // coarse counts (zero between rows)
             float red_f[2 * NW];         // the row's min and max per wavefront, then the largest bin
