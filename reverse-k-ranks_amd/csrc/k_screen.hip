@@ -530,6 +530,24 @@ __device__ __forceinline__ uint32_t guard_byte(const float (&g)[8], float xl, fl
     return twice_plus(t, l0);
 }
 
+// c = #{j < 7 : g[j] > x} by the same three-step search as guard_byte, and
+// last = g[c - 1], the smallest guard above x (+inf when c = 0), which is
+// always the last of the three tested values that passed.
+__device__ __forceinline__ uint32_t guard_count_last(const float (&g)[8], float x, float &last) {
+    const bool b2 = g[3] > x;
+    const float p1 = __builtin_unpredictable(b2) ? g[5] : g[1];
+    const bool b1 = p1 > x;
+    const float pa = __builtin_unpredictable(b1) ? g[6] : g[4], pb = __builtin_unpredictable(b1) ? g[2] : g[0];
+    const float p0 = __builtin_unpredictable(b2) ? pa : pb;
+    const bool b0 = p0 > x;
+    const float l1 = __builtin_unpredictable(b2) ? g[3] : INFINITY;
+    const float l2 = __builtin_unpredictable(b1) ? p1 : l1;
+    last = __builtin_unpredictable(b0) ? p0 : l2;
+    uint32_t t = b2 ? 1u : 0u;
+    t = twice_plus(t, b1);
+    return twice_plus(t, b0);
+}
+
 template <int KS, bool SETS>
 __global__ void __launch_bounds__(NT, SETS ? 2 : coarse_first_pass_wgs(KS)) k_ip_coarse(IpCoarseParams P) {
     // first pass: one product, so only the queries' hi halves are staged and
@@ -553,7 +571,7 @@ __global__ void __launch_bounds__(NT, SETS ? 2 : coarse_first_pass_wgs(KS)) k_ip
     for (int64_t rb = blockIdx.x; rb < n_rb; rb += gridDim.x) {
         const int64_t row_base = rb * 128 + wave * 32;
         Frag uh[2][KS], ul[2][ONE ? 1 : KS];
-        float gd[2][8], ar[2];
+        float gd[2][8], ar[2], hg[2];
         bool live[2];
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
@@ -590,6 +608,15 @@ __global__ void __launch_bounds__(NT, SETS ? 2 : coarse_first_pass_wgs(KS)) k_ip
                 }
             }
             ar[b] = live[b] ? P.C * P.user_norm[r] : 0.0f;
+            if (!SETS) {
+                // half the smallest gap between adjacent guards, rounded down
+                // (differences with -inf padding are +inf or NaN, which
+                // fminf passes over)
+                float m = INFINITY;
+#pragma unroll
+                for (int j = 0; j < 6; ++j) m = fminf(m, gd[b][j] - gd[b][j + 1]);
+                hg[b] = __double2float_rd(0.5 * (double)m);
+            }
         }
         if (SETS) {
             __syncthreads();  // the previous row block is done with sG
@@ -673,23 +700,65 @@ __global__ void __launch_bounds__(NT, SETS ? 2 : coarse_first_pass_wgs(KS)) k_ip
                     }
                 }
                 // D[4 lq + i][lr]: query q0 + g*16 + 4 lq + i, user row_base + b*16 + lr
+                if (SETS) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    // rows are padded to whole row blocks and padding queries
-                    // write the spare row, so every lane stores
-                    const int ql_ = g * 16 + 4 * lq + i;
-                    const float qn = sQn[ql_];
-                    uint8_t *out = P.cb + sQo[ql_] + row_base + lr;
+                    for (int i = 0; i < 4; ++i) {
+                        // rows are padded to whole row blocks and padding queries
+                        // write the spare row, so every lane stores
+                        const int ql_ = g * 16 + 4 * lq + i;
+                        const float qn = sQn[ql_];
+                        uint8_t *out = P.cb + sQo[ql_] + row_base + lr;
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) {
-                        if (SETS) {
+                        for (int b = 0; b < 2; ++b) {
                             const float4 *gp = sG + (sQs[ql_] * 128 + wave * 32 + b * 16 + lr) * 2;
                             const float4 x0 = gp[0], x1 = gp[1];
                             gd[b][0] = x0.x; gd[b][1] = x0.y; gd[b][2] = x0.z; gd[b][3] = x0.w;
                             gd[b][4] = x1.x; gd[b][5] = x1.y; gd[b][6] = x1.z; gd[b][7] = x1.w;
+                            const float e = __fmaf_rn(ar[b], qn, P.abs_floor);
+                            out[b * 16] = (uint8_t)guard_byte(gd[b], acc[b][i] - e, acc[b][i] + e);
                         }
-                        const float e = __fmaf_rn(ar[b], qn, P.abs_floor);
-                        out[b * 16] = (uint8_t)guard_byte(gd[b], acc[b][i] - e, acc[b][i] + e);
+                    }
+                } else {
+                    // c_hi = #{g > xh} <= c_lo.  When at most one guard can lie
+                    // in (xl, xh] -- the interval is narrower than half the
+                    // smallest gap between adjacent guards -- c_hi is c_lo less
+                    // [g[c_lo - 1] <= xh].  Two guards inside would be closer
+                    // than xh - xl (< 2 fl(xh - xl) in FP32), so a wavefront
+                    // with any pair failing the test counts xh for all its
+                    // pairs.
+                    uint32_t byte[4][2];
+                    bool redo = false;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float qn = sQn[g * 16 + 4 * lq + i];
+#pragma unroll
+                        for (int b = 0; b < 2; ++b) {
+                            const float e = __fmaf_rn(ar[b], qn, P.abs_floor);
+                            const float xl = acc[b][i] - e, xh = acc[b][i] + e;
+                            float last;
+                            const uint32_t c = guard_count_last(gd[b], xl, last);
+                            redo |= !(xh - xl < hg[b]);
+                            byte[i][b] = c | (c - (last <= xh ? 1u : 0u)) << 4;
+                        }
+                    }
+                    if (__builtin_amdgcn_ballot_w64(redo) != 0) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const float qn = sQn[g * 16 + 4 * lq + i];
+#pragma unroll
+                            for (int b = 0; b < 2; ++b) {
+                                const float e = __fmaf_rn(ar[b], qn, P.abs_floor);
+                                byte[i][b] = guard_byte(gd[b], acc[b][i] - e, acc[b][i] + e);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        // rows are padded to whole row blocks and padding
+                        // queries write the spare row, so every lane stores
+                        uint8_t *out = P.cb + sQo[g * 16 + 4 * lq + i] + row_base + lr;
+#pragma unroll
+                        for (int b = 0; b < 2; ++b) out[b * 16] = (uint8_t)byte[i][b];
                     }
                 }
             }

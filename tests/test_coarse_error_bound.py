@@ -91,3 +91,28 @@ def test_bound_with_exact_bf16_values():
     s = rule_scores(U, Q)
     est = seq_fp32(U, Q)
     assert np.all(np.abs(est.astype(np.float64) - s.astype(np.float64)) <= np.outer(nu, cs))
+
+
+def test_upper_count_from_lower_count():
+    """The first pass takes c_hi = c_lo - [g[c_lo - 1] <= xh] whenever
+    fl(xh - xl) is below half the smallest gap between adjacent guards
+    (rounded down), and counts xh in full otherwise; the shortcut must equal
+    the full count wherever it is taken, including near-tied guards."""
+    rng = np.random.default_rng(8)
+    n = 20000
+    g = -np.sort(-rng.standard_normal((n, 7)).astype(np.float32), axis=1)
+    g[:100, 3] = g[:100, 2]                                         # tied guards
+    g[100:200, 4] = np.nextafter(g[100:200, 3], np.float32(-np.inf))  # one ulp apart
+    x = rng.standard_normal(n).astype(np.float32)
+    e = (np.abs(rng.standard_normal(n)) * rng.choice([1e-4, 1e-2, 0.3], n)).astype(np.float32)
+    xl, xh = (x - e).astype(np.float32), (x + e).astype(np.float32)
+    c_lo = (g > xl[:, None]).sum(1)
+    c_hi = (g > xh[:, None]).sum(1)
+    gp = np.concatenate([np.full((n, 1), np.inf, np.float32), g], axis=1)
+    last = gp[np.arange(n), c_lo]                                   # g[c_lo - 1], +inf at c_lo = 0
+    gaps = (g[:, :-1] - g[:, 1:]).astype(np.float32)
+    hg = np.nextafter((0.5 * gaps.min(1).astype(np.float64)).astype(np.float32), np.float32(-np.inf))
+    take = (xh - xl).astype(np.float32) < hg
+    short = c_lo - (last <= xh)
+    assert take.mean() > 0.5 and (~take).sum() > 0
+    assert np.array_equal(short[take], c_hi[take])
