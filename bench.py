@@ -490,8 +490,8 @@ def cpu_baseline(wl, table, ranks, users_norm, gpu_users, k, n_max, seconds):
 PROFILED_TRAFFIC = {
     ("plain_gpu", "screened"): {"kernel": "k_ip_coarse", "bytes": 1.486e8, "fetch_kib": 23608.4,
                                 "write_kib": 97847.8, "profile": "profiles/k_ip_coarse_100k_x_20k_d150_k50.md"},
-    ("precision", "screened"): {"kernel": "k_ip_coarse", "bytes": 1.492e9, "fetch_kib": 239864.4,
-                                "write_kib": 977654.9, "profile": "profiles/k_ip_coarse_1M_x_50k_d150_k50.md"},
+    ("precision", "screened"): {"kernel": "k_ip_coarse", "bytes": 1.484e9, "fetch_kib": 235798.1,
+                                "write_kib": 977734.2, "profile": "profiles/k_ip_coarse_1M_x_50k_d150_k50.md"},
 }
 
 
